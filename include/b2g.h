@@ -1,0 +1,279 @@
+/*
+ * b2g.h -- C ABI of the MI355X b2World::Step engine (libbox2d_amd.so).
+ *
+ * Plain C: pointers, sizes and status codes; no C++ or HIP types cross this
+ * boundary. The C++ drop-in classes (b2World, b2Body, b2Fixture, b2Contact,
+ * b2RevoluteJoint, ...) in box2d-optimized_amd/host/box2d/ are implemented on
+ * top of these entry points, the same way the reference's b2World drives its
+ * b2ContactManager / b2Island / b2BroadPhase internals.
+ *
+ * Every entry point returns 0 (B2G_OK) or a B2G_ERR_* code; b2g_last_error()
+ * gives the message of the last failure on the calling thread.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   b2g_batch_step           -> b2World::Step            include/box2d/b2_world.h:127-142,
+ *                                                         src/dynamics/b2_world.cpp:1108-1171
+ *   b2g_upload_bodies        -> b2World::CreateBody / b2Body setters
+ *                                                         src/dynamics/b2_world.cpp:140-176,
+ *                                                         include/box2d/b2_body.h:183-285
+ *   b2g_upload_shapes        -> b2Body::CreateFixture + b2BroadPhase::Add
+ *                                                         src/dynamics/b2_body.cpp:230-269,
+ *                                                         src/collision/b2_broad_phase.cpp:118-180
+ *   b2g_upload_joints        -> b2World::CreateJoint (revolute)
+ *                                                         src/dynamics/b2_world.cpp:268-324
+ *   b2g_download_bodies      -> b2Body::GetTransform/GetPosition/GetAngle/
+ *                               GetLinearVelocity/GetAngularVelocity
+ *                                                         include/box2d/b2_body.h:187-217
+ *   b2g_download_contacts    -> b2World::GetContactList + b2Contact::GetManifold/IsTouching
+ *                                                         include/box2d/b2_world.h:187-192,
+ *                                                         include/box2d/b2_contact.h:71-78
+ *   b2g_upload_contacts      -> (no direct counterpart: restores the contact list with
+ *                               warm-start impulses, which b2World::Dump cannot)
+ *                                                         src/dynamics/b2_world.cpp:1469-1531
+ *   b2g_get_counts           -> b2World::GetBodyCount/GetContactCount/GetJointCount
+ *                                                         include/box2d/b2_world.h:211-223
+ *   b2g_collide_pairs        -> b2Contact::Evaluate -> b2Collide* (5 shape pairs)
+ *                                                         src/dynamics/b2_contact.cpp:48-56,
+ *                                                         include/box2d/b2_collision.h:227-250
+ *   b2g_find_pairs           -> b2BroadPhase::UpdateAndQuery pair reports, filtered as
+ *                               b2ContactManager::QueryCallback filters them
+ *                                                         include/box2d/b2_broad_phase.h:252-343,
+ *                                                         src/dynamics/b2_contact_manager.cpp:136-188
+ */
+#ifndef B2G_H
+#define B2G_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+enum {
+  B2G_OK = 0,
+  B2G_ERR_HIP = 1,          /* a HIP runtime call failed */
+  B2G_ERR_OOM = 2,          /* device allocation failed */
+  B2G_ERR_CAPACITY = 3,     /* a fixed capacity was exceeded (pairs, colours, ...) */
+  B2G_ERR_INVALID = 4,      /* bad handle, index or size */
+  B2G_ERR_UNSUPPORTED = 5,  /* feature the device path does not implement (sensors, ...) */
+  B2G_ERR_NO_DEVICE = 6     /* no HIP device visible */
+};
+
+/* ---- shape / body kinds (values match b2Shape::Type, b2BodyType) ------ */
+enum { B2G_CIRCLE = 0, B2G_EDGE = 1, B2G_POLYGON = 2 };
+enum { B2G_STATIC = 0, B2G_KINEMATIC = 1, B2G_DYNAMIC = 2 };
+
+/* ---- scene description ---------------------------------------------------
+ * The raw definitions a user hands to CreateBody / CreateFixture / CreateJoint,
+ * as flat arrays. Index order is creation order. Used by the scene loader of
+ * the drop-in shim (b2a_world_from_scene) and by test drivers.
+ *
+ * fix_kind / fix_geom (17 floats per fixture):
+ *   0 circle : [0]=radius, [1..2]=center                 (b2CircleShape m_radius, m_p)
+ *   1 edge   : [0]=oneSided(0/1), [1..8]=v0,v1,v2,v3     (b2EdgeShape::SetOneSided / SetTwoSided(v1,v2))
+ *   2 polygon: [0]=count(3..8), [1..16]=points           (b2PolygonShape::Set: weld + hull)
+ *   3 box    : [0]=hx, [1]=hy, [2..3]=center, [4]=angle, [5]=1 to use the
+ *              centred form SetAsBox(hx,hy,center,angle), else SetAsBox(hx,hy)
+ * fix_material (4 per fixture): density, friction, restitution, restitutionThreshold
+ * fix_filter   (4 per fixture): categoryBits, maskBits, groupIndex, isSensor (NULL = b2Filter defaults)
+ * joint_* : revolute joints built with b2RevoluteJointDef::Initialize(bodyA, bodyB, anchor)
+ *   joint_params (6 per joint): anchor.x, anchor.y, lowerAngle, upperAngle, motorSpeed, maxMotorTorque
+ *   joint_flags: bit0 collideConnected, bit1 enableLimit, bit2 enableMotor
+ */
+typedef struct b2g_scene {
+  float gravity[2];
+  int32_t body_count;
+  const int32_t* body_type;        /* [n] B2G_STATIC/KINEMATIC/DYNAMIC */
+  const float* body_pos;           /* [2n] */
+  const float* body_angle;         /* [n] */
+  const float* body_vel;           /* [3n] vx, vy, w; NULL = zero */
+  const float* body_damping;       /* [2n] linear, angular; NULL = zero */
+  const float* body_gravity_scale; /* [n]; NULL = 1 */
+  const int32_t* body_flags;       /* [n] bit0 fixedRotation, bit1 bullet; NULL = 0 */
+  int32_t fixture_count;
+  const int32_t* fix_body;         /* [m] */
+  const int32_t* fix_kind;         /* [m] */
+  const float* fix_geom;           /* [17m] */
+  const float* fix_material;       /* [4m] */
+  const int32_t* fix_filter;       /* [4m] or NULL */
+  int32_t joint_count;
+  const int32_t* joint_bodies;     /* [2j] */
+  const float* joint_params;       /* [6j] */
+  const int32_t* joint_flags;      /* [j] */
+} b2g_scene;
+
+/* ---- device-facing records (host side of the upload/download calls) ---- */
+
+/* One rigid body: the fields of b2Body (include/box2d/b2_body.h:498-547) that
+ * the step reads or writes. */
+typedef struct b2g_body {
+  int32_t type;           /* B2G_STATIC / B2G_KINEMATIC / B2G_DYNAMIC */
+  int32_t world;          /* index of the world in the batch */
+  float xf[4];            /* p.x, p.y, q.s, q.c */
+  float c[2];             /* sweep.c (centre of mass, world) */
+  float a;                /* sweep.a */
+  float local_center[2];  /* sweep.localCenter */
+  float v[2];             /* linear velocity */
+  float w;                /* angular velocity */
+  float mass;             /* m_mass (gravity is applied as m * g * invMass) */
+  float inv_mass, inv_i;
+  float linear_damping, angular_damping, gravity_scale;
+  float force[2];
+  float torque;
+} b2g_body;
+
+/* One fixture with its processed shape (after b2PolygonShape::Set etc.). */
+typedef struct b2g_shape {
+  int32_t type;           /* B2G_CIRCLE / B2G_EDGE / B2G_POLYGON */
+  int32_t body;           /* body index in the batch */
+  int32_t count;          /* polygon vertex count; edge: oneSided flag */
+  float radius;           /* m_radius (polygon skin, circle radius) */
+  float centroid[2];      /* polygon m_centroid; circle m_p */
+  float vertices[16];     /* polygon vertices; edge v0,v1,v2,v3 */
+  float normals[16];      /* polygon edge normals */
+  float friction, restitution, restitution_threshold;
+  int32_t category, mask, group;
+  int32_t is_sensor;
+} b2g_shape;
+
+/* One revolute joint (src/dynamics/b2_revolute_joint.cpp:52-71 fields). */
+typedef struct b2g_revolute {
+  int32_t body_a, body_b;
+  float local_anchor_a[2], local_anchor_b[2];
+  float reference_angle;
+  float lower_angle, upper_angle;
+  float motor_speed, max_motor_torque;
+  int32_t flags;          /* bit0 collideConnected, bit1 enableLimit, bit2 enableMotor */
+  /* solver state carried across steps (warm starting) */
+  float impulse[2];
+  float motor_impulse, lower_impulse, upper_impulse;
+} b2g_revolute;
+
+/* One contact with its manifold (b2Contact + b2Manifold,
+ * include/box2d/b2_contact.h:65-217, include/box2d/b2_collision.h:44-117). */
+typedef struct b2g_contact {
+  int32_t shape_a, shape_b;   /* fixture indices, in the contact's own A/B order */
+  int32_t touching;           /* b2Contact::IsTouching */
+  int32_t type;               /* b2Manifold::Type: 0 circles, 1 faceA, 2 faceB */
+  int32_t point_count;
+  float local_normal[2];
+  float local_point[2];
+  float point_local[4];       /* points[0].localPoint, points[1].localPoint */
+  float normal_impulse[2];
+  float tangent_impulse[2];
+  uint32_t id[2];             /* b2ContactID::key of each point */
+  float friction, restitution, restitution_threshold, tangent_speed;
+} b2g_contact;
+
+typedef struct b2g_counts {
+  int32_t worlds, bodies, shapes, joints;
+  int32_t contacts;           /* b2World::GetContactCount: AABB-overlapping pairs */
+  int32_t touching;           /* contacts with pointCount > 0 */
+  int32_t colours;            /* constraint colours used by the last step */
+  int32_t islands;            /* islands of the last step */
+  int32_t pos_iters_run;      /* max position iterations executed by an island */
+  int32_t status;             /* sticky device status (B2G_ERR_CAPACITY on overflow) */
+} b2g_counts;
+
+/* Per-stage device time of the last step, in milliseconds (b2Profile,
+ * include/box2d/b2_time_step.h:29-36; measured with HIP events when
+ * B2G_STEP_PROFILE is set). */
+typedef struct b2g_profile {
+  float step, collide, solve, broadphase;
+} b2g_profile;
+
+/* ---- batch (a set of independent worlds sharing device arrays) ------- */
+typedef struct b2g_batch b2g_batch;
+
+typedef struct b2g_capacity {
+  int32_t worlds, bodies, shapes, joints;
+  int32_t contacts;           /* AABB-overlap pairs kept per step */
+} b2g_capacity;
+
+/* per-world parameters of one step */
+typedef struct b2g_world_params {
+  float gravity[2];
+  float dt;
+  float dt_ratio;             /* m_inv_dt0 * dt (first step: 0) */
+  int32_t warm_starting;
+  int32_t active;             /* 0 = skip this world in the launch */
+} b2g_world_params;
+
+enum {
+  B2G_STEP_FIND_NEW_CONTACTS = 1u << 0,  /* run the broadphase before Collide (m_newContacts) */
+  B2G_STEP_PROFILE = 1u << 1,            /* time each stage with HIP events (adds syncs) */
+  B2G_STEP_ASYNC = 1u << 2,              /* return without waiting for the device */
+  B2G_STEP_SKIP_BROADPHASE = 1u << 3     /* omit the end-of-step broadphase (testing only) */
+};
+
+int b2g_device_count(int32_t* count);
+const char* b2g_last_error(void);
+
+int b2g_batch_create(b2g_batch** out, const b2g_capacity* cap);
+int b2g_batch_destroy(b2g_batch* batch);
+
+/* world ranges: bodies [body_begin, body_end) etc. are assigned by the caller */
+int b2g_set_worlds(b2g_batch* batch, int32_t count, const int32_t* body_begin,
+                   const int32_t* shape_begin, const b2g_world_params* params);
+int b2g_set_world_params(b2g_batch* batch, int32_t first, int32_t count,
+                         const b2g_world_params* params);
+
+int b2g_upload_bodies(b2g_batch* batch, int32_t first, int32_t count, const b2g_body* bodies);
+int b2g_upload_shapes(b2g_batch* batch, int32_t first, int32_t count, const b2g_shape* shapes);
+int b2g_upload_joints(b2g_batch* batch, int32_t first, int32_t count, const b2g_revolute* joints);
+/* replaces the whole contact list (sorted by the batch internally) */
+int b2g_upload_contacts(b2g_batch* batch, int32_t count, const b2g_contact* contacts);
+
+/* one b2World::Step for every active world of the batch */
+int b2g_batch_step(b2g_batch* batch, int32_t velocity_iterations, int32_t position_iterations,
+                   uint32_t flags);
+int b2g_batch_wait(b2g_batch* batch);
+
+int b2g_get_counts(b2g_batch* batch, b2g_counts* out);
+int b2g_get_profile(b2g_batch* batch, b2g_profile* out);
+int b2g_download_bodies(b2g_batch* batch, int32_t first, int32_t count, b2g_body* out);
+/* contacts are ordered by (min shape, max shape); out may be NULL to query the count */
+int b2g_download_contacts(b2g_batch* batch, int32_t first, int32_t count, b2g_contact* out);
+int b2g_download_joints(b2g_batch* batch, int32_t first, int32_t count, b2g_revolute* out);
+/* constraint colour of each touching contact of the last step (-1 = not solved);
+ * out has one entry per downloaded contact, same order */
+int b2g_download_contact_colours(b2g_batch* batch, int32_t first, int32_t count, int32_t* out);
+
+/* ---- stage entry points (pure functions over caller arrays) ------------ */
+
+/* Narrowphase: for n pairs (shape_a[i], shape_b[i]) with transforms
+ * xf_a/xf_b (4 floats each: p.x, p.y, q.s, q.c), compute the manifold into
+ * out[i] (shape_a/shape_b/touching/type/points filled; impulses zeroed). */
+int b2g_collide_pairs(int32_t n, const b2g_shape* shape_a, const float* xf_a,
+                      const b2g_shape* shape_b, const float* xf_b, b2g_contact* out);
+
+/* Broadphase: all pairs of the n AABBs (4 floats each: lower.x, lower.y,
+ * upper.x, upper.y) that overlap (inclusive test, b2TestOverlap) and whose
+ * shapes may collide (different bodies, at least one dynamic body, filter
+ * bits, no edge-edge). body_type/shape_body/filter describe the shapes.
+ * Pairs are written sorted as (min << 32 | max) keys. */
+int b2g_find_pairs(int32_t n, const float* aabbs, const int32_t* shape_body,
+                   const int32_t* shape_type, const int32_t* filter /* 3 per shape, NULL ok */,
+                   int32_t body_count, const int32_t* body_type,
+                   int64_t max_pairs, uint64_t* out_keys, int64_t* out_count);
+
+/* ---- scene loader of the drop-in shim (b2World built through its C++ API) */
+typedef struct b2a_world b2a_world;
+int b2a_world_from_scene(b2a_world** out, const b2g_scene* scene, int32_t warm_starting);
+int b2a_world_destroy(b2a_world* world);
+/* b2World::Step(dt, vel, pos) */
+int b2a_world_step(b2a_world* world, float dt, int32_t velocity_iterations,
+                   int32_t position_iterations);
+/* bodies in scene order: 6 floats each: c.x, c.y, a, v.x, v.y, w (sweep state) */
+int b2a_world_get_bodies(b2a_world* world, float* out);
+int b2a_world_contact_count(b2a_world* world, int32_t* out);
+int b2a_world_get_contacts(b2a_world* world, int32_t capacity, b2g_contact* out, int32_t* count);
+b2g_batch* b2a_world_batch(b2a_world* world);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* B2G_H */
