@@ -1,0 +1,871 @@
+// b2g_engine.hip -- device arenas, step orchestration and the b2g C ABI.
+//
+// A b2g_batch holds any number of independent worlds in shared
+// structure-of-arrays device buffers (bodies, shapes, joints, contacts); a
+// single world is a batch of one. One b2g_batch_step runs every stage of
+// b2World::Step for all active worlds with one launch per stage (one per
+// colour for the solver sweeps), so thousands of small worlds cost the same
+// number of launches as one large world.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "b2g.h"
+#include "b2g_kernels.h"
+
+using namespace b2g;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(B2G_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+
+template <typename T>
+int dalloc(T** p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) return fail(B2G_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  e = hipMemset(*p, 0, n * sizeof(T));
+  if (e != hipSuccess) return fail(B2G_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+  return B2G_OK;
+}
+
+inline int grid_for(long long n, int max_blocks = 4096) {
+  long long g = (n + kBlock - 1) / kBlock;
+  if (g < 1) g = 1;
+  if (g > max_blocks) g = max_blocks;
+  // multiples of 8 let the kernels map consecutive tiles to one XCD
+  if (g >= 16) g = (g + 7) & ~7LL;
+  return (int)g;
+}
+
+}  // namespace
+
+struct b2g_batch {
+  b2g_capacity cap{};
+  Dev d{};
+  int dev_id = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[5] = {};
+  int* pinned = nullptr;     // host mirror of counters + colour offsets
+  // host mirrors
+  std::vector<int> h_btype, h_bworld;
+  std::vector<int> h_shape_body;
+  std::vector<b2g_revolute> h_joints;
+  std::vector<int> h_jcol_off;  // joint colour ranges (host colouring)
+  std::vector<b2g_world_params> h_params;
+  bool joints_dirty = false;
+  int est_contacts = 0, est_cons = 0;
+  unsigned round = 0;
+  b2g_profile prof{};
+  // last counters
+  int last_ncolours = 0, last_islands = 0, last_pos_iters = 0;
+
+  int alloc();
+  void release();
+  int scan(const int* in, const int* n_dev, int n_host, int* out, int* total, long long n_bound);
+  int radix_sort(int n, int bits);
+  int broadphase(bool use_given_aabbs);
+  int rebuild_joint_tables();
+  int step(int vi, int pi, unsigned flags);
+  int read_counters();
+};
+
+int b2g_batch::alloc() {
+  const int NB = std::max(cap.bodies, 1), NS = std::max(cap.shapes, 1), NJ = std::max(cap.joints, 1);
+  const int NW = std::max(cap.worlds, 1), NC = std::max(cap.contacts, 1);
+  int rc;
+#define A(ptr, n) \
+  if ((rc = dalloc(&(ptr), (size_t)(n))) != B2G_OK) return rc
+  A(d.xf, NB); A(d.pos, NB); A(d.vel, NB); A(d.prop, NB); A(d.prop2, NB); A(d.force, NB);
+  A(d.btype, NB); A(d.bworld, NB);
+  A(d.wpar, NW); A(d.wflag, NW); A(d.wbounds, 4 * NW); A(d.wleaf, 2 * NW);
+  A(d.shdr, NS); A(d.geo, 9 * (size_t)NS); A(d.smat, NS); A(d.sfilt, NS); A(d.aabb, NS);
+  A(d.joints, NJ); A(d.jsol, NJ); A(d.jorder, NJ); A(d.jisland, NJ);
+  A(d.bj_off, NB + 1); A(d.bj_other, 2 * NJ);
+  A(d.ckey, NC); A(d.cpair, NC); A(d.cman, 4 * (size_t)NC); A(d.cmat, NC); A(d.ccol, NC);
+  A(d.nkey, NC); A(d.npair, NC); A(d.nman, 4 * (size_t)NC); A(d.nmat, NC); A(d.ncol, NC);
+  A(d.touch, NC); A(d.tpos, NC); A(d.clist, NC); A(d.ccolour, NC); A(d.csorted, NC);
+  A(d.cisland, NC);
+  for (int k = 0; k < 10; ++k) A(d.vc[k], NC);
+  for (int k = 0; k < 5; ++k) A(d.pc[k], NC);
+  A(d.col_count, kColourSlots + 1); A(d.col_off, kColourSlots + 1); A(d.col_cursor, kColourSlots + 1);
+  A(d.bmask, NB); A(d.bslot, NB);
+  A(d.parent, NB); A(d.minsep, 2 * (size_t)NB); A(d.jok, 2 * (size_t)NB); A(d.solved, NB);
+  A(d.mkey[0], NS); A(d.mkey[1], NS); A(d.mval[0], NS); A(d.mval[1], NS);
+  const size_t tiles = (size_t)(NS + kSortTile - 1) / kSortTile;
+  A(d.sort_hist, kRadix * tiles);
+  A(d.node_child, 2 * (size_t)NS); A(d.node_parent, NS); A(d.leaf_parent, NS); A(d.node_range, NS);
+  A(d.node_aabb, NS); A(d.leaf_aabb, NS); A(d.node_visit, NS);
+  A(d.pcount, NS); A(d.poff, NS);
+  const size_t scan_n = std::max<size_t>(std::max<size_t>(NC, NS), kRadix * tiles);
+  A(d.scan_tiles, scan_n / kScanTile + 2);
+  A(d.cnt, C_COUNT);
+#undef A
+  d.ccap = NC;
+  HIP_TRY(hipHostMalloc((void**)&pinned, 256 * sizeof(int), hipHostMallocDefault));
+  HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+  h_btype.assign(NB, 0);
+  h_bworld.assign(NB, 0);
+  h_shape_body.assign(NS, 0);
+  h_params.assign(NW, b2g_world_params{{0.0f, -10.0f}, 1.0f / 60.0f, 0.0f, 1, 1});
+  std::vector<int> zero_off(NB + 1, 0);
+  HIP_TRY(hipMemcpy(d.bj_off, zero_off.data(), (NB + 1) * sizeof(int), hipMemcpyHostToDevice));
+  return B2G_OK;
+}
+
+void b2g_batch::release() {
+  void* ptrs[] = {d.xf, d.pos, d.vel, d.prop, d.prop2, d.force, d.btype, d.bworld, d.wpar, d.wflag,
+                  d.wbounds, d.wleaf, d.shdr, d.geo, d.smat, d.sfilt, d.aabb, d.joints, d.jsol,
+                  d.jorder, d.jisland, d.bj_off, d.bj_other, d.ckey, d.cpair, d.cman, d.cmat, d.ccol,
+                  d.nkey, d.npair, d.nman, d.nmat, d.ncol, d.touch, d.tpos, d.clist, d.ccolour,
+                  d.csorted, d.cisland, d.col_count, d.col_off, d.col_cursor, d.bmask, d.bslot,
+                  d.parent, d.minsep, d.jok, d.solved, d.mkey[0], d.mkey[1], d.mval[0], d.mval[1],
+                  d.sort_hist, d.node_child, d.node_parent, d.leaf_parent, d.node_range, d.node_aabb,
+                  d.leaf_aabb, d.node_visit, d.pcount, d.poff, d.scan_tiles, d.cnt};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (int k = 0; k < 10; ++k)
+    if (d.vc[k]) (void)hipFree(d.vc[k]);
+  for (int k = 0; k < 5; ++k)
+    if (d.pc[k]) (void)hipFree(d.pc[k]);
+  if (pinned) (void)hipHostFree(pinned);
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+int b2g_batch::scan(const int* in, const int* n_dev, int n_host, int* out, int* total, long long n_bound) {
+  const int tiles = (int)std::max<long long>(1, (n_bound + kScanTile - 1) / kScanTile);
+  const int g = std::min(tiles, 4096);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(g), dim3(kBlock), 0, stream, in, n_dev, n_host, d.scan_tiles);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, stream, d.scan_tiles, n_dev, n_host, total);
+  hipLaunchKernelGGL(k_scan_apply, dim3(g), dim3(kBlock), 0, stream, in, n_dev, n_host, d.scan_tiles, out);
+  return B2G_OK;
+}
+
+int b2g_batch::radix_sort(int n, int bits) {
+  const int tiles = (n + kSortTile - 1) / kSortTile;
+  const int g = std::min(tiles, 4096);
+  int src = 0;
+  for (int shift = 0; shift < bits; shift += kRadixBits) {
+    hipLaunchKernelGGL(k_radix_hist, dim3(g), dim3(kBlock), 0, stream, d.mkey[src], n, shift, d.sort_hist);
+    scan(d.sort_hist, nullptr, kRadix * tiles, d.sort_hist, nullptr, (long long)kRadix * tiles);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(g), dim3(kBlock), 0, stream, d.mkey[src], d.mval[src], n, shift,
+                       d.sort_hist, d.mkey[1 - src], d.mval[1 - src]);
+    src = 1 - src;
+  }
+  return src;  // buffer holding the result (0 for an even number of passes)
+}
+
+int b2g_batch::broadphase(bool use_given_aabbs) {
+  const int ns = d.ns;
+  if (ns == 0) {
+    HIP_TRY(hipMemsetAsync(d.cnt + C_PAIRS, 0, sizeof(int), stream));
+    hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, stream, d);
+    return B2G_OK;
+  }
+  const int gs = grid_for(ns);
+  hipLaunchKernelGGL(k_world_bounds_reset, dim3(grid_for(d.nw)), dim3(kBlock), 0, stream, d);
+  if (!use_given_aabbs) {
+    hipLaunchKernelGGL(k_aabb, dim3(gs), dim3(kBlock), 0, stream, d);
+  } else {
+    // only the world bounds of the caller's AABBs
+    hipLaunchKernelGGL(k_world_bounds_from_aabbs, dim3(gs), dim3(kBlock), 0, stream, d);
+  }
+  hipLaunchKernelGGL(k_morton, dim3(gs), dim3(kBlock), 0, stream, d);
+  int bits = 32;
+  int wbits = 0;
+  while ((1 << wbits) < d.nw) ++wbits;
+  bits += wbits;
+  bits = ((bits + 2 * kRadixBits - 1) / (2 * kRadixBits)) * (2 * kRadixBits);  // even pass count
+  const int res = radix_sort(ns, bits);
+  if (res != 0) return fail(B2G_ERR_INVALID, "radix sort ended in the wrong buffer");
+  hipLaunchKernelGGL(k_world_leaves, dim3(gs), dim3(kBlock), 0, stream, d, d.mkey[0]);
+  if (ns >= 2) hipLaunchKernelGGL(k_lbvh_build, dim3(grid_for(ns - 1)), dim3(kBlock), 0, stream, d, d.mkey[0]);
+  hipLaunchKernelGGL(k_lbvh_refit, dim3(gs), dim3(kBlock), 0, stream, d, d.mval[0]);
+  hipLaunchKernelGGL(k_lbvh_pairs<false>, dim3(gs), dim3(kBlock), 0, stream, d);
+  scan(d.pcount, nullptr, ns, d.poff, d.cnt + C_PAIRS, ns);
+  hipLaunchKernelGGL(k_lbvh_pairs<true>, dim3(gs), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_merge_contacts, dim3(grid_for(std::max(est_contacts, ns) * 5 / 4)), dim3(kBlock), 0, stream,
+                     d);
+  hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, stream, d);
+  std::swap(d.ckey, d.nkey);
+  std::swap(d.cpair, d.npair);
+  std::swap(d.cman, d.nman);
+  std::swap(d.cmat, d.nmat);
+  std::swap(d.ccol, d.ncol);
+  return B2G_OK;
+}
+
+// Host-side greedy colouring of the joint graph (topology changes only on
+// upload), plus the CSR of joint partners that must not collide.
+int b2g_batch::rebuild_joint_tables() {
+  const int nj = d.nj, nb = d.nb;
+  std::vector<unsigned long long> used(std::max(nb, 1), 0ull);
+  std::vector<int> col(nj, 0);
+  int ncol = 0;
+  for (int q = 0; q < nj; ++q) {
+    const b2g_revolute& j = h_joints[q];
+    unsigned long long m = 0;
+    const bool da = h_btype[j.body_a] == B2G_DYNAMIC, db = h_btype[j.body_b] == B2G_DYNAMIC;
+    if (da) m |= used[j.body_a];
+    if (db) m |= used[j.body_b];
+    int c = 0;
+    while (c < 64 && (m >> c) & 1ull) ++c;
+    if (c >= 64) return fail(B2G_ERR_CAPACITY, "joint graph needs more than 64 colours");
+    col[q] = c;
+    if (da) used[j.body_a] |= 1ull << c;
+    if (db) used[j.body_b] |= 1ull << c;
+    ncol = std::max(ncol, c + 1);
+  }
+  std::vector<int> order(nj);
+  h_jcol_off.assign(ncol + 1, 0);
+  for (int q = 0; q < nj; ++q) h_jcol_off[col[q] + 1]++;
+  for (int c = 0; c < ncol; ++c) h_jcol_off[c + 1] += h_jcol_off[c];
+  std::vector<int> cur(h_jcol_off.begin(), h_jcol_off.end());
+  for (int q = 0; q < nj; ++q) order[cur[col[q]]++] = q;
+  // CSR of non-colliding joint partners
+  std::vector<int> off(nb + 1, 0), other;
+  for (int q = 0; q < nj; ++q) {
+    if (h_joints[q].flags & 1) continue;
+    off[h_joints[q].body_a + 1]++;
+    off[h_joints[q].body_b + 1]++;
+  }
+  for (int b = 0; b < nb; ++b) off[b + 1] += off[b];
+  other.assign(std::max(off[nb], 1), 0);
+  std::vector<int> pos(off.begin(), off.end());
+  for (int q = 0; q < nj; ++q) {
+    if (h_joints[q].flags & 1) continue;
+    other[pos[h_joints[q].body_a]++] = h_joints[q].body_b;
+    other[pos[h_joints[q].body_b]++] = h_joints[q].body_a;
+  }
+  if (nj) HIP_TRY(hipMemcpy(d.jorder, order.data(), nj * sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d.bj_off, off.data(), (nb + 1) * sizeof(int), hipMemcpyHostToDevice));
+  if (off[nb]) HIP_TRY(hipMemcpy(d.bj_other, other.data(), off[nb] * sizeof(int), hipMemcpyHostToDevice));
+  joints_dirty = false;
+  return B2G_OK;
+}
+
+int b2g_batch::read_counters() {
+  HIP_TRY(hipMemcpyAsync(pinned, d.cnt, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  return B2G_OK;
+}
+
+int b2g_batch::step(int vi, int pi, unsigned flags) {
+  if (joints_dirty) {
+    int rc = rebuild_joint_tables();
+    if (rc) return rc;
+  }
+  const bool prof_on = (flags & B2G_STEP_PROFILE) != 0;
+  if (prof_on) HIP_TRY(hipEventRecord(ev[0], stream));
+  if (flags & B2G_STEP_FIND_NEW_CONTACTS) {
+    int rc = broadphase(false);
+    if (rc) return rc;
+  }
+  const int nb = d.nb;
+  const int gc = grid_for(std::max(est_contacts, 256) * 5 / 4);
+  const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
+  const int gb = grid_for(nb);
+  // ---- Collide
+  hipLaunchKernelGGL(k_narrowphase, dim3(gc), dim3(kBlock), 0, stream, d);
+  scan(d.touch, d.cnt + C_CONTACTS, 0, d.tpos, d.cnt + C_CONS, d.ccap);
+  hipLaunchKernelGGL(k_compact_touching, dim3(gc), dim3(kBlock), 0, stream, d);
+  if (prof_on) HIP_TRY(hipEventRecord(ev[1], stream));
+  // ---- Solve
+  hipLaunchKernelGGL(k_integrate_velocities, dim3(gb), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_colour_clear_counts, dim3(1), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_colour_init, dim3(gk), dim3(kBlock), 0, stream, d);
+  int rc = read_counters();
+  if (rc) return rc;
+  int remain = pinned[C_REMAIN];
+  est_cons = pinned[C_CONS];
+  int rounds = 0;
+  while (remain > 0) {
+    for (int r = 0; r < 2; ++r) {
+      ++round;
+      if ((round & 0xffff) == 0) {  // priority tag wrapped: clear the claim slots
+        HIP_TRY(hipMemsetAsync(d.bslot, 0, sizeof(unsigned long long) * std::max(nb, 1), stream));
+        ++round;
+      }
+      hipLaunchKernelGGL(k_colour_claim, dim3(gk), dim3(kBlock), 0, stream, d, (int)(round & 0xffff));
+      hipLaunchKernelGGL(k_colour_resolve, dim3(gk), dim3(kBlock), 0, stream, d, (int)(round & 0xffff));
+    }
+    rounds += 2;
+    if ((rc = read_counters())) return rc;
+    remain = pinned[C_REMAIN];
+    if (rounds > 4096) return fail(B2G_ERR_CAPACITY, "colouring did not converge");
+  }
+  hipLaunchKernelGGL(k_colour_count, dim3(gk), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_colour_offsets, dim3(1), dim3(1), 0, stream, d);
+  hipLaunchKernelGGL(k_colour_scatter, dim3(gk), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_sort_serial_group, dim3(1), dim3(1), 0, stream, d);
+  hipLaunchKernelGGL(k_island_link, dim3(grid_for(std::max(est_cons, 1) + d.nj)), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_island_label, dim3(gb), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_contact_init, dim3(gk), dim3(kBlock), 0, stream, d);
+  if (d.nj) hipLaunchKernelGGL(k_joint_prepare, dim3(grid_for(d.nj)), dim3(kBlock), 0, stream, d);
+  HIP_TRY(hipMemcpyAsync(pinned + 32, d.col_off, (kColourSlots + 1) * sizeof(int), hipMemcpyDeviceToHost, stream));
+  if ((rc = read_counters())) return rc;
+  const int* off = pinned + 32;
+  const int ncol = pinned[C_NCOLOURS];
+  last_ncolours = ncol;
+  auto contact_sweep = [&](auto kernel, auto... extra) {
+    for (int c = 0; c < ncol; ++c) {
+      const int b = off[c], e = off[c + 1];
+      if (e <= b) continue;
+      if (c == kSerialColour)
+        hipLaunchKernelGGL(kernel, dim3(1), dim3(1), 0, stream, d, b, e, extra...);
+      else
+        hipLaunchKernelGGL(kernel, dim3(grid_for(e - b)), dim3(kBlock), 0, stream, d, b, e, extra...);
+    }
+  };
+  auto joint_sweep = [&](auto kernel, auto... extra) {
+    const int njc = (int)h_jcol_off.size() - 1;
+    for (int c = 0; c < njc; ++c) {
+      const int b = h_jcol_off[c], e = h_jcol_off[c + 1];
+      if (e > b) hipLaunchKernelGGL(kernel, dim3(grid_for(e - b)), dim3(kBlock), 0, stream, d, b, e, extra...);
+    }
+  };
+  contact_sweep(k_warm_start);
+  joint_sweep(k_joint_init);
+  for (int it = 0; it < vi; ++it) {
+    joint_sweep(k_joint_velocity);
+    contact_sweep(k_contact_velocity);
+  }
+  hipLaunchKernelGGL(k_store_impulses, dim3(gk), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_integrate_positions, dim3(gb), dim3(kBlock), 0, stream, d);
+  const int gf = grid_for(std::max(est_cons, 1) + d.nj);
+  for (int it = 0; it < pi; ++it) {
+    contact_sweep(k_contact_position, it);
+    joint_sweep(k_joint_position, it);
+    hipLaunchKernelGGL(k_island_finalize, dim3(gf), dim3(kBlock), 0, stream, d, it);
+  }
+  hipLaunchKernelGGL(k_finish_bodies, dim3(gb), dim3(kBlock), 0, stream, d);
+  if (prof_on) HIP_TRY(hipEventRecord(ev[2], stream));
+  // ---- FindNewContacts at the end of Solve
+  if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
+    if ((rc = broadphase(false))) return rc;
+  }
+  if (prof_on) HIP_TRY(hipEventRecord(ev[3], stream));
+  if (!(flags & B2G_STEP_ASYNC) || prof_on) {
+    if ((rc = read_counters())) return rc;
+    est_contacts = pinned[C_CONTACTS];
+    if (pinned[C_STATUS] & S_CONTACT_OVERFLOW)
+      return fail(B2G_ERR_CAPACITY, "contact capacity exceeded (" + std::to_string(pinned[C_PAIRS]) + " pairs)");
+    if (pinned[C_STATUS] & S_STACK_OVERFLOW) return fail(B2G_ERR_CAPACITY, "BVH traversal stack overflow");
+  }
+  if (prof_on) {
+    float t01, t12, t23, t03;
+    HIP_TRY(hipEventElapsedTime(&t01, ev[0], ev[1]));
+    HIP_TRY(hipEventElapsedTime(&t12, ev[1], ev[2]));
+    HIP_TRY(hipEventElapsedTime(&t23, ev[2], ev[3]));
+    HIP_TRY(hipEventElapsedTime(&t03, ev[0], ev[3]));
+    prof.collide = t01;
+    prof.solve = t12 + t23;  // b2Profile::solve includes the broadphase
+    prof.broadphase = t23;
+    prof.step = t03;
+  }
+  return B2G_OK;
+}
+
+// ---------------------------------------------------------------------------
+// record conversion helpers
+// ---------------------------------------------------------------------------
+static void shape_to_device(const b2g_shape& s, int world, int4* hdr, float4* geo, float4* mat, int4* filt) {
+  *hdr = make_int4(s.type, s.body, s.count, world);
+  geo[0] = make_float4(s.radius, s.centroid[0], s.centroid[1], 0.0f);
+  for (int k = 0; k < 4; ++k) {
+    geo[1 + k] = make_float4(s.vertices[4 * k], s.vertices[4 * k + 1], s.vertices[4 * k + 2], s.vertices[4 * k + 3]);
+    geo[5 + k] = make_float4(s.normals[4 * k], s.normals[4 * k + 1], s.normals[4 * k + 2], s.normals[4 * k + 3]);
+  }
+  *mat = make_float4(s.friction, s.restitution, s.restitution_threshold, 0.0f);
+  *filt = make_int4(s.category, s.mask, s.group, s.is_sensor);
+}
+
+static void manifold_to_record(const float4* m, b2g_contact* o) {
+  const int packed = __builtin_bit_cast(int, m[3].z);
+  o->type = packed & 0xff;
+  o->point_count = (packed >> 8) & 0xff;
+  o->touching = o->point_count > 0;
+  o->local_normal[0] = m[2].x;
+  o->local_normal[1] = m[2].y;
+  o->local_point[0] = m[2].z;
+  o->local_point[1] = m[2].w;
+  o->point_local[0] = m[0].x;
+  o->point_local[1] = m[0].y;
+  o->point_local[2] = m[1].x;
+  o->point_local[3] = m[1].y;
+  o->normal_impulse[0] = m[0].z;
+  o->tangent_impulse[0] = m[0].w;
+  o->normal_impulse[1] = m[1].z;
+  o->tangent_impulse[1] = m[1].w;
+  o->id[0] = __builtin_bit_cast(uint32_t, m[3].x);
+  o->id[1] = __builtin_bit_cast(uint32_t, m[3].y);
+  if (o->point_count < 2) {
+    o->point_local[2] = o->point_local[3] = 0.0f;
+    o->normal_impulse[1] = o->tangent_impulse[1] = 0.0f;
+    o->id[1] = 0;
+  }
+  if (o->point_count < 1) {
+    o->point_local[0] = o->point_local[1] = 0.0f;
+    o->normal_impulse[0] = o->tangent_impulse[0] = 0.0f;
+    o->id[0] = 0;
+  }
+}
+
+static void record_to_manifold(const b2g_contact& o, float4* m) {
+  m[0] = make_float4(o.point_local[0], o.point_local[1], o.normal_impulse[0], o.tangent_impulse[0]);
+  m[1] = make_float4(o.point_local[2], o.point_local[3], o.normal_impulse[1], o.tangent_impulse[1]);
+  m[2] = make_float4(o.local_normal[0], o.local_normal[1], o.local_point[0], o.local_point[1]);
+  const int packed = (o.type & 0xff) | ((o.point_count & 0xff) << 8);
+  m[3] = make_float4(__builtin_bit_cast(float, o.id[0]), __builtin_bit_cast(float, o.id[1]),
+                     __builtin_bit_cast(float, packed), 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* b2g_last_error(void) { return g_last_error.c_str(); }
+
+int b2g_device_count(int32_t* count) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  *count = n;
+  return B2G_OK;
+}
+
+int b2g_batch_create(b2g_batch** out, const b2g_capacity* cap) {
+  if (!out || !cap) return fail(B2G_ERR_INVALID, "null argument");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(B2G_ERR_NO_DEVICE, "no HIP device visible");
+  if (cap->contacts >= (1 << 24)) return fail(B2G_ERR_INVALID, "contact capacity must stay below 2^24");
+  b2g_batch* b = new b2g_batch();
+  b->cap = *cap;
+  HIP_TRY(hipGetDevice(&b->dev_id));
+  int rc = b->alloc();
+  if (rc) {
+    b->release();
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return B2G_OK;
+}
+
+int b2g_batch_destroy(b2g_batch* b) {
+  if (!b) return B2G_OK;
+  (void)hipStreamSynchronize(b->stream);
+  b->release();
+  delete b;
+  return B2G_OK;
+}
+
+int b2g_set_world_params(b2g_batch* b, int32_t first, int32_t count, const b2g_world_params* p) {
+  if (!b || first < 0 || count < 0 || first + count > b->cap.worlds) return fail(B2G_ERR_INVALID, "world range");
+  std::vector<float4> wp(count);
+  std::vector<int> wf(count);
+  for (int i = 0; i < count; ++i) {
+    b->h_params[first + i] = p[i];
+    wp[i] = make_float4(p[i].gravity[0], p[i].gravity[1], p[i].dt, p[i].dt_ratio);
+    wf[i] = (p[i].warm_starting ? 1 : 0) | (p[i].active ? 2 : 0);
+  }
+  HIP_TRY(hipMemcpyAsync(b->d.wpar + first, wp.data(), count * sizeof(float4), hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipMemcpyAsync(b->d.wflag + first, wf.data(), count * sizeof(int), hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->d.nw = std::max(b->d.nw, first + count);
+  return B2G_OK;
+}
+
+int b2g_set_worlds(b2g_batch* b, int32_t count, const int32_t* body_begin, const int32_t* shape_begin,
+                   const b2g_world_params* params) {
+  (void)body_begin;
+  (void)shape_begin;
+  if (!b || count < 1 || count > b->cap.worlds) return fail(B2G_ERR_INVALID, "world count");
+  b->d.nw = count;
+  return b2g_set_world_params(b, 0, count, params);
+}
+
+int b2g_upload_bodies(b2g_batch* b, int32_t first, int32_t count, const b2g_body* in) {
+  if (!b || first < 0 || count < 0 || first + count > b->cap.bodies) return fail(B2G_ERR_INVALID, "body range");
+  if (count == 0) return B2G_OK;
+  std::vector<float4> xf(count), pos(count), vel(count), prop(count), prop2(count), force(count);
+  std::vector<int> type(count), world(count);
+  for (int i = 0; i < count; ++i) {
+    const b2g_body& s = in[i];
+    if (s.world < 0 || s.world >= b->cap.worlds) return fail(B2G_ERR_INVALID, "body world index");
+    xf[i] = make_float4(s.xf[0], s.xf[1], s.xf[2], s.xf[3]);
+    pos[i] = make_float4(s.c[0], s.c[1], s.a, 0.0f);
+    vel[i] = make_float4(s.v[0], s.v[1], s.w, 0.0f);
+    prop[i] = make_float4(s.inv_mass, s.inv_i, s.local_center[0], s.local_center[1]);
+    prop2[i] = make_float4(s.mass, s.gravity_scale, s.linear_damping, s.angular_damping);
+    force[i] = make_float4(s.force[0], s.force[1], s.torque, 0.0f);
+    type[i] = s.type;
+    world[i] = s.world;
+    b->h_btype[first + i] = s.type;
+    b->h_bworld[first + i] = s.world;
+  }
+  hipStream_t st = b->stream;
+  Dev& d = b->d;
+  HIP_TRY(hipMemcpyAsync(d.xf + first, xf.data(), count * sizeof(float4), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(d.pos + first, pos.data(), count * sizeof(float4), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(d.vel + first, vel.data(), count * sizeof(float4), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(d.prop + first, prop.data(), count * sizeof(float4), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(d.prop2 + first, prop2.data(), count * sizeof(float4), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(d.force + first, force.data(), count * sizeof(float4), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(d.btype + first, type.data(), count * sizeof(int), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(d.bworld + first, world.data(), count * sizeof(int), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (first + count > d.nb) {
+    // extend the joint-partner CSR with empty rows
+    const int old_nb = d.nb;
+    d.nb = first + count;
+    if (b->h_joints.empty()) {
+      std::vector<int> z(d.nb - old_nb, 0);
+      HIP_TRY(hipMemcpy(d.bj_off + old_nb + 1, z.data(), z.size() * sizeof(int), hipMemcpyHostToDevice));
+    } else {
+      b->joints_dirty = true;
+    }
+  }
+  return B2G_OK;
+}
+
+int b2g_upload_shapes(b2g_batch* b, int32_t first, int32_t count, const b2g_shape* in) {
+  if (!b || first < 0 || count < 0 || first + count > b->cap.shapes) return fail(B2G_ERR_INVALID, "shape range");
+  if (count == 0) return B2G_OK;
+  std::vector<int4> hdr(count), filt(count);
+  std::vector<float4> geo(9 * (size_t)count), mat(count);
+  for (int i = 0; i < count; ++i) {
+    const b2g_shape& s = in[i];
+    if (s.body < 0 || s.body >= b->d.nb) return fail(B2G_ERR_INVALID, "shape body index (upload bodies first)");
+    if (s.is_sensor) return fail(B2G_ERR_UNSUPPORTED, "sensor fixtures are not supported on the device path");
+    if (s.type < 0 || s.type > 2 || (s.type == B2G_POLYGON && (s.count < 3 || s.count > 8)))
+      return fail(B2G_ERR_INVALID, "shape type / vertex count");
+    shape_to_device(s, b->h_bworld[s.body], &hdr[i], &geo[9 * (size_t)i], &mat[i], &filt[i]);
+    b->h_shape_body[first + i] = s.body;
+  }
+  hipStream_t st = b->stream;
+  Dev& d = b->d;
+  HIP_TRY(hipMemcpyAsync(d.shdr + first, hdr.data(), count * sizeof(int4), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(d.geo + 9 * (size_t)first, geo.data(), 9 * (size_t)count * sizeof(float4),
+                         hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(d.smat + first, mat.data(), count * sizeof(float4), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(d.sfilt + first, filt.data(), count * sizeof(int4), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  d.ns = std::max(d.ns, first + count);
+  return B2G_OK;
+}
+
+int b2g_upload_joints(b2g_batch* b, int32_t first, int32_t count, const b2g_revolute* in) {
+  if (!b || first < 0 || count < 0 || first + count > b->cap.joints) return fail(B2G_ERR_INVALID, "joint range");
+  if (count == 0) return B2G_OK;
+  if ((int)b->h_joints.size() < first + count) b->h_joints.resize(first + count);
+  std::vector<Joint> dj(count);
+  for (int i = 0; i < count; ++i) {
+    const b2g_revolute& r = in[i];
+    if (r.body_a < 0 || r.body_a >= b->d.nb || r.body_b < 0 || r.body_b >= b->d.nb)
+      return fail(B2G_ERR_INVALID, "joint body index");
+    b->h_joints[first + i] = r;
+    Joint& j = dj[i];
+    std::memset(&j, 0, sizeof(j));
+    j.a = r.body_a;
+    j.b = r.body_b;
+    j.flags = r.flags;
+    j.world = b->h_bworld[r.body_a];
+    j.la_x = r.local_anchor_a[0]; j.la_y = r.local_anchor_a[1];
+    j.lb_x = r.local_anchor_b[0]; j.lb_y = r.local_anchor_b[1];
+    j.ref = r.reference_angle; j.lower = r.lower_angle; j.upper = r.upper_angle;
+    j.speed = r.motor_speed; j.max_torque = r.max_motor_torque;
+    j.imp_x = r.impulse[0]; j.imp_y = r.impulse[1];
+    j.motor = r.motor_impulse; j.lower_imp = r.lower_impulse; j.upper_imp = r.upper_impulse;
+  }
+  HIP_TRY(hipMemcpy(b->d.joints + first, dj.data(), count * sizeof(Joint), hipMemcpyHostToDevice));
+  b->d.nj = std::max(b->d.nj, first + count);
+  b->joints_dirty = true;
+  return B2G_OK;
+}
+
+int b2g_upload_contacts(b2g_batch* b, int32_t count, const b2g_contact* in) {
+  if (!b || count < 0 || count > b->cap.contacts) return fail(B2G_ERR_INVALID, "contact count");
+  std::vector<int> idx(count);
+  std::vector<uint64_t> keys(count);
+  for (int i = 0; i < count; ++i) {
+    const uint32_t a = (uint32_t)in[i].shape_a, c = (uint32_t)in[i].shape_b;
+    keys[i] = ((uint64_t)std::min(a, c) << 32) | std::max(a, c);
+    idx[i] = i;
+  }
+  std::sort(idx.begin(), idx.end(), [&](int x, int y) { return keys[x] < keys[y]; });
+  std::vector<uint64_t> k(count);
+  std::vector<int2> pr(count);
+  std::vector<float4> man(4 * (size_t)count), mat(count);
+  std::vector<int> col(count, -1);
+  for (int q = 0; q < count; ++q) {
+    const b2g_contact& c = in[idx[q]];
+    k[q] = keys[idx[q]];
+    pr[q] = make_int2(c.shape_a, c.shape_b);
+    record_to_manifold(c, &man[4 * (size_t)q]);
+    mat[q] = make_float4(c.friction, c.restitution, c.restitution_threshold, c.tangent_speed);
+  }
+  Dev& d = b->d;
+  if (count) {
+    HIP_TRY(hipMemcpy(d.ckey, k.data(), count * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.cpair, pr.data(), count * sizeof(int2), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.cman, man.data(), man.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.cmat, mat.data(), count * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.ccol, col.data(), count * sizeof(int), hipMemcpyHostToDevice));
+  }
+  HIP_TRY(hipMemcpy(d.cnt + C_CONTACTS, &count, sizeof(int), hipMemcpyHostToDevice));
+  b->est_contacts = count;
+  return B2G_OK;
+}
+
+int b2g_batch_step(b2g_batch* b, int32_t vi, int32_t pi, uint32_t flags) {
+  if (!b) return fail(B2G_ERR_INVALID, "null batch");
+  if (vi < 0 || pi < 0) return fail(B2G_ERR_INVALID, "iteration counts");
+  return b->step(vi, pi, flags);
+}
+
+int b2g_batch_wait(b2g_batch* b) {
+  if (!b) return fail(B2G_ERR_INVALID, "null batch");
+  int rc = b->read_counters();
+  if (rc) return rc;
+  b->est_contacts = b->pinned[C_CONTACTS];
+  if (b->pinned[C_STATUS] & S_CONTACT_OVERFLOW) return fail(B2G_ERR_CAPACITY, "contact capacity exceeded");
+  if (b->pinned[C_STATUS] & S_STACK_OVERFLOW) return fail(B2G_ERR_CAPACITY, "BVH traversal stack overflow");
+  return B2G_OK;
+}
+
+int b2g_get_counts(b2g_batch* b, b2g_counts* out) {
+  if (!b || !out) return fail(B2G_ERR_INVALID, "null argument");
+  int rc = b->read_counters();
+  if (rc) return rc;
+  out->worlds = b->d.nw;
+  out->bodies = b->d.nb;
+  out->shapes = b->d.ns;
+  out->joints = b->d.nj;
+  out->contacts = b->pinned[C_CONTACTS];
+  out->touching = b->pinned[C_CONS];
+  out->colours = b->last_ncolours;
+  out->islands = b->last_islands;
+  out->pos_iters_run = b->last_pos_iters;
+  out->status = b->pinned[C_STATUS];
+  return B2G_OK;
+}
+
+int b2g_get_profile(b2g_batch* b, b2g_profile* out) {
+  if (!b || !out) return fail(B2G_ERR_INVALID, "null argument");
+  *out = b->prof;
+  return B2G_OK;
+}
+
+int b2g_download_bodies(b2g_batch* b, int32_t first, int32_t count, b2g_body* out) {
+  if (!b || first < 0 || count < 0 || first + count > b->d.nb) return fail(B2G_ERR_INVALID, "body range");
+  if (count == 0) return B2G_OK;
+  std::vector<float4> xf(count), pos(count), vel(count), prop(count), prop2(count), force(count);
+  Dev& d = b->d;
+  hipStream_t st = b->stream;
+  HIP_TRY(hipMemcpyAsync(xf.data(), d.xf + first, count * sizeof(float4), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(pos.data(), d.pos + first, count * sizeof(float4), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(vel.data(), d.vel + first, count * sizeof(float4), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(prop.data(), d.prop + first, count * sizeof(float4), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(prop2.data(), d.prop2 + first, count * sizeof(float4), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(force.data(), d.force + first, count * sizeof(float4), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  for (int i = 0; i < count; ++i) {
+    b2g_body& o = out[i];
+    std::memset(&o, 0, sizeof(o));
+    o.type = b->h_btype[first + i];
+    o.world = b->h_bworld[first + i];
+    o.xf[0] = xf[i].x; o.xf[1] = xf[i].y; o.xf[2] = xf[i].z; o.xf[3] = xf[i].w;
+    o.c[0] = pos[i].x; o.c[1] = pos[i].y; o.a = pos[i].z;
+    o.v[0] = vel[i].x; o.v[1] = vel[i].y; o.w = vel[i].z;
+    o.inv_mass = prop[i].x; o.inv_i = prop[i].y;
+    o.local_center[0] = prop[i].z; o.local_center[1] = prop[i].w;
+    o.mass = prop2[i].x; o.gravity_scale = prop2[i].y;
+    o.linear_damping = prop2[i].z; o.angular_damping = prop2[i].w;
+    o.force[0] = force[i].x; o.force[1] = force[i].y; o.torque = force[i].z;
+  }
+  return B2G_OK;
+}
+
+int b2g_download_contacts(b2g_batch* b, int32_t first, int32_t count, b2g_contact* out) {
+  if (!b) return fail(B2G_ERR_INVALID, "null batch");
+  int rc = b->read_counters();
+  if (rc) return rc;
+  const int n = b->pinned[C_CONTACTS];
+  if (!out) return n;  // count query
+  if (first < 0 || count < 0 || first + count > n) return fail(B2G_ERR_INVALID, "contact range");
+  if (count == 0) return B2G_OK;
+  std::vector<int2> pr(count);
+  std::vector<float4> man(4 * (size_t)count), mat(count);
+  Dev& d = b->d;
+  HIP_TRY(hipMemcpy(pr.data(), d.cpair + first, count * sizeof(int2), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(man.data(), d.cman + 4 * (size_t)first, man.size() * sizeof(float4), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(mat.data(), d.cmat + first, count * sizeof(float4), hipMemcpyDeviceToHost));
+  for (int i = 0; i < count; ++i) {
+    b2g_contact& o = out[i];
+    std::memset(&o, 0, sizeof(o));
+    o.shape_a = pr[i].x;
+    o.shape_b = pr[i].y;
+    manifold_to_record(&man[4 * (size_t)i], &o);
+    o.friction = mat[i].x;
+    o.restitution = mat[i].y;
+    o.restitution_threshold = mat[i].z;
+    o.tangent_speed = mat[i].w;
+  }
+  return B2G_OK;
+}
+
+int b2g_download_contact_colours(b2g_batch* b, int32_t first, int32_t count, int32_t* out) {
+  if (!b || !out) return fail(B2G_ERR_INVALID, "null argument");
+  // colours are kept per contact in the (post-broadphase) contact buffer
+  HIP_TRY(hipMemcpy(out, b->d.ccol + first, count * sizeof(int), hipMemcpyDeviceToHost));
+  return B2G_OK;
+}
+
+int b2g_download_joints(b2g_batch* b, int32_t first, int32_t count, b2g_revolute* out) {
+  if (!b || first < 0 || count < 0 || first + count > b->d.nj) return fail(B2G_ERR_INVALID, "joint range");
+  if (count == 0) return B2G_OK;
+  std::vector<Joint> dj(count);
+  HIP_TRY(hipMemcpy(dj.data(), b->d.joints + first, count * sizeof(Joint), hipMemcpyDeviceToHost));
+  for (int i = 0; i < count; ++i) {
+    b2g_revolute r = b->h_joints[first + i];
+    r.impulse[0] = dj[i].imp_x;
+    r.impulse[1] = dj[i].imp_y;
+    r.motor_impulse = dj[i].motor;
+    r.lower_impulse = dj[i].lower_imp;
+    r.upper_impulse = dj[i].upper_imp;
+    out[i] = r;
+  }
+  return B2G_OK;
+}
+
+int b2g_collide_pairs(int32_t n, const b2g_shape* sa, const float* xfa, const b2g_shape* sb, const float* xfb,
+                      b2g_contact* out) {
+  if (n < 0) return fail(B2G_ERR_INVALID, "negative count");
+  if (n == 0) return B2G_OK;
+  std::vector<int4> ha(n), hb(n), fa(n), fb(n);
+  std::vector<float4> ga(9 * (size_t)n), gb(9 * (size_t)n), ma(n), mb(n);
+  for (int i = 0; i < n; ++i) {
+    shape_to_device(sa[i], 0, &ha[i], &ga[9 * (size_t)i], &ma[i], &fa[i]);
+    shape_to_device(sb[i], 0, &hb[i], &gb[9 * (size_t)i], &mb[i], &fb[i]);
+  }
+  int4 *dha = nullptr, *dhb = nullptr;
+  float4 *dga = nullptr, *dgb = nullptr, *dxa = nullptr, *dxb = nullptr, *dout = nullptr;
+  int* dok = nullptr;
+  int rc = B2G_OK;
+  if ((rc = dalloc(&dha, n)) || (rc = dalloc(&dhb, n)) || (rc = dalloc(&dga, 9 * (size_t)n)) ||
+      (rc = dalloc(&dgb, 9 * (size_t)n)) || (rc = dalloc(&dxa, n)) || (rc = dalloc(&dxb, n)) ||
+      (rc = dalloc(&dout, 4 * (size_t)n)) || (rc = dalloc(&dok, n))) {
+    // fall through to cleanup
+  } else {
+    std::vector<float4> out4(4 * (size_t)n);
+    std::vector<int> ok(n);
+    hipError_t e = hipMemcpy(dha, ha.data(), n * sizeof(int4), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dhb, hb.data(), n * sizeof(int4), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dga, ga.data(), ga.size() * sizeof(float4), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dgb, gb.data(), gb.size() * sizeof(float4), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dxa, xfa, n * sizeof(float4), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dxb, xfb, n * sizeof(float4), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_collide_pairs, dim3(grid_for(n)), dim3(kBlock), 0, 0, n, dha, dga, dxa, dhb, dgb, dxb,
+                         dout, dok);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out4.data(), dout, out4.size() * sizeof(float4), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(ok.data(), dok, n * sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      rc = fail(B2G_ERR_HIP, std::string("b2g_collide_pairs: ") + hipGetErrorString(e));
+    } else {
+      for (int i = 0; i < n; ++i) {
+        b2g_contact& o = out[i];
+        std::memset(&o, 0, sizeof(o));
+        o.shape_a = sa[i].body;
+        o.shape_b = sb[i].body;
+        manifold_to_record(&out4[4 * (size_t)i], &o);
+        if (!ok[i]) o.point_count = -1;
+      }
+    }
+  }
+  void* ptrs[] = {dha, dhb, dga, dgb, dxa, dxb, dout, dok};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  return rc;
+}
+
+int b2g_find_pairs(int32_t n, const float* aabbs, const int32_t* shape_body, const int32_t* shape_type,
+                   const int32_t* filter, int32_t body_count, const int32_t* body_type, int64_t max_pairs,
+                   uint64_t* out_keys, int64_t* out_count) {
+  if (n < 0 || body_count < 0 || !out_count) return fail(B2G_ERR_INVALID, "bad arguments");
+  b2g_capacity cap{};
+  cap.worlds = 1;
+  cap.bodies = std::max(body_count, 1);
+  cap.shapes = std::max(n, 1);
+  cap.joints = 1;
+  cap.contacts = (int)std::min<int64_t>(std::max<int64_t>(max_pairs, 16), (1 << 24) - 1);
+  b2g_batch* b = nullptr;
+  int rc = b2g_batch_create(&b, &cap);
+  if (rc) return rc;
+  b2g_world_params wp{{0.0f, 0.0f}, 1.0f / 60.0f, 0.0f, 1, 1};
+  rc = b2g_set_worlds(b, 1, nullptr, nullptr, &wp);
+  std::vector<b2g_body> bodies(body_count);
+  for (int i = 0; i < body_count && !rc; ++i) {
+    std::memset(&bodies[i], 0, sizeof(b2g_body));
+    bodies[i].type = body_type[i];
+    bodies[i].xf[3] = 1.0f;
+  }
+  if (!rc) rc = b2g_upload_bodies(b, 0, body_count, bodies.data());
+  std::vector<b2g_shape> shapes(n);
+  for (int i = 0; i < n && !rc; ++i) {
+    b2g_shape& s = shapes[i];
+    std::memset(&s, 0, sizeof(s));
+    s.type = shape_type[i];
+    s.body = shape_body[i];
+    s.count = s.type == B2G_POLYGON ? 3 : 0;
+    s.category = filter ? filter[3 * i] : 1;
+    s.mask = filter ? filter[3 * i + 1] : 0xffff;
+    s.group = filter ? filter[3 * i + 2] : 0;
+  }
+  if (!rc) rc = b2g_upload_shapes(b, 0, n, shapes.data());
+  if (!rc && n) {
+    std::vector<float4> ab(n);
+    for (int i = 0; i < n; ++i) ab[i] = make_float4(aabbs[4 * i], aabbs[4 * i + 1], aabbs[4 * i + 2], aabbs[4 * i + 3]);
+    hipError_t e = hipMemcpy(b->d.aabb, ab.data(), n * sizeof(float4), hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = fail(B2G_ERR_HIP, hipGetErrorString(e));
+  }
+  if (!rc) rc = b->broadphase(true);
+  if (!rc) rc = b->read_counters();
+  if (!rc) {
+    const int64_t total = b->pinned[C_PAIRS];
+    *out_count = total;
+    const int64_t m = std::min<int64_t>(std::min<int64_t>(total, max_pairs), b->d.ccap);
+    if (m > 0 && out_keys) {
+      hipError_t e = hipMemcpy(out_keys, b->d.ckey, m * sizeof(uint64_t), hipMemcpyDeviceToHost);
+      if (e != hipSuccess) rc = fail(B2G_ERR_HIP, hipGetErrorString(e));
+    }
+  }
+  b2g_batch_destroy(b);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,1372 @@
+// b2g_kernels.h -- the kernels of one b2World::Step on device memory.
+//
+// Stage order follows b2World::Step (src/dynamics/b2_world.cpp:1108-1171):
+//   narrowphase (b2ContactManager::Collide) -> solve (b2World::Solve /
+//   b2Island::Solve with b2ContactSolver and b2RevoluteJoint) -> broadphase
+//   (b2ContactManager::FindNewContacts at the end of Solve) -> ClearForces.
+//
+// The island solver's sequential Gauss-Seidel is replaced by graph-coloured
+// Gauss-Seidel: constraints of one colour share no dynamic body and run in
+// parallel; colours run in sequence, joints before contacts in velocity
+// sweeps and contacts before joints in position sweeps, as b2Island::Solve
+// orders them. Position iterations stop per island (union-find components)
+// exactly as b2Island::Solve stops them.
+#pragma once
+
+#include "b2g_collide.h"
+#include "b2g_prims.h"
+
+namespace b2g {
+
+// ---- device records ---------------------------------------------------------
+struct Joint {  // b2RevoluteJoint state (src/dynamics/b2_revolute_joint.cpp:52-71)
+  int a, b, flags, world;
+  float la_x, la_y, lb_x, lb_y;
+  float ref, lower, upper, speed;
+  float max_torque, imp_x, imp_y, motor;
+  float lower_imp, upper_imp, pad0, pad1;
+};
+struct JointSolve {  // InitVelocityConstraints temporaries
+  float ra_x, ra_y, rb_x, rb_y;
+  float kxx, kxy, kyx, kyy;
+  float axial, angle, ma, mb;
+  float ia, ib, lca_x, lca_y;
+  float lcb_x, lcb_y, pad0, pad1;
+};
+
+enum Counter : int {
+  C_CONTACTS = 0,   // contacts in the current buffer
+  C_PAIRS = 1,      // pairs found by the last broadphase (may exceed capacity)
+  C_CONS = 2,       // touching contacts solved this step
+  C_STATUS = 3,     // sticky error bits
+  C_REMAIN = 4,     // constraints still uncoloured
+  C_NCOLOURS = 5,   // contact colours used (incl. the serial group)
+  C_MAXCOLOUR = 6,
+  C_COUNT = 16
+};
+enum Status : int { S_CONTACT_OVERFLOW = 1, S_STACK_OVERFLOW = 2 };
+enum BodyType : int { kStaticBody = 0, kKinematicBody = 1, kDynamicBody = 2 };
+constexpr int kMaxColours = 64;           // bit-mask colours; colour 64 = serial group
+constexpr int kSerialColour = kMaxColours;
+constexpr int kColourSlots = kMaxColours + 1;
+
+// all device arrays, passed by value to every kernel
+struct Dev {
+  int nb, ns, nj, nw;
+  // bodies
+  float4* xf;     // p.x, p.y, q.s, q.c
+  float4* pos;    // sweep c.x, c.y, a
+  float4* vel;    // v.x, v.y, w
+  float4* prop;   // invMass, invI, localCenter.x, localCenter.y
+  float4* prop2;  // mass, gravityScale, linearDamping, angularDamping
+  float4* force;  // f.x, f.y, torque
+  int* btype;
+  int* bworld;
+  // worlds
+  float4* wpar;   // gravity.x, gravity.y, dt, dtRatio
+  int* wflag;     // bit0 warm starting, bit1 active
+  // shapes
+  int4* shdr;     // type, body, count, world
+  float4* geo;    // 9 per shape
+  float4* smat;   // friction, restitution, restitutionThreshold, density
+  int4* sfilt;    // category, mask, group, sensor
+  float4* aabb;
+  // joints
+  Joint* joints;
+  JointSolve* jsol;
+  int* jorder;    // joint indices in colour order
+  int* jisland;   // island root per colour-ordered joint
+  int* bj_off;    // CSR body -> partner bodies of joints with collideConnected == false
+  int* bj_other;
+  // contacts (current / next)
+  uint64_t* ckey; int2* cpair; float4* cman; float4* cmat; int* ccol;
+  uint64_t* nkey; int2* npair; float4* nman; float4* nmat; int* ncol;
+  int ccap;
+  // constraints of this step
+  int* touch;     // per contact: 1 if solved this step
+  int* tpos;      // exclusive scan of touch
+  int* clist;     // compacted contact indices
+  int* ccolour;   // colour per compacted constraint
+  int* csorted;   // colour-sorted slot -> contact index
+  int* col_count; int* col_off; int* col_cursor;
+  float4* vc[10];
+  float4* pc[5];
+  int* cisland;
+  unsigned long long* bmask;
+  unsigned long long* bslot;
+  // islands
+  int* parent;
+  unsigned* minsep;  // 2 * nb
+  int* jok;          // 2 * nb
+  int* solved;       // nb
+  // broadphase
+  uint64_t* mkey[2]; uint32_t* mval[2];
+  int* sort_hist;
+  int* node_child;   // 2 per internal node: >= 0 internal, < 0 leaf ~k
+  int* node_parent;
+  int* leaf_parent;
+  int2* node_range;
+  float4* node_aabb;
+  float4* leaf_aabb;
+  int* node_visit;
+  int* wleaf;        // 2 per world: first leaf, last leaf + 1
+  unsigned* wbounds; // 4 per world (ordered floats): min x, min y, max x, max y
+  int* pcount;       // per shape pair count
+  int* poff;         // exclusive scan
+  int* scan_tiles;
+  int* cnt;
+};
+
+// grid-stride helpers with an XCD-aware block order: consecutive work tiles
+// go to the same XCD (blocks are dispatched round-robin over 8 XCDs), so
+// neighbouring constraints/bodies share an L2.
+__device__ __forceinline__ int xcd_block() {
+  const int g = (int)gridDim.x;
+  if (g < 16 || (g & 7)) return (int)blockIdx.x;
+  const int per = g >> 3;
+  return ((int)blockIdx.x & 7) * per + ((int)blockIdx.x >> 3);
+}
+#define B2G_FOR(i, begin, end)                                                     \
+  for (int i = (begin) + xcd_block() * (int)blockDim.x + (int)threadIdx.x; i < (end); \
+       i += (int)(gridDim.x * blockDim.x))
+
+__device__ __forceinline__ vec2 v2of(float4 f) { return mk(f.x, f.y); }
+__device__ __forceinline__ bool world_active(const Dev& d, int w) { return (d.wflag[w] & 2) != 0; }
+
+// ===========================================================================
+// Narrowphase: b2ContactManager::Collide + b2Contact::Update
+// (src/dynamics/b2_contact_manager.cpp:66-116, src/dynamics/b2_contact.cpp:126-210)
+// ===========================================================================
+__global__ void __launch_bounds__(kBlock) k_narrowphase(Dev d) {
+  const int n = d.cnt[C_CONTACTS];
+  B2G_FOR(i, 0, n) {
+    const int2 pr = d.cpair[i];
+    const int4 ha = d.shdr[pr.x], hb = d.shdr[pr.y];
+    int solve = 0;
+    if (world_active(d, ha.w)) {
+      Geo ga, gb;
+      load_geo(ga, ha.x, ha.z, d.geo + 9 * (size_t)pr.x);
+      load_geo(gb, hb.x, hb.z, d.geo + 9 * (size_t)pr.y);
+      const xform xa = xform_of(d.xf[ha.y]);
+      const xform xb = xform_of(d.xf[hb.y]);
+      float4* mp = d.cman + 4 * (size_t)i;
+      const float4 o0 = mp[0], o1 = mp[1], o3 = mp[3];
+      const int old_count = (as_int(o3.z) >> 8) & 0xff;
+      Man m;
+      m.count = 0;
+      m.type = (as_int(o3.z) & 0xff);
+      const float4 o2 = mp[2];
+      m.ln = mk(o2.x, o2.y);
+      m.lp = mk(o2.z, o2.w);
+      m.lp0 = mk(0.0f, 0.0f);
+      m.lp1 = mk(0.0f, 0.0f);
+      m.id0 = m.id1 = 0;
+      evaluate(m, ga, xa, gb, xb);
+      // match old feature keys to carry the impulses (b2_contact.cpp:158-181)
+      float ni0 = 0.0f, ti0 = 0.0f, ni1 = 0.0f, ti1 = 0.0f;
+      const uint32_t oid0 = as_uint(o3.x), oid1 = as_uint(o3.y);
+      if (m.count > 0) {
+        if (old_count > 0 && oid0 == m.id0) { ni0 = o0.z; ti0 = o0.w; }
+        else if (old_count > 1 && oid1 == m.id0) { ni0 = o1.z; ti0 = o1.w; }
+      }
+      if (m.count > 1) {
+        if (old_count > 0 && oid0 == m.id1) { ni1 = o0.z; ti1 = o0.w; }
+        else if (old_count > 1 && oid1 == m.id1) { ni1 = o1.z; ti1 = o1.w; }
+      }
+      mp[0] = make_float4(m.lp0.x, m.lp0.y, ni0, ti0);
+      mp[1] = make_float4(m.count > 1 ? m.lp1.x : 0.0f, m.count > 1 ? m.lp1.y : 0.0f, ni1, ti1);
+      mp[2] = make_float4(m.ln.x, m.ln.y, m.lp.x, m.lp.y);
+      mp[3] = make_float4(uint_as_float(m.count > 0 ? m.id0 : 0u), uint_as_float(m.count > 1 ? m.id1 : 0u),
+                          as_float((m.type & 0xff) | (m.count << 8)), 0.0f);
+      solve = m.count > 0 ? 1 : 0;
+      // a contact that is not solved this step gives up its colour: colours
+      // kept across steps are then always a subset of the previous step's
+      // valid colouring
+      if (!solve) d.ccol[i] = -1;
+    }
+    d.touch[i] = solve;
+  }
+}
+
+// compacted list of solved contacts (order preserved)
+__global__ void __launch_bounds__(kBlock) k_compact_touching(Dev d) {
+  const int n = d.cnt[C_CONTACTS];
+  B2G_FOR(i, 0, n) {
+    if (d.touch[i]) d.clist[d.tpos[i]] = i;
+  }
+}
+
+// ===========================================================================
+// Velocity integration (b2_island.cpp:257-298, SolveOrphan :187-206)
+// ===========================================================================
+__global__ void __launch_bounds__(kBlock) k_integrate_velocities(Dev d) {
+  B2G_FOR(b, 0, d.nb) {
+    d.parent[b] = b;
+    d.bmask[b] = 0ull;
+    const int w = d.bworld[b];
+    if (!world_active(d, w) || d.btype[b] != kDynamicBody) continue;
+    const float4 wp = d.wpar[w];
+    const float h = wp.z;
+    const float4 v4 = d.vel[b], p = d.prop[b], p2 = d.prop2[b], f = d.force[b];
+    // v += h * invMass * (gravityScale * mass * gravity + force)
+    const float s1 = h * p.x;
+    const float s2 = p2.y * p2.x;
+    float vx = v4.x + s1 * (s2 * wp.x + f.x);
+    float vy = v4.y + s1 * (s2 * wp.y + f.y);
+    float w_ = v4.z + h * p.y * f.z;
+    const float ld = 1.0f + h * p2.z;
+    const float ad = 1.0f + h * p2.w;
+    vx = vx / ld;
+    vy = vy / ld;
+    w_ = w_ / ad;
+    d.vel[b] = make_float4(vx, vy, w_, 0.0f);
+  }
+}
+
+// ===========================================================================
+// Graph colouring (Jones-Plassmann rounds with per-body colour masks)
+// Contacts keep their colour from the previous step while they stay
+// touching; only newly touching contacts are coloured, each taking the lowest
+// colour free at both of its dynamic bodies.
+// ===========================================================================
+__device__ __forceinline__ bool is_dyn(const Dev& d, int b) { return d.btype[b] == kDynamicBody; }
+
+__global__ void __launch_bounds__(kBlock) k_colour_init(Dev d) {
+  const int n = d.cnt[C_CONS];
+  B2G_FOR(k, 0, n) {
+    const int c = d.clist[k];
+    const int col = d.ccol[c];
+    if (col >= 0 && col < kMaxColours) {
+      const int2 pr = d.cpair[c];
+      const int ba = d.shdr[pr.x].y, bb = d.shdr[pr.y].y;
+      const unsigned long long bit = 1ull << col;
+      if (is_dyn(d, ba)) atomicOr(&d.bmask[ba], bit);
+      if (is_dyn(d, bb)) atomicOr(&d.bmask[bb], bit);
+      d.ccolour[k] = col;
+    } else {
+      d.ccolour[k] = -1;
+      atomicAdd(&d.cnt[C_REMAIN], 1);
+    }
+  }
+}
+
+__device__ __forceinline__ unsigned long long colour_priority(int round, int k, uint64_t key) {
+  const uint32_t h = hash32((uint32_t)key ^ (uint32_t)(key >> 32) * 0x9E3779B9u) & 0xffffffu;
+  return ((unsigned long long)(round & 0xffff) << 48) | ((unsigned long long)h << 24) |
+         (unsigned long long)(k & 0xffffff);
+}
+
+__global__ void __launch_bounds__(kBlock) k_colour_claim(Dev d, int round) {
+  if (d.cnt[C_REMAIN] == 0) return;
+  const int n = d.cnt[C_CONS];
+  B2G_FOR(k, 0, n) {
+    if (d.ccolour[k] >= 0) continue;
+    const int c = d.clist[k];
+    const int2 pr = d.cpair[c];
+    const int ba = d.shdr[pr.x].y, bb = d.shdr[pr.y].y;
+    const unsigned long long v = colour_priority(round, k, d.ckey[c]);
+    if (is_dyn(d, ba)) atomicMax(&d.bslot[ba], v);
+    if (is_dyn(d, bb)) atomicMax(&d.bslot[bb], v);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_colour_resolve(Dev d, int round) {
+  if (d.cnt[C_REMAIN] == 0) return;
+  const int n = d.cnt[C_CONS];
+  B2G_FOR(k, 0, n) {
+    if (d.ccolour[k] >= 0) continue;
+    const int c = d.clist[k];
+    const int2 pr = d.cpair[c];
+    const int ba = d.shdr[pr.x].y, bb = d.shdr[pr.y].y;
+    const unsigned long long v = colour_priority(round, k, d.ckey[c]);
+    const bool da = is_dyn(d, ba), db = is_dyn(d, bb);
+    const bool win = (!da || d.bslot[ba] == v) && (!db || d.bslot[bb] == v);
+    if (!win) continue;
+    const unsigned long long used = (da ? d.bmask[ba] : 0ull) | (db ? d.bmask[bb] : 0ull);
+    int col = kSerialColour;
+    if (~used != 0ull) col = __ffsll((long long)~used) - 1;
+    d.ccolour[k] = col;
+    if (col < kMaxColours) {
+      const unsigned long long bit = 1ull << col;
+      if (da) atomicOr(&d.bmask[ba], bit);
+      if (db) atomicOr(&d.bmask[bb], bit);
+    }
+    atomicSub(&d.cnt[C_REMAIN], 1);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_colour_count(Dev d) {
+  const int n = d.cnt[C_CONS];
+  B2G_FOR(k, 0, n) {
+    const int col = d.ccolour[k];
+    atomicAdd(&d.col_count[col], 1);
+    d.ccol[d.clist[k]] = col;
+  }
+}
+
+// one thread: offsets of the colour groups; resets counters for the next step
+__global__ void k_colour_offsets(Dev d) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int off = 0, ncol = 0;
+  for (int c = 0; c < kColourSlots; ++c) {
+    d.col_off[c] = off;
+    off += d.col_count[c];
+    if (d.col_count[c] > 0) ncol = c + 1;
+    d.col_cursor[c] = 0;
+  }
+  d.col_off[kColourSlots] = off;
+  d.cnt[C_NCOLOURS] = ncol;
+}
+
+__global__ void __launch_bounds__(kBlock) k_colour_scatter(Dev d) {
+  const int n = d.cnt[C_CONS];
+  B2G_FOR(k, 0, n) {
+    const int col = d.ccolour[k];
+    const int slot = d.col_off[col] + atomicAdd(&d.col_cursor[col], 1);
+    d.csorted[slot] = d.clist[k];
+  }
+}
+
+// the serial group runs on one lane; order it by contact index so its
+// Gauss-Seidel sequence is deterministic
+__global__ void k_sort_serial_group(Dev d) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int b = d.col_off[kSerialColour], e = d.col_off[kSerialColour + 1];
+  for (int a = b + 1; a < e; ++a) {
+    const int v = d.csorted[a];
+    int q = a - 1;
+    while (q >= b && d.csorted[q] > v) {
+      d.csorted[q + 1] = d.csorted[q];
+      --q;
+    }
+    d.csorted[q + 1] = v;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_colour_clear_counts(Dev d) {
+  for (int c = threadIdx.x; c < kColourSlots; c += blockDim.x) d.col_count[c] = 0;
+  if (threadIdx.x == 0) d.cnt[C_REMAIN] = 0;
+}
+
+// ===========================================================================
+// Islands: connected components over non-static bodies (b2World::Solve DFS,
+// src/dynamics/b2_world.cpp:526-659), by lock-free union-find.
+// ===========================================================================
+__device__ __forceinline__ int uf_find(int* parent, int x) {
+  int p = parent[x];
+  while (p != x) {
+    x = p;
+    p = parent[x];
+  }
+  return x;
+}
+
+__device__ __forceinline__ void uf_union(int* parent, int a, int b) {
+  for (;;) {
+    a = uf_find(parent, a);
+    b = uf_find(parent, b);
+    if (a == b) return;
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    if (atomicCAS(&parent[hi], hi, lo) == hi) return;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_island_link(Dev d) {
+  const int n = d.cnt[C_CONS];
+  B2G_FOR(k, 0, n + d.nj) {
+    int ba, bb;
+    if (k < n) {
+      const int2 pr = d.cpair[d.clist[k]];
+      ba = d.shdr[pr.x].y;
+      bb = d.shdr[pr.y].y;
+    } else {
+      const Joint& j = d.joints[k - n];
+      ba = j.a;
+      bb = j.b;
+      if (!world_active(d, j.world)) continue;
+    }
+    if (d.btype[ba] != kStaticBody && d.btype[bb] != kStaticBody) uf_union(d.parent, ba, bb);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_island_label(Dev d) {
+  B2G_FOR(b, 0, d.nb) {
+    if (d.btype[b] != kStaticBody) d.parent[b] = uf_find(d.parent, b);
+  }
+}
+
+__device__ __forceinline__ int island_of(const Dev& d, int ba, int bb) {
+  return d.btype[ba] != kStaticBody ? d.parent[ba] : d.parent[bb];
+}
+
+__device__ __forceinline__ void island_reset(const Dev& d, int isl) {
+  d.solved[isl] = 0;
+  d.minsep[isl] = float_to_ordered(0.0f);
+  d.jok[isl] = 1;
+}
+
+// ===========================================================================
+// b2ContactSolver::Initialize + InitializeVelocityConstraints
+// (src/dynamics/b2_contact_solver.cpp:65-258); one lane per solved contact,
+// in colour-sorted slot order. Velocity data (10 float4) and position data
+// (5 float4) are stored structure-of-arrays by slot.
+//   vc0 normal.x normal.y friction tangentSpeed   vc1 mA mB iA iB
+//   vc2 ia ib pointCount contact (int bits)         vc3/vc5 rA, rB of point 0/1
+//   vc4/vc6 normalImpulse tangentImpulse normalMass tangentMass of point 0/1
+//   vc7 bias0 bias1 K.ex.y K.ey.x                   vc8 normalMass (ex, ey)
+//   vc9 K (ex, ey)
+//   pc0 localPoint localNormal  pc1 localPoints[0..1]  pc2 localCenterA/B
+//   pc3 mA mB iA iB             pc4 radiusA radiusB type pointCount
+// ===========================================================================
+__global__ void __launch_bounds__(kBlock) k_contact_init(Dev d) {
+  const int n = d.cnt[C_CONS];
+  B2G_FOR(s, 0, n) {
+    const int c = d.csorted[s];
+    const int2 pr = d.cpair[c];
+    const int4 ha = d.shdr[pr.x], hb = d.shdr[pr.y];
+    const int ia = ha.y, ib = hb.y;
+    const float4 pa = d.prop[ia], pb = d.prop[ib];
+    const float mA = pa.x, iA = pa.y, mB = pb.x, iB = pb.y;
+    const vec2 lcA = mk(pa.z, pa.w), lcB = mk(pb.z, pb.w);
+    const float4 m0 = d.cman[4 * (size_t)c + 0], m1 = d.cman[4 * (size_t)c + 1];
+    const float4 m2 = d.cman[4 * (size_t)c + 2], m3 = d.cman[4 * (size_t)c + 3];
+    const int mtype = as_int(m3.z) & 0xff;
+    const int count = (as_int(m3.z) >> 8) & 0xff;
+    const float4 mat = d.cmat[c];
+    const float rA = d.geo[9 * (size_t)pr.x].x, rB = d.geo[9 * (size_t)pr.y].x;
+    const int w = ha.w;
+    const float4 wp = d.wpar[w];
+    const bool warm = (d.wflag[w] & 1) != 0;
+    const float ratio = wp.w;
+    const int isl = island_of(d, ia, ib);
+    d.cisland[s] = isl;
+    island_reset(d, isl);
+
+    float ni0 = 0.0f, ti0 = 0.0f, ni1 = 0.0f, ti1 = 0.0f;
+    if (warm) {
+      ni0 = ratio * m0.z;
+      ti0 = ratio * m0.w;
+      ni1 = ratio * m1.z;
+      ti1 = ratio * m1.w;
+    }
+    const float4 posA = d.pos[ia], posB = d.pos[ib];
+    const float4 velA = d.vel[ia], velB = d.vel[ib];
+    const vec2 cA = mk(posA.x, posA.y), cB = mk(posB.x, posB.y);
+    const vec2 vA = mk(velA.x, velA.y), vB = mk(velB.x, velB.y);
+    const float wA = velA.z, wB = velB.z;
+    xform xfA, xfB;
+    xfA.q = rot_of(posA.z);
+    xfB.q = rot_of(posB.z);
+    xfA.p = cA - mul(xfA.q, lcA);
+    xfB.p = cB - mul(xfB.q, lcB);
+    // b2WorldManifold::Initialize (src/collision/b2_collision.cpp:26-90)
+    vec2 normal, wp0 = mk(0.0f, 0.0f), wp1 = mk(0.0f, 0.0f);
+    const vec2 ln = mk(m2.x, m2.y), lp = mk(m2.z, m2.w);
+    const vec2 lp0 = mk(m0.x, m0.y), lp1 = mk(m1.x, m1.y);
+    if (mtype == kManCircles) {
+      normal = mk(1.0f, 0.0f);
+      const vec2 pA = mul(xfA, lp);
+      const vec2 pB = mul(xfB, lp0);
+      if (dist_sq(pA, pB) > kEpsilon * kEpsilon) normal = normalize(pB - pA);
+      const vec2 qA = pA + rA * normal;
+      const vec2 qB = pB - rB * normal;
+      wp0 = 0.5f * (qA + qB);
+    } else if (mtype == kManFaceA) {
+      normal = mul(xfA.q, ln);
+      const vec2 plane = mul(xfA, lp);
+      {
+        const vec2 clip = mul(xfB, lp0);
+        const vec2 qA = clip + (rA - dot(clip - plane, normal)) * normal;
+        const vec2 qB = clip - rB * normal;
+        wp0 = 0.5f * (qA + qB);
+      }
+      if (count > 1) {
+        const vec2 clip = mul(xfB, lp1);
+        const vec2 qA = clip + (rA - dot(clip - plane, normal)) * normal;
+        const vec2 qB = clip - rB * normal;
+        wp1 = 0.5f * (qA + qB);
+      }
+    } else {
+      normal = mul(xfB.q, ln);
+      const vec2 plane = mul(xfB, lp);
+      {
+        const vec2 clip = mul(xfA, lp0);
+        const vec2 qB = clip + (rB - dot(clip - plane, normal)) * normal;
+        const vec2 qA = clip - rA * normal;
+        wp0 = 0.5f * (qA + qB);
+      }
+      if (count > 1) {
+        const vec2 clip = mul(xfA, lp1);
+        const vec2 qB = clip + (rB - dot(clip - plane, normal)) * normal;
+        const vec2 qA = clip - rA * normal;
+        wp1 = 0.5f * (qA + qB);
+      }
+      normal = -normal;
+    }
+    const vec2 tangent = cross(normal, 1.0f);
+    vec2 rA0 = wp0 - cA, rB0 = wp0 - cB, rA1 = wp1 - cA, rB1 = wp1 - cB;
+    float nm0, tm0, bias0 = 0.0f, nm1 = 0.0f, tm1 = 0.0f, bias1 = 0.0f;
+    {
+      const float rnA = cross(rA0, normal), rnB = cross(rB0, normal);
+      const float kn = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
+      nm0 = kn > 0.0f ? 1.0f / kn : 0.0f;
+      const float rtA = cross(rA0, tangent), rtB = cross(rB0, tangent);
+      const float kt = mA + mB + iA * rtA * rtA + iB * rtB * rtB;
+      tm0 = kt > 0.0f ? 1.0f / kt : 0.0f;
+      const float vrel = dot(normal, vB + cross(wB, rB0) - (vA + cross(wA, rA0)));
+      if (vrel < -mat.z) bias0 = -mat.y * vrel;
+    }
+    if (count > 1) {
+      const float rnA = cross(rA1, normal), rnB = cross(rB1, normal);
+      const float kn = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
+      nm1 = kn > 0.0f ? 1.0f / kn : 0.0f;
+      const float rtA = cross(rA1, tangent), rtB = cross(rB1, tangent);
+      const float kt = mA + mB + iA * rtA * rtA + iB * rtB * rtB;
+      tm1 = kt > 0.0f ? 1.0f / kt : 0.0f;
+      const float vrel = dot(normal, vB + cross(wB, rB1) - (vA + cross(wA, rA1)));
+      if (vrel < -mat.z) bias1 = -mat.y * vrel;
+    }
+    int vcount = count;
+    mat22 K = {mk(0.0f, 0.0f), mk(0.0f, 0.0f)}, NM = K;
+    if (count == 2) {  // block solver preparation (:236-256)
+      const float rn1A = cross(rA0, normal), rn1B = cross(rB0, normal);
+      const float rn2A = cross(rA1, normal), rn2B = cross(rB1, normal);
+      const float k11 = mA + mB + iA * rn1A * rn1A + iB * rn1B * rn1B;
+      const float k22 = mA + mB + iA * rn2A * rn2A + iB * rn2B * rn2B;
+      const float k12 = mA + mB + iA * rn1A * rn2A + iB * rn1B * rn2B;
+      const float kMaxCond = 1000.0f;
+      if (k11 * k11 < kMaxCond * (k11 * k22 - k12 * k12)) {
+        K.ex = mk(k11, k12);
+        K.ey = mk(k12, k22);
+        NM = inverse(K);
+      } else {
+        vcount = 1;
+      }
+    }
+    d.vc[0][s] = make_float4(normal.x, normal.y, mat.x, mat.w);
+    d.vc[1][s] = make_float4(mA, mB, iA, iB);
+    d.vc[2][s] = make_float4(as_float(ia), as_float(ib), as_float(vcount), as_float(c));
+    d.vc[3][s] = make_float4(rA0.x, rA0.y, rB0.x, rB0.y);
+    d.vc[4][s] = make_float4(ni0, ti0, nm0, tm0);
+    d.vc[5][s] = make_float4(rA1.x, rA1.y, rB1.x, rB1.y);
+    d.vc[6][s] = make_float4(ni1, ti1, nm1, tm1);
+    d.vc[7][s] = make_float4(bias0, bias1, K.ex.y, K.ey.x);
+    d.vc[8][s] = make_float4(NM.ex.x, NM.ex.y, NM.ey.x, NM.ey.y);
+    d.vc[9][s] = make_float4(K.ex.x, K.ex.y, K.ey.x, K.ey.y);
+    d.pc[0][s] = make_float4(lp.x, lp.y, ln.x, ln.y);
+    d.pc[1][s] = make_float4(lp0.x, lp0.y, lp1.x, lp1.y);
+    d.pc[2][s] = make_float4(lcA.x, lcA.y, lcB.x, lcB.y);
+    d.pc[3][s] = make_float4(mA, mB, iA, iB);
+    d.pc[4][s] = make_float4(rA, rB, as_float(mtype), as_float(count));
+  }
+}
+
+__device__ __forceinline__ bool writes_body(const Dev& d, int b) { return d.btype[b] == kDynamicBody; }
+
+// b2ContactSolver::WarmStart (:260-299) for slots [begin, end)
+__global__ void __launch_bounds__(kBlock) k_warm_start(Dev d, int begin, int end) {
+  B2G_FOR(s, begin, end) {
+    const float4 v2i = d.vc[2][s];
+    const int ia = as_int(v2i.x), ib = as_int(v2i.y), count = as_int(v2i.z);
+    if ((d.wflag[d.bworld[ia]] & 1) == 0) continue;
+    const float4 n4 = d.vc[0][s], m = d.vc[1][s];
+    const vec2 normal = mk(n4.x, n4.y), tangent = cross(normal, 1.0f);
+    float4 va = d.vel[ia], vb = d.vel[ib];
+    vec2 vA = mk(va.x, va.y), vB = mk(vb.x, vb.y);
+    float wA = va.z, wB = vb.z;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      if (j < count) {
+        const float4 r = d.vc[3 + 2 * j][s], im = d.vc[4 + 2 * j][s];
+        const vec2 P = im.x * normal + im.y * tangent;
+        wA -= m.z * cross(mk(r.x, r.y), P);
+        vA = vA - m.x * P;
+        wB += m.w * cross(mk(r.z, r.w), P);
+        vB = vB + m.y * P;
+      }
+    }
+    if (writes_body(d, ia)) d.vel[ia] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (writes_body(d, ib)) d.vel[ib] = make_float4(vB.x, vB.y, wB, 0.0f);
+  }
+}
+
+// b2ContactSolver::SolveVelocityConstraints (:301-639) for slots [begin, end)
+__device__ __forceinline__ void contact_velocity(const Dev& d, int s) {
+  const float4 v2i = d.vc[2][s];
+  const int ia = as_int(v2i.x), ib = as_int(v2i.y), count = as_int(v2i.z);
+  const float4 n4 = d.vc[0][s], m = d.vc[1][s];
+  const float mA = m.x, mB = m.y, iA = m.z, iB = m.w;
+  const vec2 normal = mk(n4.x, n4.y);
+  const float friction = n4.z, tangent_speed = n4.w;
+  float4 va = d.vel[ia], vb = d.vel[ib];
+  vec2 vA = mk(va.x, va.y), vB = mk(vb.x, vb.y);
+  float wA = va.z, wB = vb.z;
+  const float4 r0 = d.vc[3][s], r1 = d.vc[5][s];
+  float4 q0 = d.vc[4][s], q1 = d.vc[6][s];
+  const vec2 rA0 = mk(r0.x, r0.y), rB0 = mk(r0.z, r0.w);
+  const vec2 rA1 = mk(r1.x, r1.y), rB1 = mk(r1.z, r1.w);
+  const vec2 tangent = cross(normal, 1.0f);
+  // friction first (:324-351)
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    if (j < count) {
+      float4& q = j == 0 ? q0 : q1;
+      const vec2 rA = j == 0 ? rA0 : rA1, rB = j == 0 ? rB0 : rB1;
+      const vec2 dv = vB + cross(wB, rB) - (vA + cross(wA, rA));
+      const float vt = dot(dv, tangent) - tangent_speed;
+      float lambda = q.w * (-vt);
+      const float max_friction = friction * q.x;
+      const float new_impulse = clampf(q.y + lambda, -max_friction, max_friction);
+      lambda = new_impulse - q.y;
+      q.y = new_impulse;
+      const vec2 P = lambda * tangent;
+      vA = vA - mA * P;
+      wA -= iA * cross(rA, P);
+      vB = vB + mB * P;
+      wB += iB * cross(rB, P);
+    }
+  }
+  const float4 b7 = d.vc[7][s];
+  if (count == 1) {
+    const vec2 dv = vB + cross(wB, rB0) - (vA + cross(wA, rA0));
+    const float vn = dot(dv, normal);
+    float lambda = -q0.z * (vn - b7.x);
+    const float new_impulse = maxf(q0.x + lambda, 0.0f);
+    lambda = new_impulse - q0.x;
+    q0.x = new_impulse;
+    const vec2 P = lambda * normal;
+    vA = vA - mA * P;
+    wA -= iA * cross(rA0, P);
+    vB = vB + mB * P;
+    wB += iB * cross(rB0, P);
+  } else {
+    // 2-point block LCP by total enumeration (:389-631)
+    const float4 nm4 = d.vc[8][s], k4 = d.vc[9][s];
+    const mat22 K = {mk(k4.x, k4.y), mk(k4.z, k4.w)};
+    const mat22 NM = {mk(nm4.x, nm4.y), mk(nm4.z, nm4.w)};
+    const vec2 a = mk(q0.x, q1.x);
+    const vec2 dv1 = vB + cross(wB, rB0) - (vA + cross(wA, rA0));
+    const vec2 dv2 = vB + cross(wB, rB1) - (vA + cross(wA, rA1));
+    float vn1 = dot(dv1, normal);
+    float vn2 = dot(dv2, normal);
+    vec2 b = mk(vn1 - b7.x, vn2 - b7.y);
+    b = b - mul(K, a);
+    vec2 x;
+    bool solved = false;
+    x = -mul(NM, b);
+    if (x.x >= 0.0f && x.y >= 0.0f) {
+      solved = true;
+    } else {
+      x = mk(-q0.z * b.x, 0.0f);
+      vn2 = K.ex.y * x.x + b.y;
+      if (x.x >= 0.0f && vn2 >= 0.0f) {
+        solved = true;
+      } else {
+        x = mk(0.0f, -q1.z * b.y);
+        vn1 = K.ey.x * x.y + b.x;
+        if (x.y >= 0.0f && vn1 >= 0.0f) {
+          solved = true;
+        } else {
+          x = mk(0.0f, 0.0f);
+          vn1 = b.x;
+          vn2 = b.y;
+          if (vn1 >= 0.0f && vn2 >= 0.0f) solved = true;
+        }
+      }
+    }
+    if (solved) {
+      const vec2 dd = x - a;
+      const vec2 P1 = dd.x * normal;
+      const vec2 P2 = dd.y * normal;
+      vA = vA - mA * (P1 + P2);
+      wA -= iA * (cross(rA0, P1) + cross(rA1, P2));
+      vB = vB + mB * (P1 + P2);
+      wB += iB * (cross(rB0, P1) + cross(rB1, P2));
+      q0.x = x.x;
+      q1.x = x.y;
+    }
+  }
+  d.vc[4][s] = q0;
+  if (count > 1) d.vc[6][s] = q1;
+  if (writes_body(d, ia)) d.vel[ia] = make_float4(vA.x, vA.y, wA, 0.0f);
+  if (writes_body(d, ib)) d.vel[ib] = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+
+__global__ void __launch_bounds__(kBlock) k_contact_velocity(Dev d, int begin, int end) {
+  B2G_FOR(s, begin, end) contact_velocity(d, s);
+}
+
+// b2ContactSolver::StoreImpulses (:641-657): only the vc pointCount points
+__global__ void __launch_bounds__(kBlock) k_store_impulses(Dev d) {
+  const int n = d.cnt[C_CONS];
+  B2G_FOR(s, 0, n) {
+    const float4 v2i = d.vc[2][s];
+    const int count = as_int(v2i.z), c = as_int(v2i.w);
+    const float4 q0 = d.vc[4][s];
+    float4* mp = d.cman + 4 * (size_t)c;
+    float4 m0 = mp[0];
+    m0.z = q0.x;
+    m0.w = q0.y;
+    mp[0] = m0;
+    if (count > 1) {
+      const float4 q1 = d.vc[6][s];
+      float4 m1 = mp[1];
+      m1.z = q1.x;
+      m1.w = q1.y;
+      mp[1] = m1;
+    }
+  }
+}
+
+// ===========================================================================
+// Position integration with the translation / rotation clamp
+// (src/dynamics/b2_island.cpp:341-385)
+// ===========================================================================
+__global__ void __launch_bounds__(kBlock) k_integrate_positions(Dev d) {
+  B2G_FOR(b, 0, d.nb) {
+    const int w = d.bworld[b];
+    if (!world_active(d, w) || d.btype[b] == kStaticBody) continue;
+    const float h = d.wpar[w].z;
+    const float4 v4 = d.vel[b];
+    vec2 v = mk(v4.x, v4.y);
+    float wv = v4.z;
+    const vec2 tr = h * v;
+    if (dot(tr, tr) > kMaxTranslationSq) {
+      const float ratio = kMaxTranslation / length(tr);
+      v = ratio * v;
+    }
+    const float rotation = h * wv;
+    if (rotation * rotation > kMaxRotationSq) {
+      const float ratio = kMaxRotation / absf(rotation);
+      wv *= ratio;
+    }
+    const float4 p = d.pos[b];
+    const vec2 c = mk(p.x, p.y) + h * v;
+    const float a = p.z + h * wv;
+    d.pos[b] = make_float4(c.x, c.y, a, 0.0f);
+    d.vel[b] = make_float4(v.x, v.y, wv, 0.0f);
+  }
+}
+
+// b2ContactSolver::SolvePositionConstraints (:659-787) for slots [begin, end)
+__device__ __forceinline__ void contact_position(const Dev& d, int s, int it) {
+  const int isl = d.cisland[s];
+  if (d.solved[isl]) return;
+  const float4 v2i = d.vc[2][s];
+  const int ia = as_int(v2i.x), ib = as_int(v2i.y);
+  const float4 q0 = d.pc[0][s], q1 = d.pc[1][s], q2 = d.pc[2][s], q3 = d.pc[3][s], q4 = d.pc[4][s];
+  const vec2 lp = mk(q0.x, q0.y), ln = mk(q0.z, q0.w);
+  const vec2 lcA = mk(q2.x, q2.y), lcB = mk(q2.z, q2.w);
+  const float mA = q3.x, mB = q3.y, iA = q3.z, iB = q3.w;
+  const float radA = q4.x, radB = q4.y;
+  const int mtype = as_int(q4.z), count = as_int(q4.w);
+  const float4 pa = d.pos[ia], pb = d.pos[ib];
+  vec2 cA = mk(pa.x, pa.y), cB = mk(pb.x, pb.y);
+  float aA = pa.z, aB = pb.z;
+  float min_sep = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    if (j < count) {
+      xform xfA, xfB;
+      xfA.q = rot_of(aA);
+      xfB.q = rot_of(aB);
+      xfA.p = cA - mul(xfA.q, lcA);
+      xfB.p = cB - mul(xfB.q, lcB);
+      const vec2 lpj = j == 0 ? mk(q1.x, q1.y) : mk(q1.z, q1.w);
+      vec2 normal, point;
+      float separation;
+      if (mtype == kManCircles) {
+        const vec2 pA = mul(xfA, lp);
+        const vec2 pB = mul(xfB, mk(q1.x, q1.y));
+        normal = normalize(pB - pA);
+        point = 0.5f * (pA + pB);
+        separation = dot(pB - pA, normal) - radA - radB;
+      } else if (mtype == kManFaceA) {
+        normal = mul(xfA.q, ln);
+        const vec2 plane = mul(xfA, lp);
+        const vec2 clip = mul(xfB, lpj);
+        separation = dot(clip - plane, normal) - radA - radB;
+        point = clip;
+      } else {
+        normal = mul(xfB.q, ln);
+        const vec2 plane = mul(xfB, lp);
+        const vec2 clip = mul(xfA, lpj);
+        separation = dot(clip - plane, normal) - radA - radB;
+        point = clip;
+        normal = -normal;
+      }
+      const vec2 rA = point - cA, rB = point - cB;
+      min_sep = minf(min_sep, separation);
+      const float C = clampf(kBaumgarte * (separation + kLinearSlop), -kMaxLinearCorrection, 0.0f);
+      const float rnA = cross(rA, normal), rnB = cross(rB, normal);
+      const float K = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
+      const float impulse = K > 0.0f ? -C / K : 0.0f;
+      const vec2 P = impulse * normal;
+      cA = cA - mA * P;
+      aA -= iA * cross(rA, P);
+      cB = cB + mB * P;
+      aB += iB * cross(rB, P);
+    }
+  }
+  if (writes_body(d, ia)) d.pos[ia] = make_float4(cA.x, cA.y, aA, 0.0f);
+  if (writes_body(d, ib)) d.pos[ib] = make_float4(cB.x, cB.y, aB, 0.0f);
+  atomicMin(&d.minsep[(it & 1) * d.nb + isl], float_to_ordered(min_sep));
+}
+
+__global__ void __launch_bounds__(kBlock) k_contact_position(Dev d, int begin, int end, int it) {
+  B2G_FOR(s, begin, end) contact_position(d, s, it);
+}
+
+// ===========================================================================
+// b2RevoluteJoint (src/dynamics/b2_revolute_joint.cpp:73-321), colour order q
+// ===========================================================================
+__global__ void __launch_bounds__(kBlock) k_joint_prepare(Dev d) {
+  B2G_FOR(q, 0, d.nj) {
+    const Joint& j = d.joints[d.jorder[q]];
+    int isl = -1;
+    if (world_active(d, j.world) && (d.btype[j.a] != kStaticBody || d.btype[j.b] != kStaticBody)) {
+      isl = island_of(d, j.a, j.b);
+      island_reset(d, isl);
+    }
+    d.jisland[q] = isl;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_joint_init(Dev d, int begin, int end) {
+  B2G_FOR(q, begin, end) {
+    if (d.jisland[q] < 0) continue;
+    const int ji = d.jorder[q];
+    Joint j = d.joints[ji];
+    JointSolve s;
+    const float4 pa = d.prop[j.a], pb = d.prop[j.b];
+    s.lca_x = pa.z; s.lca_y = pa.w; s.lcb_x = pb.z; s.lcb_y = pb.w;
+    s.ma = pa.x; s.mb = pb.x; s.ia = pa.y; s.ib = pb.y;
+    const float aA = d.pos[j.a].z, aB = d.pos[j.b].z;
+    const float4 va = d.vel[j.a], vb = d.vel[j.b];
+    vec2 vA = mk(va.x, va.y), vB = mk(vb.x, vb.y);
+    float wA = va.z, wB = vb.z;
+    const rot2 qA = rot_of(aA), qB = rot_of(aB);
+    const vec2 rA = mul(qA, mk(j.la_x, j.la_y) - mk(s.lca_x, s.lca_y));
+    const vec2 rB = mul(qB, mk(j.lb_x, j.lb_y) - mk(s.lcb_x, s.lcb_y));
+    s.ra_x = rA.x; s.ra_y = rA.y; s.rb_x = rB.x; s.rb_y = rB.y;
+    const float mA = s.ma, mB = s.mb, iA = s.ia, iB = s.ib;
+    s.kxx = mA + mB + rA.y * rA.y * iA + rB.y * rB.y * iB;
+    s.kyx = -rA.y * rA.x * iA - rB.y * rB.x * iB;
+    s.kxy = s.kyx;
+    s.kyy = mA + mB + rA.x * rA.x * iA + rB.x * rB.x * iB;
+    s.axial = iA + iB;
+    bool fixed_rotation;
+    if (s.axial > 0.0f) {
+      s.axial = 1.0f / s.axial;
+      fixed_rotation = false;
+    } else {
+      fixed_rotation = true;
+    }
+    s.angle = aB - aA - j.ref;
+    if ((j.flags & 2) == 0 || fixed_rotation) {
+      j.lower_imp = 0.0f;
+      j.upper_imp = 0.0f;
+    }
+    if ((j.flags & 4) == 0 || fixed_rotation) j.motor = 0.0f;
+    const float4 wp = d.wpar[j.world];
+    if (d.wflag[j.world] & 1) {
+      const float r = wp.w;
+      j.imp_x *= r;
+      j.imp_y *= r;
+      j.motor *= r;
+      j.lower_imp *= r;
+      j.upper_imp *= r;
+      const float axial_impulse = j.motor + j.lower_imp - j.upper_imp;
+      const vec2 P = mk(j.imp_x, j.imp_y);
+      vA = vA - mA * P;
+      wA -= iA * (cross(rA, P) + axial_impulse);
+      vB = vB + mB * P;
+      wB += iB * (cross(rB, P) + axial_impulse);
+    } else {
+      j.imp_x = j.imp_y = 0.0f;
+      j.motor = j.lower_imp = j.upper_imp = 0.0f;
+    }
+    d.joints[ji] = j;
+    d.jsol[q] = s;
+    if (writes_body(d, j.a)) d.vel[j.a] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (writes_body(d, j.b)) d.vel[j.b] = make_float4(vB.x, vB.y, wB, 0.0f);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_joint_velocity(Dev d, int begin, int end) {
+  B2G_FOR(q, begin, end) {
+    if (d.jisland[q] < 0) continue;
+    const int ji = d.jorder[q];
+    Joint j = d.joints[ji];
+    const JointSolve s = d.jsol[q];
+    const float4 va = d.vel[j.a], vb = d.vel[j.b];
+    vec2 vA = mk(va.x, va.y), vB = mk(vb.x, vb.y);
+    float wA = va.z, wB = vb.z;
+    const float mA = s.ma, mB = s.mb, iA = s.ia, iB = s.ib;
+    const bool fixed_rotation = (iA + iB == 0.0f);
+    const float4 wp = d.wpar[j.world];
+    const float h = wp.z;
+    const float inv_dt = h > 0.0f ? 1.0f / h : 0.0f;
+    if ((j.flags & 4) && !fixed_rotation) {
+      const float Cdot = wB - wA - j.speed;
+      float impulse = -s.axial * Cdot;
+      const float old = j.motor;
+      const float max_impulse = h * j.max_torque;
+      j.motor = clampf(j.motor + impulse, -max_impulse, max_impulse);
+      impulse = j.motor - old;
+      wA -= iA * impulse;
+      wB += iB * impulse;
+    }
+    if ((j.flags & 2) && !fixed_rotation) {
+      {
+        const float C = s.angle - j.lower;
+        const float Cdot = wB - wA;
+        float impulse = -s.axial * (Cdot + maxf(C, 0.0f) * inv_dt);
+        const float old = j.lower_imp;
+        j.lower_imp = maxf(j.lower_imp + impulse, 0.0f);
+        impulse = j.lower_imp - old;
+        wA -= iA * impulse;
+        wB += iB * impulse;
+      }
+      {
+        const float C = j.upper - s.angle;
+        const float Cdot = wA - wB;
+        float impulse = -s.axial * (Cdot + maxf(C, 0.0f) * inv_dt);
+        const float old = j.upper_imp;
+        j.upper_imp = maxf(j.upper_imp + impulse, 0.0f);
+        impulse = j.upper_imp - old;
+        wA += iA * impulse;
+        wB -= iB * impulse;
+      }
+    }
+    {
+      const vec2 rA = mk(s.ra_x, s.ra_y), rB = mk(s.rb_x, s.rb_y);
+      const vec2 Cdot = vB + cross(wB, rB) - vA - cross(wA, rA);
+      const mat22 K = {mk(s.kxx, s.kxy), mk(s.kyx, s.kyy)};
+      const vec2 impulse = solve(K, -Cdot);
+      j.imp_x += impulse.x;
+      j.imp_y += impulse.y;
+      vA = vA - mA * impulse;
+      wA -= iA * cross(rA, impulse);
+      vB = vB + mB * impulse;
+      wB += iB * cross(rB, impulse);
+    }
+    d.joints[ji] = j;
+    if (writes_body(d, j.a)) d.vel[j.a] = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (writes_body(d, j.b)) d.vel[j.b] = make_float4(vB.x, vB.y, wB, 0.0f);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_joint_position(Dev d, int begin, int end, int it) {
+  B2G_FOR(q, begin, end) {
+    const int isl = d.jisland[q];
+    if (isl < 0 || d.solved[isl]) continue;
+    const Joint j = d.joints[d.jorder[q]];
+    const JointSolve s = d.jsol[q];
+    const float4 pa = d.pos[j.a], pb = d.pos[j.b];
+    vec2 cA = mk(pa.x, pa.y), cB = mk(pb.x, pb.y);
+    float aA = pa.z, aB = pb.z;
+    float angular_error = 0.0f, position_error = 0.0f;
+    const bool fixed_rotation = (s.ia + s.ib == 0.0f);
+    if ((j.flags & 2) && !fixed_rotation) {
+      const float angle = aB - aA - j.ref;
+      float C = 0.0f;
+      if (absf(j.upper - j.lower) < 2.0f * kAngularSlop)
+        C = clampf(angle - j.lower, -kMaxAngularCorrection, kMaxAngularCorrection);
+      else if (angle <= j.lower)
+        C = clampf(angle - j.lower + kAngularSlop, -kMaxAngularCorrection, 0.0f);
+      else if (angle >= j.upper)
+        C = clampf(angle - j.upper - kAngularSlop, 0.0f, kMaxAngularCorrection);
+      const float limit_impulse = -s.axial * C;
+      aA -= s.ia * limit_impulse;
+      aB += s.ib * limit_impulse;
+      angular_error = absf(C);
+    }
+    {
+      const rot2 qA = rot_of(aA), qB = rot_of(aB);
+      const vec2 rA = mul(qA, mk(j.la_x, j.la_y) - mk(s.lca_x, s.lca_y));
+      const vec2 rB = mul(qB, mk(j.lb_x, j.lb_y) - mk(s.lcb_x, s.lcb_y));
+      const vec2 C = cB + rB - cA - rA;
+      position_error = length(C);
+      const float mA = s.ma, mB = s.mb, iA = s.ia, iB = s.ib;
+      mat22 K;
+      K.ex.x = mA + mB + iA * rA.y * rA.y + iB * rB.y * rB.y;
+      K.ex.y = -iA * rA.x * rA.y - iB * rB.x * rB.y;
+      K.ey.x = K.ex.y;
+      K.ey.y = mA + mB + iA * rA.x * rA.x + iB * rB.x * rB.x;
+      const vec2 impulse = -solve(K, C);
+      cA = cA - mA * impulse;
+      aA -= iA * cross(rA, impulse);
+      cB = cB + mB * impulse;
+      aB += iB * cross(rB, impulse);
+    }
+    if (writes_body(d, j.a)) d.pos[j.a] = make_float4(cA.x, cA.y, aA, 0.0f);
+    if (writes_body(d, j.b)) d.pos[j.b] = make_float4(cB.x, cB.y, aB, 0.0f);
+    const bool ok = position_error <= kLinearSlop && angular_error <= kAngularSlop;
+    if (!ok) d.jok[(it & 1) * d.nb + isl] = 0;
+  }
+}
+
+// Per-island early exit after position iteration `it` (b2_island.cpp:403-408):
+// solved when min separation >= -3 slop and every joint is within slop. Also
+// re-arms the other parity buffer for iteration it + 1 (a different buffer,
+// so the two roles never race).
+__global__ void __launch_bounds__(kBlock) k_island_finalize(Dev d, int it) {
+  const int n = d.cnt[C_CONS];
+  B2G_FOR(k, 0, n + d.nj) {
+    const int isl = k < n ? d.cisland[k] : d.jisland[k - n];
+    if (isl < 0) continue;
+    const int nxt = ((it + 1) & 1) * d.nb + isl;
+    d.minsep[nxt] = float_to_ordered(0.0f);
+    d.jok[nxt] = 1;
+    if (d.solved[isl]) continue;
+    const int cur = (it & 1) * d.nb + isl;
+    const bool ok = ordered_to_float(d.minsep[cur]) >= -3.0f * kLinearSlop && d.jok[cur] != 0;
+    if (ok) d.solved[isl] = 1;
+  }
+}
+
+// write back: SynchronizeTransform (b2_body.h:957-961) and ClearForces
+// (b2_world.cpp:1173-1180)
+__global__ void __launch_bounds__(kBlock) k_finish_bodies(Dev d) {
+  B2G_FOR(b, 0, d.nb) {
+    const int w = d.bworld[b];
+    if (!world_active(d, w)) continue;
+    if (d.btype[b] != kStaticBody) {
+      const float4 p = d.pos[b], pr = d.prop[b];
+      const rot2 q = rot_of(p.z);
+      const vec2 c = mk(p.x, p.y) - mul(q, mk(pr.z, pr.w));
+      d.xf[b] = make_float4(c.x, c.y, q.s, q.c);
+    }
+    d.force[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+}
+
+// ===========================================================================
+// Broadphase: LBVH over Morton-sorted tight AABBs (the pair set that
+// b2BroadPhase::UpdateAndQuery reports, include/box2d/b2_broad_phase.h:252-511,
+// filtered as b2ContactManager::QueryCallback filters it,
+// src/dynamics/b2_contact_manager.cpp:136-188).
+// ===========================================================================
+__global__ void __launch_bounds__(kBlock) k_world_bounds_reset(Dev d) {
+  B2G_FOR(w, 0, d.nw) {
+    d.wbounds[4 * w + 0] = 0xffffffffu;
+    d.wbounds[4 * w + 1] = 0xffffffffu;
+    d.wbounds[4 * w + 2] = 0u;
+    d.wbounds[4 * w + 3] = 0u;
+  }
+}
+
+// ComputeAABB of the three shapes (circle :91-96, polygon :373-388, edge :156-167)
+__device__ __forceinline__ float4 shape_aabb(int type, int count, const float4* g, const xform& xf) {
+  const float4 h = g[0];
+  const float r = h.x;
+  if (type == kCircle) {
+    const vec2 p = xf.p + mul(xf.q, mk(h.y, h.z));
+    return make_float4(p.x - r, p.y - r, p.x + r, p.y + r);
+  }
+  if (type == kEdge) {
+    const float4 e = g[1], f = g[2];
+    const vec2 v1 = mul(xf, mk(e.z, e.w));
+    const vec2 v2 = mul(xf, mk(f.x, f.y));
+    const vec2 lo = vmin(v1, v2), hi = vmax(v1, v2);
+    return make_float4(lo.x - r, lo.y - r, hi.x + r, hi.y + r);
+  }
+  const float4 a0 = g[1];
+  vec2 lo = mul(xf, mk(a0.x, a0.y));
+  vec2 hi = lo;
+#pragma unroll
+  for (int i = 1; i < kMaxVerts; ++i) {
+    if (i < count) {
+      const float4 a = g[1 + (i >> 1)];
+      const vec2 v = mul(xf, (i & 1) ? mk(a.z, a.w) : mk(a.x, a.y));
+      lo = vmin(lo, v);
+      hi = vmax(hi, v);
+    }
+  }
+  return make_float4(lo.x - r, lo.y - r, hi.x + r, hi.y + r);
+}
+
+__global__ void __launch_bounds__(kBlock) k_aabb(Dev d) {
+  B2G_FOR(i, 0, d.ns) {
+    const int4 h = d.shdr[i];
+    const float4 bb = shape_aabb(h.x, h.z, d.geo + 9 * (size_t)i, xform_of(d.xf[h.y]));
+    d.aabb[i] = bb;
+    // world bounds of centres (2 * centre, as b2BroadPhase::GetCenter2)
+    const float cx = bb.x + bb.z, cy = bb.y + bb.w;
+    unsigned* wb = d.wbounds + 4 * h.w;
+    atomicMin(&wb[0], float_to_ordered(cx));
+    atomicMin(&wb[1], float_to_ordered(cy));
+    atomicMax(&wb[2], float_to_ordered(cx));
+    atomicMax(&wb[3], float_to_ordered(cy));
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_world_bounds_from_aabbs(Dev d) {
+  B2G_FOR(i, 0, d.ns) {
+    const float4 bb = d.aabb[i];
+    const float cx = bb.x + bb.z, cy = bb.y + bb.w;
+    unsigned* wb = d.wbounds + 4 * d.shdr[i].w;
+    atomicMin(&wb[0], float_to_ordered(cx));
+    atomicMin(&wb[1], float_to_ordered(cy));
+    atomicMax(&wb[2], float_to_ordered(cx));
+    atomicMax(&wb[3], float_to_ordered(cy));
+  }
+}
+
+__device__ __forceinline__ uint32_t spread15(uint32_t x) {
+  x &= 0x7fffu;
+  x = (x | (x << 8)) & 0x00ff00ffu;
+  x = (x | (x << 4)) & 0x0f0f0f0fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  x = (x | (x << 1)) & 0x55555555u;
+  return x;
+}
+
+__global__ void __launch_bounds__(kBlock) k_morton(Dev d) {
+  B2G_FOR(i, 0, d.ns) {
+    const int4 h = d.shdr[i];
+    const float4 bb = d.aabb[i];
+    const unsigned* wb = d.wbounds + 4 * h.w;
+    const float x0 = ordered_to_float(wb[0]), y0 = ordered_to_float(wb[1]);
+    const float x1 = ordered_to_float(wb[2]), y1 = ordered_to_float(wb[3]);
+    const float sx = x1 > x0 ? 32767.0f / (x1 - x0) : 0.0f;
+    const float sy = y1 > y0 ? 32767.0f / (y1 - y0) : 0.0f;
+    const float cx = bb.x + bb.z, cy = bb.y + bb.w;
+    const uint32_t qx = (uint32_t)minf(maxf((cx - x0) * sx, 0.0f), 32767.0f);
+    const uint32_t qy = (uint32_t)minf(maxf((cy - y0) * sy, 0.0f), 32767.0f);
+    const uint32_t code = spread15(qx) | (spread15(qy) << 1);
+    d.mkey[0][i] = ((uint64_t)(uint32_t)h.w << 32) | code;
+    d.mval[0][i] = (uint32_t)i;
+  }
+}
+
+// delta of Karras (2012): common prefix of (key, index) at sorted positions i, j
+__device__ __forceinline__ int lbvh_delta(const uint64_t* __restrict__ k, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  const uint64_t x = k[i] ^ k[j];
+  if (x == 0) return 64 + __clz((unsigned)(i ^ j));
+  return __clzll((long long)x);
+}
+
+__global__ void __launch_bounds__(kBlock) k_lbvh_build(Dev d, const uint64_t* __restrict__ keys) {
+  const int n = d.ns;
+  B2G_FOR(i, 0, n - 1) {
+    const int dir = (lbvh_delta(keys, n, i, i + 1) - lbvh_delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = lbvh_delta(keys, n, i, i - dir);
+    int lmax = 2;
+    while (lbvh_delta(keys, n, i, i + lmax * dir) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+      if (lbvh_delta(keys, n, i, i + (l + t) * dir) > dmin) l += t;
+    const int j = i + l * dir;
+    const int dnode = lbvh_delta(keys, n, i, j);
+    int s = 0;
+    int t = l;
+    do {
+      t = (t + 1) >> 1;
+      if (lbvh_delta(keys, n, i, i + (s + t) * dir) > dnode) s += t;
+    } while (t > 1);
+    const int gamma = i + s * dir + (dir < 0 ? -1 : 0);
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    const int left = (lo == gamma) ? ~gamma : gamma;
+    const int right = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+    d.node_child[2 * i] = left;
+    d.node_child[2 * i + 1] = right;
+    d.node_range[i] = make_int2(lo, hi);
+    if (left < 0) d.leaf_parent[~left] = i; else d.node_parent[left] = i;
+    if (right < 0) d.leaf_parent[~right] = i; else d.node_parent[right] = i;
+    d.node_visit[i] = 0;
+    if (i == 0) d.node_parent[0] = -1;
+  }
+}
+
+// leaf ranges of each world in sorted order
+__global__ void __launch_bounds__(kBlock) k_world_leaves(Dev d, const uint64_t* __restrict__ keys) {
+  const int n = d.ns;
+  B2G_FOR(k, 0, n) {
+    const int w = (int)(keys[k] >> 32);
+    if (k == 0 || (int)(keys[k - 1] >> 32) != w) d.wleaf[2 * w] = k;
+    if (k == n - 1 || (int)(keys[k + 1] >> 32) != w) d.wleaf[2 * w + 1] = k + 1;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_lbvh_refit(Dev d, const uint32_t* __restrict__ vals) {
+  const int n = d.ns;
+  B2G_FOR(k, 0, n) {
+    const float4 bb = d.aabb[vals[k]];
+    d.leaf_aabb[k] = bb;
+    if (n < 2) continue;
+    __threadfence();
+    int node = d.leaf_parent[k];
+    while (node >= 0) {
+      if (atomicAdd(&d.node_visit[node], 1) == 0) break;  // first child to arrive stops
+      __threadfence();
+      const int l = d.node_child[2 * node], r = d.node_child[2 * node + 1];
+      const float4 a = l < 0 ? d.leaf_aabb[~l] : d.node_aabb[l];
+      const float4 b = r < 0 ? d.leaf_aabb[~r] : d.node_aabb[r];
+      d.node_aabb[node] = make_float4(minf(a.x, b.x), minf(a.y, b.y), maxf(a.z, b.z), maxf(a.w, b.w));
+      __threadfence();
+      node = d.node_parent[node];
+    }
+  }
+}
+
+__device__ __forceinline__ bool overlap(float4 a, float4 b) {  // b2TestOverlap, inclusive
+  return (a.z >= b.x) & (a.x <= b.z) & (a.w >= b.y) & (a.y <= b.w);
+}
+
+// b2Body::ShouldCollide + b2ContactFilter::ShouldCollide + factory existence
+__device__ __forceinline__ bool pair_allowed(const Dev& d, int i, int j) {
+  const int4 hi = d.shdr[i], hj = d.shdr[j];
+  const int ba = hi.y, bb = hj.y;
+  if (ba == bb) return false;
+  if (d.btype[ba] != kDynamicBody && d.btype[bb] != kDynamicBody) return false;
+  if (!has_collide_function(hi.x, hj.x) && !has_collide_function(hj.x, hi.x)) return false;
+  const int4 fa = d.sfilt[i], fb = d.sfilt[j];
+  if (fa.z == fb.z && fa.z != 0) {
+    if (fa.z <= 0) return false;
+  } else if ((fa.y & fb.x) == 0 || (fa.x & fb.y) == 0) {
+    return false;
+  }
+  for (int e = d.bj_off[ba]; e < d.bj_off[ba + 1]; ++e)
+    if (d.bj_other[e] == bb) return false;
+  return true;
+}
+
+constexpr int kStack = 64;
+
+// write == false: count pairs (i, j > i) per shape; write == true: write them
+// at poff[i], sorted by j.
+template <bool kWrite>
+__global__ void __launch_bounds__(kBlock) k_lbvh_pairs(Dev d) {
+  const int n = d.ns;
+  B2G_FOR(i, 0, n) {
+    const float4 q = d.aabb[i];
+    const int w = d.shdr[i].w;
+    const int wlo = d.wleaf[2 * w], whi = d.wleaf[2 * w + 1] - 1;
+    int found = 0;
+    int base = 0, cap = 0;
+    if (kWrite) {
+      base = d.poff[i];
+      cap = d.ccap;
+    }
+    if (n >= 2) {
+      int stack[kStack];
+      int sp = 0;
+      stack[sp++] = 0;
+      while (sp > 0) {
+        const int node = stack[--sp];
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+          const int ch = d.node_child[2 * node + side];
+          if (ch < 0) {
+            const int k = ~ch;
+            if (k < wlo || k > whi) continue;
+            if (!overlap(q, d.leaf_aabb[k])) continue;
+            const int j = (int)d.mval[0][k];
+            if (j <= i || !pair_allowed(d, i, j)) continue;
+            if (kWrite) {
+              const int slot = base + found;
+              if (slot < cap) d.nkey[slot] = ((uint64_t)(uint32_t)i << 32) | (uint32_t)j;
+            }
+            ++found;
+          } else {
+            const int2 r = d.node_range[ch];
+            if (r.y < wlo || r.x > whi) continue;
+            if (!overlap(q, d.node_aabb[ch])) continue;
+            if (sp < kStack) stack[sp++] = ch;
+            else atomicOr(&d.cnt[C_STATUS], S_STACK_OVERFLOW);
+          }
+        }
+      }
+    }
+    if (!kWrite) {
+      d.pcount[i] = found;
+    } else {
+      // insertion sort of this shape's pairs by key
+      const int end = min(base + found, cap);
+      for (int a = base + 1; a < end; ++a) {
+        const uint64_t key = d.nkey[a];
+        int b = a - 1;
+        while (b >= base && d.nkey[b] > key) {
+          d.nkey[b + 1] = d.nkey[b];
+          --b;
+        }
+        d.nkey[b + 1] = key;
+      }
+    }
+  }
+}
+
+// New contact list: persisting pairs keep their contact (orientation,
+// manifold, impulses, material, colour); new pairs start empty with the
+// factory's orientation and the mixing laws (src/dynamics/b2_contact.cpp:58-124,
+// include/box2d/b2_contact.h:42-60). Same-type pairs take A = lower index.
+__global__ void __launch_bounds__(kBlock) k_merge_contacts(Dev d) {
+  const int old_count = d.cnt[C_CONTACTS];
+  const int total = d.cnt[C_PAIRS];
+  const int n = total < d.ccap ? total : d.ccap;
+  B2G_FOR(p, 0, n) {
+    const uint64_t key = d.nkey[p];
+    int lo = 0, hi = old_count;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (d.ckey[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    if (lo < old_count && d.ckey[lo] == key) {
+      d.npair[p] = d.cpair[lo];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) d.nman[4 * (size_t)p + q] = d.cman[4 * (size_t)lo + q];
+      d.nmat[p] = d.cmat[lo];
+      d.ncol[p] = d.ccol[lo];
+    } else {
+      const int i = (int)(key >> 32), j = (int)(key & 0xffffffffu);
+      const int ti = d.shdr[i].x, tj = d.shdr[j].x;
+      const int2 pr = has_collide_function(ti, tj) ? make_int2(i, j) : make_int2(j, i);
+      d.npair[p] = pr;
+      const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      d.nman[4 * (size_t)p + 0] = z;
+      d.nman[4 * (size_t)p + 1] = z;
+      d.nman[4 * (size_t)p + 2] = z;
+      d.nman[4 * (size_t)p + 3] = z;
+      const float4 ma = d.smat[pr.x], mb = d.smat[pr.y];
+      d.nmat[p] = make_float4(sqrtf(ma.x * mb.x), ma.y > mb.y ? ma.y : mb.y, ma.z < mb.z ? ma.z : mb.z, 0.0f);
+      d.ncol[p] = -1;
+    }
+  }
+}
+
+__global__ void k_finish_pairs(Dev d) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const int total = d.cnt[C_PAIRS];
+    if (total > d.ccap) atomicOr(&d.cnt[C_STATUS], S_CONTACT_OVERFLOW);
+    d.cnt[C_CONTACTS] = total < d.ccap ? total : d.ccap;
+  }
+}
+
+// ---- standalone narrowphase over caller pairs (b2g_collide_pairs) -----------
+__global__ void __launch_bounds__(kBlock) k_collide_pairs(int n, const int4* __restrict__ ha, const float4* __restrict__ ga,
+                                                          const float4* __restrict__ xa, const int4* __restrict__ hb,
+                                                          const float4* __restrict__ gb, const float4* __restrict__ xb,
+                                                          float4* __restrict__ out /* 4 per pair */, int* __restrict__ ok) {
+  B2G_FOR(i, 0, n) {
+    Geo a, b;
+    load_geo(a, ha[i].x, ha[i].z, ga + 9 * (size_t)i);
+    load_geo(b, hb[i].x, hb[i].z, gb + 9 * (size_t)i);
+    Man m;
+    m.count = 0;
+    m.type = 0;
+    m.ln = m.lp = m.lp0 = m.lp1 = mk(0.0f, 0.0f);
+    m.id0 = m.id1 = 0;
+    const bool has = evaluate(m, a, xform_of(xa[i]), b, xform_of(xb[i]));
+    ok[i] = has ? 1 : 0;
+    out[4 * (size_t)i + 0] = make_float4(m.lp0.x, m.lp0.y, 0.0f, 0.0f);
+    out[4 * (size_t)i + 1] = make_float4(m.count > 1 ? m.lp1.x : 0.0f, m.count > 1 ? m.lp1.y : 0.0f, 0.0f, 0.0f);
+    out[4 * (size_t)i + 2] = make_float4(m.ln.x, m.ln.y, m.lp.x, m.lp.y);
+    out[4 * (size_t)i + 3] = make_float4(uint_as_float(m.count > 0 ? m.id0 : 0u), uint_as_float(m.count > 1 ? m.id1 : 0u),
+                                         as_float((m.type & 0xff) | (m.count << 8)), 0.0f);
+  }
+}
+
+}  // namespace b2g

@@ -1,0 +1,313 @@
+"""ctypes binding of lib/libbox2d_amd.so (the b2g C ABI in include/b2g.h).
+
+The library is the product: HIP kernels for gfx950 behind a C ABI. This
+module only marshals arrays; it raises if the library is missing or no HIP
+device is visible -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libbox2d_amd.so")
+
+# status codes (include/b2g.h)
+OK, ERR_HIP, ERR_OOM, ERR_CAPACITY, ERR_INVALID, ERR_UNSUPPORTED, ERR_NO_DEVICE = range(7)
+STEP_FIND_NEW_CONTACTS, STEP_PROFILE, STEP_ASYNC, STEP_SKIP_BROADPHASE = 1, 2, 4, 8
+
+BODY_DTYPE = np.dtype([
+    ("type", "<i4"), ("world", "<i4"), ("xf", "<f4", 4), ("c", "<f4", 2), ("a", "<f4"),
+    ("local_center", "<f4", 2), ("v", "<f4", 2), ("w", "<f4"), ("mass", "<f4"),
+    ("inv_mass", "<f4"), ("inv_i", "<f4"), ("linear_damping", "<f4"),
+    ("angular_damping", "<f4"), ("gravity_scale", "<f4"), ("force", "<f4", 2), ("torque", "<f4"),
+])
+SHAPE_DTYPE = np.dtype([
+    ("type", "<i4"), ("body", "<i4"), ("count", "<i4"), ("radius", "<f4"),
+    ("centroid", "<f4", 2), ("vertices", "<f4", 16), ("normals", "<f4", 16),
+    ("friction", "<f4"), ("restitution", "<f4"), ("restitution_threshold", "<f4"),
+    ("category", "<i4"), ("mask", "<i4"), ("group", "<i4"), ("is_sensor", "<i4"),
+])
+JOINT_DTYPE = np.dtype([
+    ("body_a", "<i4"), ("body_b", "<i4"), ("local_anchor_a", "<f4", 2),
+    ("local_anchor_b", "<f4", 2), ("reference_angle", "<f4"), ("lower_angle", "<f4"),
+    ("upper_angle", "<f4"), ("motor_speed", "<f4"), ("max_motor_torque", "<f4"),
+    ("flags", "<i4"), ("impulse", "<f4", 2), ("motor_impulse", "<f4"),
+    ("lower_impulse", "<f4"), ("upper_impulse", "<f4"),
+])
+CONTACT_DTYPE = np.dtype([
+    ("shape_a", "<i4"), ("shape_b", "<i4"), ("touching", "<i4"), ("type", "<i4"),
+    ("point_count", "<i4"), ("local_normal", "<f4", 2), ("local_point", "<f4", 2),
+    ("point_local", "<f4", 4), ("normal_impulse", "<f4", 2), ("tangent_impulse", "<f4", 2),
+    ("id", "<u4", 2), ("friction", "<f4"), ("restitution", "<f4"),
+    ("restitution_threshold", "<f4"), ("tangent_speed", "<f4"),
+])
+
+
+class Capacity(ctypes.Structure):
+    _fields_ = [("worlds", ctypes.c_int32), ("bodies", ctypes.c_int32), ("shapes", ctypes.c_int32),
+                ("joints", ctypes.c_int32), ("contacts", ctypes.c_int32)]
+
+
+class WorldParams(ctypes.Structure):
+    _fields_ = [("gravity", ctypes.c_float * 2), ("dt", ctypes.c_float), ("dt_ratio", ctypes.c_float),
+                ("warm_starting", ctypes.c_int32), ("active", ctypes.c_int32)]
+
+
+class Counts(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "worlds", "bodies", "shapes", "joints", "contacts", "touching", "colours", "islands",
+        "pos_iters_run", "status")]
+
+
+class Profile(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in ("step", "collide", "solve", "broadphase")]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int32
+_F = ctypes.c_float
+_lib = None
+
+# every entry point declared in include/b2g.h
+EXPORTS = [
+    "b2g_device_count", "b2g_last_error", "b2g_batch_create", "b2g_batch_destroy",
+    "b2g_set_worlds", "b2g_set_world_params", "b2g_upload_bodies", "b2g_upload_shapes",
+    "b2g_upload_joints", "b2g_upload_contacts", "b2g_batch_step", "b2g_batch_wait",
+    "b2g_get_counts", "b2g_get_profile", "b2g_download_bodies", "b2g_download_contacts",
+    "b2g_download_joints", "b2g_download_contact_colours", "b2g_collide_pairs", "b2g_find_pairs",
+    "b2a_world_from_scene", "b2a_world_destroy", "b2a_world_step", "b2a_world_get_bodies",
+    "b2a_world_contact_count", "b2a_world_get_contacts", "b2a_world_batch", "b2a_world_prepare",
+]
+
+
+class B2GError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the HIP library; raises when it is missing (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise B2GError(f"{LIB_PATH} is missing: build it (make -C box2d-optimized_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.b2g_last_error.restype = ctypes.c_char_p
+        L.b2g_device_count.argtypes = [_P]
+        L.b2g_batch_create.argtypes = [_P, _P]
+        L.b2g_batch_destroy.argtypes = [_P]
+        L.b2g_set_worlds.argtypes = [_P, _I, _P, _P, _P]
+        L.b2g_set_world_params.argtypes = [_P, _I, _I, _P]
+        for f in ("b2g_upload_bodies", "b2g_upload_shapes", "b2g_upload_joints",
+                  "b2g_download_bodies", "b2g_download_contacts", "b2g_download_joints",
+                  "b2g_download_contact_colours"):
+            getattr(L, f).argtypes = [_P, _I, _I, _P]
+        L.b2g_upload_contacts.argtypes = [_P, _I, _P]
+        L.b2g_batch_step.argtypes = [_P, _I, _I, ctypes.c_uint32]
+        L.b2g_batch_wait.argtypes = [_P]
+        L.b2g_get_counts.argtypes = [_P, _P]
+        L.b2g_get_profile.argtypes = [_P, _P]
+        L.b2g_collide_pairs.argtypes = [_I, _P, _P, _P, _P, _P]
+        L.b2g_find_pairs.argtypes = [_I, _P, _P, _P, _P, _I, _P, ctypes.c_int64, _P, _P]
+        L.b2a_world_from_scene.argtypes = [_P, _P, _I]
+        L.b2a_world_destroy.argtypes = [_P]
+        L.b2a_world_step.argtypes = [_P, _F, _I, _I]
+        L.b2a_world_get_bodies.argtypes = [_P, _P]
+        L.b2a_world_contact_count.argtypes = [_P, _P]
+        L.b2a_world_get_contacts.argtypes = [_P, _I, _P, _P]
+        L.b2a_world_batch.restype = _P
+        L.b2a_world_batch.argtypes = [_P]
+        L.b2a_world_prepare.argtypes = [_P]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != OK:
+        raise B2GError(f"b2g error {rc}: {lib().b2g_last_error().decode()}")
+    return rc
+
+
+def device_count() -> int:
+    n = ctypes.c_int32(0)
+    lib().b2g_device_count(ctypes.byref(n))
+    return n.value
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Batch:
+    """A set of independent worlds in device memory (b2g_batch)."""
+
+    def __init__(self, worlds=1, bodies=64, shapes=64, joints=16, contacts=4096, handle=None):
+        self.L = lib()
+        self.owned = handle is None
+        if handle is None:
+            cap = Capacity(worlds, bodies, shapes, joints, contacts)
+            h = ctypes.c_void_p()
+            check(self.L.b2g_batch_create(ctypes.byref(h), ctypes.byref(cap)))
+            self.h = h.value
+        else:
+            self.h = handle
+
+    def close(self):
+        if self.h and self.owned:
+            self.L.b2g_batch_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_worlds(self, params):
+        arr = (WorldParams * len(params))(*params)
+        check(self.L.b2g_set_worlds(self.h, len(params), None, None, arr))
+
+    def set_world_params(self, first, params):
+        arr = (WorldParams * len(params))(*params)
+        check(self.L.b2g_set_world_params(self.h, first, len(params), arr))
+
+    def upload_bodies(self, first, rec):
+        rec = np.ascontiguousarray(rec, dtype=BODY_DTYPE)
+        check(self.L.b2g_upload_bodies(self.h, first, len(rec), _ptr(rec)))
+
+    def upload_shapes(self, first, rec):
+        rec = np.ascontiguousarray(rec, dtype=SHAPE_DTYPE)
+        check(self.L.b2g_upload_shapes(self.h, first, len(rec), _ptr(rec)))
+
+    def upload_joints(self, first, rec):
+        rec = np.ascontiguousarray(rec, dtype=JOINT_DTYPE)
+        if len(rec):
+            check(self.L.b2g_upload_joints(self.h, first, len(rec), _ptr(rec)))
+
+    def upload_contacts(self, rec):
+        rec = np.ascontiguousarray(rec, dtype=CONTACT_DTYPE)
+        check(self.L.b2g_upload_contacts(self.h, len(rec), _ptr(rec) if len(rec) else None))
+
+    def step(self, vi=8, pi=3, flags=0):
+        check(self.L.b2g_batch_step(self.h, vi, pi, flags))
+
+    def wait(self):
+        check(self.L.b2g_batch_wait(self.h))
+
+    def counts(self) -> Counts:
+        c = Counts()
+        check(self.L.b2g_get_counts(self.h, ctypes.byref(c)))
+        return c
+
+    def profile(self) -> Profile:
+        p = Profile()
+        check(self.L.b2g_get_profile(self.h, ctypes.byref(p)))
+        return p
+
+    def bodies(self, first=0, count=None):
+        if count is None:
+            count = self.counts().bodies - first
+        out = np.zeros(count, dtype=BODY_DTYPE)
+        if count:
+            check(self.L.b2g_download_bodies(self.h, first, count, _ptr(out)))
+        return out
+
+    def contacts(self):
+        n = self.L.b2g_download_contacts(self.h, 0, 0, None)
+        if n < 0:
+            check(n)
+        out = np.zeros(n, dtype=CONTACT_DTYPE)
+        if n:
+            check(self.L.b2g_download_contacts(self.h, 0, n, _ptr(out)))
+        return out
+
+    def contact_colours(self, n):
+        out = np.zeros(n, dtype=np.int32)
+        if n:
+            check(self.L.b2g_download_contact_colours(self.h, 0, n, _ptr(out)))
+        return out
+
+    def joints(self, count):
+        out = np.zeros(count, dtype=JOINT_DTYPE)
+        if count:
+            check(self.L.b2g_download_joints(self.h, 0, count, _ptr(out)))
+        return out
+
+
+class World:
+    """A b2World (drop-in C++ API) built from a scene, stepped on the GPU."""
+
+    def __init__(self, scene, warm_starting=True):
+        self.L = lib()
+        self.scene = scene
+        self.cscene = scene.to_c()
+        h = ctypes.c_void_p()
+        check(self.L.b2a_world_from_scene(ctypes.byref(h), self.cscene.ptr(), int(warm_starting)))
+        self.h = h.value
+        self.nb = scene.body_count
+
+    def close(self):
+        if self.h:
+            self.L.b2a_world_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n=1, dt=1.0 / 60.0, vi=8, pi=3):
+        for _ in range(n):
+            check(self.L.b2a_world_step(self.h, dt, vi, pi))
+
+    def bodies(self):
+        """10 floats per body: centre c (2), angle, v (2), w, transform (p 2, q.s, q.c)."""
+        out = np.zeros((self.nb, 10), dtype=np.float32)
+        check(self.L.b2a_world_get_bodies(self.h, _ptr(out)))
+        return out
+
+    def contacts(self):
+        n = ctypes.c_int32(0)
+        check(self.L.b2a_world_get_contacts(self.h, 0, None, ctypes.byref(n)))
+        out = np.zeros(n.value, dtype=CONTACT_DTYPE)
+        if n.value:
+            check(self.L.b2a_world_get_contacts(self.h, n.value, _ptr(out), ctypes.byref(n)))
+        return out
+
+    def prepare(self):
+        """Upload everything to the device without stepping."""
+        check(self.L.b2a_world_prepare(self.h))
+
+    def batch(self) -> Batch:
+        return Batch(handle=self.L.b2a_world_batch(self.h))
+
+
+def collide_pairs(shape_a, xf_a, shape_b, xf_b):
+    sa = np.ascontiguousarray(shape_a, dtype=SHAPE_DTYPE)
+    sb = np.ascontiguousarray(shape_b, dtype=SHAPE_DTYPE)
+    xa = np.ascontiguousarray(xf_a, dtype=np.float32).reshape(-1, 4)
+    xb = np.ascontiguousarray(xf_b, dtype=np.float32).reshape(-1, 4)
+    n = len(sa)
+    out = np.zeros(n, dtype=CONTACT_DTYPE)
+    if n:
+        check(lib().b2g_collide_pairs(n, _ptr(sa), _ptr(xa), _ptr(sb), _ptr(xb), _ptr(out)))
+    return out
+
+
+def find_pairs(aabbs, shape_body, shape_type, filt, body_type, max_pairs=None):
+    aabbs = np.ascontiguousarray(aabbs, dtype=np.float32)
+    sb = np.ascontiguousarray(shape_body, dtype=np.int32)
+    st = np.ascontiguousarray(shape_type, dtype=np.int32)
+    bt = np.ascontiguousarray(body_type, dtype=np.int32)
+    f = None if filt is None else np.ascontiguousarray(filt, dtype=np.int32)
+    n = len(sb)
+    if max_pairs is None:
+        max_pairs = max(64, 32 * n)
+    keys = np.zeros(max_pairs, dtype=np.uint64)
+    cnt = ctypes.c_int64(0)
+    check(lib().b2g_find_pairs(n, _ptr(aabbs), _ptr(sb), _ptr(st), None if f is None else _ptr(f),
+                               len(bt), _ptr(bt), max_pairs, _ptr(keys), ctypes.byref(cnt)))
+    return keys[:min(cnt.value, max_pairs)], cnt.value

@@ -271,6 +271,8 @@ int b2a_world_get_bodies(b2a_world* world, float* out);
 int b2a_world_contact_count(b2a_world* world, int32_t* out);
 int b2a_world_get_contacts(b2a_world* world, int32_t capacity, b2g_contact* out, int32_t* count);
 b2g_batch* b2a_world_batch(b2a_world* world);
+/* allocate the device batch and upload bodies/shapes/joints without stepping */
+int b2a_world_prepare(b2a_world* world);
 
 #ifdef __cplusplus
 }

@@ -401,3 +401,11 @@ def ref_state_to_records(rw: RefWorld, restated_bodies: np.ndarray) -> np.ndarra
     out["w"] = rb[:, 5]
     out["xf"] = rb[:, 6:10]
     return out
+
+
+def ref_aabbs(rw: RefWorld) -> np.ndarray:
+    lib = Ref.lib()
+    lib.ref_world_get_aabbs.argtypes = [_P, _P]
+    out = np.zeros((rw.scene.fixture_count, 4), dtype=np.float32)
+    lib.ref_world_get_aabbs(rw.h, _ptr(out))
+    return out

@@ -466,3 +466,17 @@ extern "C" int ref_world_step_ordered(void* w, float dt, int vi, int pi, int cap
   if (out_nj) *out_nj = nj;
   return nc <= cap ? 0 : 1;
 }
+
+// 4 floats per fixture: the AABB the reference broadphase last used
+// (b2Fixture::GetAABB, refreshed by b2BroadPhase::UpdateAndQuery for non-static
+// fixtures; static ones keep the AABB computed at creation).
+extern "C" void ref_world_get_aabbs(void* w, float* out) {
+  RefWorld* rw = (RefWorld*)w;
+  for (size_t i = 0; i < rw->fixtures.size(); ++i) {
+    const b2AABB& a = rw->fixtures[i]->GetAABB();
+    out[4 * i + 0] = a.lowerBound.x;
+    out[4 * i + 1] = a.lowerBound.y;
+    out[4 * i + 2] = a.upperBound.x;
+    out[4 * i + 3] = a.upperBound.y;
+  }
+}

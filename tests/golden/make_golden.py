@@ -1,0 +1,213 @@
+"""Writes the golden fixtures under tests/golden/ from the compiled reference.
+
+Run in a container where the reference sources exist (oracle/Makefile builds
+oracle/_ref/libbox2d_ref.so from them):
+
+    python tests/golden/make_golden.py
+
+Outputs (inputs and expected outputs only):
+  narrowphase.npz  shape pairs + transforms -> reference manifolds (b2Collide*)
+                   and world manifolds (b2WorldManifold::Initialize)
+  pairs.npz        fixture AABBs + filter data of reference worlds -> the
+                   reference's contact (pair) set after its broadphase
+  onestep_<scene>.npz
+                   reference state after N steps (bodies, contacts with
+                   impulses, joint impulses) -> reference state after N+1,
+                   its contact list, and the order its island solver used
+  trajectory_pyramid.npz
+                   reference pyramid (20 rows) body states every 50 steps
+                   over 1000 steps
+"""
+from __future__ import annotations
+
+import importlib.util
+import math
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+
+
+def load_pkg():
+    name = "box2d_optimized_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    d = os.path.join(REPO, "box2d-optimized_amd")
+    spec = importlib.util.spec_from_file_location(name, os.path.join(d, "__init__.py"),
+                                                  submodule_search_locations=[d])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+import oracle  # noqa: E402
+
+S = load_pkg().scenes
+
+
+def chain_scene():
+    s = S.Scene(name="chain")
+    g = s.add_body(S.STATIC, 0.0, 0.0)
+    s.add_edge(g, (-40.0, 0.0), (40.0, 0.0))
+    prev = g
+    for k in range(12):
+        b = s.add_body(S.DYNAMIC, 0.5 + k, 10.0)
+        s.add_box(b, 0.5, 0.1)
+        s.add_revolute(prev, b, (float(k), 10.0))
+        prev = b
+    for k in range(20):
+        b = s.add_body(S.DYNAMIC, -3.0 + 0.3 * k, 2.0 + 1.1 * k)
+        s.add_circle(b, 0.4)
+    return s
+
+
+def filtered_scene(seed=11, n=400):
+    """Random shapes with category/mask/group filter data."""
+    s = S.Scene(name="filtered")
+    g = s.add_body(S.STATIC, 0.0, 0.0)
+    s.add_edge(g, (-15.0, 0.0), (15.0, 0.0))
+    rs = S.Stream(seed)
+    u = rs.uniform(n * 8).reshape(n, 8)
+    for i in range(n):
+        b = s.add_body(S.DYNAMIC, float(np.float32(-14 + 28 * u[i, 0])), float(np.float32(0.5 + 12 * u[i, 1])),
+                       float(np.float32(6.28 * u[i, 2])))
+        cat = 1 << int(u[i, 3] * 4)
+        mask = 0xFFFF ^ (1 << int(u[i, 4] * 4)) if u[i, 5] < 0.5 else 0xFFFF
+        group = int(u[i, 6] * 5) - 2
+        filt = (cat, mask, group, 0)
+        if i % 2:
+            s.add_circle(b, float(np.float32(0.3 + 0.4 * u[i, 7])), filt=filt)
+        else:
+            s.add_box(b, float(np.float32(0.3 + 0.4 * u[i, 7])), 0.4, filt=filt)
+    return s
+
+
+SCENES = {
+    "pyramid": (lambda: S.pyramid(20), 150),
+    "mixed": (lambda: S.small_mixed(), 200),
+    "chain": (chain_scene, 60),
+    "pile": (lambda: S.random_pile(1500, seed=9, half_w=25, height=60), 120),
+}
+
+
+def narrowphase(rng_seed=1234, per_kind=400):
+    src = S.small_mixed(seed=3, n=90)
+    pile = S.random_pile(300, seed=77, half_w=20, height=40)
+    shapes = np.concatenate([oracle.ref_shapes(src), oracle.ref_shapes(pile)])
+    kinds = {0: np.where(shapes["type"] == 0)[0], 1: np.where(shapes["type"] == 1)[0],
+             2: np.where(shapes["type"] == 2)[0]}
+    rs = S.Stream(rng_seed)
+    combos = [(0, 0), (2, 0), (2, 2), (1, 0), (1, 2)]
+    A, B, XA, XB, OUT, WM = [], [], [], [], [], []
+    for ta, tb in combos:
+        u = rs.uniform(per_kind * 8).reshape(per_kind, 8)
+        for k in range(per_kind):
+            ia = kinds[ta][int(u[k, 0] * len(kinds[ta]))]
+            ib = kinds[tb][int(u[k, 1] * len(kinds[tb]))]
+            sa, sb = shapes[ia].copy(), shapes[ib].copy()
+            aa = np.float32(u[k, 2] * 2 * math.pi)
+            ab = np.float32(u[k, 3] * 2 * math.pi)
+            if ta == 1:  # edges keep their own frame
+                pa = np.array([0.0, 0.0], np.float32)
+                aa = np.float32(0.0)
+                v1 = sa["vertices"][2:4]
+                v2 = sa["vertices"][4:6]
+                t = u[k, 4]
+                base = v1 + t * (v2 - v1)
+                n = np.array([(v2 - v1)[1], -(v2 - v1)[0]])
+                n = n / (np.linalg.norm(n) + 1e-9)
+                side = 1.0 if u[k, 6] < 0.8 else -1.0
+                pb = (base + side * n * (0.2 + 1.2 * u[k, 5])).astype(np.float32)
+            else:
+                pa = np.array([0.0, 0.0], np.float32)
+                r = 0.4 + 1.6 * u[k, 4]
+                th = u[k, 5] * 2 * math.pi
+                pb = np.array([r * math.cos(th), r * math.sin(th)], np.float32)
+            xa = np.array([pa[0], pa[1], math.sin(aa), math.cos(aa)], np.float32)
+            xb = np.array([pb[0], pb[1], math.sin(ab), math.cos(ab)], np.float32)
+            c = oracle.ref_collide(sa, xa, sb, xb)
+            wm = oracle.ref_world_manifold(c, xa, sa["radius"], xb, sb["radius"]) \
+                if c["point_count"] > 0 else np.zeros(8, np.float32)
+            A.append(sa); B.append(sb); XA.append(xa); XB.append(xb); OUT.append(c); WM.append(wm)
+    np.savez_compressed(os.path.join(HERE, "narrowphase.npz"), shape_a=np.array(A), shape_b=np.array(B),
+                        xf_a=np.array(XA), xf_b=np.array(XB), manifold=np.array(OUT),
+                        world_manifold=np.array(WM))
+    print("narrowphase:", len(OUT), "pairs,", int(sum(o["point_count"] > 0 for o in OUT)), "touching")
+
+
+def shapes():
+    """Processed shapes (Set / SetAsBox) and body mass data of two scenes."""
+    out = {}
+    for name, sc in {"mixed90": S.small_mixed(seed=3, n=90),
+                     "pile300": S.random_pile(300, seed=77, half_w=20, height=40)}.items():
+        out[f"{name}_shapes"] = oracle.ref_shapes(sc)
+        rw = oracle.RefWorld(sc)
+        out[f"{name}_mass"] = rw.mass()
+        out[f"{name}_bodies"] = rw.bodies()
+    np.savez_compressed(os.path.join(HERE, "shapes.npz"), **out)
+    print("shapes: written")
+
+
+def pairs():
+    out = {}
+    for name, (mk, steps) in {"pile2000": (lambda: S.random_pile(2000, seed=21, half_w=30, height=70), 60),
+                              "filtered": (filtered_scene, 30),
+                              "mixed": (lambda: S.small_mixed(), 100)}.items():
+        sc = mk()
+        rw = oracle.RefWorld(sc)
+        rw.step(steps)
+        arr = sc.arrays()
+        ab = oracle.ref_aabbs(rw)
+        keys = np.sort(oracle.contact_keys(rw.contacts()))
+        out[f"{name}_aabbs"] = ab
+        out[f"{name}_shape_body"] = arr["fix_body"]
+        out[f"{name}_shape_type"] = np.where(arr["fix_kind"] == 3, 2, arr["fix_kind"]).astype(np.int32)
+        out[f"{name}_filter"] = arr["fix_filter"][:, :3].copy()
+        out[f"{name}_body_type"] = arr["body_type"]
+        out[f"{name}_keys"] = keys
+        print("pairs:", name, len(keys), "pairs over", len(ab), "fixtures")
+    np.savez_compressed(os.path.join(HERE, "pairs.npz"), **out)
+
+
+def onestep():
+    for name, (mk, steps) in SCENES.items():
+        sc = mk()
+        rw = oracle.RefWorld(sc)
+        rw.step(steps)
+        state0 = rw.bodies()
+        contacts0 = rw.contacts()
+        joints0 = rw.joint_impulses()
+        order_keys, order_joints = oracle.ref_step_ordered(rw)
+        state1 = rw.bodies()
+        contacts1 = rw.contacts()
+        joints1 = rw.joint_impulses()
+        np.savez_compressed(os.path.join(HERE, f"onestep_{name}.npz"), steps=np.int32(steps), state0=state0,
+                            contacts0=contacts0, joints0=joints0, order_keys=order_keys,
+                            order_joints=order_joints, state1=state1, contacts1=contacts1, joints1=joints1)
+        print("onestep:", name, "bodies", len(state0), "contacts", len(contacts0), "solved", len(order_keys))
+
+
+def trajectory():
+    sc = S.pyramid(20)
+    rw = oracle.RefWorld(sc)
+    snaps = []
+    for k in range(20):
+        rw.step(50)
+        snaps.append(rw.bodies())
+    np.savez_compressed(os.path.join(HERE, "trajectory_pyramid.npz"), every=np.int32(50), states=np.array(snaps))
+    print("trajectory: pyramid", len(snaps), "snapshots")
+
+
+if __name__ == "__main__":
+    if not oracle.Ref.available():
+        oracle.build(ref=True)
+    narrowphase()
+    shapes()
+    pairs()
+    onestep()
+    trajectory()
