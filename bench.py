@@ -1,0 +1,189 @@
+"""Benchmark of the b2World::Step hot path on one MI355X.
+
+Default workload: the large random pile (20,000 dynamic shapes -- 60%
+convex polygons with 3-8 vertices, 40% circles -- in a static 200 x 200 m box
+of 4 edges; dt 1/60, 8 velocity / 3 position iterations, continuous physics
+and sleep off), the configuration the body-steps/s metric is quoted on.
+
+A step is one full b2World::Step of that world: narrowphase, coloured
+Gauss-Seidel solve and the LBVH broadphase, all on the GPU with the world
+resident in HBM. `value` is dynamic bodies x steps / wall time of the timed
+steps. The first W steps (warm-up) go through the drop-in b2World API; the K
+timed steps call the same kernels through the b2g C ABI without per-step
+host synchronisation.
+
+roofline: HIP events around every launch of the dominant kernel over a
+further measured segment give its average launch duration; algorithmic
+bytes per launch are the per-item payloads of the reference's data layout
+(counted once per pass) times the items the launch processed.
+cpu_baseline: the reference library itself (oracle/_ref, compiled from the
+reference sources) stepping the same scene on one host core for a bounded
+sample of steps; the CPU restatement stands in where the reference library is
+not present.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.join(REPO, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+
+# Per-item algorithmic bytes (reference FP32 payload sizes: manifold 64 B,
+# velocity constraint 168 B, position constraint 88 B, body position or
+# velocity 12 B, transform 16 B, AABB 16 B).
+#   narrowphase (per contact): 2 transforms + 2 geometries + old manifold R + new W
+#   contact velocity sweep (per solved contact): VC R + impulses W + 2 bodies' velocity R+W
+#   contact position sweep (per solved contact): PC R + 2 bodies' position R+W
+#   pair write (per shape): AABB R + key/index W (+ 8 B per reported pair)
+def item_bytes(kernel: str, geo_bytes: float) -> float:
+    return {
+        "narrowphase": 2 * 16 + 2 * geo_bytes + 64 + 64,
+        "contact_velocity": 168 + 16 + 2 * 2 * 12,
+        "contact_position": 88 + 2 * 2 * 12,
+        "pairs_write": 16 + 8,
+    }[kernel]
+
+
+def workloads(pkg):
+    S = pkg.scenes
+    return {
+        "pile20k": dict(make=lambda: S.random_pile(20000, seed=0xC2), desc="random pile, 20000 shapes "
+                        "(60% polygons 3-8 verts r U[0.25,1], 40% circles r U[0.25,0.75]) in a 200x200 m "
+                        "edge box, seed 0xC2", steps_total=500),
+        "pyramid20": dict(make=lambda: S.pyramid(20), desc="pyramid of 20 rows (210 boxes) on an edge ground",
+                          steps_total=1000),
+        "gas100k_chains500": dict(make=lambda: S.gas_and_chains(), desc="100000 circles + 500 chains of 30 "
+                                  "revolute-jointed boxes, zero gravity, 1000x1000 m box", steps_total=300),
+    }
+
+
+def mean_geometry_bytes(scene) -> float:
+    import numpy as np
+    kinds = np.asarray(scene.fix_kind)
+    geo = np.asarray(scene.fix_geom)
+    b = np.where(kinds == 0, 12.0, np.where(kinds == 1, 40.0, 0.0))
+    poly = kinds == 2
+    b[poly] = 16.0 * geo[poly, 0] + 16.0
+    b[kinds == 3] = 16.0 * 4 + 16.0
+    dyn_shapes = np.asarray(scene.body_type)[np.asarray(scene.fix_body)] == 2
+    return float(b[dyn_shapes].mean()) if dyn_shapes.any() else float(b.mean())
+
+
+def cpu_baseline(scene, steps: int):
+    import numpy as np
+    import oracle
+    if oracle.Ref.available():
+        w = oracle.RefWorld(scene)
+        kind = "reference"
+    else:
+        w = oracle.RestatedWorld(scene)
+        kind = "port"
+    t0 = time.perf_counter()
+    w.step(steps)
+    el = time.perf_counter() - t0
+    ndyn = scene.dynamic_count
+    return {"value": ndyn * steps / el, "unit": "body-steps/s", "cores": 1, "kind": kind,
+            "sample": f"steps 1-{steps} of the same scene, single thread, {el:.1f} s wall, "
+                      f"{el / steps * 1e3:.2f} ms/step mean"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="pile20k")
+    ap.add_argument("--kernel", default="contact_velocity",
+                    choices=["narrowphase", "contact_velocity", "contact_position", "pairs_write"])
+    ap.add_argument("--kernel-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=-1, help="-1: about 15 s of reference CPU work")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: worlds shard by world index across GPUs with no "
+                                   "exchange, but no multi-GPU run is part of it; use --gpus 1"}))
+        sys.exit(2)
+
+    from util import load_pkg
+    import numpy as np
+    pkg = load_pkg()
+    eng = pkg.engine
+    wl = workloads(pkg)[args.workload]
+    scene = wl["make"]()
+    ndyn = scene.dynamic_count
+
+    world = eng.World(scene)
+    world.step(max(args.warmup, 1))
+    batch = world.batch()
+    dt = np.float32(1.0 / 60.0)
+    ratio = np.float32(np.float32(1.0) / dt) * dt
+    batch.set_world_params(0, [eng.WorldParams((scene.gravity[0], scene.gravity[1]), dt, ratio, 1, 1)])
+    batch.wait()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.step(8, 3, eng.STEP_ASYNC)
+    batch.wait()
+    elapsed = time.perf_counter() - t0
+    counts = batch.counts()
+
+    kid = {"narrowphase": eng.KERNEL_NARROWPHASE, "contact_velocity": eng.KERNEL_CONTACT_VELOCITY,
+           "contact_position": eng.KERNEL_CONTACT_POSITION, "pairs_write": eng.KERNEL_PAIRS_WRITE}[args.kernel]
+    batch.time_kernel(kid)
+    for _ in range(args.kernel_steps):
+        batch.step(8, 3, eng.STEP_ASYNC)
+    batch.wait()
+    k_ms, k_launches, k_items = batch.kernel_time()
+    batch.time_kernel(-1)
+    geo = mean_geometry_bytes(scene)
+    per_item = item_bytes(args.kernel, geo)
+    avg_ms = k_ms / max(k_launches, 1)
+    bytes_per_launch = per_item * k_items / max(k_launches, 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+
+    cpu = None
+    if not args.no_cpu:
+        cpu_steps = args.cpu_steps
+        if cpu_steps < 0:
+            cpu_steps = 200 if args.workload == "pile20k" else 1000 if args.workload == "pyramid20" else 30
+        cpu = cpu_baseline(wl["make"](), cpu_steps)
+
+    value = ndyn * args.steps / elapsed
+    line = {
+        "metric": "body-steps/s on 1 GPU vs host CPU (20k pile)",
+        "value": value,
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene generator)",
+        "config": {"workload": args.workload, "description": wl["desc"], "dynamic_bodies": ndyn,
+                   "shapes": scene.fixture_count, "joints": scene.joint_count, "dt": "1/60",
+                   "velocity_iterations": 8, "position_iterations": 3,
+                   "contacts": counts.contacts, "touching": counts.touching, "colours": counts.colours,
+                   "steps_timed": f"{args.warmup + 1}-{args.warmup + args.steps}"},
+        "roofline": {"bound": "hbm", "kernel": args.kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": avg_ms, "launches": k_launches,
+                     "bytes_per_launch": bytes_per_launch, "bytes_per_item": per_item},
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "host": platform.node(),
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -76,6 +76,42 @@ struct b2g_batch {
   b2g_profile prof{};
   // last counters
   int last_ncolours = 0, last_islands = 0, last_pos_iters = 0;
+  // HIP-event timing of one selected kernel (b2g_time_kernel)
+  int timed_kernel = -1;
+  std::vector<hipEvent_t> tev;
+  size_t tev_used = 0;
+  std::vector<long long> t_items;
+  double t_ms = 0.0;
+  long long t_launches = 0, t_total_items = 0;
+  void tbegin(int id) {
+    if (id != timed_kernel) return;
+    if (tev_used + 2 > tev.size()) {
+      hipEvent_t e0, e1;
+      (void)hipEventCreate(&e0);
+      (void)hipEventCreate(&e1);
+      tev.push_back(e0);
+      tev.push_back(e1);
+    }
+    (void)hipEventRecord(tev[tev_used], stream);
+  }
+  void tend(int id, long long items) {
+    if (id != timed_kernel) return;
+    (void)hipEventRecord(tev[tev_used + 1], stream);
+    tev_used += 2;
+    t_items.push_back(items);
+  }
+  void tcollect() {
+    if (tev_used == 0) return;
+    (void)hipStreamSynchronize(stream);
+    for (size_t k = 0; k + 1 < tev_used; k += 2) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, tev[k], tev[k + 1]) == hipSuccess) t_ms += ms;
+      ++t_launches;
+      t_total_items += t_items[k / 2];
+    }
+    tev_used = 0;
+    t_items.clear();
+  }
 
   int alloc();
   void release();
@@ -203,7 +239,9 @@ int b2g_batch::broadphase(bool use_given_aabbs) {
   hipLaunchKernelGGL(k_lbvh_refit, dim3(gs), dim3(kBlock), 0, stream, d, d.mval[0]);
   hipLaunchKernelGGL(k_lbvh_pairs<false>, dim3(gs), dim3(kBlock), 0, stream, d);
   scan(d.pcount, nullptr, ns, d.poff, d.cnt + C_PAIRS, ns);
+  tbegin(B2G_KERNEL_PAIRS_WRITE);
   hipLaunchKernelGGL(k_lbvh_pairs<true>, dim3(gs), dim3(kBlock), 0, stream, d);
+  tend(B2G_KERNEL_PAIRS_WRITE, ns);
   hipLaunchKernelGGL(k_merge_contacts, dim3(grid_for(std::max(est_contacts, ns) * 5 / 4)), dim3(kBlock), 0, stream,
                      d);
   hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, stream, d);
@@ -286,7 +324,9 @@ int b2g_batch::step(int vi, int pi, unsigned flags) {
   const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
   const int gb = grid_for(nb);
   // ---- Collide
+  tbegin(B2G_KERNEL_NARROWPHASE);
   hipLaunchKernelGGL(k_narrowphase, dim3(gc), dim3(kBlock), 0, stream, d);
+  tend(B2G_KERNEL_NARROWPHASE, est_contacts);
   scan(d.touch, d.cnt + C_CONTACTS, 0, d.tpos, d.cnt + C_CONS, d.ccap);
   hipLaunchKernelGGL(k_compact_touching, dim3(gc), dim3(kBlock), 0, stream, d);
   if (prof_on) HIP_TRY(hipEventRecord(ev[1], stream));
@@ -327,14 +367,16 @@ int b2g_batch::step(int vi, int pi, unsigned flags) {
   const int* off = pinned + 32;
   const int ncol = pinned[C_NCOLOURS];
   last_ncolours = ncol;
-  auto contact_sweep = [&](auto kernel, auto... extra) {
+  auto contact_sweep = [&](int id, auto kernel, auto... extra) {
     for (int c = 0; c < ncol; ++c) {
       const int b = off[c], e = off[c + 1];
       if (e <= b) continue;
+      tbegin(id);
       if (c == kSerialColour)
         hipLaunchKernelGGL(kernel, dim3(1), dim3(1), 0, stream, d, b, e, extra...);
       else
         hipLaunchKernelGGL(kernel, dim3(grid_for(e - b)), dim3(kBlock), 0, stream, d, b, e, extra...);
+      tend(id, e - b);
     }
   };
   auto joint_sweep = [&](auto kernel, auto... extra) {
@@ -344,17 +386,17 @@ int b2g_batch::step(int vi, int pi, unsigned flags) {
       if (e > b) hipLaunchKernelGGL(kernel, dim3(grid_for(e - b)), dim3(kBlock), 0, stream, d, b, e, extra...);
     }
   };
-  contact_sweep(k_warm_start);
+  contact_sweep(B2G_KERNEL_WARM_START, k_warm_start);
   joint_sweep(k_joint_init);
   for (int it = 0; it < vi; ++it) {
     joint_sweep(k_joint_velocity);
-    contact_sweep(k_contact_velocity);
+    contact_sweep(B2G_KERNEL_CONTACT_VELOCITY, k_contact_velocity);
   }
   hipLaunchKernelGGL(k_store_impulses, dim3(gk), dim3(kBlock), 0, stream, d);
   hipLaunchKernelGGL(k_integrate_positions, dim3(gb), dim3(kBlock), 0, stream, d);
   const int gf = grid_for(std::max(est_cons, 1) + d.nj);
   for (int it = 0; it < pi; ++it) {
-    contact_sweep(k_contact_position, it);
+    contact_sweep(B2G_KERNEL_CONTACT_POSITION, k_contact_position, it);
     joint_sweep(k_joint_position, it);
     hipLaunchKernelGGL(k_island_finalize, dim3(gf), dim3(kBlock), 0, stream, d, it);
   }
@@ -365,12 +407,14 @@ int b2g_batch::step(int vi, int pi, unsigned flags) {
     if ((rc = broadphase(false))) return rc;
   }
   if (prof_on) HIP_TRY(hipEventRecord(ev[3], stream));
+  if (tev_used > 64) tcollect();
   if (!(flags & B2G_STEP_ASYNC) || prof_on) {
     if ((rc = read_counters())) return rc;
     est_contacts = pinned[C_CONTACTS];
     if (pinned[C_STATUS] & S_CONTACT_OVERFLOW)
       return fail(B2G_ERR_CAPACITY, "contact capacity exceeded (" + std::to_string(pinned[C_PAIRS]) + " pairs)");
     if (pinned[C_STATUS] & S_STACK_OVERFLOW) return fail(B2G_ERR_CAPACITY, "BVH traversal stack overflow");
+    if (pinned[C_STATUS] & S_TREE_CORRUPT) return fail(B2G_ERR_INVALID, "BVH inconsistent (sort/build)");
   }
   if (prof_on) {
     float t01, t12, t23, t03;
@@ -610,6 +654,9 @@ int b2g_upload_contacts(b2g_batch* b, int32_t count, const b2g_contact* in) {
   std::vector<int> idx(count);
   std::vector<uint64_t> keys(count);
   for (int i = 0; i < count; ++i) {
+    if (in[i].shape_a < 0 || in[i].shape_a >= b->d.ns || in[i].shape_b < 0 || in[i].shape_b >= b->d.ns ||
+        in[i].shape_a == in[i].shape_b || in[i].point_count < 0 || in[i].point_count > 2)
+      return fail(B2G_ERR_INVALID, "contact record out of range (upload shapes first)");
     const uint32_t a = (uint32_t)in[i].shape_a, c = (uint32_t)in[i].shape_b;
     keys[i] = ((uint64_t)std::min(a, c) << 32) | std::max(a, c);
     idx[i] = i;
@@ -669,6 +716,25 @@ int b2g_get_counts(b2g_batch* b, b2g_counts* out) {
   out->islands = b->last_islands;
   out->pos_iters_run = b->last_pos_iters;
   out->status = b->pinned[C_STATUS];
+  return B2G_OK;
+}
+
+int b2g_time_kernel(b2g_batch* b, int32_t kernel) {
+  if (!b) return fail(B2G_ERR_INVALID, "null batch");
+  b->tcollect();
+  b->timed_kernel = kernel;
+  b->t_ms = 0.0;
+  b->t_launches = 0;
+  b->t_total_items = 0;
+  return B2G_OK;
+}
+
+int b2g_kernel_time(b2g_batch* b, double* total_ms, int64_t* launches, int64_t* items) {
+  if (!b) return fail(B2G_ERR_INVALID, "null batch");
+  b->tcollect();
+  if (total_ms) *total_ms = b->t_ms;
+  if (launches) *launches = b->t_launches;
+  if (items) *items = b->t_total_items;
   return B2G_OK;
 }
 

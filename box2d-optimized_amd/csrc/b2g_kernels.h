@@ -44,7 +44,7 @@ enum Counter : int {
   C_MAXCOLOUR = 6,
   C_COUNT = 16
 };
-enum Status : int { S_CONTACT_OVERFLOW = 1, S_STACK_OVERFLOW = 2 };
+enum Status : int { S_CONTACT_OVERFLOW = 1, S_STACK_OVERFLOW = 2, S_TREE_CORRUPT = 4 };
 enum BodyType : int { kStaticBody = 0, kKinematicBody = 1, kDynamicBody = 2 };
 constexpr int kMaxColours = 64;           // bit-mask colours; colour 64 = serial group
 constexpr int kSerialColour = kMaxColours;
@@ -129,6 +129,60 @@ __device__ __forceinline__ int xcd_block() {
 #define B2G_FOR(i, begin, end)                                                     \
   for (int i = (begin) + xcd_block() * (int)blockDim.x + (int)threadIdx.x; i < (end); \
        i += (int)(gridDim.x * blockDim.x))
+
+// Same iteration space as B2G_FOR, but every lane of a block stays in the
+// loop until the block is done (lanes past `end` run with valid == false), so
+// wave-wide shuffles inside the body see all 64 lanes.
+#define B2G_FOR_ALL(i, valid, begin, end)                                                     \
+  for (int base_##i = (begin) + xcd_block() * (int)blockDim.x; base_##i < (end);              \
+       base_##i += (int)(gridDim.x * blockDim.x))                                             \
+    for (int i = base_##i + (int)threadIdx.x, valid = (i < (end)); valid >= 0; valid = -1)
+
+// one atomic per wave for a counter update from any subset of active lanes
+__device__ __forceinline__ void wave_atomic_add(int* addr, int per_lane) {
+  const uint64_t act = __ballot(1);
+  const int leader = __ffsll((long long)act) - 1;
+  if (lane_id() == leader) atomicAdd(addr, per_lane * __popcll(act));
+}
+
+__device__ __forceinline__ float wave_min_all(float v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = fminf(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+__device__ __forceinline__ float wave_max_all(float v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+
+// atomicMin of v into base[key] for the valid lanes; when all valid lanes of
+// the wave share one key (one island / one world, the common case) the wave
+// reduces first and issues a single atomic. All 64 lanes must call this.
+__device__ __forceinline__ void wave_atomic_min_keyed(unsigned* base, int key, bool valid, float v) {
+  const uint64_t act = __ballot(valid);
+  if (act == 0) return;
+  const int first = __ffsll((long long)act) - 1;
+  const int key0 = __shfl(key, first, kWave);
+  if (__ballot(valid && key != key0) == 0) {
+    const float m = wave_min_all(valid ? v : FLT_MAX);
+    if (lane_id() == first) atomicMin(base + key0, float_to_ordered(m));
+  } else if (valid) {
+    atomicMin(base + key, float_to_ordered(v));
+  }
+}
+__device__ __forceinline__ void wave_atomic_max_keyed(unsigned* base, int key, bool valid, float v) {
+  const uint64_t act = __ballot(valid);
+  if (act == 0) return;
+  const int first = __ffsll((long long)act) - 1;
+  const int key0 = __shfl(key, first, kWave);
+  if (__ballot(valid && key != key0) == 0) {
+    const float m = wave_max_all(valid ? v : -FLT_MAX);
+    if (lane_id() == first) atomicMax(base + key0, float_to_ordered(m));
+  } else if (valid) {
+    atomicMax(base + key, float_to_ordered(v));
+  }
+}
 
 __device__ __forceinline__ vec2 v2of(float4 f) { return mk(f.x, f.y); }
 __device__ __forceinline__ bool world_active(const Dev& d, int w) { return (d.wflag[w] & 2) != 0; }
@@ -245,7 +299,7 @@ __global__ void __launch_bounds__(kBlock) k_colour_init(Dev d) {
       d.ccolour[k] = col;
     } else {
       d.ccolour[k] = -1;
-      atomicAdd(&d.cnt[C_REMAIN], 1);
+      wave_atomic_add(&d.cnt[C_REMAIN], 1);
     }
   }
 }
@@ -291,17 +345,23 @@ __global__ void __launch_bounds__(kBlock) k_colour_resolve(Dev d, int round) {
       if (da) atomicOr(&d.bmask[ba], bit);
       if (db) atomicOr(&d.bmask[bb], bit);
     }
-    atomicSub(&d.cnt[C_REMAIN], 1);
+    wave_atomic_add(&d.cnt[C_REMAIN], -1);
   }
 }
 
 __global__ void __launch_bounds__(kBlock) k_colour_count(Dev d) {
+  __shared__ int hist[kColourSlots];
+  for (int c = threadIdx.x; c < kColourSlots; c += blockDim.x) hist[c] = 0;
+  __syncthreads();
   const int n = d.cnt[C_CONS];
   B2G_FOR(k, 0, n) {
     const int col = d.ccolour[k];
-    atomicAdd(&d.col_count[col], 1);
+    atomicAdd(&hist[col], 1);
     d.ccol[d.clist[k]] = col;
   }
+  __syncthreads();
+  for (int c = threadIdx.x; c < kColourSlots; c += blockDim.x)
+    if (hist[c]) atomicAdd(&d.col_count[c], hist[c]);
 }
 
 // one thread: offsets of the colour groups; resets counters for the next step
@@ -318,12 +378,25 @@ __global__ void k_colour_offsets(Dev d) {
   d.cnt[C_NCOLOURS] = ncol;
 }
 
+// Each block reserves a contiguous range per colour with one atomic, then
+// places its constraints inside it (order inside a colour does not matter:
+// those constraints share no dynamic body).
 __global__ void __launch_bounds__(kBlock) k_colour_scatter(Dev d) {
+  __shared__ int hist[kColourSlots];
+  __shared__ int base[kColourSlots];
   const int n = d.cnt[C_CONS];
-  B2G_FOR(k, 0, n) {
-    const int col = d.ccolour[k];
-    const int slot = d.col_off[col] + atomicAdd(&d.col_cursor[col], 1);
-    d.csorted[slot] = d.clist[k];
+  for (int start = xcd_block() * kBlock; start < n; start += gridDim.x * kBlock) {
+    for (int c = threadIdx.x; c < kColourSlots; c += blockDim.x) hist[c] = 0;
+    __syncthreads();
+    const int k = start + (int)threadIdx.x;
+    const int col = k < n ? d.ccolour[k] : -1;
+    const int local = col >= 0 ? atomicAdd(&hist[col], 1) : 0;
+    __syncthreads();
+    for (int c = threadIdx.x; c < kColourSlots; c += blockDim.x)
+      base[c] = hist[c] ? d.col_off[c] + atomicAdd(&d.col_cursor[c], hist[c]) : 0;
+    __syncthreads();
+    if (col >= 0) d.csorted[base[col] + local] = d.clist[k];
+    __syncthreads();
   }
 }
 
@@ -749,9 +822,9 @@ __global__ void __launch_bounds__(kBlock) k_integrate_positions(Dev d) {
 }
 
 // b2ContactSolver::SolvePositionConstraints (:659-787) for slots [begin, end)
-__device__ __forceinline__ void contact_position(const Dev& d, int s, int it) {
-  const int isl = d.cisland[s];
-  if (d.solved[isl]) return;
+// returns the minimum separation seen (0 when the island is already solved)
+__device__ __forceinline__ float contact_position(const Dev& d, int s, int isl) {
+  if (d.solved[isl]) return 0.0f;
   const float4 v2i = d.vc[2][s];
   const int ia = as_int(v2i.x), ib = as_int(v2i.y);
   const float4 q0 = d.pc[0][s], q1 = d.pc[1][s], q2 = d.pc[2][s], q3 = d.pc[3][s], q4 = d.pc[4][s];
@@ -810,11 +883,23 @@ __device__ __forceinline__ void contact_position(const Dev& d, int s, int it) {
   }
   if (writes_body(d, ia)) d.pos[ia] = make_float4(cA.x, cA.y, aA, 0.0f);
   if (writes_body(d, ib)) d.pos[ib] = make_float4(cB.x, cB.y, aB, 0.0f);
-  atomicMin(&d.minsep[(it & 1) * d.nb + isl], float_to_ordered(min_sep));
+  return min_sep;
 }
 
 __global__ void __launch_bounds__(kBlock) k_contact_position(Dev d, int begin, int end, int it) {
-  B2G_FOR(s, begin, end) contact_position(d, s, it);
+  if (blockDim.x == 1) {  // the serial colour group: one lane, slot order
+    for (int s = begin; s < end; ++s) {
+      const int isl = d.cisland[s];
+      const float m = contact_position(d, s, isl);
+      atomicMin(&d.minsep[(it & 1) * d.nb + isl], float_to_ordered(m));
+    }
+    return;
+  }
+  B2G_FOR_ALL(s, valid, begin, end) {
+    const int isl = valid ? d.cisland[s] : 0;
+    const float m = valid ? contact_position(d, s, isl) : 0.0f;
+    wave_atomic_min_keyed(d.minsep + (it & 1) * d.nb, isl, valid != 0, m);
+  }
 }
 
 // ===========================================================================
@@ -1088,17 +1173,23 @@ __device__ __forceinline__ float4 shape_aabb(int type, int count, const float4* 
 }
 
 __global__ void __launch_bounds__(kBlock) k_aabb(Dev d) {
-  B2G_FOR(i, 0, d.ns) {
-    const int4 h = d.shdr[i];
-    const float4 bb = shape_aabb(h.x, h.z, d.geo + 9 * (size_t)i, xform_of(d.xf[h.y]));
-    d.aabb[i] = bb;
-    // world bounds of centres (2 * centre, as b2BroadPhase::GetCenter2)
-    const float cx = bb.x + bb.z, cy = bb.y + bb.w;
-    unsigned* wb = d.wbounds + 4 * h.w;
-    atomicMin(&wb[0], float_to_ordered(cx));
-    atomicMin(&wb[1], float_to_ordered(cy));
-    atomicMax(&wb[2], float_to_ordered(cx));
-    atomicMax(&wb[3], float_to_ordered(cy));
+  B2G_FOR_ALL(i, valid, 0, d.ns) {
+    int w = 0;
+    float cx = 0.0f, cy = 0.0f;
+    if (valid) {
+      const int4 h = d.shdr[i];
+      const float4 bb = shape_aabb(h.x, h.z, d.geo + 9 * (size_t)i, xform_of(d.xf[h.y]));
+      d.aabb[i] = bb;
+      // world bounds of centres (2 * centre, as b2BroadPhase::GetCenter2)
+      cx = bb.x + bb.z;
+      cy = bb.y + bb.w;
+      w = h.w;
+    }
+    unsigned* wb = d.wbounds;
+    wave_atomic_min_keyed(wb + 0, 4 * w, valid != 0, cx);
+    wave_atomic_min_keyed(wb + 1, 4 * w, valid != 0, cy);
+    wave_atomic_max_keyed(wb + 2, 4 * w, valid != 0, cx);
+    wave_atomic_max_keyed(wb + 3, 4 * w, valid != 0, cy);
   }
 }
 
@@ -1199,7 +1290,12 @@ __global__ void __launch_bounds__(kBlock) k_lbvh_refit(Dev d, const uint32_t* __
     if (n < 2) continue;
     __threadfence();
     int node = d.leaf_parent[k];
+    int guard = 0;
     while (node >= 0) {
+      if (++guard > 256 || node >= n - 1) {  // deeper than any tree over 64-bit keys
+        atomicOr(&d.cnt[C_STATUS], S_TREE_CORRUPT);
+        break;
+      }
       if (atomicAdd(&d.node_visit[node], 1) == 0) break;  // first child to arrive stops
       __threadfence();
       const int l = d.node_child[2 * node], r = d.node_child[2 * node + 1];
@@ -1255,8 +1351,13 @@ __global__ void __launch_bounds__(kBlock) k_lbvh_pairs(Dev d) {
       int stack[kStack];
       int sp = 0;
       stack[sp++] = 0;
+      int visits = 0;
       while (sp > 0) {
         const int node = stack[--sp];
+        if (++visits > 2 * n) {  // a tree has n - 1 internal nodes
+          atomicOr(&d.cnt[C_STATUS], S_TREE_CORRUPT);
+          break;
+        }
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
           const int ch = d.node_child[2 * node + side];

@@ -93,10 +93,10 @@ __global__ void __launch_bounds__(kBlock) k_scan_sums(int* __restrict__ tile_sum
   if (threadIdx.x == 0 && total_out) *total_out = carry;
 }
 
-// pass 3: scan inside each tile and add the tile offset
-__global__ void __launch_bounds__(kBlock) k_scan_apply(const int* __restrict__ in, const int* __restrict__ n_dev,
-                                                       int n_host, const int* __restrict__ tile_sums,
-                                                       int* __restrict__ out) {
+// pass 3: scan inside each tile and add the tile offset (in and out may alias:
+// every thread reads its own items before the block barrier, writes after)
+__global__ void __launch_bounds__(kBlock) k_scan_apply(const int* in, const int* __restrict__ n_dev, int n_host,
+                                                       const int* __restrict__ tile_sums, int* out) {
   __shared__ int lds[kBlock / kWave];
   const int n = n_dev ? *n_dev : n_host;
   const int tiles = (n + kScanTile - 1) / kScanTile;
@@ -180,8 +180,10 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint64_t* __rest
         for (int q = 0; q < kWaves; ++q)
           if (q < w) before += wave_cnt[q][digit];
         const int dst = running[digit] + before + rank_in_wave;
-        keys_out[dst] = key;
-        vals_out[dst] = vals_in[i];
+        if (dst >= 0 && dst < n) {
+          keys_out[dst] = key;
+          vals_out[dst] = vals_in[i];
+        }
       }
       __syncthreads();
       for (int d = threadIdx.x; d < kRadix; d += kBlock) {

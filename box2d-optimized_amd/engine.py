@@ -80,7 +80,10 @@ EXPORTS = [
     "b2g_download_joints", "b2g_download_contact_colours", "b2g_collide_pairs", "b2g_find_pairs",
     "b2a_world_from_scene", "b2a_world_destroy", "b2a_world_step", "b2a_world_get_bodies",
     "b2a_world_contact_count", "b2a_world_get_contacts", "b2a_world_batch", "b2a_world_prepare",
+    "b2g_time_kernel", "b2g_kernel_time",
 ]
+KERNEL_NARROWPHASE, KERNEL_WARM_START, KERNEL_CONTACT_VELOCITY, KERNEL_CONTACT_POSITION, \
+    KERNEL_PAIRS_WRITE, KERNEL_SOLVER = range(6)
 
 
 class B2GError(RuntimeError):
@@ -120,6 +123,8 @@ def lib():
         L.b2a_world_batch.restype = _P
         L.b2a_world_batch.argtypes = [_P]
         L.b2a_world_prepare.argtypes = [_P]
+        L.b2g_time_kernel.argtypes = [_P, _I]
+        L.b2g_kernel_time.argtypes = [_P, _P, _P, _P]
         _lib = L
     return _lib
 
@@ -205,6 +210,17 @@ class Batch:
         p = Profile()
         check(self.L.b2g_get_profile(self.h, ctypes.byref(p)))
         return p
+
+    def time_kernel(self, kernel):
+        check(self.L.b2g_time_kernel(self.h, kernel))
+
+    def kernel_time(self):
+        """(total ms, launches, items) of the timed kernel since time_kernel()."""
+        ms = ctypes.c_double(0.0)
+        n = ctypes.c_int64(0)
+        items = ctypes.c_int64(0)
+        check(self.L.b2g_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(items)))
+        return ms.value, n.value, items.value
 
     def bodies(self, first=0, count=None):
         if count is None:

@@ -232,6 +232,20 @@ int b2g_batch_step(b2g_batch* batch, int32_t velocity_iterations, int32_t positi
 int b2g_batch_wait(b2g_batch* batch);
 
 int b2g_get_counts(b2g_batch* batch, b2g_counts* out);
+
+/* HIP-event timing of every launch of one kernel during the following steps
+ * (-1 = off). b2g_kernel_time returns the summed device time, the number of
+ * launches and the work items they covered (contacts, constraints or shapes). */
+enum {
+  B2G_KERNEL_NARROWPHASE = 0,
+  B2G_KERNEL_WARM_START = 1,
+  B2G_KERNEL_CONTACT_VELOCITY = 2,
+  B2G_KERNEL_CONTACT_POSITION = 3,
+  B2G_KERNEL_PAIRS_WRITE = 4,
+  B2G_KERNEL_SOLVER = 5
+};
+int b2g_time_kernel(b2g_batch* batch, int32_t kernel);
+int b2g_kernel_time(b2g_batch* batch, double* total_ms, int64_t* launches, int64_t* items);
 int b2g_get_profile(b2g_batch* batch, b2g_profile* out);
 int b2g_download_bodies(b2g_batch* batch, int32_t first, int32_t count, b2g_body* out);
 /* contacts are ordered by (min shape, max shape); out may be NULL to query the count */

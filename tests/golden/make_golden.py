@@ -193,14 +193,28 @@ def onestep():
 
 
 def trajectory():
+    """Reference pyramid (20 rows): body states, minimum world-manifold
+    separation over touching contacts and kinetic energy every 25 steps."""
     sc = S.pyramid(20)
     rw = oracle.RefWorld(sc)
-    snaps = []
-    for k in range(20):
-        rw.step(50)
-        snaps.append(rw.bodies())
-    np.savez_compressed(os.path.join(HERE, "trajectory_pyramid.npz"), every=np.int32(50), states=np.array(snaps))
-    print("trajectory: pyramid", len(snaps), "snapshots")
+    snaps, minsep, ke = [], [], []
+    mass = rw.mass()[:, 0]
+    for k in range(40):
+        rw.step(25)
+        st = rw.bodies()
+        c = rw.contacts()
+        seps = [0.0]
+        for rec in c[c["point_count"] > 0]:
+            xa = st[sc.fix_body[rec["shape_a"]], 6:10]
+            xb = st[sc.fix_body[rec["shape_b"]], 6:10]
+            out = oracle.ref_world_manifold(rec, xa, 0.01, xb, 0.01)
+            seps.extend(out[6:6 + rec["point_count"]].tolist())
+        snaps.append(st)
+        minsep.append(min(seps))
+        ke.append(float(np.sum(0.5 * mass * (st[:, 3] ** 2 + st[:, 4] ** 2))))
+    np.savez_compressed(os.path.join(HERE, "trajectory_pyramid.npz"), every=np.int32(25), states=np.array(snaps),
+                        minsep=np.array(minsep, np.float32), ke=np.array(ke, np.float64))
+    print("trajectory: pyramid", len(snaps), "snapshots; final min sep", minsep[-1], "KE", ke[-1])
 
 
 if __name__ == "__main__":
