@@ -77,7 +77,9 @@ def mean_geometry_bytes(scene) -> float:
     return float(b[dyn_shapes].mean()) if dyn_shapes.any() else float(b.mean())
 
 
-def cpu_baseline(scene, steps: int):
+def cpu_baseline(scene, warmup: int, steps: int):
+    """The reference Step on one host core over the same simulation steps the
+    GPU timed (warm-up steps untimed, as for the GPU)."""
     import numpy as np
     import oracle
     if oracle.Ref.available():
@@ -86,13 +88,21 @@ def cpu_baseline(scene, steps: int):
     else:
         w = oracle.RestatedWorld(scene)
         kind = "port"
+    if warmup:
+        w.step(warmup)
     t0 = time.perf_counter()
-    w.step(steps)
+    if kind == "reference":
+        ms = w.step(steps, timed=True)
+    else:
+        w.step(steps)
+        ms = None
     el = time.perf_counter() - t0
     ndyn = scene.dynamic_count
+    p99 = float(np.percentile(ms, 99)) if ms is not None else None
     return {"value": ndyn * steps / el, "unit": "body-steps/s", "cores": 1, "kind": kind,
-            "sample": f"steps 1-{steps} of the same scene, single thread, {el:.1f} s wall, "
-                      f"{el / steps * 1e3:.2f} ms/step mean"}
+            "sample": f"steps {warmup + 1}-{warmup + steps} of the same scene (the GPU's timed steps), "
+                      f"single thread, {el:.1f} s wall, {el / steps * 1e3:.2f} ms/step mean"
+                      + (f", p99 {p99:.2f} ms" if p99 is not None else "")}
 
 
 def main():
@@ -104,7 +114,7 @@ def main():
     ap.add_argument("--kernel", default="contact_velocity",
                     choices=["narrowphase", "contact_velocity", "contact_position", "pairs_write"])
     ap.add_argument("--kernel-steps", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=-1, help="-1: about 15 s of reference CPU work")
+    ap.add_argument("--cpu-steps", type=int, default=-1, help="-1: the GPU's timed steps (bounded)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -152,8 +162,9 @@ def main():
     if not args.no_cpu:
         cpu_steps = args.cpu_steps
         if cpu_steps < 0:
-            cpu_steps = 200 if args.workload == "pile20k" else 1000 if args.workload == "pyramid20" else 30
-        cpu = cpu_baseline(wl["make"](), cpu_steps)
+            cpu_steps = min(args.steps, 400 if args.workload == "pile20k" else 1000 if args.workload == "pyramid20"
+                            else 40)
+        cpu = cpu_baseline(wl["make"](), args.warmup, cpu_steps)
 
     value = ndyn * args.steps / elapsed
     line = {

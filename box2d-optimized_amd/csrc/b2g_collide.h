@@ -54,12 +54,18 @@ __device__ __forceinline__ void load_geo(Geo& g, int type, int count, const floa
   }
 }
 
-// value of a[i] for a data-dependent i, without dynamic register indexing
+// value of a[i] for a data-dependent i, as a chain of register selects. The
+// empty asm after each select is an optimisation barrier: without it the
+// compiler folds the chain back into a[i], which forces the arrays into
+// scratch memory.
 __device__ __forceinline__ vec2 pick(const vec2 (&a)[kMaxVerts], int i) {
   vec2 r = a[0];
 #pragma unroll
-  for (int k = 1; k < kMaxVerts; ++k)
-    if (k == i) r = a[k];
+  for (int k = 1; k < kMaxVerts; ++k) {
+    r.x = (k == i) ? a[k].x : r.x;
+    r.y = (k == i) ? a[k].y : r.y;
+    asm volatile("" : "+v"(r.x), "+v"(r.y));
+  }
   return r;
 }
 
@@ -183,21 +189,6 @@ __device__ __forceinline__ void collide_polygon_circle(Man& m, const Geo& a, con
   }
 }
 
-// field-wise select (keeps both operands in registers)
-__device__ __forceinline__ Geo select_geo(bool c, const Geo& x, const Geo& y) {
-  Geo r;
-  r.type = c ? x.type : y.type;
-  r.count = c ? x.count : y.count;
-  r.radius = c ? x.radius : y.radius;
-  r.centroid = c ? x.centroid : y.centroid;
-#pragma unroll
-  for (int k = 0; k < kMaxVerts; ++k) {
-    r.v[k] = c ? x.v[k] : y.v[k];
-    r.n[k] = c ? x.n[k] : y.n[k];
-  }
-  return r;
-}
-
 // b2FindMaxSeparation (src/collision/b2_collide_polygon.cpp:27-66)
 __device__ __forceinline__ float find_max_separation(int& edge, const Geo& p1, const xform& x1,
                                                      const Geo& p2, const xform& x2) {
@@ -224,26 +215,11 @@ __device__ __forceinline__ float find_max_separation(int& edge, const Geo& p1, c
   return max_sep;
 }
 
-// b2CollidePolygons (src/collision/b2_collide_polygon.cpp:68-243)
-__device__ __forceinline__ void collide_polygons(Man& m, const Geo& a, const xform& xa, const Geo& b,
-                                                 const xform& xb) {
-  m.count = 0;
-  const float total_radius = a.radius + b.radius;
-  int edge_a = 0;
-  const float sep_a = find_max_separation(edge_a, a, xa, b, xb);
-  if (sep_a > total_radius) return;
-  int edge_b = 0;
-  const float sep_b = find_max_separation(edge_b, b, xb, a, xa);
-  if (sep_b > total_radius) return;
-  const float k_tol = 0.1f * kLinearSlop;
-  const bool flip = sep_b > sep_a + k_tol;
-  const Geo p1 = select_geo(flip, b, a);
-  const Geo p2 = select_geo(flip, a, b);
-  const xform x1 = flip ? xb : xa;
-  const xform x2 = flip ? xa : xb;
-  const int edge1 = flip ? edge_b : edge_a;
-  m.type = flip ? kManFaceB : kManFaceA;
-
+// Reference face on p1 (edge1), incident edge on p2; the tail of
+// b2CollidePolygons (src/collision/b2_collide_polygon.cpp:68-111, 164-243).
+// Called with (a, b) or (b, a) so the roles never need a runtime copy.
+__device__ __forceinline__ void clip_polygons(Man& m, const Geo& p1, const xform& x1, int edge1, const Geo& p2,
+                                              const xform& x2, float total_radius, bool flip) {
   // b2FindIncidentEdge (:68-111)
   const vec2 normal1 = mul_t(x2.q, mul(x1.q, pick(p1.n, edge1)));
   int index = 0;
@@ -282,23 +258,42 @@ __device__ __forceinline__ void collide_polygons(Man& m, const Geo& a, const xfo
   m.ln = local_normal;
   m.lp = plane_point;
   int pc = 0;
-  {
-    const float s0 = dot(normal, c2a.v) - front_offset;
-    if (s0 <= total_radius) {
-      m.lp0 = mul_t(x2, c2a.v);
-      m.id0 = flip ? feature_swap(c2a.id) : c2a.id;
-      pc = 1;
-    }
-    const float s1 = dot(normal, c2b.v) - front_offset;
-    if (s1 <= total_radius) {
-      const vec2 lpt = mul_t(x2, c2b.v);
-      const uint32_t id = flip ? feature_swap(c2b.id) : c2b.id;
-      if (pc == 0) { m.lp0 = lpt; m.id0 = id; }
-      else { m.lp1 = lpt; m.id1 = id; }
-      ++pc;
-    }
+  const float s0 = dot(normal, c2a.v) - front_offset;
+  if (s0 <= total_radius) {
+    m.lp0 = mul_t(x2, c2a.v);
+    m.id0 = flip ? feature_swap(c2a.id) : c2a.id;
+    pc = 1;
+  }
+  const float s1 = dot(normal, c2b.v) - front_offset;
+  if (s1 <= total_radius) {
+    const vec2 lpt = mul_t(x2, c2b.v);
+    const uint32_t id = flip ? feature_swap(c2b.id) : c2b.id;
+    if (pc == 0) { m.lp0 = lpt; m.id0 = id; }
+    else { m.lp1 = lpt; m.id1 = id; }
+    ++pc;
   }
   m.count = pc;
+}
+
+// b2CollidePolygons (src/collision/b2_collide_polygon.cpp:120-163)
+__device__ __forceinline__ void collide_polygons(Man& m, const Geo& a, const xform& xa, const Geo& b,
+                                                 const xform& xb) {
+  m.count = 0;
+  const float total_radius = a.radius + b.radius;
+  int edge_a = 0;
+  const float sep_a = find_max_separation(edge_a, a, xa, b, xb);
+  if (sep_a > total_radius) return;
+  int edge_b = 0;
+  const float sep_b = find_max_separation(edge_b, b, xb, a, xa);
+  if (sep_b > total_radius) return;
+  const float k_tol = 0.1f * kLinearSlop;
+  if (sep_b > sep_a + k_tol) {
+    m.type = kManFaceB;
+    clip_polygons(m, b, xb, edge_b, a, xa, total_radius, true);
+  } else {
+    m.type = kManFaceA;
+    clip_polygons(m, a, xa, edge_a, b, xb, total_radius, false);
+  }
 }
 
 // b2CollideEdgeAndCircle (src/collision/b2_collide_edge.cpp:31-165)

@@ -46,6 +46,8 @@ int dalloc(T** p, size_t n) {
   return B2G_OK;
 }
 
+constexpr int kColourRoundsPerStep = 4;
+
 inline int grid_for(long long n, int max_blocks = 4096) {
   long long g = (n + kBlock - 1) / kBlock;
   if (g < 1) g = 1;
@@ -148,7 +150,14 @@ int b2g_batch::alloc() {
   const size_t tiles = (size_t)(NS + kSortTile - 1) / kSortTile;
   A(d.sort_hist, kRadix * tiles);
   A(d.node_child, 2 * (size_t)NS); A(d.node_parent, NS); A(d.leaf_parent, NS); A(d.node_range, NS);
-  A(d.node_aabb, NS); A(d.leaf_aabb, NS); A(d.node_visit, NS);
+  A(d.node_aabb, NS); A(d.node_visit, NS);
+  {
+    int levels = 1;
+    while ((1 << levels) <= NS) ++levels;  // levels 0 .. floor(log2(NS))
+    d.sparse_levels = levels;
+    A(d.sparse, (size_t)levels * NS);
+  }
+  A(d.pslots, (size_t)kPairSlots * NS);
   A(d.pcount, NS); A(d.poff, NS);
   const size_t scan_n = std::max<size_t>(std::max<size_t>(NC, NS), kRadix * tiles);
   A(d.scan_tiles, scan_n / kScanTile + 2);
@@ -175,7 +184,7 @@ void b2g_batch::release() {
                   d.csorted, d.cisland, d.col_count, d.col_off, d.col_cursor, d.bmask, d.bslot,
                   d.parent, d.minsep, d.jok, d.solved, d.mkey[0], d.mkey[1], d.mval[0], d.mval[1],
                   d.sort_hist, d.node_child, d.node_parent, d.leaf_parent, d.node_range, d.node_aabb,
-                  d.leaf_aabb, d.node_visit, d.pcount, d.poff, d.scan_tiles, d.cnt};
+                  d.sparse, d.pslots, d.node_visit, d.pcount, d.poff, d.scan_tiles, d.cnt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int k = 0; k < 10; ++k)
@@ -236,12 +245,22 @@ int b2g_batch::broadphase(bool use_given_aabbs) {
   if (res != 0) return fail(B2G_ERR_INVALID, "radix sort ended in the wrong buffer");
   hipLaunchKernelGGL(k_world_leaves, dim3(gs), dim3(kBlock), 0, stream, d, d.mkey[0]);
   if (ns >= 2) hipLaunchKernelGGL(k_lbvh_build, dim3(grid_for(ns - 1)), dim3(kBlock), 0, stream, d, d.mkey[0]);
-  hipLaunchKernelGGL(k_lbvh_refit, dim3(gs), dim3(kBlock), 0, stream, d, d.mval[0]);
-  hipLaunchKernelGGL(k_lbvh_pairs<false>, dim3(gs), dim3(kBlock), 0, stream, d);
-  scan(d.pcount, nullptr, ns, d.poff, d.cnt + C_PAIRS, ns);
+  hipLaunchKernelGGL(k_leaf_aabbs, dim3(gs), dim3(kBlock), 0, stream, d, d.mval[0]);
+  if (ns >= 2) {
+    int top = 0;
+    while ((2 << top) <= ns) ++top;  // floor(log2(ns))
+    for (int j = 1; j <= top; j += 2) {
+      Dev dj = d;
+      dj.sparse_levels = top + 1;
+      hipLaunchKernelGGL(k_sparse_levels, dim3(gs), dim3(kBlock), 0, stream, dj, j);
+    }
+    hipLaunchKernelGGL(k_node_aabbs, dim3(grid_for(ns - 1)), dim3(kBlock), 0, stream, d, d.mkey[0]);
+  }
   tbegin(B2G_KERNEL_PAIRS_WRITE);
-  hipLaunchKernelGGL(k_lbvh_pairs<true>, dim3(gs), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_lbvh_pairs, dim3(gs), dim3(kBlock), 0, stream, d);
   tend(B2G_KERNEL_PAIRS_WRITE, ns);
+  scan(d.pcount, nullptr, ns, d.poff, d.cnt + C_PAIRS, ns);
+  hipLaunchKernelGGL(k_pairs_compact, dim3(gs), dim3(kBlock), 0, stream, d);
   hipLaunchKernelGGL(k_merge_contacts, dim3(grid_for(std::max(est_contacts, ns) * 5 / 4)), dim3(kBlock), 0, stream,
                      d);
   hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, stream, d);
@@ -321,7 +340,7 @@ int b2g_batch::step(int vi, int pi, unsigned flags) {
   }
   const int nb = d.nb;
   const int gc = grid_for(std::max(est_contacts, 256) * 5 / 4);
-  const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
+  int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
   const int gb = grid_for(nb);
   // ---- Collide
   tbegin(B2G_KERNEL_NARROWPHASE);
@@ -332,38 +351,50 @@ int b2g_batch::step(int vi, int pi, unsigned flags) {
   if (prof_on) HIP_TRY(hipEventRecord(ev[1], stream));
   // ---- Solve
   hipLaunchKernelGGL(k_integrate_velocities, dim3(gb), dim3(kBlock), 0, stream, d);
-  hipLaunchKernelGGL(k_colour_clear_counts, dim3(1), dim3(kBlock), 0, stream, d);
   hipLaunchKernelGGL(k_colour_init, dim3(gk), dim3(kBlock), 0, stream, d);
-  int rc = read_counters();
-  if (rc) return rc;
-  int remain = pinned[C_REMAIN];
-  est_cons = pinned[C_CONS];
-  int rounds = 0;
-  while (remain > 0) {
-    for (int r = 0; r < 2; ++r) {
+  auto colour_rounds = [&](int count) {
+    for (int r = 0; r < count; ++r) {
       ++round;
       if ((round & 0xffff) == 0) {  // priority tag wrapped: clear the claim slots
-        HIP_TRY(hipMemsetAsync(d.bslot, 0, sizeof(unsigned long long) * std::max(nb, 1), stream));
+        (void)hipMemsetAsync(d.bslot, 0, sizeof(unsigned long long) * std::max(nb, 1), stream);
         ++round;
       }
       hipLaunchKernelGGL(k_colour_claim, dim3(gk), dim3(kBlock), 0, stream, d, (int)(round & 0xffff));
       hipLaunchKernelGGL(k_colour_resolve, dim3(gk), dim3(kBlock), 0, stream, d, (int)(round & 0xffff));
     }
-    rounds += 2;
-    if ((rc = read_counters())) return rc;
-    remain = pinned[C_REMAIN];
-    if (rounds > 4096) return fail(B2G_ERR_CAPACITY, "colouring did not converge");
-  }
-  hipLaunchKernelGGL(k_colour_count, dim3(gk), dim3(kBlock), 0, stream, d);
-  hipLaunchKernelGGL(k_colour_offsets, dim3(1), dim3(1), 0, stream, d);
-  hipLaunchKernelGGL(k_colour_scatter, dim3(gk), dim3(kBlock), 0, stream, d);
-  hipLaunchKernelGGL(k_sort_serial_group, dim3(1), dim3(1), 0, stream, d);
+  };
+  auto colour_groups = [&]() {
+    hipLaunchKernelGGL(k_colour_count, dim3(gk), dim3(kBlock), 0, stream, d);
+    hipLaunchKernelGGL(k_colour_offsets, dim3(1), dim3(1), 0, stream, d);
+    hipLaunchKernelGGL(k_colour_scatter, dim3(gk), dim3(kBlock), 0, stream, d);
+    hipLaunchKernelGGL(k_sort_serial_group, dim3(1), dim3(1), 0, stream, d);
+  };
+  // Contacts keep their colour while they stay touching, so a few rounds
+  // colour the newcomers of a step; the kernels return at once when nothing
+  // is left, and the one read-back below checks it.
+  colour_rounds(kColourRoundsPerStep);
+  colour_groups();
   hipLaunchKernelGGL(k_island_link, dim3(grid_for(std::max(est_cons, 1) + d.nj)), dim3(kBlock), 0, stream, d);
   hipLaunchKernelGGL(k_island_label, dim3(gb), dim3(kBlock), 0, stream, d);
   hipLaunchKernelGGL(k_contact_init, dim3(gk), dim3(kBlock), 0, stream, d);
   if (d.nj) hipLaunchKernelGGL(k_joint_prepare, dim3(grid_for(d.nj)), dim3(kBlock), 0, stream, d);
   HIP_TRY(hipMemcpyAsync(pinned + 32, d.col_off, (kColourSlots + 1) * sizeof(int), hipMemcpyDeviceToHost, stream));
+  int rc;
   if ((rc = read_counters())) return rc;
+  est_cons = pinned[C_CONS];
+  for (int extra = 0; pinned[C_REMAIN] > 0; extra += 2) {  // first steps / bursts of new contacts
+    if (extra > 4096) return fail(B2G_ERR_CAPACITY, "colouring did not converge");
+    colour_rounds(2);
+    if ((rc = read_counters())) return rc;
+    if (pinned[C_REMAIN] == 0) {
+      hipLaunchKernelGGL(k_colour_clear_counts, dim3(1), dim3(kBlock), 0, stream, d);
+      colour_groups();
+      hipLaunchKernelGGL(k_contact_init, dim3(grid_for(std::max(est_cons, 256) * 5 / 4)), dim3(kBlock), 0, stream, d);
+      HIP_TRY(hipMemcpyAsync(pinned + 32, d.col_off, (kColourSlots + 1) * sizeof(int), hipMemcpyDeviceToHost,
+                             stream));
+      if ((rc = read_counters())) return rc;
+    }
+  }
   const int* off = pinned + 32;
   const int ncol = pinned[C_NCOLOURS];
   last_ncolours = ncol;
@@ -392,8 +423,7 @@ int b2g_batch::step(int vi, int pi, unsigned flags) {
     joint_sweep(k_joint_velocity);
     contact_sweep(B2G_KERNEL_CONTACT_VELOCITY, k_contact_velocity);
   }
-  hipLaunchKernelGGL(k_store_impulses, dim3(gk), dim3(kBlock), 0, stream, d);
-  hipLaunchKernelGGL(k_integrate_positions, dim3(gb), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_store_and_integrate, dim3(grid_for(std::max(est_cons, nb))), dim3(kBlock), 0, stream, d);
   const int gf = grid_for(std::max(est_cons, 1) + d.nj);
   for (int it = 0; it < pi; ++it) {
     contact_sweep(B2G_KERNEL_CONTACT_POSITION, k_contact_position, it);

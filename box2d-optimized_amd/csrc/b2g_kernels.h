@@ -107,7 +107,9 @@ struct Dev {
   int* leaf_parent;
   int2* node_range;
   float4* node_aabb;
-  float4* leaf_aabb;
+  float4* sparse;    // (levels) x ns AABB table over sorted leaves; level 0 = leaf AABBs
+  int sparse_levels;
+  uint64_t* pslots;  // kPairSlots per shape: pairs found by the traversal, sorted
   int* node_visit;
   int* wleaf;        // 2 per world: first leaf, last leaf + 1
   unsigned* wbounds; // 4 per world (ordered floats): min x, min y, max x, max y
@@ -254,6 +256,10 @@ __global__ void __launch_bounds__(kBlock) k_compact_touching(Dev d) {
 // Velocity integration (b2_island.cpp:257-298, SolveOrphan :187-206)
 // ===========================================================================
 __global__ void __launch_bounds__(kBlock) k_integrate_velocities(Dev d) {
+  if (blockIdx.x == 0) {  // colour counters of this step
+    for (int c = threadIdx.x; c < kColourSlots; c += blockDim.x) d.col_count[c] = 0;
+    if (threadIdx.x == 0) d.cnt[C_REMAIN] = 0;
+  }
   B2G_FOR(b, 0, d.nb) {
     d.parent[b] = b;
     d.bmask[b] = 0ull;
@@ -356,6 +362,7 @@ __global__ void __launch_bounds__(kBlock) k_colour_count(Dev d) {
   const int n = d.cnt[C_CONS];
   B2G_FOR(k, 0, n) {
     const int col = d.ccolour[k];
+    if (col < 0) continue;  // still uncoloured: the host runs more rounds
     atomicAdd(&hist[col], 1);
     d.ccol[d.clist[k]] = col;
   }
@@ -418,7 +425,6 @@ __global__ void k_sort_serial_group(Dev d) {
 
 __global__ void __launch_bounds__(kBlock) k_colour_clear_counts(Dev d) {
   for (int c = threadIdx.x; c < kColourSlots; c += blockDim.x) d.col_count[c] = 0;
-  if (threadIdx.x == 0) d.cnt[C_REMAIN] = 0;
 }
 
 // ===========================================================================
@@ -770,9 +776,8 @@ __global__ void __launch_bounds__(kBlock) k_contact_velocity(Dev d, int begin, i
 }
 
 // b2ContactSolver::StoreImpulses (:641-657): only the vc pointCount points
-__global__ void __launch_bounds__(kBlock) k_store_impulses(Dev d) {
-  const int n = d.cnt[C_CONS];
-  B2G_FOR(s, 0, n) {
+__device__ __forceinline__ void store_impulses(const Dev& d, int s) {
+  {
     const float4 v2i = d.vc[2][s];
     const int count = as_int(v2i.z), c = as_int(v2i.w);
     const float4 q0 = d.vc[4][s];
@@ -795,10 +800,10 @@ __global__ void __launch_bounds__(kBlock) k_store_impulses(Dev d) {
 // Position integration with the translation / rotation clamp
 // (src/dynamics/b2_island.cpp:341-385)
 // ===========================================================================
-__global__ void __launch_bounds__(kBlock) k_integrate_positions(Dev d) {
-  B2G_FOR(b, 0, d.nb) {
+__device__ __forceinline__ void integrate_position(const Dev& d, int b) {
+  {
     const int w = d.bworld[b];
-    if (!world_active(d, w) || d.btype[b] == kStaticBody) continue;
+    if (!world_active(d, w) || d.btype[b] == kStaticBody) return;
     const float h = d.wpar[w].z;
     const float4 v4 = d.vel[b];
     vec2 v = mk(v4.x, v4.y);
@@ -818,6 +823,17 @@ __global__ void __launch_bounds__(kBlock) k_integrate_positions(Dev d) {
     const float a = p.z + h * wv;
     d.pos[b] = make_float4(c.x, c.y, a, 0.0f);
     d.vel[b] = make_float4(v.x, v.y, wv, 0.0f);
+  }
+}
+
+// StoreImpulses for every solved contact and position integration for every
+// body, in one launch (the two touch disjoint data)
+__global__ void __launch_bounds__(kBlock) k_store_and_integrate(Dev d) {
+  const int n = d.cnt[C_CONS];
+  const int m = n > d.nb ? n : d.nb;
+  B2G_FOR(k, 0, m) {
+    if (k < n) store_impulses(d, k);
+    if (k < d.nb) integrate_position(d, k);
   }
 }
 
@@ -1267,7 +1283,6 @@ __global__ void __launch_bounds__(kBlock) k_lbvh_build(Dev d, const uint64_t* __
     d.node_range[i] = make_int2(lo, hi);
     if (left < 0) d.leaf_parent[~left] = i; else d.node_parent[left] = i;
     if (right < 0) d.leaf_parent[~right] = i; else d.node_parent[right] = i;
-    d.node_visit[i] = 0;
     if (i == 0) d.node_parent[0] = -1;
   }
 }
@@ -1282,29 +1297,55 @@ __global__ void __launch_bounds__(kBlock) k_world_leaves(Dev d, const uint64_t* 
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_lbvh_refit(Dev d, const uint32_t* __restrict__ vals) {
+// Node bounds without a bottom-up walk: every internal node covers a
+// contiguous range of sorted leaves, so its AABB is a range union, answered in
+// O(1) from a sparse table (level j holds the union of 2^j leaves starting at
+// each position). Building the table is log2(n) element-wise passes (two per
+// launch); no atomics, fences or inter-block ordering are involved.
+__device__ __forceinline__ float4 aabb_union(float4 a, float4 b) {
+  return make_float4(minf(a.x, b.x), minf(a.y, b.y), maxf(a.z, b.z), maxf(a.w, b.w));
+}
+
+__global__ void __launch_bounds__(kBlock) k_leaf_aabbs(Dev d, const uint32_t* __restrict__ vals) {
+  B2G_FOR(k, 0, d.ns) d.sparse[k] = d.aabb[vals[k]];
+}
+
+// levels j and j + 1 from level j - 1
+__global__ void __launch_bounds__(kBlock) k_sparse_levels(Dev d, int j) {
   const int n = d.ns;
-  B2G_FOR(k, 0, n) {
-    const float4 bb = d.aabb[vals[k]];
-    d.leaf_aabb[k] = bb;
-    if (n < 2) continue;
-    __threadfence();
-    int node = d.leaf_parent[k];
-    int guard = 0;
-    while (node >= 0) {
-      if (++guard > 256 || node >= n - 1) {  // deeper than any tree over 64-bit keys
-        atomicOr(&d.cnt[C_STATUS], S_TREE_CORRUPT);
-        break;
-      }
-      if (atomicAdd(&d.node_visit[node], 1) == 0) break;  // first child to arrive stops
-      __threadfence();
-      const int l = d.node_child[2 * node], r = d.node_child[2 * node + 1];
-      const float4 a = l < 0 ? d.leaf_aabb[~l] : d.node_aabb[l];
-      const float4 b = r < 0 ? d.leaf_aabb[~r] : d.node_aabb[r];
-      d.node_aabb[node] = make_float4(minf(a.x, b.x), minf(a.y, b.y), maxf(a.z, b.z), maxf(a.w, b.w));
-      __threadfence();
-      node = d.node_parent[node];
+  const int h = 1 << (j - 1);
+  const float4* prev = d.sparse + (size_t)(j - 1) * n;
+  float4* cur = d.sparse + (size_t)j * n;
+  float4* nxt = d.sparse + (size_t)(j + 1) * n;
+  const bool two = j + 1 < d.sparse_levels;
+  B2G_FOR(i, 0, n) {
+    const float4 a = prev[i];
+    const float4 b = prev[min(i + h, n - 1)];
+    const float4 u = aabb_union(a, b);
+    cur[i] = u;
+    if (two) {
+      const float4 c = prev[min(i + 2 * h, n - 1)];
+      const float4 e = prev[min(i + 3 * h, n - 1)];
+      nxt[i] = aabb_union(u, aabb_union(c, e));
     }
+  }
+}
+
+// AABB of every internal node from its leaf range; nodes that span several
+// worlds get an unbounded box (the traversal's world-range test skips them
+// for every query outside their worlds).
+__global__ void __launch_bounds__(kBlock) k_node_aabbs(Dev d, const uint64_t* __restrict__ keys) {
+  const int n = d.ns;
+  B2G_FOR(i, 0, n - 1) {
+    const int2 r = d.node_range[i];
+    if ((keys[r.x] >> 32) != (keys[r.y] >> 32)) {
+      d.node_aabb[i] = make_float4(-FLT_MAX, -FLT_MAX, FLT_MAX, FLT_MAX);
+      continue;
+    }
+    const int len = r.y - r.x + 1;
+    const int lev = 31 - __clz(len);
+    const float4* t = d.sparse + (size_t)lev * n;
+    d.node_aabb[i] = aabb_union(t[r.x], t[r.y - (1 << lev) + 1]);
   }
 }
 
@@ -1331,71 +1372,81 @@ __device__ __forceinline__ bool pair_allowed(const Dev& d, int i, int j) {
 }
 
 constexpr int kStack = 64;
+constexpr int kPairSlots = 16;
 
-// write == false: count pairs (i, j > i) per shape; write == true: write them
-// at poff[i], sorted by j.
-template <bool kWrite>
-__global__ void __launch_bounds__(kBlock) k_lbvh_pairs(Dev d) {
+// Traversal for shape i: pairs (i, j > i) of overlapping, collidable shapes.
+// Found keys are kept sorted in i's slots (kPairSlots); the count can exceed
+// the slots, in which case k_pairs_compact traverses again for that shape.
+template <bool kDirect>
+__device__ __forceinline__ int lbvh_query(const Dev& d, int i, uint64_t* out, int cap) {
   const int n = d.ns;
-  B2G_FOR(i, 0, n) {
-    const float4 q = d.aabb[i];
-    const int w = d.shdr[i].w;
-    const int wlo = d.wleaf[2 * w], whi = d.wleaf[2 * w + 1] - 1;
-    int found = 0;
-    int base = 0, cap = 0;
-    if (kWrite) {
-      base = d.poff[i];
-      cap = d.ccap;
+  if (n < 2) return 0;
+  const float4 q = d.aabb[i];
+  const int w = d.shdr[i].w;
+  const int wlo = d.wleaf[2 * w], whi = d.wleaf[2 * w + 1] - 1;
+  int found = 0;
+  int stack[kStack];
+  int sp = 0;
+  stack[sp++] = 0;
+  int visits = 0;
+  while (sp > 0) {
+    const int node = stack[--sp];
+    if (++visits > 2 * n) {  // a tree has n - 1 internal nodes
+      atomicOr(&d.cnt[C_STATUS], S_TREE_CORRUPT);
+      break;
     }
-    if (n >= 2) {
-      int stack[kStack];
-      int sp = 0;
-      stack[sp++] = 0;
-      int visits = 0;
-      while (sp > 0) {
-        const int node = stack[--sp];
-        if (++visits > 2 * n) {  // a tree has n - 1 internal nodes
-          atomicOr(&d.cnt[C_STATUS], S_TREE_CORRUPT);
-          break;
-        }
 #pragma unroll
-        for (int side = 0; side < 2; ++side) {
-          const int ch = d.node_child[2 * node + side];
-          if (ch < 0) {
-            const int k = ~ch;
-            if (k < wlo || k > whi) continue;
-            if (!overlap(q, d.leaf_aabb[k])) continue;
-            const int j = (int)d.mval[0][k];
-            if (j <= i || !pair_allowed(d, i, j)) continue;
-            if (kWrite) {
-              const int slot = base + found;
-              if (slot < cap) d.nkey[slot] = ((uint64_t)(uint32_t)i << 32) | (uint32_t)j;
-            }
-            ++found;
-          } else {
-            const int2 r = d.node_range[ch];
-            if (r.y < wlo || r.x > whi) continue;
-            if (!overlap(q, d.node_aabb[ch])) continue;
-            if (sp < kStack) stack[sp++] = ch;
-            else atomicOr(&d.cnt[C_STATUS], S_STACK_OVERFLOW);
+    for (int side = 0; side < 2; ++side) {
+      const int ch = d.node_child[2 * node + side];
+      if (ch < 0) {
+        const int k = ~ch;
+        if (k < wlo || k > whi) continue;
+        if (!overlap(q, d.sparse[k])) continue;
+        const int j = (int)d.mval[0][k];
+        if (j <= i || !pair_allowed(d, i, j)) continue;
+        const uint64_t key = ((uint64_t)(uint32_t)i << 32) | (uint32_t)j;
+        if (found < cap) {
+          // sorted insert
+          int b = found - 1;
+          while (b >= 0 && out[b] > key) {
+            out[b + 1] = out[b];
+            --b;
           }
+          out[b + 1] = key;
         }
+        ++found;
+      } else {
+        const int2 r = d.node_range[ch];
+        if (r.y < wlo || r.x > whi) continue;
+        if (!overlap(q, d.node_aabb[ch])) continue;
+        if (sp < kStack) stack[sp++] = ch;
+        else atomicOr(&d.cnt[C_STATUS], S_STACK_OVERFLOW);
       }
     }
-    if (!kWrite) {
-      d.pcount[i] = found;
+  }
+  return found;
+}
+
+__global__ void __launch_bounds__(kBlock) k_lbvh_pairs(Dev d) {
+  B2G_FOR(i, 0, d.ns) {
+    d.pcount[i] = lbvh_query<false>(d, i, d.pslots + (size_t)i * kPairSlots, kPairSlots);
+  }
+}
+
+// copy each shape's sorted pairs to its place in the pair list (poff); a
+// shape with more pairs than slots is traversed again straight into place
+__global__ void __launch_bounds__(kBlock) k_pairs_compact(Dev d) {
+  B2G_FOR(i, 0, d.ns) {
+    const int cnt = d.pcount[i];
+    const int base = d.poff[i];
+    if (cnt <= kPairSlots) {
+      const uint64_t* src = d.pslots + (size_t)i * kPairSlots;
+      for (int k = 0; k < cnt; ++k)
+        if (base + k < d.ccap) d.nkey[base + k] = src[k];
+    } else if (base + cnt <= d.ccap) {
+      lbvh_query<true>(d, i, d.nkey + base, cnt);
     } else {
-      // insertion sort of this shape's pairs by key
-      const int end = min(base + found, cap);
-      for (int a = base + 1; a < end; ++a) {
-        const uint64_t key = d.nkey[a];
-        int b = a - 1;
-        while (b >= base && d.nkey[b] > key) {
-          d.nkey[b + 1] = d.nkey[b];
-          --b;
-        }
-        d.nkey[b + 1] = key;
-      }
+      atomicOr(&d.cnt[C_STATUS], S_CONTACT_OVERFLOW);
     }
   }
 }

@@ -67,7 +67,11 @@ struct rot2 {
   float s, c;
 };
 
-B2G_INLINE rot2 rot_of(float angle) { return rot2{sinf(angle), cosf(angle)}; }
+__device__ __forceinline__ rot2 rot_of(float angle) {
+  rot2 r;
+  sincosf(angle, &r.s, &r.c);
+  return r;
+}
 B2G_INLINE vec2 mul(rot2 q, vec2 v) { return vec2{q.c * v.x - q.s * v.y, q.s * v.x + q.c * v.y}; }
 B2G_INLINE vec2 mul_t(rot2 q, vec2 v) { return vec2{q.c * v.x + q.s * v.y, -q.s * v.x + q.c * v.y}; }
 B2G_INLINE rot2 mul_t(rot2 q, rot2 r) { return rot2{q.c * r.s - q.s * r.c, q.c * r.c + q.s * r.s}; }

@@ -219,8 +219,9 @@ def test_gpu_pyramid_1000_steps_stable(engine):
     """C1 pyramid over 1000 steps, every 25 steps: the deepest penetration
     (world-manifold separation) stays within 2 linear slop of the
     reference's own at the same step, kinetic energy stays within 5% of the
-    reference's while the pyramid moves (KE > 1 J), and the final resting
-    positions agree to 5 cm."""
+    reference's while the pyramid moves (KE above 5% of its peak) and below
+    1% of the peak once it rests, and the final resting positions agree to
+    5 cm."""
     sc = S.pyramid(20)
     tr = golden("trajectory_pyramid.npz")
     states, minsep_ref, ke_ref = tr["states"], tr["minsep"], tr["ke"]
@@ -241,8 +242,10 @@ def test_gpu_pyramid_1000_steps_stable(engine):
         print(f"step {25 * (k + 1)}: min sep gpu {min(seps):.4f} ref {minsep_ref[k]:.4f}; "
               f"KE gpu {ke:.4g} ref {ke_ref[k]:.4g}", flush=True)
         assert min(seps) >= minsep_ref[k] - 2 * 0.005
-        if ke_ref[k] > 1.0:
+        if ke_ref[k] > 0.05 * ke_ref.max():
             assert abs(ke - ke_ref[k]) <= 0.05 * ke_ref[k]
+        else:  # at rest: both residuals stay below 1% of the peak
+            assert ke <= 0.01 * ke_ref.max()
     d = float(np.abs(w.bodies()[1:, 0:2] - states[-1][1:, 0:2]).max())
     print("pyramid final max |dc| vs reference:", d, flush=True)
     assert d < 0.05
