@@ -41,3 +41,23 @@ def test_errors_without_device_are_reported_not_faked():
     import pytest
     with pytest.raises(eng.B2GError):
         eng.Batch()
+
+
+def _example():
+    return os.path.join(REPO, "box2d-optimized_amd", "build", "falling_box")
+
+
+def test_drop_in_example_links_and_fails_loudly_without_device():
+    """User code written against the b2World API links against the library;
+    without a device it reports that, rather than stepping on the CPU."""
+    import subprocess
+    import pytest
+    exe = _example()
+    if not os.path.exists(exe):
+        pytest.skip("example not built (run __graft_entry__.build())")
+    eng = load_pkg().engine
+    if eng.device_count() > 0:
+        pytest.skip("device present: covered by the GPU test")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2, r.stdout + r.stderr
+    assert "device" in r.stdout

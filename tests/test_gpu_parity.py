@@ -16,11 +16,13 @@ Tolerances:
     reference's order) run in key/reverse/random order measures how far a
     different Gauss-Seidel order moves one step.
 """
+import os
+
 import numpy as np
 import pytest
 
 import oracle
-from util import bits, golden, load_pkg, manifold_fields_equal, scene_for
+from util import REPO, bits, golden, load_pkg, manifold_fields_equal, scene_for
 
 pytestmark = pytest.mark.gpu
 S = load_pkg().scenes
@@ -249,3 +251,18 @@ def test_gpu_pyramid_1000_steps_stable(engine):
     d = float(np.abs(w.bodies()[1:, 0:2] - states[-1][1:, 0:2]).max())
     print("pyramid final max |dc| vs reference:", d, flush=True)
     assert d < 0.05
+
+
+@pytest.mark.gpu
+def test_gpu_drop_in_cpp_example():
+    """A C++ program written against the b2World API (examples/falling_box.cpp),
+    linked against libbox2d_amd.so: the box rests at y = 1.01 +- 0.01 after 60
+    steps and BeginContact fires once (unit-test/hello_world.cpp:109-111)."""
+    import subprocess
+    exe = os.path.join(REPO, "box2d-optimized_amd", "build", "falling_box")
+    assert os.path.exists(exe), "example not built"
+    print("running", exe, flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    x, y, a, begins, contacts = r.stdout.split()
+    assert abs(float(y) - 1.01) < 0.01 and int(begins) == 1 and int(contacts) == 1

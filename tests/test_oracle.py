@@ -272,3 +272,112 @@ def test_restatement_matches_reference_live_when_built():
     ob = w.bodies()
     # the first steps have no contact-order ambiguity between the two
     assert np.abs(rb[:, 0:2] - ob["c"]).max() < 1e-4
+
+
+# ---- closed forms for the collide functions --------------------------------
+# Each case has a manifold fixed by geometry alone; the restatement's world
+# manifold (b2WorldManifold::Initialize, b2_collision.cpp:26-90) must match
+# it to float32 accuracy. polygonRadius (b2_common.h) is the skin of polygons
+# and edges.
+POLY_R = 0.01
+
+
+def _shapes(build):
+    s = S.Scene(name="closed")
+    g = s.add_body(S.STATIC, 0.0, 0.0)
+    b = s.add_body(S.DYNAMIC, 0.0, 0.0)
+    build(s, g, b)
+    return oracle.RestatedWorld(s).shapes()
+
+
+def _xf(x, y, a=0.0):
+    return np.array([x, y, math.sin(a), math.cos(a)], np.float32)
+
+
+def _world(sa, xa, sb, xb):
+    c = oracle.orc_collide(sa, xa, sb, xb)
+    n = int(c["point_count"])
+    wm = oracle.orc_world_manifold(c, xa, sa["radius"], xb, sb["radius"])
+    return n, wm[0:2], wm[2:6].reshape(2, 2)[:n], wm[6:8][:n]
+
+
+def test_circle_circle_closed_form():
+    sh = _shapes(lambda s, g, b: (s.add_circle(g, 0.5), s.add_circle(b, 0.3)))
+    ca, cb = np.array([0.1, -0.2]), np.array([0.75, 0.15])
+    n, nrm, pts, sep = _world(sh[0], _xf(*ca), sh[1], _xf(*cb))
+    d = cb - ca
+    L = np.linalg.norm(d)
+    u = d / L
+    assert n == 1
+    assert np.allclose(nrm, u, atol=1e-6)
+    assert abs(sep[0] - (L - 0.8)) < 2e-6
+    mid = 0.5 * ((ca + 0.5 * u) + (cb - 0.3 * u))
+    assert np.allclose(pts[0], mid, atol=2e-6)
+    # separated by more than the radii: no point
+    n, *_ = _world(sh[0], _xf(0, 0), sh[1], _xf(0.81, 0))
+    assert n == 0
+
+
+def test_polygon_circle_face_and_vertex_regions():
+    sh = _shapes(lambda s, g, b: (s.add_box(g, 0.5, 0.5), s.add_circle(b, 0.25), s.add_circle(b, 0.3)))
+    box, c25, c30 = sh[0], sh[1], sh[2]
+    # face region of the top face
+    n, nrm, pts, sep = _world(box, _xf(0, 0), c25, _xf(0.2, 0.7))
+    assert n == 1 and np.allclose(nrm, [0, 1], atol=1e-7)
+    assert abs(sep[0] - (0.7 - 0.5 - 0.25 - POLY_R)) < 2e-6
+    # world point: midway between the polygon's skin and the circle's surface
+    assert np.allclose(pts[0], [0.2, 0.5 * ((0.5 + POLY_R) + (0.7 - 0.25))], atol=2e-6)
+    # vertex region of the corner (0.5, 0.5)
+    cc = np.array([0.7, 0.72])
+    n, nrm, pts, sep = _world(box, _xf(0, 0), c30, _xf(*cc))
+    d = cc - np.array([0.5, 0.5])
+    L = np.linalg.norm(d)
+    assert n == 1 and np.allclose(nrm, d / L, atol=1e-6)
+    assert abs(sep[0] - (L - 0.3 - POLY_R)) < 2e-6
+    # rotating the box by 90 degrees about its centre changes nothing
+    n2, nrm2, _, sep2 = _world(box, _xf(0, 0, math.pi / 2), c30, _xf(*cc))
+    assert n2 == 1 and np.allclose(nrm2, nrm, atol=1e-6) and abs(sep2[0] - sep[0]) < 2e-6
+
+
+def test_box_on_box_axis_aligned_closed_form():
+    sh = _shapes(lambda s, g, b: (s.add_box(g, 0.5, 0.5), s.add_box(b, 0.5, 0.5)))
+    pen = 0.02
+    xb = 0.3
+    n, nrm, pts, sep = _world(sh[0], _xf(0, 0), sh[1], _xf(xb, 1.0 - pen))
+    assert n == 2
+    assert np.allclose(nrm, [0, 1], atol=1e-7)
+    assert np.allclose(sep, -pen - 2 * POLY_R, atol=2e-6)
+    # reference face = A's top face (ties go to A), incident = B's bottom
+    # edge; its corner x = xb - 0.5 lies inside A's side planes, the other end
+    # is clipped at the side plane, pushed out by the total skin radius
+    # (b2_collide_polygon.cpp side offsets + totalRadius)
+    assert np.allclose(sorted(pts[:, 0]), [xb - 0.5, 0.5 + 2 * POLY_R], atol=2e-6)
+    assert np.allclose(pts[:, 1], 0.5 - pen / 2, atol=2e-6)
+
+
+def test_box_on_edge_ground_closed_form():
+    sh = _shapes(lambda s, g, b: (s.add_edge(g, (-5.0, 0.0), (5.0, 0.0)), s.add_box(b, 0.5, 0.5)))
+    edge, box = sh[0], sh[1]
+    gap = -0.01                                  # box bottom 1 cm below the edge line
+    n, nrm, pts, sep = _world(edge, _xf(0, 0), box, _xf(0.3, 0.5 + gap))
+    assert n == 2
+    assert np.allclose(nrm, [0, 1], atol=1e-7)
+    assert np.allclose(sep, gap - 2 * POLY_R, atol=2e-6)
+    assert np.allclose(sorted(pts[:, 0]), [-0.2, 0.8], atol=2e-6)
+    assert np.allclose(pts[:, 1], 0.5 * ((0.0 + POLY_R) + (gap - POLY_R)), atol=2e-6)
+
+
+def test_edge_circle_regions_closed_form():
+    sh = _shapes(lambda s, g, b: (s.add_edge(g, (-1.0, 0.0), (1.0, 0.0)), s.add_circle(b, 0.5)))
+    edge, circ = sh[0], sh[1]
+    # region AB: the segment interior
+    n, nrm, pts, sep = _world(edge, _xf(0, 0), circ, _xf(0.3, 0.4))
+    assert n == 1 and np.allclose(nrm, [0, 1], atol=1e-7)
+    assert abs(sep[0] - (0.4 - 0.5 - POLY_R)) < 2e-6
+    # region A: beyond vertex (-1, 0)
+    cc = np.array([-1.3, 0.3])
+    n, nrm, pts, sep = _world(edge, _xf(0, 0), circ, _xf(*cc))
+    d = cc - np.array([-1.0, 0.0])
+    L = np.linalg.norm(d)
+    assert n == 1 and np.allclose(nrm, d / L, atol=1e-6)
+    assert abs(sep[0] - (L - 0.5 - POLY_R)) < 2e-6
