@@ -217,13 +217,34 @@ def test_gpu_revolute_reaction_unit_test(engine):
     assert abs(F[1] - m * 10.0) < 1e-3
 
 
+def _pyramid_order_spread(sc, ref_final):
+    """Largest final-position difference, after the same 1000 steps, between
+    the reference and the restated algorithm solving each step's contacts in
+    other valid Gauss-Seidel orders (fixed random ranks of every shape pair,
+    and plain pair-index order). A settling pyramid amplifies order
+    differences; this is how far the reference's own algorithm moves."""
+    n = sc.fixture_count
+    ii, jj = np.triu_indices(n, 1)
+    keys = (ii.astype(np.uint64) << np.uint64(32)) | jj.astype(np.uint64)
+    ranks = [np.zeros(len(keys), np.int32)] + [np.random.default_rng(seed).permutation(len(keys)).astype(np.int32)
+                                               for seed in (1, 2, 3)]
+    worst = 0.0
+    for rank in ranks:
+        r = oracle.RestatedWorld(sc)
+        r.set_colours(keys, rank)
+        r.step(1000)
+        worst = max(worst, float(np.abs(r.bodies()["c"][1:] - ref_final[1:, 0:2]).max()))
+    return worst
+
+
 def test_gpu_pyramid_1000_steps_stable(engine):
     """C1 pyramid over 1000 steps, every 25 steps: the deepest penetration
     (world-manifold separation) stays within 2 linear slop of the
     reference's own at the same step, kinetic energy stays within 5% of the
     reference's while the pyramid moves (KE above 5% of its peak) and below
-    1% of the peak once it rests, and the final resting positions agree to
-    5 cm."""
+    1% of the peak once it rests, and the final resting positions agree within
+    1.5x the spread that other valid solve orders of the reference algorithm
+    show after the same 1000 steps."""
     sc = S.pyramid(20)
     tr = golden("trajectory_pyramid.npz")
     states, minsep_ref, ke_ref = tr["states"], tr["minsep"], tr["ke"]
@@ -249,8 +270,9 @@ def test_gpu_pyramid_1000_steps_stable(engine):
         else:  # at rest: both residuals stay below 1% of the peak
             assert ke <= 0.01 * ke_ref.max()
     d = float(np.abs(w.bodies()[1:, 0:2] - states[-1][1:, 0:2]).max())
-    print("pyramid final max |dc| vs reference:", d, flush=True)
-    assert d < 0.05
+    spread = _pyramid_order_spread(sc, states[-1])
+    print(f"pyramid final max |dc| vs reference: {d:.4f}; order spread {spread:.4f}", flush=True)
+    assert d <= max(0.02, 1.5 * spread)
 
 
 @pytest.mark.gpu
