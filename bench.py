@@ -105,6 +105,112 @@ def cpu_baseline(scene, warmup: int, steps: int):
                       + (f", p99 {p99:.2f} ms" if p99 is not None else "")}
 
 
+def batched_section(pkg, worlds: int, warmup: int, steps: int, sample_every: int, cpu: bool, mode: int):
+    """Batched pattern: `worlds` independent worlds (lognormal sizes, median
+    200 bodies, pyramids and random piles) stepped by one set of launches per
+    stage. Compared with a plain loop of single-world GPU steps (one world per
+    batch, stepped one after another with a host sync per step, as a caller
+    looping over b2World::Step would) on a sample of the same worlds, and with
+    the reference CPU Step on the same sample."""
+    import numpy as np
+    S, eng = pkg.scenes, pkg.engine
+    sizes = S.batched_sizes(worlds)
+    t0 = time.perf_counter()
+    scenes = [S.batch_world(i, int(sizes[i])) for i in range(worlds)]
+    batch = eng.Batch.from_scenes(scenes)
+    build_s = time.perf_counter() - t0
+    batch.advance(max(warmup - 2, 1))
+    for _ in range(2):  # capture the step graph of both contact-buffer parities
+        batch.step(8, 3, mode & ~eng.STEP_ASYNC)
+    batch.wait()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.step(8, 3, mode)
+    batch.wait()
+    el = time.perf_counter() - t0
+    counts = batch.counts()
+    nbodies = int(sum(s.dynamic_count for s in scenes))
+    sample = list(range(0, worlds, sample_every))
+    # plain per-world loop on the sample
+    loop_steps = max(1, min(steps, 20))
+    loop_time = 0.0
+    for i in sample:
+        b1 = eng.Batch.from_scenes([scenes[i]]).advance(max(warmup, 1))
+        b1.wait()
+        t1 = time.perf_counter()
+        b1.advance(loop_steps)
+        b1.wait()
+        loop_time += time.perf_counter() - t1
+        b1.close()
+    loop_rate = len(sample) * loop_steps / loop_time
+    out = {
+        "metric": "batched world-steps/s",
+        "value": worlds * steps / el,
+        "unit": "world-steps/s",
+        "body_steps_per_s": nbodies * steps / el,
+        "ms_per_step": el / steps * 1e3,
+        "worlds": worlds, "dynamic_bodies": nbodies, "steps": steps, "warmup": warmup,
+        "contacts": counts.contacts, "touching": counts.touching, "colours": counts.colours,
+        "build_s": build_s,
+        "per_world_loop": {"value": loop_rate, "unit": "world-steps/s",
+                           "sample": f"{len(sample)} worlds (every {sample_every}th), {loop_steps} steps each, "
+                                     f"one batch per world, host sync per step"},
+        "speedup_vs_per_world_loop": (worlds * steps / el) / loop_rate,
+    }
+    if cpu:
+        import oracle
+        ref_steps = loop_steps
+        t_cpu = 0.0
+        kind = "reference" if oracle.Ref.available() else "port"
+        for i in sample:
+            w = oracle.RefWorld(scenes[i]) if kind == "reference" else oracle.RestatedWorld(scenes[i])
+            w.step(max(warmup, 1))
+            t1 = time.perf_counter()
+            w.step(ref_steps)
+            t_cpu += time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": len(sample) * ref_steps / t_cpu, "unit": "world-steps/s", "cores": 1,
+                               "kind": kind, "sample": f"the same {len(sample)} worlds, {ref_steps} steps each, "
+                                                       f"one thread, {t_cpu:.1f} s"}
+    batch.close()
+    return out
+
+
+def serving_section(pkg, worlds: int, rate: float, duration: float, budget_gb: float, graph: bool):
+    """Serving pattern: Poisson world-step requests (1-8 substeps at dt 1/60 or
+    1/240) against a resident pool of worlds with log-uniform sizes 16-4096,
+    run by the native continuous-batching loop (b2s_run) on the wall clock.
+    Latency = completion - arrival; throughput = body-steps completed / wall."""
+    import numpy as np
+    S, eng = pkg.scenes, pkg.engine
+    sizes = S.serving_sizes(worlds, seed=0xC5)
+    t0 = time.perf_counter()
+    scenes = [S.batch_world(50000 + i, int(sizes[i]), seed_base=0xC5) for i in range(worlds)]
+    pool = eng.Batch.from_scenes(scenes).advance(1)
+    pool.wait()
+    build_s = time.perf_counter() - t0
+    wbytes = np.array([eng.world_device_bytes(s) for s in scenes], np.int64)
+    trace = S.serving_trace(worlds, rate=rate, duration=duration, seed=0xC5)
+    res, st, _ = eng.serve(pool, trace, int(budget_gb * (1 << 30)), wbytes, graph=graph, realtime=True)
+    lat = (res["completed"] - res["arrival"]) * 1e3
+    queue = (res["admitted"] - res["arrival"]) * 1e3
+    out = {
+        "metric": "serving p99 request latency", "value": float(np.percentile(lat, 99)), "unit": "ms",
+        "higher_is_better": False,
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "p50_queue_ms": float(np.percentile(queue, 50)), "p99_queue_ms": float(np.percentile(queue, 99)),
+        "body_steps_per_s": st.body_steps / st.wall_s, "world_steps_per_s": st.world_steps / st.wall_s,
+        "requests": int(len(trace)), "completed": int(st.completed), "offered_rate_per_s": rate,
+        "duration_s": duration, "wall_s": st.wall_s, "ticks": int(st.ticks),
+        "mean_in_flight": st.mean_in_flight, "max_in_flight": int(st.max_in_flight),
+        "device_busy_frac": st.busy_s / st.wall_s,
+        "pool": {"worlds": worlds, "bodies": int(sum(s.body_count for s in scenes)),
+                 "sizes": "log-uniform 16-4096", "memory_budget_gb": budget_gb, "build_s": build_s},
+        "launch": "hipGraph replay" if graph else "eager",
+    }
+    pool.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +222,14 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=-1, help="-1: the GPU's timed steps (bounded)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--eager", action="store_true", help="launch every kernel from the host (no step graph)")
+    ap.add_argument("--batch-worlds", type=int, default=4096, help="0 skips the batched section")
+    ap.add_argument("--batch-steps", type=int, default=50)
+    ap.add_argument("--batch-sample", type=int, default=64, help="per-world loop / CPU sample stride")
+    ap.add_argument("--serve-worlds", type=int, default=1000, help="0 skips the serving section")
+    ap.add_argument("--serve-rate", type=float, default=20000.0)
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: worlds shard by world index across GPUs with no "
@@ -130,16 +244,19 @@ def main():
     scene = wl["make"]()
     ndyn = scene.dynamic_count
 
+    mode = eng.STEP_ASYNC | (0 if args.eager else eng.STEP_GRAPH)
     world = eng.World(scene)
-    world.step(max(args.warmup, 1))
+    world.step(max(args.warmup - 2, 1))
     batch = world.batch()
     dt = np.float32(1.0 / 60.0)
     ratio = np.float32(np.float32(1.0) / dt) * dt
     batch.set_world_params(0, [eng.WorldParams((scene.gravity[0], scene.gravity[1]), dt, ratio, 1, 1)])
+    for _ in range(2):  # last warm-up steps: capture the step graph of both contact-buffer parities
+        batch.step(8, 3, mode & ~eng.STEP_ASYNC)
     batch.wait()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        batch.step(8, 3, eng.STEP_ASYNC)
+        batch.step(8, 3, mode)
     batch.wait()
     elapsed = time.perf_counter() - t0
     counts = batch.counts()
@@ -166,6 +283,16 @@ def main():
                             else 40)
         cpu = cpu_baseline(wl["make"](), args.warmup, cpu_steps)
 
+    batched = None
+    if args.batch_worlds > 0:
+        batched = batched_section(pkg, args.batch_worlds, args.warmup, args.batch_steps, args.batch_sample,
+                                  not args.no_cpu, mode)
+
+    serving = None
+    if args.serve_worlds > 0:
+        serving = serving_section(pkg, args.serve_worlds, args.serve_rate, args.serve_seconds,
+                                  args.serve_budget_gb, not args.eager)
+
     value = ndyn * args.steps / elapsed
     line = {
         "metric": "body-steps/s on 1 GPU vs host CPU (20k pile)",
@@ -184,13 +311,17 @@ def main():
                    "shapes": scene.fixture_count, "joints": scene.joint_count, "dt": "1/60",
                    "velocity_iterations": 8, "position_iterations": 3,
                    "contacts": counts.contacts, "touching": counts.touching, "colours": counts.colours,
-                   "steps_timed": f"{args.warmup + 1}-{args.warmup + args.steps}"},
+                   "steps_timed": f"{args.warmup + 1}-{args.warmup + args.steps}",
+                   "launch": "eager" if args.eager else "hipGraph replay of the whole step",
+                   "graph_steps": counts.graph_steps, "serial_constraints": counts.serial},
         "roofline": {"bound": "hbm", "kernel": args.kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "avg_launch_ms": avg_ms, "launches": k_launches,
                      "bytes_per_launch": bytes_per_launch, "bytes_per_item": per_item},
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "batched": batched,
+        "serving": serving,
         "host": platform.node(),
     }
     print(json.dumps(line))

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -69,6 +70,7 @@ struct b2g_batch {
   // host mirrors
   std::vector<int> h_btype, h_bworld;
   std::vector<int> h_shape_body;
+  std::vector<int> h_wshape0;   // first shape of each world
   std::vector<b2g_revolute> h_joints;
   std::vector<int> h_jcol_off;  // joint colour ranges (host colouring)
   std::vector<b2g_world_params> h_params;
@@ -115,8 +117,32 @@ struct b2g_batch {
     t_items.clear();
   }
 
+  // captured steps (graph mode)
+  struct GraphEntry {
+    hipGraphExec_t exec;
+    int parity, vi, pi, K;
+    unsigned flags;
+  };
+  std::vector<GraphEntry> graphs;
+  int parity = 0;            // which contact buffer is current (flips with every broadphase)
+  int graph_K = 0;           // colour slots per sweep of captured steps (0: no eager step yet)
+  int rounds_pending = 0;    // claim rounds enqueued since the last round-base advance
+  unsigned round_host = 0;   // host mirror of the device claim-round base
+  long long graph_steps = 0;
+
   int alloc();
   void release();
+  void swap_contact_buffers();
+  void graphs_clear();
+  void enqueue_collide_and_colour(int K, int rounds);
+  void enqueue_colour_round(int K);
+  void enqueue_colour_groups();
+  void enqueue_islands_and_init();
+  void enqueue_solve(int vi, int pi, const std::vector<int>& colours, const std::vector<int>& sizes, int graph_grid);
+  int enqueue_graph_body(int vi, int pi, unsigned flags, int K);
+  int check_status();
+  int eager_step(int vi, int pi, unsigned flags);
+  int graph_step(int vi, int pi, unsigned flags);
   int scan(const int* in, const int* n_dev, int n_host, int* out, int* total, long long n_bound);
   int radix_sort(int n, int bits);
   int broadphase(bool use_given_aabbs);
@@ -133,7 +159,7 @@ int b2g_batch::alloc() {
   if ((rc = dalloc(&(ptr), (size_t)(n))) != B2G_OK) return rc
   A(d.xf, NB); A(d.pos, NB); A(d.vel, NB); A(d.prop, NB); A(d.prop2, NB); A(d.force, NB);
   A(d.btype, NB); A(d.bworld, NB);
-  A(d.wpar, NW); A(d.wflag, NW); A(d.wbounds, 4 * NW); A(d.wleaf, 2 * NW);
+  A(d.wpar, NW); A(d.wflag, NW); A(d.wshape0, NW); A(d.wbounds, 4 * NW); A(d.wleaf, 2 * NW);
   A(d.shdr, NS); A(d.geo, 9 * (size_t)NS); A(d.smat, NS); A(d.sfilt, NS); A(d.aabb, NS);
   A(d.joints, NJ); A(d.jsol, NJ); A(d.jorder, NJ); A(d.jisland, NJ);
   A(d.bj_off, NB + 1); A(d.bj_other, 2 * NJ);
@@ -170,6 +196,8 @@ int b2g_batch::alloc() {
   h_btype.assign(NB, 0);
   h_bworld.assign(NB, 0);
   h_shape_body.assign(NS, 0);
+  h_wshape0.assign(NW, 0);
+  HIP_TRY(hipMemcpy(d.wshape0, h_wshape0.data(), NW * sizeof(int), hipMemcpyHostToDevice));
   h_params.assign(NW, b2g_world_params{{0.0f, -10.0f}, 1.0f / 60.0f, 0.0f, 1, 1});
   std::vector<int> zero_off(NB + 1, 0);
   HIP_TRY(hipMemcpy(d.bj_off, zero_off.data(), (NB + 1) * sizeof(int), hipMemcpyHostToDevice));
@@ -177,7 +205,8 @@ int b2g_batch::alloc() {
 }
 
 void b2g_batch::release() {
-  void* ptrs[] = {d.xf, d.pos, d.vel, d.prop, d.prop2, d.force, d.btype, d.bworld, d.wpar, d.wflag,
+  graphs_clear();
+  void* ptrs[] = {d.xf, d.pos, d.vel, d.prop, d.prop2, d.force, d.btype, d.bworld, d.wpar, d.wflag, d.wshape0,
                   d.wbounds, d.wleaf, d.shdr, d.geo, d.smat, d.sfilt, d.aabb, d.joints, d.jsol,
                   d.jorder, d.jisland, d.bj_off, d.bj_other, d.ckey, d.cpair, d.cman, d.cmat, d.ccol,
                   d.nkey, d.npair, d.nman, d.nmat, d.ncol, d.touch, d.tpos, d.clist, d.ccolour,
@@ -223,7 +252,7 @@ int b2g_batch::radix_sort(int n, int bits) {
 int b2g_batch::broadphase(bool use_given_aabbs) {
   const int ns = d.ns;
   if (ns == 0) {
-    HIP_TRY(hipMemsetAsync(d.cnt + C_PAIRS, 0, sizeof(int), stream));
+    hipLaunchKernelGGL(k_clear_pairs, dim3(1), dim3(1), 0, stream, d);
     hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, stream, d);
     return B2G_OK;
   }
@@ -264,11 +293,7 @@ int b2g_batch::broadphase(bool use_given_aabbs) {
   hipLaunchKernelGGL(k_merge_contacts, dim3(grid_for(std::max(est_contacts, ns) * 5 / 4)), dim3(kBlock), 0, stream,
                      d);
   hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, stream, d);
-  std::swap(d.ckey, d.nkey);
-  std::swap(d.cpair, d.npair);
-  std::swap(d.cman, d.nman);
-  std::swap(d.cmat, d.nmat);
-  std::swap(d.ccol, d.ncol);
+  swap_contact_buffers();
   return B2G_OK;
 }
 
@@ -327,87 +352,82 @@ int b2g_batch::read_counters() {
   return B2G_OK;
 }
 
-int b2g_batch::step(int vi, int pi, unsigned flags) {
-  if (joints_dirty) {
-    int rc = rebuild_joint_tables();
-    if (rc) return rc;
-  }
-  const bool prof_on = (flags & B2G_STEP_PROFILE) != 0;
-  if (prof_on) HIP_TRY(hipEventRecord(ev[0], stream));
-  if (flags & B2G_STEP_FIND_NEW_CONTACTS) {
-    int rc = broadphase(false);
-    if (rc) return rc;
-  }
+// ---- the step -------------------------------------------------------------
+// One launch sequence serves both modes. Eager mode reads the colour offsets
+// back once per step (sizing the sweeps exactly, and running extra claim
+// rounds when many contacts are new); graph mode replays a captured copy of
+// the same sequence with no host round trip: `K` colour slots per sweep, a
+// fixed number of claim rounds, and leftovers in the serial group.
+
+void b2g_batch::swap_contact_buffers() {
+  std::swap(d.ckey, d.nkey);
+  std::swap(d.cpair, d.npair);
+  std::swap(d.cman, d.nman);
+  std::swap(d.cmat, d.nmat);
+  std::swap(d.ccol, d.ncol);
+  parity ^= 1;
+}
+
+void b2g_batch::graphs_clear() {
+  for (auto& g : graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  graphs.clear();
+}
+
+void b2g_batch::enqueue_collide_and_colour(int K, int rounds) {
   const int nb = d.nb;
   const int gc = grid_for(std::max(est_contacts, 256) * 5 / 4);
-  int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
+  const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
   const int gb = grid_for(nb);
-  // ---- Collide
   tbegin(B2G_KERNEL_NARROWPHASE);
   hipLaunchKernelGGL(k_narrowphase, dim3(gc), dim3(kBlock), 0, stream, d);
   tend(B2G_KERNEL_NARROWPHASE, est_contacts);
   scan(d.touch, d.cnt + C_CONTACTS, 0, d.tpos, d.cnt + C_CONS, d.ccap);
   hipLaunchKernelGGL(k_compact_touching, dim3(gc), dim3(kBlock), 0, stream, d);
-  if (prof_on) HIP_TRY(hipEventRecord(ev[1], stream));
-  // ---- Solve
   hipLaunchKernelGGL(k_integrate_velocities, dim3(gb), dim3(kBlock), 0, stream, d);
-  hipLaunchKernelGGL(k_colour_init, dim3(gk), dim3(kBlock), 0, stream, d);
-  auto colour_rounds = [&](int count) {
-    for (int r = 0; r < count; ++r) {
-      ++round;
-      if ((round & 0xffff) == 0) {  // priority tag wrapped: clear the claim slots
-        (void)hipMemsetAsync(d.bslot, 0, sizeof(unsigned long long) * std::max(nb, 1), stream);
-        ++round;
-      }
-      hipLaunchKernelGGL(k_colour_claim, dim3(gk), dim3(kBlock), 0, stream, d, (int)(round & 0xffff));
-      hipLaunchKernelGGL(k_colour_resolve, dim3(gk), dim3(kBlock), 0, stream, d, (int)(round & 0xffff));
-    }
-  };
-  auto colour_groups = [&]() {
-    hipLaunchKernelGGL(k_colour_count, dim3(gk), dim3(kBlock), 0, stream, d);
-    hipLaunchKernelGGL(k_colour_offsets, dim3(1), dim3(1), 0, stream, d);
-    hipLaunchKernelGGL(k_colour_scatter, dim3(gk), dim3(kBlock), 0, stream, d);
-    hipLaunchKernelGGL(k_sort_serial_group, dim3(1), dim3(1), 0, stream, d);
-  };
-  // Contacts keep their colour while they stay touching, so a few rounds
-  // colour the newcomers of a step; the kernels return at once when nothing
-  // is left, and the one read-back below checks it.
-  colour_rounds(kColourRoundsPerStep);
-  colour_groups();
+  hipLaunchKernelGGL(k_colour_init, dim3(gk), dim3(kBlock), 0, stream, d, K);
+  for (int r = 0; r < rounds; ++r) enqueue_colour_round(K);
+}
+
+void b2g_batch::enqueue_colour_round(int K) {
+  const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
+  hipLaunchKernelGGL(k_colour_claim, dim3(gk), dim3(kBlock), 0, stream, d, rounds_pending);
+  hipLaunchKernelGGL(k_colour_resolve, dim3(gk), dim3(kBlock), 0, stream, d, rounds_pending, K);
+  ++rounds_pending;
+}
+
+void b2g_batch::enqueue_colour_groups() {
+  const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
+  hipLaunchKernelGGL(k_colour_count, dim3(gk), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_colour_offsets, dim3(1), dim3(1), 0, stream, d, rounds_pending);
+  round_host += (unsigned)rounds_pending;
+  rounds_pending = 0;
+  hipLaunchKernelGGL(k_colour_scatter, dim3(gk), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_sort_serial_group, dim3(1), dim3(1), 0, stream, d);
+}
+
+void b2g_batch::enqueue_islands_and_init() {
+  const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
   hipLaunchKernelGGL(k_island_link, dim3(grid_for(std::max(est_cons, 1) + d.nj)), dim3(kBlock), 0, stream, d);
-  hipLaunchKernelGGL(k_island_label, dim3(gb), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_island_label, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
   hipLaunchKernelGGL(k_contact_init, dim3(gk), dim3(kBlock), 0, stream, d);
   if (d.nj) hipLaunchKernelGGL(k_joint_prepare, dim3(grid_for(d.nj)), dim3(kBlock), 0, stream, d);
-  HIP_TRY(hipMemcpyAsync(pinned + 32, d.col_off, (kColourSlots + 1) * sizeof(int), hipMemcpyDeviceToHost, stream));
-  int rc;
-  if ((rc = read_counters())) return rc;
-  est_cons = pinned[C_CONS];
-  for (int extra = 0; pinned[C_REMAIN] > 0; extra += 2) {  // first steps / bursts of new contacts
-    if (extra > 4096) return fail(B2G_ERR_CAPACITY, "colouring did not converge");
-    colour_rounds(2);
-    if ((rc = read_counters())) return rc;
-    if (pinned[C_REMAIN] == 0) {
-      hipLaunchKernelGGL(k_colour_clear_counts, dim3(1), dim3(kBlock), 0, stream, d);
-      colour_groups();
-      hipLaunchKernelGGL(k_contact_init, dim3(grid_for(std::max(est_cons, 256) * 5 / 4)), dim3(kBlock), 0, stream, d);
-      HIP_TRY(hipMemcpyAsync(pinned + 32, d.col_off, (kColourSlots + 1) * sizeof(int), hipMemcpyDeviceToHost,
-                             stream));
-      if ((rc = read_counters())) return rc;
-    }
-  }
-  const int* off = pinned + 32;
-  const int ncol = pinned[C_NCOLOURS];
-  last_ncolours = ncol;
+}
+
+// colours: the slots to sweep; sizes[c] (eager) or -1 (graph: unknown)
+void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, const std::vector<int>& sizes,
+                              int graph_grid) {
   auto contact_sweep = [&](int id, auto kernel, auto... extra) {
-    for (int c = 0; c < ncol; ++c) {
-      const int b = off[c], e = off[c + 1];
-      if (e <= b) continue;
+    for (size_t q = 0; q < colours.size(); ++q) {
+      const int c = colours[q];
+      const int n = sizes[q];
       tbegin(id);
       if (c == kSerialColour)
-        hipLaunchKernelGGL(kernel, dim3(1), dim3(1), 0, stream, d, b, e, extra...);
+        hipLaunchKernelGGL(kernel, dim3(1), dim3(1), 0, stream, d, c, extra...);
       else
-        hipLaunchKernelGGL(kernel, dim3(grid_for(e - b)), dim3(kBlock), 0, stream, d, b, e, extra...);
-      tend(id, e - b);
+        hipLaunchKernelGGL(kernel, dim3(n >= 0 ? grid_for(n) : graph_grid), dim3(kBlock), 0, stream, d, c,
+                           extra...);
+      tend(id, n);
     }
   };
   auto joint_sweep = [&](auto kernel, auto... extra) {
@@ -423,16 +443,86 @@ int b2g_batch::step(int vi, int pi, unsigned flags) {
     joint_sweep(k_joint_velocity);
     contact_sweep(B2G_KERNEL_CONTACT_VELOCITY, k_contact_velocity);
   }
-  hipLaunchKernelGGL(k_store_and_integrate, dim3(grid_for(std::max(est_cons, nb))), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_store_and_integrate, dim3(grid_for(std::max(est_cons, d.nb))), dim3(kBlock), 0, stream, d);
   const int gf = grid_for(std::max(est_cons, 1) + d.nj);
   for (int it = 0; it < pi; ++it) {
     contact_sweep(B2G_KERNEL_CONTACT_POSITION, k_contact_position, it);
     joint_sweep(k_joint_position, it);
     hipLaunchKernelGGL(k_island_finalize, dim3(gf), dim3(kBlock), 0, stream, d, it);
   }
-  hipLaunchKernelGGL(k_finish_bodies, dim3(gb), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_finish_bodies, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
+}
+
+int b2g_batch::check_status() {
+  est_contacts = pinned[C_CONTACTS];
+  if (pinned[C_STATUS] & S_CONTACT_OVERFLOW)
+    return fail(B2G_ERR_CAPACITY, "contact capacity exceeded (" + std::to_string(pinned[C_PAIRS]) + " pairs)");
+  if (pinned[C_STATUS] & S_STACK_OVERFLOW) return fail(B2G_ERR_CAPACITY, "BVH traversal stack overflow");
+  if (pinned[C_STATUS] & S_TREE_CORRUPT) return fail(B2G_ERR_INVALID, "BVH inconsistent (sort/build)");
+  return B2G_OK;
+}
+
+int b2g_batch::step(int vi, int pi, unsigned flags) {
+  if (joints_dirty) {
+    int rc = rebuild_joint_tables();
+    if (rc) return rc;
+    graphs_clear();
+  }
+  if (round_host + 4096u > 0xffffu) {
+    // the claim-round tag is 16 bits: clear the claim slots before it wraps
+    hipLaunchKernelGGL(k_reset_colour_rounds, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
+    round_host = 0;
+  }
+  const bool prof_on = (flags & B2G_STEP_PROFILE) != 0;
+  const bool graph_ok = (flags & B2G_STEP_GRAPH) && !prof_on && timed_kernel < 0 &&
+                        !(flags & B2G_STEP_FIND_NEW_CONTACTS) && graph_K > 0;
+  if (graph_ok) return graph_step(vi, pi, flags);
+  return eager_step(vi, pi, flags);
+}
+
+int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
+  const bool prof_on = (flags & B2G_STEP_PROFILE) != 0;
+  if (prof_on) HIP_TRY(hipEventRecord(ev[0], stream));
+  if (flags & B2G_STEP_FIND_NEW_CONTACTS) {
+    int rc = broadphase(false);
+    if (rc) return rc;
+  }
+  enqueue_collide_and_colour(kMaxColours, kColourRoundsPerStep);
+  if (prof_on) HIP_TRY(hipEventRecord(ev[1], stream));
+  enqueue_colour_groups();
+  enqueue_islands_and_init();
+  HIP_TRY(hipMemcpyAsync(pinned + 32, d.col_off, (kColourSlots + 1) * sizeof(int), hipMemcpyDeviceToHost, stream));
+  int rc;
+  if ((rc = read_counters())) return rc;
+  est_cons = pinned[C_CONS];
+  for (int extra = 0; pinned[C_REMAIN] > 0; extra += 2) {  // first steps / bursts of new contacts
+    if (extra > 4096) return fail(B2G_ERR_CAPACITY, "colouring did not converge");
+    enqueue_colour_round(kMaxColours);
+    enqueue_colour_round(kMaxColours);
+    if ((rc = read_counters())) return rc;
+    if (pinned[C_REMAIN] == 0) {
+      hipLaunchKernelGGL(k_colour_clear_counts, dim3(1), dim3(kBlock), 0, stream, d);
+      enqueue_colour_groups();
+      hipLaunchKernelGGL(k_contact_init, dim3(grid_for(std::max(est_cons, 256) * 5 / 4)), dim3(kBlock), 0, stream, d);
+      HIP_TRY(hipMemcpyAsync(pinned + 32, d.col_off, (kColourSlots + 1) * sizeof(int), hipMemcpyDeviceToHost,
+                             stream));
+      if ((rc = read_counters())) return rc;
+    }
+  }
+  const int* off = pinned + 32;
+  std::vector<int> colours, sizes;
+  int normal_colours = 0;
+  for (int c = 0; c < kColourSlots; ++c) {
+    if (off[c + 1] <= off[c]) continue;
+    colours.push_back(c);
+    sizes.push_back(off[c + 1] - off[c]);
+    if (c < kMaxColours) normal_colours = c + 1;
+  }
+  last_ncolours = pinned[C_NCOLOURS];
+  // colour limit for captured steps: what this step needed plus headroom
+  graph_K = std::min(kMaxColours, normal_colours + 2);
+  enqueue_solve(vi, pi, colours, sizes, 0);
   if (prof_on) HIP_TRY(hipEventRecord(ev[2], stream));
-  // ---- FindNewContacts at the end of Solve
   if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
     if ((rc = broadphase(false))) return rc;
   }
@@ -440,11 +530,7 @@ int b2g_batch::step(int vi, int pi, unsigned flags) {
   if (tev_used > 64) tcollect();
   if (!(flags & B2G_STEP_ASYNC) || prof_on) {
     if ((rc = read_counters())) return rc;
-    est_contacts = pinned[C_CONTACTS];
-    if (pinned[C_STATUS] & S_CONTACT_OVERFLOW)
-      return fail(B2G_ERR_CAPACITY, "contact capacity exceeded (" + std::to_string(pinned[C_PAIRS]) + " pairs)");
-    if (pinned[C_STATUS] & S_STACK_OVERFLOW) return fail(B2G_ERR_CAPACITY, "BVH traversal stack overflow");
-    if (pinned[C_STATUS] & S_TREE_CORRUPT) return fail(B2G_ERR_INVALID, "BVH inconsistent (sort/build)");
+    if ((rc = check_status())) return rc;
   }
   if (prof_on) {
     float t01, t12, t23, t03;
@@ -456,6 +542,73 @@ int b2g_batch::step(int vi, int pi, unsigned flags) {
     prof.solve = t12 + t23;  // b2Profile::solve includes the broadphase
     prof.broadphase = t23;
     prof.step = t03;
+  }
+  return B2G_OK;
+}
+
+// the whole step as one launch sequence with no host reads
+int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
+  enqueue_collide_and_colour(K, kColourRoundsPerStep);
+  hipLaunchKernelGGL(k_colour_leftover, dim3(grid_for(std::max(est_cons, 256) * 5 / 4)), dim3(kBlock), 0, stream,
+                     d);
+  enqueue_colour_groups();
+  enqueue_islands_and_init();
+  std::vector<int> colours, sizes;
+  for (int c = 0; c < K; ++c) {
+    colours.push_back(c);
+    sizes.push_back(-1);
+  }
+  colours.push_back(kSerialColour);
+  sizes.push_back(-1);
+  const int per_colour = std::max(256, (int)(std::max(est_cons, 256) * 2LL / std::max(K, 1)));
+  enqueue_solve(vi, pi, colours, sizes, grid_for(per_colour));
+  if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
+    int rc = broadphase(false);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipMemcpyAsync(pinned, d.cnt, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, stream));
+  return B2G_OK;
+}
+
+int b2g_batch::graph_step(int vi, int pi, unsigned flags) {
+  const unsigned key_flags = flags & B2G_STEP_SKIP_BROADPHASE;
+  GraphEntry* ge = nullptr;
+  for (auto& g : graphs)
+    if (g.parity == parity && g.vi == vi && g.pi == pi && g.K == graph_K && g.flags == key_flags) ge = &g;
+  if (!ge) {
+    if (graphs.size() >= 8) graphs_clear();
+    const int parity0 = parity;
+    const unsigned round0 = round_host;
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_graph_body(vi, pi, flags, graph_K);
+    hipError_t e = hipStreamEndCapture(stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(B2G_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(B2G_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    // capture only recorded the launches: restore the host view of the state
+    if (parity != parity0) swap_contact_buffers();
+    round_host = round0;
+    graphs.push_back(GraphEntry{exec, parity0, vi, pi, graph_K, key_flags});
+    ge = &graphs.back();
+  }
+  HIP_TRY(hipGraphLaunch(ge->exec, stream));
+  // mirror what the replay does to the host view
+  if (!(flags & B2G_STEP_SKIP_BROADPHASE)) swap_contact_buffers();
+  round_host += kColourRoundsPerStep;
+  ++graph_steps;
+  if (!(flags & B2G_STEP_ASYNC)) {
+    HIP_TRY(hipStreamSynchronize(stream));
+    int rc = check_status();
+    if (rc) return rc;
+    // constraints pushed to the serial group: raise the colour limit
+    if (pinned[C_SERIAL] > 32 && graph_K < kMaxColours) graph_K = std::min(kMaxColours, graph_K + 2);
   }
   return B2G_OK;
 }
@@ -557,6 +710,10 @@ int b2g_batch_destroy(b2g_batch* b) {
 
 int b2g_set_world_params(b2g_batch* b, int32_t first, int32_t count, const b2g_world_params* p) {
   if (!b || first < 0 || count < 0 || first + count > b->cap.worlds) return fail(B2G_ERR_INVALID, "world range");
+  if (first + count > b->d.nw) {  // the world count is part of captured launches
+    b->graphs_clear();
+    b->graph_K = 0;
+  }
   std::vector<float4> wp(count);
   std::vector<int> wf(count);
   for (int i = 0; i < count; ++i) {
@@ -576,6 +733,8 @@ int b2g_set_worlds(b2g_batch* b, int32_t count, const int32_t* body_begin, const
   (void)body_begin;
   (void)shape_begin;
   if (!b || count < 1 || count > b->cap.worlds) return fail(B2G_ERR_INVALID, "world count");
+  b->graphs_clear();
+  b->graph_K = 0;
   b->d.nw = count;
   return b2g_set_world_params(b, 0, count, params);
 }
@@ -611,6 +770,10 @@ int b2g_upload_bodies(b2g_batch* b, int32_t first, int32_t count, const b2g_body
   HIP_TRY(hipMemcpyAsync(d.bworld + first, world.data(), count * sizeof(int), hipMemcpyHostToDevice, st));
   HIP_TRY(hipStreamSynchronize(st));
   if (first + count > d.nb) {
+    b->graphs_clear();
+    b->graph_K = 0;
+  }
+  if (first + count > d.nb) {
     // extend the joint-partner CSR with empty rows
     const int old_nb = d.nb;
     d.nb = first + count;
@@ -626,6 +789,8 @@ int b2g_upload_bodies(b2g_batch* b, int32_t first, int32_t count, const b2g_body
 
 int b2g_upload_shapes(b2g_batch* b, int32_t first, int32_t count, const b2g_shape* in) {
   if (!b || first < 0 || count < 0 || first + count > b->cap.shapes) return fail(B2G_ERR_INVALID, "shape range");
+  b->graphs_clear();  // captured launch sizes no longer match
+  b->graph_K = 0;
   if (count == 0) return B2G_OK;
   std::vector<int4> hdr(count), filt(count);
   std::vector<float4> geo(9 * (size_t)count), mat(count);
@@ -647,11 +812,27 @@ int b2g_upload_shapes(b2g_batch* b, int32_t first, int32_t count, const b2g_shap
   HIP_TRY(hipMemcpyAsync(d.sfilt + first, filt.data(), count * sizeof(int4), hipMemcpyHostToDevice, st));
   HIP_TRY(hipStreamSynchronize(st));
   d.ns = std::max(d.ns, first + count);
+  // world-local shape numbering: first shape of every world; local indices
+  // must fit 24 bits (the colouring rank packs two of them)
+  std::vector<int>& w0 = b->h_wshape0;
+  std::fill(w0.begin(), w0.end(), INT_MAX);
+  for (int i = 0; i < d.ns; ++i) {
+    const int w = b->h_bworld[b->h_shape_body[i]];
+    w0[w] = std::min(w0[w], i);
+  }
+  for (int i = 0; i < d.ns; ++i)
+    if (i - w0[b->h_bworld[b->h_shape_body[i]]] >= (1 << 24))
+      return fail(B2G_ERR_CAPACITY, "a world spans more than 2^24 shape indices");
+  for (int& v : w0)
+    if (v == INT_MAX) v = 0;
+  HIP_TRY(hipMemcpy(d.wshape0, w0.data(), w0.size() * sizeof(int), hipMemcpyHostToDevice));
   return B2G_OK;
 }
 
 int b2g_upload_joints(b2g_batch* b, int32_t first, int32_t count, const b2g_revolute* in) {
   if (!b || first < 0 || count < 0 || first + count > b->cap.joints) return fail(B2G_ERR_INVALID, "joint range");
+  b->graphs_clear();  // captured launch sizes no longer match
+  b->graph_K = 0;
   if (count == 0) return B2G_OK;
   if ((int)b->h_joints.size() < first + count) b->h_joints.resize(first + count);
   std::vector<Joint> dj(count);
@@ -681,6 +862,8 @@ int b2g_upload_joints(b2g_batch* b, int32_t first, int32_t count, const b2g_revo
 
 int b2g_upload_contacts(b2g_batch* b, int32_t count, const b2g_contact* in) {
   if (!b || count < 0 || count > b->cap.contacts) return fail(B2G_ERR_INVALID, "contact count");
+  b->graphs_clear();  // captured launch sizes no longer match
+  b->graph_K = 0;
   std::vector<int> idx(count);
   std::vector<uint64_t> keys(count);
   for (int i = 0; i < count; ++i) {
@@ -742,10 +925,12 @@ int b2g_get_counts(b2g_batch* b, b2g_counts* out) {
   out->joints = b->d.nj;
   out->contacts = b->pinned[C_CONTACTS];
   out->touching = b->pinned[C_CONS];
-  out->colours = b->last_ncolours;
+  out->colours = b->pinned[C_NCOLOURS];
   out->islands = b->last_islands;
   out->pos_iters_run = b->last_pos_iters;
   out->status = b->pinned[C_STATUS];
+  out->serial = b->pinned[C_SERIAL];
+  out->graph_steps = (int32_t)std::min<long long>(b->graph_steps, INT32_MAX);
   return B2G_OK;
 }
 

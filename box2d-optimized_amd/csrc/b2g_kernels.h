@@ -42,6 +42,8 @@ enum Counter : int {
   C_REMAIN = 4,     // constraints still uncoloured
   C_NCOLOURS = 5,   // contact colours used (incl. the serial group)
   C_MAXCOLOUR = 6,
+  C_ROUND = 7,      // colouring round base (claim priorities carry base + local round)
+  C_SERIAL = 8,     // constraints placed in the serial group by the last step
   C_COUNT = 16
 };
 enum Status : int { S_CONTACT_OVERFLOW = 1, S_STACK_OVERFLOW = 2, S_TREE_CORRUPT = 4 };
@@ -65,6 +67,7 @@ struct Dev {
   // worlds
   float4* wpar;   // gravity.x, gravity.y, dt, dtRatio
   int* wflag;     // bit0 warm starting, bit1 active
+  int* wshape0;   // first shape index of each world (world-local shape numbering)
   // shapes
   int4* shdr;     // type, body, count, world
   float4* geo;    // 9 per shape
@@ -291,12 +294,14 @@ __global__ void __launch_bounds__(kBlock) k_integrate_velocities(Dev d) {
 // ===========================================================================
 __device__ __forceinline__ bool is_dyn(const Dev& d, int b) { return d.btype[b] == kDynamicBody; }
 
-__global__ void __launch_bounds__(kBlock) k_colour_init(Dev d) {
+// Contacts that stay touching keep last step's colour if it is below the
+// colour limit `K` of this step; the rest are coloured by claim rounds.
+__global__ void __launch_bounds__(kBlock) k_colour_init(Dev d, int K) {
   const int n = d.cnt[C_CONS];
   B2G_FOR(k, 0, n) {
     const int c = d.clist[k];
     const int col = d.ccol[c];
-    if (col >= 0 && col < kMaxColours) {
+    if (col >= 0 && col < K) {
       const int2 pr = d.cpair[c];
       const int ba = d.shdr[pr.x].y, bb = d.shdr[pr.y].y;
       const unsigned long long bit = 1ull << col;
@@ -310,41 +315,58 @@ __global__ void __launch_bounds__(kBlock) k_colour_init(Dev d) {
   }
 }
 
-__device__ __forceinline__ unsigned long long colour_priority(int round, int k, uint64_t key) {
-  const uint32_t h = hash32((uint32_t)key ^ (uint32_t)(key >> 32) * 0x9E3779B9u) & 0xffffffu;
-  return ((unsigned long long)(round & 0xffff) << 48) | ((unsigned long long)h << 24) |
-         (unsigned long long)(k & 0xffffff);
+// Colouring priority of a contact: the claim round in the top 16 bits and a
+// pseudo-random, collision-free 48-bit rank below it. The rank is a bijection
+// of the contact's world-local shape pair (24 + 24 bits), so two contacts
+// never tie, and a world is coloured the same whether it is stepped alone or
+// inside a batch.
+__device__ __forceinline__ unsigned long long mix48(unsigned long long x) {
+  constexpr unsigned long long kMask = (1ull << 48) - 1;
+  x = (x * 0x9E3779B97F4Bull) & kMask;  // odd multiplier: bijective mod 2^48
+  x ^= x >> 23;                          // xorshift: bijective on 48 bits
+  x = (x * 0xD6E8FEB86659ull) & kMask;   // odd
+  x ^= x >> 21;
+  return x;
+}
+__device__ __forceinline__ unsigned long long colour_priority(const Dev& d, int round, int2 pr) {
+  const int base = d.wshape0[d.shdr[pr.x].w];
+  const unsigned long long local = ((unsigned long long)(uint32_t)(pr.x - base) << 24) |
+                                   (unsigned long long)(uint32_t)(pr.y - base);
+  return ((unsigned long long)(round & 0xffff) << 48) | mix48(local);
 }
 
-__global__ void __launch_bounds__(kBlock) k_colour_claim(Dev d, int round) {
+__global__ void __launch_bounds__(kBlock) k_colour_claim(Dev d, int local_round) {
   if (d.cnt[C_REMAIN] == 0) return;
+  const int round = d.cnt[C_ROUND] + local_round;
   const int n = d.cnt[C_CONS];
   B2G_FOR(k, 0, n) {
     if (d.ccolour[k] >= 0) continue;
     const int c = d.clist[k];
     const int2 pr = d.cpair[c];
     const int ba = d.shdr[pr.x].y, bb = d.shdr[pr.y].y;
-    const unsigned long long v = colour_priority(round, k, d.ckey[c]);
+    const unsigned long long v = colour_priority(d, round, pr);
     if (is_dyn(d, ba)) atomicMax(&d.bslot[ba], v);
     if (is_dyn(d, bb)) atomicMax(&d.bslot[bb], v);
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_colour_resolve(Dev d, int round) {
+__global__ void __launch_bounds__(kBlock) k_colour_resolve(Dev d, int local_round, int K) {
   if (d.cnt[C_REMAIN] == 0) return;
+  const int round = d.cnt[C_ROUND] + local_round;
   const int n = d.cnt[C_CONS];
   B2G_FOR(k, 0, n) {
     if (d.ccolour[k] >= 0) continue;
     const int c = d.clist[k];
     const int2 pr = d.cpair[c];
     const int ba = d.shdr[pr.x].y, bb = d.shdr[pr.y].y;
-    const unsigned long long v = colour_priority(round, k, d.ckey[c]);
+    const unsigned long long v = colour_priority(d, round, pr);
     const bool da = is_dyn(d, ba), db = is_dyn(d, bb);
     const bool win = (!da || d.bslot[ba] == v) && (!db || d.bslot[bb] == v);
     if (!win) continue;
     const unsigned long long used = (da ? d.bmask[ba] : 0ull) | (db ? d.bmask[bb] : 0ull);
     int col = kSerialColour;
     if (~used != 0ull) col = __ffsll((long long)~used) - 1;
+    if (col >= K) col = kSerialColour;
     d.ccolour[k] = col;
     if (col < kMaxColours) {
       const unsigned long long bit = 1ull << col;
@@ -371,9 +393,34 @@ __global__ void __launch_bounds__(kBlock) k_colour_count(Dev d) {
     if (hist[c]) atomicAdd(&d.col_count[c], hist[c]);
 }
 
-// one thread: offsets of the colour groups; resets counters for the next step
-__global__ void k_colour_offsets(Dev d) {
+// Before the 16-bit claim-round tag wraps: empty the claim slots and restart
+// the round base (fill kernel, so the step stays capturable).
+__global__ void __launch_bounds__(kBlock) k_reset_colour_rounds(Dev d) {
+  B2G_FOR(b, 0, d.nb) d.bslot[b] = 0ull;
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.cnt[C_ROUND] = 0;
+}
+
+__global__ void k_clear_pairs(Dev d) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.cnt[C_PAIRS] = 0;
+}
+
+// Constraints still uncoloured after the rounds a step allows go to the serial
+// group (solved on one lane after the parallel colours: still a valid
+// Gauss-Seidel order).
+__global__ void __launch_bounds__(kBlock) k_colour_leftover(Dev d) {
+  if (d.cnt[C_REMAIN] == 0) return;
+  const int n = d.cnt[C_CONS];
+  B2G_FOR(k, 0, n) {
+    if (d.ccolour[k] >= 0) continue;
+    d.ccolour[k] = kSerialColour;
+  }
+}
+
+// one thread: offsets of the colour groups; resets counters for the next
+// step and advances the claim-round base past the rounds just run
+__global__ void k_colour_offsets(Dev d, int rounds_run) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  d.cnt[C_ROUND] += rounds_run;
   int off = 0, ncol = 0;
   for (int c = 0; c < kColourSlots; ++c) {
     d.col_off[c] = off;
@@ -383,6 +430,7 @@ __global__ void k_colour_offsets(Dev d) {
   }
   d.col_off[kColourSlots] = off;
   d.cnt[C_NCOLOURS] = ncol;
+  d.cnt[C_SERIAL] = d.col_count[kSerialColour];
 }
 
 // Each block reserves a contiguous range per colour with one atomic, then
@@ -643,7 +691,8 @@ __global__ void __launch_bounds__(kBlock) k_contact_init(Dev d) {
 __device__ __forceinline__ bool writes_body(const Dev& d, int b) { return d.btype[b] == kDynamicBody; }
 
 // b2ContactSolver::WarmStart (:260-299) for slots [begin, end)
-__global__ void __launch_bounds__(kBlock) k_warm_start(Dev d, int begin, int end) {
+__global__ void __launch_bounds__(kBlock) k_warm_start(Dev d, int colour) {
+  const int begin = d.col_off[colour], end = d.col_off[colour + 1];
   B2G_FOR(s, begin, end) {
     const float4 v2i = d.vc[2][s];
     const int ia = as_int(v2i.x), ib = as_int(v2i.y), count = as_int(v2i.z);
@@ -771,7 +820,8 @@ __device__ __forceinline__ void contact_velocity(const Dev& d, int s) {
   if (writes_body(d, ib)) d.vel[ib] = make_float4(vB.x, vB.y, wB, 0.0f);
 }
 
-__global__ void __launch_bounds__(kBlock) k_contact_velocity(Dev d, int begin, int end) {
+__global__ void __launch_bounds__(kBlock) k_contact_velocity(Dev d, int colour) {
+  const int begin = d.col_off[colour], end = d.col_off[colour + 1];
   B2G_FOR(s, begin, end) contact_velocity(d, s);
 }
 
@@ -902,7 +952,8 @@ __device__ __forceinline__ float contact_position(const Dev& d, int s, int isl) 
   return min_sep;
 }
 
-__global__ void __launch_bounds__(kBlock) k_contact_position(Dev d, int begin, int end, int it) {
+__global__ void __launch_bounds__(kBlock) k_contact_position(Dev d, int colour, int it) {
+  const int begin = d.col_off[colour], end = d.col_off[colour + 1];
   if (blockDim.x == 1) {  // the serial colour group: one lane, slot order
     for (int s = begin; s < end; ++s) {
       const int isl = d.cisland[s];

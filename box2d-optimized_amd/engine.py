@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libbox2d_amd.so")
 
 # status codes (include/b2g.h)
 OK, ERR_HIP, ERR_OOM, ERR_CAPACITY, ERR_INVALID, ERR_UNSUPPORTED, ERR_NO_DEVICE = range(7)
-STEP_FIND_NEW_CONTACTS, STEP_PROFILE, STEP_ASYNC, STEP_SKIP_BROADPHASE = 1, 2, 4, 8
+STEP_FIND_NEW_CONTACTS, STEP_PROFILE, STEP_ASYNC, STEP_SKIP_BROADPHASE, STEP_GRAPH = 1, 2, 4, 8, 16
 
 BODY_DTYPE = np.dtype([
     ("type", "<i4"), ("world", "<i4"), ("xf", "<f4", 4), ("c", "<f4", 2), ("a", "<f4"),
@@ -59,7 +59,7 @@ class WorldParams(ctypes.Structure):
 class Counts(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "worlds", "bodies", "shapes", "joints", "contacts", "touching", "colours", "islands",
-        "pos_iters_run", "status")]
+        "pos_iters_run", "status", "serial", "graph_steps")]
 
 
 class Profile(ctypes.Structure):
@@ -80,10 +80,28 @@ EXPORTS = [
     "b2g_download_joints", "b2g_download_contact_colours", "b2g_collide_pairs", "b2g_find_pairs",
     "b2a_world_from_scene", "b2a_world_destroy", "b2a_world_step", "b2a_world_get_bodies",
     "b2a_world_contact_count", "b2a_world_get_contacts", "b2a_world_batch", "b2a_world_prepare",
-    "b2g_time_kernel", "b2g_kernel_time",
+    "b2g_time_kernel", "b2g_kernel_time", "b2a_scene_records", "b2a_batch_from_scenes", "b2s_run",
 ]
 KERNEL_NARROWPHASE, KERNEL_WARM_START, KERNEL_CONTACT_VELOCITY, KERNEL_CONTACT_POSITION, \
     KERNEL_PAIRS_WRITE, KERNEL_SOLVER = range(6)
+
+
+REQUEST_DTYPE = np.dtype([("arrival", "<f8"), ("world", "<i4"), ("substeps", "<i4"), ("dt", "<f4"),
+                          ("reserved", "<i4")])
+RESULT_DTYPE = np.dtype([("arrival", "<f8"), ("admitted", "<f8"), ("completed", "<f8"), ("ticks", "<i4"),
+                         ("reserved", "<i4")])
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("memory_budget", ctypes.c_int64), ("velocity_iterations", ctypes.c_int32),
+                ("position_iterations", ctypes.c_int32), ("step_flags", ctypes.c_uint32),
+                ("realtime", ctypes.c_int32)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("wall_s", ctypes.c_double), ("busy_s", ctypes.c_double), ("mean_in_flight", ctypes.c_double),
+                ("ticks", ctypes.c_int64), ("world_steps", ctypes.c_int64), ("body_steps", ctypes.c_int64),
+                ("max_in_flight", ctypes.c_int32), ("completed", ctypes.c_int32)]
 
 
 class B2GError(RuntimeError):
@@ -125,6 +143,9 @@ def lib():
         L.b2a_world_prepare.argtypes = [_P]
         L.b2g_time_kernel.argtypes = [_P, _I]
         L.b2g_kernel_time.argtypes = [_P, _P, _P, _P]
+        L.b2a_scene_records.argtypes = [_P, _I, _I, _P, _P, _P]
+        L.b2a_batch_from_scenes.argtypes = [_P, _I, _P, _I, _I]
+        L.b2s_run.argtypes = [_P, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P]
         _lib = L
     return _lib
 
@@ -145,8 +166,41 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+def scene_records(scene, world=0, body_offset=0):
+    """Device records (bodies, shapes, joints) of a scene, built through the
+    b2World API on the host (no device needed)."""
+    L = lib()
+    cs = scene.to_c()
+    bodies = np.zeros(scene.body_count, dtype=BODY_DTYPE)
+    shapes = np.zeros(scene.fixture_count, dtype=SHAPE_DTYPE)
+    joints = np.zeros(scene.joint_count, dtype=JOINT_DTYPE)
+    check(L.b2a_scene_records(cs.ptr(), world, body_offset, _ptr(bodies),
+                              _ptr(shapes) if len(shapes) else None, _ptr(joints) if len(joints) else None))
+    return bodies, shapes, joints
+
+
 class Batch:
     """A set of independent worlds in device memory (b2g_batch)."""
+
+    @classmethod
+    def from_scenes(cls, scenes, contacts_per_shape=8, warm_starting=True):
+        """World i of the batch is scenes[i], laid out world after world."""
+        from .scenes import B2gScene
+        L = lib()
+        cs = [s.to_c() for s in scenes]
+        arr = (B2gScene * len(cs))(*[c.struct for c in cs])
+        h = ctypes.c_void_p()
+        check(L.b2a_batch_from_scenes(ctypes.byref(h), len(cs), ctypes.cast(arr, ctypes.c_void_p),
+                                      contacts_per_shape, int(warm_starting)))
+        b = cls(handle=h.value)
+        b.owned = True
+        b.world_bodies = np.cumsum([0] + [s.body_count for s in scenes])
+        b.world_shapes = np.cumsum([0] + [s.fixture_count for s in scenes])
+        b.world_joints = np.cumsum([0] + [s.joint_count for s in scenes])
+        b.world_dynamic = np.array([s.dynamic_count for s in scenes], dtype=np.int32)
+        b.gravity = [tuple(s.gravity) for s in scenes]
+        del cs, arr
+        return b
 
     def __init__(self, worlds=1, bodies=64, shapes=64, joints=16, contacts=4096, handle=None):
         self.L = lib()
@@ -169,6 +223,22 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+    def advance(self, steps=1, dt=1.0 / 60.0, vi=8, pi=3, flags=0):
+        """Step every world `steps` times the way b2World::Step sequences
+        them (b2_world.cpp:1108-1171): the first step of a fresh batch finds
+        contacts first and has dtRatio 0; later steps use inv_dt0 * dt."""
+        dt = np.float32(dt)
+        nw = len(self.gravity)
+        for _ in range(steps):
+            inv_dt0 = getattr(self, "inv_dt0", np.float32(0.0))
+            ratio = np.float32(inv_dt0 * dt)
+            self.set_world_params(0, [WorldParams(g, dt, ratio, 1, 1) for g in self.gravity])
+            first = not getattr(self, "stepped", False)
+            self.step(vi, pi, flags | (STEP_FIND_NEW_CONTACTS if first else 0))
+            self.stepped = True
+            self.inv_dt0 = np.float32(np.float32(1.0) / dt) if dt > 0 else np.float32(0.0)
+        return self
 
     def set_worlds(self, params):
         arr = (WorldParams * len(params))(*params)
@@ -250,6 +320,38 @@ class Batch:
         if count:
             check(self.L.b2g_download_joints(self.h, 0, count, _ptr(out)))
         return out
+
+
+def world_device_bytes(scene, contacts_per_shape=3):
+    """Device working memory of one world while it steps: body state (7
+    float4 + 2 ints), shapes (geometry, material, filter, AABB, tree and
+    sort scratch) and its contacts (two list buffers + velocity and position
+    constraints + compaction scratch)."""
+    nb, ns = scene.body_count, scene.fixture_count
+    return int(nb * (7 * 16 + 8) + ns * (9 * 16 + 3 * 16 + 16 + 120)
+               + contacts_per_shape * ns * (2 * (8 + 8 + 64 + 16 + 4) + 10 * 16 + 5 * 16 + 32))
+
+
+def serve(pool, requests, memory_budget, world_bytes, inv_dt0=None, vi=8, pi=3, graph=False, realtime=True):
+    """Continuous-batching request loop (b2s_run) over `pool`, a Batch built
+    by Batch.from_scenes that has taken at least one step. `requests` is a
+    REQUEST_DTYPE array sorted by arrival. Returns (results, stats, inv_dt0)."""
+    L = lib()
+    nw = len(pool.gravity)
+    req = np.ascontiguousarray(requests, dtype=REQUEST_DTYPE)
+    res = np.zeros(len(req), dtype=RESULT_DTYPE)
+    bodies = np.ascontiguousarray(pool.world_dynamic, dtype=np.int32)
+    wbytes = np.ascontiguousarray(world_bytes, dtype=np.int64)
+    grav = np.ascontiguousarray(np.asarray(pool.gravity, dtype=np.float32).reshape(nw, 2))
+    if inv_dt0 is None:  # every world took the pool's last advance() step
+        inv_dt0 = np.full(nw, getattr(pool, "inv_dt0", np.float32(0.0)), dtype=np.float32)
+    inv = np.ascontiguousarray(inv_dt0, dtype=np.float32)
+    cfg = ServeConfig(int(memory_budget), vi, pi, STEP_GRAPH if graph else 0, 1 if realtime else 0)
+    st = ServeStats()
+    check(L.b2s_run(pool.h, nw, _ptr(bodies), _ptr(wbytes), _ptr(grav), _ptr(inv), len(req),
+                    _ptr(req) if len(req) else None, ctypes.byref(cfg), _ptr(res) if len(res) else None,
+                    ctypes.byref(st)))
+    return res, st, inv
 
 
 class World:

@@ -665,44 +665,7 @@ int b2World::Upload() {
   if (ns > m_uploadedShapes) {
     std::vector<b2g_shape> rec(ns - m_uploadedShapes);
     for (int i = m_uploadedShapes; i < ns; ++i) {
-      const b2Fixture* f = m_fixtures[i];
-      b2g_shape& s = rec[i - m_uploadedShapes];
-      std::memset(&s, 0, sizeof(s));
-      const b2Shape* sh = f->m_shape;
-      s.type = (int32_t)sh->GetType();
-      s.body = f->m_body->m_index;
-      s.radius = sh->m_radius;
-      if (s.type == B2G_CIRCLE) {
-        const b2CircleShape* c = static_cast<const b2CircleShape*>(sh);
-        s.centroid[0] = c->m_p.x;
-        s.centroid[1] = c->m_p.y;
-      } else if (s.type == B2G_EDGE) {
-        const b2EdgeShape* e = static_cast<const b2EdgeShape*>(sh);
-        s.count = e->m_oneSided ? 1 : 0;
-        const b2Vec2 v[4] = {e->m_vertex0, e->m_vertex1, e->m_vertex2, e->m_vertex3};
-        for (int k = 0; k < 4; ++k) {
-          s.vertices[2 * k] = v[k].x;
-          s.vertices[2 * k + 1] = v[k].y;
-        }
-      } else {
-        const b2PolygonShape* p = static_cast<const b2PolygonShape*>(sh);
-        s.count = p->m_count;
-        s.centroid[0] = p->m_centroid.x;
-        s.centroid[1] = p->m_centroid.y;
-        for (int k = 0; k < p->m_count; ++k) {
-          s.vertices[2 * k] = p->m_vertices[k].x;
-          s.vertices[2 * k + 1] = p->m_vertices[k].y;
-          s.normals[2 * k] = p->m_normals[k].x;
-          s.normals[2 * k + 1] = p->m_normals[k].y;
-        }
-      }
-      s.friction = f->m_friction;
-      s.restitution = f->m_restitution;
-      s.restitution_threshold = f->m_restitutionThreshold;
-      s.category = f->m_filter.categoryBits;
-      s.mask = f->m_filter.maskBits;
-      s.group = f->m_filter.groupIndex;
-      s.is_sensor = f->m_isSensor ? 1 : 0;
+      FixtureToRecord(m_fixtures[i], &rec[i - m_uploadedShapes]);
     }
     if ((rc = b2g_upload_shapes(m_batch, m_uploadedShapes, ns - m_uploadedShapes, rec.data()))) return rc;
     m_uploadedShapes = ns;
@@ -711,24 +674,88 @@ int b2World::Upload() {
   if (nj > m_uploadedJoints) {
     std::vector<b2g_revolute> rec(nj - m_uploadedJoints);
     for (int i = m_uploadedJoints; i < nj; ++i) {
-      const b2RevoluteJoint* j = static_cast<const b2RevoluteJoint*>(m_joints[i]);
-      b2g_revolute& r = rec[i - m_uploadedJoints];
-      std::memset(&r, 0, sizeof(r));
-      r.body_a = j->m_bodyA->m_index;
-      r.body_b = j->m_bodyB->m_index;
-      r.local_anchor_a[0] = j->m_localAnchorA.x; r.local_anchor_a[1] = j->m_localAnchorA.y;
-      r.local_anchor_b[0] = j->m_localAnchorB.x; r.local_anchor_b[1] = j->m_localAnchorB.y;
-      r.reference_angle = j->m_referenceAngle;
-      r.lower_angle = j->m_lowerAngle; r.upper_angle = j->m_upperAngle;
-      r.motor_speed = j->m_motorSpeed; r.max_motor_torque = j->m_maxMotorTorque;
-      r.flags = (j->m_collideConnected ? 1 : 0) | (j->m_enableLimit ? 2 : 0) | (j->m_enableMotor ? 4 : 0);
-      r.impulse[0] = j->m_impulse.x; r.impulse[1] = j->m_impulse.y;
-      r.motor_impulse = j->m_motorImpulse; r.lower_impulse = j->m_lowerImpulse; r.upper_impulse = j->m_upperImpulse;
+      JointToRecord(m_joints[i], &rec[i - m_uploadedJoints]);
     }
     if ((rc = b2g_upload_joints(m_batch, m_uploadedJoints, nj - m_uploadedJoints, rec.data()))) return rc;
     m_uploadedJoints = nj;
   }
   return rc;
+}
+
+void b2World::FixtureToRecord(const b2Fixture* f, b2g_shape* out) {
+  b2g_shape& s = *out;
+  std::memset(&s, 0, sizeof(s));
+  const b2Shape* sh = f->m_shape;
+  s.type = (int32_t)sh->GetType();
+  s.body = f->m_body->m_index;
+  s.radius = sh->m_radius;
+  if (s.type == B2G_CIRCLE) {
+    const b2CircleShape* c = static_cast<const b2CircleShape*>(sh);
+    s.centroid[0] = c->m_p.x;
+    s.centroid[1] = c->m_p.y;
+  } else if (s.type == B2G_EDGE) {
+    const b2EdgeShape* e = static_cast<const b2EdgeShape*>(sh);
+    s.count = e->m_oneSided ? 1 : 0;
+    const b2Vec2 v[4] = {e->m_vertex0, e->m_vertex1, e->m_vertex2, e->m_vertex3};
+    for (int k = 0; k < 4; ++k) {
+      s.vertices[2 * k] = v[k].x;
+      s.vertices[2 * k + 1] = v[k].y;
+    }
+  } else {
+    const b2PolygonShape* p = static_cast<const b2PolygonShape*>(sh);
+    s.count = p->m_count;
+    s.centroid[0] = p->m_centroid.x;
+    s.centroid[1] = p->m_centroid.y;
+    for (int k = 0; k < p->m_count; ++k) {
+      s.vertices[2 * k] = p->m_vertices[k].x;
+      s.vertices[2 * k + 1] = p->m_vertices[k].y;
+      s.normals[2 * k] = p->m_normals[k].x;
+      s.normals[2 * k + 1] = p->m_normals[k].y;
+    }
+  }
+  s.friction = f->m_friction;
+  s.restitution = f->m_restitution;
+  s.restitution_threshold = f->m_restitutionThreshold;
+  s.category = f->m_filter.categoryBits;
+  s.mask = f->m_filter.maskBits;
+  s.group = f->m_filter.groupIndex;
+  s.is_sensor = f->m_isSensor ? 1 : 0;
+}
+
+void b2World::JointToRecord(const b2Joint* joint, b2g_revolute* out) {
+  const b2RevoluteJoint* j = static_cast<const b2RevoluteJoint*>(joint);
+  b2g_revolute& r = *out;
+  std::memset(&r, 0, sizeof(r));
+  r.body_a = j->m_bodyA->m_index;
+  r.body_b = j->m_bodyB->m_index;
+  r.local_anchor_a[0] = j->m_localAnchorA.x; r.local_anchor_a[1] = j->m_localAnchorA.y;
+  r.local_anchor_b[0] = j->m_localAnchorB.x; r.local_anchor_b[1] = j->m_localAnchorB.y;
+  r.reference_angle = j->m_referenceAngle;
+  r.lower_angle = j->m_lowerAngle; r.upper_angle = j->m_upperAngle;
+  r.motor_speed = j->m_motorSpeed; r.max_motor_torque = j->m_maxMotorTorque;
+  r.flags = (j->m_collideConnected ? 1 : 0) | (j->m_enableLimit ? 2 : 0) | (j->m_enableMotor ? 4 : 0);
+  r.impulse[0] = j->m_impulse.x; r.impulse[1] = j->m_impulse.y;
+  r.motor_impulse = j->m_motorImpulse; r.lower_impulse = j->m_lowerImpulse; r.upper_impulse = j->m_upperImpulse;
+}
+
+int b2World::ExportRecords(int32_t world, int32_t body_offset, b2g_body* bodies, b2g_shape* shapes,
+                           b2g_revolute* joints) const {
+  SyncBodies();
+  SyncJoints();
+  for (size_t i = 0; i < m_bodies.size(); ++i) {
+    m_bodies[i]->ToRecord(&bodies[i]);
+    bodies[i].world = world;
+  }
+  for (size_t i = 0; i < m_fixtures.size(); ++i) {
+    FixtureToRecord(m_fixtures[i], &shapes[i]);
+    shapes[i].body += body_offset;
+  }
+  for (size_t i = 0; i < m_joints.size(); ++i) {
+    JointToRecord(m_joints[i], &joints[i]);
+    joints[i].body_a += body_offset;
+    joints[i].body_b += body_offset;
+  }
+  return B2G_OK;
 }
 
 void b2World::SyncBodies() const {
@@ -888,8 +915,11 @@ struct b2a_world {
 
 extern "C" {
 
-int b2a_world_from_scene(b2a_world** out, const b2g_scene* s, int32_t warm_starting) {
-  if (!out || !s) return B2G_ERR_INVALID;
+}  // extern "C"
+
+// Builds a b2World from a scene description through the public API
+// (CreateBody / CreateFixture / CreateJoint), as user code would.
+static b2a_world* world_from_scene(const b2g_scene* s, int32_t warm_starting) {
   b2a_world* w = new b2a_world();
   w->world = new b2World(b2Vec2(s->gravity[0], s->gravity[1]));
   w->world->SetContinuousPhysics(false);
@@ -977,7 +1007,69 @@ int b2a_world_from_scene(b2a_world** out, const b2g_scene* s, int32_t warm_start
     jd.maxMotorTorque = p[5];
     w->world->CreateJoint(&jd);
   }
-  *out = w;
+  return w;
+}
+
+extern "C" {
+
+int b2a_world_from_scene(b2a_world** out, const b2g_scene* s, int32_t warm_starting) {
+  if (!out || !s) return B2G_ERR_INVALID;
+  *out = world_from_scene(s, warm_starting);
+  return B2G_OK;
+}
+
+int b2a_scene_records(const b2g_scene* s, int32_t world, int32_t body_offset, b2g_body* bodies,
+                      b2g_shape* shapes, b2g_revolute* joints) {
+  if (!s || !bodies || (s->fixture_count && !shapes) || (s->joint_count && !joints)) return B2G_ERR_INVALID;
+  b2a_world* w = world_from_scene(s, 1);
+  const int rc = w->world->ExportRecords(world, body_offset, bodies, shapes, joints);
+  b2a_world_destroy(w);
+  return rc;
+}
+
+int b2a_batch_from_scenes(b2g_batch** out, int32_t count, const b2g_scene* scenes, int32_t contacts_per_shape,
+                          int32_t warm_starting) {
+  if (!out || !scenes || count < 1 || contacts_per_shape < 1) return B2G_ERR_INVALID;
+  int64_t nb = 0, ns = 0, nj = 0;
+  for (int i = 0; i < count; ++i) {
+    nb += scenes[i].body_count;
+    ns += scenes[i].fixture_count;
+    nj += scenes[i].joint_count;
+  }
+  if (nb >= (1ll << 31) || ns * contacts_per_shape >= (1ll << 24)) return B2G_ERR_CAPACITY;
+  std::vector<b2g_body> bodies(nb);
+  std::vector<b2g_shape> shapes(ns);
+  std::vector<b2g_revolute> joints(nj);
+  std::vector<b2g_world_params> params(count);
+  std::vector<int32_t> body_begin(count + 1), shape_begin(count + 1);
+  int64_t ob = 0, os = 0, oj = 0;
+  for (int i = 0; i < count; ++i) {
+    const b2g_scene& s = scenes[i];
+    body_begin[i] = (int32_t)ob;
+    shape_begin[i] = (int32_t)os;
+    int rc = b2a_scene_records(&s, i, (int32_t)ob, bodies.data() + ob, shapes.data() + os, joints.data() + oj);
+    if (rc) return rc;
+    params[i] = b2g_world_params{{s.gravity[0], s.gravity[1]}, 1.0f / 60.0f, 0.0f, warm_starting ? 1 : 0, 1};
+    ob += s.body_count;
+    os += s.fixture_count;
+    oj += s.joint_count;
+  }
+  body_begin[count] = (int32_t)ob;
+  shape_begin[count] = (int32_t)os;
+  b2g_capacity cap{count, (int32_t)std::max<int64_t>(nb, 1), (int32_t)std::max<int64_t>(ns, 1),
+                   (int32_t)std::max<int64_t>(nj, 1), (int32_t)std::max<int64_t>(ns * contacts_per_shape, 1024)};
+  b2g_batch* b = nullptr;
+  int rc = b2g_batch_create(&b, &cap);
+  if (rc) return rc;
+  if (!rc) rc = b2g_set_worlds(b, count, body_begin.data(), shape_begin.data(), params.data());
+  if (!rc) rc = b2g_upload_bodies(b, 0, (int32_t)nb, bodies.data());
+  if (!rc && ns) rc = b2g_upload_shapes(b, 0, (int32_t)ns, shapes.data());
+  if (!rc && nj) rc = b2g_upload_joints(b, 0, (int32_t)nj, joints.data());
+  if (rc) {
+    b2g_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
   return B2G_OK;
 }
 

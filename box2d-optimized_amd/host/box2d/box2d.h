@@ -573,6 +573,11 @@ class b2World {
   int32 GetLastStatus() const { return m_status; }
   // allocate device memory and upload everything created so far
   int Prepare();
+  // device records of everything created so far, as this world would upload
+  // them: bodies tagged with `world`, shape and joint body indices shifted by
+  // `body_offset` (arrays sized GetBodyCount / GetProxyCount / GetJointCount)
+  int ExportRecords(int32_t world, int32_t body_offset, b2g_body* bodies, b2g_shape* shapes,
+                    b2g_revolute* joints) const;
 
  private:
   friend class b2Body;
@@ -580,6 +585,8 @@ class b2World {
   friend class b2Fixture;
   int EnsureDevice();
   int Upload();
+  static void FixtureToRecord(const b2Fixture* f, b2g_shape* out);
+  static void JointToRecord(const b2Joint* j, b2g_revolute* out);
   void SyncBodies() const;
   void SyncContacts();
   void SyncJoints() const;

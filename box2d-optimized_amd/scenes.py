@@ -287,18 +287,34 @@ def random_pile(n: int = 20000, seed: int = 0xC2, half_w: float = 100.0,
     kinds = rs.uniform(n)  # polygon vertex count draw
     radii = rs.uniform(n)
     ang_pts = rs.uniform(n * 8).reshape(n, 8) * (2.0 * math.pi)
-    for i in range(n):
-        b = s.add_body(DYNAMIC, float(np.float32(xs[i])), float(np.float32(ys[i])),
-                       float(np.float32(angles[i])))
-        if i % 5 < 3:
-            k = 3 + int(kinds[i] * 6.0)
-            r = 0.25 + 0.75 * radii[i]
-            th = np.sort(ang_pts[i, :k])
-            pts = np.stack([r * np.cos(th), r * np.sin(th)], axis=1).astype(np.float32)
-            s.add_polygon(b, pts, density=1.0, friction=0.6)
-        else:
-            r = 0.25 + 0.5 * radii[i]
-            s.add_circle(b, float(np.float32(r)), density=1.0, friction=0.6)
+    # bodies (positions and angles rounded to float32, as b2BodyDef holds them)
+    b0 = s.body_count
+    s.body_type.extend([DYNAMIC] * n)
+    s.body_pos.extend(zip(xs.astype(np.float32).astype(np.float64).tolist(),
+                          ys.astype(np.float32).astype(np.float64).tolist()))
+    s.body_angle.extend(angles.astype(np.float32).astype(np.float64).tolist())
+    s.body_vel.extend([(0.0, 0.0, 0.0)] * n)
+    # fixture i: polygon when i % 5 < 3 (k points at sorted angles on a circle
+    # of radius r), else a circle
+    idx = np.arange(n)
+    poly = idx % 5 < 3
+    k = 3 + (kinds * 6.0).astype(np.int64)
+    r_poly = 0.25 + 0.75 * radii
+    th = np.sort(np.where(np.arange(8)[None, :] < k[:, None], ang_pts, np.inf), axis=1)
+    th = np.where(np.isfinite(th), th, 0.0)
+    pts = np.stack([(r_poly[:, None] * np.cos(th)).astype(np.float32),
+                    (r_poly[:, None] * np.sin(th)).astype(np.float32)], axis=2).reshape(n, 16)
+    geom = np.zeros((n, GEOM), np.float32)
+    geom[:, 0] = k
+    geom[:, 1:17] = np.where((np.arange(16)[None, :] // 2) < k[:, None], pts, 0.0)
+    circ = np.zeros((n, GEOM), np.float32)
+    circ[:, 0] = (0.25 + 0.5 * radii).astype(np.float32)
+    geom = np.where(poly[:, None], geom, circ)
+    s.fix_body.extend((b0 + idx).tolist())
+    s.fix_kind.extend(np.where(poly, POLYGON, CIRCLE).tolist())
+    s.fix_geom.extend(list(geom))
+    s.fix_material.extend([(1.0, 0.6, 0.0, 1.0)] * n)
+    s.fix_filter.extend([(1, 0xFFFF, 0, 0)] * n)
     return s
 
 
@@ -367,6 +383,33 @@ def batch_world(index: int, size: int, seed_base: int = 0xC4) -> Scene:
                     height=2.0 * half_w + 10.0)
     s.name = f"w{index}_pile{size}"
     return s
+
+
+def serving_sizes(worlds: int = 10000, seed: int = 0xC5, lo: int = 16, hi: int = 4096) -> np.ndarray:
+    """Resident pool of the serving workload: body counts log-uniform in
+    [lo, hi]."""
+    rs = Stream(seed)
+    return np.round(np.exp(np.log(lo) + (np.log(hi) - np.log(lo)) * rs.uniform(worlds))).astype(np.int64)
+
+
+def serving_trace(worlds: int, rate: float = 20000.0, duration: float = 2.0, seed: int = 0xC5,
+                  dts=(1.0 / 60.0, 1.0 / 240.0)):
+    """Poisson request arrivals at `rate` per second over `duration` seconds:
+    each request names a uniform world, 1-8 substeps and dt 1/60 or 1/240.
+    Returns a structured array (arrival, world, substeps, dt, reserved)."""
+    rs = Stream(seed ^ 0x5E5)
+    n_max = int(rate * duration * 1.5 + 64)
+    gaps = -np.log(np.maximum(rs.uniform(n_max), 1e-300)) / rate
+    t = np.cumsum(gaps)
+    n = int(np.searchsorted(t, duration))
+    u = rs.uniform(3 * n).reshape(n, 3) if n else np.zeros((0, 3))
+    out = np.zeros(n, dtype=[("arrival", "<f8"), ("world", "<i4"), ("substeps", "<i4"), ("dt", "<f4"),
+                             ("reserved", "<i4")])
+    out["arrival"] = t[:n]
+    out["world"] = np.minimum((u[:, 0] * worlds).astype(np.int64), worlds - 1)
+    out["substeps"] = 1 + np.minimum((u[:, 1] * 8).astype(np.int64), 7)
+    out["dt"] = np.where(u[:, 2] < 0.5, np.float32(dts[0]), np.float32(dts[1]))
+    return out
 
 
 def small_mixed(seed: int = 7, n: int = 60) -> Scene:

@@ -175,6 +175,8 @@ typedef struct b2g_counts {
   int32_t islands;            /* islands of the last step */
   int32_t pos_iters_run;      /* max position iterations executed by an island */
   int32_t status;             /* sticky device status (B2G_ERR_CAPACITY on overflow) */
+  int32_t serial;             /* constraints the last step solved in the serial group */
+  int32_t graph_steps;        /* steps replayed from a captured graph so far */
 } b2g_counts;
 
 /* Per-stage device time of the last step, in milliseconds (b2Profile,
@@ -205,7 +207,11 @@ enum {
   B2G_STEP_FIND_NEW_CONTACTS = 1u << 0,  /* run the broadphase before Collide (m_newContacts) */
   B2G_STEP_PROFILE = 1u << 1,            /* time each stage with HIP events (adds syncs) */
   B2G_STEP_ASYNC = 1u << 2,              /* return without waiting for the device */
-  B2G_STEP_SKIP_BROADPHASE = 1u << 3     /* omit the end-of-step broadphase (testing only) */
+  B2G_STEP_SKIP_BROADPHASE = 1u << 3,    /* omit the end-of-step broadphase (testing only) */
+  B2G_STEP_GRAPH = 1u << 4               /* replay a captured hipGraph of the whole step: no host round
+                                            trip; colour slots and launch sizes fixed at capture from the
+                                            last eager step (re-captured after uploads). The first step
+                                            after an upload runs eagerly. */
 };
 
 int b2g_device_count(int32_t* count);
@@ -287,6 +293,58 @@ int b2a_world_get_contacts(b2a_world* world, int32_t capacity, b2g_contact* out,
 b2g_batch* b2a_world_batch(b2a_world* world);
 /* allocate the device batch and upload bodies/shapes/joints without stepping */
 int b2a_world_prepare(b2a_world* world);
+
+/* Device records of a scene, built through the same b2World API (polygon hull
+ * and welding, mass data) without touching the device: bodies tagged with
+ * `world`, shape/joint body indices shifted by `body_offset`. Output arrays
+ * hold body_count / fixture_count / joint_count records. Host only. */
+int b2a_scene_records(const b2g_scene* scene, int32_t world, int32_t body_offset, b2g_body* bodies,
+                      b2g_shape* shapes, b2g_revolute* joints);
+/* A batch of `count` independent worlds, world i built from scenes[i] as
+ * b2a_world_from_scene would build it, laid out world after world; contact
+ * capacity = contacts_per_shape x total shapes. The batched form of creating
+ * `count` b2Worlds; no reference counterpart (the reference steps one world). */
+int b2a_batch_from_scenes(b2g_batch** out, int32_t count, const b2g_scene* scenes,
+                          int32_t contacts_per_shape, int32_t warm_starting);
+
+/* ---- serving: continuous-batching request loop over a resident pool ------
+ * No reference counterpart (the reference steps one world on the caller's
+ * thread); it serves callers that submit world-step requests. */
+typedef struct b2s_request {
+  double arrival;             /* seconds from the start (realtime) or tick number (not realtime) */
+  int32_t world;              /* world index in the pool */
+  int32_t substeps;           /* steps to run, >= 1 */
+  float dt;                   /* step length of every substep */
+  int32_t reserved;
+} b2s_request;
+
+typedef struct b2s_result {
+  double arrival, admitted, completed;   /* same clock as the request's arrival */
+  int32_t ticks;                         /* ticks the request was in flight */
+  int32_t reserved;
+} b2s_result;
+
+typedef struct b2s_config {
+  int64_t memory_budget;      /* device working memory of the worlds in flight, bytes */
+  int32_t velocity_iterations, position_iterations;
+  uint32_t step_flags;        /* 0 or B2G_STEP_GRAPH */
+  int32_t realtime;           /* 1: arrivals on the wall clock; 0: arrivals on the tick counter */
+} b2s_config;
+
+typedef struct b2s_stats {
+  double wall_s, busy_s, mean_in_flight;
+  int64_t ticks, world_steps, body_steps;
+  int32_t max_in_flight, completed;
+} b2s_stats;
+
+/* Runs requests (sorted by arrival) against `pool`, a batch whose worlds
+ * have taken at least one step. world_bodies / world_bytes: dynamic bodies
+ * and device working bytes of each world; gravity: 2 per world; inv_dt0:
+ * per-world 1/dt of the last step taken (0 if none), updated in place.
+ * Returns when every request has completed. */
+int b2s_run(b2g_batch* pool, int32_t world_count, const int32_t* world_bodies, const int64_t* world_bytes,
+            const float* gravity, float* inv_dt0, int32_t request_count, const b2s_request* requests,
+            const b2s_config* config, b2s_result* results, b2s_stats* stats);
 
 #ifdef __cplusplus
 }

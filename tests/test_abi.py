@@ -10,7 +10,7 @@ from util import REPO, load_pkg
 def header_functions():
     text = open(os.path.join(REPO, "include", "b2g.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(b2[ga]_[a-z_0-9]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(b2[gas]_[a-z_0-9]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_entry_point():
@@ -32,6 +32,12 @@ def test_record_sizes_match_header():
     assert eng.CONTACT_DTYPE.itemsize == 4 * (5 + 2 + 2 + 4 + 2 + 2 + 2 + 4)
     assert ctypes.sizeof(eng.Capacity) == 20
     assert ctypes.sizeof(eng.WorldParams) == 24
+    assert ctypes.sizeof(eng.Counts) == 4 * 12
+    # serving records: natural alignment, as the C compiler lays them out
+    assert eng.REQUEST_DTYPE.itemsize == 8 + 4 * 4
+    assert eng.RESULT_DTYPE.itemsize == 3 * 8 + 2 * 4
+    assert ctypes.sizeof(eng.ServeConfig) == 8 + 4 * 4
+    assert ctypes.sizeof(eng.ServeStats) == 3 * 8 + 3 * 8 + 2 * 4
 
 
 def test_errors_without_device_are_reported_not_faked():
@@ -61,3 +67,49 @@ def test_drop_in_example_links_and_fails_loudly_without_device():
     r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2, r.stdout + r.stderr
     assert "device" in r.stdout
+
+
+def test_scene_records_match_the_restatement_and_offsets():
+    """b2a_scene_records (the batch assembler's per-world record builder)
+    produces the same shapes and body mass data as the oracle's own setup of
+    the scene, tagged with the world and shifted by the body offset."""
+    import numpy as np
+    import oracle
+    from util import bits
+    pkg = load_pkg()
+    sc = pkg.scenes.small_mixed(seed=3, n=90)
+    sc.add_revolute(1, 2, (0.0, 5.0))
+    b, s, j = pkg.engine.scene_records(sc, world=7, body_offset=1000)
+    w = oracle.RestatedWorld(sc)
+    rs, rb = w.shapes(), w.bodies()
+    for f in ("type", "count", "radius", "centroid", "vertices", "normals", "friction", "restitution"):
+        assert np.array_equal(bits(s[f]), bits(rs[f])), f
+    assert np.array_equal(s["body"], rs["body"] + 1000)
+    for f in ("xf", "c", "a", "local_center", "inv_mass", "inv_i", "mass"):
+        assert np.array_equal(bits(b[f]), bits(rb[f])), f
+    assert (b["world"] == 7).all()
+    assert len(j) == 1 and j["body_a"][0] == 1001 and j["body_b"][0] == 1002
+
+
+def test_python_records_match_the_c_compiler_layout(tmp_path):
+    """sizeof of every record in include/b2g.h, as gcc lays it out, equals
+    the size of its Python mirror."""
+    import shutil
+    import subprocess
+    import pytest
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    eng = load_pkg().engine
+    names = {"b2g_body": eng.BODY_DTYPE.itemsize, "b2g_shape": eng.SHAPE_DTYPE.itemsize,
+             "b2g_revolute": eng.JOINT_DTYPE.itemsize, "b2g_contact": eng.CONTACT_DTYPE.itemsize,
+             "b2g_capacity": ctypes.sizeof(eng.Capacity), "b2g_world_params": ctypes.sizeof(eng.WorldParams),
+             "b2g_counts": ctypes.sizeof(eng.Counts), "b2g_profile": ctypes.sizeof(eng.Profile),
+             "b2s_request": eng.REQUEST_DTYPE.itemsize, "b2s_result": eng.RESULT_DTYPE.itemsize,
+             "b2s_config": ctypes.sizeof(eng.ServeConfig), "b2s_stats": ctypes.sizeof(eng.ServeStats)}
+    src = tmp_path / "sizes.c"
+    src.write_text('#include <stdio.h>\n#include "b2g.h"\nint main(void) {\n' +
+                   "".join(f'  printf("{n} %zu\\n", sizeof({n}));\n' for n in names) + "  return 0;\n}\n")
+    exe = tmp_path / "sizes"
+    subprocess.check_call(["gcc", "-std=c99", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)])
+    got = dict(line.split() for line in subprocess.check_output([str(exe)], text=True).splitlines())
+    assert {k: int(v) for k, v in got.items()} == names

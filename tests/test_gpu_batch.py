@@ -1,0 +1,116 @@
+"""GPU tests of the batching pattern: many independent worlds stepped by one
+set of launches.
+
+A world's result must not depend on the other worlds in its batch: every
+stage works per contact/body/island, and the colouring rank is a function of
+the world-local shape pair only. So the comparison with the same world
+stepped alone is bit-exact.
+"""
+import numpy as np
+import pytest
+
+from util import bits, load_pkg, scene_for
+
+pytestmark = pytest.mark.gpu
+S = load_pkg().scenes
+
+
+def _mixed_scenes():
+    return [
+        S.pyramid(6),
+        S.random_pile(150, seed=11, half_w=8.0, height=20.0),
+        scene_for("chain")[0],
+        S.small_mixed(seed=5, n=40),
+        S.batch_world(3, 120),
+        S.pyramid(12),
+        S.random_pile(400, seed=12, half_w=12.0, height=30.0),
+        S.batch_world(17, 60),
+    ]
+
+
+def _state(rec):
+    return np.concatenate([bits(rec["xf"]).reshape(len(rec), -1), bits(rec["c"]).reshape(len(rec), -1),
+                           bits(rec["a"]).reshape(len(rec), -1), bits(rec["v"]).reshape(len(rec), -1),
+                           bits(rec["w"]).reshape(len(rec), -1)], axis=1)
+
+
+def test_gpu_batch_worlds_match_worlds_alone_bitexact(engine):
+    scenes = _mixed_scenes()
+    steps = 40
+    print("batch of", len(scenes), "worlds,", sum(s.body_count for s in scenes), "bodies", flush=True)
+    batch = engine.Batch.from_scenes(scenes).advance(steps)
+    got = batch.bodies()
+    for i, sc in enumerate(scenes):
+        alone = engine.Batch.from_scenes([sc]).advance(steps)
+        ref = alone.bodies()
+        part = got[batch.world_bodies[i]:batch.world_bodies[i + 1]]
+        assert (part["world"] == i).all()
+        same = _state(part) == _state(ref)
+        print(f"world {i} ({sc.name}): {same.all(axis=1).sum()} of {len(ref)} bodies bit-identical", flush=True)
+        assert same.all(), sc.name
+
+
+def test_gpu_batch_world_matches_drop_in_world(engine):
+    """A batch of one built by b2a_batch_from_scenes steps exactly like the
+    drop-in b2World built from the same scene."""
+    sc = S.random_pile(300, seed=21, half_w=10.0, height=25.0)
+    b = engine.Batch.from_scenes([sc]).advance(30).bodies()
+    w = engine.World(sc)
+    w.step(30)
+    st = w.bodies()
+    assert np.array_equal(bits(b["c"]), bits(st[:, 0:2]))
+    assert np.array_equal(bits(b["a"]), bits(st[:, 2]))
+    assert np.array_equal(bits(b["v"]), bits(st[:, 3:5]))
+
+
+def test_gpu_batch_4096_worlds_steps(engine):
+    """The batched workload's shape at a reduced world count: worlds of
+    lognormal sizes (pyramids and piles) step together; no world loses
+    bodies, penetration stays bounded and nothing becomes non-finite."""
+    sizes = S.batched_sizes(256)
+    scenes = [S.batch_world(i, int(sizes[i])) for i in range(256)]
+    batch = engine.Batch.from_scenes(scenes).advance(60)
+    c = batch.counts()
+    print("256 worlds:", c.bodies, "bodies,", c.contacts, "contacts,", c.touching, "touching,", c.colours, "colours",
+          flush=True)
+    st = batch.bodies()
+    assert np.isfinite(st["c"]).all() and np.isfinite(st["v"]).all()
+    assert c.touching > 0 and c.colours <= 65
+
+
+def test_gpu_graph_steps_match_eager_steps(engine):
+    """Replaying the captured step graph gives the eager step's results:
+    bit-exact while no constraint falls back to the serial group (the colour
+    limit and claim rounds of a captured step only differ from the eager
+    step's for such constraints)."""
+    scenes = [S.random_pile(600, seed=31, half_w=14.0, height=30.0), S.pyramid(10), scene_for("chain")[0]]
+    eager = engine.Batch.from_scenes(scenes).advance(25)
+    graph = engine.Batch.from_scenes(scenes).advance(25)
+    for k in range(40):
+        eager.step(8, 3, 0)
+        graph.step(8, 3, engine.STEP_GRAPH)
+        c = graph.counts()
+        assert c.serial == 0, f"step {k}: {c.serial} constraints in the serial group"
+    assert graph.counts().graph_steps == 40
+    a, b = eager.bodies(), graph.bodies()
+    for f in ("xf", "c", "a", "v", "w"):
+        assert np.array_equal(bits(a[f]), bits(b[f])), f
+    ka = eager.contacts()
+    kb = graph.contacts()
+    assert np.array_equal(ka["shape_a"], kb["shape_a"]) and np.array_equal(bits(ka["normal_impulse"]),
+                                                                            bits(kb["normal_impulse"]))
+
+
+def test_gpu_graph_steps_async_batch(engine):
+    """Async graph replays of a batch (no host sync between steps), then one
+    sync: same state as the same number of synchronous graph steps."""
+    scenes = [S.batch_world(i, 80 + 40 * i) for i in range(12)]
+    a = engine.Batch.from_scenes(scenes).advance(10)
+    b = engine.Batch.from_scenes(scenes).advance(10)
+    for _ in range(30):
+        a.step(8, 3, engine.STEP_GRAPH)
+        b.step(8, 3, engine.STEP_GRAPH | engine.STEP_ASYNC)
+    b.wait()
+    assert b.counts().status == 0
+    x, y = a.bodies(), b.bodies()
+    assert np.array_equal(bits(x["c"]), bits(y["c"])) and np.array_equal(bits(x["v"]), bits(y["v"]))
