@@ -65,7 +65,7 @@ struct b2g_batch {
   Dev d{};
   int dev_id = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[5] = {};
+  hipEvent_t ev[7] = {};
   int* pinned = nullptr;     // host mirror of counters + colour offsets
   // host mirrors
   std::vector<int> h_btype, h_bworld;
@@ -75,6 +75,7 @@ struct b2g_batch {
   std::vector<int> h_jcol_off;  // joint colour ranges (host colouring)
   std::vector<b2g_world_params> h_params;
   bool joints_dirty = false;
+  bool filters_dirty = true;
   int est_contacts = 0, est_cons = 0;
   unsigned round = 0;
   b2g_profile prof{};
@@ -122,6 +123,7 @@ struct b2g_batch {
     hipGraphExec_t exec;
     int parity, vi, pi, K;
     unsigned flags;
+    int block;  // per-world workgroup size (0: colour-major sweeps)
   };
   std::vector<GraphEntry> graphs;
   int parity = 0;            // which contact buffer is current (flips with every broadphase)
@@ -129,6 +131,16 @@ struct b2g_batch {
   int rounds_pending = 0;    // claim rounds enqueued since the last round-base advance
   unsigned round_host = 0;   // host mirror of the device claim-round base
   long long graph_steps = 0;
+  // per-world solver (one workgroup per world)
+  bool layout_dirty = true;  // world ranges need recomputing
+  bool per_world_ok = false; // joint-free, bodies and shapes laid out world after world
+  int max_world_bodies = 0;
+  int solver_mode = 0;       // B2G_SOLVER_AUTO / _COLOURS / _WORLDS
+  int world_block = 0;       // threads per world workgroup (0: autotune)
+  int tuned_block = 0;       // chosen by the autotuner (or world_block)
+  int tune_step = 0;         // autotuner progress: two timed steps per candidate
+  float tune_ms[8] = {};
+  int last_path = 0;         // 1 colour-major sweeps, 2 per-world
 
   int alloc();
   void release();
@@ -141,6 +153,9 @@ struct b2g_batch {
   void enqueue_solve(int vi, int pi, const std::vector<int>& colours, const std::vector<int>& sizes, int graph_grid);
   int enqueue_graph_body(int vi, int pi, unsigned flags, int K);
   int check_status();
+  int update_world_layout();
+  bool use_per_world() const;
+  int world_block_now() const;
   int eager_step(int vi, int pi, unsigned flags);
   int graph_step(int vi, int pi, unsigned flags);
   int scan(const int* in, const int* n_dev, int n_host, int* out, int* total, long long n_bound);
@@ -159,14 +174,15 @@ int b2g_batch::alloc() {
   if ((rc = dalloc(&(ptr), (size_t)(n))) != B2G_OK) return rc
   A(d.xf, NB); A(d.pos, NB); A(d.vel, NB); A(d.prop, NB); A(d.prop2, NB); A(d.force, NB);
   A(d.btype, NB); A(d.bworld, NB);
-  A(d.wpar, NW); A(d.wflag, NW); A(d.wshape0, NW); A(d.wbounds, 4 * NW); A(d.wleaf, 2 * NW);
+  A(d.wpar, NW); A(d.wflag, NW); A(d.wshape0, NW); A(d.wshape1, NW); A(d.wbody, NW); A(d.worder, NW);
+  A(d.wcol, (size_t)NW * (kColourSlots + 1)); A(d.wbounds, 4 * NW); A(d.wleaf, 2 * NW);
   A(d.shdr, NS); A(d.geo, 9 * (size_t)NS); A(d.smat, NS); A(d.sfilt, NS); A(d.aabb, NS);
   A(d.joints, NJ); A(d.jsol, NJ); A(d.jorder, NJ); A(d.jisland, NJ);
   A(d.bj_off, NB + 1); A(d.bj_other, 2 * NJ);
   A(d.ckey, NC); A(d.cpair, NC); A(d.cman, 4 * (size_t)NC); A(d.cmat, NC); A(d.ccol, NC);
   A(d.nkey, NC); A(d.npair, NC); A(d.nman, 4 * (size_t)NC); A(d.nmat, NC); A(d.ncol, NC);
   A(d.touch, NC); A(d.tpos, NC); A(d.clist, NC); A(d.ccolour, NC); A(d.csorted, NC);
-  A(d.cisland, NC);
+  A(d.cisland, NC); A(d.cpend, NC);
   for (int k = 0; k < 10; ++k) A(d.vc[k], NC);
   for (int k = 0; k < 5; ++k) A(d.pc[k], NC);
   A(d.col_count, kColourSlots + 1); A(d.col_off, kColourSlots + 1); A(d.col_cursor, kColourSlots + 1);
@@ -176,7 +192,7 @@ int b2g_batch::alloc() {
   const size_t tiles = (size_t)(NS + kSortTile - 1) / kSortTile;
   A(d.sort_hist, kRadix * tiles);
   A(d.node_child, 2 * (size_t)NS); A(d.node_parent, NS); A(d.leaf_parent, NS); A(d.node_range, NS);
-  A(d.node_aabb, NS); A(d.node_visit, NS);
+  A(d.node_rec, 3 * (size_t)NS); A(d.sfast, NS);
   {
     int levels = 1;
     while ((1 << levels) <= NS) ++levels;  // levels 0 .. floor(log2(NS))
@@ -206,14 +222,14 @@ int b2g_batch::alloc() {
 
 void b2g_batch::release() {
   graphs_clear();
-  void* ptrs[] = {d.xf, d.pos, d.vel, d.prop, d.prop2, d.force, d.btype, d.bworld, d.wpar, d.wflag, d.wshape0,
+  void* ptrs[] = {d.xf, d.pos, d.vel, d.prop, d.prop2, d.force, d.btype, d.bworld, d.wpar, d.wflag, d.wshape0, d.wshape1, d.wbody, d.worder, d.wcol,
                   d.wbounds, d.wleaf, d.shdr, d.geo, d.smat, d.sfilt, d.aabb, d.joints, d.jsol,
                   d.jorder, d.jisland, d.bj_off, d.bj_other, d.ckey, d.cpair, d.cman, d.cmat, d.ccol,
                   d.nkey, d.npair, d.nman, d.nmat, d.ncol, d.touch, d.tpos, d.clist, d.ccolour,
-                  d.csorted, d.cisland, d.col_count, d.col_off, d.col_cursor, d.bmask, d.bslot,
+                  d.csorted, d.cisland, d.cpend, d.col_count, d.col_off, d.col_cursor, d.bmask, d.bslot,
                   d.parent, d.minsep, d.jok, d.solved, d.mkey[0], d.mkey[1], d.mval[0], d.mval[1],
-                  d.sort_hist, d.node_child, d.node_parent, d.leaf_parent, d.node_range, d.node_aabb,
-                  d.sparse, d.pslots, d.node_visit, d.pcount, d.poff, d.scan_tiles, d.cnt};
+                  d.sort_hist, d.node_child, d.node_parent, d.leaf_parent, d.node_range, d.node_rec,
+                  d.sfast, d.sparse, d.pslots, d.pcount, d.poff, d.scan_tiles, d.cnt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int k = 0; k < 10; ++k)
@@ -257,6 +273,10 @@ int b2g_batch::broadphase(bool use_given_aabbs) {
     return B2G_OK;
   }
   const int gs = grid_for(ns);
+  if (filters_dirty) {  // bodies, shapes or joints changed: packed pair-filter words
+    hipLaunchKernelGGL(k_shape_filters, dim3(gs), dim3(kBlock), 0, stream, d);
+    filters_dirty = false;
+  }
   hipLaunchKernelGGL(k_world_bounds_reset, dim3(grid_for(d.nw)), dim3(kBlock), 0, stream, d);
   if (!use_given_aabbs) {
     hipLaunchKernelGGL(k_aabb, dim3(gs), dim3(kBlock), 0, stream, d);
@@ -283,13 +303,14 @@ int b2g_batch::broadphase(bool use_given_aabbs) {
       dj.sparse_levels = top + 1;
       hipLaunchKernelGGL(k_sparse_levels, dim3(gs), dim3(kBlock), 0, stream, dj, j);
     }
-    hipLaunchKernelGGL(k_node_aabbs, dim3(grid_for(ns - 1)), dim3(kBlock), 0, stream, d, d.mkey[0]);
+    hipLaunchKernelGGL(k_node_records, dim3(grid_for(ns - 1)), dim3(kBlock), 0, stream, d, d.mkey[0], d.mval[0]);
   }
   tbegin(B2G_KERNEL_PAIRS_WRITE);
-  hipLaunchKernelGGL(k_lbvh_pairs, dim3(gs), dim3(kBlock), 0, stream, d);
+  const int gq = grid_for((long long)ns * kBlock / kQueryBlock);  // blocks of kQueryBlock threads
+  hipLaunchKernelGGL(k_lbvh_pairs, dim3(gq), dim3(kQueryBlock), 0, stream, d);
   tend(B2G_KERNEL_PAIRS_WRITE, ns);
   scan(d.pcount, nullptr, ns, d.poff, d.cnt + C_PAIRS, ns);
-  hipLaunchKernelGGL(k_pairs_compact, dim3(gs), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_pairs_compact, dim3(gq), dim3(kQueryBlock), 0, stream, d);
   hipLaunchKernelGGL(k_merge_contacts, dim3(grid_for(std::max(est_contacts, ns) * 5 / 4)), dim3(kBlock), 0, stream,
                      d);
   hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, stream, d);
@@ -343,6 +364,7 @@ int b2g_batch::rebuild_joint_tables() {
   HIP_TRY(hipMemcpy(d.bj_off, off.data(), (nb + 1) * sizeof(int), hipMemcpyHostToDevice));
   if (off[nb]) HIP_TRY(hipMemcpy(d.bj_other, other.data(), off[nb] * sizeof(int), hipMemcpyHostToDevice));
   joints_dirty = false;
+  filters_dirty = true;
   return B2G_OK;
 }
 
@@ -417,16 +439,19 @@ void b2g_batch::enqueue_islands_and_init() {
 // colours: the slots to sweep; sizes[c] (eager) or -1 (graph: unknown)
 void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, const std::vector<int>& sizes,
                               int graph_grid) {
+  // graph mode (sizes unknown): the last colour slot's kernel also runs the
+  // serial group in its last-finishing block
+  const bool tail = !sizes.empty() && sizes[0] < 0;
   auto contact_sweep = [&](int id, auto kernel, auto... extra) {
     for (size_t q = 0; q < colours.size(); ++q) {
       const int c = colours[q];
       const int n = sizes[q];
       tbegin(id);
       if (c == kSerialColour)
-        hipLaunchKernelGGL(kernel, dim3(1), dim3(1), 0, stream, d, c, extra...);
+        hipLaunchKernelGGL(kernel, dim3(1), dim3(1), 0, stream, d, c, extra..., 0);
       else
         hipLaunchKernelGGL(kernel, dim3(n >= 0 ? grid_for(n) : graph_grid), dim3(kBlock), 0, stream, d, c,
-                           extra...);
+                           extra..., (tail && q + 1 == colours.size()) ? 1 : 0);
       tend(id, n);
     }
   };
@@ -462,7 +487,67 @@ int b2g_batch::check_status() {
   return B2G_OK;
 }
 
+// World ranges for the per-world solver: valid when every world's bodies and
+// shapes are contiguous (the layout b2a_batch_from_scenes produces) and the
+// batch has no joints.
+int b2g_batch::update_world_layout() {
+  const int nw = std::max(d.nw, 1), nb = d.nb, ns = d.ns;
+  std::vector<int> b0(nw, INT_MAX), b1(nw, -1), bc(nw, 0), s0(nw, INT_MAX), s1(nw, -1), sc(nw, 0);
+  for (int b = 0; b < nb; ++b) {
+    const int w = h_bworld[b];
+    if (w < 0 || w >= nw) continue;
+    b0[w] = std::min(b0[w], b);
+    b1[w] = std::max(b1[w], b);
+    ++bc[w];
+  }
+  for (int i = 0; i < ns; ++i) {
+    const int w = h_bworld[h_shape_body[i]];
+    s0[w] = std::min(s0[w], i);
+    s1[w] = std::max(s1[w], i);
+    ++sc[w];
+  }
+  bool ok = d.nj == 0;
+  std::vector<int2> wb(nw);
+  std::vector<int> ws1(nw), order(nw);
+  max_world_bodies = 0;
+  for (int w = 0; w < nw; ++w) {
+    if (bc[w] && b1[w] - b0[w] + 1 != bc[w]) ok = false;
+    if (sc[w] && s1[w] - s0[w] + 1 != sc[w]) ok = false;
+    wb[w] = bc[w] ? make_int2(b0[w], b1[w] + 1) : make_int2(0, 0);
+    ws1[w] = sc[w] ? s1[w] + 1 : h_wshape0[w];
+    max_world_bodies = std::max(max_world_bodies, bc[w]);
+    order[w] = w;
+  }
+  // largest worlds first: their workgroups start before the small ones fill in
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return bc[a] > bc[b]; });
+  HIP_TRY(hipMemcpy(d.wbody, wb.data(), nw * sizeof(int2), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d.wshape1, ws1.data(), nw * sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d.worder, order.data(), nw * sizeof(int), hipMemcpyHostToDevice));
+  per_world_ok = ok;
+  layout_dirty = false;
+  return B2G_OK;
+}
+
+bool b2g_batch::use_per_world() const {
+  if (!per_world_ok || solver_mode == B2G_SOLVER_COLOURS) return false;
+  if (solver_mode == B2G_SOLVER_WORLDS) return true;
+  return d.nw >= 2 && max_world_bodies <= 8192;
+}
+
+static const int kWorldBlocks[] = {64, 128, 256, 512, 1024};
+constexpr int kWorldBlockCount = 5;
+
+int b2g_batch::world_block_now() const {
+  if (world_block > 0) return world_block;
+  if (tuned_block > 0) return tuned_block;
+  return kWorldBlocks[std::min(tune_step / 2, kWorldBlockCount - 1)];
+}
+
 int b2g_batch::step(int vi, int pi, unsigned flags) {
+  if (layout_dirty) {
+    int rc = update_world_layout();
+    if (rc) return rc;
+  }
   if (joints_dirty) {
     int rc = rebuild_joint_tables();
     if (rc) return rc;
@@ -475,7 +560,8 @@ int b2g_batch::step(int vi, int pi, unsigned flags) {
   }
   const bool prof_on = (flags & B2G_STEP_PROFILE) != 0;
   const bool graph_ok = (flags & B2G_STEP_GRAPH) && !prof_on && timed_kernel < 0 &&
-                        !(flags & B2G_STEP_FIND_NEW_CONTACTS) && graph_K > 0;
+                        !(flags & B2G_STEP_FIND_NEW_CONTACTS) && graph_K > 0 &&
+                        (!use_per_world() || world_block > 0 || tuned_block > 0);
   if (graph_ok) return graph_step(vi, pi, flags);
   return eager_step(vi, pi, flags);
 }
@@ -489,10 +575,69 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
   }
   enqueue_collide_and_colour(kMaxColours, kColourRoundsPerStep);
   if (prof_on) HIP_TRY(hipEventRecord(ev[1], stream));
+  int rc;
+  if (use_per_world()) {
+    last_path = 2;
+    if ((rc = read_counters())) return rc;
+    for (int extra = 0; pinned[C_REMAIN] > 0; extra += 2) {  // colour everything first
+      if (extra > 4096) return fail(B2G_ERR_CAPACITY, "colouring did not converge");
+      enqueue_colour_round(kMaxColours);
+      enqueue_colour_round(kMaxColours);
+      if ((rc = read_counters())) return rc;
+    }
+    est_cons = pinned[C_CONS];
+    hipLaunchKernelGGL(k_world_colour_sort, dim3(d.nw), dim3(kBlock), 0, stream, d, rounds_pending);
+    round_host += (unsigned)rounds_pending;
+    rounds_pending = 0;
+    enqueue_islands_and_init();
+    const bool tuning = world_block == 0 && tuned_block == 0;
+    const int wblock = world_block_now();
+    if (tuning) HIP_TRY(hipEventRecord(ev[5], stream));
+    tbegin(B2G_KERNEL_SOLVER);
+    hipLaunchKernelGGL(k_solve_worlds, dim3(d.nw), dim3(wblock), 0, stream, d, vi, pi);
+    tend(B2G_KERNEL_SOLVER, est_cons);
+    if (tuning) {
+      HIP_TRY(hipEventRecord(ev[6], stream));
+      HIP_TRY(hipEventSynchronize(ev[6]));
+      float ms = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&ms, ev[5], ev[6]));
+      const int cand = tune_step / 2;
+      tune_ms[cand] = (tune_step & 1) ? std::min(tune_ms[cand], ms) : ms;
+      if (++tune_step == 2 * kWorldBlockCount) {
+        int best = 0;
+        for (int c = 1; c < kWorldBlockCount; ++c)
+          if (tune_ms[c] < tune_ms[best]) best = c;
+        tuned_block = kWorldBlocks[best];
+      }
+    }
+    graph_K = kMaxColours;
+    if (prof_on) HIP_TRY(hipEventRecord(ev[2], stream));
+    if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
+      if ((rc = broadphase(false))) return rc;
+    }
+    if (prof_on) HIP_TRY(hipEventRecord(ev[3], stream));
+    if (tev_used > 64) tcollect();
+    if (!(flags & B2G_STEP_ASYNC) || prof_on) {
+      if ((rc = read_counters())) return rc;
+      if ((rc = check_status())) return rc;
+    }
+    if (prof_on) {
+      float t01, t12, t23, t03;
+      HIP_TRY(hipEventElapsedTime(&t01, ev[0], ev[1]));
+      HIP_TRY(hipEventElapsedTime(&t12, ev[1], ev[2]));
+      HIP_TRY(hipEventElapsedTime(&t23, ev[2], ev[3]));
+      HIP_TRY(hipEventElapsedTime(&t03, ev[0], ev[3]));
+      prof.collide = t01;
+      prof.solve = t12 + t23;
+      prof.broadphase = t23;
+      prof.step = t03;
+    }
+    return B2G_OK;
+  }
+  last_path = 1;
   enqueue_colour_groups();
   enqueue_islands_and_init();
   HIP_TRY(hipMemcpyAsync(pinned + 32, d.col_off, (kColourSlots + 1) * sizeof(int), hipMemcpyDeviceToHost, stream));
-  int rc;
   if ((rc = read_counters())) return rc;
   est_cons = pinned[C_CONS];
   for (int extra = 0; pinned[C_REMAIN] > 0; extra += 2) {  // first steps / bursts of new contacts
@@ -519,8 +664,8 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
     if (c < kMaxColours) normal_colours = c + 1;
   }
   last_ncolours = pinned[C_NCOLOURS];
-  // colour limit for captured steps: what this step needed plus headroom
-  graph_K = std::min(kMaxColours, normal_colours + 2);
+  // colour limit for captured steps: what this step needed plus one
+  graph_K = std::min(kMaxColours, std::max(normal_colours, 1) + 1);
   enqueue_solve(vi, pi, colours, sizes, 0);
   if (prof_on) HIP_TRY(hipEventRecord(ev[2], stream));
   if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
@@ -551,6 +696,19 @@ int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
   enqueue_collide_and_colour(K, kColourRoundsPerStep);
   hipLaunchKernelGGL(k_colour_leftover, dim3(grid_for(std::max(est_cons, 256) * 5 / 4)), dim3(kBlock), 0, stream,
                      d);
+  if (use_per_world()) {
+    hipLaunchKernelGGL(k_world_colour_sort, dim3(d.nw), dim3(kBlock), 0, stream, d, rounds_pending);
+    round_host += (unsigned)rounds_pending;
+    rounds_pending = 0;
+    enqueue_islands_and_init();
+    hipLaunchKernelGGL(k_solve_worlds, dim3(d.nw), dim3(world_block_now()), 0, stream, d, vi, pi);
+    if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
+      int rc = broadphase(false);
+      if (rc) return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(pinned, d.cnt, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, stream));
+    return B2G_OK;
+  }
   enqueue_colour_groups();
   enqueue_islands_and_init();
   std::vector<int> colours, sizes;
@@ -558,8 +716,6 @@ int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
     colours.push_back(c);
     sizes.push_back(-1);
   }
-  colours.push_back(kSerialColour);
-  sizes.push_back(-1);
   const int per_colour = std::max(256, (int)(std::max(est_cons, 256) * 2LL / std::max(K, 1)));
   enqueue_solve(vi, pi, colours, sizes, grid_for(per_colour));
   if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
@@ -572,9 +728,17 @@ int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
 
 int b2g_batch::graph_step(int vi, int pi, unsigned flags) {
   const unsigned key_flags = flags & B2G_STEP_SKIP_BROADPHASE;
+  const int block = use_per_world() ? world_block_now() : 0;
+  last_path = block ? 2 : 1;
+  if (filters_dirty) {  // refreshed outside the captured sequence
+    hipLaunchKernelGGL(k_shape_filters, dim3(grid_for(d.ns)), dim3(kBlock), 0, stream, d);
+    filters_dirty = false;
+  }
   GraphEntry* ge = nullptr;
   for (auto& g : graphs)
-    if (g.parity == parity && g.vi == vi && g.pi == pi && g.K == graph_K && g.flags == key_flags) ge = &g;
+    if (g.parity == parity && g.vi == vi && g.pi == pi && g.K == graph_K && g.flags == key_flags &&
+        g.block == block)
+      ge = &g;
   if (!ge) {
     if (graphs.size() >= 8) graphs_clear();
     const int parity0 = parity;
@@ -595,7 +759,7 @@ int b2g_batch::graph_step(int vi, int pi, unsigned flags) {
     // capture only recorded the launches: restore the host view of the state
     if (parity != parity0) swap_contact_buffers();
     round_host = round0;
-    graphs.push_back(GraphEntry{exec, parity0, vi, pi, graph_K, key_flags});
+    graphs.push_back(GraphEntry{exec, parity0, vi, pi, graph_K, key_flags, block});
     ge = &graphs.back();
   }
   HIP_TRY(hipGraphLaunch(ge->exec, stream));
@@ -608,7 +772,7 @@ int b2g_batch::graph_step(int vi, int pi, unsigned flags) {
     int rc = check_status();
     if (rc) return rc;
     // constraints pushed to the serial group: raise the colour limit
-    if (pinned[C_SERIAL] > 32 && graph_K < kMaxColours) graph_K = std::min(kMaxColours, graph_K + 2);
+    if (pinned[C_SERIAL] > 32 && graph_K < kMaxColours) graph_K = std::min(kMaxColours, graph_K + 1);
   }
   return B2G_OK;
 }
@@ -713,6 +877,8 @@ int b2g_set_world_params(b2g_batch* b, int32_t first, int32_t count, const b2g_w
   if (first + count > b->d.nw) {  // the world count is part of captured launches
     b->graphs_clear();
     b->graph_K = 0;
+    b->filters_dirty = true;
+    b->layout_dirty = true;
   }
   std::vector<float4> wp(count);
   std::vector<int> wf(count);
@@ -735,6 +901,8 @@ int b2g_set_worlds(b2g_batch* b, int32_t count, const int32_t* body_begin, const
   if (!b || count < 1 || count > b->cap.worlds) return fail(B2G_ERR_INVALID, "world count");
   b->graphs_clear();
   b->graph_K = 0;
+    b->filters_dirty = true;
+    b->layout_dirty = true;
   b->d.nw = count;
   return b2g_set_world_params(b, 0, count, params);
 }
@@ -742,6 +910,8 @@ int b2g_set_worlds(b2g_batch* b, int32_t count, const int32_t* body_begin, const
 int b2g_upload_bodies(b2g_batch* b, int32_t first, int32_t count, const b2g_body* in) {
   if (!b || first < 0 || count < 0 || first + count > b->cap.bodies) return fail(B2G_ERR_INVALID, "body range");
   if (count == 0) return B2G_OK;
+  b->filters_dirty = true;  // body types feed the pair filter
+  if (first + count > b->d.nb) b->layout_dirty = true;
   std::vector<float4> xf(count), pos(count), vel(count), prop(count), prop2(count), force(count);
   std::vector<int> type(count), world(count);
   for (int i = 0; i < count; ++i) {
@@ -755,6 +925,7 @@ int b2g_upload_bodies(b2g_batch* b, int32_t first, int32_t count, const b2g_body
     force[i] = make_float4(s.force[0], s.force[1], s.torque, 0.0f);
     type[i] = s.type;
     world[i] = s.world;
+    if (b->h_bworld[first + i] != s.world) b->layout_dirty = true;
     b->h_btype[first + i] = s.type;
     b->h_bworld[first + i] = s.world;
   }
@@ -772,6 +943,8 @@ int b2g_upload_bodies(b2g_batch* b, int32_t first, int32_t count, const b2g_body
   if (first + count > d.nb) {
     b->graphs_clear();
     b->graph_K = 0;
+    b->filters_dirty = true;
+    b->layout_dirty = true;
   }
   if (first + count > d.nb) {
     // extend the joint-partner CSR with empty rows
@@ -791,6 +964,8 @@ int b2g_upload_shapes(b2g_batch* b, int32_t first, int32_t count, const b2g_shap
   if (!b || first < 0 || count < 0 || first + count > b->cap.shapes) return fail(B2G_ERR_INVALID, "shape range");
   b->graphs_clear();  // captured launch sizes no longer match
   b->graph_K = 0;
+    b->filters_dirty = true;
+    b->layout_dirty = true;
   if (count == 0) return B2G_OK;
   std::vector<int4> hdr(count), filt(count);
   std::vector<float4> geo(9 * (size_t)count), mat(count);
@@ -833,6 +1008,8 @@ int b2g_upload_joints(b2g_batch* b, int32_t first, int32_t count, const b2g_revo
   if (!b || first < 0 || count < 0 || first + count > b->cap.joints) return fail(B2G_ERR_INVALID, "joint range");
   b->graphs_clear();  // captured launch sizes no longer match
   b->graph_K = 0;
+    b->filters_dirty = true;
+    b->layout_dirty = true;
   if (count == 0) return B2G_OK;
   if ((int)b->h_joints.size() < first + count) b->h_joints.resize(first + count);
   std::vector<Joint> dj(count);
@@ -864,6 +1041,8 @@ int b2g_upload_contacts(b2g_batch* b, int32_t count, const b2g_contact* in) {
   if (!b || count < 0 || count > b->cap.contacts) return fail(B2G_ERR_INVALID, "contact count");
   b->graphs_clear();  // captured launch sizes no longer match
   b->graph_K = 0;
+    b->filters_dirty = true;
+    b->layout_dirty = true;
   std::vector<int> idx(count);
   std::vector<uint64_t> keys(count);
   for (int i = 0; i < count; ++i) {
@@ -931,6 +1110,45 @@ int b2g_get_counts(b2g_batch* b, b2g_counts* out) {
   out->status = b->pinned[C_STATUS];
   out->serial = b->pinned[C_SERIAL];
   out->graph_steps = (int32_t)std::min<long long>(b->graph_steps, INT32_MAX);
+  return B2G_OK;
+}
+
+int b2g_set_option(b2g_batch* b, int32_t option, int32_t value) {
+  if (!b) return fail(B2G_ERR_INVALID, "null batch");
+  switch (option) {
+    case B2G_OPT_SOLVER:
+      if (value < B2G_SOLVER_AUTO || value > B2G_SOLVER_WORLDS) return fail(B2G_ERR_INVALID, "solver mode");
+      b->solver_mode = value;
+      break;
+    case B2G_OPT_WORLD_BLOCK:
+      if (value != 0 && (value < 64 || value > 1024 || (value & 63))) {
+        return fail(B2G_ERR_INVALID, "world block: 0 (autotune) or a multiple of 64 in [64, 1024]");
+      }
+      b->world_block = value;
+      b->tuned_block = 0;
+      b->tune_step = 0;
+      break;
+    default:
+      return fail(B2G_ERR_INVALID, "unknown option");
+  }
+  b->graphs_clear();
+  return B2G_OK;
+}
+
+int b2g_get_option(b2g_batch* b, int32_t option, int32_t* value) {
+  if (!b || !value) return fail(B2G_ERR_INVALID, "null argument");
+  if (b->layout_dirty) {
+    int rc = b->update_world_layout();
+    if (rc) return rc;
+  }
+  switch (option) {
+    case B2G_OPT_SOLVER: *value = b->solver_mode; break;
+    case B2G_OPT_WORLD_BLOCK: *value = b->world_block; break;
+    case B2G_OPT_TUNED_WORLD_BLOCK: *value = b->world_block > 0 ? b->world_block : b->tuned_block; break;
+    case B2G_OPT_LAST_SOLVER: *value = b->last_path; break;
+    case B2G_OPT_PER_WORLD_ELIGIBLE: *value = b->per_world_ok ? 1 : 0; break;
+    default: return fail(B2G_ERR_INVALID, "unknown option");
+  }
   return B2G_OK;
 }
 

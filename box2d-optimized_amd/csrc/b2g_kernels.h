@@ -44,6 +44,8 @@ enum Counter : int {
   C_MAXCOLOUR = 6,
   C_ROUND = 7,      // colouring round base (claim priorities carry base + local round)
   C_SERIAL = 8,     // constraints placed in the serial group by the last step
+  C_PENDING = 9,    // entries of the pending-colour list of this step
+  C_TICKET = 10,    // blocks finished in a sweep's last colour kernel (serial tail)
   C_COUNT = 16
 };
 enum Status : int { S_CONTACT_OVERFLOW = 1, S_STACK_OVERFLOW = 2, S_TREE_CORRUPT = 4 };
@@ -68,6 +70,10 @@ struct Dev {
   float4* wpar;   // gravity.x, gravity.y, dt, dtRatio
   int* wflag;     // bit0 warm starting, bit1 active
   int* wshape0;   // first shape index of each world (world-local shape numbering)
+  int* wshape1;   // one past the last shape index of each world
+  int2* wbody;    // body range of each world (worlds laid out world after world)
+  int* worder;    // worlds by decreasing size: the per-world solver's block order
+  int* wcol;      // per world: kColourSlots + 1 slot offsets of its colour groups
   // shapes
   int4* shdr;     // type, body, count, world
   float4* geo;    // 9 per shape
@@ -95,6 +101,7 @@ struct Dev {
   float4* vc[10];
   float4* pc[5];
   int* cisland;
+  int* cpend;     // constraints (compacted slots) without a colour at the start of this step
   unsigned long long* bmask;
   unsigned long long* bslot;
   // islands
@@ -109,11 +116,11 @@ struct Dev {
   int* node_parent;
   int* leaf_parent;
   int2* node_range;
-  float4* node_aabb;
+  float4* node_rec;  // 3 per internal node: left child box, right child box, (left, right, split, -)
+  int4* sfast;       // per shape: body, type | dynamic << 4 | joint partners << 5, category | mask << 16, group
   float4* sparse;    // (levels) x ns AABB table over sorted leaves; level 0 = leaf AABBs
   int sparse_levels;
   uint64_t* pslots;  // kPairSlots per shape: pairs found by the traversal, sorted
-  int* node_visit;
   int* wleaf;        // 2 per world: first leaf, last leaf + 1
   unsigned* wbounds; // 4 per world (ordered floats): min x, min y, max x, max y
   int* pcount;       // per shape pair count
@@ -148,6 +155,17 @@ __device__ __forceinline__ void wave_atomic_add(int* addr, int per_lane) {
   const uint64_t act = __ballot(1);
   const int leader = __ffsll((long long)act) - 1;
   if (lane_id() == leader) atomicAdd(addr, per_lane * __popcll(act));
+}
+
+// slot of this lane in an append to a device list, from the calling lanes
+// (one atomic per wave); lanes get consecutive slots in lane order
+__device__ __forceinline__ int wave_append(int* counter) {
+  const uint64_t act = __ballot(1);
+  const int leader = __ffsll((long long)act) - 1;
+  int base = 0;
+  if (lane_id() == leader) base = atomicAdd(counter, __popcll(act));
+  base = __shfl(base, leader, kWave);
+  return base + __popcll(act & lanemask_lt());
 }
 
 __device__ __forceinline__ float wave_min_all(float v) {
@@ -261,7 +279,12 @@ __global__ void __launch_bounds__(kBlock) k_compact_touching(Dev d) {
 __global__ void __launch_bounds__(kBlock) k_integrate_velocities(Dev d) {
   if (blockIdx.x == 0) {  // colour counters of this step
     for (int c = threadIdx.x; c < kColourSlots; c += blockDim.x) d.col_count[c] = 0;
-    if (threadIdx.x == 0) d.cnt[C_REMAIN] = 0;
+    if (threadIdx.x == 0) {
+      d.cnt[C_REMAIN] = 0;
+      d.cnt[C_PENDING] = 0;
+      d.cnt[C_NCOLOURS] = 0;
+      d.cnt[C_SERIAL] = 0;
+    }
   }
   B2G_FOR(b, 0, d.nb) {
     d.parent[b] = b;
@@ -311,6 +334,7 @@ __global__ void __launch_bounds__(kBlock) k_colour_init(Dev d, int K) {
     } else {
       d.ccolour[k] = -1;
       wave_atomic_add(&d.cnt[C_REMAIN], 1);
+      d.cpend[wave_append(&d.cnt[C_PENDING])] = k;
     }
   }
 }
@@ -338,8 +362,9 @@ __device__ __forceinline__ unsigned long long colour_priority(const Dev& d, int 
 __global__ void __launch_bounds__(kBlock) k_colour_claim(Dev d, int local_round) {
   if (d.cnt[C_REMAIN] == 0) return;
   const int round = d.cnt[C_ROUND] + local_round;
-  const int n = d.cnt[C_CONS];
-  B2G_FOR(k, 0, n) {
+  const int n = d.cnt[C_PENDING];
+  B2G_FOR(q, 0, n) {
+    const int k = d.cpend[q];
     if (d.ccolour[k] >= 0) continue;
     const int c = d.clist[k];
     const int2 pr = d.cpair[c];
@@ -353,8 +378,9 @@ __global__ void __launch_bounds__(kBlock) k_colour_claim(Dev d, int local_round)
 __global__ void __launch_bounds__(kBlock) k_colour_resolve(Dev d, int local_round, int K) {
   if (d.cnt[C_REMAIN] == 0) return;
   const int round = d.cnt[C_ROUND] + local_round;
-  const int n = d.cnt[C_CONS];
-  B2G_FOR(k, 0, n) {
+  const int n = d.cnt[C_PENDING];
+  B2G_FOR(q, 0, n) {
+    const int k = d.cpend[q];
     if (d.ccolour[k] >= 0) continue;
     const int c = d.clist[k];
     const int2 pr = d.cpair[c];
@@ -409,8 +435,9 @@ __global__ void k_clear_pairs(Dev d) {
 // Gauss-Seidel order).
 __global__ void __launch_bounds__(kBlock) k_colour_leftover(Dev d) {
   if (d.cnt[C_REMAIN] == 0) return;
-  const int n = d.cnt[C_CONS];
-  B2G_FOR(k, 0, n) {
+  const int n = d.cnt[C_PENDING];
+  B2G_FOR(q, 0, n) {
+    const int k = d.cpend[q];
     if (d.ccolour[k] >= 0) continue;
     d.ccolour[k] = kSerialColour;
   }
@@ -479,13 +506,25 @@ __global__ void __launch_bounds__(kBlock) k_colour_clear_counts(Dev d) {
 // Islands: connected components over non-static bodies (b2World::Solve DFS,
 // src/dynamics/b2_world.cpp:526-659), by lock-free union-find.
 // ===========================================================================
+// Parent links always point to a smaller index (the union links the larger
+// root under the smaller), so every walk below strictly decreases and ends.
+// Loads are agent-scope atomics: a root linked by another CU (possibly on
+// another XCD) is seen at the next read, not served from a stale cache line.
+__device__ __forceinline__ int uf_load(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// find with path halving (each visited node skips to its grandparent, still
+// an ancestor in the same set)
 __device__ __forceinline__ int uf_find(int* parent, int x) {
-  int p = parent[x];
-  while (p != x) {
-    x = p;
-    p = parent[x];
+  for (;;) {
+    const int p = uf_load(parent + x);
+    if (p == x) return x;
+    const int gp = uf_load(parent + p);
+    if (gp == p) return p;
+    __hip_atomic_store(parent + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    x = gp;
   }
-  return x;
 }
 
 __device__ __forceinline__ void uf_union(int* parent, int a, int b) {
@@ -690,13 +729,12 @@ __global__ void __launch_bounds__(kBlock) k_contact_init(Dev d) {
 
 __device__ __forceinline__ bool writes_body(const Dev& d, int b) { return d.btype[b] == kDynamicBody; }
 
-// b2ContactSolver::WarmStart (:260-299) for slots [begin, end)
-__global__ void __launch_bounds__(kBlock) k_warm_start(Dev d, int colour) {
-  const int begin = d.col_off[colour], end = d.col_off[colour + 1];
-  B2G_FOR(s, begin, end) {
+// b2ContactSolver::WarmStart (:260-299) of slot s
+__device__ __forceinline__ void warm_start(const Dev& d, int s) {
+  {
     const float4 v2i = d.vc[2][s];
     const int ia = as_int(v2i.x), ib = as_int(v2i.y), count = as_int(v2i.z);
-    if ((d.wflag[d.bworld[ia]] & 1) == 0) continue;
+    if ((d.wflag[d.bworld[ia]] & 1) == 0) return;
     const float4 n4 = d.vc[0][s], m = d.vc[1][s];
     const vec2 normal = mk(n4.x, n4.y), tangent = cross(normal, 1.0f);
     float4 va = d.vel[ia], vb = d.vel[ib];
@@ -820,9 +858,40 @@ __device__ __forceinline__ void contact_velocity(const Dev& d, int s) {
   if (writes_body(d, ib)) d.vel[ib] = make_float4(vB.x, vB.y, wB, 0.0f);
 }
 
-__global__ void __launch_bounds__(kBlock) k_contact_velocity(Dev d, int colour) {
+// Serial tail: when a sweep's last colour kernel is launched with
+// serial_tail set, the block that finishes last runs the serial group on one
+// lane, so a captured step needs no separate one-lane launch per sweep. Every
+// thread publishes its writes (agent-scope fence) before the block takes a
+// ticket; the last ticket holder fences again before reading. All threads of
+// every block must call this.
+__device__ __forceinline__ bool serial_tail_block(const Dev& d) {
+  __shared__ int last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int t = atomicAdd(&d.cnt[C_TICKET], 1);
+    last = (t == (int)gridDim.x - 1) ? 1 : 0;
+    if (last) {
+      atomicExch(&d.cnt[C_TICKET], 0);
+      __threadfence();
+    }
+  }
+  __syncthreads();
+  return last != 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_warm_start(Dev d, int colour, int serial_tail) {
+  const int begin = d.col_off[colour], end = d.col_off[colour + 1];
+  B2G_FOR(s, begin, end) warm_start(d, s);
+  if (serial_tail && serial_tail_block(d) && threadIdx.x == 0)
+    for (int s = d.col_off[kSerialColour]; s < d.col_off[kSerialColour + 1]; ++s) warm_start(d, s);
+}
+
+__global__ void __launch_bounds__(kBlock) k_contact_velocity(Dev d, int colour, int serial_tail) {
   const int begin = d.col_off[colour], end = d.col_off[colour + 1];
   B2G_FOR(s, begin, end) contact_velocity(d, s);
+  if (serial_tail && serial_tail_block(d) && threadIdx.x == 0)
+    for (int s = d.col_off[kSerialColour]; s < d.col_off[kSerialColour + 1]; ++s) contact_velocity(d, s);
 }
 
 // b2ContactSolver::StoreImpulses (:641-657): only the vc pointCount points
@@ -952,14 +1021,18 @@ __device__ __forceinline__ float contact_position(const Dev& d, int s, int isl) 
   return min_sep;
 }
 
-__global__ void __launch_bounds__(kBlock) k_contact_position(Dev d, int colour, int it) {
+__device__ __forceinline__ void contact_position_serial(const Dev& d, int begin, int end, int it) {
+  for (int s = begin; s < end; ++s) {
+    const int isl = d.cisland[s];
+    const float m = contact_position(d, s, isl);
+    atomicMin(&d.minsep[(it & 1) * d.nb + isl], float_to_ordered(m));
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_contact_position(Dev d, int colour, int it, int serial_tail) {
   const int begin = d.col_off[colour], end = d.col_off[colour + 1];
   if (blockDim.x == 1) {  // the serial colour group: one lane, slot order
-    for (int s = begin; s < end; ++s) {
-      const int isl = d.cisland[s];
-      const float m = contact_position(d, s, isl);
-      atomicMin(&d.minsep[(it & 1) * d.nb + isl], float_to_ordered(m));
-    }
+    contact_position_serial(d, begin, end, it);
     return;
   }
   B2G_FOR_ALL(s, valid, begin, end) {
@@ -967,6 +1040,8 @@ __global__ void __launch_bounds__(kBlock) k_contact_position(Dev d, int colour, 
     const float m = valid ? contact_position(d, s, isl) : 0.0f;
     wave_atomic_min_keyed(d.minsep + (it & 1) * d.nb, isl, valid != 0, m);
   }
+  if (serial_tail && serial_tail_block(d) && threadIdx.x == 0)
+    contact_position_serial(d, d.col_off[kSerialColour], d.col_off[kSerialColour + 1], it);
 }
 
 // ===========================================================================
@@ -1195,6 +1270,169 @@ __global__ void __launch_bounds__(kBlock) k_finish_bodies(Dev d) {
 }
 
 // ===========================================================================
+// Per-world solver: one workgroup runs a whole world's solve (b2Island::Solve,
+// src/dynamics/b2_island.cpp:253-438, with the contact and position phases
+// of b2ContactSolver) with workgroup barriers between colours instead of one
+// launch per colour. Constraint slots are world-major (each world's
+// constraints are contiguous, grouped by colour inside), so a world reads one
+// contiguous slice. Results equal the colour-major path's: constraints of one
+// colour share no body, the serial group keeps its per-world order, and the
+// island minimum separation is an order-free atomic minimum.
+// ===========================================================================
+__device__ __forceinline__ int lower_bound_u64(const uint64_t* a, int n, uint64_t key) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Slots of world w's constraints (world-major, colour-minor): the touching
+// contacts of its shapes are contiguous in the key-sorted contact list and
+// keep their order in the compacted list. Also persists the colours and
+// advances the claim-round base (block 0).
+__global__ void __launch_bounds__(kBlock) k_world_colour_sort(Dev d, int rounds_run) {
+  __shared__ int hist[kColourSlots];
+  __shared__ int off[kColourSlots + 1];
+  __shared__ int cur[kColourSlots];
+  __shared__ int range[2];
+  const int w = d.worder[blockIdx.x];
+  if (threadIdx.x == 0) {
+    if (blockIdx.x == 0) d.cnt[C_ROUND] += rounds_run;
+    const int nc = d.cnt[C_CONTACTS], total = d.cnt[C_CONS];
+    const int c0 = lower_bound_u64(d.ckey, nc, (uint64_t)(uint32_t)d.wshape0[w] << 32);
+    const int c1 = lower_bound_u64(d.ckey, nc, (uint64_t)(uint32_t)d.wshape1[w] << 32);
+    range[0] = c0 < nc ? d.tpos[c0] : total;
+    range[1] = c1 < nc ? d.tpos[c1] : total;
+  }
+  for (int c = threadIdx.x; c < kColourSlots; c += blockDim.x) {
+    hist[c] = 0;
+    cur[c] = 0;
+  }
+  __syncthreads();
+  const int t0 = range[0], t1 = range[1];
+  for (int k = t0 + (int)threadIdx.x; k < t1; k += blockDim.x) {
+    int col = d.ccolour[k];
+    if (col < 0) col = kSerialColour;  // unreachable once colouring has finished
+    atomicAdd(&hist[col], 1);
+    d.ccol[d.clist[k]] = col;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int o = 0, top = 0;
+    for (int c = 0; c < kColourSlots; ++c) {
+      off[c] = o;
+      o += hist[c];
+      if (hist[c]) top = c + 1;
+    }
+    off[kColourSlots] = o;
+    if (top) atomicMax(&d.cnt[C_NCOLOURS], top);
+    if (hist[kSerialColour]) atomicAdd(&d.cnt[C_SERIAL], hist[kSerialColour]);
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c <= kColourSlots; c += blockDim.x) d.wcol[w * (kColourSlots + 1) + c] = t0 + off[c];
+  for (int k = t0 + (int)threadIdx.x; k < t1; k += blockDim.x) {
+    int col = d.ccolour[k];
+    if (col < 0) col = kSerialColour;
+    d.csorted[t0 + off[col] + atomicAdd(&cur[col], 1)] = d.clist[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {  // the serial group in contact order
+    const int b = t0 + off[kSerialColour], e = t0 + off[kSerialColour + 1];
+    for (int a = b + 1; a < e; ++a) {
+      const int v = d.csorted[a];
+      int q = a - 1;
+      while (q >= b && d.csorted[q] > v) {
+        d.csorted[q + 1] = d.csorted[q];
+        --q;
+      }
+      d.csorted[q + 1] = v;
+    }
+  }
+}
+
+// The solve of world worder[blockIdx.x]: warm start, velocity iterations,
+// store impulses + integrate positions, position iterations with the
+// per-island early exit, write-back. Worlds without joints only.
+__global__ void __launch_bounds__(1024) k_solve_worlds(Dev d, int vi, int pi) {
+  __shared__ int off[kColourSlots + 1];
+  const int w = d.worder[blockIdx.x];
+  if (!world_active(d, w)) return;  // whole block
+  for (int c = threadIdx.x; c <= kColourSlots; c += blockDim.x) off[c] = d.wcol[w * (kColourSlots + 1) + c];
+  __syncthreads();
+  const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
+  const int t0 = off[0], t1 = off[kColourSlots];
+  const int2 wb = d.wbody[w];
+  // one colour at a time; the serial group on one lane, in slot order
+  auto sweep = [&](auto fn) {
+    for (int c = 0; c < kColourSlots; ++c) {
+      const int lo = off[c], hi = off[c + 1];
+      if (hi <= lo) continue;  // uniform across the block
+      if (c == kSerialColour) {
+        if (tid == 0)
+          for (int s = lo; s < hi; ++s) fn(s);
+      } else {
+        for (int s = lo + tid; s < hi; s += nt) fn(s);
+      }
+      __syncthreads();
+    }
+  };
+  if (t1 > t0) {
+    sweep([&](int s) { warm_start(d, s); });
+    for (int it = 0; it < vi; ++it) sweep([&](int s) { contact_velocity(d, s); });
+  }
+  for (int s = t0 + tid; s < t1; s += nt) store_impulses(d, s);
+  for (int b = wb.x + tid; b < wb.y; b += nt) integrate_position(d, b);
+  __syncthreads();
+  if (t1 > t0) {
+    for (int it = 0; it < pi; ++it) {
+      for (int c = 0; c < kColourSlots; ++c) {
+        const int lo = off[c], hi = off[c + 1];
+        if (hi <= lo) continue;
+        if (c == kSerialColour) {
+          if (tid == 0)
+            for (int s = lo; s < hi; ++s) {
+              const int isl = d.cisland[s];
+              atomicMin(&d.minsep[(it & 1) * d.nb + isl], float_to_ordered(contact_position(d, s, isl)));
+            }
+        } else {
+          for (int base = lo; base < hi; base += nt) {  // every lane of a wave takes part in the min
+            const int s = base + tid;
+            const bool valid = s < hi;
+            const int isl = valid ? d.cisland[s] : 0;
+            const float m = valid ? contact_position(d, s, isl) : 0.0f;
+            wave_atomic_min_keyed(d.minsep + (it & 1) * d.nb, isl, valid, m);
+          }
+        }
+        __syncthreads();
+      }
+      // per-island early exit (b2_island.cpp:403-408), re-arming the other parity
+      for (int k = t0 + tid; k < t1; k += nt) {
+        const int isl = d.cisland[k];
+        const int nxt = ((it + 1) & 1) * d.nb + isl;
+        d.minsep[nxt] = float_to_ordered(0.0f);
+        d.jok[nxt] = 1;
+        if (d.solved[isl]) continue;
+        const int cur = (it & 1) * d.nb + isl;
+        if (ordered_to_float(d.minsep[cur]) >= -3.0f * kLinearSlop && d.jok[cur] != 0) d.solved[isl] = 1;
+      }
+      __syncthreads();
+    }
+  }
+  // write back: SynchronizeTransform and ClearForces
+  for (int b = wb.x + tid; b < wb.y; b += nt) {
+    if (d.btype[b] != kStaticBody) {
+      const float4 p = d.pos[b], pr = d.prop[b];
+      const rot2 q = rot_of(p.z);
+      const vec2 c = mk(p.x, p.y) - mul(q, mk(pr.z, pr.w));
+      d.xf[b] = make_float4(c.x, c.y, q.s, q.c);
+    }
+    d.force[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+}
+
+// ===========================================================================
 // Broadphase: LBVH over Morton-sorted tight AABBs (the pair set that
 // b2BroadPhase::UpdateAndQuery reports, include/box2d/b2_broad_phase.h:252-511,
 // filtered as b2ContactManager::QueryCallback filters it,
@@ -1382,21 +1620,47 @@ __global__ void __launch_bounds__(kBlock) k_sparse_levels(Dev d, int j) {
   }
 }
 
-// AABB of every internal node from its leaf range; nodes that span several
-// worlds get an unbounded box (the traversal's world-range test skips them
-// for every query outside their worlds).
-__global__ void __launch_bounds__(kBlock) k_node_aabbs(Dev d, const uint64_t* __restrict__ keys) {
+// Node records: every internal node stores both children's boxes, the
+// children (internal index, or ~shape index for a leaf) and its split, so a
+// traversal step is one 48-byte read. Every node covers a contiguous range of
+// sorted leaves, so a child's box is a range union answered in O(1) from the
+// sparse table; a child spanning several worlds gets an unbounded box (the
+// traversal's split test keeps queries inside their own world's leaf range).
+__device__ __forceinline__ float4 range_box(const Dev& d, const uint64_t* keys, int n, int lo, int hi) {
+  if ((keys[lo] >> 32) != (keys[hi] >> 32)) return make_float4(-FLT_MAX, -FLT_MAX, FLT_MAX, FLT_MAX);
+  const int len = hi - lo + 1;
+  const int lev = 31 - __clz(len);
+  const float4* t = d.sparse + (size_t)lev * n;
+  return aabb_union(t[lo], t[hi - (1 << lev) + 1]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_node_records(Dev d, const uint64_t* __restrict__ keys,
+                                                         const uint32_t* __restrict__ vals) {
   const int n = d.ns;
   B2G_FOR(i, 0, n - 1) {
     const int2 r = d.node_range[i];
-    if ((keys[r.x] >> 32) != (keys[r.y] >> 32)) {
-      d.node_aabb[i] = make_float4(-FLT_MAX, -FLT_MAX, FLT_MAX, FLT_MAX);
-      continue;
-    }
-    const int len = r.y - r.x + 1;
-    const int lev = 31 - __clz(len);
-    const float4* t = d.sparse + (size_t)lev * n;
-    d.node_aabb[i] = aabb_union(t[r.x], t[r.y - (1 << lev) + 1]);
+    const int lc = d.node_child[2 * i], rc = d.node_child[2 * i + 1];
+    const int split = lc < 0 ? ~lc : d.node_range[lc].y;
+    const float4 lb = range_box(d, keys, n, r.x, split);
+    const float4 rb = range_box(d, keys, n, split + 1, r.y);
+    const int lcode = lc < 0 ? ~(int)vals[~lc] : lc;
+    const int rcode = rc < 0 ? ~(int)vals[~rc] : rc;
+    float4* o = d.node_rec + 3 * (size_t)i;
+    o[0] = lb;
+    o[1] = rb;
+    o[2] = make_float4(__int_as_float(lcode), __int_as_float(rcode), __int_as_float(split), 0.0f);
+  }
+}
+
+// Per-shape filter word for the pair test (one 16-byte read per candidate).
+__global__ void __launch_bounds__(kBlock) k_shape_filters(Dev d) {
+  B2G_FOR(i, 0, d.ns) {
+    const int4 h = d.shdr[i];
+    const int4 f = d.sfilt[i];
+    const int b = h.y;
+    const int dyn = d.btype[b] == kDynamicBody ? 1 : 0;
+    const int jp = d.bj_off[b + 1] > d.bj_off[b] ? 1 : 0;
+    d.sfast[i] = make_int4(b, (h.x & 15) | (dyn << 4) | (jp << 5), (f.x & 0xffff) | ((f.y & 0xffff) << 16), f.z);
   }
 }
 
@@ -1404,61 +1668,72 @@ __device__ __forceinline__ bool overlap(float4 a, float4 b) {  // b2TestOverlap,
   return (a.z >= b.x) & (a.x <= b.z) & (a.w >= b.y) & (a.y <= b.w);
 }
 
-// b2Body::ShouldCollide + b2ContactFilter::ShouldCollide + factory existence
-__device__ __forceinline__ bool pair_allowed(const Dev& d, int i, int j) {
-  const int4 hi = d.shdr[i], hj = d.shdr[j];
-  const int ba = hi.y, bb = hj.y;
-  if (ba == bb) return false;
-  if (d.btype[ba] != kDynamicBody && d.btype[bb] != kDynamicBody) return false;
-  if (!has_collide_function(hi.x, hj.x) && !has_collide_function(hj.x, hi.x)) return false;
-  const int4 fa = d.sfilt[i], fb = d.sfilt[j];
-  if (fa.z == fb.z && fa.z != 0) {
-    if (fa.z <= 0) return false;
-  } else if ((fa.y & fb.x) == 0 || (fa.x & fb.y) == 0) {
-    return false;
+// b2Body::ShouldCollide (src/dynamics/b2_body.cpp:459-480) +
+// b2ContactFilter::ShouldCollide (src/dynamics/b2_world_callbacks.cpp:28-40)
+// + existence of a collide function (src/dynamics/b2_contact.cpp:48-77), on
+// the packed per-shape filter words
+__device__ __forceinline__ bool pair_allowed(const Dev& d, int4 fa, int4 fb) {
+  if (fa.x == fb.x) return false;                          // same body
+  if (((fa.y | fb.y) & 16) == 0) return false;             // neither body dynamic
+  const int ta = fa.y & 15, tb = fb.y & 15;
+  if (!has_collide_function(ta, tb) && !has_collide_function(tb, ta)) return false;
+  if (fa.w == fb.w && fa.w != 0) {
+    if (fa.w <= 0) return false;
+  } else {
+    const int cata = fa.z & 0xffff, maska = (int)((unsigned)fa.z >> 16);
+    const int catb = fb.z & 0xffff, maskb = (int)((unsigned)fb.z >> 16);
+    if ((maska & catb) == 0 || (cata & maskb) == 0) return false;
   }
-  for (int e = d.bj_off[ba]; e < d.bj_off[ba + 1]; ++e)
-    if (d.bj_other[e] == bb) return false;
+  if ((fa.y | fb.y) & 32) {  // a joint with collideConnected == false between the bodies
+    for (int e = d.bj_off[fa.x]; e < d.bj_off[fa.x + 1]; ++e)
+      if (d.bj_other[e] == fb.x) return false;
+  }
   return true;
 }
 
 constexpr int kStack = 64;
 constexpr int kPairSlots = 16;
+constexpr int kQueryBlock = 128;  // LDS traversal stack: kStack x kQueryBlock ints (32 KB)
 
-// Traversal for shape i: pairs (i, j > i) of overlapping, collidable shapes.
-// Found keys are kept sorted in i's slots (kPairSlots); the count can exceed
-// the slots, in which case k_pairs_compact traverses again for that shape.
-template <bool kDirect>
-__device__ __forceinline__ int lbvh_query(const Dev& d, int i, uint64_t* out, int cap) {
+// Traversal for shape i: pairs (i, j > i) of overlapping, collidable shapes
+// of its world. The stack lives in LDS, thread-minor (stk[level * blockDim +
+// thread]), so lanes at different depths do not conflict. Found keys are kept
+// sorted in `out`; the count can exceed `cap`, in which case k_pairs_compact
+// traverses again for that shape straight into the pair list.
+__device__ __forceinline__ int lbvh_query(const Dev& d, int i, uint64_t* out, int cap, int* stk) {
   const int n = d.ns;
   if (n < 2) return 0;
   const float4 q = d.aabb[i];
+  const int4 fi = d.sfast[i];
   const int w = d.shdr[i].w;
   const int wlo = d.wleaf[2 * w], whi = d.wleaf[2 * w + 1] - 1;
+  const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
   int found = 0;
-  int stack[kStack];
   int sp = 0;
-  stack[sp++] = 0;
+  stk[tid] = 0;
+  sp = 1;
   int visits = 0;
   while (sp > 0) {
-    const int node = stack[--sp];
-    if (++visits > 2 * n) {  // a tree has n - 1 internal nodes
+    const int node = stk[--sp * nt + tid];
+    if (++visits > n) {  // a tree has n - 1 internal nodes
       atomicOr(&d.cnt[C_STATUS], S_TREE_CORRUPT);
       break;
     }
+    const float4* rec = d.node_rec + 3 * (size_t)node;
+    const float4 lb = rec[0], rb = rec[1], cw = rec[2];
+    const int split = __float_as_int(cw.z);
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
-      const int ch = d.node_child[2 * node + side];
-      if (ch < 0) {
-        const int k = ~ch;
-        if (k < wlo || k > whi) continue;
-        if (!overlap(q, d.sparse[k])) continue;
-        const int j = (int)d.mval[0][k];
-        if (j <= i || !pair_allowed(d, i, j)) continue;
+      // the child's leaf range must meet the world's: [lo, split] / [split + 1, hi]
+      if (side == 0 ? split < wlo : split + 1 > whi) continue;
+      if (!overlap(q, side == 0 ? lb : rb)) continue;
+      const int code = __float_as_int(side == 0 ? cw.x : cw.y);
+      if (code < 0) {
+        const int j = ~code;
+        if (j <= i || !pair_allowed(d, fi, d.sfast[j])) continue;
         const uint64_t key = ((uint64_t)(uint32_t)i << 32) | (uint32_t)j;
         if (found < cap) {
-          // sorted insert
-          int b = found - 1;
+          int b = found - 1;  // sorted insert
           while (b >= 0 && out[b] > key) {
             out[b + 1] = out[b];
             --b;
@@ -1466,38 +1741,45 @@ __device__ __forceinline__ int lbvh_query(const Dev& d, int i, uint64_t* out, in
           out[b + 1] = key;
         }
         ++found;
+      } else if (sp < kStack) {
+        stk[sp++ * nt + tid] = code;
       } else {
-        const int2 r = d.node_range[ch];
-        if (r.y < wlo || r.x > whi) continue;
-        if (!overlap(q, d.node_aabb[ch])) continue;
-        if (sp < kStack) stack[sp++] = ch;
-        else atomicOr(&d.cnt[C_STATUS], S_STACK_OVERFLOW);
+        atomicOr(&d.cnt[C_STATUS], S_STACK_OVERFLOW);
       }
     }
   }
   return found;
 }
 
-__global__ void __launch_bounds__(kBlock) k_lbvh_pairs(Dev d) {
-  B2G_FOR(i, 0, d.ns) {
-    d.pcount[i] = lbvh_query<false>(d, i, d.pslots + (size_t)i * kPairSlots, kPairSlots);
+// Queries run in Morton order (lane t takes the shape at sorted position t):
+// neighbouring lanes query neighbouring boxes and walk the same nodes.
+__global__ void __launch_bounds__(kQueryBlock) k_lbvh_pairs(Dev d) {
+  __shared__ int stk[kStack * kQueryBlock];
+  B2G_FOR_ALL(t, valid, 0, d.ns) {
+    if (valid) {
+      const int i = (int)d.mval[0][t];
+      d.pcount[i] = lbvh_query(d, i, d.pslots + (size_t)i * kPairSlots, kPairSlots, stk);
+    }
   }
 }
 
 // copy each shape's sorted pairs to its place in the pair list (poff); a
 // shape with more pairs than slots is traversed again straight into place
-__global__ void __launch_bounds__(kBlock) k_pairs_compact(Dev d) {
-  B2G_FOR(i, 0, d.ns) {
-    const int cnt = d.pcount[i];
-    const int base = d.poff[i];
-    if (cnt <= kPairSlots) {
-      const uint64_t* src = d.pslots + (size_t)i * kPairSlots;
-      for (int k = 0; k < cnt; ++k)
-        if (base + k < d.ccap) d.nkey[base + k] = src[k];
-    } else if (base + cnt <= d.ccap) {
-      lbvh_query<true>(d, i, d.nkey + base, cnt);
-    } else {
-      atomicOr(&d.cnt[C_STATUS], S_CONTACT_OVERFLOW);
+__global__ void __launch_bounds__(kQueryBlock) k_pairs_compact(Dev d) {
+  __shared__ int stk[kStack * kQueryBlock];
+  B2G_FOR_ALL(i, valid, 0, d.ns) {
+    if (valid) {
+      const int cnt = d.pcount[i];
+      const int base = d.poff[i];
+      if (cnt <= kPairSlots) {
+        const uint64_t* src = d.pslots + (size_t)i * kPairSlots;
+        for (int k = 0; k < cnt; ++k)
+          if (base + k < d.ccap) d.nkey[base + k] = src[k];
+      } else if (base + cnt <= d.ccap) {
+        lbvh_query(d, i, d.nkey + base, cnt, stk);
+      } else {
+        atomicOr(&d.cnt[C_STATUS], S_CONTACT_OVERFLOW);
+      }
     }
   }
 }

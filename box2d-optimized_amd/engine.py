@@ -81,7 +81,10 @@ EXPORTS = [
     "b2a_world_from_scene", "b2a_world_destroy", "b2a_world_step", "b2a_world_get_bodies",
     "b2a_world_contact_count", "b2a_world_get_contacts", "b2a_world_batch", "b2a_world_prepare",
     "b2g_time_kernel", "b2g_kernel_time", "b2a_scene_records", "b2a_batch_from_scenes", "b2s_run",
+    "b2g_set_option", "b2g_get_option",
 ]
+SOLVER_AUTO, SOLVER_COLOURS, SOLVER_WORLDS = 0, 1, 2
+OPT_SOLVER, OPT_WORLD_BLOCK, OPT_TUNED_WORLD_BLOCK, OPT_LAST_SOLVER, OPT_PER_WORLD_ELIGIBLE = 1, 2, 3, 4, 5
 KERNEL_NARROWPHASE, KERNEL_WARM_START, KERNEL_CONTACT_VELOCITY, KERNEL_CONTACT_POSITION, \
     KERNEL_PAIRS_WRITE, KERNEL_SOLVER = range(6)
 
@@ -146,6 +149,8 @@ def lib():
         L.b2a_scene_records.argtypes = [_P, _I, _I, _P, _P, _P]
         L.b2a_batch_from_scenes.argtypes = [_P, _I, _P, _I, _I]
         L.b2s_run.argtypes = [_P, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P]
+        L.b2g_set_option.argtypes = [_P, _I, _I]
+        L.b2g_get_option.argtypes = [_P, _I, _P]
         _lib = L
     return _lib
 
@@ -280,6 +285,14 @@ class Batch:
         p = Profile()
         check(self.L.b2g_get_profile(self.h, ctypes.byref(p)))
         return p
+
+    def set_option(self, option, value):
+        check(self.L.b2g_set_option(self.h, option, value))
+
+    def get_option(self, option):
+        v = ctypes.c_int32(0)
+        check(self.L.b2g_get_option(self.h, option, ctypes.byref(v)))
+        return v.value
 
     def time_kernel(self, kernel):
         check(self.L.b2g_time_kernel(self.h, kernel))

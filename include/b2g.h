@@ -251,6 +251,27 @@ enum {
   B2G_KERNEL_SOLVER = 5
 };
 int b2g_time_kernel(b2g_batch* batch, int32_t kernel);
+
+/* Solver layout and tuning. B2G_OPT_SOLVER: B2G_SOLVER_AUTO (default: one
+ * workgroup per world when the batch has several joint-free worlds of at most
+ * 8192 bodies laid out world after world, else colour-major sweeps),
+ * B2G_SOLVER_COLOURS (one launch per colour over all worlds) or
+ * B2G_SOLVER_WORLDS (one workgroup solves a whole world, barriers between
+ * colours; needs that layout). Both give the same results.
+ * B2G_OPT_WORLD_BLOCK: threads per world workgroup, 0 = autotune (the first
+ * ten per-world steps time 64..1024 threads and keep the fastest).
+ * Read-only: B2G_OPT_TUNED_WORLD_BLOCK, B2G_OPT_LAST_SOLVER (1 colours,
+ * 2 worlds), B2G_OPT_PER_WORLD_ELIGIBLE. No reference counterpart. */
+enum { B2G_SOLVER_AUTO = 0, B2G_SOLVER_COLOURS = 1, B2G_SOLVER_WORLDS = 2 };
+enum {
+  B2G_OPT_SOLVER = 1,
+  B2G_OPT_WORLD_BLOCK = 2,
+  B2G_OPT_TUNED_WORLD_BLOCK = 3,
+  B2G_OPT_LAST_SOLVER = 4,
+  B2G_OPT_PER_WORLD_ELIGIBLE = 5
+};
+int b2g_set_option(b2g_batch* batch, int32_t option, int32_t value);
+int b2g_get_option(b2g_batch* batch, int32_t option, int32_t* value);
 int b2g_kernel_time(b2g_batch* batch, double* total_ms, int64_t* launches, int64_t* items);
 int b2g_get_profile(b2g_batch* batch, b2g_profile* out);
 int b2g_download_bodies(b2g_batch* batch, int32_t first, int32_t count, b2g_body* out);

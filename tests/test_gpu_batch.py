@@ -114,3 +114,43 @@ def test_gpu_graph_steps_async_batch(engine):
     assert b.counts().status == 0
     x, y = a.bodies(), b.bodies()
     assert np.array_equal(bits(x["c"]), bits(y["c"])) and np.array_equal(bits(x["v"]), bits(y["v"]))
+
+
+def test_gpu_per_world_solver_matches_colour_sweeps_bitexact(engine):
+    """The one-workgroup-per-world solver gives the colour-major sweeps'
+    results bit for bit, for every workgroup size the autotuner may pick."""
+    scenes = _mixed_scenes()[:2] + [S.batch_world(i, 50 + 30 * i) for i in range(10)]
+    scenes = [s for s in scenes if s.joint_count == 0]
+    ref = engine.Batch.from_scenes(scenes)
+    ref.set_option(engine.OPT_SOLVER, engine.SOLVER_COLOURS)
+    ref.advance(30)
+    assert ref.get_option(engine.OPT_LAST_SOLVER) == 1
+    want = _state(ref.bodies())
+    for block in (64, 256, 1024):
+        b = engine.Batch.from_scenes(scenes)
+        assert b.get_option(engine.OPT_PER_WORLD_ELIGIBLE) == 1
+        b.set_option(engine.OPT_SOLVER, engine.SOLVER_WORLDS)
+        b.set_option(engine.OPT_WORLD_BLOCK, block)
+        b.advance(30)
+        assert b.get_option(engine.OPT_LAST_SOLVER) == 2
+        same = (_state(b.bodies()) == want).all(axis=1)
+        print(f"block {block}: {same.sum()} of {len(same)} bodies bit-identical", flush=True)
+        assert same.all()
+
+
+def test_gpu_per_world_autotune_and_graph(engine):
+    """Auto mode picks the per-world solver for a batch of small worlds, the
+    autotuner settles on a block size within ten steps, and graph replays of
+    the per-world step match eager steps."""
+    scenes = [S.batch_world(i, 60 + 20 * i) for i in range(16)]
+    a = engine.Batch.from_scenes(scenes).advance(12)
+    assert a.get_option(engine.OPT_LAST_SOLVER) == 2
+    tuned = a.get_option(engine.OPT_TUNED_WORLD_BLOCK)
+    print("autotuned world block:", tuned, flush=True)
+    assert tuned in (64, 128, 256, 512, 1024)
+    b = engine.Batch.from_scenes(scenes).advance(12)
+    for _ in range(20):
+        a.step(8, 3, 0)
+        b.step(8, 3, engine.STEP_GRAPH)
+    assert b.counts().graph_steps == 20
+    assert np.array_equal(_state(a.bodies()), _state(b.bodies()))
