@@ -175,6 +175,39 @@ def batched_section(pkg, worlds: int, warmup: int, steps: int, sample_every: int
     return out
 
 
+def single_world_section(pkg, name: str, warmup: int, steps: int, mode: int, cpu_steps: int):
+    """Another single-world workload (default: 100k circles + 500 revolute
+    chains of 30 links, zero gravity): GPU body-steps/s over `steps` steps
+    after `warmup`, and the reference CPU Step over the same steps."""
+    eng = pkg.engine
+    wl = workloads(pkg)[name]
+    scene = wl["make"]()
+    world = eng.World(scene)
+    world.step(max(warmup - 2, 1))
+    batch = world.batch()
+    import numpy as np
+    dt = np.float32(1.0 / 60.0)
+    ratio = np.float32(np.float32(1.0) / dt) * dt
+    batch.set_world_params(0, [eng.WorldParams((scene.gravity[0], scene.gravity[1]), dt, ratio, 1, 1)])
+    for _ in range(2):
+        batch.step(8, 3, mode & ~eng.STEP_ASYNC)
+    batch.wait()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.step(8, 3, mode)
+    batch.wait()
+    el = time.perf_counter() - t0
+    c = batch.counts()
+    out = {"workload": name, "description": wl["desc"], "value": scene.dynamic_count * steps / el,
+           "unit": "body-steps/s", "ms_per_step": el / steps * 1e3, "steps": steps, "warmup": warmup,
+           "dynamic_bodies": scene.dynamic_count, "joints": scene.joint_count, "contacts": c.contacts,
+           "touching": c.touching, "colours": c.colours}
+    if cpu_steps > 0:
+        out["cpu_baseline"] = cpu_baseline(wl["make"](), warmup, cpu_steps)
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def serving_section(pkg, worlds: int, rate: float, duration: float, budget_gb: float, graph: bool):
     """Serving pattern: Poisson world-step requests (1-8 substeps at dt 1/60 or
     1/240) against a resident pool of worlds with log-uniform sizes 16-4096,
@@ -211,6 +244,23 @@ def serving_section(pkg, worlds: int, rate: float, duration: float, budget_gb: f
     return out
 
 
+def copy_probe(mib: int, reps: int) -> None:
+    """A plain device-to-device copy of `mib` MiB, `reps` times: the reference
+    kernel for scaling the FETCH_SIZE / WRITE_SIZE counters of a --pmc run
+    (each dispatch reads and writes exactly mib MiB)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(json.dumps({"copy_probe_mib": mib, "reps": reps, "GB_per_s": 2 * mib * (1 << 20) * reps / el / 1e9}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,10 +277,16 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--batch-sample", type=int, default=64, help="per-world loop / CPU sample stride")
     ap.add_argument("--serve-worlds", type=int, default=1000, help="0 skips the serving section")
+    ap.add_argument("--gas-steps", type=int, default=100, help="0 skips the gas + chains section")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration) and exit")
     ap.add_argument("--serve-rate", type=float, default=20000.0)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe, 10)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: worlds shard by world index across GPUs with no "
                                    "exchange, but no multi-GPU run is part of it; use --gpus 1"}))
@@ -288,6 +344,11 @@ def main():
         batched = batched_section(pkg, args.batch_worlds, args.warmup, args.batch_steps, args.batch_sample,
                                   not args.no_cpu, mode)
 
+    gas = None
+    if args.gas_steps > 0:
+        gas = single_world_section(pkg, "gas100k_chains500", args.warmup, args.gas_steps, mode,
+                                   0 if args.no_cpu else min(args.gas_steps, 40))
+
     serving = None
     if args.serve_worlds > 0:
         serving = serving_section(pkg, args.serve_worlds, args.serve_rate, args.serve_seconds,
@@ -320,6 +381,7 @@ def main():
                      "bytes_per_launch": bytes_per_launch, "bytes_per_item": per_item},
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "gas_chains": gas,
         "batched": batched,
         "serving": serving,
         "host": platform.node(),

@@ -70,6 +70,7 @@ struct b2g_batch {
   // host mirrors
   std::vector<int> h_btype, h_bworld;
   std::vector<int> h_shape_body;
+  std::vector<b2g_shape> h_shape_rec;  // uploaded shape records (snapshots)
   std::vector<int> h_wshape0;   // first shape of each world
   std::vector<b2g_revolute> h_joints;
   std::vector<int> h_jcol_off;  // joint colour ranges (host colouring)
@@ -877,8 +878,8 @@ int b2g_set_world_params(b2g_batch* b, int32_t first, int32_t count, const b2g_w
   if (first + count > b->d.nw) {  // the world count is part of captured launches
     b->graphs_clear();
     b->graph_K = 0;
-    b->filters_dirty = true;
-    b->layout_dirty = true;
+  b->filters_dirty = true;
+  b->layout_dirty = true;
   }
   std::vector<float4> wp(count);
   std::vector<int> wf(count);
@@ -901,8 +902,8 @@ int b2g_set_worlds(b2g_batch* b, int32_t count, const int32_t* body_begin, const
   if (!b || count < 1 || count > b->cap.worlds) return fail(B2G_ERR_INVALID, "world count");
   b->graphs_clear();
   b->graph_K = 0;
-    b->filters_dirty = true;
-    b->layout_dirty = true;
+  b->filters_dirty = true;
+  b->layout_dirty = true;
   b->d.nw = count;
   return b2g_set_world_params(b, 0, count, params);
 }
@@ -943,8 +944,8 @@ int b2g_upload_bodies(b2g_batch* b, int32_t first, int32_t count, const b2g_body
   if (first + count > d.nb) {
     b->graphs_clear();
     b->graph_K = 0;
-    b->filters_dirty = true;
-    b->layout_dirty = true;
+  b->filters_dirty = true;
+  b->layout_dirty = true;
   }
   if (first + count > d.nb) {
     // extend the joint-partner CSR with empty rows
@@ -964,8 +965,8 @@ int b2g_upload_shapes(b2g_batch* b, int32_t first, int32_t count, const b2g_shap
   if (!b || first < 0 || count < 0 || first + count > b->cap.shapes) return fail(B2G_ERR_INVALID, "shape range");
   b->graphs_clear();  // captured launch sizes no longer match
   b->graph_K = 0;
-    b->filters_dirty = true;
-    b->layout_dirty = true;
+  b->filters_dirty = true;
+  b->layout_dirty = true;
   if (count == 0) return B2G_OK;
   std::vector<int4> hdr(count), filt(count);
   std::vector<float4> geo(9 * (size_t)count), mat(count);
@@ -977,6 +978,8 @@ int b2g_upload_shapes(b2g_batch* b, int32_t first, int32_t count, const b2g_shap
       return fail(B2G_ERR_INVALID, "shape type / vertex count");
     shape_to_device(s, b->h_bworld[s.body], &hdr[i], &geo[9 * (size_t)i], &mat[i], &filt[i]);
     b->h_shape_body[first + i] = s.body;
+    if ((int)b->h_shape_rec.size() < first + count) b->h_shape_rec.resize(first + count);
+    b->h_shape_rec[first + i] = s;
   }
   hipStream_t st = b->stream;
   Dev& d = b->d;
@@ -1008,8 +1011,8 @@ int b2g_upload_joints(b2g_batch* b, int32_t first, int32_t count, const b2g_revo
   if (!b || first < 0 || count < 0 || first + count > b->cap.joints) return fail(B2G_ERR_INVALID, "joint range");
   b->graphs_clear();  // captured launch sizes no longer match
   b->graph_K = 0;
-    b->filters_dirty = true;
-    b->layout_dirty = true;
+  b->filters_dirty = true;
+  b->layout_dirty = true;
   if (count == 0) return B2G_OK;
   if ((int)b->h_joints.size() < first + count) b->h_joints.resize(first + count);
   std::vector<Joint> dj(count);
@@ -1041,8 +1044,8 @@ int b2g_upload_contacts(b2g_batch* b, int32_t count, const b2g_contact* in) {
   if (!b || count < 0 || count > b->cap.contacts) return fail(B2G_ERR_INVALID, "contact count");
   b->graphs_clear();  // captured launch sizes no longer match
   b->graph_K = 0;
-    b->filters_dirty = true;
-    b->layout_dirty = true;
+  b->filters_dirty = true;
+  b->layout_dirty = true;
   std::vector<int> idx(count);
   std::vector<uint64_t> keys(count);
   for (int i = 0; i < count; ++i) {
@@ -1236,9 +1239,107 @@ int b2g_download_contacts(b2g_batch* b, int32_t first, int32_t count, b2g_contac
 }
 
 int b2g_download_contact_colours(b2g_batch* b, int32_t first, int32_t count, int32_t* out) {
-  if (!b || !out) return fail(B2G_ERR_INVALID, "null argument");
+  if (!b || (count && !out)) return fail(B2G_ERR_INVALID, "null argument");
+  int rc = b->read_counters();
+  if (rc) return rc;
+  if (first < 0 || count < 0 || first + count > b->pinned[C_CONTACTS]) return fail(B2G_ERR_INVALID, "contact range");
+  if (count == 0) return B2G_OK;
   // colours are kept per contact in the (post-broadphase) contact buffer
   HIP_TRY(hipMemcpy(out, b->d.ccol + first, count * sizeof(int), hipMemcpyDeviceToHost));
+  return B2G_OK;
+}
+
+int b2g_upload_contact_colours(b2g_batch* b, int32_t count, const int32_t* colours) {
+  if (!b || (count && !colours)) return fail(B2G_ERR_INVALID, "null argument");
+  int rc = b->read_counters();
+  if (rc) return rc;
+  if (count != b->pinned[C_CONTACTS]) return fail(B2G_ERR_INVALID, "colour count must equal the contact count");
+  for (int i = 0; i < count; ++i)
+    if (colours[i] < -1 || colours[i] > kSerialColour) return fail(B2G_ERR_INVALID, "colour out of range");
+  if (count) HIP_TRY(hipMemcpy(b->d.ccol, colours, count * sizeof(int), hipMemcpyHostToDevice));
+  b->graphs_clear();
+  b->graph_K = 0;
+  return B2G_OK;
+}
+
+// ---- snapshots ---------------------------------------------------------------
+// File: "B2GSNAP1", then int32 version, capacity (5), world count, body,
+// shape, joint and contact counts, then the world params, body, shape, joint
+// and contact records and the contact colours, all little-endian as laid out
+// in include/b2g.h.
+static const char kSnapMagic[8] = {'B', '2', 'G', 'S', 'N', 'A', 'P', '1'};
+
+int b2g_snapshot_save(b2g_batch* b, const char* path) {
+  if (!b || !path) return fail(B2G_ERR_INVALID, "null argument");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  int rc = b->read_counters();
+  if (rc) return rc;
+  const int nw = b->d.nw, nb = b->d.nb, ns = b->d.ns, nj = b->d.nj, nc = b->pinned[C_CONTACTS];
+  std::vector<b2g_body> bodies(nb);
+  std::vector<b2g_revolute> joints(nj);
+  std::vector<b2g_contact> contacts(nc);
+  std::vector<int32_t> colours(nc);
+  if (nb && (rc = b2g_download_bodies(b, 0, nb, bodies.data()))) return rc;
+  if (nj && (rc = b2g_download_joints(b, 0, nj, joints.data()))) return rc;
+  if (nc && (rc = b2g_download_contacts(b, 0, nc, contacts.data()))) return rc;
+  if (nc && (rc = b2g_download_contact_colours(b, 0, nc, colours.data()))) return rc;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(B2G_ERR_INVALID, std::string("cannot write ") + path);
+  const int32_t head[11] = {1, b->cap.worlds, b->cap.bodies, b->cap.shapes, b->cap.joints, b->cap.contacts,
+                            nw, nb, ns, nj, nc};
+  bool ok = std::fwrite(kSnapMagic, 1, 8, f) == 8 && std::fwrite(head, sizeof(head), 1, f) == 1;
+  ok = ok && (nw == 0 || std::fwrite(b->h_params.data(), sizeof(b2g_world_params), nw, f) == (size_t)nw);
+  ok = ok && (nb == 0 || std::fwrite(bodies.data(), sizeof(b2g_body), nb, f) == (size_t)nb);
+  ok = ok && (ns == 0 || std::fwrite(b->h_shape_rec.data(), sizeof(b2g_shape), ns, f) == (size_t)ns);
+  ok = ok && (nj == 0 || std::fwrite(joints.data(), sizeof(b2g_revolute), nj, f) == (size_t)nj);
+  ok = ok && (nc == 0 || std::fwrite(contacts.data(), sizeof(b2g_contact), nc, f) == (size_t)nc);
+  ok = ok && (nc == 0 || std::fwrite(colours.data(), sizeof(int32_t), nc, f) == (size_t)nc);
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) return fail(B2G_ERR_INVALID, std::string("short write to ") + path);
+  return B2G_OK;
+}
+
+int b2g_snapshot_load(b2g_batch** out, const char* path) {
+  if (!out || !path) return fail(B2G_ERR_INVALID, "null argument");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(B2G_ERR_INVALID, std::string("cannot read ") + path);
+  char magic[8];
+  int32_t head[11];
+  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kSnapMagic, 8) == 0 &&
+            std::fread(head, sizeof(head), 1, f) == 1 && head[0] == 1;
+  const int nw = ok ? head[6] : 0, nb = ok ? head[7] : 0, ns = ok ? head[8] : 0, nj = ok ? head[9] : 0,
+            nc = ok ? head[10] : 0;
+  b2g_capacity cap{head[1], head[2], head[3], head[4], head[5]};
+  ok = ok && nw >= 1 && nb >= 0 && ns >= 0 && nj >= 0 && nc >= 0 && nw <= cap.worlds && nb <= cap.bodies &&
+       ns <= cap.shapes && nj <= cap.joints && nc <= cap.contacts;
+  std::vector<b2g_world_params> params(ok ? nw : 0);
+  std::vector<b2g_body> bodies(ok ? nb : 0);
+  std::vector<b2g_shape> shapes(ok ? ns : 0);
+  std::vector<b2g_revolute> joints(ok ? nj : 0);
+  std::vector<b2g_contact> contacts(ok ? nc : 0);
+  std::vector<int32_t> colours(ok ? nc : 0);
+  ok = ok && std::fread(params.data(), sizeof(b2g_world_params), nw, f) == (size_t)nw;
+  ok = ok && (nb == 0 || std::fread(bodies.data(), sizeof(b2g_body), nb, f) == (size_t)nb);
+  ok = ok && (ns == 0 || std::fread(shapes.data(), sizeof(b2g_shape), ns, f) == (size_t)ns);
+  ok = ok && (nj == 0 || std::fread(joints.data(), sizeof(b2g_revolute), nj, f) == (size_t)nj);
+  ok = ok && (nc == 0 || std::fread(contacts.data(), sizeof(b2g_contact), nc, f) == (size_t)nc);
+  ok = ok && (nc == 0 || std::fread(colours.data(), sizeof(int32_t), nc, f) == (size_t)nc);
+  std::fclose(f);
+  if (!ok) return fail(B2G_ERR_INVALID, std::string("not a readable snapshot: ") + path);
+  b2g_batch* b = nullptr;
+  int rc = b2g_batch_create(&b, &cap);
+  if (rc) return rc;
+  if (!rc) rc = b2g_set_worlds(b, nw, nullptr, nullptr, params.data());
+  if (!rc && nb) rc = b2g_upload_bodies(b, 0, nb, bodies.data());
+  if (!rc && ns) rc = b2g_upload_shapes(b, 0, ns, shapes.data());
+  if (!rc && nj) rc = b2g_upload_joints(b, 0, nj, joints.data());
+  if (!rc) rc = b2g_upload_contacts(b, nc, contacts.data());
+  if (!rc) rc = b2g_upload_contact_colours(b, nc, colours.data());
+  if (rc) {
+    b2g_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
   return B2G_OK;
 }
 

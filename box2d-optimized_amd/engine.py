@@ -81,7 +81,8 @@ EXPORTS = [
     "b2a_world_from_scene", "b2a_world_destroy", "b2a_world_step", "b2a_world_get_bodies",
     "b2a_world_contact_count", "b2a_world_get_contacts", "b2a_world_batch", "b2a_world_prepare",
     "b2g_time_kernel", "b2g_kernel_time", "b2a_scene_records", "b2a_batch_from_scenes", "b2s_run",
-    "b2g_set_option", "b2g_get_option",
+    "b2g_set_option", "b2g_get_option", "b2g_upload_contact_colours", "b2g_snapshot_save",
+    "b2g_snapshot_load",
 ]
 SOLVER_AUTO, SOLVER_COLOURS, SOLVER_WORLDS = 0, 1, 2
 OPT_SOLVER, OPT_WORLD_BLOCK, OPT_TUNED_WORLD_BLOCK, OPT_LAST_SOLVER, OPT_PER_WORLD_ELIGIBLE = 1, 2, 3, 4, 5
@@ -150,6 +151,9 @@ def lib():
         L.b2a_batch_from_scenes.argtypes = [_P, _I, _P, _I, _I]
         L.b2s_run.argtypes = [_P, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P]
         L.b2g_set_option.argtypes = [_P, _I, _I]
+        L.b2g_upload_contact_colours.argtypes = [_P, _I, _P]
+        L.b2g_snapshot_save.argtypes = [_P, ctypes.c_char_p]
+        L.b2g_snapshot_load.argtypes = [_P, ctypes.c_char_p]
         L.b2g_get_option.argtypes = [_P, _I, _P]
         _lib = L
     return _lib
@@ -285,6 +289,26 @@ class Batch:
         p = Profile()
         check(self.L.b2g_get_profile(self.h, ctypes.byref(p)))
         return p
+
+    def save(self, path):
+        """Binary snapshot (b2g_snapshot_save)."""
+        check(self.L.b2g_snapshot_save(self.h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path, like=None):
+        """Batch restored from a snapshot; `like` (a Batch) supplies the
+        per-world layout metadata the Python side keeps."""
+        L = lib()
+        h = ctypes.c_void_p()
+        check(L.b2g_snapshot_load(ctypes.byref(h), os.fsencode(path)))
+        b = cls(handle=h.value)
+        b.owned = True
+        if like is not None:
+            for k in ("world_bodies", "world_shapes", "world_joints", "world_dynamic", "gravity", "inv_dt0"):
+                if hasattr(like, k):
+                    setattr(b, k, getattr(like, k))
+            b.stepped = getattr(like, "stepped", False)
+        return b
 
     def set_option(self, option, value):
         check(self.L.b2g_set_option(self.h, option, value))

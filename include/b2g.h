@@ -281,6 +281,21 @@ int b2g_download_joints(b2g_batch* batch, int32_t first, int32_t count, b2g_revo
 /* constraint colour of each touching contact of the last step (-1 = not solved);
  * out has one entry per downloaded contact, same order */
 int b2g_download_contact_colours(b2g_batch* batch, int32_t first, int32_t count, int32_t* out);
+/* Sets the persisted colours of the current contacts (count = contact count,
+ * in the key-sorted order b2g_download_contacts returns; -1 = uncoloured).
+ * No reference counterpart: with b2g_upload_contacts it restores a solver
+ * state exactly. */
+int b2g_upload_contact_colours(b2g_batch* batch, int32_t count, const int32_t* colours);
+
+/* Binary snapshot of a batch: capacity, world params, bodies, shapes,
+ * joints with their impulses, contacts with manifolds, impulses and colours.
+ * A batch loaded from it continues bit for bit as the saved one would
+ * (world params hold the last dt and dtRatio; the caller continues dtRatio
+ * as inv_dt0 * dt). Covers what b2World::Dump
+ * (src/dynamics/b2_world.cpp:1469-1531) writes plus the contact state it
+ * loses. */
+int b2g_snapshot_save(b2g_batch* batch, const char* path);
+int b2g_snapshot_load(b2g_batch** out, const char* path);
 
 /* ---- stage entry points (pure functions over caller arrays) ------------ */
 
