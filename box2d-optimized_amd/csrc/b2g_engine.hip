@@ -183,7 +183,7 @@ int b2g_batch::alloc() {
   A(d.ckey, NC); A(d.cpair, NC); A(d.cman, 4 * (size_t)NC); A(d.cmat, NC); A(d.ccol, NC);
   A(d.nkey, NC); A(d.npair, NC); A(d.nman, 4 * (size_t)NC); A(d.nmat, NC); A(d.ncol, NC);
   A(d.touch, NC); A(d.tpos, NC); A(d.clist, NC); A(d.ccolour, NC); A(d.csorted, NC);
-  A(d.cisland, NC); A(d.cpend, NC);
+  A(d.cisland, NC); A(d.cpend, NC); A(d.events, (size_t)kEventWords * NC);
   for (int k = 0; k < 10; ++k) A(d.vc[k], NC);
   for (int k = 0; k < 5; ++k) A(d.pc[k], NC);
   A(d.col_count, kColourSlots + 1); A(d.col_off, kColourSlots + 1); A(d.col_cursor, kColourSlots + 1);
@@ -207,6 +207,7 @@ int b2g_batch::alloc() {
   A(d.cnt, C_COUNT);
 #undef A
   d.ccap = NC;
+  d.ecap = NC;
   HIP_TRY(hipHostMalloc((void**)&pinned, 256 * sizeof(int), hipHostMallocDefault));
   HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
@@ -227,7 +228,7 @@ void b2g_batch::release() {
                   d.wbounds, d.wleaf, d.shdr, d.geo, d.smat, d.sfilt, d.aabb, d.joints, d.jsol,
                   d.jorder, d.jisland, d.bj_off, d.bj_other, d.ckey, d.cpair, d.cman, d.cmat, d.ccol,
                   d.nkey, d.npair, d.nman, d.nmat, d.ncol, d.touch, d.tpos, d.clist, d.ccolour,
-                  d.csorted, d.cisland, d.cpend, d.col_count, d.col_off, d.col_cursor, d.bmask, d.bslot,
+                  d.csorted, d.cisland, d.cpend, d.events, d.col_count, d.col_off, d.col_cursor, d.bmask, d.bslot,
                   d.parent, d.minsep, d.jok, d.solved, d.mkey[0], d.mkey[1], d.mval[0], d.mval[1],
                   d.sort_hist, d.node_child, d.node_parent, d.leaf_parent, d.node_range, d.node_rec,
                   d.sfast, d.sparse, d.pslots, d.pcount, d.poff, d.scan_tiles, d.cnt};
@@ -314,6 +315,9 @@ int b2g_batch::broadphase(bool use_given_aabbs) {
   hipLaunchKernelGGL(k_pairs_compact, dim3(gq), dim3(kQueryBlock), 0, stream, d);
   hipLaunchKernelGGL(k_merge_contacts, dim3(grid_for(std::max(est_contacts, ns) * 5 / 4)), dim3(kBlock), 0, stream,
                      d);
+  if (d.events_on)
+    hipLaunchKernelGGL(k_dead_contact_events, dim3(grid_for(std::max(est_contacts, 256))), dim3(kBlock), 0, stream,
+                       d);
   hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, stream, d);
   swap_contact_buffers();
   return B2G_OK;
@@ -559,6 +563,8 @@ int b2g_batch::step(int vi, int pi, unsigned flags) {
     hipLaunchKernelGGL(k_reset_colour_rounds, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
     round_host = 0;
   }
+  d.events_on = (flags & B2G_STEP_EVENTS) ? 1 : 0;
+  if (d.events_on) hipLaunchKernelGGL(k_events_reset, dim3(1), dim3(1), 0, stream, d);
   const bool prof_on = (flags & B2G_STEP_PROFILE) != 0;
   const bool graph_ok = (flags & B2G_STEP_GRAPH) && !prof_on && timed_kernel < 0 &&
                         !(flags & B2G_STEP_FIND_NEW_CONTACTS) && graph_K > 0 &&
@@ -728,7 +734,7 @@ int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
 }
 
 int b2g_batch::graph_step(int vi, int pi, unsigned flags) {
-  const unsigned key_flags = flags & B2G_STEP_SKIP_BROADPHASE;
+  const unsigned key_flags = flags & (B2G_STEP_SKIP_BROADPHASE | B2G_STEP_EVENTS);
   const int block = use_per_world() ? world_block_now() : 0;
   last_path = block ? 2 : 1;
   if (filters_dirty) {  // refreshed outside the captured sequence
@@ -1246,6 +1252,45 @@ int b2g_download_contact_colours(b2g_batch* b, int32_t first, int32_t count, int
   if (count == 0) return B2G_OK;
   // colours are kept per contact in the (post-broadphase) contact buffer
   HIP_TRY(hipMemcpy(out, b->d.ccol + first, count * sizeof(int), hipMemcpyDeviceToHost));
+  return B2G_OK;
+}
+
+int b2g_download_events(b2g_batch* b, int32_t capacity, b2g_contact_event* out, int32_t* count) {
+  if (!b || !count || (capacity && !out) || capacity < 0) return fail(B2G_ERR_INVALID, "null argument");
+  int rc = b->read_counters();
+  if (rc) return rc;
+  const int n = std::min(b->pinned[C_EVENTS], b->d.ecap);
+  *count = n;
+  if (b->pinned[C_STATUS] & S_EVENT_OVERFLOW) return fail(B2G_ERR_CAPACITY, "contact event buffer overflow");
+  if (capacity == 0 || n == 0) return B2G_OK;
+  std::vector<float4> raw((size_t)kEventWords * n);
+  HIP_TRY(hipMemcpy(raw.data(), b->d.events, raw.size() * sizeof(float4), hipMemcpyDeviceToHost));
+  std::vector<b2g_contact_event> ev(n);
+  for (int i = 0; i < n; ++i) {
+    const float4* e = &raw[(size_t)kEventWords * i];
+    b2g_contact_event& o = ev[i];
+    std::memset(&o, 0, sizeof(o));
+    o.kind = __builtin_bit_cast(int, e[0].z);
+    o.contact.shape_a = __builtin_bit_cast(int, e[0].x);
+    o.contact.shape_b = __builtin_bit_cast(int, e[0].y);
+    manifold_to_record(e + 1, &o.contact);
+    o.contact.friction = e[5].x;
+    o.contact.restitution = e[5].y;
+    o.contact.restitution_threshold = e[5].z;
+    o.contact.tangent_speed = e[5].w;
+  }
+  // Collide's events first, then the broadphase's; by shape pair inside each
+  auto key = [](const b2g_contact_event& e) {
+    const uint64_t a = (uint32_t)e.contact.shape_a, c = (uint32_t)e.contact.shape_b;
+    return (std::min(a, c) << 32) | std::max(a, c);
+  };
+  std::sort(ev.begin(), ev.end(), [&](const b2g_contact_event& x, const b2g_contact_event& y) {
+    const int px = x.kind == 3, py = y.kind == 3;
+    return px != py ? px < py : key(x) < key(y);
+  });
+  const int m = std::min(n, (int)capacity);
+  std::memcpy(out, ev.data(), m * sizeof(b2g_contact_event));
+  *count = n;
   return B2G_OK;
 }
 

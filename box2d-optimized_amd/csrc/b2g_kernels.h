@@ -46,9 +46,14 @@ enum Counter : int {
   C_SERIAL = 8,     // constraints placed in the serial group by the last step
   C_PENDING = 9,    // entries of the pending-colour list of this step
   C_TICKET = 10,    // blocks finished in a sweep's last colour kernel (serial tail)
+  C_EVENTS = 11,    // contact events recorded this step
   C_COUNT = 16
 };
-enum Status : int { S_CONTACT_OVERFLOW = 1, S_STACK_OVERFLOW = 2, S_TREE_CORRUPT = 4 };
+enum Status : int { S_CONTACT_OVERFLOW = 1, S_STACK_OVERFLOW = 2, S_TREE_CORRUPT = 4, S_EVENT_OVERFLOW = 8 };
+// contact event kinds: touching began / ended in Collide, touching contact
+// destroyed by the broadphase (b2ContactManager::Destroy)
+enum EventKind : int { kEventBegin = 1, kEventEnd = 2, kEventDestroyed = 3 };
+constexpr int kEventWords = 6;  // float4 per event: (shapeA, shapeB, kind, -), manifold (4), material
 enum BodyType : int { kStaticBody = 0, kKinematicBody = 1, kDynamicBody = 2 };
 constexpr int kMaxColours = 64;           // bit-mask colours; colour 64 = serial group
 constexpr int kSerialColour = kMaxColours;
@@ -102,6 +107,9 @@ struct Dev {
   float4* pc[5];
   int* cisland;
   int* cpend;     // constraints (compacted slots) without a colour at the start of this step
+  float4* events; // kEventWords per contact event (recorded when events_on)
+  int events_on;
+  int ecap;
   unsigned long long* bmask;
   unsigned long long* bslot;
   // islands
@@ -210,6 +218,26 @@ __device__ __forceinline__ void wave_atomic_max_keyed(unsigned* base, int key, b
 __device__ __forceinline__ vec2 v2of(float4 f) { return mk(f.x, f.y); }
 __device__ __forceinline__ bool world_active(const Dev& d, int w) { return (d.wflag[w] & 2) != 0; }
 
+// one contact event: shapes, kind, the manifold as stored, the material
+__device__ __forceinline__ void record_event(const Dev& d, int2 pr, int kind, const float4* man, float4 mat) {
+  const int slot = wave_append(&d.cnt[C_EVENTS]);
+  if (slot >= d.ecap) {
+    atomicOr(&d.cnt[C_STATUS], S_EVENT_OVERFLOW);
+    return;
+  }
+  float4* e = d.events + (size_t)kEventWords * slot;
+  e[0] = make_float4(__int_as_float(pr.x), __int_as_float(pr.y), __int_as_float(kind), 0.0f);
+  e[1] = man[0];
+  e[2] = man[1];
+  e[3] = man[2];
+  e[4] = man[3];
+  e[5] = mat;
+}
+
+__global__ void k_events_reset(Dev d) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) d.cnt[C_EVENTS] = 0;
+}
+
 // ===========================================================================
 // Narrowphase: b2ContactManager::Collide + b2Contact::Update
 // (src/dynamics/b2_contact_manager.cpp:66-116, src/dynamics/b2_contact.cpp:126-210)
@@ -256,6 +284,11 @@ __global__ void __launch_bounds__(kBlock) k_narrowphase(Dev d) {
       mp[3] = make_float4(uint_as_float(m.count > 0 ? m.id0 : 0u), uint_as_float(m.count > 1 ? m.id1 : 0u),
                           as_float((m.type & 0xff) | (m.count << 8)), 0.0f);
       solve = m.count > 0 ? 1 : 0;
+      // BeginContact / EndContact on touching transitions (b2_contact.cpp:197-203)
+      if (d.events_on && ((old_count > 0) != (m.count > 0))) {
+        const float4 man[4] = {mp[0], mp[1], mp[2], mp[3]};
+        record_event(d, pr, m.count > 0 ? kEventBegin : kEventEnd, man, d.cmat[i]);
+      }
       // a contact that is not solved this step gives up its colour: colours
       // kept across steps are then always a subset of the previous step's
       // valid colouring
@@ -1819,6 +1852,29 @@ __global__ void __launch_bounds__(kBlock) k_merge_contacts(Dev d) {
       d.nmat[p] = make_float4(sqrtf(ma.x * mb.x), ma.y > mb.y ? ma.y : mb.y, ma.z < mb.z ? ma.z : mb.z, 0.0f);
       d.ncol[p] = -1;
     }
+  }
+}
+
+// EndContact for touching contacts the broadphase no longer reports
+// (b2ContactManager::Destroy, src/dynamics/b2_contact_manager.cpp:48-51):
+// old contacts whose key is missing from the new, sorted pair list.
+__global__ void __launch_bounds__(kBlock) k_dead_contact_events(Dev d) {
+  const int old_count = d.cnt[C_CONTACTS];
+  const int total = d.cnt[C_PAIRS];
+  const int n = total < d.ccap ? total : d.ccap;
+  B2G_FOR(i, 0, old_count) {
+    const float4* mp = d.cman + 4 * (size_t)i;
+    const float4 m3 = mp[3];
+    if (((as_int(m3.z) >> 8) & 0xff) == 0) continue;  // not touching
+    const uint64_t key = d.ckey[i];
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (d.nkey[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    if (lo < n && d.nkey[lo] == key) continue;  // persists
+    const float4 man[4] = {mp[0], mp[1], mp[2], m3};
+    record_event(d, d.cpair[i], kEventDestroyed, man, d.cmat[i]);
   }
 }
 

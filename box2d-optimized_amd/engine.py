@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libbox2d_amd.so")
 
 # status codes (include/b2g.h)
 OK, ERR_HIP, ERR_OOM, ERR_CAPACITY, ERR_INVALID, ERR_UNSUPPORTED, ERR_NO_DEVICE = range(7)
-STEP_FIND_NEW_CONTACTS, STEP_PROFILE, STEP_ASYNC, STEP_SKIP_BROADPHASE, STEP_GRAPH = 1, 2, 4, 8, 16
+STEP_FIND_NEW_CONTACTS, STEP_PROFILE, STEP_ASYNC, STEP_SKIP_BROADPHASE, STEP_GRAPH, STEP_EVENTS = 1, 2, 4, 8, 16, 32
+EVENT_BEGIN, EVENT_END, EVENT_DESTROYED = 1, 2, 3
 
 BODY_DTYPE = np.dtype([
     ("type", "<i4"), ("world", "<i4"), ("xf", "<f4", 4), ("c", "<f4", 2), ("a", "<f4"),
@@ -82,7 +83,7 @@ EXPORTS = [
     "b2a_world_contact_count", "b2a_world_get_contacts", "b2a_world_batch", "b2a_world_prepare",
     "b2g_time_kernel", "b2g_kernel_time", "b2a_scene_records", "b2a_batch_from_scenes", "b2s_run",
     "b2g_set_option", "b2g_get_option", "b2g_upload_contact_colours", "b2g_snapshot_save",
-    "b2g_snapshot_load",
+    "b2g_snapshot_load", "b2g_download_events",
 ]
 SOLVER_AUTO, SOLVER_COLOURS, SOLVER_WORLDS = 0, 1, 2
 OPT_SOLVER, OPT_WORLD_BLOCK, OPT_TUNED_WORLD_BLOCK, OPT_LAST_SOLVER, OPT_PER_WORLD_ELIGIBLE = 1, 2, 3, 4, 5
@@ -90,6 +91,7 @@ KERNEL_NARROWPHASE, KERNEL_WARM_START, KERNEL_CONTACT_VELOCITY, KERNEL_CONTACT_P
     KERNEL_PAIRS_WRITE, KERNEL_SOLVER = range(6)
 
 
+EVENT_DTYPE = np.dtype([("kind", "<i4"), ("reserved", "<i4"), ("contact", CONTACT_DTYPE)])
 REQUEST_DTYPE = np.dtype([("arrival", "<f8"), ("world", "<i4"), ("substeps", "<i4"), ("dt", "<f4"),
                           ("reserved", "<i4")])
 RESULT_DTYPE = np.dtype([("arrival", "<f8"), ("admitted", "<f8"), ("completed", "<f8"), ("ticks", "<i4"),
@@ -153,6 +155,7 @@ def lib():
         L.b2g_set_option.argtypes = [_P, _I, _I]
         L.b2g_upload_contact_colours.argtypes = [_P, _I, _P]
         L.b2g_snapshot_save.argtypes = [_P, ctypes.c_char_p]
+        L.b2g_download_events.argtypes = [_P, _I, _P, _P]
         L.b2g_snapshot_load.argtypes = [_P, ctypes.c_char_p]
         L.b2g_get_option.argtypes = [_P, _I, _P]
         _lib = L
@@ -309,6 +312,15 @@ class Batch:
                     setattr(b, k, getattr(like, k))
             b.stepped = getattr(like, "stepped", False)
         return b
+
+    def events(self):
+        """Contact events of the last step taken with STEP_EVENTS."""
+        n = ctypes.c_int32(0)
+        check(self.L.b2g_download_events(self.h, 0, None, ctypes.byref(n)))
+        out = np.zeros(n.value, dtype=EVENT_DTYPE)
+        if n.value:
+            check(self.L.b2g_download_events(self.h, n.value, _ptr(out), ctypes.byref(n)))
+        return out
 
     def set_option(self, option, value):
         check(self.L.b2g_set_option(self.h, option, value))

@@ -786,6 +786,28 @@ void b2World::SyncJoints() const {
   const_cast<b2World*>(this)->m_jointsValid = true;
 }
 
+void b2World::ContactFromRecord(const b2g_contact& r, b2Contact* out) const {
+  b2Contact& c = *out;
+  c.m_fixtureA = m_fixtures[r.shape_a];
+  c.m_fixtureB = m_fixtures[r.shape_b];
+  c.m_touching = r.touching != 0;
+  b2Manifold& m = c.m_manifold;
+  m.type = (b2Manifold::Type)r.type;
+  m.pointCount = r.point_count;
+  m.localNormal.Set(r.local_normal[0], r.local_normal[1]);
+  m.localPoint.Set(r.local_point[0], r.local_point[1]);
+  for (int k = 0; k < 2; ++k) {
+    m.points[k].localPoint.Set(r.point_local[2 * k], r.point_local[2 * k + 1]);
+    m.points[k].normalImpulse = r.normal_impulse[k];
+    m.points[k].tangentImpulse = r.tangent_impulse[k];
+    m.points[k].id.key = r.id[k];
+  }
+  c.m_friction = r.friction;
+  c.m_restitution = r.restitution;
+  c.m_restitutionThreshold = r.restitution_threshold;
+  c.m_tangentSpeed = r.tangent_speed;
+}
+
 void b2World::SyncContacts() {
   if (m_contactsValid) return;
   m_contactsValid = true;
@@ -796,28 +818,7 @@ void b2World::SyncContacts() {
   std::vector<b2g_contact> rec(n);
   if (b2g_download_contacts(m_batch, 0, n, rec.data()) != B2G_OK) return;
   m_contacts.resize(n);
-  for (int i = 0; i < n; ++i) {
-    const b2g_contact& r = rec[i];
-    b2Contact& c = m_contacts[i];
-    c.m_fixtureA = m_fixtures[r.shape_a];
-    c.m_fixtureB = m_fixtures[r.shape_b];
-    c.m_touching = r.touching != 0;
-    b2Manifold& m = c.m_manifold;
-    m.type = (b2Manifold::Type)r.type;
-    m.pointCount = r.point_count;
-    m.localNormal.Set(r.local_normal[0], r.local_normal[1]);
-    m.localPoint.Set(r.local_point[0], r.local_point[1]);
-    for (int k = 0; k < 2; ++k) {
-      m.points[k].localPoint.Set(r.point_local[2 * k], r.point_local[2 * k + 1]);
-      m.points[k].normalImpulse = r.normal_impulse[k];
-      m.points[k].tangentImpulse = r.tangent_impulse[k];
-      m.points[k].id.key = r.id[k];
-    }
-    c.m_friction = r.friction;
-    c.m_restitution = r.restitution;
-    c.m_restitutionThreshold = r.restitution_threshold;
-    c.m_tangentSpeed = r.tangent_speed;
-  }
+  for (int i = 0; i < n; ++i) ContactFromRecord(rec[i], &m_contacts[i]);
   for (int i = 0; i + 1 < n; ++i) m_contacts[i].m_next = &m_contacts[i + 1];
 }
 
@@ -847,6 +848,7 @@ void b2World::Step(float dt, int32 velocityIterations, int32 positionIterations)
   if (!m_status) {
     uint32_t flags = B2G_STEP_PROFILE;
     if (m_newContacts) flags |= B2G_STEP_FIND_NEW_CONTACTS;
+    if (m_listener) flags |= B2G_STEP_EVENTS;
     int32 vi = velocityIterations, pi = positionIterations;
     if (dt <= 0.0f) {  // Collide only: no solve, no broadphase (b2_world.cpp:1141-1163)
       flags |= B2G_STEP_SKIP_BROADPHASE;
@@ -876,31 +878,21 @@ void b2World::Step(float dt, int32 velocityIterations, int32 positionIterations)
     b->m_force.SetZero();
     b->m_torque = 0.0f;
   }
-  if (m_listener) {
-    // Begin/End from touching transitions of the device contact list
-    SyncContacts();
-    std::vector<uint64_t> now;
-    for (b2Contact& c : m_contacts) {
-      if (!c.m_touching) continue;
-      const uint64_t a = (uint64_t)c.m_fixtureA->m_index, b = (uint64_t)c.m_fixtureB->m_index;
-      now.push_back((std::min(a, b) << 32) | std::max(a, b));
+  if (m_listener && !m_status) {
+    // Begin/End from the device event stream of this step: transitions seen
+    // by Collide, then touching contacts destroyed by the broadphase
+    int32_t n = 0;
+    if (b2g_download_events(m_batch, 0, nullptr, &n) == B2G_OK && n > 0) {
+      std::vector<b2g_contact_event> ev(n);
+      if (b2g_download_events(m_batch, n, ev.data(), &n) == B2G_OK) {
+        for (const b2g_contact_event& e : ev) {
+          b2Contact c;
+          ContactFromRecord(e.contact, &c);
+          if (e.kind == B2G_EVENT_BEGIN) m_listener->BeginContact(&c);
+          else m_listener->EndContact(&c);
+        }
+      }
     }
-    std::sort(now.begin(), now.end());
-    for (b2Contact& c : m_contacts) {
-      if (!c.m_touching) continue;
-      const uint64_t a = (uint64_t)c.m_fixtureA->m_index, b = (uint64_t)c.m_fixtureB->m_index;
-      const uint64_t k = (std::min(a, b) << 32) | std::max(a, b);
-      if (!std::binary_search(m_touchingKeys.begin(), m_touchingKeys.end(), k)) m_listener->BeginContact(&c);
-    }
-    for (uint64_t k : m_touchingKeys) {
-      if (std::binary_search(now.begin(), now.end(), k)) continue;
-      b2Contact ended;
-      ended.m_fixtureA = m_fixtures[(size_t)(k >> 32)];
-      ended.m_fixtureB = m_fixtures[(size_t)(k & 0xffffffffu)];
-      ended.m_manifold.pointCount = 0;
-      m_listener->EndContact(&ended);
-    }
-    m_touchingKeys.swap(now);
   }
   m_locked = false;
 }
@@ -912,10 +904,6 @@ struct b2a_world {
   b2World* world;
   std::vector<b2Body*> bodies;
 };
-
-extern "C" {
-
-}  // extern "C"
 
 // Builds a b2World from a scene description through the public API
 // (CreateBody / CreateFixture / CreateJoint), as user code would.

@@ -589,6 +589,7 @@ class b2World {
   static void JointToRecord(const b2Joint* j, b2g_revolute* out);
   void SyncBodies() const;
   void SyncContacts();
+  void ContactFromRecord(const b2g_contact& r, b2Contact* c) const;
   void SyncJoints() const;
 
   b2Vec2 m_gravity;
@@ -613,7 +614,6 @@ class b2World {
   float m_inv_dt0 = 0.0f;
   b2Profile m_profile;
   int32 m_status = 0;
-  std::vector<uint64_t> m_touchingKeys;  // for Begin/End events
 };
 
 #endif

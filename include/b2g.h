@@ -208,6 +208,7 @@ enum {
   B2G_STEP_PROFILE = 1u << 1,            /* time each stage with HIP events (adds syncs) */
   B2G_STEP_ASYNC = 1u << 2,              /* return without waiting for the device */
   B2G_STEP_SKIP_BROADPHASE = 1u << 3,    /* omit the end-of-step broadphase (testing only) */
+  B2G_STEP_EVENTS = 1u << 5,             /* record contact events (b2g_download_events) */
   B2G_STEP_GRAPH = 1u << 4               /* replay a captured hipGraph of the whole step: no host round
                                             trip; colour slots and launch sizes fixed at capture from the
                                             last eager step (re-captured after uploads). The first step
@@ -286,6 +287,21 @@ int b2g_download_contact_colours(b2g_batch* batch, int32_t first, int32_t count,
  * No reference counterpart: with b2g_upload_contacts it restores a solver
  * state exactly. */
 int b2g_upload_contact_colours(b2g_batch* batch, int32_t count, const int32_t* colours);
+
+/* Contact events of the last step taken with B2G_STEP_EVENTS: the calls
+ * b2ContactListener::BeginContact / EndContact receive in b2Contact::Update
+ * (src/dynamics/b2_contact.cpp:197-203) and b2ContactManager::Destroy
+ * (src/dynamics/b2_contact_manager.cpp:48-51). Collide's events come first,
+ * then the broadphase's, each group ordered by shape pair (the reference
+ * orders them by its contact list). `contact` holds the manifold as the
+ * listener would see it. *count = number of events (may exceed capacity). */
+enum { B2G_EVENT_BEGIN = 1, B2G_EVENT_END = 2, B2G_EVENT_DESTROYED = 3 };
+typedef struct b2g_contact_event {
+  int32_t kind;
+  int32_t reserved;
+  b2g_contact contact;
+} b2g_contact_event;
+int b2g_download_events(b2g_batch* batch, int32_t capacity, b2g_contact_event* out, int32_t* count);
 
 /* Binary snapshot of a batch: capacity, world params, bodies, shapes,
  * joints with their impulses, contacts with manifolds, impulses and colours.

@@ -174,3 +174,51 @@ def test_gpu_snapshot_resume_bitexact(engine, tmp_path):
     ca, cb = a.contacts(), b.contacts()
     assert np.array_equal(ca["shape_a"], cb["shape_a"])
     assert np.array_equal(bits(ca["normal_impulse"]), bits(cb["normal_impulse"]))
+
+
+def _touching_keys(contacts):
+    t = contacts[contacts["point_count"] > 0]
+    a = np.minimum(t["shape_a"], t["shape_b"]).astype(np.uint64)
+    b = np.maximum(t["shape_a"], t["shape_b"]).astype(np.uint64)
+    return set(((a << np.uint64(32)) | b).tolist())
+
+
+def _event_keys(ev, kinds):
+    e = ev[np.isin(ev["kind"], kinds)]["contact"]
+    a = np.minimum(e["shape_a"], e["shape_b"]).astype(np.uint64)
+    b = np.maximum(e["shape_a"], e["shape_b"]).astype(np.uint64)
+    return ((a << np.uint64(32)) | b).tolist()
+
+
+def test_gpu_contact_events_match_touching_transitions(engine):
+    """The device event stream (b2ContactListener Begin/EndContact) equals
+    the difference of the touching sets before and after each step; Begin
+    events carry a touching manifold, End events an empty one (Collide) or
+    the last touching one (destroyed by the broadphase); Collide's events
+    come before the broadphase's."""
+    sc = S.random_pile(400, seed=51, half_w=10.0, height=30.0)
+    b = engine.Batch.from_scenes([sc]).advance(3)
+    begins = ends = destroyed = 0
+    for k in range(60):
+        before = _touching_keys(b.contacts())
+        b.step(8, 3, engine.STEP_EVENTS)
+        after = _touching_keys(b.contacts())
+        ev = b.events()
+        beg = _event_keys(ev, [engine.EVENT_BEGIN])
+        end = _event_keys(ev, [engine.EVENT_END, engine.EVENT_DESTROYED])
+        assert len(beg) == len(set(beg)) and len(end) == len(set(end))
+        # a contact may begin and end within one step only if destroyed after beginning
+        assert set(beg) - set(end) == after - before, f"step {k}"
+        assert set(end) - set(beg) == before - after, f"step {k}"
+        assert (ev[ev["kind"] == engine.EVENT_BEGIN]["contact"]["point_count"] > 0).all()
+        assert (ev[ev["kind"] == engine.EVENT_END]["contact"]["point_count"] == 0).all()
+        assert (ev[ev["kind"] == engine.EVENT_DESTROYED]["contact"]["point_count"] > 0).all()
+        kinds = ev["kind"].tolist()
+        if engine.EVENT_DESTROYED in kinds:
+            first = kinds.index(engine.EVENT_DESTROYED)
+            assert all(x == engine.EVENT_DESTROYED for x in kinds[first:])
+        begins += len(beg)
+        ends += int((ev["kind"] == engine.EVENT_END).sum())
+        destroyed += int((ev["kind"] == engine.EVENT_DESTROYED).sum())
+    print(f"events over 60 steps: {begins} begin, {ends} end, {destroyed} destroyed", flush=True)
+    assert begins > 0
