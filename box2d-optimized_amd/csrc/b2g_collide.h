@@ -534,6 +534,79 @@ __device__ __forceinline__ bool evaluate(Man& m, const Geo& a, const xform& xa, 
   return true;
 }
 
+// ---- ray casts: b2CircleShape::RayCast (src/collision/b2_circle_shape.cpp:56-86),
+// b2EdgeShape::RayCast (b2_edge_shape.cpp:89-154), b2PolygonShape::RayCast
+// (b2_polygon_shape.cpp:303-371); same operation order, so the hits match the
+// host's bit for bit when compiled without contraction.
+B2G_INLINE bool raycast_shape(const Geo& g, const xform& xf, vec2 in1, vec2 in2, float max_fraction,
+                              float& fraction, vec2& normal) {
+  if (g.type == kCircle) {
+    const vec2 position = xf.p + mul(xf.q, g.centroid);
+    const vec2 s = in1 - position;
+    const float b = dot(s, s) - g.radius * g.radius;
+    const vec2 r = in2 - in1;
+    const float c = dot(s, r);
+    const float rr = dot(r, r);
+    const float sigma = c * c - rr * b;
+    if (sigma < 0.0f || rr < kEpsilon) return false;
+    float a = -(c + sqrtf(sigma));
+    if (0.0f <= a && a <= max_fraction * rr) {
+      a /= rr;
+      fraction = a;
+      normal = normalize(s + a * r);
+      return true;
+    }
+    return false;
+  }
+  const vec2 p1 = mul_t(xf.q, in1 - xf.p);
+  const vec2 p2 = mul_t(xf.q, in2 - xf.p);
+  const vec2 d = p2 - p1;
+  if (g.type == kEdge) {
+    const vec2 v1 = g.v[1], v2 = g.v[2];
+    const vec2 e = v2 - v1;
+    const vec2 nrm = normalize(mk(e.y, -e.x));
+    const float numerator = dot(nrm, v1 - p1);
+    if (g.count != 0 && numerator > 0.0f) return false;
+    const float denominator = dot(nrm, d);
+    if (denominator == 0.0f) return false;
+    const float t = numerator / denominator;
+    if (t < 0.0f || max_fraction < t) return false;
+    const vec2 q = p1 + t * d;
+    const vec2 r = v2 - v1;
+    const float rr = dot(r, r);
+    if (rr == 0.0f) return false;
+    const float s = dot(q - v1, r) / rr;
+    if (s < 0.0f || 1.0f < s) return false;
+    fraction = t;
+    normal = numerator > 0.0f ? -mul(xf.q, nrm) : mul(xf.q, nrm);
+    return true;
+  }
+  float lower = 0.0f, upper = max_fraction;
+  int index = -1;
+#pragma unroll
+  for (int i = 0; i < kMaxVerts; ++i) {
+    if (i < g.count) {
+      const float numerator = dot(g.n[i], g.v[i] - p1);
+      const float denominator = dot(g.n[i], d);
+      if (denominator == 0.0f) {
+        if (numerator < 0.0f) return false;
+      } else if (denominator < 0.0f && numerator < lower * denominator) {
+        lower = numerator / denominator;
+        index = i;
+      } else if (denominator > 0.0f && numerator < upper * denominator) {
+        upper = numerator / denominator;
+      }
+      if (upper < lower) return false;
+    }
+  }
+  if (index >= 0) {
+    fraction = lower;
+    normal = mul(xf.q, pick(g.n, index));
+    return true;
+  }
+  return false;
+}
+
 B2G_INLINE bool has_collide_function(int ta, int tb) {
   return (ta == kCircle && tb == kCircle) || (ta == kPolygon && (tb == kCircle || tb == kPolygon)) ||
          (ta == kEdge && (tb == kCircle || tb == kPolygon));

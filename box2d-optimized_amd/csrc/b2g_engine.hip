@@ -161,7 +161,12 @@ struct b2g_batch {
   int graph_step(int vi, int pi, unsigned flags);
   int scan(const int* in, const int* n_dev, int n_host, int* out, int* total, long long n_bound);
   int radix_sort(int n, int bits);
+  int build_tree(bool use_given_aabbs);
   int broadphase(bool use_given_aabbs);
+  bool tree_stale = true;     // bodies or shapes changed since the last tree build
+  void* qbuf = nullptr;       // scratch of the user queries
+  size_t qbuf_bytes = 0;
+  int query_scratch(size_t bytes);
   int rebuild_joint_tables();
   int step(int vi, int pi, unsigned flags);
   int read_counters();
@@ -224,6 +229,8 @@ int b2g_batch::alloc() {
 
 void b2g_batch::release() {
   graphs_clear();
+  if (qbuf) (void)hipFree(qbuf);
+  qbuf = nullptr;
   void* ptrs[] = {d.xf, d.pos, d.vel, d.prop, d.prop2, d.force, d.btype, d.bworld, d.wpar, d.wflag, d.wshape0, d.wshape1, d.wbody, d.worder, d.wcol,
                   d.wbounds, d.wleaf, d.shdr, d.geo, d.smat, d.sfilt, d.aabb, d.joints, d.jsol,
                   d.jorder, d.jisland, d.bj_off, d.bj_other, d.ckey, d.cpair, d.cman, d.cmat, d.ccol,
@@ -267,18 +274,13 @@ int b2g_batch::radix_sort(int n, int bits) {
   return src;  // buffer holding the result (0 for an even number of passes)
 }
 
-int b2g_batch::broadphase(bool use_given_aabbs) {
+// The tree over the current AABBs (refreshed from the transforms unless the
+// caller gave AABBs): world bounds, Morton keys, radix sort, Karras build,
+// sparse table and node records.
+int b2g_batch::build_tree(bool use_given_aabbs) {
   const int ns = d.ns;
-  if (ns == 0) {
-    hipLaunchKernelGGL(k_clear_pairs, dim3(1), dim3(1), 0, stream, d);
-    hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, stream, d);
-    return B2G_OK;
-  }
+  if (ns == 0) return B2G_OK;
   const int gs = grid_for(ns);
-  if (filters_dirty) {  // bodies, shapes or joints changed: packed pair-filter words
-    hipLaunchKernelGGL(k_shape_filters, dim3(gs), dim3(kBlock), 0, stream, d);
-    filters_dirty = false;
-  }
   hipLaunchKernelGGL(k_world_bounds_reset, dim3(grid_for(d.nw)), dim3(kBlock), 0, stream, d);
   if (!use_given_aabbs) {
     hipLaunchKernelGGL(k_aabb, dim3(gs), dim3(kBlock), 0, stream, d);
@@ -307,6 +309,24 @@ int b2g_batch::broadphase(bool use_given_aabbs) {
     }
     hipLaunchKernelGGL(k_node_records, dim3(grid_for(ns - 1)), dim3(kBlock), 0, stream, d, d.mkey[0], d.mval[0]);
   }
+  tree_stale = false;
+  return B2G_OK;
+}
+
+int b2g_batch::broadphase(bool use_given_aabbs) {
+  const int ns = d.ns;
+  if (ns == 0) {
+    hipLaunchKernelGGL(k_clear_pairs, dim3(1), dim3(1), 0, stream, d);
+    hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, stream, d);
+    return B2G_OK;
+  }
+  const int gs = grid_for(ns);
+  if (filters_dirty) {  // bodies, shapes or joints changed: packed pair-filter words
+    hipLaunchKernelGGL(k_shape_filters, dim3(gs), dim3(kBlock), 0, stream, d);
+    filters_dirty = false;
+  }
+  int rc = build_tree(use_given_aabbs);
+  if (rc) return rc;
   tbegin(B2G_KERNEL_PAIRS_WRITE);
   const int gq = grid_for((long long)ns * kBlock / kQueryBlock);  // blocks of kQueryBlock threads
   hipLaunchKernelGGL(k_lbvh_pairs, dim3(gq), dim3(kQueryBlock), 0, stream, d);
@@ -564,6 +584,7 @@ int b2g_batch::step(int vi, int pi, unsigned flags) {
     round_host = 0;
   }
   d.events_on = (flags & B2G_STEP_EVENTS) ? 1 : 0;
+  if (flags & B2G_STEP_SKIP_BROADPHASE) tree_stale = true;  // bodies move without a rebuild
   if (d.events_on) hipLaunchKernelGGL(k_events_reset, dim3(1), dim3(1), 0, stream, d);
   const bool prof_on = (flags & B2G_STEP_PROFILE) != 0;
   const bool graph_ok = (flags & B2G_STEP_GRAPH) && !prof_on && timed_kernel < 0 &&
@@ -918,6 +939,7 @@ int b2g_upload_bodies(b2g_batch* b, int32_t first, int32_t count, const b2g_body
   if (!b || first < 0 || count < 0 || first + count > b->cap.bodies) return fail(B2G_ERR_INVALID, "body range");
   if (count == 0) return B2G_OK;
   b->filters_dirty = true;  // body types feed the pair filter
+  b->tree_stale = true;
   if (first + count > b->d.nb) b->layout_dirty = true;
   std::vector<float4> xf(count), pos(count), vel(count), prop(count), prop2(count), force(count);
   std::vector<int> type(count), world(count);
@@ -969,6 +991,7 @@ int b2g_upload_bodies(b2g_batch* b, int32_t first, int32_t count, const b2g_body
 
 int b2g_upload_shapes(b2g_batch* b, int32_t first, int32_t count, const b2g_shape* in) {
   if (!b || first < 0 || count < 0 || first + count > b->cap.shapes) return fail(B2G_ERR_INVALID, "shape range");
+  b->tree_stale = true;
   b->graphs_clear();  // captured launch sizes no longer match
   b->graph_K = 0;
   b->filters_dirty = true;
@@ -1253,6 +1276,82 @@ int b2g_download_contact_colours(b2g_batch* b, int32_t first, int32_t count, int
   // colours are kept per contact in the (post-broadphase) contact buffer
   HIP_TRY(hipMemcpy(out, b->d.ccol + first, count * sizeof(int), hipMemcpyDeviceToHost));
   return B2G_OK;
+}
+
+}  // extern "C"
+
+int b2g_batch::query_scratch(size_t bytes) {
+  if (bytes <= qbuf_bytes) return B2G_OK;
+  if (qbuf) (void)hipFree(qbuf);
+  qbuf = nullptr;
+  qbuf_bytes = 0;
+  const size_t want = std::max<size_t>(bytes, 1 << 16);
+  hipError_t e = hipMalloc(&qbuf, want);
+  if (e != hipSuccess) return fail(B2G_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  qbuf_bytes = want;
+  return B2G_OK;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// shared by the AABB and ray queries: inputs (n float4 + n worlds) in, up to
+// `cap` results of `rec` bytes per query + counts out
+template <typename Launch>
+static int run_queries(b2g_batch* b, int32_t n, const int32_t* worlds, const float* in4, int32_t cap, size_t rec,
+                       void* out, int32_t* counts, Launch launch) {
+  if (!b || n < 0 || cap < 0 || (n && (!worlds || !in4 || !counts)) || (n && cap && !out))
+    return fail(B2G_ERR_INVALID, "null argument");
+  for (int i = 0; i < n; ++i)
+    if (worlds[i] < 0 || worlds[i] >= b->d.nw) return fail(B2G_ERR_INVALID, "query world index");
+  if (n == 0) return B2G_OK;
+  if (b->tree_stale) {  // bodies or shapes changed since the last broadphase
+    int rc = b->build_tree(false);
+    if (rc) return rc;
+  }
+  const size_t o_in = 0, o_w = align256(o_in + 16 * (size_t)n), o_out = align256(o_w + 4 * (size_t)n),
+               o_cnt = align256(o_out + rec * (size_t)n * (size_t)cap), total = o_cnt + 4 * (size_t)n;
+  int rc = b->query_scratch(total);
+  if (rc) return rc;
+  char* base = (char*)b->qbuf;
+  HIP_TRY(hipMemcpyAsync(base + o_in, in4, 16 * (size_t)n, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipMemcpyAsync(base + o_w, worlds, 4 * (size_t)n, hipMemcpyHostToDevice, b->stream));
+  launch((const float4*)(base + o_in), (const int*)(base + o_w), (void*)(base + o_out), (int*)(base + o_cnt));
+  HIP_TRY(hipMemcpyAsync(counts, base + o_cnt, 4 * (size_t)n, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  if (cap) {
+    // copy back each query's slots that hold results
+    std::vector<char> tmp(rec * (size_t)n * (size_t)cap);
+    HIP_TRY(hipMemcpy(tmp.data(), base + o_out, tmp.size(), hipMemcpyDeviceToHost));
+    std::memcpy(out, tmp.data(), tmp.size());
+  }
+  rc = b->read_counters();
+  if (rc) return rc;
+  if (b->pinned[C_STATUS] & S_STACK_OVERFLOW) return fail(B2G_ERR_CAPACITY, "BVH traversal stack overflow");
+  if (b->pinned[C_STATUS] & S_TREE_CORRUPT) return fail(B2G_ERR_INVALID, "BVH inconsistent (sort/build)");
+  return B2G_OK;
+}
+
+extern "C" {
+
+int b2g_query_aabbs(b2g_batch* b, int32_t n, const int32_t* worlds, const float* aabbs, int32_t capacity,
+                    int32_t* out_shapes, int32_t* out_counts) {
+  return run_queries(b, n, worlds, aabbs, capacity, sizeof(int32_t), out_shapes, out_counts,
+                     [&](const float4* in, const int* w, void* out, int* cnt) {
+                       const int g = grid_for((long long)n * kBlock / kQueryBlock);
+                       hipLaunchKernelGGL(k_query_aabbs, dim3(g), dim3(kQueryBlock), 0, b->stream, b->d, n, in, w,
+                                          capacity, (int*)out, cnt);
+                     });
+}
+
+int b2g_raycast(b2g_batch* b, int32_t n, const int32_t* worlds, const float* segments, int32_t capacity,
+                b2g_ray_hit* out_hits, int32_t* out_counts) {
+  static_assert(sizeof(b2g_ray_hit) == sizeof(float4), "ray hit record is one float4");
+  return run_queries(b, n, worlds, segments, capacity, sizeof(b2g_ray_hit), out_hits, out_counts,
+                     [&](const float4* in, const int* w, void* out, int* cnt) {
+                       const int g = grid_for((long long)n * kBlock / kQueryBlock);
+                       hipLaunchKernelGGL(k_raycast, dim3(g), dim3(kQueryBlock), 0, b->stream, b->d, n, in, w,
+                                          capacity, (float4*)out, cnt);
+                     });
 }
 
 int b2g_download_events(b2g_batch* b, int32_t capacity, b2g_contact_event* out, int32_t* count) {

@@ -1817,6 +1817,142 @@ __global__ void __launch_bounds__(kQueryBlock) k_pairs_compact(Dev d) {
   }
 }
 
+// ---- user queries on the tree of the last broadphase ---------------------
+// b2World::QueryAABB (src/dynamics/b2_world.cpp:1193-1198 ->
+// b2BroadPhase::QueryAABB, include/box2d/b2_broad_phase.h:622-642): shapes of
+// world w whose tight AABB overlaps the box; a world with one shape reports
+// it unconditionally (its root is that leaf). Results beyond `cap` are
+// counted, not written.
+__global__ void __launch_bounds__(kQueryBlock) k_query_aabbs(Dev d, int n, const float4* __restrict__ boxes,
+                                                             const int* __restrict__ worlds, int cap,
+                                                             int* __restrict__ out, int* __restrict__ counts) {
+  __shared__ int stk[kStack * kQueryBlock];
+  const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
+  B2G_FOR_ALL(qi, valid, 0, n) {
+    if (valid) {
+      const float4 q = boxes[qi];
+      const int w = worlds[qi];
+      const int wlo = d.wleaf[2 * w], whi = d.wleaf[2 * w + 1] - 1;
+      int* o = out + (size_t)qi * cap;
+      int found = 0;
+      if (whi < wlo) {
+        // no shapes in this world
+      } else if (wlo == whi || d.ns < 2) {
+        if (cap > 0) o[0] = (int)d.mval[0][wlo];
+        found = 1;
+      } else {
+        int sp = 1;
+        stk[tid] = 0;
+        int visits = 0;
+        while (sp > 0) {
+          const int node = stk[--sp * nt + tid];
+          if (++visits > d.ns) {
+            atomicOr(&d.cnt[C_STATUS], S_TREE_CORRUPT);
+            break;
+          }
+          const float4* rec = d.node_rec + 3 * (size_t)node;
+          const float4 lb = rec[0], rb = rec[1], cw = rec[2];
+          const int split = __float_as_int(cw.z);
+#pragma unroll
+          for (int side = 0; side < 2; ++side) {
+            if (side == 0 ? split < wlo : split + 1 > whi) continue;
+            if (!overlap(q, side == 0 ? lb : rb)) continue;
+            const int code = __float_as_int(side == 0 ? cw.x : cw.y);
+            if (code < 0) {
+              if (found < cap) o[found] = ~code;
+              ++found;
+            } else if (sp < kStack) {
+              stk[sp++ * nt + tid] = code;
+            } else {
+              atomicOr(&d.cnt[C_STATUS], S_STACK_OVERFLOW);
+            }
+          }
+        }
+      }
+      counts[qi] = found;
+    }
+  }
+}
+
+// b2World::RayCast (src/dynamics/b2_world.cpp:1228-1238 -> b2BroadPhase::RayCast,
+// include/box2d/b2_broad_phase.h:646-716) over the whole segment: every shape
+// of world w the segment hits (fraction in [0, 1]), pruned as the reference
+// prunes at maxFraction 1 (segment AABB and the separating axis of the
+// segment). The caller applies its callback's clipping in fraction order.
+// Hits: (shape, fraction, normal.x, normal.y) per slot.
+__global__ void __launch_bounds__(kQueryBlock) k_raycast(Dev d, int n, const float4* __restrict__ segs,
+                                                         const int* __restrict__ worlds, int cap,
+                                                         float4* __restrict__ out, int* __restrict__ counts) {
+  __shared__ int stk[kStack * kQueryBlock];
+  const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
+  B2G_FOR_ALL(qi, valid, 0, n) {
+    if (valid) {
+      const float4 sg = segs[qi];
+      const vec2 p1 = mk(sg.x, sg.y), p2 = mk(sg.z, sg.w);
+      const int w = worlds[qi];
+      const int wlo = d.wleaf[2 * w], whi = d.wleaf[2 * w + 1] - 1;
+      const vec2 r = normalize(p2 - p1);
+      const vec2 v = cross(1.0f, r);
+      const vec2 abs_v = mk(absf(v.x), absf(v.y));
+      const vec2 t = p1 + 1.0f * (p2 - p1);
+      const float4 seg_box = make_float4(minf(p1.x, t.x), minf(p1.y, t.y), maxf(p1.x, t.x), maxf(p1.y, t.y));
+      float4* o = out + (size_t)qi * cap;
+      int found = 0;
+      auto test_box = [&](float4 b) {
+        if (!overlap(b, seg_box)) return false;
+        const vec2 c = mk(0.5f * (b.x + b.z), 0.5f * (b.y + b.w));
+        const vec2 h = mk(0.5f * (b.z - b.x), 0.5f * (b.w - b.y));
+        const float separation = absf(dot(v, p1 - c)) - dot(abs_v, h);
+        return !(separation > 0.0f);
+      };
+      auto test_leaf = [&](int j) {
+        const int4 hj = d.shdr[j];
+        Geo g;
+        load_geo(g, hj.x, hj.z, d.geo + 9 * (size_t)j);
+        float fraction = 0.0f;
+        vec2 normal = mk(0.0f, 0.0f);
+        if (raycast_shape(g, xform_of(d.xf[hj.y]), p1, p2, 1.0f, fraction, normal)) {
+          if (found < cap) o[found] = make_float4(__int_as_float(j), fraction, normal.x, normal.y);
+          ++found;
+        }
+      };
+      if (whi < wlo) {
+        // no shapes in this world
+      } else if (wlo == whi || d.ns < 2) {
+        if (test_box(d.sparse[wlo])) test_leaf((int)d.mval[0][wlo]);
+      } else {
+        int sp = 1;
+        stk[tid] = 0;
+        int visits = 0;
+        while (sp > 0) {
+          const int node = stk[--sp * nt + tid];
+          if (++visits > d.ns) {
+            atomicOr(&d.cnt[C_STATUS], S_TREE_CORRUPT);
+            break;
+          }
+          const float4* rec = d.node_rec + 3 * (size_t)node;
+          const float4 lb = rec[0], rb = rec[1], cw = rec[2];
+          const int split = __float_as_int(cw.z);
+#pragma unroll
+          for (int side = 0; side < 2; ++side) {
+            if (side == 0 ? split < wlo : split + 1 > whi) continue;
+            if (!test_box(side == 0 ? lb : rb)) continue;
+            const int code = __float_as_int(side == 0 ? cw.x : cw.y);
+            if (code < 0) {
+              test_leaf(~code);
+            } else if (sp < kStack) {
+              stk[sp++ * nt + tid] = code;
+            } else {
+              atomicOr(&d.cnt[C_STATUS], S_STACK_OVERFLOW);
+            }
+          }
+        }
+      }
+      counts[qi] = found;
+    }
+  }
+}
+
 // New contact list: persisting pairs keep their contact (orientation,
 // manifold, impulses, material, colour); new pairs start empty with the
 // factory's orientation and the mixing laws (src/dynamics/b2_contact.cpp:58-124,

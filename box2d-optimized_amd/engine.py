@@ -83,7 +83,7 @@ EXPORTS = [
     "b2a_world_contact_count", "b2a_world_get_contacts", "b2a_world_batch", "b2a_world_prepare",
     "b2g_time_kernel", "b2g_kernel_time", "b2a_scene_records", "b2a_batch_from_scenes", "b2s_run",
     "b2g_set_option", "b2g_get_option", "b2g_upload_contact_colours", "b2g_snapshot_save",
-    "b2g_snapshot_load", "b2g_download_events",
+    "b2g_snapshot_load", "b2g_download_events", "b2g_query_aabbs", "b2g_raycast",
 ]
 SOLVER_AUTO, SOLVER_COLOURS, SOLVER_WORLDS = 0, 1, 2
 OPT_SOLVER, OPT_WORLD_BLOCK, OPT_TUNED_WORLD_BLOCK, OPT_LAST_SOLVER, OPT_PER_WORLD_ELIGIBLE = 1, 2, 3, 4, 5
@@ -92,6 +92,7 @@ KERNEL_NARROWPHASE, KERNEL_WARM_START, KERNEL_CONTACT_VELOCITY, KERNEL_CONTACT_P
 
 
 EVENT_DTYPE = np.dtype([("kind", "<i4"), ("reserved", "<i4"), ("contact", CONTACT_DTYPE)])
+RAY_HIT_DTYPE = np.dtype([("shape", "<i4"), ("fraction", "<f4"), ("normal", "<f4", 2)])
 REQUEST_DTYPE = np.dtype([("arrival", "<f8"), ("world", "<i4"), ("substeps", "<i4"), ("dt", "<f4"),
                           ("reserved", "<i4")])
 RESULT_DTYPE = np.dtype([("arrival", "<f8"), ("admitted", "<f8"), ("completed", "<f8"), ("ticks", "<i4"),
@@ -156,6 +157,8 @@ def lib():
         L.b2g_upload_contact_colours.argtypes = [_P, _I, _P]
         L.b2g_snapshot_save.argtypes = [_P, ctypes.c_char_p]
         L.b2g_download_events.argtypes = [_P, _I, _P, _P]
+        L.b2g_query_aabbs.argtypes = [_P, _I, _P, _P, _I, _P, _P]
+        L.b2g_raycast.argtypes = [_P, _I, _P, _P, _I, _P, _P]
         L.b2g_snapshot_load.argtypes = [_P, ctypes.c_char_p]
         L.b2g_get_option.argtypes = [_P, _I, _P]
         _lib = L
@@ -312,6 +315,30 @@ class Batch:
                     setattr(b, k, getattr(like, k))
             b.stepped = getattr(like, "stepped", False)
         return b
+
+    def query_aabbs(self, worlds, aabbs, capacity=64):
+        """Shapes overlapping each AABB: list of int arrays (b2g_query_aabbs)."""
+        w = np.ascontiguousarray(worlds, dtype=np.int32)
+        a = np.ascontiguousarray(aabbs, dtype=np.float32).reshape(len(w), 4)
+        while True:
+            out = np.zeros((len(w), capacity), np.int32)
+            cnt = np.zeros(len(w), np.int32)
+            check(self.L.b2g_query_aabbs(self.h, len(w), _ptr(w), _ptr(a), capacity, _ptr(out), _ptr(cnt)))
+            if len(cnt) == 0 or cnt.max() <= capacity:
+                return [out[i, :cnt[i]].copy() for i in range(len(w))]
+            capacity = int(cnt.max())
+
+    def raycast(self, worlds, segments, capacity=64):
+        """All hits of each segment (p1, p2): list of RAY_HIT_DTYPE arrays (b2g_raycast)."""
+        w = np.ascontiguousarray(worlds, dtype=np.int32)
+        sg = np.ascontiguousarray(segments, dtype=np.float32).reshape(len(w), 4)
+        while True:
+            out = np.zeros((len(w), capacity), RAY_HIT_DTYPE)
+            cnt = np.zeros(len(w), np.int32)
+            check(self.L.b2g_raycast(self.h, len(w), _ptr(w), _ptr(sg), capacity, _ptr(out), _ptr(cnt)))
+            if len(cnt) == 0 or cnt.max() <= capacity:
+                return [out[i, :cnt[i]].copy() for i in range(len(w))]
+            capacity = int(cnt.max())
 
     def events(self):
         """Contact events of the last step taken with STEP_EVENTS."""

@@ -786,6 +786,49 @@ void b2World::SyncJoints() const {
   const_cast<b2World*>(this)->m_jointsValid = true;
 }
 
+void b2World::QueryAABB(b2QueryCallback* callback, const b2AABB& aabb) {
+  if (!callback || m_fixtures.empty()) return;
+  if ((m_status = EnsureDevice()) || (m_status = Upload())) return;
+  const float box[4] = {aabb.lowerBound.x, aabb.lowerBound.y, aabb.upperBound.x, aabb.upperBound.y};
+  const int32_t world = 0;
+  std::vector<int32_t> out(64);
+  int32_t n = 0;
+  for (;;) {
+    if ((m_status = b2g_query_aabbs(m_batch, 1, &world, box, (int32_t)out.size(), out.data(), &n))) return;
+    if (n <= (int32_t)out.size()) break;
+    out.resize(n);
+  }
+  for (int32_t i = 0; i < n; ++i) callback->ReportFixture(m_fixtures[out[i]]);
+}
+
+void b2World::RayCast(b2RayCastCallback* callback, const b2Vec2& point1, const b2Vec2& point2) {
+  if (!callback || m_fixtures.empty()) return;
+  if ((m_status = EnsureDevice()) || (m_status = Upload())) return;
+  const float seg[4] = {point1.x, point1.y, point2.x, point2.y};
+  const int32_t world = 0;
+  std::vector<b2g_ray_hit> hits(64);
+  int32_t n = 0;
+  for (;;) {
+    if ((m_status = b2g_raycast(m_batch, 1, &world, seg, (int32_t)hits.size(), hits.data(), &n))) return;
+    if (n <= (int32_t)hits.size()) break;
+    hits.resize(n);
+  }
+  hits.resize(n);
+  std::sort(hits.begin(), hits.end(), [](const b2g_ray_hit& a, const b2g_ray_hit& b) {
+    return a.fraction != b.fraction ? a.fraction < b.fraction : a.shape < b.shape;
+  });
+  // the reference's clipping (include/box2d/b2_broad_phase.h:696-712), in fraction order
+  float max_fraction = 1.0f;
+  for (const b2g_ray_hit& h : hits) {
+    if (h.fraction > max_fraction) break;
+    const float f = h.fraction;
+    const b2Vec2 point = (1.0f - f) * point1 + f * point2;
+    const float value = callback->ReportFixture(m_fixtures[h.shape], point, b2Vec2(h.normal[0], h.normal[1]), f);
+    if (value == 0.0f) return;
+    if (value > 0.0f) max_fraction = value;
+  }
+}
+
 void b2World::ContactFromRecord(const b2g_contact& r, b2Contact* out) const {
   b2Contact& c = *out;
   c.m_fixtureA = m_fixtures[r.shape_a];

@@ -534,6 +534,20 @@ class b2ContactListener {
   }
 };
 
+// include/box2d/b2_world_callbacks.h:213-264
+class b2QueryCallback {
+ public:
+  virtual ~b2QueryCallback() {}
+  virtual bool ReportFixture(b2Fixture* fixture) = 0;
+};
+
+class b2RayCastCallback {
+ public:
+  virtual ~b2RayCastCallback() {}
+  // -1 ignore and continue, 0 terminate, fraction clip, 1 continue
+  virtual float ReportFixture(b2Fixture* fixture, const b2Vec2& point, const b2Vec2& normal, float fraction) = 0;
+};
+
 class b2World {
  public:
   explicit b2World(const b2Vec2& gravity);
@@ -544,6 +558,12 @@ class b2World {
   b2Body* CreateBody(const b2BodyDef* def);
   b2Joint* CreateJoint(const b2JointDef* def);
   void Step(float timeStep, int32 velocityIterations, int32 positionIterations);
+  // fixtures whose AABB overlaps `aabb`, on the device tree (as in the
+  // reference's broadphase, the callback's return value does not stop it)
+  void QueryAABB(b2QueryCallback* callback, const b2AABB& aabb);
+  // fixtures hit by the segment, reported nearest first; the callback's
+  // return value clips (fraction), ignores (-1) or terminates (0)
+  void RayCast(b2RayCastCallback* callback, const b2Vec2& point1, const b2Vec2& point2);
   void ClearForces() {}
   void SetContactListener(b2ContactListener* listener) { m_listener = listener; }
   b2Body* GetBodyList() { return m_bodyListHead; }

@@ -303,6 +303,25 @@ typedef struct b2g_contact_event {
 } b2g_contact_event;
 int b2g_download_events(b2g_batch* batch, int32_t capacity, b2g_contact_event* out, int32_t* count);
 
+/* Queries on the tree of the last broadphase (rebuilt first when bodies or
+ * shapes changed since). b2g_query_aabbs: shapes of world worlds[i] whose
+ * AABB overlaps aabbs[4i..4i+3] (b2World::QueryAABB,
+ * src/dynamics/b2_world.cpp:1193-1198). b2g_raycast: every shape of world
+ * worlds[i] hit by the segment segments[4i..4i+3] = (p1, p2), with the hit's
+ * fraction and normal (b2World::RayCast, src/dynamics/b2_world.cpp:1228-1238,
+ * at maxFraction 1; a caller applies its callback's clipping in fraction
+ * order). Up to `capacity` results per query go to out[i * capacity + k];
+ * out_counts[i] is the full count (may exceed capacity). */
+typedef struct b2g_ray_hit {
+  int32_t shape;
+  float fraction;
+  float normal[2];
+} b2g_ray_hit;
+int b2g_query_aabbs(b2g_batch* batch, int32_t n, const int32_t* worlds, const float* aabbs, int32_t capacity,
+                    int32_t* out_shapes, int32_t* out_counts);
+int b2g_raycast(b2g_batch* batch, int32_t n, const int32_t* worlds, const float* segments, int32_t capacity,
+                b2g_ray_hit* out_hits, int32_t* out_counts);
+
 /* Binary snapshot of a batch: capacity, world params, bodies, shapes,
  * joints with their impulses, contacts with manifolds, impulses and colours.
  * A batch loaded from it continues bit for bit as the saved one would

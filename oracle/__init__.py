@@ -114,6 +114,10 @@ class Ref:
             lib.ref_collide.argtypes = [_P, _P, _P, _P, _P]
             lib.ref_world_manifold.argtypes = [_P, _P, _F, _P, _F, _P]
             lib.ref_world_profile.argtypes = [_P, _P]
+            lib.ref_world_query_aabb.argtypes = [_P, _P, _I, _P]
+            lib.ref_world_query_aabb.restype = _I
+            lib.ref_world_raycast_all.argtypes = [_P, _P, _I, _P]
+            lib.ref_world_raycast_all.restype = _I
             cls._lib = lib
         return cls._lib
 
@@ -164,6 +168,23 @@ class RefWorld:
         if self.nj:
             self.lib.ref_world_get_joint_impulses(self.h, _ptr(out))
         return out
+
+    def query_aabb(self, box) -> np.ndarray:
+        """Fixture indices b2World::QueryAABB reports for the box."""
+        b = np.ascontiguousarray(box, dtype=np.float32)
+        n = self.lib.ref_world_query_aabb(self.h, _ptr(b), 0, None)
+        out = np.zeros(max(n, 1), np.int32)
+        self.lib.ref_world_query_aabb(self.h, _ptr(b), n, _ptr(out))
+        return out[:n]
+
+    def raycast_all(self, seg) -> np.ndarray:
+        """Every hit b2World::RayCast reports for the segment (callback
+        returning 1): rows of (fixture, fraction, normal.x, normal.y, point.x, point.y)."""
+        sg = np.ascontiguousarray(seg, dtype=np.float32)
+        n = self.lib.ref_world_raycast_all(self.h, _ptr(sg), 0, None)
+        out = np.zeros((max(n, 1), 6), np.float32)
+        self.lib.ref_world_raycast_all(self.h, _ptr(sg), n, _ptr(out))
+        return out[:n]
 
     def profile(self):
         out = np.zeros(4, dtype=np.float32)

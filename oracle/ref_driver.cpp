@@ -480,3 +480,58 @@ extern "C" void ref_world_get_aabbs(void* w, float* out) {
     out[4 * i + 3] = a.upperBound.y;
   }
 }
+
+// ---- user queries through the reference's b2World (QueryAABB / RayCast) ----
+namespace {
+struct QueryCollect : public b2QueryCallback {
+  std::unordered_map<const b2Fixture*, int>* index;
+  std::vector<int> found;
+  bool ReportFixture(b2Fixture* f) override {
+    found.push_back((*index)[f]);
+    return true;
+  }
+};
+
+// every fixture the ray hits: return 1 keeps the segment unclipped
+struct RayCollectAll : public b2RayCastCallback {
+  std::unordered_map<const b2Fixture*, int>* index;
+  std::vector<float> hits;  // index, fraction, normal.x, normal.y, point.x, point.y
+  float ReportFixture(b2Fixture* f, const b2Vec2& p, const b2Vec2& n, float fraction) override {
+    hits.insert(hits.end(), {(float)(*index)[f], fraction, n.x, n.y, p.x, p.y});
+    return 1.0f;
+  }
+};
+
+std::unordered_map<const b2Fixture*, int> fixture_index(RefWorld* rw) {
+  std::unordered_map<const b2Fixture*, int> m;
+  for (size_t i = 0; i < rw->fixtures.size(); ++i) m[rw->fixtures[i]] = (int)i;
+  return m;
+}
+}  // namespace
+
+// fixture indices whose AABB overlaps the box; returns the count (<= cap written)
+extern "C" int ref_world_query_aabb(void* w, const float* box, int cap, int* out) {
+  RefWorld* rw = (RefWorld*)w;
+  auto idx = fixture_index(rw);
+  QueryCollect cb;
+  cb.index = &idx;
+  b2AABB a;
+  a.lowerBound.Set(box[0], box[1]);
+  a.upperBound.Set(box[2], box[3]);
+  rw->world.QueryAABB(&cb, a);
+  for (size_t i = 0; i < cb.found.size() && (int)i < cap; ++i) out[i] = cb.found[i];
+  return (int)cb.found.size();
+}
+
+// every hit of the segment (6 floats each); returns the count (<= cap written)
+extern "C" int ref_world_raycast_all(void* w, const float* seg, int cap, float* out) {
+  RefWorld* rw = (RefWorld*)w;
+  auto idx = fixture_index(rw);
+  RayCollectAll cb;
+  cb.index = &idx;
+  rw->world.RayCast(&cb, b2Vec2(seg[0], seg[1]), b2Vec2(seg[2], seg[3]));
+  const int n = (int)cb.hits.size() / 6;
+  for (int i = 0; i < n && i < cap; ++i)
+    for (int k = 0; k < 6; ++k) out[6 * i + k] = cb.hits[6 * i + k];
+  return n;
+}

@@ -106,7 +106,7 @@ def test_python_records_match_the_c_compiler_layout(tmp_path):
              "b2g_counts": ctypes.sizeof(eng.Counts), "b2g_profile": ctypes.sizeof(eng.Profile),
              "b2s_request": eng.REQUEST_DTYPE.itemsize, "b2s_result": eng.RESULT_DTYPE.itemsize,
              "b2s_config": ctypes.sizeof(eng.ServeConfig), "b2s_stats": ctypes.sizeof(eng.ServeStats),
-             "b2g_contact_event": eng.EVENT_DTYPE.itemsize}
+             "b2g_contact_event": eng.EVENT_DTYPE.itemsize, "b2g_ray_hit": eng.RAY_HIT_DTYPE.itemsize}
     src = tmp_path / "sizes.c"
     src.write_text('#include <stdio.h>\n#include "b2g.h"\nint main(void) {\n' +
                    "".join(f'  printf("{n} %zu\\n", sizeof({n}));\n' for n in names) + "  return 0;\n}\n")

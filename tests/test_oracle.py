@@ -381,3 +381,46 @@ def test_edge_circle_regions_closed_form():
     L = np.linalg.norm(d)
     assert n == 1 and np.allclose(nrm, d / L, atol=1e-6)
     assert abs(sep[0] - (L - 0.5 - POLY_R)) < 2e-6
+
+
+# ---- user queries of the reference world, pinned by brute force / closed forms
+def test_reference_query_aabb_matches_brute_force():
+    """b2World::QueryAABB of the compiled reference reports exactly the
+    fixtures whose AABB overlaps the box (inclusive), on a mixed scene."""
+    if not oracle.Ref.available():
+        pytest.skip("reference library not built here")
+    sc = S.random_pile(500, seed=61, half_w=15.0, height=30.0)
+    w = oracle.RefWorld(sc)
+    aabbs = oracle.ref_aabbs(w)
+    rs = S.Stream(62)
+    for k in range(40):
+        u = rs.uniform(4)
+        lo = np.array([-15 + 30 * u[0], 30 * u[1]], np.float32)
+        box = np.array([lo[0], lo[1], lo[0] + 0.5 + 6 * u[2], lo[1] + 0.5 + 6 * u[3]], np.float32)
+        got = set(w.query_aabb(box).tolist())
+        want = {i for i, a in enumerate(aabbs)
+                if a[2] >= box[0] and a[0] <= box[2] and a[3] >= box[1] and a[1] <= box[3]}
+        assert got == want, k
+
+
+def test_reference_raycast_closed_forms():
+    """b2World::RayCast on shapes with known hits: a circle of radius r at
+    distance D along the ray (fraction (D - r) / L, normal -ray), and an
+    axis-aligned box face (fraction, outward normal); a ray that misses
+    reports nothing."""
+    if not oracle.Ref.available():
+        pytest.skip("reference library not built here")
+    s = S.Scene(name="rays", gravity=(0.0, 0.0))
+    b = s.add_body(S.DYNAMIC, 10.0, 0.0)
+    s.add_circle(b, 1.0)
+    b2 = s.add_body(S.DYNAMIC, 0.0, 10.0)
+    s.add_box(b2, 2.0, 1.0)
+    w = oracle.RefWorld(s)
+    h = w.raycast_all(np.array([0.0, 0.0, 20.0, 0.0], np.float32))
+    assert len(h) == 1 and int(h[0, 0]) == 0
+    assert abs(h[0, 1] - 9.0 / 20.0) < 1e-6 and np.allclose(h[0, 2:4], [-1.0, 0.0], atol=1e-6)
+    assert np.allclose(h[0, 4:6], [9.0, 0.0], atol=1e-5)
+    h = w.raycast_all(np.array([0.5, 0.0, 0.5, 20.0], np.float32))
+    assert len(h) == 1 and int(h[0, 0]) == 1
+    assert abs(h[0, 1] - 9.0 / 20.0) < 1e-6 and np.allclose(h[0, 2:4], [0.0, -1.0], atol=1e-6)
+    assert len(w.raycast_all(np.array([0.0, 5.0, 20.0, 5.0], np.float32))) == 0
