@@ -1,25 +1,31 @@
 """Benchmark of the b2World::Step hot path on one MI355X.
 
-Default workload: the large random pile (20,000 dynamic shapes -- 60%
-convex polygons with 3-8 vertices, 40% circles -- in a static 200 x 200 m box
-of 4 edges; dt 1/60, 8 velocity / 3 position iterations, continuous physics
-and sleep off), the configuration the body-steps/s metric is quoted on.
+Headline workload: the large random pile.
+- 20,000 dynamic shapes: 60% convex polygons with 3-8 vertices, 40% circles.
+- A static 200 x 200 m box of 4 edges.
+- dt 1/60, 8 velocity / 3 position iterations, continuous physics and sleep
+  off: the configuration the body-steps/s metric is quoted on.
+- 500 steps: 20 warm-up steps through the drop-in b2World API, then 480
+  timed steps through the b2g C ABI. Each timed step replays the captured
+  step graph (--eager launches kernel by kernel), with no host
+  synchronisation between steps and the world resident in HBM.
+- `value` = dynamic bodies x timed steps / wall time.
 
-A step is one full b2World::Step of that world: narrowphase, coloured
-Gauss-Seidel solve and the LBVH broadphase, all on the GPU with the world
-resident in HBM. `value` is dynamic bodies x steps / wall time of the timed
-steps. The first W steps (warm-up) go through the drop-in b2World API; the K
-timed steps call the same kernels through the b2g C ABI without per-step
-host synchronisation.
+Further sections of the same JSON line:
+- `gas_chains`: 100k circles + 500 revolute chains, one world.
+- `batched`: 4096 worlds, about 1.14M bodies, in one launch per stage,
+  against a plain per-world loop.
+- `serving`: Poisson requests against a resident pool; p50/p99 latency
+  beside throughput.
 
-roofline: HIP events around every launch of the dominant kernel over a
-further measured segment give its average launch duration; algorithmic
-bytes per launch are the per-item payloads of the reference's data layout
-(counted once per pass) times the items the launch processed.
+roofline: HIP events around every launch of the dominant kernel, over a
+further measured segment, give its average launch duration. Algorithmic bytes
+per launch are the per-item payloads of the reference's data layout (counted
+once per pass) times the items the launch processed.
+
 cpu_baseline: the reference library itself (oracle/_ref, compiled from the
-reference sources) stepping the same scene on one host core for a bounded
-sample of steps; the CPU restatement stands in where the reference library is
-not present.
+reference sources) steps the same scene over the same steps on one host core.
+The CPU restatement stands in where the reference library is not present.
 """
 from __future__ import annotations
 
@@ -264,7 +270,7 @@ def copy_probe(mib: int, reps: int) -> None:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=480, help="timed steps (with the warm-up: the 500-step run)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="pile20k")
     ap.add_argument("--kernel", default="contact_velocity",
@@ -335,7 +341,7 @@ def main():
     if not args.no_cpu:
         cpu_steps = args.cpu_steps
         if cpu_steps < 0:
-            cpu_steps = min(args.steps, 400 if args.workload == "pile20k" else 1000 if args.workload == "pyramid20"
+            cpu_steps = min(args.steps, 480 if args.workload == "pile20k" else 1000 if args.workload == "pyramid20"
                             else 40)
         cpu = cpu_baseline(wl["make"](), args.warmup, cpu_steps)
 
