@@ -139,8 +139,10 @@ struct b2g_batch {
   int solver_mode = 0;       // B2G_SOLVER_AUTO / _COLOURS / _WORLDS
   int world_block = 0;       // threads per world workgroup (0: autotune)
   int tuned_block = 0;       // chosen by the autotuner (or world_block)
+  bool tuned_lds = false;    // autotuner: stage world bodies in LDS
+  int world_lds = -1;        // -1 autotune, 0 off, 1 on (when the worlds fit)
   int tune_step = 0;         // autotuner progress: two timed steps per candidate
-  float tune_ms[8] = {};
+  float tune_ms[16] = {};
   int last_path = 0;         // 1 colour-major sweeps, 2 per-world
 
   int alloc();
@@ -157,6 +159,8 @@ struct b2g_batch {
   int update_world_layout();
   bool use_per_world() const;
   int world_block_now() const;
+  bool world_lds_now() const;
+  void launch_solve_worlds(int vi, int pi);
   int eager_step(int vi, int pi, unsigned flags);
   int graph_step(int vi, int pi, unsigned flags);
   int scan(const int* in, const int* n_dev, int n_host, int* out, int* total, long long n_bound);
@@ -559,13 +563,32 @@ bool b2g_batch::use_per_world() const {
   return d.nw >= 2 && max_world_bodies <= 8192;
 }
 
+// autotuner candidates: workgroup size x bodies staged in LDS or not (one
+// timed step each); LDS staging needs the largest world to fit 63 KB
 static const int kWorldBlocks[] = {64, 128, 256, 512, 1024};
 constexpr int kWorldBlockCount = 5;
+constexpr int kLdsBodyLimit = (63 * 1024) / 16;
 
 int b2g_batch::world_block_now() const {
   if (world_block > 0) return world_block;
   if (tuned_block > 0) return tuned_block;
-  return kWorldBlocks[std::min(tune_step / 2, kWorldBlockCount - 1)];
+  return kWorldBlocks[std::min(tune_step, 2 * kWorldBlockCount - 1) % kWorldBlockCount];
+}
+
+bool b2g_batch::world_lds_now() const {
+  if (max_world_bodies > kLdsBodyLimit || world_lds == 0) return false;
+  if (world_lds > 0) return true;
+  if (world_block > 0 || tuned_block > 0) return tuned_lds;
+  return std::min(tune_step, 2 * kWorldBlockCount - 1) >= kWorldBlockCount;
+}
+
+void b2g_batch::launch_solve_worlds(int vi, int pi) {
+  const int wblock = world_block_now();
+  if (world_lds_now())
+    hipLaunchKernelGGL(k_solve_worlds<true>, dim3(d.nw), dim3(wblock), sizeof(float4) * (size_t)max_world_bodies,
+                       stream, d, vi, pi);
+  else
+    hipLaunchKernelGGL(k_solve_worlds<false>, dim3(d.nw), dim3(wblock), 0, stream, d, vi, pi);
 }
 
 int b2g_batch::step(int vi, int pi, unsigned flags) {
@@ -619,23 +642,24 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
     rounds_pending = 0;
     enqueue_islands_and_init();
     const bool tuning = world_block == 0 && tuned_block == 0;
-    const int wblock = world_block_now();
     if (tuning) HIP_TRY(hipEventRecord(ev[5], stream));
     tbegin(B2G_KERNEL_SOLVER);
-    hipLaunchKernelGGL(k_solve_worlds, dim3(d.nw), dim3(wblock), 0, stream, d, vi, pi);
+    launch_solve_worlds(vi, pi);
     tend(B2G_KERNEL_SOLVER, est_cons);
     if (tuning) {
       HIP_TRY(hipEventRecord(ev[6], stream));
       HIP_TRY(hipEventSynchronize(ev[6]));
       float ms = 0.0f;
       HIP_TRY(hipEventElapsedTime(&ms, ev[5], ev[6]));
-      const int cand = tune_step / 2;
-      tune_ms[cand] = (tune_step & 1) ? std::min(tune_ms[cand], ms) : ms;
-      if (++tune_step == 2 * kWorldBlockCount) {
+      const bool lds_ok = max_world_bodies <= kLdsBodyLimit && world_lds != 0;
+      const int ncand = lds_ok ? 2 * kWorldBlockCount : kWorldBlockCount;
+      tune_ms[tune_step] = ms;
+      if (++tune_step >= ncand) {
         int best = 0;
-        for (int c = 1; c < kWorldBlockCount; ++c)
+        for (int c = 1; c < ncand; ++c)
           if (tune_ms[c] < tune_ms[best]) best = c;
-        tuned_block = kWorldBlocks[best];
+        tuned_block = kWorldBlocks[best % kWorldBlockCount];
+        tuned_lds = best >= kWorldBlockCount;
       }
     }
     graph_K = kMaxColours;
@@ -729,7 +753,7 @@ int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
     round_host += (unsigned)rounds_pending;
     rounds_pending = 0;
     enqueue_islands_and_init();
-    hipLaunchKernelGGL(k_solve_worlds, dim3(d.nw), dim3(world_block_now()), 0, stream, d, vi, pi);
+    launch_solve_worlds(vi, pi);
     if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
       int rc = broadphase(false);
       if (rc) return rc;
@@ -756,7 +780,7 @@ int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
 
 int b2g_batch::graph_step(int vi, int pi, unsigned flags) {
   const unsigned key_flags = flags & (B2G_STEP_SKIP_BROADPHASE | B2G_STEP_EVENTS);
-  const int block = use_per_world() ? world_block_now() : 0;
+  const int block = use_per_world() ? world_block_now() * 2 + (world_lds_now() ? 1 : 0) : 0;
   last_path = block ? 2 : 1;
   if (filters_dirty) {  // refreshed outside the captured sequence
     hipLaunchKernelGGL(k_shape_filters, dim3(grid_for(d.ns)), dim3(kBlock), 0, stream, d);
@@ -1152,6 +1176,12 @@ int b2g_set_option(b2g_batch* b, int32_t option, int32_t value) {
       if (value < B2G_SOLVER_AUTO || value > B2G_SOLVER_WORLDS) return fail(B2G_ERR_INVALID, "solver mode");
       b->solver_mode = value;
       break;
+    case B2G_OPT_WORLD_LDS:
+      if (value < -1 || value > 1) return fail(B2G_ERR_INVALID, "world LDS staging: -1 (autotune), 0 or 1");
+      b->world_lds = value;
+      b->tuned_block = 0;
+      b->tune_step = 0;
+      break;
     case B2G_OPT_WORLD_BLOCK:
       if (value != 0 && (value < 64 || value > 1024 || (value & 63))) {
         return fail(B2G_ERR_INVALID, "world block: 0 (autotune) or a multiple of 64 in [64, 1024]");
@@ -1179,6 +1209,8 @@ int b2g_get_option(b2g_batch* b, int32_t option, int32_t* value) {
     case B2G_OPT_TUNED_WORLD_BLOCK: *value = b->world_block > 0 ? b->world_block : b->tuned_block; break;
     case B2G_OPT_LAST_SOLVER: *value = b->last_path; break;
     case B2G_OPT_PER_WORLD_ELIGIBLE: *value = b->per_world_ok ? 1 : 0; break;
+    case B2G_OPT_WORLD_LDS: *value = b->world_lds; break;
+    case B2G_OPT_TUNED_WORLD_LDS: *value = b->world_lds_now() ? 1 : 0; break;
     default: return fail(B2G_ERR_INVALID, "unknown option");
   }
   return B2G_OK;

@@ -763,14 +763,27 @@ __global__ void __launch_bounds__(kBlock) k_contact_init(Dev d) {
 __device__ __forceinline__ bool writes_body(const Dev& d, int b) { return d.btype[b] == kDynamicBody; }
 
 // b2ContactSolver::WarmStart (:260-299) of slot s
-__device__ __forceinline__ void warm_start(const Dev& d, int s) {
+// Body-state access for the constraint functions: global arrays, or a
+// world's bodies staged in LDS (bodies [base, base + n) of one world).
+struct GlobalBodies {
+  float4* p;
+  __device__ __forceinline__ float4* operator()(int i) const { return p + i; }
+};
+struct LdsBodies {
+  float4* p;
+  int base;
+  __device__ __forceinline__ float4* operator()(int i) const { return p + (i - base); }
+};
+
+template <class BV>
+__device__ __forceinline__ void warm_start_t(const Dev& d, int s, BV vel) {
   {
     const float4 v2i = d.vc[2][s];
     const int ia = as_int(v2i.x), ib = as_int(v2i.y), count = as_int(v2i.z);
     if ((d.wflag[d.bworld[ia]] & 1) == 0) return;
     const float4 n4 = d.vc[0][s], m = d.vc[1][s];
     const vec2 normal = mk(n4.x, n4.y), tangent = cross(normal, 1.0f);
-    float4 va = d.vel[ia], vb = d.vel[ib];
+    float4 va = *vel(ia), vb = *vel(ib);
     vec2 vA = mk(va.x, va.y), vB = mk(vb.x, vb.y);
     float wA = va.z, wB = vb.z;
 #pragma unroll
@@ -784,20 +797,22 @@ __device__ __forceinline__ void warm_start(const Dev& d, int s) {
         vB = vB + m.y * P;
       }
     }
-    if (writes_body(d, ia)) d.vel[ia] = make_float4(vA.x, vA.y, wA, 0.0f);
-    if (writes_body(d, ib)) d.vel[ib] = make_float4(vB.x, vB.y, wB, 0.0f);
+    if (writes_body(d, ia)) *vel(ia) = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (writes_body(d, ib)) *vel(ib) = make_float4(vB.x, vB.y, wB, 0.0f);
   }
 }
+__device__ __forceinline__ void warm_start(const Dev& d, int s) { warm_start_t(d, s, GlobalBodies{d.vel}); }
 
 // b2ContactSolver::SolveVelocityConstraints (:301-639) for slots [begin, end)
-__device__ __forceinline__ void contact_velocity(const Dev& d, int s) {
+template <class BV>
+__device__ __forceinline__ void contact_velocity_t(const Dev& d, int s, BV vel) {
   const float4 v2i = d.vc[2][s];
   const int ia = as_int(v2i.x), ib = as_int(v2i.y), count = as_int(v2i.z);
   const float4 n4 = d.vc[0][s], m = d.vc[1][s];
   const float mA = m.x, mB = m.y, iA = m.z, iB = m.w;
   const vec2 normal = mk(n4.x, n4.y);
   const float friction = n4.z, tangent_speed = n4.w;
-  float4 va = d.vel[ia], vb = d.vel[ib];
+  float4 va = *vel(ia), vb = *vel(ib);
   vec2 vA = mk(va.x, va.y), vB = mk(vb.x, vb.y);
   float wA = va.z, wB = vb.z;
   const float4 r0 = d.vc[3][s], r1 = d.vc[5][s];
@@ -887,8 +902,11 @@ __device__ __forceinline__ void contact_velocity(const Dev& d, int s) {
   }
   d.vc[4][s] = q0;
   if (count > 1) d.vc[6][s] = q1;
-  if (writes_body(d, ia)) d.vel[ia] = make_float4(vA.x, vA.y, wA, 0.0f);
-  if (writes_body(d, ib)) d.vel[ib] = make_float4(vB.x, vB.y, wB, 0.0f);
+  if (writes_body(d, ia)) *vel(ia) = make_float4(vA.x, vA.y, wA, 0.0f);
+  if (writes_body(d, ib)) *vel(ib) = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+__device__ __forceinline__ void contact_velocity(const Dev& d, int s) {
+  contact_velocity_t(d, s, GlobalBodies{d.vel});
 }
 
 // Serial tail: when a sweep's last colour kernel is launched with
@@ -991,7 +1009,8 @@ __global__ void __launch_bounds__(kBlock) k_store_and_integrate(Dev d) {
 
 // b2ContactSolver::SolvePositionConstraints (:659-787) for slots [begin, end)
 // returns the minimum separation seen (0 when the island is already solved)
-__device__ __forceinline__ float contact_position(const Dev& d, int s, int isl) {
+template <class BP>
+__device__ __forceinline__ float contact_position_t(const Dev& d, int s, int isl, BP pos) {
   if (d.solved[isl]) return 0.0f;
   const float4 v2i = d.vc[2][s];
   const int ia = as_int(v2i.x), ib = as_int(v2i.y);
@@ -1001,7 +1020,7 @@ __device__ __forceinline__ float contact_position(const Dev& d, int s, int isl) 
   const float mA = q3.x, mB = q3.y, iA = q3.z, iB = q3.w;
   const float radA = q4.x, radB = q4.y;
   const int mtype = as_int(q4.z), count = as_int(q4.w);
-  const float4 pa = d.pos[ia], pb = d.pos[ib];
+  const float4 pa = *pos(ia), pb = *pos(ib);
   vec2 cA = mk(pa.x, pa.y), cB = mk(pb.x, pb.y);
   float aA = pa.z, aB = pb.z;
   float min_sep = 0.0f;
@@ -1049,9 +1068,12 @@ __device__ __forceinline__ float contact_position(const Dev& d, int s, int isl) 
       aB += iB * cross(rB, P);
     }
   }
-  if (writes_body(d, ia)) d.pos[ia] = make_float4(cA.x, cA.y, aA, 0.0f);
-  if (writes_body(d, ib)) d.pos[ib] = make_float4(cB.x, cB.y, aB, 0.0f);
+  if (writes_body(d, ia)) *pos(ia) = make_float4(cA.x, cA.y, aA, 0.0f);
+  if (writes_body(d, ib)) *pos(ib) = make_float4(cB.x, cB.y, aB, 0.0f);
   return min_sep;
+}
+__device__ __forceinline__ float contact_position(const Dev& d, int s, int isl) {
+  return contact_position_t(d, s, isl, GlobalBodies{d.pos});
 }
 
 __device__ __forceinline__ void contact_position_serial(const Dev& d, int begin, int end, int it) {
@@ -1387,9 +1409,14 @@ __global__ void __launch_bounds__(kBlock) k_world_colour_sort(Dev d, int rounds_
 
 // The solve of world worder[blockIdx.x]: warm start, velocity iterations,
 // store impulses + integrate positions, position iterations with the
-// per-island early exit, write-back. Worlds without joints only.
+// per-island early exit, write-back. Worlds without joints only. With kLds
+// the world's body velocities (then positions) live in LDS during the sweeps
+// (dynamic shared memory of 16 bytes per body of the largest world); the
+// arithmetic is the same either way.
+template <bool kLds>
 __global__ void __launch_bounds__(1024) k_solve_worlds(Dev d, int vi, int pi) {
   __shared__ int off[kColourSlots + 1];
+  extern __shared__ float4 s_body[];
   const int w = d.worder[blockIdx.x];
   if (!world_active(d, w)) return;  // whole block
   for (int c = threadIdx.x; c <= kColourSlots; c += blockDim.x) off[c] = d.wcol[w * (kColourSlots + 1) + c];
@@ -1411,14 +1438,33 @@ __global__ void __launch_bounds__(1024) k_solve_worlds(Dev d, int vi, int pi) {
       __syncthreads();
     }
   };
+  auto stage_in = [&](const float4* src) {
+    for (int b = wb.x + tid; b < wb.y; b += nt) s_body[b - wb.x] = src[b];
+    __syncthreads();
+  };
+  auto stage_out = [&](float4* dst) {
+    for (int b = wb.x + tid; b < wb.y; b += nt) dst[b] = s_body[b - wb.x];
+    __syncthreads();
+  };
   if (t1 > t0) {
-    sweep([&](int s) { warm_start(d, s); });
-    for (int it = 0; it < vi; ++it) sweep([&](int s) { contact_velocity(d, s); });
+    if (kLds) {
+      stage_in(d.vel);
+      const LdsBodies vel{s_body, wb.x};
+      sweep([&](int s) { warm_start_t(d, s, vel); });
+      for (int it = 0; it < vi; ++it) sweep([&](int s) { contact_velocity_t(d, s, vel); });
+      stage_out(d.vel);
+    } else {
+      sweep([&](int s) { warm_start(d, s); });
+      for (int it = 0; it < vi; ++it) sweep([&](int s) { contact_velocity(d, s); });
+    }
   }
   for (int s = t0 + tid; s < t1; s += nt) store_impulses(d, s);
   for (int b = wb.x + tid; b < wb.y; b += nt) integrate_position(d, b);
   __syncthreads();
   if (t1 > t0) {
+    if (kLds) stage_in(d.pos);
+    const LdsBodies lpos{s_body, wb.x};
+    const GlobalBodies gpos{d.pos};
     for (int it = 0; it < pi; ++it) {
       for (int c = 0; c < kColourSlots; ++c) {
         const int lo = off[c], hi = off[c + 1];
@@ -1427,14 +1473,16 @@ __global__ void __launch_bounds__(1024) k_solve_worlds(Dev d, int vi, int pi) {
           if (tid == 0)
             for (int s = lo; s < hi; ++s) {
               const int isl = d.cisland[s];
-              atomicMin(&d.minsep[(it & 1) * d.nb + isl], float_to_ordered(contact_position(d, s, isl)));
+              const float m = kLds ? contact_position_t(d, s, isl, lpos) : contact_position_t(d, s, isl, gpos);
+              atomicMin(&d.minsep[(it & 1) * d.nb + isl], float_to_ordered(m));
             }
         } else {
           for (int base = lo; base < hi; base += nt) {  // every lane of a wave takes part in the min
             const int s = base + tid;
             const bool valid = s < hi;
             const int isl = valid ? d.cisland[s] : 0;
-            const float m = valid ? contact_position(d, s, isl) : 0.0f;
+            float m = 0.0f;
+            if (valid) m = kLds ? contact_position_t(d, s, isl, lpos) : contact_position_t(d, s, isl, gpos);
             wave_atomic_min_keyed(d.minsep + (it & 1) * d.nb, isl, valid, m);
           }
         }
@@ -1452,6 +1500,7 @@ __global__ void __launch_bounds__(1024) k_solve_worlds(Dev d, int vi, int pi) {
       }
       __syncthreads();
     }
+    if (kLds) stage_out(d.pos);
   }
   // write back: SynchronizeTransform and ClearForces
   for (int b = wb.x + tid; b < wb.y; b += nt) {

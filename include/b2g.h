@@ -259,8 +259,10 @@ int b2g_time_kernel(b2g_batch* batch, int32_t kernel);
  * B2G_SOLVER_COLOURS (one launch per colour over all worlds) or
  * B2G_SOLVER_WORLDS (one workgroup solves a whole world, barriers between
  * colours; needs that layout). Both give the same results.
- * B2G_OPT_WORLD_BLOCK: threads per world workgroup, 0 = autotune (the first
- * ten per-world steps time 64..1024 threads and keep the fastest).
+ * B2G_OPT_WORLD_BLOCK: threads per world workgroup, 0 = autotune: the first
+ * per-world steps time 64..1024 threads, each with the world's bodies in
+ * global memory and staged in LDS (when the largest world fits 63 KB), one
+ * step per candidate, and keep the fastest.
  * Read-only: B2G_OPT_TUNED_WORLD_BLOCK, B2G_OPT_LAST_SOLVER (1 colours,
  * 2 worlds), B2G_OPT_PER_WORLD_ELIGIBLE. No reference counterpart. */
 enum { B2G_SOLVER_AUTO = 0, B2G_SOLVER_COLOURS = 1, B2G_SOLVER_WORLDS = 2 };
@@ -269,7 +271,9 @@ enum {
   B2G_OPT_WORLD_BLOCK = 2,
   B2G_OPT_TUNED_WORLD_BLOCK = 3,
   B2G_OPT_LAST_SOLVER = 4,
-  B2G_OPT_PER_WORLD_ELIGIBLE = 5
+  B2G_OPT_PER_WORLD_ELIGIBLE = 5,
+  B2G_OPT_WORLD_LDS = 6,        /* -1 autotune (default), 0 off, 1 stage each world's bodies in LDS */
+  B2G_OPT_TUNED_WORLD_LDS = 7   /* read-only: whether the per-world solver stages bodies in LDS */
 };
 int b2g_set_option(b2g_batch* batch, int32_t option, int32_t value);
 int b2g_get_option(b2g_batch* batch, int32_t option, int32_t* value);

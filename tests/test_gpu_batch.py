@@ -126,15 +126,17 @@ def test_gpu_per_world_solver_matches_colour_sweeps_bitexact(engine):
     ref.advance(30)
     assert ref.get_option(engine.OPT_LAST_SOLVER) == 1
     want = _state(ref.bodies())
-    for block in (64, 256, 1024):
+    for block, lds in ((64, 0), (256, 0), (1024, 0), (128, 1), (512, 1)):
         b = engine.Batch.from_scenes(scenes)
         assert b.get_option(engine.OPT_PER_WORLD_ELIGIBLE) == 1
         b.set_option(engine.OPT_SOLVER, engine.SOLVER_WORLDS)
         b.set_option(engine.OPT_WORLD_BLOCK, block)
+        b.set_option(engine.OPT_WORLD_LDS, lds)
         b.advance(30)
         assert b.get_option(engine.OPT_LAST_SOLVER) == 2
+        assert b.get_option(engine.OPT_TUNED_WORLD_LDS) == lds
         same = (_state(b.bodies()) == want).all(axis=1)
-        print(f"block {block}: {same.sum()} of {len(same)} bodies bit-identical", flush=True)
+        print(f"block {block} lds {lds}: {same.sum()} of {len(same)} bodies bit-identical", flush=True)
         assert same.all()
 
 
