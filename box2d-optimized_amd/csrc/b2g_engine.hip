@@ -48,6 +48,9 @@ int dalloc(T** p, size_t n) {
 }
 
 constexpr int kColourRoundsPerStep = 4;
+// claim rounds of a captured step: a round with nothing pending returns at
+// once, so spare rounds cost only their launch
+constexpr int kGraphColourRounds = 8;
 
 inline int grid_for(long long n, int max_blocks = 4096) {
   long long g = (n + kBlock - 1) / kBlock;
@@ -745,7 +748,7 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
 
 // the whole step as one launch sequence with no host reads
 int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
-  enqueue_collide_and_colour(K, kColourRoundsPerStep);
+  enqueue_collide_and_colour(K, kGraphColourRounds);
   hipLaunchKernelGGL(k_colour_leftover, dim3(grid_for(std::max(est_cons, 256) * 5 / 4)), dim3(kBlock), 0, stream,
                      d);
   if (use_per_world()) {
@@ -817,7 +820,7 @@ int b2g_batch::graph_step(int vi, int pi, unsigned flags) {
   HIP_TRY(hipGraphLaunch(ge->exec, stream));
   // mirror what the replay does to the host view
   if (!(flags & B2G_STEP_SKIP_BROADPHASE)) swap_contact_buffers();
-  round_host += kColourRoundsPerStep;
+  round_host += kGraphColourRounds;
   ++graph_steps;
   if (!(flags & B2G_STEP_ASYNC)) {
     HIP_TRY(hipStreamSynchronize(stream));

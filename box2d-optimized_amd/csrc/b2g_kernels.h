@@ -588,9 +588,21 @@ __global__ void __launch_bounds__(kBlock) k_island_link(Dev d) {
   }
 }
 
+// read-only walk to the root (no halving): while the labels below are being
+// written, a halving store could replace a finished label with a non-root
+// ancestor
+__device__ __forceinline__ int uf_root(const int* parent, int x) {
+  for (;;) {
+    const int p = uf_load(parent + x);
+    if (p == x) return x;
+    x = p;
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_island_label(Dev d) {
   B2G_FOR(b, 0, d.nb) {
-    if (d.btype[b] != kStaticBody) d.parent[b] = uf_find(d.parent, b);
+    if (d.btype[b] != kStaticBody)
+      __hip_atomic_store(d.parent + b, uf_root(d.parent, b), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 

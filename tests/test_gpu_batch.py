@@ -151,9 +151,10 @@ def test_gpu_per_world_autotune_and_graph(engine):
     print("autotuned world block:", tuned, flush=True)
     assert tuned in (64, 128, 256, 512, 1024)
     b = engine.Batch.from_scenes(scenes).advance(12)
-    for _ in range(20):
+    for k in range(20):
         a.step(8, 3, 0)
         b.step(8, 3, engine.STEP_GRAPH)
+        assert b.counts().serial == 0, f"step {k}: constraints left to the serial group"
     assert b.counts().graph_steps == 20
     assert np.array_equal(_state(a.bodies()), _state(b.bodies()))
 
@@ -224,3 +225,17 @@ def test_gpu_contact_events_match_touching_transitions(engine):
         destroyed += int((ev["kind"] == engine.EVENT_DESTROYED).sum())
     print(f"events over 60 steps: {begins} begin, {ends} end, {destroyed} destroyed", flush=True)
     assert begins > 0
+
+
+def test_gpu_steps_are_deterministic(engine):
+    """The same world stepped twice gives the same bits at every step. Island
+    labels come from a read-only root walk after the lock-free union: a
+    labelling pass that also shortened paths could overwrite a finished label
+    with a non-root ancestor and split an island's early exit."""
+    pile = S.random_pile(400, seed=12, half_w=12.0, height=30.0)
+    a = engine.Batch.from_scenes([pile])
+    b = engine.Batch.from_scenes([pile])
+    for k in range(40):
+        a.advance(1)
+        b.advance(1)
+        assert np.array_equal(_state(a.bodies()), _state(b.bodies())), f"step {k + 1}"
