@@ -58,6 +58,19 @@ def item_bytes(kernel: str, geo_bytes: float) -> float:
     }[kernel]
 
 
+def measured_traffic_ratio(kernel: str, workload: str):
+    """HBM bytes / algorithmic bytes per launch of `kernel`, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE runs scaled by a plain
+    copy kernel), or None when that kernel/workload was not measured."""
+    path = os.path.join(REPO, "profiles", f"pmc_traffic_{workload}.md")
+    if kernel != "contact_velocity" or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_ratio:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def workloads(pkg):
     S = pkg.scenes
     return {
@@ -360,6 +373,7 @@ def main():
         serving = serving_section(pkg, args.serve_worlds, args.serve_rate, args.serve_seconds,
                                   args.serve_budget_gb, not args.eager)
 
+    ratio = measured_traffic_ratio(args.kernel, args.workload)
     value = ndyn * args.steps / elapsed
     line = {
         "metric": "body-steps/s on 1 GPU vs host CPU (20k pile)",
@@ -382,7 +396,11 @@ def main():
                    "launch": "eager" if args.eager else "hipGraph replay of the whole step",
                    "graph_steps": counts.graph_steps, "serial_constraints": counts.serial},
         "roofline": {"bound": "hbm", "kernel": args.kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (ratio * bytes_per_launch) if ratio else None,
+                     "traffic_source": (f"profiles/pmc_traffic_{args.workload}.md: measured HBM/algorithmic "
+                                        f"ratio {ratio:.2f} x this run's algorithmic bytes per launch")
+                     if ratio else None,
                      "avg_launch_ms": avg_ms, "launches": k_launches,
                      "bytes_per_launch": bytes_per_launch, "bytes_per_item": per_item},
         "cpu_baseline": cpu,
