@@ -291,7 +291,10 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=-1, help="-1: the GPU's timed steps (bounded)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--eager", action="store_true", help="launch every kernel from the host (no step graph)")
+    ap.add_argument("--eager", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--launch", choices=["eager", "graph"], default="eager",
+                    help="timed steps: kernels launched from the host with one colour read-back per step "
+                         "(eager, default) or a replayed hipGraph of the whole step (graph)")
     ap.add_argument("--batch-worlds", type=int, default=4096, help="0 skips the batched section")
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--batch-sample", type=int, default=64, help="per-world loop / CPU sample stride")
@@ -303,6 +306,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     args = ap.parse_args()
+    if args.eager:
+        args.launch = "eager"
+    args.eager = args.launch == "eager"
     if args.copy_probe:
         copy_probe(args.copy_probe, 10)
         return

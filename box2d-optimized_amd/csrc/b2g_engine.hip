@@ -50,7 +50,7 @@ int dalloc(T** p, size_t n) {
 constexpr int kColourRoundsPerStep = 4;
 // claim rounds of a captured step: a round with nothing pending returns at
 // once, so spare rounds cost only their launch
-constexpr int kGraphColourRounds = 8;
+constexpr int kGraphColourRounds = 16;  // enough for a world colouring from scratch
 
 inline int grid_for(long long n, int max_blocks = 4096) {
   long long g = (n + kBlock - 1) / kBlock;
@@ -153,6 +153,7 @@ struct b2g_batch {
   void swap_contact_buffers();
   void graphs_clear();
   void enqueue_collide_and_colour(int K, int rounds);
+
   void enqueue_colour_round(int K);
   void enqueue_colour_groups();
   void enqueue_islands_and_init();
@@ -188,7 +189,8 @@ int b2g_batch::alloc() {
   A(d.xf, NB); A(d.pos, NB); A(d.vel, NB); A(d.prop, NB); A(d.prop2, NB); A(d.force, NB);
   A(d.btype, NB); A(d.bworld, NB);
   A(d.wpar, NW); A(d.wflag, NW); A(d.wshape0, NW); A(d.wshape1, NW); A(d.wbody, NW); A(d.worder, NW);
-  A(d.wcol, (size_t)NW * (kColourSlots + 1)); A(d.wbounds, 4 * NW); A(d.wleaf, 2 * NW);
+  A(d.wcol, (size_t)NW * (kColourSlots + 1));
+  A(d.wcolmax, NW); A(d.wfresh, NW); A(d.wrecolour, NW); A(d.wbounds, 4 * NW); A(d.wleaf, 2 * NW);
   A(d.shdr, NS); A(d.geo, 9 * (size_t)NS); A(d.smat, NS); A(d.sfilt, NS); A(d.aabb, NS);
   A(d.joints, NJ); A(d.jsol, NJ); A(d.jorder, NJ); A(d.jisland, NJ);
   A(d.bj_off, NB + 1); A(d.bj_other, 2 * NJ);
@@ -212,7 +214,7 @@ int b2g_batch::alloc() {
     d.sparse_levels = levels;
     A(d.sparse, (size_t)levels * NS);
   }
-  A(d.pslots, (size_t)kPairSlots * NS);
+  A(d.pslots, (size_t)kPairSlots * NS); A(d.heavy, NS);
   A(d.pcount, NS); A(d.poff, NS);
   const size_t scan_n = std::max<size_t>(std::max<size_t>(NC, NS), kRadix * tiles);
   A(d.scan_tiles, scan_n / kScanTile + 2);
@@ -228,6 +230,11 @@ int b2g_batch::alloc() {
   h_shape_body.assign(NS, 0);
   h_wshape0.assign(NW, 0);
   HIP_TRY(hipMemcpy(d.wshape0, h_wshape0.data(), NW * sizeof(int), hipMemcpyHostToDevice));
+  {  // colouring upkeep: nothing used yet, every world colours from scratch first
+    std::vector<int> minus1(NW, -1), one(NW, 1);
+    HIP_TRY(hipMemcpy(d.wcolmax, minus1.data(), NW * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.wrecolour, one.data(), NW * sizeof(int), hipMemcpyHostToDevice));
+  }
   h_params.assign(NW, b2g_world_params{{0.0f, -10.0f}, 1.0f / 60.0f, 0.0f, 1, 1});
   std::vector<int> zero_off(NB + 1, 0);
   HIP_TRY(hipMemcpy(d.bj_off, zero_off.data(), (NB + 1) * sizeof(int), hipMemcpyHostToDevice));
@@ -238,14 +245,15 @@ void b2g_batch::release() {
   graphs_clear();
   if (qbuf) (void)hipFree(qbuf);
   qbuf = nullptr;
-  void* ptrs[] = {d.xf, d.pos, d.vel, d.prop, d.prop2, d.force, d.btype, d.bworld, d.wpar, d.wflag, d.wshape0, d.wshape1, d.wbody, d.worder, d.wcol,
+  void* ptrs[] = {d.xf, d.pos, d.vel, d.prop, d.prop2, d.force, d.btype, d.bworld, d.wpar, d.wflag, d.wshape0, d.wshape1, d.wbody, d.worder, d.wcol, d.wcolmax, d.wfresh,
+                  d.wrecolour,
                   d.wbounds, d.wleaf, d.shdr, d.geo, d.smat, d.sfilt, d.aabb, d.joints, d.jsol,
                   d.jorder, d.jisland, d.bj_off, d.bj_other, d.ckey, d.cpair, d.cman, d.cmat, d.ccol,
                   d.nkey, d.npair, d.nman, d.nmat, d.ncol, d.touch, d.tpos, d.clist, d.ccolour,
                   d.csorted, d.cisland, d.cpend, d.events, d.col_count, d.col_off, d.col_cursor, d.bmask, d.bslot,
                   d.parent, d.minsep, d.jok, d.solved, d.mkey[0], d.mkey[1], d.mval[0], d.mval[1],
                   d.sort_hist, d.node_child, d.node_parent, d.leaf_parent, d.node_range, d.node_rec,
-                  d.sfast, d.sparse, d.pslots, d.pcount, d.poff, d.scan_tiles, d.cnt};
+                  d.sfast, d.sparse, d.pslots, d.heavy, d.pcount, d.poff, d.scan_tiles, d.cnt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int k = 0; k < 10; ++k)
@@ -259,6 +267,10 @@ void b2g_batch::release() {
 }
 
 int b2g_batch::scan(const int* in, const int* n_dev, int n_host, int* out, int* total, long long n_bound) {
+  if (n_bound <= 8192) {  // one launch; still correct if the device count turns out larger
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScan1Threads), 0, stream, in, n_dev, n_host, out, total);
+    return B2G_OK;
+  }
   const int tiles = (int)std::max<long long>(1, (n_bound + kScanTile - 1) / kScanTile);
   const int g = std::min(tiles, 4096);
   hipLaunchKernelGGL(k_scan_tiles, dim3(g), dim3(kBlock), 0, stream, in, n_dev, n_host, d.scan_tiles);
@@ -309,7 +321,7 @@ int b2g_batch::build_tree(bool use_given_aabbs) {
   if (ns >= 2) {
     int top = 0;
     while ((2 << top) <= ns) ++top;  // floor(log2(ns))
-    for (int j = 1; j <= top; j += 2) {
+    for (int j = 1; j <= top; j += 3) {
       Dev dj = d;
       dj.sparse_levels = top + 1;
       hipLaunchKernelGGL(k_sparse_levels, dim3(gs), dim3(kBlock), 0, stream, dj, j);
@@ -340,6 +352,7 @@ int b2g_batch::broadphase(bool use_given_aabbs) {
   tend(B2G_KERNEL_PAIRS_WRITE, ns);
   scan(d.pcount, nullptr, ns, d.poff, d.cnt + C_PAIRS, ns);
   hipLaunchKernelGGL(k_pairs_compact, dim3(gq), dim3(kQueryBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_sort_pair_segments, dim3(64), dim3(kBlock), 0, stream, d);
   hipLaunchKernelGGL(k_merge_contacts, dim3(grid_for(std::max(est_contacts, ns) * 5 / 4)), dim3(kBlock), 0, stream,
                      d);
   if (d.events_on)
@@ -436,7 +449,8 @@ void b2g_batch::enqueue_collide_and_colour(int K, int rounds) {
   tbegin(B2G_KERNEL_NARROWPHASE);
   hipLaunchKernelGGL(k_narrowphase, dim3(gc), dim3(kBlock), 0, stream, d);
   tend(B2G_KERNEL_NARROWPHASE, est_contacts);
-  scan(d.touch, d.cnt + C_CONTACTS, 0, d.tpos, d.cnt + C_CONS, d.ccap);
+  // the bound only picks the scan's launch shape; any device count is handled
+  scan(d.touch, d.cnt + C_CONTACTS, 0, d.tpos, d.cnt + C_CONS, std::max(est_contacts, 1) * 5LL / 4);
   hipLaunchKernelGGL(k_compact_touching, dim3(gc), dim3(kBlock), 0, stream, d);
   hipLaunchKernelGGL(k_integrate_velocities, dim3(gb), dim3(kBlock), 0, stream, d);
   hipLaunchKernelGGL(k_colour_init, dim3(gk), dim3(kBlock), 0, stream, d, K);
@@ -444,7 +458,9 @@ void b2g_batch::enqueue_collide_and_colour(int K, int rounds) {
 }
 
 void b2g_batch::enqueue_colour_round(int K) {
-  const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
+  // the rounds walk the pending list: after the first steps it holds only the
+  // newly touching constraints, so a modest grid covers it
+  const int gk = std::min(grid_for(std::max(est_cons, 256) * 5 / 4), 512);
   hipLaunchKernelGGL(k_colour_claim, dim3(gk), dim3(kBlock), 0, stream, d, rounds_pending);
   hipLaunchKernelGGL(k_colour_resolve, dim3(gk), dim3(kBlock), 0, stream, d, rounds_pending, K);
   ++rounds_pending;
@@ -453,7 +469,7 @@ void b2g_batch::enqueue_colour_round(int K) {
 void b2g_batch::enqueue_colour_groups() {
   const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
   hipLaunchKernelGGL(k_colour_count, dim3(gk), dim3(kBlock), 0, stream, d);
-  hipLaunchKernelGGL(k_colour_offsets, dim3(1), dim3(1), 0, stream, d, rounds_pending);
+  hipLaunchKernelGGL(k_colour_offsets, dim3(1), dim3(kWave), 0, stream, d, rounds_pending);
   round_host += (unsigned)rounds_pending;
   rounds_pending = 0;
   hipLaunchKernelGGL(k_colour_scatter, dim3(gk), dim3(kBlock), 0, stream, d);
@@ -633,6 +649,7 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
   if (use_per_world()) {
     last_path = 2;
     if ((rc = read_counters())) return rc;
+    const int pw_colours = pinned[C_NCOLOURS];  // last step's (normal colours are what it reports)
     for (int extra = 0; pinned[C_REMAIN] > 0; extra += 2) {  // colour everything first
       if (extra > 4096) return fail(B2G_ERR_CAPACITY, "colouring did not converge");
       enqueue_colour_round(kMaxColours);
@@ -641,6 +658,7 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
     }
     est_cons = pinned[C_CONS];
     hipLaunchKernelGGL(k_world_colour_sort, dim3(d.nw), dim3(kBlock), 0, stream, d, rounds_pending);
+    hipLaunchKernelGGL(k_world_colour_stats, dim3(grid_for(d.nw)), dim3(kBlock), 0, stream, d);
     round_host += (unsigned)rounds_pending;
     rounds_pending = 0;
     enqueue_islands_and_init();
@@ -666,6 +684,7 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
       }
     }
     graph_K = kMaxColours;
+    (void)pw_colours;
     if (prof_on) HIP_TRY(hipEventRecord(ev[2], stream));
     if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
       if ((rc = broadphase(false))) return rc;
@@ -719,6 +738,7 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
     if (c < kMaxColours) normal_colours = c + 1;
   }
   last_ncolours = pinned[C_NCOLOURS];
+  hipLaunchKernelGGL(k_world_colour_stats, dim3(grid_for(d.nw)), dim3(kBlock), 0, stream, d);
   // colour limit for captured steps: what this step needed plus one
   graph_K = std::min(kMaxColours, std::max(normal_colours, 1) + 1);
   enqueue_solve(vi, pi, colours, sizes, 0);
@@ -753,6 +773,7 @@ int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
                      d);
   if (use_per_world()) {
     hipLaunchKernelGGL(k_world_colour_sort, dim3(d.nw), dim3(kBlock), 0, stream, d, rounds_pending);
+    hipLaunchKernelGGL(k_world_colour_stats, dim3(grid_for(d.nw)), dim3(kBlock), 0, stream, d);
     round_host += (unsigned)rounds_pending;
     rounds_pending = 0;
     enqueue_islands_and_init();
@@ -765,6 +786,7 @@ int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
     return B2G_OK;
   }
   enqueue_colour_groups();
+  hipLaunchKernelGGL(k_world_colour_stats, dim3(grid_for(d.nw)), dim3(kBlock), 0, stream, d);
   enqueue_islands_and_init();
   std::vector<int> colours, sizes;
   for (int c = 0; c < K; ++c) {
@@ -1442,10 +1464,11 @@ int b2g_upload_contact_colours(b2g_batch* b, int32_t count, const int32_t* colou
 }
 
 // ---- snapshots ---------------------------------------------------------------
-// File: "B2GSNAP1", then int32 version, capacity (5), world count, body,
-// shape, joint and contact counts, then the world params, body, shape, joint
-// and contact records and the contact colours, all little-endian as laid out
-// in include/b2g.h.
+// File: "B2GSNAP1", then int32 version (2), capacity (5), world count, body,
+// shape, joint and contact counts, then the world params, the per-world
+// colouring state (fresh colour counts, then recolour flags), the body, shape,
+// joint and contact records and the contact colours, all little-endian as
+// laid out in include/b2g.h.
 static const char kSnapMagic[8] = {'B', '2', 'G', 'S', 'N', 'A', 'P', '1'};
 
 int b2g_snapshot_save(b2g_batch* b, const char* path) {
@@ -1462,12 +1485,16 @@ int b2g_snapshot_save(b2g_batch* b, const char* path) {
   if (nj && (rc = b2g_download_joints(b, 0, nj, joints.data()))) return rc;
   if (nc && (rc = b2g_download_contacts(b, 0, nc, contacts.data()))) return rc;
   if (nc && (rc = b2g_download_contact_colours(b, 0, nc, colours.data()))) return rc;
+  std::vector<int32_t> wstate(2 * (size_t)nw);  // per world: fresh colour count, recolour flag
+  HIP_TRY(hipMemcpy(wstate.data(), b->d.wfresh, nw * sizeof(int), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(wstate.data() + nw, b->d.wrecolour, nw * sizeof(int), hipMemcpyDeviceToHost));
   FILE* f = std::fopen(path, "wb");
   if (!f) return fail(B2G_ERR_INVALID, std::string("cannot write ") + path);
-  const int32_t head[11] = {1, b->cap.worlds, b->cap.bodies, b->cap.shapes, b->cap.joints, b->cap.contacts,
+  const int32_t head[11] = {2, b->cap.worlds, b->cap.bodies, b->cap.shapes, b->cap.joints, b->cap.contacts,
                             nw, nb, ns, nj, nc};
   bool ok = std::fwrite(kSnapMagic, 1, 8, f) == 8 && std::fwrite(head, sizeof(head), 1, f) == 1;
   ok = ok && (nw == 0 || std::fwrite(b->h_params.data(), sizeof(b2g_world_params), nw, f) == (size_t)nw);
+  ok = ok && (nw == 0 || std::fwrite(wstate.data(), sizeof(int32_t), 2 * (size_t)nw, f) == 2 * (size_t)nw);
   ok = ok && (nb == 0 || std::fwrite(bodies.data(), sizeof(b2g_body), nb, f) == (size_t)nb);
   ok = ok && (ns == 0 || std::fwrite(b->h_shape_rec.data(), sizeof(b2g_shape), ns, f) == (size_t)ns);
   ok = ok && (nj == 0 || std::fwrite(joints.data(), sizeof(b2g_revolute), nj, f) == (size_t)nj);
@@ -1485,19 +1512,21 @@ int b2g_snapshot_load(b2g_batch** out, const char* path) {
   char magic[8];
   int32_t head[11];
   bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kSnapMagic, 8) == 0 &&
-            std::fread(head, sizeof(head), 1, f) == 1 && head[0] == 1;
+            std::fread(head, sizeof(head), 1, f) == 1 && head[0] == 2;
   const int nw = ok ? head[6] : 0, nb = ok ? head[7] : 0, ns = ok ? head[8] : 0, nj = ok ? head[9] : 0,
             nc = ok ? head[10] : 0;
   b2g_capacity cap{head[1], head[2], head[3], head[4], head[5]};
   ok = ok && nw >= 1 && nb >= 0 && ns >= 0 && nj >= 0 && nc >= 0 && nw <= cap.worlds && nb <= cap.bodies &&
        ns <= cap.shapes && nj <= cap.joints && nc <= cap.contacts;
   std::vector<b2g_world_params> params(ok ? nw : 0);
+  std::vector<int32_t> wstate(ok ? 2 * (size_t)nw : 0);
   std::vector<b2g_body> bodies(ok ? nb : 0);
   std::vector<b2g_shape> shapes(ok ? ns : 0);
   std::vector<b2g_revolute> joints(ok ? nj : 0);
   std::vector<b2g_contact> contacts(ok ? nc : 0);
   std::vector<int32_t> colours(ok ? nc : 0);
   ok = ok && std::fread(params.data(), sizeof(b2g_world_params), nw, f) == (size_t)nw;
+  ok = ok && std::fread(wstate.data(), sizeof(int32_t), 2 * (size_t)nw, f) == 2 * (size_t)nw;
   ok = ok && (nb == 0 || std::fread(bodies.data(), sizeof(b2g_body), nb, f) == (size_t)nb);
   ok = ok && (ns == 0 || std::fread(shapes.data(), sizeof(b2g_shape), ns, f) == (size_t)ns);
   ok = ok && (nj == 0 || std::fread(joints.data(), sizeof(b2g_revolute), nj, f) == (size_t)nj);
@@ -1514,6 +1543,10 @@ int b2g_snapshot_load(b2g_batch** out, const char* path) {
   if (!rc && nj) rc = b2g_upload_joints(b, 0, nj, joints.data());
   if (!rc) rc = b2g_upload_contacts(b, nc, contacts.data());
   if (!rc) rc = b2g_upload_contact_colours(b, nc, colours.data());
+  if (!rc && hipMemcpy(b->d.wfresh, wstate.data(), nw * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(B2G_ERR_HIP, "restoring the colouring state");
+  if (!rc && hipMemcpy(b->d.wrecolour, wstate.data() + nw, nw * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(B2G_ERR_HIP, "restoring the colouring state");
   if (rc) {
     b2g_batch_destroy(b);
     return rc;

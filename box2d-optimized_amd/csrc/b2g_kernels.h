@@ -15,6 +15,7 @@
 
 #include "b2g_collide.h"
 #include "b2g_prims.h"
+#include <climits>
 
 namespace b2g {
 
@@ -47,6 +48,7 @@ enum Counter : int {
   C_PENDING = 9,    // entries of the pending-colour list of this step
   C_TICKET = 10,    // blocks finished in a sweep's last colour kernel (serial tail)
   C_EVENTS = 11,    // contact events recorded this step
+  C_HEAVY = 12,     // over-full shapes whose pair segments need sorting (this broadphase)
   C_COUNT = 16
 };
 enum Status : int { S_CONTACT_OVERFLOW = 1, S_STACK_OVERFLOW = 2, S_TREE_CORRUPT = 4, S_EVENT_OVERFLOW = 8 };
@@ -79,6 +81,9 @@ struct Dev {
   int2* wbody;    // body range of each world (worlds laid out world after world)
   int* worder;    // worlds by decreasing size: the per-world solver's block order
   int* wcol;      // per world: kColourSlots + 1 slot offsets of its colour groups
+  int* wcolmax;   // per world: highest normal colour used this step (-1: none)
+  int* wfresh;    // per world: colours of its last colouring from scratch (0: none yet)
+  int* wrecolour; // per world: 1 = colour every constraint afresh this step
   // shapes
   int4* shdr;     // type, body, count, world
   float4* geo;    // 9 per shape
@@ -129,6 +134,7 @@ struct Dev {
   float4* sparse;    // (levels) x ns AABB table over sorted leaves; level 0 = leaf AABBs
   int sparse_levels;
   uint64_t* pslots;  // kPairSlots per shape: pairs found by the traversal, sorted
+  int2* heavy;       // (first, count) of the pair segments of over-full shapes, to be sorted
   int* wleaf;        // 2 per world: first leaf, last leaf + 1
   unsigned* wbounds; // 4 per world (ordered floats): min x, min y, max x, max y
   int* pcount;       // per shape pair count
@@ -174,6 +180,71 @@ __device__ __forceinline__ int wave_append(int* counter) {
   if (lane_id() == leader) base = atomicAdd(counter, __popcll(act));
   base = __shfl(base, leader, kWave);
   return base + __popcll(act & lanemask_lt());
+}
+
+// atomicMax of v into base[key] for the valid lanes, one atomic per wave when
+// all valid lanes share the key. All 64 lanes must call this.
+__device__ __forceinline__ void wave_atomic_max_int_keyed(int* base, int key, bool valid, int v) {
+  const uint64_t act = __ballot(valid);
+  if (act == 0) return;
+  const int first = __ffsll((long long)act) - 1;
+  const int key0 = __shfl(key, first, kWave);
+  if (__ballot(valid && key != key0) == 0) {
+    int m = valid ? v : INT_MIN;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) m = max(m, __shfl_xor(m, off, kWave));
+    if (lane_id() == first) atomicMax(base + key0, m);
+  } else if (valid) {
+    atomicMax(base + key, v);
+  }
+}
+
+__device__ __forceinline__ float wave_min_all(float v);
+
+// atomicMin of v into base[key] from a whole block: waves reduce first; when
+// every wave of the block has one key (one island, the common case in large
+// islands) the block issues a single atomic, and only when its minimum is
+// below the slot's current value (the slot only decreases, so a stale read
+// can cost an extra atomic but never skip a needed one). All threads of the
+// block must call this.
+__device__ __forceinline__ void block_atomic_min_keyed(unsigned* base, int key, bool valid, float v) {
+  __shared__ int s_key[kBlock / kWave];
+  __shared__ float s_val[kBlock / kWave];
+  __shared__ int s_mixed;
+  const uint64_t act = __ballot(valid);
+  const int first = act ? __ffsll((long long)act) - 1 : 0;
+  const int key0 = __shfl(key, first, kWave);
+  const bool wave_uniform = __ballot(valid && key != key0) == 0;
+  const float m = wave_min_all(valid ? v : FLT_MAX);
+  if (threadIdx.x == 0) s_mixed = 0;
+  __syncthreads();
+  if (lane_id() == 0) {
+    s_key[wave_id()] = act ? key0 : -1;
+    s_val[wave_id()] = m;
+    if (act && !wave_uniform) s_mixed = 1;
+  }
+  __syncthreads();
+  if (s_mixed) {  // several islands in one wave: per-lane atomics
+    if (valid && v < ordered_to_float(base[key])) atomicMin(base + key, float_to_ordered(v));
+  } else if (threadIdx.x == 0) {
+    int k0 = -1;
+    bool one = true;
+    float bm = FLT_MAX;
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) {
+      if (s_key[w] < 0) continue;
+      if (k0 < 0) k0 = s_key[w];
+      else if (s_key[w] != k0) one = false;
+      bm = fminf(bm, s_val[w]);
+    }
+    if (k0 >= 0 && one) {
+      if (bm < ordered_to_float(base[k0])) atomicMin(base + k0, float_to_ordered(bm));
+    } else if (k0 >= 0) {
+      for (int w = 0; w < (int)(blockDim.x / kWave); ++w)
+        if (s_key[w] >= 0 && s_val[w] < ordered_to_float(base[s_key[w]]))
+          atomicMin(base + s_key[w], float_to_ordered(s_val[w]));
+    }
+  }
+  __syncthreads();
 }
 
 __device__ __forceinline__ float wave_min_all(float v) {
@@ -357,7 +428,7 @@ __global__ void __launch_bounds__(kBlock) k_colour_init(Dev d, int K) {
   B2G_FOR(k, 0, n) {
     const int c = d.clist[k];
     const int col = d.ccol[c];
-    if (col >= 0 && col < K) {
+    if (col >= 0 && col < K && !d.wrecolour[d.shdr[d.cpair[c].x].w]) {
       const int2 pr = d.cpair[c];
       const int ba = d.shdr[pr.x].y, bb = d.shdr[pr.y].y;
       const unsigned long long bit = 1ull << col;
@@ -441,11 +512,16 @@ __global__ void __launch_bounds__(kBlock) k_colour_count(Dev d) {
   for (int c = threadIdx.x; c < kColourSlots; c += blockDim.x) hist[c] = 0;
   __syncthreads();
   const int n = d.cnt[C_CONS];
-  B2G_FOR(k, 0, n) {
-    const int col = d.ccolour[k];
-    if (col < 0) continue;  // still uncoloured: the host runs more rounds
-    atomicAdd(&hist[col], 1);
-    d.ccol[d.clist[k]] = col;
+  B2G_FOR_ALL(k, valid, 0, n) {
+    const int col = valid ? d.ccolour[k] : -1;  // < 0: still uncoloured, the host runs more rounds
+    int w = 0;
+    if (col >= 0) {
+      atomicAdd(&hist[col], 1);
+      const int c = d.clist[k];
+      d.ccol[c] = col;
+      w = d.shdr[d.cpair[c].x].w;
+    }
+    wave_atomic_max_int_keyed(d.wcolmax, w, col >= 0 && col < kMaxColours, col);
   }
   __syncthreads();
   for (int c = threadIdx.x; c < kColourSlots; c += blockDim.x)
@@ -463,6 +539,26 @@ __global__ void k_clear_pairs(Dev d) {
   if (blockIdx.x == 0 && threadIdx.x == 0) d.cnt[C_PAIRS] = 0;
 }
 
+// Per-world colouring upkeep, after a step's colouring is complete. Kept
+// colours fragment as contacts come and go (a new constraint sees coloured
+// neighbourhoods at both of its bodies); when a world uses kRecolourSlack
+// more colours than its last colouring from scratch, its next step colours
+// every constraint afresh. Each world decides from its own steps only.
+constexpr int kRecolourSlack = 3;
+__global__ void __launch_bounds__(kBlock) k_world_colour_stats(Dev d) {
+  B2G_FOR(w, 0, d.nw) {
+    if (!world_active(d, w)) continue;
+    const int used = d.wcolmax[w] + 1;
+    d.wcolmax[w] = -1;
+    if (d.wrecolour[w]) {
+      d.wfresh[w] = used > 0 ? used : 1;
+      d.wrecolour[w] = 0;
+    } else if (used >= d.wfresh[w] + kRecolourSlack) {
+      d.wrecolour[w] = 1;
+    }
+  }
+}
+
 // Constraints still uncoloured after the rounds a step allows go to the serial
 // group (solved on one lane after the parallel colours: still a valid
 // Gauss-Seidel order).
@@ -476,21 +572,26 @@ __global__ void __launch_bounds__(kBlock) k_colour_leftover(Dev d) {
   }
 }
 
-// one thread: offsets of the colour groups; resets counters for the next
-// step and advances the claim-round base past the rounds just run
-__global__ void k_colour_offsets(Dev d, int rounds_run) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  d.cnt[C_ROUND] += rounds_run;
-  int off = 0, ncol = 0;
-  for (int c = 0; c < kColourSlots; ++c) {
-    d.col_off[c] = off;
-    off += d.col_count[c];
-    if (d.col_count[c] > 0) ncol = c + 1;
-    d.col_cursor[c] = 0;
+// one wave: offsets of the colour groups (lane c scans colour c, lane 0 also
+// the serial slot); resets the scatter cursors and advances the claim-round
+// base past the rounds just run
+__global__ void __launch_bounds__(kWave) k_colour_offsets(Dev d, int rounds_run) {
+  static_assert(kColourSlots == kWave + 1, "one lane per normal colour plus the serial slot");
+  const int lane = (int)threadIdx.x;
+  const int cnt = d.col_count[lane];
+  const int inc = wave_inclusive_scan(cnt);
+  d.col_off[lane] = inc - cnt;
+  d.col_cursor[lane] = 0;
+  const int top = __popcll(__ballot(cnt > 0)) ? 64 - __clzll((long long)__ballot(cnt > 0)) : 0;
+  if (lane == kWave - 1) {
+    const int serial = d.col_count[kSerialColour];
+    d.col_off[kSerialColour] = inc;
+    d.col_off[kSerialColour + 1] = inc + serial;
+    d.col_cursor[kSerialColour] = 0;
+    d.cnt[C_NCOLOURS] = serial > 0 ? kSerialColour + 1 : top;
+    d.cnt[C_SERIAL] = serial;
+    d.cnt[C_ROUND] += rounds_run;
   }
-  d.col_off[kColourSlots] = off;
-  d.cnt[C_NCOLOURS] = ncol;
-  d.cnt[C_SERIAL] = d.col_count[kSerialColour];
 }
 
 // Each block reserves a contiguous range per colour with one atomic, then
@@ -1105,7 +1206,7 @@ __global__ void __launch_bounds__(kBlock) k_contact_position(Dev d, int colour, 
   B2G_FOR_ALL(s, valid, begin, end) {
     const int isl = valid ? d.cisland[s] : 0;
     const float m = valid ? contact_position(d, s, isl) : 0.0f;
-    wave_atomic_min_keyed(d.minsep + (it & 1) * d.nb, isl, valid != 0, m);
+    block_atomic_min_keyed(d.minsep + (it & 1) * d.nb, isl, valid != 0, m);
   }
   if (serial_tail && serial_tail_block(d) && threadIdx.x == 0)
     contact_position_serial(d, d.col_off[kSerialColour], d.col_off[kSerialColour + 1], it);
@@ -1396,6 +1497,10 @@ __global__ void __launch_bounds__(kBlock) k_world_colour_sort(Dev d, int rounds_
     off[kColourSlots] = o;
     if (top) atomicMax(&d.cnt[C_NCOLOURS], top);
     if (hist[kSerialColour]) atomicAdd(&d.cnt[C_SERIAL], hist[kSerialColour]);
+    int topn = 0;  // normal colours only
+    for (int c = 0; c < kMaxColours; ++c)
+      if (hist[c]) topn = c + 1;
+    d.wcolmax[w] = topn - 1;
   }
   __syncthreads();
   for (int c = threadIdx.x; c <= kColourSlots; c += blockDim.x) d.wcol[w * (kColourSlots + 1) + c] = t0 + off[c];
@@ -1533,6 +1638,7 @@ __global__ void __launch_bounds__(1024) k_solve_worlds(Dev d, int vi, int pi) {
 // src/dynamics/b2_contact_manager.cpp:136-188).
 // ===========================================================================
 __global__ void __launch_bounds__(kBlock) k_world_bounds_reset(Dev d) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.cnt[C_HEAVY] = 0;
   B2G_FOR(w, 0, d.nw) {
     d.wbounds[4 * w + 0] = 0xffffffffu;
     d.wbounds[4 * w + 1] = 0xffffffffu;
@@ -1693,23 +1799,28 @@ __global__ void __launch_bounds__(kBlock) k_leaf_aabbs(Dev d, const uint32_t* __
   B2G_FOR(k, 0, d.ns) d.sparse[k] = d.aabb[vals[k]];
 }
 
-// levels j and j + 1 from level j - 1
+// levels j, j + 1 and j + 2 from level j - 1 (2, 4 and 8 entries of it;
+// indices past the end clamp to the last leaf, which the unions absorb)
 __global__ void __launch_bounds__(kBlock) k_sparse_levels(Dev d, int j) {
   const int n = d.ns;
   const int h = 1 << (j - 1);
   const float4* prev = d.sparse + (size_t)(j - 1) * n;
-  float4* cur = d.sparse + (size_t)j * n;
-  float4* nxt = d.sparse + (size_t)(j + 1) * n;
-  const bool two = j + 1 < d.sparse_levels;
+  float4* l1 = d.sparse + (size_t)j * n;
+  float4* l2 = d.sparse + (size_t)(j + 1) * n;
+  float4* l3 = d.sparse + (size_t)(j + 2) * n;
+  const bool has2 = j + 1 < d.sparse_levels, has3 = j + 2 < d.sparse_levels;
   B2G_FOR(i, 0, n) {
-    const float4 a = prev[i];
-    const float4 b = prev[min(i + h, n - 1)];
-    const float4 u = aabb_union(a, b);
-    cur[i] = u;
-    if (two) {
-      const float4 c = prev[min(i + 2 * h, n - 1)];
-      const float4 e = prev[min(i + 3 * h, n - 1)];
-      nxt[i] = aabb_union(u, aabb_union(c, e));
+    const float4 u1 = aabb_union(prev[i], prev[min(i + h, n - 1)]);
+    l1[i] = u1;
+    if (has2) {
+      const float4 u2 = aabb_union(u1, aabb_union(prev[min(i + 2 * h, n - 1)], prev[min(i + 3 * h, n - 1)]));
+      l2[i] = u2;
+      if (has3) {
+        float4 u3 = u2;
+#pragma unroll
+        for (int k = 4; k < 8; ++k) u3 = aabb_union(u3, prev[min(i + k * h, n - 1)]);
+        l3[i] = u3;
+      }
     }
   }
 }
@@ -1794,7 +1905,7 @@ constexpr int kQueryBlock = 128;  // LDS traversal stack: kStack x kQueryBlock i
 // thread]), so lanes at different depths do not conflict. Found keys are kept
 // sorted in `out`; the count can exceed `cap`, in which case k_pairs_compact
 // traverses again for that shape straight into the pair list.
-__device__ __forceinline__ int lbvh_query(const Dev& d, int i, uint64_t* out, int cap, int* stk) {
+__device__ __forceinline__ int lbvh_query(const Dev& d, int i, uint64_t* out, int cap, int* stk, bool sorted) {
   const int n = d.ns;
   if (n < 2) return 0;
   const float4 q = d.aabb[i];
@@ -1827,12 +1938,16 @@ __device__ __forceinline__ int lbvh_query(const Dev& d, int i, uint64_t* out, in
         if (j <= i || !pair_allowed(d, fi, d.sfast[j])) continue;
         const uint64_t key = ((uint64_t)(uint32_t)i << 32) | (uint32_t)j;
         if (found < cap) {
-          int b = found - 1;  // sorted insert
-          while (b >= 0 && out[b] > key) {
-            out[b + 1] = out[b];
-            --b;
+          if (sorted) {
+            int b = found - 1;  // sorted insert
+            while (b >= 0 && out[b] > key) {
+              out[b + 1] = out[b];
+              --b;
+            }
+            out[b + 1] = key;
+          } else {
+            out[found] = key;
           }
-          out[b + 1] = key;
         }
         ++found;
       } else if (sp < kStack) {
@@ -1852,7 +1967,7 @@ __global__ void __launch_bounds__(kQueryBlock) k_lbvh_pairs(Dev d) {
   B2G_FOR_ALL(t, valid, 0, d.ns) {
     if (valid) {
       const int i = (int)d.mval[0][t];
-      d.pcount[i] = lbvh_query(d, i, d.pslots + (size_t)i * kPairSlots, kPairSlots, stk);
+      d.pcount[i] = lbvh_query(d, i, d.pslots + (size_t)i * kPairSlots, kPairSlots, stk, true);
     }
   }
 }
@@ -1870,7 +1985,10 @@ __global__ void __launch_bounds__(kQueryBlock) k_pairs_compact(Dev d) {
         for (int k = 0; k < cnt; ++k)
           if (base + k < d.ccap) d.nkey[base + k] = src[k];
       } else if (base + cnt <= d.ccap) {
-        lbvh_query(d, i, d.nkey + base, cnt, stk);
+        // over-full shape: append in traversal order, sorted afterwards by
+        // k_sort_pair_segments (an insertion sort here would be quadratic)
+        lbvh_query(d, i, d.nkey + base, cnt, stk, false);
+        d.heavy[atomicAdd(&d.cnt[C_HEAVY], 1)] = make_int2(base, cnt);
       } else {
         atomicOr(&d.cnt[C_STATUS], S_CONTACT_OVERFLOW);
       }
@@ -2010,6 +2128,53 @@ __global__ void __launch_bounds__(kQueryBlock) k_raycast(Dev d, int n, const flo
         }
       }
       counts[qi] = found;
+    }
+  }
+}
+
+// Ascending sort of the pair segments of over-full shapes: one workgroup per
+// segment, a bitonic network whose comparators all put the smaller key at the
+// lower index (so padding past the end behaves as +infinity). Segments of up
+// to kSortLds keys are sorted in LDS, longer ones in place in global memory.
+constexpr int kSortLds = 4096;
+__device__ __forceinline__ void bitonic_steps(uint64_t* a, int n, int p2) {
+  for (int k = 2; k <= p2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = threadIdx.x; t < (p2 >> 1); t += blockDim.x) {
+        // element pair of this comparator: first step of a merge mirrors
+        const int lo = ((t / j) * 2 * j) + (t % j);
+        const int hi = (j == (k >> 1)) ? (lo ^ (k - 1)) : (lo + j);
+        const int a0 = lo < hi ? lo : hi, a1 = lo < hi ? hi : lo;
+        if (a1 < n) {
+          const uint64_t x = a[a0], y = a[a1];
+          if (x > y) {
+            a[a0] = y;
+            a[a1] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_sort_pair_segments(Dev d) {
+  __shared__ uint64_t buf[kSortLds];
+  const int nseg = d.cnt[C_HEAVY];
+  for (int sgi = blockIdx.x; sgi < nseg; sgi += gridDim.x) {
+    const int2 sg = d.heavy[sgi];
+    uint64_t* a = d.nkey + sg.x;
+    const int n = sg.y;
+    int p2 = 1;
+    while (p2 < n) p2 <<= 1;
+    if (n <= kSortLds) {
+      for (int t = threadIdx.x; t < n; t += blockDim.x) buf[t] = a[t];
+      __syncthreads();
+      bitonic_steps(buf, n, p2);
+      for (int t = threadIdx.x; t < n; t += blockDim.x) a[t] = buf[t];
+      __syncthreads();
+    } else {
+      bitonic_steps(a, n, p2);
     }
   }
 }

@@ -119,6 +119,37 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(const int* in, const int*
   }
 }
 
+// ---- one-block exclusive scan for short arrays: each of the 1024 threads
+// owns a contiguous chunk (sum, block scan of the sums, rewrite); one launch
+// instead of three. Correct for any n, meant for n up to ~64k. In and out may
+// alias: every thread reads its whole chunk before the block barrier.
+constexpr int kScan1Threads = 1024;
+__global__ void __launch_bounds__(kScan1Threads) k_scan_single(const int* in, const int* __restrict__ n_dev,
+                                                               int n_host, int* out, int* __restrict__ total_out) {
+  __shared__ int waves[kScan1Threads / kWave];
+  const int n = n_dev ? *n_dev : n_host;
+  const int chunk = (n + kScan1Threads - 1) / kScan1Threads;
+  const int lo = min(n, (int)threadIdx.x * chunk), hi = min(n, lo + chunk);
+  int s = 0;
+  for (int i = lo; i < hi; ++i) s += in[i];
+  const int inc = wave_inclusive_scan(s);
+  if (lane_id() == kWave - 1) waves[wave_id()] = inc;
+  __syncthreads();
+  int before = 0, total = 0;
+  for (int w = 0; w < kScan1Threads / kWave; ++w) {
+    const int v = waves[w];
+    if (w < wave_id()) before += v;
+    total += v;
+  }
+  int run = before + inc - s;
+  for (int i = lo; i < hi; ++i) {
+    const int v = in[i];
+    out[i] = run;
+    run += v;
+  }
+  if (threadIdx.x == 0 && total_out) *total_out = total;
+}
+
 // ---- stable LSD radix sort of (key, value), 8-bit digits -------------------
 // histogram layout: hist[digit * tiles + tile] so one exclusive scan over the
 // whole array yields the scatter base of every (digit, tile).
