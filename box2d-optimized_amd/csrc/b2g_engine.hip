@@ -215,7 +215,7 @@ int b2g_batch::alloc() {
     A(d.sparse, (size_t)levels * NS);
   }
   A(d.pslots, (size_t)kPairSlots * NS); A(d.heavy, NS);
-  A(d.pcount, NS); A(d.poff, NS);
+  A(d.pcount, NS); A(d.pfound, NS); A(d.poff, NS);
   const size_t scan_n = std::max<size_t>(std::max<size_t>(NC, NS), kRadix * tiles);
   A(d.scan_tiles, scan_n / kScanTile + 2);
   A(d.cnt, C_COUNT);
@@ -253,7 +253,7 @@ void b2g_batch::release() {
                   d.csorted, d.cisland, d.cpend, d.events, d.col_count, d.col_off, d.col_cursor, d.bmask, d.bslot,
                   d.parent, d.minsep, d.jok, d.solved, d.mkey[0], d.mkey[1], d.mval[0], d.mval[1],
                   d.sort_hist, d.node_child, d.node_parent, d.leaf_parent, d.node_range, d.node_rec,
-                  d.sfast, d.sparse, d.pslots, d.heavy, d.pcount, d.poff, d.scan_tiles, d.cnt};
+                  d.sfast, d.sparse, d.pslots, d.heavy, d.pcount, d.pfound, d.poff, d.scan_tiles, d.cnt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int k = 0; k < 10; ++k)

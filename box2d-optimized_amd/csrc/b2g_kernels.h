@@ -130,14 +130,17 @@ struct Dev {
   int* leaf_parent;
   int2* node_range;
   float4* node_rec;  // 3 per internal node: left child box, right child box, (left, right, split, -)
-  int4* sfast;       // per shape: body, type | dynamic << 4 | joint partners << 5, category | mask << 16, group
+  int4* sfast;       // per shape: body, type | dynamic << 4 | joint partners << 5 | static << 6,
+                     // category | mask << 16, group
   float4* sparse;    // (levels) x ns AABB table over sorted leaves; level 0 = leaf AABBs
   int sparse_levels;
   uint64_t* pslots;  // kPairSlots per shape: pairs found by the traversal, sorted
   int2* heavy;       // (first, count) of the pair segments of over-full shapes, to be sorted
   int* wleaf;        // 2 per world: first leaf, last leaf + 1
   unsigned* wbounds; // 4 per world (ordered floats): min x, min y, max x, max y
-  int* pcount;       // per shape pair count
+  int* pcount;       // per shape: pairs whose lower shape it is (the pair list's segment length)
+  int* pfound;       // per shape: pairs its traversal found (own + those of a static partner);
+                     // for a static shape, the fill cursor of its segment
   int* poff;         // exclusive scan
   int* scan_tiles;
   int* cnt;
@@ -1734,6 +1737,7 @@ __global__ void __launch_bounds__(kBlock) k_morton(Dev d) {
     const uint32_t code = spread15(qx) | (spread15(qy) << 1);
     d.mkey[0][i] = ((uint64_t)(uint32_t)h.w << 32) | code;
     d.mval[0][i] = (uint32_t)i;
+    d.pcount[i] = 0;  // static shapes' segment lengths are counted up by their partners' traversals
   }
 }
 
@@ -1865,7 +1869,9 @@ __global__ void __launch_bounds__(kBlock) k_shape_filters(Dev d) {
     const int b = h.y;
     const int dyn = d.btype[b] == kDynamicBody ? 1 : 0;
     const int jp = d.bj_off[b + 1] > d.bj_off[b] ? 1 : 0;
-    d.sfast[i] = make_int4(b, (h.x & 15) | (dyn << 4) | (jp << 5), (f.x & 0xffff) | ((f.y & 0xffff) << 16), f.z);
+    const int stat = d.btype[b] == kStaticBody ? 1 : 0;
+    d.sfast[i] = make_int4(b, (h.x & 15) | (dyn << 4) | (jp << 5) | (stat << 6), (f.x & 0xffff) | ((f.y & 0xffff) << 16),
+                           f.z);
   }
 }
 
@@ -1900,20 +1906,29 @@ constexpr int kStack = 64;
 constexpr int kPairSlots = 16;
 constexpr int kQueryBlock = 128;  // LDS traversal stack: kStack x kQueryBlock ints (32 KB)
 
-// Traversal for shape i: pairs (i, j > i) of overlapping, collidable shapes
-// of its world. The stack lives in LDS, thread-minor (stk[level * blockDim +
-// thread]), so lanes at different depths do not conflict. Found keys are kept
-// sorted in `out`; the count can exceed `cap`, in which case k_pairs_compact
-// traverses again for that shape straight into the pair list.
-__device__ __forceinline__ int lbvh_query(const Dev& d, int i, uint64_t* out, int cap, int* stk, bool sorted) {
+// Pair ownership. Every pair is found by exactly one traversal: a shape
+// queries for partners j > i and for static partners of any index, and a
+// static shape does not query. A static shape (a ground, a wall) can overlap
+// thousands of shapes; its partners each find it at the cost of one more
+// leaf test, instead of one lane walking all of them. The pair list stays
+// sorted by key (lower shape << 32 | higher shape): a key with a static lower
+// shape s belongs to s's segment, whose length the finders count up
+// (pcount[s]) and whose order k_sort_pair_segments restores.
+//
+// The traversal stack lives in LDS, thread-minor (stk[level * blockDim +
+// thread]), so lanes at different depths do not conflict. `emit(key, s)` is
+// called for every pair found; s >= 0 is the static lower shape of a key that
+// is not the querying shape's own.
+template <class Emit>
+__device__ __forceinline__ void lbvh_query(const Dev& d, int i, int* stk, Emit emit) {
   const int n = d.ns;
-  if (n < 2) return 0;
-  const float4 q = d.aabb[i];
+  if (n < 2) return;
   const int4 fi = d.sfast[i];
+  if (fi.y & 64) return;  // static: found by its partners
+  const float4 q = d.aabb[i];
   const int w = d.shdr[i].w;
   const int wlo = d.wleaf[2 * w], whi = d.wleaf[2 * w + 1] - 1;
   const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
-  int found = 0;
   int sp = 0;
   stk[tid] = 0;
   sp = 1;
@@ -1935,21 +1950,15 @@ __device__ __forceinline__ int lbvh_query(const Dev& d, int i, uint64_t* out, in
       const int code = __float_as_int(side == 0 ? cw.x : cw.y);
       if (code < 0) {
         const int j = ~code;
-        if (j <= i || !pair_allowed(d, fi, d.sfast[j])) continue;
-        const uint64_t key = ((uint64_t)(uint32_t)i << 32) | (uint32_t)j;
-        if (found < cap) {
-          if (sorted) {
-            int b = found - 1;  // sorted insert
-            while (b >= 0 && out[b] > key) {
-              out[b + 1] = out[b];
-              --b;
-            }
-            out[b + 1] = key;
-          } else {
-            out[found] = key;
-          }
-        }
-        ++found;
+        if (j == i) continue;
+        const int4 fj = d.sfast[j];
+        const bool stat_j = (fj.y & 64) != 0;
+        if (j < i && !stat_j) continue;  // j's own traversal finds it
+        if (!pair_allowed(d, fi, fj)) continue;
+        if (j < i)
+          emit(((uint64_t)(uint32_t)j << 32) | (uint32_t)i, j);
+        else
+          emit(((uint64_t)(uint32_t)i << 32) | (uint32_t)j, -1);
       } else if (sp < kStack) {
         stk[sp++ * nt + tid] = code;
       } else {
@@ -1957,40 +1966,87 @@ __device__ __forceinline__ int lbvh_query(const Dev& d, int i, uint64_t* out, in
       }
     }
   }
-  return found;
 }
 
 // Queries run in Morton order (lane t takes the shape at sorted position t):
-// neighbouring lanes query neighbouring boxes and walk the same nodes.
+// neighbouring lanes query neighbouring boxes and walk the same nodes. A lane
+// keeps up to kPairSlots keys, sorted (keys of static lower partners sort
+// first); the counts go on past kPairSlots, and k_pairs_compact traverses
+// such a shape again.
 __global__ void __launch_bounds__(kQueryBlock) k_lbvh_pairs(Dev d) {
   __shared__ int stk[kStack * kQueryBlock];
   B2G_FOR_ALL(t, valid, 0, d.ns) {
     if (valid) {
       const int i = (int)d.mval[0][t];
-      d.pcount[i] = lbvh_query(d, i, d.pslots + (size_t)i * kPairSlots, kPairSlots, stk, true);
+      uint64_t* out = d.pslots + (size_t)i * kPairSlots;
+      int found = 0, own = 0;
+      lbvh_query(d, i, stk, [&](uint64_t key, int s) {
+        if (found < kPairSlots) {
+          int b = found - 1;  // sorted insert
+          while (b >= 0 && out[b] > key) {
+            out[b + 1] = out[b];
+            --b;
+          }
+          out[b + 1] = key;
+        }
+        ++found;
+        if (s >= 0)
+          atomicAdd(&d.pcount[s], 1);
+        else
+          ++own;
+      });
+      d.pfound[i] = found;  // a static shape: 0, its segment's fill cursor
+      if (!(d.sfast[i].y & 64)) d.pcount[i] = own;
     }
   }
 }
 
-// copy each shape's sorted pairs to its place in the pair list (poff); a
-// shape with more pairs than slots is traversed again straight into place
+// Write each shape's pairs to the pair list: its own keys to its segment
+// (poff), keys of a static lower shape s to s's segment at a cursor. A shape
+// with more pairs than slots is traversed again straight into place.
+// Segments filled in traversal or cursor order are queued for sorting.
 __global__ void __launch_bounds__(kQueryBlock) k_pairs_compact(Dev d) {
   __shared__ int stk[kStack * kQueryBlock];
   B2G_FOR_ALL(i, valid, 0, d.ns) {
     if (valid) {
-      const int cnt = d.pcount[i];
+      const int own = d.pcount[i];
       const int base = d.poff[i];
-      if (cnt <= kPairSlots) {
+      const bool stat = (d.sfast[i].y & 64) != 0;
+      if (base + own > d.ccap) {
+        atomicOr(&d.cnt[C_STATUS], S_CONTACT_OVERFLOW);
+        continue;
+      }
+      auto foreign = [&](uint64_t key, int s) {
+        const int at = d.poff[s] + atomicAdd(&d.pfound[s], 1);
+        if (at < d.ccap) d.nkey[at] = key;  // poff[s] + pcount[s] <= ccap checked by s's lane
+      };
+      if (stat) {
+        if (own > 1) d.heavy[atomicAdd(&d.cnt[C_HEAVY], 1)] = make_int2(base, own);
+        continue;
+      }
+      const int found = d.pfound[i];
+      if (found <= kPairSlots) {
         const uint64_t* src = d.pslots + (size_t)i * kPairSlots;
-        for (int k = 0; k < cnt; ++k)
-          if (base + k < d.ccap) d.nkey[base + k] = src[k];
-      } else if (base + cnt <= d.ccap) {
+        int k = 0;
+        for (int m = 0; m < found; ++m) {
+          const uint64_t key = src[m];
+          const int lo = (int)(key >> 32);
+          if (lo == i)
+            d.nkey[base + k++] = key;
+          else
+            foreign(key, lo);
+        }
+      } else {
         // over-full shape: append in traversal order, sorted afterwards by
         // k_sort_pair_segments (an insertion sort here would be quadratic)
-        lbvh_query(d, i, d.nkey + base, cnt, stk, false);
-        d.heavy[atomicAdd(&d.cnt[C_HEAVY], 1)] = make_int2(base, cnt);
-      } else {
-        atomicOr(&d.cnt[C_STATUS], S_CONTACT_OVERFLOW);
+        int k = 0;
+        lbvh_query(d, i, stk, [&](uint64_t key, int s) {
+          if (s >= 0)
+            foreign(key, s);
+          else if (k < own)
+            d.nkey[base + k++] = key;
+        });
+        if (own > 1) d.heavy[atomicAdd(&d.cnt[C_HEAVY], 1)] = make_int2(base, own);
       }
     }
   }

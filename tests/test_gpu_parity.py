@@ -63,6 +63,35 @@ def test_gpu_pair_set_random_vs_restatement(engine):
     assert total == len(ref) and np.array_equal(keys, ref)
 
 
+def test_gpu_pair_set_static_and_crowded_shapes_vs_restatement(engine):
+    """Pairs of static shapes are found from the partner's side and those
+    segments are sorted afterwards; shapes with more pairs than the per-shape
+    slots are traversed twice. Large static and dynamic boxes at low, middle
+    and high indices overlap hundreds of small shapes (and each other)."""
+    rs = S.Stream(777)
+    n = 6000
+    lo = (rs.uniform(2 * n).reshape(n, 2) * 120.0).astype(np.float32)
+    hi = (lo + 0.2 + rs.uniform(2 * n).reshape(n, 2) * 1.5).astype(np.float32)
+    btype = np.where(np.arange(n) % 23 == 0, 0, 2).astype(np.int32)
+    big = {0: (0, [-1, -1, 121, 2]), 1: (0, [-1, -1, 2, 121]), 2: (2, [40, 40, 80, 80]),
+           3000: (0, [-1, 60, 121, 62]), 3001: (2, [10, 10, 30, 100]),
+           n - 2: (0, [118, -1, 121, 121]), n - 1: (2, [50, 0, 52, 121])}
+    for i, (bt, box) in big.items():
+        lo[i] = box[:2]
+        hi[i] = box[2:]
+        btype[i] = bt
+    aabbs = np.concatenate([lo, hi], axis=1).astype(np.float32)
+    body = np.arange(n, dtype=np.int32)
+    stype = (np.arange(n) % 3).astype(np.int32)
+    keys, total = engine.find_pairs(aabbs, body, stype, None, btype)
+    ref = oracle.orc_find_pairs(aabbs, body, stype, None, btype)
+    lows = ref >> np.uint64(32)
+    per_shape = np.bincount(lows.astype(np.int64), minlength=n)
+    print("pairs", len(ref), "largest segments", np.sort(per_shape)[-4:], flush=True)
+    assert per_shape[0] > 100 and per_shape[3000] > 16  # static segments filled by partners
+    assert total == len(ref) and np.array_equal(keys, ref)
+
+
 def _gpu_world_at(engine, sc, g):
     """GPU world built by the drop-in API, then set to the reference state."""
     w = engine.World(sc)
