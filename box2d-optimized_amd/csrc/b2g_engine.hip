@@ -69,6 +69,8 @@ struct b2g_batch {
   int dev_id = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[7] = {};
+  hipStream_t side = nullptr;  // the island pass, beside the velocity phase
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   int* pinned = nullptr;     // host mirror of counters + colour offsets
   // host mirrors
   std::vector<int> h_btype, h_bworld;
@@ -138,6 +140,8 @@ struct b2g_batch {
   // per-world solver (one workgroup per world)
   bool layout_dirty = true;  // world ranges need recomputing
   bool per_world_ok = false; // joint-free, bodies and shapes laid out world after world
+  bool worlds_contiguous = false;  // bodies and shapes laid out world after world
+  int graph_tail_from = 0;         // first colour of the shared tail launch (from the last eager step)
   int max_world_bodies = 0;
   int solver_mode = 0;       // B2G_SOLVER_AUTO / _COLOURS / _WORLDS
   int world_block = 0;       // threads per world workgroup (0: autotune)
@@ -157,7 +161,9 @@ struct b2g_batch {
   void enqueue_colour_round(int K);
   void enqueue_colour_groups();
   void enqueue_islands_and_init();
-  void enqueue_solve(int vi, int pi, const std::vector<int>& colours, const std::vector<int>& sizes, int graph_grid);
+  void enqueue_islands_join();
+  void enqueue_solve(int vi, int pi, const std::vector<int>& colours, const std::vector<int>& sizes,
+                     const std::vector<int>& begins, int graph_grid);
   int enqueue_graph_body(int vi, int pi, unsigned flags, int K);
   int check_status();
   int update_world_layout();
@@ -172,6 +178,9 @@ struct b2g_batch {
   int build_tree(bool use_given_aabbs);
   int broadphase(bool use_given_aabbs);
   bool tree_stale = true;     // bodies or shapes changed since the last tree build
+  int refits_left = 0;        // steps that may refit the last tree before it is rebuilt
+  int tree_ns = -1;           // shape count of the last full build
+  bool capturing = false;     // a step graph is being captured: full builds only
   void* qbuf = nullptr;       // scratch of the user queries
   size_t qbuf_bytes = 0;
   int query_scratch(size_t bytes);
@@ -192,11 +201,11 @@ int b2g_batch::alloc() {
   A(d.wcol, (size_t)NW * (kColourSlots + 1));
   A(d.wcolmax, NW); A(d.wfresh, NW); A(d.wrecolour, NW); A(d.wbounds, 4 * NW); A(d.wleaf, 2 * NW);
   A(d.shdr, NS); A(d.geo, 9 * (size_t)NS); A(d.smat, NS); A(d.sfilt, NS); A(d.aabb, NS);
-  A(d.joints, NJ); A(d.jsol, NJ); A(d.jorder, NJ); A(d.jisland, NJ);
+  A(d.joints, NJ); A(d.jsol, NJ); A(d.jorder, NJ); A(d.jisland, NJ); A(d.wj_off, NW + 1); A(d.wj_list, NJ);
   A(d.bj_off, NB + 1); A(d.bj_other, 2 * NJ);
   A(d.ckey, NC); A(d.cpair, NC); A(d.cman, 4 * (size_t)NC); A(d.cmat, NC); A(d.ccol, NC);
   A(d.nkey, NC); A(d.npair, NC); A(d.nman, 4 * (size_t)NC); A(d.nmat, NC); A(d.ncol, NC);
-  A(d.touch, NC); A(d.tpos, NC); A(d.clist, NC); A(d.ccolour, NC); A(d.csorted, NC);
+  A(d.touch, NC); A(d.tpos, NC); A(d.clist, NC); A(d.cbody, NC); A(d.ccolour, NC); A(d.csorted, NC);
   A(d.cisland, NC); A(d.cpend, NC); A(d.events, (size_t)kEventWords * NC);
   for (int k = 0; k < 10; ++k) A(d.vc[k], NC);
   for (int k = 0; k < 5; ++k) A(d.pc[k], NC);
@@ -224,6 +233,9 @@ int b2g_batch::alloc() {
   d.ecap = NC;
   HIP_TRY(hipHostMalloc((void**)&pinned, 256 * sizeof(int), hipHostMallocDefault));
   HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
   for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
   h_btype.assign(NB, 0);
   h_bworld.assign(NB, 0);
@@ -236,8 +248,9 @@ int b2g_batch::alloc() {
     HIP_TRY(hipMemcpy(d.wrecolour, one.data(), NW * sizeof(int), hipMemcpyHostToDevice));
   }
   h_params.assign(NW, b2g_world_params{{0.0f, -10.0f}, 1.0f / 60.0f, 0.0f, 1, 1});
-  std::vector<int> zero_off(NB + 1, 0);
+  std::vector<int> zero_off(std::max(NB, NW) + 1, 0);
   HIP_TRY(hipMemcpy(d.bj_off, zero_off.data(), (NB + 1) * sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d.wj_off, zero_off.data(), (NW + 1) * sizeof(int), hipMemcpyHostToDevice));
   return B2G_OK;
 }
 
@@ -248,8 +261,8 @@ void b2g_batch::release() {
   void* ptrs[] = {d.xf, d.pos, d.vel, d.prop, d.prop2, d.force, d.btype, d.bworld, d.wpar, d.wflag, d.wshape0, d.wshape1, d.wbody, d.worder, d.wcol, d.wcolmax, d.wfresh,
                   d.wrecolour,
                   d.wbounds, d.wleaf, d.shdr, d.geo, d.smat, d.sfilt, d.aabb, d.joints, d.jsol,
-                  d.jorder, d.jisland, d.bj_off, d.bj_other, d.ckey, d.cpair, d.cman, d.cmat, d.ccol,
-                  d.nkey, d.npair, d.nman, d.nmat, d.ncol, d.touch, d.tpos, d.clist, d.ccolour,
+                  d.jorder, d.wj_off, d.wj_list, d.jisland, d.bj_off, d.bj_other, d.ckey, d.cpair, d.cman, d.cmat, d.ccol,
+                  d.nkey, d.npair, d.nman, d.nmat, d.ncol, d.touch, d.tpos, d.clist, d.cbody, d.ccolour,
                   d.csorted, d.cisland, d.cpend, d.events, d.col_count, d.col_off, d.col_cursor, d.bmask, d.bslot,
                   d.parent, d.minsep, d.jok, d.solved, d.mkey[0], d.mkey[1], d.mval[0], d.mval[1],
                   d.sort_hist, d.node_child, d.node_parent, d.leaf_parent, d.node_range, d.node_rec,
@@ -263,6 +276,9 @@ void b2g_batch::release() {
   if (pinned) (void)hipHostFree(pinned);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
+  if (fork_ev) (void)hipEventDestroy(fork_ev);
+  if (join_ev) (void)hipEventDestroy(join_ev);
+  if (side) (void)hipStreamDestroy(side);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -294,8 +310,10 @@ int b2g_batch::radix_sort(int n, int bits) {
 }
 
 // The tree over the current AABBs (refreshed from the transforms unless the
-// caller gave AABBs): world bounds, Morton keys, radix sort, Karras build,
-// sparse table and node records.
+// caller gave AABBs): world bounds, Morton keys, radix sort, Karras build
+// (skipped by a refit), sparse table and node records.
+constexpr int kRefitSteps = 3;
+
 int b2g_batch::build_tree(bool use_given_aabbs) {
   const int ns = d.ns;
   if (ns == 0) return B2G_OK;
@@ -307,16 +325,29 @@ int b2g_batch::build_tree(bool use_given_aabbs) {
     // only the world bounds of the caller's AABBs
     hipLaunchKernelGGL(k_world_bounds_from_aabbs, dim3(gs), dim3(kBlock), 0, stream, d);
   }
-  hipLaunchKernelGGL(k_morton, dim3(gs), dim3(kBlock), 0, stream, d);
-  int bits = 32;
-  int wbits = 0;
-  while ((1 << wbits) < d.nw) ++wbits;
-  bits += wbits;
-  bits = ((bits + 2 * kRadixBits - 1) / (2 * kRadixBits)) * (2 * kRadixBits);  // even pass count
-  const int res = radix_sort(ns, bits);
-  if (res != 0) return fail(B2G_ERR_INVALID, "radix sort ended in the wrong buffer");
-  hipLaunchKernelGGL(k_world_leaves, dim3(gs), dim3(kBlock), 0, stream, d, d.mkey[0]);
-  if (ns >= 2) hipLaunchKernelGGL(k_lbvh_build, dim3(grid_for(ns - 1)), dim3(kBlock), 0, stream, d, d.mkey[0]);
+  // Refit: the last build's leaf order and topology with new boxes. Any tree
+  // over the shapes answers the pair queries exactly; a refit one only gets
+  // looser as bodies move, so it is rebuilt every kRefitSteps + 1 steps and
+  // after any upload.
+  const bool refit = !use_given_aabbs && !tree_stale && !capturing && refits_left > 0 && tree_ns == ns;
+  if (refit) {
+    --refits_left;
+  } else {
+    refits_left = use_given_aabbs ? 0 : kRefitSteps;
+    tree_ns = use_given_aabbs ? -1 : ns;
+  }
+  if (!refit) {
+    hipLaunchKernelGGL(k_morton, dim3(gs), dim3(kBlock), 0, stream, d);
+    int bits = 32;
+    int wbits = 0;
+    while ((1 << wbits) < d.nw) ++wbits;
+    bits += wbits;
+    bits = ((bits + 2 * kRadixBits - 1) / (2 * kRadixBits)) * (2 * kRadixBits);  // even pass count
+    const int res = radix_sort(ns, bits);
+    if (res != 0) return fail(B2G_ERR_INVALID, "radix sort ended in the wrong buffer");
+    hipLaunchKernelGGL(k_world_leaves, dim3(gs), dim3(kBlock), 0, stream, d, d.mkey[0]);
+    if (ns >= 2) hipLaunchKernelGGL(k_lbvh_build, dim3(grid_for(ns - 1)), dim3(kBlock), 0, stream, d, d.mkey[0]);
+  }
   hipLaunchKernelGGL(k_leaf_aabbs, dim3(gs), dim3(kBlock), 0, stream, d, d.mval[0]);
   if (ns >= 2) {
     int top = 0;
@@ -406,6 +437,19 @@ int b2g_batch::rebuild_joint_tables() {
     other[pos[h_joints[q].body_b]++] = h_joints[q].body_a;
   }
   if (nj) HIP_TRY(hipMemcpy(d.jorder, order.data(), nj * sizeof(int), hipMemcpyHostToDevice));
+  {  // joints grouped by world (of body A), for the per-world island pass
+    const int nw = d.nw;
+    std::vector<int> woff(nw + 1, 0), wlist(std::max(nj, 1));
+    auto world_of = [&](int q) { return h_bworld[h_joints[q].body_a]; };
+    for (int q = 0; q < nj; ++q)
+      if (world_of(q) >= 0 && world_of(q) < nw) woff[world_of(q) + 1]++;
+    for (int w = 0; w < nw; ++w) woff[w + 1] += woff[w];
+    std::vector<int> wcur(woff.begin(), woff.end() - 1);
+    for (int q = 0; q < nj; ++q)
+      if (world_of(q) >= 0 && world_of(q) < nw) wlist[wcur[world_of(q)]++] = q;
+    HIP_TRY(hipMemcpy(d.wj_off, woff.data(), (nw + 1) * sizeof(int), hipMemcpyHostToDevice));
+    if (nj) HIP_TRY(hipMemcpy(d.wj_list, wlist.data(), nj * sizeof(int), hipMemcpyHostToDevice));
+  }
   HIP_TRY(hipMemcpy(d.bj_off, off.data(), (nb + 1) * sizeof(int), hipMemcpyHostToDevice));
   if (off[nb]) HIP_TRY(hipMemcpy(d.bj_other, other.data(), off[nb] * sizeof(int), hipMemcpyHostToDevice));
   joints_dirty = false;
@@ -476,31 +520,100 @@ void b2g_batch::enqueue_colour_groups() {
   hipLaunchKernelGGL(k_sort_serial_group, dim3(1), dim3(1), 0, stream, d);
 }
 
+// per-world island pass: LDS forest sizes (bodies); larger worlds use the
+// device-wide union-find
+constexpr int kIslandLdsSmall = 2048, kIslandLdsMid = 8192, kIslandLdsLarge = 32768;
+
+// The island pass forks onto the side stream (it needs the compacted solved
+// contacts, which exist by now, and nothing of the velocity phase); the
+// position phase joins it (enqueue_islands_join).
 void b2g_batch::enqueue_islands_and_init() {
   const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
-  hipLaunchKernelGGL(k_island_link, dim3(grid_for(std::max(est_cons, 1) + d.nj)), dim3(kBlock), 0, stream, d);
-  hipLaunchKernelGGL(k_island_label, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
+  (void)hipEventRecord(fork_ev, stream);
+  (void)hipStreamWaitEvent(side, fork_ev, 0);
+  if (worlds_contiguous && max_world_bodies <= kIslandLdsLarge) {
+    // one workgroup per world, the forest in LDS
+    if (max_world_bodies <= kIslandLdsSmall)
+      hipLaunchKernelGGL(k_world_islands<kIslandLdsSmall>, dim3(d.nw), dim3(256), 0, side, d);
+    else if (max_world_bodies <= kIslandLdsMid)
+      hipLaunchKernelGGL(k_world_islands<kIslandLdsMid>, dim3(d.nw), dim3(512), 0, side, d);
+    else
+      hipLaunchKernelGGL(k_world_islands<kIslandLdsLarge>, dim3(d.nw), dim3(1024), 0, side, d);
+  } else {
+    hipLaunchKernelGGL(k_island_link, dim3(grid_for(std::max(est_cons, 1) + d.nj)), dim3(kBlock), 0, side, d);
+    hipLaunchKernelGGL(k_island_label, dim3(grid_for(d.nb)), dim3(kBlock), 0, side, d);
+  }
+  (void)hipEventRecord(join_ev, side);
   hipLaunchKernelGGL(k_contact_init, dim3(gk), dim3(kBlock), 0, stream, d);
   if (d.nj) hipLaunchKernelGGL(k_joint_prepare, dim3(grid_for(d.nj)), dim3(kBlock), 0, stream, d);
 }
 
-// colours: the slots to sweep; sizes[c] (eager) or -1 (graph: unknown)
+void b2g_batch::enqueue_islands_join() {
+  (void)hipStreamWaitEvent(stream, join_ev, 0);
+  hipLaunchKernelGGL(k_island_assign, dim3(grid_for(std::max(est_cons, 1) + d.nj)), dim3(kBlock), 0, stream, d);
+}
+
+// colours of at most this many slots can share the tail launch
+constexpr int kTailColourMax = 512;
+
+// colours: the slots to sweep; sizes[c] (eager) or -1 (graph: unknown);
+// begins[c]: first slot of each swept colour (eager; empty in graph mode)
 void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, const std::vector<int>& sizes,
-                              int graph_grid) {
+                              const std::vector<int>& begins, int graph_grid) {
   // graph mode (sizes unknown): the last colour slot's kernel also runs the
   // serial group in its last-finishing block
   const bool tail = !sizes.empty() && sizes[0] < 0;
-  auto contact_sweep = [&](int id, auto kernel, auto... extra) {
-    for (size_t q = 0; q < colours.size(); ++q) {
+  // the trailing colours [tq, nnorm) of `colours` share one launch: in eager
+  // mode those of at most kTailColourMax slots; captured steps start the tail
+  // where the last eager step's did (sizes are not known there)
+  const int nq = (int)colours.size();
+  const bool has_serial = nq > 0 && colours[nq - 1] == kSerialColour;
+  const int nnorm = has_serial ? nq - 1 : nq;
+  int tq = nnorm;
+  if (!tail) {
+    while (tq > 0 && sizes[tq - 1] <= kTailColourMax && nnorm - tq < kTailMax) --tq;
+    if (nnorm - tq < 2) tq = nnorm;  // one small colour: its own launch is as cheap
+    graph_tail_from = tq < nnorm ? colours[tq] : (nnorm ? colours[nnorm - 1] + 1 : 0);
+  } else if (nnorm > 0) {
+    tq = std::max(std::max(0, nnorm - kTailMax), std::min(nnorm - 1, graph_tail_from));
+  }
+  TailSlots ts{};
+  int tail_items = 0;
+  if (tq < nnorm) {
+    ts.c0 = colours[tq];
+    ts.n = nnorm - tq;
+    ts.known = tail ? 0 : 1;
+    if (!tail) {
+      for (int k = 0; k < ts.n; ++k) {
+        ts.off[k] = begins[tq + k];
+        tail_items += sizes[tq + k];
+      }
+      ts.off[ts.n] = begins[nnorm - 1] + sizes[nnorm - 1];
+    }
+  }
+  auto contact_sweep = [&](int id, auto kernel, auto tail_kernel, auto... extra) {
+    for (int q = 0; q < tq; ++q) {
       const int c = colours[q];
       const int n = sizes[q];
+      const int2 slots = begins.empty() ? make_int2(-1, -1) : make_int2(begins[q], begins[q] + n);
       tbegin(id);
-      if (c == kSerialColour)
-        hipLaunchKernelGGL(kernel, dim3(1), dim3(1), 0, stream, d, c, extra..., 0);
-      else
-        hipLaunchKernelGGL(kernel, dim3(n >= 0 ? grid_for(n) : graph_grid), dim3(kBlock), 0, stream, d, c,
-                           extra..., (tail && q + 1 == colours.size()) ? 1 : 0);
+      hipLaunchKernelGGL(kernel, dim3(n >= 0 ? grid_for(n) : graph_grid), dim3(kBlock), 0, stream, d, c, slots,
+                         extra..., 0);
       tend(id, n);
+    }
+    if (tq < nnorm) {  // graph mode: the serial group runs at the end of the tail launch
+      tbegin(id);
+      const int it_arg[] = {0, extra...};  // the position iteration, 0 for the velocity sweeps
+      hipLaunchKernelGGL(tail_kernel, dim3(1), dim3(kTailThreads), 0, stream, d, ts, it_arg[sizeof...(extra)],
+                         tail ? 1 : 0);
+      tend(id, tail ? -1 : tail_items);
+    }
+    if (has_serial) {
+      const int q = nq - 1;
+      const int2 slots = begins.empty() ? make_int2(-1, -1) : make_int2(begins[q], begins[q] + sizes[q]);
+      tbegin(id);
+      hipLaunchKernelGGL(kernel, dim3(1), dim3(1), 0, stream, d, kSerialColour, slots, extra..., 0);
+      tend(id, sizes[q]);
     }
   };
   auto joint_sweep = [&](auto kernel, auto... extra) {
@@ -510,16 +623,17 @@ void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, c
       if (e > b) hipLaunchKernelGGL(kernel, dim3(grid_for(e - b)), dim3(kBlock), 0, stream, d, b, e, extra...);
     }
   };
-  contact_sweep(B2G_KERNEL_WARM_START, k_warm_start);
+  contact_sweep(B2G_KERNEL_WARM_START, k_warm_start, k_sweep_tail<kTailWarm>);
   joint_sweep(k_joint_init);
   for (int it = 0; it < vi; ++it) {
     joint_sweep(k_joint_velocity);
-    contact_sweep(B2G_KERNEL_CONTACT_VELOCITY, k_contact_velocity);
+    contact_sweep(B2G_KERNEL_CONTACT_VELOCITY, k_contact_velocity, k_sweep_tail<kTailVelocity>);
   }
   hipLaunchKernelGGL(k_store_and_integrate, dim3(grid_for(std::max(est_cons, d.nb))), dim3(kBlock), 0, stream, d);
+  enqueue_islands_join();
   const int gf = grid_for(std::max(est_cons, 1) + d.nj);
   for (int it = 0; it < pi; ++it) {
-    contact_sweep(B2G_KERNEL_CONTACT_POSITION, k_contact_position, it);
+    contact_sweep(B2G_KERNEL_CONTACT_POSITION, k_contact_position, k_sweep_tail<kTailPosition>, it);
     joint_sweep(k_joint_position, it);
     hipLaunchKernelGGL(k_island_finalize, dim3(gf), dim3(kBlock), 0, stream, d, it);
   }
@@ -554,7 +668,7 @@ int b2g_batch::update_world_layout() {
     s1[w] = std::max(s1[w], i);
     ++sc[w];
   }
-  bool ok = d.nj == 0;
+  bool ok = true;
   std::vector<int2> wb(nw);
   std::vector<int> ws1(nw), order(nw);
   max_world_bodies = 0;
@@ -571,8 +685,10 @@ int b2g_batch::update_world_layout() {
   HIP_TRY(hipMemcpy(d.wbody, wb.data(), nw * sizeof(int2), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(d.wshape1, ws1.data(), nw * sizeof(int), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(d.worder, order.data(), nw * sizeof(int), hipMemcpyHostToDevice));
-  per_world_ok = ok;
+  worlds_contiguous = ok;
+  per_world_ok = ok && d.nj == 0;
   layout_dirty = false;
+  joints_dirty = true;  // joints are grouped by world
   return B2G_OK;
 }
 
@@ -662,6 +778,7 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
     round_host += (unsigned)rounds_pending;
     rounds_pending = 0;
     enqueue_islands_and_init();
+    enqueue_islands_join();
     const bool tuning = world_block == 0 && tuned_block == 0;
     if (tuning) HIP_TRY(hipEventRecord(ev[5], stream));
     tbegin(B2G_KERNEL_SOLVER);
@@ -729,19 +846,20 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
     }
   }
   const int* off = pinned + 32;
-  std::vector<int> colours, sizes;
+  std::vector<int> colours, sizes, begins;
   int normal_colours = 0;
   for (int c = 0; c < kColourSlots; ++c) {
     if (off[c + 1] <= off[c]) continue;
     colours.push_back(c);
     sizes.push_back(off[c + 1] - off[c]);
+    begins.push_back(off[c]);
     if (c < kMaxColours) normal_colours = c + 1;
   }
   last_ncolours = pinned[C_NCOLOURS];
   hipLaunchKernelGGL(k_world_colour_stats, dim3(grid_for(d.nw)), dim3(kBlock), 0, stream, d);
   // colour limit for captured steps: what this step needed plus one
   graph_K = std::min(kMaxColours, std::max(normal_colours, 1) + 1);
-  enqueue_solve(vi, pi, colours, sizes, 0);
+  enqueue_solve(vi, pi, colours, sizes, begins, 0);
   if (prof_on) HIP_TRY(hipEventRecord(ev[2], stream));
   if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
     if ((rc = broadphase(false))) return rc;
@@ -777,6 +895,7 @@ int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
     round_host += (unsigned)rounds_pending;
     rounds_pending = 0;
     enqueue_islands_and_init();
+    enqueue_islands_join();
     launch_solve_worlds(vi, pi);
     if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
       int rc = broadphase(false);
@@ -794,7 +913,7 @@ int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
     sizes.push_back(-1);
   }
   const int per_colour = std::max(256, (int)(std::max(est_cons, 256) * 2LL / std::max(K, 1)));
-  enqueue_solve(vi, pi, colours, sizes, grid_for(per_colour));
+  enqueue_solve(vi, pi, colours, sizes, {}, grid_for(per_colour));
   if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
     int rc = broadphase(false);
     if (rc) return rc;
@@ -822,7 +941,9 @@ int b2g_batch::graph_step(int vi, int pi, unsigned flags) {
     const unsigned round0 = round_host;
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    capturing = true;
     const int rc = enqueue_graph_body(vi, pi, flags, graph_K);
+    capturing = false;
     hipError_t e = hipStreamEndCapture(stream, &g);
     if (rc) {
       if (g) (void)hipGraphDestroy(g);

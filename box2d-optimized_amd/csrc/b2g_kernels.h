@@ -94,6 +94,8 @@ struct Dev {
   Joint* joints;
   JointSolve* jsol;
   int* jorder;    // joint indices in colour order
+  int* wj_off;    // per world: first entry in wj_list (nw + 1)
+  int* wj_list;   // joint indices grouped by world
   int* jisland;   // island root per colour-ordered joint
   int* bj_off;    // CSR body -> partner bodies of joints with collideConnected == false
   int* bj_other;
@@ -105,6 +107,7 @@ struct Dev {
   int* touch;     // per contact: 1 if solved this step
   int* tpos;      // exclusive scan of touch
   int* clist;     // compacted contact indices
+  int2* cbody;    // per compacted contact: its two bodies, a static body as ~index
   int* ccolour;   // colour per compacted constraint
   int* csorted;   // colour-sorted slot -> contact index
   int* col_count; int* col_off; int* col_cursor;
@@ -376,7 +379,13 @@ __global__ void __launch_bounds__(kBlock) k_narrowphase(Dev d) {
 __global__ void __launch_bounds__(kBlock) k_compact_touching(Dev d) {
   const int n = d.cnt[C_CONTACTS];
   B2G_FOR(i, 0, n) {
-    if (d.touch[i]) d.clist[d.tpos[i]] = i;
+    if (d.touch[i]) {
+      const int t = d.tpos[i];
+      d.clist[t] = i;
+      const int2 pr = d.cpair[i];
+      const int ba = d.shdr[pr.x].y, bb = d.shdr[pr.y].y;
+      d.cbody[t] = make_int2(d.btype[ba] == kStaticBody ? ~ba : ba, d.btype[bb] == kStaticBody ? ~bb : bb);
+    }
   }
 }
 
@@ -677,18 +686,14 @@ __device__ __forceinline__ void uf_union(int* parent, int a, int b) {
 __global__ void __launch_bounds__(kBlock) k_island_link(Dev d) {
   const int n = d.cnt[C_CONS];
   B2G_FOR(k, 0, n + d.nj) {
-    int ba, bb;
     if (k < n) {
-      const int2 pr = d.cpair[d.clist[k]];
-      ba = d.shdr[pr.x].y;
-      bb = d.shdr[pr.y].y;
+      const int2 bp = d.cbody[k];
+      if ((bp.x | bp.y) >= 0) uf_union(d.parent, bp.x, bp.y);
     } else {
       const Joint& j = d.joints[k - n];
-      ba = j.a;
-      bb = j.b;
       if (!world_active(d, j.world)) continue;
+      if (d.btype[j.a] != kStaticBody && d.btype[j.b] != kStaticBody) uf_union(d.parent, j.a, j.b);
     }
-    if (d.btype[ba] != kStaticBody && d.btype[bb] != kStaticBody) uf_union(d.parent, ba, bb);
   }
 }
 
@@ -707,6 +712,93 @@ __global__ void __launch_bounds__(kBlock) k_island_label(Dev d) {
   B2G_FOR(b, 0, d.nb) {
     if (d.btype[b] != kStaticBody)
       __hip_atomic_store(d.parent + b, uf_root(d.parent, b), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// first index of a sorted array whose value is >= key
+__device__ __forceinline__ int lower_bound_u64(const uint64_t* a, int n, uint64_t key) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Islands of one world per workgroup, with the union-find forest in LDS
+// (kCap >= the world's body count). Same forest rule as k_island_link (the
+// larger root hooks under the smaller), so every island is labelled by its
+// smallest body index, as there. The world's solved contacts are contiguous
+// in the compacted list (they are those of its shapes, whose keys are
+// contiguous in the sorted contact list); its joints come from wj_list.
+__device__ __forceinline__ int lds_load(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ int lds_find(int* par, int x) {
+  for (;;) {
+    const int p = lds_load(par + x);
+    if (p == x) return x;
+    const int gp = lds_load(par + p);
+    if (gp == p) return p;
+    __hip_atomic_store(par + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    x = gp;
+  }
+}
+__device__ __forceinline__ void lds_union(int* par, int a, int b) {
+  for (;;) {
+    a = lds_find(par, a);
+    b = lds_find(par, b);
+    if (a == b) return;
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    if (atomicCAS(&par[hi], hi, lo) == hi) return;
+  }
+}
+
+template <int kCap>
+__global__ void __launch_bounds__(1024) k_world_islands(Dev d) {
+  __shared__ int par[kCap];
+  __shared__ int range[2];
+  const int w = d.worder[blockIdx.x];
+  if (!world_active(d, w)) return;  // the whole workgroup leaves
+  const int2 wb = d.wbody[w];
+  const int b0 = wb.x, nbw = wb.y - wb.x;
+  if (threadIdx.x == 0) {
+    const int nc = d.cnt[C_CONTACTS], total = d.cnt[C_CONS];
+    const int c0 = lower_bound_u64(d.ckey, nc, (uint64_t)(uint32_t)d.wshape0[w] << 32);
+    const int c1 = lower_bound_u64(d.ckey, nc, (uint64_t)(uint32_t)d.wshape1[w] << 32);
+    range[0] = c0 < nc ? d.tpos[c0] : total;
+    range[1] = c1 < nc ? d.tpos[c1] : total;
+  }
+  for (int i = threadIdx.x; i < nbw; i += blockDim.x) par[i] = i;
+  __syncthreads();
+  const int t0 = range[0], t1 = range[1];
+  constexpr int kBatch = 8;  // body pairs loaded ahead of the unions (independent loads in flight)
+  const int stride = (int)blockDim.x;
+  for (int k0 = t0 + (int)threadIdx.x; k0 < t1; k0 += kBatch * stride) {
+    int2 bp[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int k = k0 + u * stride;
+      bp[u] = k < t1 ? d.cbody[k] : make_int2(-1, -1);
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const unsigned la = (unsigned)(bp[u].x - b0), lb = (unsigned)(bp[u].y - b0);
+      if ((bp[u].x | bp[u].y) >= 0 && la < (unsigned)nbw && lb < (unsigned)nbw) lds_union(par, (int)la, (int)lb);
+    }
+  }
+  for (int q = d.wj_off[w] + (int)threadIdx.x; q < d.wj_off[w + 1]; q += blockDim.x) {
+    const Joint& j = d.joints[d.wj_list[q]];
+    const unsigned la = (unsigned)(j.a - b0), lb = (unsigned)(j.b - b0);
+    if (d.btype[j.a] != kStaticBody && d.btype[j.b] != kStaticBody && la < (unsigned)nbw && lb < (unsigned)nbw)
+      lds_union(par, (int)la, (int)lb);
+  }
+  __syncthreads();
+  // labels (a static body is never linked: its own root, as the step's reset left it)
+  for (int i = threadIdx.x; i < nbw; i += blockDim.x) {
+    int x = i;  // no writes to the forest remain: plain walk to the root
+    for (int p = par[x]; p != x; p = par[x]) x = p;
+    d.parent[b0 + i] = b0 + x;
   }
 }
 
@@ -753,9 +845,6 @@ __global__ void __launch_bounds__(kBlock) k_contact_init(Dev d) {
     const float4 wp = d.wpar[w];
     const bool warm = (d.wflag[w] & 1) != 0;
     const float ratio = wp.w;
-    const int isl = island_of(d, ia, ib);
-    d.cisland[s] = isl;
-    island_reset(d, isl);
 
     float ni0 = 0.0f, ti0 = 0.0f, ni1 = 0.0f, ti1 = 0.0f;
     if (warm) {
@@ -860,7 +949,8 @@ __global__ void __launch_bounds__(kBlock) k_contact_init(Dev d) {
     }
     d.vc[0][s] = make_float4(normal.x, normal.y, mat.x, mat.w);
     d.vc[1][s] = make_float4(mA, mB, iA, iB);
-    d.vc[2][s] = make_float4(as_float(ia), as_float(ib), as_float(vcount), as_float(c));
+    // point count, bit 8: the world warm-starts
+    d.vc[2][s] = make_float4(as_float(ia), as_float(ib), as_float(vcount | (warm ? 256 : 0)), as_float(c));
     d.vc[3][s] = make_float4(rA0.x, rA0.y, rB0.x, rB0.y);
     d.vc[4][s] = make_float4(ni0, ti0, nm0, tm0);
     d.vc[5][s] = make_float4(rA1.x, rA1.y, rB1.x, rB1.y);
@@ -895,8 +985,8 @@ template <class BV>
 __device__ __forceinline__ void warm_start_t(const Dev& d, int s, BV vel) {
   {
     const float4 v2i = d.vc[2][s];
-    const int ia = as_int(v2i.x), ib = as_int(v2i.y), count = as_int(v2i.z);
-    if ((d.wflag[d.bworld[ia]] & 1) == 0) return;
+    const int ia = as_int(v2i.x), ib = as_int(v2i.y), count = as_int(v2i.z) & 0xff;
+    if ((as_int(v2i.z) & 256) == 0) return;  // no warm starting in this world
     const float4 n4 = d.vc[0][s], m = d.vc[1][s];
     const vec2 normal = mk(n4.x, n4.y), tangent = cross(normal, 1.0f);
     float4 va = *vel(ia), vb = *vel(ib);
@@ -923,7 +1013,7 @@ __device__ __forceinline__ void warm_start(const Dev& d, int s) { warm_start_t(d
 template <class BV>
 __device__ __forceinline__ void contact_velocity_t(const Dev& d, int s, BV vel) {
   const float4 v2i = d.vc[2][s];
-  const int ia = as_int(v2i.x), ib = as_int(v2i.y), count = as_int(v2i.z);
+  const int ia = as_int(v2i.x), ib = as_int(v2i.y), count = as_int(v2i.z) & 0xff;
   const float4 n4 = d.vc[0][s], m = d.vc[1][s];
   const float mA = m.x, mB = m.y, iA = m.z, iB = m.w;
   const vec2 normal = mk(n4.x, n4.y);
@@ -1047,15 +1137,23 @@ __device__ __forceinline__ bool serial_tail_block(const Dev& d) {
   return last != 0;
 }
 
-__global__ void __launch_bounds__(kBlock) k_warm_start(Dev d, int colour, int serial_tail) {
-  const int begin = d.col_off[colour], end = d.col_off[colour + 1];
+// Slots of a colour: given by the host when it knows them (one dependent
+// load less at the start of every sweep), else read from the offsets.
+__device__ __forceinline__ int2 colour_slots(const Dev& d, int colour, int2 given) {
+  return given.x >= 0 ? given : make_int2(d.col_off[colour], d.col_off[colour + 1]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_warm_start(Dev d, int colour, int2 slots, int serial_tail) {
+  const int2 r = colour_slots(d, colour, slots);
+  const int begin = r.x, end = r.y;
   B2G_FOR(s, begin, end) warm_start(d, s);
   if (serial_tail && serial_tail_block(d) && threadIdx.x == 0)
     for (int s = d.col_off[kSerialColour]; s < d.col_off[kSerialColour + 1]; ++s) warm_start(d, s);
 }
 
-__global__ void __launch_bounds__(kBlock) k_contact_velocity(Dev d, int colour, int serial_tail) {
-  const int begin = d.col_off[colour], end = d.col_off[colour + 1];
+__global__ void __launch_bounds__(kBlock) k_contact_velocity(Dev d, int colour, int2 slots, int serial_tail) {
+  const int2 r = colour_slots(d, colour, slots);
+  const int begin = r.x, end = r.y;
   B2G_FOR(s, begin, end) contact_velocity(d, s);
   if (serial_tail && serial_tail_block(d) && threadIdx.x == 0)
     for (int s = d.col_off[kSerialColour]; s < d.col_off[kSerialColour + 1]; ++s) contact_velocity(d, s);
@@ -1065,7 +1163,7 @@ __global__ void __launch_bounds__(kBlock) k_contact_velocity(Dev d, int colour, 
 __device__ __forceinline__ void store_impulses(const Dev& d, int s) {
   {
     const float4 v2i = d.vc[2][s];
-    const int count = as_int(v2i.z), c = as_int(v2i.w);
+    const int count = as_int(v2i.z) & 0xff, c = as_int(v2i.w);
     const float4 q0 = d.vc[4][s];
     float4* mp = d.cman + 4 * (size_t)c;
     float4 m0 = mp[0];
@@ -1127,10 +1225,12 @@ __global__ void __launch_bounds__(kBlock) k_store_and_integrate(Dev d) {
 // returns the minimum separation seen (0 when the island is already solved)
 template <class BP>
 __device__ __forceinline__ float contact_position_t(const Dev& d, int s, int isl, BP pos) {
-  if (d.solved[isl]) return 0.0f;
+  // the constraint's loads are issued before the island test, so they are in
+  // flight together with it
   const float4 v2i = d.vc[2][s];
-  const int ia = as_int(v2i.x), ib = as_int(v2i.y);
   const float4 q0 = d.pc[0][s], q1 = d.pc[1][s], q2 = d.pc[2][s], q3 = d.pc[3][s], q4 = d.pc[4][s];
+  if (d.solved[isl]) return 0.0f;
+  const int ia = as_int(v2i.x), ib = as_int(v2i.y);
   const vec2 lp = mk(q0.x, q0.y), ln = mk(q0.z, q0.w);
   const vec2 lcA = mk(q2.x, q2.y), lcB = mk(q2.z, q2.w);
   const float mA = q3.x, mB = q3.y, iA = q3.z, iB = q3.w;
@@ -1200,8 +1300,10 @@ __device__ __forceinline__ void contact_position_serial(const Dev& d, int begin,
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_contact_position(Dev d, int colour, int it, int serial_tail) {
-  const int begin = d.col_off[colour], end = d.col_off[colour + 1];
+__global__ void __launch_bounds__(kBlock) k_contact_position(Dev d, int colour, int2 slots, int it,
+                                                             int serial_tail) {
+  const int2 r = colour_slots(d, colour, slots);
+  const int begin = r.x, end = r.y;
   if (blockDim.x == 1) {  // the serial colour group: one lane, slot order
     contact_position_serial(d, begin, end, it);
     return;
@@ -1215,18 +1317,62 @@ __global__ void __launch_bounds__(kBlock) k_contact_position(Dev d, int colour, 
     contact_position_serial(d, d.col_off[kSerialColour], d.col_off[kSerialColour + 1], it);
 }
 
+// Several small colours in one launch: a single workgroup sweeps them in
+// colour order with a workgroup barrier between colours. A launch costs more
+// than a small colour's work, so the last colours of a sweep (a few hundred
+// constraints or fewer each: one CU's bandwidth bounds a workgroup's round)
+// share one launch; the order of the updates is the same as with a launch per
+// colour, and so are the results. The colours'
+// slot ranges come from the host when it knows them, else from the offsets.
+constexpr int kTailThreads = 512;
+constexpr int kTailMax = 16;  // colours per tail launch
+struct TailSlots {
+  int c0, n, known;
+  int off[kTailMax + 1];  // known: slot offsets of colours c0 .. c0 + n
+};
+enum TailOp : int { kTailWarm = 0, kTailVelocity = 1, kTailPosition = 2 };
+
+template <int kOp>
+__global__ void __launch_bounds__(kTailThreads) k_sweep_tail(Dev d, TailSlots t, int it, int serial) {
+  for (int q = 0; q < t.n; ++q) {
+    const int begin = t.known ? t.off[q] : d.col_off[t.c0 + q];
+    const int end = t.known ? t.off[q + 1] : d.col_off[t.c0 + q + 1];
+    for (int base = begin; base < end; base += (int)blockDim.x) {
+      const int s = base + (int)threadIdx.x;
+      const bool valid = s < end;
+      if (kOp == kTailWarm) {
+        if (valid) warm_start(d, s);
+      } else if (kOp == kTailVelocity) {
+        if (valid) contact_velocity(d, s);
+      } else {
+        const int isl = valid ? d.cisland[s] : 0;
+        const float m = valid ? contact_position(d, s, isl) : 0.0f;
+        wave_atomic_min_keyed(d.minsep + (it & 1) * d.nb, isl, valid, m);
+      }
+    }
+    __syncthreads();  // the next colour sees this colour's body updates
+  }
+  if (serial && threadIdx.x == 0) {  // the serial group follows the last colour
+    const int b = d.col_off[kSerialColour], e = d.col_off[kSerialColour + 1];
+    if (kOp == kTailPosition) {
+      contact_position_serial(d, b, e, it);
+    } else {
+      for (int s = b; s < e; ++s) {
+        if (kOp == kTailWarm) warm_start(d, s); else contact_velocity(d, s);
+      }
+    }
+  }
+}
+
 // ===========================================================================
 // b2RevoluteJoint (src/dynamics/b2_revolute_joint.cpp:73-321), colour order q
 // ===========================================================================
 __global__ void __launch_bounds__(kBlock) k_joint_prepare(Dev d) {
   B2G_FOR(q, 0, d.nj) {
     const Joint& j = d.joints[d.jorder[q]];
-    int isl = -1;
-    if (world_active(d, j.world) && (d.btype[j.a] != kStaticBody || d.btype[j.b] != kStaticBody)) {
-      isl = island_of(d, j.a, j.b);
-      island_reset(d, isl);
-    }
-    d.jisland[q] = isl;
+    // solved or not; the island itself is filled in by k_island_assign
+    const bool on = world_active(d, j.world) && (d.btype[j.a] != kStaticBody || d.btype[j.b] != kStaticBody);
+    d.jisland[q] = on ? 0 : -1;
   }
 }
 
@@ -1409,6 +1555,28 @@ __global__ void __launch_bounds__(kBlock) k_joint_position(Dev d, int begin, int
 // solved when min separation >= -3 slop and every joint is within slop. Also
 // re-arms the other parity buffer for iteration it + 1 (a different buffer,
 // so the two roles never race).
+// Island of every solved contact slot and joint, and the island's position
+// state reset (the islands come from a pass that runs beside the velocity
+// phase; the position phase is the first to need them).
+__global__ void __launch_bounds__(kBlock) k_island_assign(Dev d) {
+  const int n = d.cnt[C_CONS];
+  B2G_FOR(k, 0, n + d.nj) {
+    int isl;
+    if (k < n) {
+      const float4 v2i = d.vc[2][k];
+      isl = island_of(d, as_int(v2i.x), as_int(v2i.y));
+      d.cisland[k] = isl;
+    } else {
+      const int q = k - n;
+      if (d.jisland[q] < 0) continue;
+      const Joint& j = d.joints[d.jorder[q]];
+      isl = island_of(d, j.a, j.b);
+      d.jisland[q] = isl;
+    }
+    island_reset(d, isl);
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_island_finalize(Dev d, int it) {
   const int n = d.cnt[C_CONS];
   B2G_FOR(k, 0, n + d.nj) {
@@ -1440,6 +1608,7 @@ __global__ void __launch_bounds__(kBlock) k_finish_bodies(Dev d) {
   }
 }
 
+
 // ===========================================================================
 // Per-world solver: one workgroup runs a whole world's solve (b2Island::Solve,
 // src/dynamics/b2_island.cpp:253-438, with the contact and position phases
@@ -1450,15 +1619,6 @@ __global__ void __launch_bounds__(kBlock) k_finish_bodies(Dev d) {
 // colour share no body, the serial group keeps its per-world order, and the
 // island minimum separation is an order-free atomic minimum.
 // ===========================================================================
-__device__ __forceinline__ int lower_bound_u64(const uint64_t* a, int n, uint64_t key) {
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (a[mid] < key) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
-
 // Slots of world w's constraints (world-major, colour-minor): the touching
 // contacts of its shapes are contiguous in the key-sorted contact list and
 // keep their order in the compacted list. Also persists the colours and
@@ -1734,10 +1894,14 @@ __global__ void __launch_bounds__(kBlock) k_morton(Dev d) {
     const float cx = bb.x + bb.z, cy = bb.y + bb.w;
     const uint32_t qx = (uint32_t)minf(maxf((cx - x0) * sx, 0.0f), 32767.0f);
     const uint32_t qy = (uint32_t)minf(maxf((cy - y0) * sy, 0.0f), 32767.0f);
-    const uint32_t code = spread15(qx) | (spread15(qy) << 1);
+    uint32_t code = spread15(qx) | (spread15(qy) << 1);
+    // a shape spanning more than 1/8 of its world's extent (a ground, a
+    // wall) goes into a subtree of its own next to the rest: its box would
+    // otherwise enlarge every node on its path, and every query would walk
+    // that path
+    if ((bb.z - bb.x) * 8.0f > x1 - x0 || (bb.w - bb.y) * 8.0f > y1 - y0) code |= 1u << 30;
     d.mkey[0][i] = ((uint64_t)(uint32_t)h.w << 32) | code;
     d.mval[0][i] = (uint32_t)i;
-    d.pcount[i] = 0;  // static shapes' segment lengths are counted up by their partners' traversals
   }
 }
 
@@ -1799,8 +1963,15 @@ __device__ __forceinline__ float4 aabb_union(float4 a, float4 b) {
   return make_float4(minf(a.x, b.x), minf(a.y, b.y), maxf(a.z, b.z), maxf(a.w, b.w));
 }
 
+// level 0 of the sparse table: leaf boxes in sorted order (a full build and a
+// refit alike); also clears the pair counts, which static shapes' partners
+// count up in k_lbvh_pairs
 __global__ void __launch_bounds__(kBlock) k_leaf_aabbs(Dev d, const uint32_t* __restrict__ vals) {
-  B2G_FOR(k, 0, d.ns) d.sparse[k] = d.aabb[vals[k]];
+  B2G_FOR(k, 0, d.ns) {
+    const uint32_t i = vals[k];
+    d.sparse[k] = d.aabb[i];
+    d.pcount[i] = 0;
+  }
 }
 
 // levels j, j + 1 and j + 2 from level j - 1 (2, 4 and 8 entries of it;
