@@ -2025,10 +2025,12 @@ __global__ void __launch_bounds__(kBlock) k_node_records(Dev d, const uint64_t* 
     const float4 rb = range_box(d, keys, n, split + 1, r.y);
     const int lcode = lc < 0 ? ~(int)vals[~lc] : lc;
     const int rcode = rc < 0 ? ~(int)vals[~rc] : rc;
+    // a leaf child's static flag, so a query can pass over it without its filter word
+    const int sflags = (lc < 0 && (d.sfast[~lcode].y & 64) ? 1 : 0) | (rc < 0 && (d.sfast[~rcode].y & 64) ? 2 : 0);
     float4* o = d.node_rec + 3 * (size_t)i;
     o[0] = lb;
     o[1] = rb;
-    o[2] = make_float4(__int_as_float(lcode), __int_as_float(rcode), __int_as_float(split), 0.0f);
+    o[2] = make_float4(__int_as_float(lcode), __int_as_float(rcode), __int_as_float(split), __int_as_float(sflags));
   }
 }
 
@@ -2122,10 +2124,9 @@ __device__ __forceinline__ void lbvh_query(const Dev& d, int i, int* stk, Emit e
       if (code < 0) {
         const int j = ~code;
         if (j == i) continue;
-        const int4 fj = d.sfast[j];
-        const bool stat_j = (fj.y & 64) != 0;
+        const bool stat_j = ((__float_as_int(cw.w) >> side) & 1) != 0;
         if (j < i && !stat_j) continue;  // j's own traversal finds it
-        if (!pair_allowed(d, fi, fj)) continue;
+        if (!pair_allowed(d, fi, d.sfast[j])) continue;
         if (j < i)
           emit(((uint64_t)(uint32_t)j << 32) | (uint32_t)i, j);
         else
@@ -2146,19 +2147,20 @@ __device__ __forceinline__ void lbvh_query(const Dev& d, int i, int* stk, Emit e
 // such a shape again.
 __global__ void __launch_bounds__(kQueryBlock) k_lbvh_pairs(Dev d) {
   __shared__ int stk[kStack * kQueryBlock];
+  __shared__ uint64_t slots[kPairSlots * kQueryBlock];  // thread-minor, like the stack
   B2G_FOR_ALL(t, valid, 0, d.ns) {
     if (valid) {
       const int i = (int)d.mval[0][t];
-      uint64_t* out = d.pslots + (size_t)i * kPairSlots;
+      const int tid = (int)threadIdx.x;
       int found = 0, own = 0;
       lbvh_query(d, i, stk, [&](uint64_t key, int s) {
         if (found < kPairSlots) {
           int b = found - 1;  // sorted insert
-          while (b >= 0 && out[b] > key) {
-            out[b + 1] = out[b];
+          while (b >= 0 && slots[b * kQueryBlock + tid] > key) {
+            slots[(b + 1) * kQueryBlock + tid] = slots[b * kQueryBlock + tid];
             --b;
           }
-          out[b + 1] = key;
+          slots[(b + 1) * kQueryBlock + tid] = key;
         }
         ++found;
         if (s >= 0)
@@ -2166,6 +2168,9 @@ __global__ void __launch_bounds__(kQueryBlock) k_lbvh_pairs(Dev d) {
         else
           ++own;
       });
+      uint64_t* out = d.pslots + (size_t)i * kPairSlots;
+      const int kept = found < kPairSlots ? found : kPairSlots;
+      for (int k = 0; k < kept; ++k) out[k] = slots[k * kQueryBlock + tid];
       d.pfound[i] = found;  // a static shape: 0, its segment's fill cursor
       if (!(d.sfast[i].y & 64)) d.pcount[i] = own;
     }
