@@ -211,7 +211,7 @@ int b2g_batch::alloc() {
   for (int k = 0; k < 5; ++k) A(d.pc[k], NC);
   A(d.col_count, kColourSlots + 1); A(d.col_off, kColourSlots + 1); A(d.col_cursor, kColourSlots + 1);
   A(d.bmask, NB); A(d.bslot, NB);
-  A(d.parent, NB); A(d.minsep, 2 * (size_t)NB); A(d.jok, 2 * (size_t)NB); A(d.solved, NB);
+  A(d.btab, (size_t)kBodyTab * NB); A(d.parent, NB); A(d.minsep, 2 * (size_t)NB); A(d.jok, 2 * (size_t)NB); A(d.solved, NB);
   A(d.mkey[0], NS); A(d.mkey[1], NS); A(d.mval[0], NS); A(d.mval[1], NS);
   const size_t tiles = (size_t)(NS + kSortTile - 1) / kSortTile;
   A(d.sort_hist, kRadix * tiles);
@@ -264,7 +264,7 @@ void b2g_batch::release() {
                   d.jorder, d.wj_off, d.wj_list, d.jisland, d.bj_off, d.bj_other, d.ckey, d.cpair, d.cman, d.cmat, d.ccol,
                   d.nkey, d.npair, d.nman, d.nmat, d.ncol, d.touch, d.tpos, d.clist, d.cbody, d.ccolour,
                   d.csorted, d.cisland, d.cpend, d.events, d.col_count, d.col_off, d.col_cursor, d.bmask, d.bslot,
-                  d.parent, d.minsep, d.jok, d.solved, d.mkey[0], d.mkey[1], d.mval[0], d.mval[1],
+                  d.btab, d.parent, d.minsep, d.jok, d.solved, d.mkey[0], d.mkey[1], d.mval[0], d.mval[1],
                   d.sort_hist, d.node_child, d.node_parent, d.leaf_parent, d.node_range, d.node_rec,
                   d.sfast, d.sparse, d.pslots, d.heavy, d.pcount, d.pfound, d.poff, d.scan_tiles, d.cnt};
   for (void* p : ptrs)
@@ -318,13 +318,6 @@ int b2g_batch::build_tree(bool use_given_aabbs) {
   const int ns = d.ns;
   if (ns == 0) return B2G_OK;
   const int gs = grid_for(ns);
-  hipLaunchKernelGGL(k_world_bounds_reset, dim3(grid_for(d.nw)), dim3(kBlock), 0, stream, d);
-  if (!use_given_aabbs) {
-    hipLaunchKernelGGL(k_aabb, dim3(gs), dim3(kBlock), 0, stream, d);
-  } else {
-    // only the world bounds of the caller's AABBs
-    hipLaunchKernelGGL(k_world_bounds_from_aabbs, dim3(gs), dim3(kBlock), 0, stream, d);
-  }
   // Refit: the last build's leaf order and topology with new boxes. Any tree
   // over the shapes answers the pair queries exactly; a refit one only gets
   // looser as bodies move, so it is rebuilt every kRefitSteps + 1 steps and
@@ -335,6 +328,13 @@ int b2g_batch::build_tree(bool use_given_aabbs) {
   } else {
     refits_left = use_given_aabbs ? 0 : kRefitSteps;
     tree_ns = use_given_aabbs ? -1 : ns;
+  }
+  if (!refit) hipLaunchKernelGGL(k_world_bounds_reset, dim3(grid_for(d.nw)), dim3(kBlock), 0, stream, d);
+  if (!use_given_aabbs) {
+    hipLaunchKernelGGL(k_aabb, dim3(gs), dim3(kBlock), 0, stream, d, refit ? 0 : 1);
+  } else {
+    // only the world bounds of the caller's AABBs
+    hipLaunchKernelGGL(k_world_bounds_from_aabbs, dim3(gs), dim3(kBlock), 0, stream, d);
   }
   if (!refit) {
     hipLaunchKernelGGL(k_morton, dim3(gs), dim3(kBlock), 0, stream, d);
@@ -623,7 +623,22 @@ void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, c
       if (e > b) hipLaunchKernelGGL(kernel, dim3(grid_for(e - b)), dim3(kBlock), 0, stream, d, b, e, extra...);
     }
   };
-  contact_sweep(B2G_KERNEL_WARM_START, k_warm_start, k_sweep_tail<kTailWarm>);
+  // warm start: a gather per body when every colour has a body-table entry
+  const int top_colour = nnorm ? colours[nnorm - 1] : -1;
+  if (top_colour < kBodyTab) {
+    int items = 0;
+    for (int q = 0; q < nnorm; ++q) items += std::max(sizes[q], 0);
+    tbegin(B2G_KERNEL_WARM_START);
+    hipLaunchKernelGGL(k_warm_gather, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
+    tend(B2G_KERNEL_WARM_START, tail ? -1 : items);
+    if (has_serial || tail) {  // captured steps: a serial group may appear
+      const int2 slots = has_serial && !begins.empty() ? make_int2(begins[nq - 1], begins[nq - 1] + sizes[nq - 1])
+                                                       : make_int2(-1, -1);
+      hipLaunchKernelGGL(k_warm_start, dim3(1), dim3(1), 0, stream, d, kSerialColour, slots, 0);
+    }
+  } else {
+    contact_sweep(B2G_KERNEL_WARM_START, k_warm_start, k_sweep_tail<kTailWarm>);
+  }
   joint_sweep(k_joint_init);
   for (int it = 0; it < vi; ++it) {
     joint_sweep(k_joint_velocity);

@@ -60,6 +60,7 @@ enum BodyType : int { kStaticBody = 0, kKinematicBody = 1, kDynamicBody = 2 };
 constexpr int kMaxColours = 64;           // bit-mask colours; colour 64 = serial group
 constexpr int kSerialColour = kMaxColours;
 constexpr int kColourSlots = kMaxColours + 1;
+constexpr int kBodyTab = 32;  // colours a body's warm-start gather covers
 
 // all device arrays, passed by value to every kernel
 struct Dev {
@@ -119,6 +120,7 @@ struct Dev {
   int events_on;
   int ecap;
   unsigned long long* bmask;
+  int* btab;      // per body, kBodyTab entries: its constraints' slots in colour order (slot * 2 + side)
   unsigned long long* bslot;
   // islands
   int* parent;
@@ -951,6 +953,15 @@ __global__ void __launch_bounds__(kBlock) k_contact_init(Dev d) {
     d.vc[1][s] = make_float4(mA, mB, iA, iB);
     // point count, bit 8: the world warm-starts
     d.vc[2][s] = make_float4(as_float(ia), as_float(ib), as_float(vcount | (warm ? 256 : 0)), as_float(c));
+    {  // this slot's place in each dynamic body's colour-ordered list (rank of
+       // its colour among the body's colours)
+      const int col = d.ccol[c];
+      if (col >= 0 && col < kBodyTab) {
+        const unsigned long long below = (1ull << col) - 1ull;
+        if (d.btype[ia] == kDynamicBody) d.btab[(size_t)ia * kBodyTab + __popcll(d.bmask[ia] & below)] = 2 * s;
+        if (d.btype[ib] == kDynamicBody) d.btab[(size_t)ib * kBodyTab + __popcll(d.bmask[ib] & below)] = 2 * s + 1;
+      }
+    }
     d.vc[3][s] = make_float4(rA0.x, rA0.y, rB0.x, rB0.y);
     d.vc[4][s] = make_float4(ni0, ti0, nm0, tm0);
     d.vc[5][s] = make_float4(rA1.x, rA1.y, rB1.x, rB1.y);
@@ -1229,14 +1240,14 @@ __device__ __forceinline__ float contact_position_t(const Dev& d, int s, int isl
   // flight together with it
   const float4 v2i = d.vc[2][s];
   const float4 q0 = d.pc[0][s], q1 = d.pc[1][s], q2 = d.pc[2][s], q3 = d.pc[3][s], q4 = d.pc[4][s];
-  if (d.solved[isl]) return 0.0f;
   const int ia = as_int(v2i.x), ib = as_int(v2i.y);
+  const float4 pa = *pos(ia), pb = *pos(ib);  // in flight with the island test too
+  if (d.solved[isl]) return 0.0f;
   const vec2 lp = mk(q0.x, q0.y), ln = mk(q0.z, q0.w);
   const vec2 lcA = mk(q2.x, q2.y), lcB = mk(q2.z, q2.w);
   const float mA = q3.x, mB = q3.y, iA = q3.z, iB = q3.w;
   const float radA = q4.x, radB = q4.y;
   const int mtype = as_int(q4.z), count = as_int(q4.w);
-  const float4 pa = *pos(ia), pb = *pos(ib);
   vec2 cA = mk(pa.x, pa.y), cB = mk(pb.x, pb.y);
   float aA = pa.z, aB = pb.z;
   float min_sep = 0.0f;
@@ -1315,6 +1326,67 @@ __global__ void __launch_bounds__(kBlock) k_contact_position(Dev d, int colour, 
   }
   if (serial_tail && serial_tail_block(d) && threadIdx.x == 0)
     contact_position_serial(d, d.col_off[kSerialColour], d.col_off[kSerialColour + 1], it);
+}
+
+// b2ContactSolver::WarmStart as a gather: each dynamic body applies the
+// impulses of its constraints itself, in colour order (its list in btab),
+// which is the order in which the colour sweeps update it; the arithmetic per
+// point is warm_start_t's, so the velocities are the same bit for bit. One
+// launch instead of one per colour. Needs every colour below kBodyTab; the
+// serial group (not in the body masks) follows in its own launch.
+__global__ void __launch_bounds__(kBlock) k_warm_gather(Dev d) {
+  B2G_FOR(b, 0, d.nb) {
+    const unsigned long long mask = d.bmask[b];  // only dynamic bodies have bits
+    if (mask == 0ull) continue;
+    const int n = __popcll(mask);
+    const float4 v0 = d.vel[b];
+    vec2 v = mk(v0.x, v0.y);
+    float w = v0.z;
+    bool touched = false;
+    constexpr int kChunk = 4;  // constraints whose loads are issued together
+    for (int r0 = 0; r0 < n; r0 += kChunk) {
+      int slot[kChunk];
+      float4 v2i[kChunk], n4[kChunk], mm[kChunk], r0p[kChunk], i0p[kChunk], r1p[kChunk], i1p[kChunk];
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) slot[u] = r0 + u < n ? d.btab[(size_t)b * kBodyTab + r0 + u] : -1;
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        const int s = slot[u] >= 0 ? slot[u] >> 1 : 0;
+        v2i[u] = d.vc[2][s];
+        n4[u] = d.vc[0][s];
+        mm[u] = d.vc[1][s];
+        r0p[u] = d.vc[3][s];
+        i0p[u] = d.vc[4][s];
+        r1p[u] = d.vc[5][s];
+        i1p[u] = d.vc[6][s];
+      }
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        if (slot[u] < 0) continue;
+        if ((as_int(v2i[u].z) & 256) == 0) continue;  // no warm starting in this world
+        touched = true;
+        const bool side_b = (slot[u] & 1) != 0;
+        const int count = as_int(v2i[u].z) & 0xff;
+        const float4 m = mm[u];
+        const vec2 normal = mk(n4[u].x, n4[u].y), tangent = cross(normal, 1.0f);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          if (j < count) {
+            const float4 rr = j == 0 ? r0p[u] : r1p[u], im = j == 0 ? i0p[u] : i1p[u];
+            const vec2 P = im.x * normal + im.y * tangent;
+            if (!side_b) {
+              w -= m.z * cross(mk(rr.x, rr.y), P);
+              v = v - m.x * P;
+            } else {
+              w += m.w * cross(mk(rr.z, rr.w), P);
+              v = v + m.y * P;
+            }
+          }
+        }
+      }
+    }
+    if (touched) d.vel[b] = make_float4(v.x, v.y, w, 0.0f);
+  }
 }
 
 // Several small colours in one launch: a single workgroup sweeps them in
@@ -1801,7 +1873,6 @@ __global__ void __launch_bounds__(1024) k_solve_worlds(Dev d, int vi, int pi) {
 // src/dynamics/b2_contact_manager.cpp:136-188).
 // ===========================================================================
 __global__ void __launch_bounds__(kBlock) k_world_bounds_reset(Dev d) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) d.cnt[C_HEAVY] = 0;
   B2G_FOR(w, 0, d.nw) {
     d.wbounds[4 * w + 0] = 0xffffffffu;
     d.wbounds[4 * w + 1] = 0xffffffffu;
@@ -1840,7 +1911,15 @@ __device__ __forceinline__ float4 shape_aabb(int type, int count, const float4* 
   return make_float4(lo.x - r, lo.y - r, hi.x + r, hi.y + r);
 }
 
-__global__ void __launch_bounds__(kBlock) k_aabb(Dev d) {
+// Shape AABBs; with `bounds`, also the world bounds of the AABB centres
+// (2 * centre, as b2BroadPhase::GetCenter2), which only the Morton keys of a
+// full tree build use. Bounds are reduced per wave, then per block (runs of
+// equal worlds merge), so one world's shapes do not all hit the same four
+// words.
+__global__ void __launch_bounds__(kBlock) k_aabb(Dev d, int bounds) {
+  constexpr int kWaves = kBlock / kWave;
+  __shared__ int s_w[kWaves];
+  __shared__ float s_b[kWaves][4];
   B2G_FOR_ALL(i, valid, 0, d.ns) {
     int w = 0;
     float cx = 0.0f, cy = 0.0f;
@@ -1848,16 +1927,55 @@ __global__ void __launch_bounds__(kBlock) k_aabb(Dev d) {
       const int4 h = d.shdr[i];
       const float4 bb = shape_aabb(h.x, h.z, d.geo + 9 * (size_t)i, xform_of(d.xf[h.y]));
       d.aabb[i] = bb;
-      // world bounds of centres (2 * centre, as b2BroadPhase::GetCenter2)
       cx = bb.x + bb.z;
       cy = bb.y + bb.w;
       w = h.w;
     }
+    if (!bounds) continue;
     unsigned* wb = d.wbounds;
-    wave_atomic_min_keyed(wb + 0, 4 * w, valid != 0, cx);
-    wave_atomic_min_keyed(wb + 1, 4 * w, valid != 0, cy);
-    wave_atomic_max_keyed(wb + 2, 4 * w, valid != 0, cx);
-    wave_atomic_max_keyed(wb + 3, 4 * w, valid != 0, cy);
+    const uint64_t act = __ballot(valid);
+    const int first = act ? __ffsll((long long)act) - 1 : 0;
+    const int key0 = __shfl(w, first, kWave);
+    const bool uniform = __ballot(valid && w != key0) == 0;
+    const float x0 = wave_min_all(valid ? cx : FLT_MAX), y0 = wave_min_all(valid ? cy : FLT_MAX);
+    const float x1 = wave_max_all(valid ? cx : -FLT_MAX), y1 = wave_max_all(valid ? cy : -FLT_MAX);
+    if (!uniform && valid) {  // several worlds in this wave
+      atomicMin(wb + 4 * w + 0, float_to_ordered(cx));
+      atomicMin(wb + 4 * w + 1, float_to_ordered(cy));
+      atomicMax(wb + 4 * w + 2, float_to_ordered(cx));
+      atomicMax(wb + 4 * w + 3, float_to_ordered(cy));
+    }
+    if (lane_id() == 0) {
+      s_w[wave_id()] = act && uniform ? key0 : -1;
+      s_b[wave_id()][0] = x0;
+      s_b[wave_id()][1] = y0;
+      s_b[wave_id()][2] = x1;
+      s_b[wave_id()][3] = y1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int k = -1;
+      float m[4] = {FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX};
+      for (int v = 0; v <= kWaves; ++v) {
+        const int kv = v < kWaves ? s_w[v] : -1;
+        if (k >= 0 && kv != k) {  // flush the run of world k
+          atomicMin(wb + 4 * k + 0, float_to_ordered(m[0]));
+          atomicMin(wb + 4 * k + 1, float_to_ordered(m[1]));
+          atomicMax(wb + 4 * k + 2, float_to_ordered(m[2]));
+          atomicMax(wb + 4 * k + 3, float_to_ordered(m[3]));
+          m[0] = m[1] = FLT_MAX;
+          m[2] = m[3] = -FLT_MAX;
+        }
+        k = kv;
+        if (kv >= 0) {
+          m[0] = fminf(m[0], s_b[v][0]);
+          m[1] = fminf(m[1], s_b[v][1]);
+          m[2] = fmaxf(m[2], s_b[v][2]);
+          m[3] = fmaxf(m[3], s_b[v][3]);
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -1965,8 +2083,9 @@ __device__ __forceinline__ float4 aabb_union(float4 a, float4 b) {
 
 // level 0 of the sparse table: leaf boxes in sorted order (a full build and a
 // refit alike); also clears the pair counts, which static shapes' partners
-// count up in k_lbvh_pairs
+// count up in k_lbvh_pairs, and the list of pair segments to sort
 __global__ void __launch_bounds__(kBlock) k_leaf_aabbs(Dev d, const uint32_t* __restrict__ vals) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.cnt[C_HEAVY] = 0;
   B2G_FOR(k, 0, d.ns) {
     const uint32_t i = vals[k];
     d.sparse[k] = d.aabb[i];
