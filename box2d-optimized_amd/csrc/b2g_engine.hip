@@ -92,8 +92,15 @@ struct b2g_batch {
   std::vector<hipEvent_t> tev;
   size_t tev_used = 0;
   std::vector<long long> t_items;
+  std::vector<int> t_runs;  // launches per timed run
   double t_ms = 0.0;
   long long t_launches = 0, t_total_items = 0;
+  static constexpr int kUntimed = -2;  // a launch that no kernel timing selects
+  // Kernel timing: an event pair around a run of consecutive launches of the
+  // timed class (the per-colour sweeps of one iteration, or a single launch),
+  // divided over the run's launches. Back-to-back launches on the stream tile
+  // the device timeline, so this is each launch's duration plus the pair's
+  // own overhead spread over the run.
   void tbegin(int id) {
     if (id != timed_kernel) return;
     if (tev_used + 2 > tev.size()) {
@@ -105,11 +112,18 @@ struct b2g_batch {
     }
     (void)hipEventRecord(tev[tev_used], stream);
   }
-  void tend(int id, long long items) {
+  void tend(int id, long long items, int launches = 1) {
     if (id != timed_kernel) return;
     (void)hipEventRecord(tev[tev_used + 1], stream);
     tev_used += 2;
     t_items.push_back(items);
+    t_runs.push_back(launches);
+  }
+  template <class K, class... Args>
+  void launch(int id, long long items, K kernel, dim3 grid, dim3 block, uint32_t shmem, Args... args) {
+    tbegin(id);
+    hipLaunchKernelGGL(kernel, grid, block, shmem, stream, args...);
+    tend(id, items);
   }
   void tcollect() {
     if (tev_used == 0) return;
@@ -117,11 +131,12 @@ struct b2g_batch {
     for (size_t k = 0; k + 1 < tev_used; k += 2) {
       float ms = 0.0f;
       if (hipEventElapsedTime(&ms, tev[k], tev[k + 1]) == hipSuccess) t_ms += ms;
-      ++t_launches;
+      t_launches += t_runs[k / 2];
       t_total_items += t_items[k / 2];
     }
     tev_used = 0;
     t_items.clear();
+    t_runs.clear();
   }
 
   // captured steps (graph mode)
@@ -377,10 +392,8 @@ int b2g_batch::broadphase(bool use_given_aabbs) {
   }
   int rc = build_tree(use_given_aabbs);
   if (rc) return rc;
-  tbegin(B2G_KERNEL_PAIRS_WRITE);
   const int gq = grid_for((long long)ns * kBlock / kQueryBlock);  // blocks of kQueryBlock threads
-  hipLaunchKernelGGL(k_lbvh_pairs, dim3(gq), dim3(kQueryBlock), 0, stream, d);
-  tend(B2G_KERNEL_PAIRS_WRITE, ns);
+  launch(B2G_KERNEL_PAIRS_WRITE, ns, k_lbvh_pairs, dim3(gq), dim3(kQueryBlock), 0, d);
   scan(d.pcount, nullptr, ns, d.poff, d.cnt + C_PAIRS, ns);
   hipLaunchKernelGGL(k_pairs_compact, dim3(gq), dim3(kQueryBlock), 0, stream, d);
   hipLaunchKernelGGL(k_sort_pair_segments, dim3(64), dim3(kBlock), 0, stream, d);
@@ -490,9 +503,7 @@ void b2g_batch::enqueue_collide_and_colour(int K, int rounds) {
   const int gc = grid_for(std::max(est_contacts, 256) * 5 / 4);
   const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
   const int gb = grid_for(nb);
-  tbegin(B2G_KERNEL_NARROWPHASE);
-  hipLaunchKernelGGL(k_narrowphase, dim3(gc), dim3(kBlock), 0, stream, d);
-  tend(B2G_KERNEL_NARROWPHASE, est_contacts);
+  launch(B2G_KERNEL_NARROWPHASE, est_contacts, k_narrowphase, dim3(gc), dim3(kBlock), 0, d);
   // the bound only picks the scan's launch shape; any device count is handled
   scan(d.touch, d.cnt + C_CONTACTS, 0, d.tpos, d.cnt + C_CONS, std::max(est_contacts, 1) * 5LL / 4);
   hipLaunchKernelGGL(k_compact_touching, dim3(gc), dim3(kBlock), 0, stream, d);
@@ -592,28 +603,27 @@ void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, c
     }
   }
   auto contact_sweep = [&](int id, auto kernel, auto tail_kernel, auto... extra) {
+    long long run_items = 0;
+    if (tq > 0) tbegin(id);  // one timed run: the per-colour launches
     for (int q = 0; q < tq; ++q) {
       const int c = colours[q];
       const int n = sizes[q];
       const int2 slots = begins.empty() ? make_int2(-1, -1) : make_int2(begins[q], begins[q] + n);
-      tbegin(id);
       hipLaunchKernelGGL(kernel, dim3(n >= 0 ? grid_for(n) : graph_grid), dim3(kBlock), 0, stream, d, c, slots,
                          extra..., 0);
-      tend(id, n);
+      run_items += n;
     }
+    if (tq > 0) tend(id, run_items, tq);
     if (tq < nnorm) {  // graph mode: the serial group runs at the end of the tail launch
-      tbegin(id);
+      // the tail is another kernel: not timed with the per-colour sweeps
       const int it_arg[] = {0, extra...};  // the position iteration, 0 for the velocity sweeps
-      hipLaunchKernelGGL(tail_kernel, dim3(1), dim3(kTailThreads), 0, stream, d, ts, it_arg[sizeof...(extra)],
-                         tail ? 1 : 0);
-      tend(id, tail ? -1 : tail_items);
+      launch(kUntimed, tail ? -1 : tail_items, tail_kernel, dim3(1), dim3(kTailThreads), 0, d, ts,
+             it_arg[sizeof...(extra)], tail ? 1 : 0);
     }
     if (has_serial) {
       const int q = nq - 1;
       const int2 slots = begins.empty() ? make_int2(-1, -1) : make_int2(begins[q], begins[q] + sizes[q]);
-      tbegin(id);
-      hipLaunchKernelGGL(kernel, dim3(1), dim3(1), 0, stream, d, kSerialColour, slots, extra..., 0);
-      tend(id, sizes[q]);
+      launch(id, sizes[q], kernel, dim3(1), dim3(1), 0, d, kSerialColour, slots, extra..., 0);
     }
   };
   auto joint_sweep = [&](auto kernel, auto... extra) {
@@ -628,9 +638,7 @@ void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, c
   if (top_colour < kBodyTab) {
     int items = 0;
     for (int q = 0; q < nnorm; ++q) items += std::max(sizes[q], 0);
-    tbegin(B2G_KERNEL_WARM_START);
-    hipLaunchKernelGGL(k_warm_gather, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
-    tend(B2G_KERNEL_WARM_START, tail ? -1 : items);
+    launch(B2G_KERNEL_WARM_START, tail ? -1 : items, k_warm_gather, dim3(grid_for(d.nb)), dim3(kBlock), 0, d);
     if (has_serial || tail) {  // captured steps: a serial group may appear
       const int2 slots = has_serial && !begins.empty() ? make_int2(begins[nq - 1], begins[nq - 1] + sizes[nq - 1])
                                                        : make_int2(-1, -1);
@@ -735,10 +743,10 @@ bool b2g_batch::world_lds_now() const {
 void b2g_batch::launch_solve_worlds(int vi, int pi) {
   const int wblock = world_block_now();
   if (world_lds_now())
-    hipLaunchKernelGGL(k_solve_worlds<true>, dim3(d.nw), dim3(wblock), sizeof(float4) * (size_t)max_world_bodies,
-                       stream, d, vi, pi);
+    launch(B2G_KERNEL_SOLVER, est_cons, k_solve_worlds<true>, dim3(d.nw), dim3(wblock),
+           (uint32_t)(sizeof(float4) * (size_t)max_world_bodies), d, vi, pi);
   else
-    hipLaunchKernelGGL(k_solve_worlds<false>, dim3(d.nw), dim3(wblock), 0, stream, d, vi, pi);
+    launch(B2G_KERNEL_SOLVER, est_cons, k_solve_worlds<false>, dim3(d.nw), dim3(wblock), 0, d, vi, pi);
 }
 
 int b2g_batch::step(int vi, int pi, unsigned flags) {
@@ -796,9 +804,7 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
     enqueue_islands_join();
     const bool tuning = world_block == 0 && tuned_block == 0;
     if (tuning) HIP_TRY(hipEventRecord(ev[5], stream));
-    tbegin(B2G_KERNEL_SOLVER);
     launch_solve_worlds(vi, pi);
-    tend(B2G_KERNEL_SOLVER, est_cons);
     if (tuning) {
       HIP_TRY(hipEventRecord(ev[6], stream));
       HIP_TRY(hipEventSynchronize(ev[6]));

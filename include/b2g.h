@@ -240,9 +240,13 @@ int b2g_batch_wait(b2g_batch* batch);
 
 int b2g_get_counts(b2g_batch* batch, b2g_counts* out);
 
-/* HIP-event timing of every launch of one kernel during the following steps
- * (-1 = off). b2g_kernel_time returns the summed device time, the number of
- * launches and the work items they covered (contacts, constraints or shapes). */
+/* Timing of every launch of one kernel during the following steps (-1 =
+ * off): HIP events around each run of consecutive launches of that kernel
+ * (the per-colour launches of one sweep, or a single launch). The contact
+ * sweeps are the per-colour launches (not the launch shared by the small
+ * trailing colours). b2g_kernel_time returns the summed device time, the
+ * number of launches and the work items they covered (contacts, constraints
+ * or shapes). */
 enum {
   B2G_KERNEL_NARROWPHASE = 0,
   B2G_KERNEL_WARM_START = 1,
