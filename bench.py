@@ -6,9 +6,9 @@ Headline workload: the large random pile.
 - dt 1/60, 8 velocity / 3 position iterations, continuous physics and sleep
   off: the configuration the body-steps/s metric is quoted on.
 - 500 steps: 20 warm-up steps through the drop-in b2World API, then 480
-  timed steps through the b2g C ABI. Each timed step replays the captured
-  step graph (--eager launches kernel by kernel), with no host
-  synchronisation between steps and the world resident in HBM.
+  timed steps through the b2g C ABI, launched kernel by kernel from the host
+  with one colour read-back per step (--launch graph replays a captured step
+  graph instead), with the world resident in HBM.
 - `value` = dynamic bodies x timed steps / wall time.
 
 Further sections of the same JSON line:
@@ -18,8 +18,9 @@ Further sections of the same JSON line:
 - `serving`: Poisson requests against a resident pool; p50/p99 latency
   beside throughput.
 
-roofline: HIP events around every launch of the dominant kernel, over a
-further measured segment, give its average launch duration. Algorithmic bytes
+roofline: HIP events around each run of consecutive launches of the
+dominant kernel (one sweep's per-colour launches), over a further measured
+segment, give its average launch duration. Algorithmic bytes
 per launch are the per-item payloads of the reference's data layout (counted
 once per pass) times the items the launch processed.
 

@@ -226,7 +226,7 @@ int b2g_batch::alloc() {
   for (int k = 0; k < 5; ++k) A(d.pc[k], NC);
   A(d.col_count, kColourSlots + 1); A(d.col_off, kColourSlots + 1); A(d.col_cursor, kColourSlots + 1);
   A(d.bmask, NB); A(d.bslot, NB);
-  A(d.btab, (size_t)kBodyTab * NB); A(d.parent, NB); A(d.minsep, 2 * (size_t)NB); A(d.jok, 2 * (size_t)NB); A(d.solved, NB);
+  A(d.btab, (size_t)kBodyTab * NB); A(d.parent, NB); A(d.sepbad, 2 * (size_t)NB); A(d.jok, 2 * (size_t)NB); A(d.solved, NB);
   A(d.mkey[0], NS); A(d.mkey[1], NS); A(d.mval[0], NS); A(d.mval[1], NS);
   const size_t tiles = (size_t)(NS + kSortTile - 1) / kSortTile;
   A(d.sort_hist, kRadix * tiles);
@@ -279,7 +279,7 @@ void b2g_batch::release() {
                   d.jorder, d.wj_off, d.wj_list, d.jisland, d.bj_off, d.bj_other, d.ckey, d.cpair, d.cman, d.cmat, d.ccol,
                   d.nkey, d.npair, d.nman, d.nmat, d.ncol, d.touch, d.tpos, d.clist, d.cbody, d.ccolour,
                   d.csorted, d.cisland, d.cpend, d.events, d.col_count, d.col_off, d.col_cursor, d.bmask, d.bslot,
-                  d.btab, d.parent, d.minsep, d.jok, d.solved, d.mkey[0], d.mkey[1], d.mval[0], d.mval[1],
+                  d.btab, d.parent, d.sepbad, d.jok, d.solved, d.mkey[0], d.mkey[1], d.mval[0], d.mval[1],
                   d.sort_hist, d.node_child, d.node_parent, d.leaf_parent, d.node_range, d.node_rec,
                   d.sfast, d.sparse, d.pslots, d.heavy, d.pcount, d.pfound, d.poff, d.scan_tiles, d.cnt};
   for (void* p : ptrs)

@@ -124,7 +124,8 @@ struct Dev {
   unsigned long long* bslot;
   // islands
   int* parent;
-  unsigned* minsep;  // 2 * nb
+  unsigned* sepbad;  // 2 * nb: per position-iteration parity and island, 1 when a contact of the
+                     // island separated by less than -3 linearSlop in that iteration
   int* jok;          // 2 * nb
   int* solved;       // nb
   // broadphase
@@ -207,54 +208,6 @@ __device__ __forceinline__ void wave_atomic_max_int_keyed(int* base, int key, bo
   }
 }
 
-__device__ __forceinline__ float wave_min_all(float v);
-
-// atomicMin of v into base[key] from a whole block: waves reduce first; when
-// every wave of the block has one key (one island, the common case in large
-// islands) the block issues a single atomic, and only when its minimum is
-// below the slot's current value (the slot only decreases, so a stale read
-// can cost an extra atomic but never skip a needed one). All threads of the
-// block must call this.
-__device__ __forceinline__ void block_atomic_min_keyed(unsigned* base, int key, bool valid, float v) {
-  __shared__ int s_key[kBlock / kWave];
-  __shared__ float s_val[kBlock / kWave];
-  __shared__ int s_mixed;
-  const uint64_t act = __ballot(valid);
-  const int first = act ? __ffsll((long long)act) - 1 : 0;
-  const int key0 = __shfl(key, first, kWave);
-  const bool wave_uniform = __ballot(valid && key != key0) == 0;
-  const float m = wave_min_all(valid ? v : FLT_MAX);
-  if (threadIdx.x == 0) s_mixed = 0;
-  __syncthreads();
-  if (lane_id() == 0) {
-    s_key[wave_id()] = act ? key0 : -1;
-    s_val[wave_id()] = m;
-    if (act && !wave_uniform) s_mixed = 1;
-  }
-  __syncthreads();
-  if (s_mixed) {  // several islands in one wave: per-lane atomics
-    if (valid && v < ordered_to_float(base[key])) atomicMin(base + key, float_to_ordered(v));
-  } else if (threadIdx.x == 0) {
-    int k0 = -1;
-    bool one = true;
-    float bm = FLT_MAX;
-    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) {
-      if (s_key[w] < 0) continue;
-      if (k0 < 0) k0 = s_key[w];
-      else if (s_key[w] != k0) one = false;
-      bm = fminf(bm, s_val[w]);
-    }
-    if (k0 >= 0 && one) {
-      if (bm < ordered_to_float(base[k0])) atomicMin(base + k0, float_to_ordered(bm));
-    } else if (k0 >= 0) {
-      for (int w = 0; w < (int)(blockDim.x / kWave); ++w)
-        if (s_key[w] >= 0 && s_val[w] < ordered_to_float(base[s_key[w]]))
-          atomicMin(base + s_key[w], float_to_ordered(s_val[w]));
-    }
-  }
-  __syncthreads();
-}
-
 __device__ __forceinline__ float wave_min_all(float v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v = fminf(v, __shfl_xor(v, off, kWave));
@@ -264,34 +217,6 @@ __device__ __forceinline__ float wave_max_all(float v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
   return v;
-}
-
-// atomicMin of v into base[key] for the valid lanes; when all valid lanes of
-// the wave share one key (one island / one world, the common case) the wave
-// reduces first and issues a single atomic. All 64 lanes must call this.
-__device__ __forceinline__ void wave_atomic_min_keyed(unsigned* base, int key, bool valid, float v) {
-  const uint64_t act = __ballot(valid);
-  if (act == 0) return;
-  const int first = __ffsll((long long)act) - 1;
-  const int key0 = __shfl(key, first, kWave);
-  if (__ballot(valid && key != key0) == 0) {
-    const float m = wave_min_all(valid ? v : FLT_MAX);
-    if (lane_id() == first) atomicMin(base + key0, float_to_ordered(m));
-  } else if (valid) {
-    atomicMin(base + key, float_to_ordered(v));
-  }
-}
-__device__ __forceinline__ void wave_atomic_max_keyed(unsigned* base, int key, bool valid, float v) {
-  const uint64_t act = __ballot(valid);
-  if (act == 0) return;
-  const int first = __ffsll((long long)act) - 1;
-  const int key0 = __shfl(key, first, kWave);
-  if (__ballot(valid && key != key0) == 0) {
-    const float m = wave_max_all(valid ? v : -FLT_MAX);
-    if (lane_id() == first) atomicMax(base + key0, float_to_ordered(m));
-  } else if (valid) {
-    atomicMax(base + key, float_to_ordered(v));
-  }
 }
 
 __device__ __forceinline__ vec2 v2of(float4 f) { return mk(f.x, f.y); }
@@ -810,7 +735,7 @@ __device__ __forceinline__ int island_of(const Dev& d, int ba, int bb) {
 
 __device__ __forceinline__ void island_reset(const Dev& d, int isl) {
   d.solved[isl] = 0;
-  d.minsep[isl] = float_to_ordered(0.0f);
+  d.sepbad[isl] = 0u;
   d.jok[isl] = 1;
 }
 
@@ -1303,11 +1228,34 @@ __device__ __forceinline__ float contact_position(const Dev& d, int s, int isl) 
   return contact_position_t(d, s, isl, GlobalBodies{d.pos});
 }
 
+// The island early exit (b2_island.cpp:403-408) asks whether the island's
+// minimum separation this iteration (b2ContactSolver::SolvePositionConstraints,
+// b2_contact_solver.cpp:676-779, starting from 0) is >= -3 linearSlop, that
+// is, whether no contact separated by less. So a contact whose own minimum
+// `m` is below marks its island, and settled contacts write nothing: no
+// minimum is reduced across the island, which would put every block of a
+// large island on one address. The marks of a wave collapse to one store
+// when they share an island.
+__device__ __forceinline__ bool separation_bad(float m) { return !(m >= -3.0f * kLinearSlop); }
+__device__ __forceinline__ void mark_island_sep(const Dev& d, int it, int isl, bool valid, float m) {
+  const bool bad = valid && separation_bad(m);
+  const uint64_t act = __ballot(bad);
+  if (act == 0) return;
+  unsigned* base = d.sepbad + (size_t)(it & 1) * d.nb;
+  const int first = __ffsll((long long)act) - 1;
+  const int key0 = __shfl(isl, first, kWave);
+  if (__ballot(bad && isl != key0) == 0) {
+    if (lane_id() == first) __hip_atomic_store(base + key0, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else if (bad) {
+    __hip_atomic_store(base + isl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 __device__ __forceinline__ void contact_position_serial(const Dev& d, int begin, int end, int it) {
   for (int s = begin; s < end; ++s) {
     const int isl = d.cisland[s];
     const float m = contact_position(d, s, isl);
-    atomicMin(&d.minsep[(it & 1) * d.nb + isl], float_to_ordered(m));
+    if (separation_bad(m)) d.sepbad[(size_t)(it & 1) * d.nb + isl] = 1u;
   }
 }
 
@@ -1322,7 +1270,7 @@ __global__ void __launch_bounds__(kBlock) k_contact_position(Dev d, int colour, 
   B2G_FOR_ALL(s, valid, begin, end) {
     const int isl = valid ? d.cisland[s] : 0;
     const float m = valid ? contact_position(d, s, isl) : 0.0f;
-    block_atomic_min_keyed(d.minsep + (it & 1) * d.nb, isl, valid != 0, m);
+    mark_island_sep(d, it, isl, valid != 0, m);
   }
   if (serial_tail && serial_tail_block(d) && threadIdx.x == 0)
     contact_position_serial(d, d.col_off[kSerialColour], d.col_off[kSerialColour + 1], it);
@@ -1419,7 +1367,7 @@ __global__ void __launch_bounds__(kTailThreads) k_sweep_tail(Dev d, TailSlots t,
       } else {
         const int isl = valid ? d.cisland[s] : 0;
         const float m = valid ? contact_position(d, s, isl) : 0.0f;
-        wave_atomic_min_keyed(d.minsep + (it & 1) * d.nb, isl, valid, m);
+        mark_island_sep(d, it, isl, valid, m);
       }
     }
     __syncthreads();  // the next colour sees this colour's body updates
@@ -1655,11 +1603,11 @@ __global__ void __launch_bounds__(kBlock) k_island_finalize(Dev d, int it) {
     const int isl = k < n ? d.cisland[k] : d.jisland[k - n];
     if (isl < 0) continue;
     const int nxt = ((it + 1) & 1) * d.nb + isl;
-    d.minsep[nxt] = float_to_ordered(0.0f);
+    d.sepbad[nxt] = 0u;
     d.jok[nxt] = 1;
     if (d.solved[isl]) continue;
     const int cur = (it & 1) * d.nb + isl;
-    const bool ok = ordered_to_float(d.minsep[cur]) >= -3.0f * kLinearSlop && d.jok[cur] != 0;
+    const bool ok = d.sepbad[cur] == 0u && d.jok[cur] != 0;
     if (ok) d.solved[isl] = 1;
   }
 }
@@ -1826,7 +1774,7 @@ __global__ void __launch_bounds__(1024) k_solve_worlds(Dev d, int vi, int pi) {
             for (int s = lo; s < hi; ++s) {
               const int isl = d.cisland[s];
               const float m = kLds ? contact_position_t(d, s, isl, lpos) : contact_position_t(d, s, isl, gpos);
-              atomicMin(&d.minsep[(it & 1) * d.nb + isl], float_to_ordered(m));
+              if (separation_bad(m)) d.sepbad[(size_t)(it & 1) * d.nb + isl] = 1u;
             }
         } else {
           for (int base = lo; base < hi; base += nt) {  // every lane of a wave takes part in the min
@@ -1835,7 +1783,7 @@ __global__ void __launch_bounds__(1024) k_solve_worlds(Dev d, int vi, int pi) {
             const int isl = valid ? d.cisland[s] : 0;
             float m = 0.0f;
             if (valid) m = kLds ? contact_position_t(d, s, isl, lpos) : contact_position_t(d, s, isl, gpos);
-            wave_atomic_min_keyed(d.minsep + (it & 1) * d.nb, isl, valid, m);
+            mark_island_sep(d, it, isl, valid, m);
           }
         }
         __syncthreads();
@@ -1844,11 +1792,11 @@ __global__ void __launch_bounds__(1024) k_solve_worlds(Dev d, int vi, int pi) {
       for (int k = t0 + tid; k < t1; k += nt) {
         const int isl = d.cisland[k];
         const int nxt = ((it + 1) & 1) * d.nb + isl;
-        d.minsep[nxt] = float_to_ordered(0.0f);
+        d.sepbad[nxt] = 0u;
         d.jok[nxt] = 1;
         if (d.solved[isl]) continue;
         const int cur = (it & 1) * d.nb + isl;
-        if (ordered_to_float(d.minsep[cur]) >= -3.0f * kLinearSlop && d.jok[cur] != 0) d.solved[isl] = 1;
+        if (d.sepbad[cur] == 0u && d.jok[cur] != 0) d.solved[isl] = 1;
       }
       __syncthreads();
     }
