@@ -2173,14 +2173,9 @@ __device__ __forceinline__ void lbvh_query(const Dev& d, int i, int* stk, Emit e
   stk[tid] = 0;
   sp = 1;
   int visits = 0;
-  while (sp > 0) {
-    const int node = stk[--sp * nt + tid];
-    if (++visits > n) {  // a tree has n - 1 internal nodes
-      atomicOr(&d.cnt[C_STATUS], S_TREE_CORRUPT);
-      break;
-    }
-    const float4* rec = d.node_rec + 3 * (size_t)node;
-    const float4 lb = rec[0], rb = rec[1], cw = rec[2];
+  // test both children of one node record; leaves are emitted, overlapping
+  // inner children pushed
+  auto visit = [&](float4 lb, float4 rb, float4 cw) {
     const int split = __float_as_int(cw.z);
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
@@ -2204,6 +2199,24 @@ __device__ __forceinline__ void lbvh_query(const Dev& d, int i, int* stk, Emit e
         atomicOr(&d.cnt[C_STATUS], S_STACK_OVERFLOW);
       }
     }
+  };
+  // two nodes per trip: both records are loaded before either is tested, so
+  // each lane has two loads in flight on its dependent walk
+  while (sp > 0) {
+    const int na = stk[--sp * nt + tid];
+    const bool two = sp > 0;
+    const int nb = two ? stk[--sp * nt + tid] : na;
+    visits += two ? 2 : 1;
+    if (visits > n) {  // a tree has n - 1 internal nodes
+      atomicOr(&d.cnt[C_STATUS], S_TREE_CORRUPT);
+      break;
+    }
+    const float4* ra = d.node_rec + 3 * (size_t)na;
+    const float4* rb = d.node_rec + 3 * (size_t)nb;
+    const float4 a0 = ra[0], a1 = ra[1], a2 = ra[2];
+    const float4 b0 = rb[0], b1 = rb[1], b2 = rb[2];
+    visit(a0, a1, a2);
+    if (two) visit(b0, b1, b2);
   }
 }
 
