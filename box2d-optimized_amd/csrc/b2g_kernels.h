@@ -2200,23 +2200,33 @@ __device__ __forceinline__ void lbvh_query(const Dev& d, int i, int* stk, Emit e
       }
     }
   };
-  // two nodes per trip: both records are loaded before either is tested, so
-  // each lane has two loads in flight on its dependent walk
+  // up to kTrip nodes per trip: their records are loaded before any is
+  // tested, so each lane has several loads in flight on its dependent walk
+  constexpr int kTrip = 4;
   while (sp > 0) {
-    const int na = stk[--sp * nt + tid];
-    const bool two = sp > 0;
-    const int nb = two ? stk[--sp * nt + tid] : na;
-    visits += two ? 2 : 1;
+    int nodes[kTrip];
+    int take = 0;
+#pragma unroll
+    for (int u = 0; u < kTrip; ++u) {
+      nodes[u] = sp > 0 ? stk[--sp * nt + tid] : -1;
+      take += nodes[u] >= 0 ? 1 : 0;
+    }
+    visits += take;
     if (visits > n) {  // a tree has n - 1 internal nodes
       atomicOr(&d.cnt[C_STATUS], S_TREE_CORRUPT);
       break;
     }
-    const float4* ra = d.node_rec + 3 * (size_t)na;
-    const float4* rb = d.node_rec + 3 * (size_t)nb;
-    const float4 a0 = ra[0], a1 = ra[1], a2 = ra[2];
-    const float4 b0 = rb[0], b1 = rb[1], b2 = rb[2];
-    visit(a0, a1, a2);
-    if (two) visit(b0, b1, b2);
+    float4 r0[kTrip], r1[kTrip], r2[kTrip];
+#pragma unroll
+    for (int u = 0; u < kTrip; ++u) {
+      const float4* rec = d.node_rec + 3 * (size_t)(nodes[u] >= 0 ? nodes[u] : nodes[0]);
+      r0[u] = rec[0];
+      r1[u] = rec[1];
+      r2[u] = rec[2];
+    }
+#pragma unroll
+    for (int u = 0; u < kTrip; ++u)
+      if (nodes[u] >= 0) visit(r0[u], r1[u], r2[u]);
   }
 }
 
