@@ -910,11 +910,13 @@ __device__ __forceinline__ bool writes_body(const Dev& d, int b) { return d.btyp
 struct GlobalBodies {
   float4* p;
   __device__ __forceinline__ float4* operator()(int i) const { return p + i; }
+  __device__ __forceinline__ bool writes(const Dev& d, int i) const { return writes_body(d, i); }
 };
 struct LdsBodies {
   float4* p;
   int base;
   __device__ __forceinline__ float4* operator()(int i) const { return p + (i - base); }
+  __device__ __forceinline__ bool writes(const Dev& d, int i) const { return writes_body(d, i); }
 };
 
 template <class BV>
@@ -939,26 +941,62 @@ __device__ __forceinline__ void warm_start_t(const Dev& d, int s, BV vel) {
         vB = vB + m.y * P;
       }
     }
-    if (writes_body(d, ia)) *vel(ia) = make_float4(vA.x, vA.y, wA, 0.0f);
-    if (writes_body(d, ib)) *vel(ib) = make_float4(vB.x, vB.y, wB, 0.0f);
+    if (vel.writes(d, ia)) *vel(ia) = make_float4(vA.x, vA.y, wA, 0.0f);
+    if (vel.writes(d, ib)) *vel(ib) = make_float4(vB.x, vB.y, wB, 0.0f);
   }
 }
 __device__ __forceinline__ void warm_start(const Dev& d, int s) { warm_start_t(d, s, GlobalBodies{d.vel}); }
 
 // b2ContactSolver::SolveVelocityConstraints (:301-639) for slots [begin, end)
-template <class BV>
-__device__ __forceinline__ void contact_velocity_t(const Dev& d, int s, BV vel) {
-  const float4 v2i = d.vc[2][s];
+// Where a constraint solve reads its slot's fields: straight from the
+// constraint arrays (each load issued where the solve needs it), or from a
+// copy loaded ahead into registers (the tail launch loads the next colour's
+// constraints while the current colour is solved).
+struct SlotFields {
+  const Dev& d;
+  int s;
+  __device__ __forceinline__ float4 vc(int k) const { return d.vc[k][s]; }
+  __device__ __forceinline__ float4 pc(int k) const { return d.pc[k][s]; }
+};
+struct VelocityRegs {
+  float4 v[10];
+  __device__ __forceinline__ float4 vc(int k) const { return v[k]; }
+};
+__device__ __forceinline__ VelocityRegs load_velocity_regs(const Dev& d, int s) {
+  VelocityRegs r;
+#pragma unroll
+  for (int k = 0; k < 10; ++k) r.v[k] = d.vc[k][s];
+  return r;
+}
+struct PositionRegs {
+  float4 v2;
+  float4 p[5];
+  int isl;
+  __device__ __forceinline__ float4 vc(int) const { return v2; }  // only field 2 is read
+  __device__ __forceinline__ float4 pc(int k) const { return p[k]; }
+};
+__device__ __forceinline__ PositionRegs load_position_regs(const Dev& d, int s) {
+  PositionRegs r;
+  r.v2 = d.vc[2][s];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) r.p[k] = d.pc[k][s];
+  r.isl = d.cisland[s];
+  return r;
+}
+
+template <class BV, class Src>
+__device__ __forceinline__ void contact_velocity_src(const Dev& d, int s, BV vel, const Src& src) {
+  const float4 v2i = src.vc(2);
   const int ia = as_int(v2i.x), ib = as_int(v2i.y), count = as_int(v2i.z) & 0xff;
-  const float4 n4 = d.vc[0][s], m = d.vc[1][s];
+  const float4 n4 = src.vc(0), m = src.vc(1);
   const float mA = m.x, mB = m.y, iA = m.z, iB = m.w;
   const vec2 normal = mk(n4.x, n4.y);
   const float friction = n4.z, tangent_speed = n4.w;
   float4 va = *vel(ia), vb = *vel(ib);
   vec2 vA = mk(va.x, va.y), vB = mk(vb.x, vb.y);
   float wA = va.z, wB = vb.z;
-  const float4 r0 = d.vc[3][s], r1 = d.vc[5][s];
-  float4 q0 = d.vc[4][s], q1 = d.vc[6][s];
+  const float4 r0 = src.vc(3), r1 = src.vc(5);
+  float4 q0 = src.vc(4), q1 = src.vc(6);
   const vec2 rA0 = mk(r0.x, r0.y), rB0 = mk(r0.z, r0.w);
   const vec2 rA1 = mk(r1.x, r1.y), rB1 = mk(r1.z, r1.w);
   const vec2 tangent = cross(normal, 1.0f);
@@ -982,7 +1020,7 @@ __device__ __forceinline__ void contact_velocity_t(const Dev& d, int s, BV vel) 
       wB += iB * cross(rB, P);
     }
   }
-  const float4 b7 = d.vc[7][s];
+  const float4 b7 = src.vc(7);
   if (count == 1) {
     const vec2 dv = vB + cross(wB, rB0) - (vA + cross(wA, rA0));
     const float vn = dot(dv, normal);
@@ -997,7 +1035,7 @@ __device__ __forceinline__ void contact_velocity_t(const Dev& d, int s, BV vel) 
     wB += iB * cross(rB0, P);
   } else {
     // 2-point block LCP by total enumeration (:389-631)
-    const float4 nm4 = d.vc[8][s], k4 = d.vc[9][s];
+    const float4 nm4 = src.vc(8), k4 = src.vc(9);
     const mat22 K = {mk(k4.x, k4.y), mk(k4.z, k4.w)};
     const mat22 NM = {mk(nm4.x, nm4.y), mk(nm4.z, nm4.w)};
     const vec2 a = mk(q0.x, q1.x);
@@ -1044,8 +1082,12 @@ __device__ __forceinline__ void contact_velocity_t(const Dev& d, int s, BV vel) 
   }
   d.vc[4][s] = q0;
   if (count > 1) d.vc[6][s] = q1;
-  if (writes_body(d, ia)) *vel(ia) = make_float4(vA.x, vA.y, wA, 0.0f);
-  if (writes_body(d, ib)) *vel(ib) = make_float4(vB.x, vB.y, wB, 0.0f);
+  if (vel.writes(d, ia)) *vel(ia) = make_float4(vA.x, vA.y, wA, 0.0f);
+  if (vel.writes(d, ib)) *vel(ib) = make_float4(vB.x, vB.y, wB, 0.0f);
+}
+template <class BV>
+__device__ __forceinline__ void contact_velocity_t(const Dev& d, int s, BV vel) {
+  contact_velocity_src(d, s, vel, SlotFields{d, s});
 }
 __device__ __forceinline__ void contact_velocity(const Dev& d, int s) {
   contact_velocity_t(d, s, GlobalBodies{d.vel});
@@ -1159,12 +1201,12 @@ __global__ void __launch_bounds__(kBlock) k_store_and_integrate(Dev d) {
 
 // b2ContactSolver::SolvePositionConstraints (:659-787) for slots [begin, end)
 // returns the minimum separation seen (0 when the island is already solved)
-template <class BP>
-__device__ __forceinline__ float contact_position_t(const Dev& d, int s, int isl, BP pos) {
+template <class BP, class Src>
+__device__ __forceinline__ float contact_position_src(const Dev& d, int isl, BP pos, const Src& src) {
   // the constraint's loads are issued before the island test, so they are in
   // flight together with it
-  const float4 v2i = d.vc[2][s];
-  const float4 q0 = d.pc[0][s], q1 = d.pc[1][s], q2 = d.pc[2][s], q3 = d.pc[3][s], q4 = d.pc[4][s];
+  const float4 v2i = src.vc(2);
+  const float4 q0 = src.pc(0), q1 = src.pc(1), q2 = src.pc(2), q3 = src.pc(3), q4 = src.pc(4);
   const int ia = as_int(v2i.x), ib = as_int(v2i.y);
   const float4 pa = *pos(ia), pb = *pos(ib);  // in flight with the island test too
   if (d.solved[isl]) return 0.0f;
@@ -1220,9 +1262,13 @@ __device__ __forceinline__ float contact_position_t(const Dev& d, int s, int isl
       aB += iB * cross(rB, P);
     }
   }
-  if (writes_body(d, ia)) *pos(ia) = make_float4(cA.x, cA.y, aA, 0.0f);
-  if (writes_body(d, ib)) *pos(ib) = make_float4(cB.x, cB.y, aB, 0.0f);
+  if (pos.writes(d, ia)) *pos(ia) = make_float4(cA.x, cA.y, aA, 0.0f);
+  if (pos.writes(d, ib)) *pos(ib) = make_float4(cB.x, cB.y, aB, 0.0f);
   return min_sep;
+}
+template <class BP>
+__device__ __forceinline__ float contact_position_t(const Dev& d, int s, int isl, BP pos) {
+  return contact_position_src(d, isl, pos, SlotFields{d, s});
 }
 __device__ __forceinline__ float contact_position(const Dev& d, int s, int isl) {
   return contact_position_t(d, s, isl, GlobalBodies{d.pos});
@@ -1354,23 +1400,68 @@ enum TailOp : int { kTailWarm = 0, kTailVelocity = 1, kTailPosition = 2 };
 
 template <int kOp>
 __global__ void __launch_bounds__(kTailThreads) k_sweep_tail(Dev d, TailSlots t, int it, int serial) {
-  for (int q = 0; q < t.n; ++q) {
-    const int begin = t.known ? t.off[q] : d.col_off[t.c0 + q];
-    const int end = t.known ? t.off[q + 1] : d.col_off[t.c0 + q + 1];
-    for (int base = begin; base < end; base += (int)blockDim.x) {
-      const int s = base + (int)threadIdx.x;
-      const bool valid = s < end;
-      if (kOp == kTailWarm) {
-        if (valid) warm_start(d, s);
-      } else if (kOp == kTailVelocity) {
-        if (valid) contact_velocity(d, s);
-      } else {
-        const int isl = valid ? d.cisland[s] : 0;
-        const float m = valid ? contact_position(d, s, isl) : 0.0f;
-        mark_island_sep(d, it, isl, valid, m);
+  const int nt = (int)blockDim.x, tid = (int)threadIdx.x;
+  bool one_pass = t.known != 0 && kOp != kTailWarm;
+  for (int q = 0; q < t.n && one_pass; ++q) one_pass = t.off[q + 1] - t.off[q] <= nt;
+  if (one_pass) {
+    // every colour fits one pass: lane tid takes slot off[q] + tid of colour
+    // q, and loads the next colour's constraint while this one is solved
+    // (the constraint fields do not depend on the bodies; only the bodies
+    // wait for the barrier)
+    auto slot_of = [&](int q) {
+      const int s = t.off[q] + tid;
+      return s < t.off[q + 1] ? s : -1;
+    };
+    if (kOp == kTailVelocity) {
+      int s_next = slot_of(0);
+      VelocityRegs nxt{};
+      if (s_next >= 0) nxt = load_velocity_regs(d, s_next);
+      for (int q = 0; q < t.n; ++q) {
+        const int s_cur = s_next;
+        const VelocityRegs cur = nxt;
+        if (q + 1 < t.n) {
+          s_next = slot_of(q + 1);
+          if (s_next >= 0) nxt = load_velocity_regs(d, s_next);
+        }
+        if (s_cur >= 0) contact_velocity_src(d, s_cur, GlobalBodies{d.vel}, cur);
+        __syncthreads();  // the next colour sees this colour's body updates
+      }
+    } else {
+      int s_next = slot_of(0);
+      PositionRegs nxt{};
+      if (s_next >= 0) nxt = load_position_regs(d, s_next);
+      for (int q = 0; q < t.n; ++q) {
+        const int s_cur = s_next;
+        const PositionRegs cur = nxt;
+        if (q + 1 < t.n) {
+          s_next = slot_of(q + 1);
+          if (s_next >= 0) nxt = load_position_regs(d, s_next);
+        }
+        const bool valid = s_cur >= 0;
+        const float m = valid ? contact_position_src(d, cur.isl, GlobalBodies{d.pos}, cur) : 0.0f;
+        mark_island_sep(d, it, valid ? cur.isl : 0, valid, m);
+        __syncthreads();
       }
     }
-    __syncthreads();  // the next colour sees this colour's body updates
+  } else {
+    for (int q = 0; q < t.n; ++q) {
+      const int begin = t.known ? t.off[q] : d.col_off[t.c0 + q];
+      const int end = t.known ? t.off[q + 1] : d.col_off[t.c0 + q + 1];
+      for (int base = begin; base < end; base += nt) {
+        const int s = base + tid;
+        const bool valid = s < end;
+        if (kOp == kTailWarm) {
+          if (valid) warm_start(d, s);
+        } else if (kOp == kTailVelocity) {
+          if (valid) contact_velocity(d, s);
+        } else {
+          const int isl = valid ? d.cisland[s] : 0;
+          const float m = valid ? contact_position(d, s, isl) : 0.0f;
+          mark_island_sep(d, it, isl, valid, m);
+        }
+      }
+      __syncthreads();  // the next colour sees this colour's body updates
+    }
   }
   if (serial && threadIdx.x == 0) {  // the serial group follows the last colour
     const int b = d.col_off[kSerialColour], e = d.col_off[kSerialColour + 1];
