@@ -571,8 +571,8 @@ constexpr int kTailColourMax = 512;
 // begins[c]: first slot of each swept colour (eager; empty in graph mode)
 void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, const std::vector<int>& sizes,
                               const std::vector<int>& begins, int graph_grid) {
-  // graph mode (sizes unknown): the last colour slot's kernel also runs the
-  // serial group in its last-finishing block
+  // graph mode (sizes unknown): the shared tail launch also runs the serial
+  // group, after the last colour
   const bool tail = !sizes.empty() && sizes[0] < 0;
   // the trailing colours [tq, nnorm) of `colours` share one launch: in eager
   // mode those of at most kTailColourMax slots; captured steps start the tail
@@ -610,7 +610,7 @@ void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, c
       const int n = sizes[q];
       const int2 slots = begins.empty() ? make_int2(-1, -1) : make_int2(begins[q], begins[q] + n);
       hipLaunchKernelGGL(kernel, dim3(n >= 0 ? grid_for(n) : graph_grid), dim3(kBlock), 0, stream, d, c, slots,
-                         extra..., 0);
+                         extra...);
       run_items += n;
     }
     if (tq > 0) tend(id, run_items, tq);
@@ -623,7 +623,7 @@ void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, c
     if (has_serial) {
       const int q = nq - 1;
       const int2 slots = begins.empty() ? make_int2(-1, -1) : make_int2(begins[q], begins[q] + sizes[q]);
-      launch(id, sizes[q], kernel, dim3(1), dim3(1), 0, d, kSerialColour, slots, extra..., 0);
+      launch(id, sizes[q], kernel, dim3(1), dim3(1), 0, d, kSerialColour, slots, extra...);
     }
   };
   auto joint_sweep = [&](auto kernel, auto... extra) {
@@ -642,7 +642,7 @@ void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, c
     if (has_serial || tail) {  // captured steps: a serial group may appear
       const int2 slots = has_serial && !begins.empty() ? make_int2(begins[nq - 1], begins[nq - 1] + sizes[nq - 1])
                                                        : make_int2(-1, -1);
-      hipLaunchKernelGGL(k_warm_start, dim3(1), dim3(1), 0, stream, d, kSerialColour, slots, 0);
+      hipLaunchKernelGGL(k_warm_start, dim3(1), dim3(1), 0, stream, d, kSerialColour, slots);
     }
   } else {
     contact_sweep(B2G_KERNEL_WARM_START, k_warm_start, k_sweep_tail<kTailWarm>);

@@ -46,7 +46,6 @@ enum Counter : int {
   C_ROUND = 7,      // colouring round base (claim priorities carry base + local round)
   C_SERIAL = 8,     // constraints placed in the serial group by the last step
   C_PENDING = 9,    // entries of the pending-colour list of this step
-  C_TICKET = 10,    // blocks finished in a sweep's last colour kernel (serial tail)
   C_EVENTS = 11,    // contact events recorded this step
   C_HEAVY = 12,     // over-full shapes whose pair segments need sorting (this broadphase)
   C_COUNT = 16
@@ -1093,48 +1092,22 @@ __device__ __forceinline__ void contact_velocity(const Dev& d, int s) {
   contact_velocity_t(d, s, GlobalBodies{d.vel});
 }
 
-// Serial tail: when a sweep's last colour kernel is launched with
-// serial_tail set, the block that finishes last runs the serial group on one
-// lane, so a captured step needs no separate one-lane launch per sweep. Every
-// thread publishes its writes (agent-scope fence) before the block takes a
-// ticket; the last ticket holder fences again before reading. All threads of
-// every block must call this.
-__device__ __forceinline__ bool serial_tail_block(const Dev& d) {
-  __shared__ int last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int t = atomicAdd(&d.cnt[C_TICKET], 1);
-    last = (t == (int)gridDim.x - 1) ? 1 : 0;
-    if (last) {
-      atomicExch(&d.cnt[C_TICKET], 0);
-      __threadfence();
-    }
-  }
-  __syncthreads();
-  return last != 0;
-}
-
 // Slots of a colour: given by the host when it knows them (one dependent
 // load less at the start of every sweep), else read from the offsets.
 __device__ __forceinline__ int2 colour_slots(const Dev& d, int colour, int2 given) {
   return given.x >= 0 ? given : make_int2(d.col_off[colour], d.col_off[colour + 1]);
 }
 
-__global__ void __launch_bounds__(kBlock) k_warm_start(Dev d, int colour, int2 slots, int serial_tail) {
+__global__ void __launch_bounds__(kBlock) k_warm_start(Dev d, int colour, int2 slots) {
   const int2 r = colour_slots(d, colour, slots);
   const int begin = r.x, end = r.y;
   B2G_FOR(s, begin, end) warm_start(d, s);
-  if (serial_tail && serial_tail_block(d) && threadIdx.x == 0)
-    for (int s = d.col_off[kSerialColour]; s < d.col_off[kSerialColour + 1]; ++s) warm_start(d, s);
 }
 
-__global__ void __launch_bounds__(kBlock) k_contact_velocity(Dev d, int colour, int2 slots, int serial_tail) {
+__global__ void __launch_bounds__(kBlock) k_contact_velocity(Dev d, int colour, int2 slots) {
   const int2 r = colour_slots(d, colour, slots);
   const int begin = r.x, end = r.y;
   B2G_FOR(s, begin, end) contact_velocity(d, s);
-  if (serial_tail && serial_tail_block(d) && threadIdx.x == 0)
-    for (int s = d.col_off[kSerialColour]; s < d.col_off[kSerialColour + 1]; ++s) contact_velocity(d, s);
 }
 
 // b2ContactSolver::StoreImpulses (:641-657): only the vc pointCount points
@@ -1305,8 +1278,7 @@ __device__ __forceinline__ void contact_position_serial(const Dev& d, int begin,
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_contact_position(Dev d, int colour, int2 slots, int it,
-                                                             int serial_tail) {
+__global__ void __launch_bounds__(kBlock) k_contact_position(Dev d, int colour, int2 slots, int it) {
   const int2 r = colour_slots(d, colour, slots);
   const int begin = r.x, end = r.y;
   if (blockDim.x == 1) {  // the serial colour group: one lane, slot order
@@ -1318,8 +1290,6 @@ __global__ void __launch_bounds__(kBlock) k_contact_position(Dev d, int colour, 
     const float m = valid ? contact_position(d, s, isl) : 0.0f;
     mark_island_sep(d, it, isl, valid != 0, m);
   }
-  if (serial_tail && serial_tail_block(d) && threadIdx.x == 0)
-    contact_position_serial(d, d.col_off[kSerialColour], d.col_off[kSerialColour + 1], it);
 }
 
 // b2ContactSolver::WarmStart as a gather: each dynamic body applies the
