@@ -240,16 +240,18 @@ class Batch:
         except Exception:
             pass
 
-    def advance(self, steps=1, dt=1.0 / 60.0, vi=8, pi=3, flags=0):
+    def advance(self, steps=1, dt=1.0 / 60.0, vi=8, pi=3, flags=0, warm=None):
         """Step every world `steps` times the way b2World::Step sequences
         them (b2_world.cpp:1108-1171): the first step of a fresh batch finds
-        contacts first and has dtRatio 0; later steps use inv_dt0 * dt."""
+        contacts first and has dtRatio 0; later steps use inv_dt0 * dt.
+        `warm`: per-world b2World::SetWarmStarting flags (default all on)."""
         dt = np.float32(dt)
         nw = len(self.gravity)
+        warm = [1] * nw if warm is None else [1 if w else 0 for w in warm]
         for _ in range(steps):
             inv_dt0 = getattr(self, "inv_dt0", np.float32(0.0))
             ratio = np.float32(inv_dt0 * dt)
-            self.set_world_params(0, [WorldParams(g, dt, ratio, 1, 1) for g in self.gravity])
+            self.set_world_params(0, [WorldParams(g, dt, ratio, warm[k], 1) for k, g in enumerate(self.gravity)])
             first = not getattr(self, "stepped", False)
             self.step(vi, pi, flags | (STEP_FIND_NEW_CONTACTS if first else 0))
             self.stepped = True

@@ -50,6 +50,36 @@ def test_gpu_batch_worlds_match_worlds_alone_bitexact(engine):
         assert same.all(), sc.name
 
 
+@pytest.mark.parametrize("solver", ["colours", "worlds"])
+def test_gpu_mixed_warm_starting_matches_worlds_alone_bitexact(engine, solver):
+    """Worlds with and without warm starting in one batch: the warm-start
+    pass (a per-body gather on the colour-major path, in-kernel on the
+    per-world path) reads each world's flag; each world still equals itself
+    stepped alone with the same flag, and the flag changes the result."""
+    scenes = _mixed_scenes()
+    if solver == "worlds":  # the per-world solver takes joint-free batches
+        scenes = [sc for sc in scenes if sc.joint_count == 0]
+    warm = [k % 2 == 0 for k in range(len(scenes))]
+    steps = 30
+    opt = engine.SOLVER_WORLDS if solver == "worlds" else engine.SOLVER_COLOURS
+    print("batch of", len(scenes), "worlds, solver", solver, "warm", warm, flush=True)
+    batch = engine.Batch.from_scenes(scenes)
+    batch.set_option(engine.OPT_SOLVER, opt)
+    got = batch.advance(steps, warm=warm).bodies()
+    changed = 0
+    for i, sc in enumerate(scenes):
+        alone = engine.Batch.from_scenes([sc]).advance(steps, warm=[warm[i]]).bodies()
+        part = got[batch.world_bodies[i]:batch.world_bodies[i + 1]]
+        same = _state(part) == _state(alone)
+        print(f"world {i} ({sc.name}, warm {warm[i]}): {same.all(axis=1).sum()} of {len(alone)} bit-identical",
+              flush=True)
+        assert same.all(), sc.name
+        if not warm[i]:
+            other = engine.Batch.from_scenes([sc]).advance(steps, warm=[True]).bodies()
+            changed += int(not (_state(other) == _state(alone)).all())
+    assert changed > 0  # switching warm starting off is visible
+
+
 def test_gpu_batch_world_matches_drop_in_world(engine):
     """A batch of one built by b2a_batch_from_scenes steps exactly like the
     drop-in b2World built from the same scene."""
