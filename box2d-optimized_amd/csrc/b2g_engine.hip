@@ -305,6 +305,11 @@ int b2g_batch::scan(const int* in, const int* n_dev, int n_host, int* out, int* 
   const int tiles = (int)std::max<long long>(1, (n_bound + kScanTile - 1) / kScanTile);
   const int g = std::min(tiles, 4096);
   hipLaunchKernelGGL(k_scan_tiles, dim3(g), dim3(kBlock), 0, stream, in, n_dev, n_host, d.scan_tiles);
+  if (tiles <= kScanFoldTiles) {  // the tile offsets are folded into the apply pass
+    hipLaunchKernelGGL(k_scan_apply_fold, dim3(g), dim3(kBlock), 0, stream, in, n_dev, n_host, d.scan_tiles, out,
+                       total);
+    return B2G_OK;
+  }
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, stream, d.scan_tiles, n_dev, n_host, total);
   hipLaunchKernelGGL(k_scan_apply, dim3(g), dim3(kBlock), 0, stream, in, n_dev, n_host, d.scan_tiles, out);
   return B2G_OK;
@@ -652,8 +657,11 @@ void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, c
     joint_sweep(k_joint_velocity);
     contact_sweep(B2G_KERNEL_CONTACT_VELOCITY, k_contact_velocity, k_sweep_tail<kTailVelocity>);
   }
-  hipLaunchKernelGGL(k_store_and_integrate, dim3(grid_for(std::max(est_cons, d.nb))), dim3(kBlock), 0, stream, d);
-  enqueue_islands_join();
+  // the island pass (side stream) ended long before the velocity phase did;
+  // the same launch stores impulses, integrates positions and assigns islands
+  (void)hipStreamWaitEvent(stream, join_ev, 0);
+  hipLaunchKernelGGL(k_store_and_integrate, dim3(grid_for(std::max(est_cons + d.nj, d.nb))), dim3(kBlock), 0, stream,
+                     d, 1);
   const int gf = grid_for(std::max(est_cons, 1) + d.nj);
   for (int it = 0; it < pi; ++it) {
     contact_sweep(B2G_KERNEL_CONTACT_POSITION, k_contact_position, k_sweep_tail<kTailPosition>, it);

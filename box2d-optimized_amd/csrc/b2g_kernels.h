@@ -738,6 +738,24 @@ __device__ __forceinline__ void island_reset(const Dev& d, int isl) {
   d.jok[isl] = 1;
 }
 
+// island of solved constraint slot k (k < n) or joint k - n, and that
+// island's position state reset
+__device__ __forceinline__ void island_assign(const Dev& d, int n, int k) {
+  int isl;
+  if (k < n) {
+    const float4 v2i = d.vc[2][k];
+    isl = island_of(d, as_int(v2i.x), as_int(v2i.y));
+    d.cisland[k] = isl;
+  } else {
+    const int q = k - n;
+    if (d.jisland[q] < 0) return;
+    const Joint& j = d.joints[d.jorder[q]];
+    isl = island_of(d, j.a, j.b);
+    d.jisland[q] = isl;
+  }
+  island_reset(d, isl);
+}
+
 // ===========================================================================
 // b2ContactSolver::Initialize + InitializeVelocityConstraints
 // (src/dynamics/b2_contact_solver.cpp:65-258); one lane per solved contact,
@@ -1163,12 +1181,16 @@ __device__ __forceinline__ void integrate_position(const Dev& d, int b) {
 
 // StoreImpulses for every solved contact and position integration for every
 // body, in one launch (the two touch disjoint data)
-__global__ void __launch_bounds__(kBlock) k_store_and_integrate(Dev d) {
+// after the velocity iterations: store impulses, integrate positions and
+// (islands_ready) give every constraint its island for the position phase
+__global__ void __launch_bounds__(kBlock) k_store_and_integrate(Dev d, int islands_ready) {
   const int n = d.cnt[C_CONS];
-  const int m = n > d.nb ? n : d.nb;
+  const int nc = n + (islands_ready ? d.nj : 0);
+  const int m = nc > d.nb ? nc : d.nb;
   B2G_FOR(k, 0, m) {
     if (k < n) store_impulses(d, k);
     if (k < d.nb) integrate_position(d, k);
+    if (islands_ready && k < nc) island_assign(d, n, k);
   }
 }
 
@@ -1451,7 +1473,7 @@ __global__ void __launch_bounds__(kTailThreads) k_sweep_tail(Dev d, TailSlots t,
 __global__ void __launch_bounds__(kBlock) k_joint_prepare(Dev d) {
   B2G_FOR(q, 0, d.nj) {
     const Joint& j = d.joints[d.jorder[q]];
-    // solved or not; the island itself is filled in by k_island_assign
+    // solved or not; the island itself is filled in when the islands are assigned
     const bool on = world_active(d, j.world) && (d.btype[j.a] != kStaticBody || d.btype[j.b] != kStaticBody);
     d.jisland[q] = on ? 0 : -1;
   }
@@ -1641,21 +1663,7 @@ __global__ void __launch_bounds__(kBlock) k_joint_position(Dev d, int begin, int
 // phase; the position phase is the first to need them).
 __global__ void __launch_bounds__(kBlock) k_island_assign(Dev d) {
   const int n = d.cnt[C_CONS];
-  B2G_FOR(k, 0, n + d.nj) {
-    int isl;
-    if (k < n) {
-      const float4 v2i = d.vc[2][k];
-      isl = island_of(d, as_int(v2i.x), as_int(v2i.y));
-      d.cisland[k] = isl;
-    } else {
-      const int q = k - n;
-      if (d.jisland[q] < 0) continue;
-      const Joint& j = d.joints[d.jorder[q]];
-      isl = island_of(d, j.a, j.b);
-      d.jisland[q] = isl;
-    }
-    island_reset(d, isl);
-  }
+  B2G_FOR(k, 0, n + d.nj) island_assign(d, n, k);
 }
 
 __global__ void __launch_bounds__(kBlock) k_island_finalize(Dev d, int it) {

@@ -119,6 +119,42 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(const int* in, const int*
   }
 }
 
+// passes 2 and 3 in one launch, for up to kScanFoldTiles tiles: each block
+// adds up the raw sums of the tiles before its own (a short reduction), so no
+// separate pass scans the tile sums; the block of the last tile writes the
+// grand total
+constexpr int kScanFoldTiles = 1024;
+__global__ void __launch_bounds__(kBlock) k_scan_apply_fold(const int* in, const int* __restrict__ n_dev,
+                                                            int n_host, const int* __restrict__ tile_sums, int* out,
+                                                            int* __restrict__ total_out) {
+  __shared__ int lds[kBlock / kWave];
+  const int n = n_dev ? *n_dev : n_host;
+  const int tiles = (n + kScanTile - 1) / kScanTile;
+  if (tiles == 0 && blockIdx.x == 0 && threadIdx.x == 0 && total_out) *total_out = 0;
+  for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
+    int part = 0;
+    for (int u = threadIdx.x; u < t; u += blockDim.x) part += tile_sums[u];
+    int offset;
+    (void)block_exclusive_scan(part, lds, &offset);
+    const int base = t * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      v[k] = base + k < n ? in[base + k] : 0;
+      s += v[k];
+    }
+    int total;
+    int ex = block_exclusive_scan(s, lds, &total) + offset;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      if (base + k < n) out[base + k] = ex;
+      ex += v[k];
+    }
+    if (t == tiles - 1 && threadIdx.x == 0 && total_out) *total_out = offset + total;
+  }
+}
+
 // ---- one-block exclusive scan for short arrays: each of the 1024 threads
 // owns a contiguous chunk (sum, block scan of the sums, rewrite); one launch
 // instead of three. Correct for any n, meant for n up to ~64k. In and out may
