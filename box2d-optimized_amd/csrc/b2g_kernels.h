@@ -365,10 +365,10 @@ __global__ void __launch_bounds__(kBlock) k_colour_init(Dev d, int K) {
   const int n = d.cnt[C_CONS];
   B2G_FOR(k, 0, n) {
     const int c = d.clist[k];
+    const int2 bp = d.cbody[k];  // the bodies, without the contact -> shape -> body chain
+    const int ba = bp.x >= 0 ? bp.x : ~bp.x, bb = bp.y >= 0 ? bp.y : ~bp.y;
     const int col = d.ccol[c];
-    if (col >= 0 && col < K && !d.wrecolour[d.shdr[d.cpair[c].x].w]) {
-      const int2 pr = d.cpair[c];
-      const int ba = d.shdr[pr.x].y, bb = d.shdr[pr.y].y;
+    if (col >= 0 && col < K && !d.wrecolour[d.bworld[ba]]) {
       const unsigned long long bit = 1ull << col;
       if (is_dyn(d, ba)) atomicOr(&d.bmask[ba], bit);
       if (is_dyn(d, bb)) atomicOr(&d.bmask[bb], bit);
@@ -450,6 +450,7 @@ __global__ void __launch_bounds__(kBlock) k_colour_count(Dev d) {
   for (int c = threadIdx.x; c < kColourSlots; c += blockDim.x) hist[c] = 0;
   __syncthreads();
   const int n = d.cnt[C_CONS];
+  const bool per_world = d.nw > 1;  // one world: k_colour_offsets reads its top colour off the histogram
   B2G_FOR_ALL(k, valid, 0, n) {
     const int col = valid ? d.ccolour[k] : -1;  // < 0: still uncoloured, the host runs more rounds
     int w = 0;
@@ -457,9 +458,9 @@ __global__ void __launch_bounds__(kBlock) k_colour_count(Dev d) {
       atomicAdd(&hist[col], 1);
       const int c = d.clist[k];
       d.ccol[c] = col;
-      w = d.shdr[d.cpair[c].x].w;
+      if (per_world) w = d.bworld[d.cbody[k].x >= 0 ? d.cbody[k].x : ~d.cbody[k].x];
     }
-    wave_atomic_max_int_keyed(d.wcolmax, w, col >= 0 && col < kMaxColours, col);
+    if (per_world) wave_atomic_max_int_keyed(d.wcolmax, w, col >= 0 && col < kMaxColours, col);
   }
   __syncthreads();
   for (int c = threadIdx.x; c < kColourSlots; c += blockDim.x)
@@ -529,6 +530,7 @@ __global__ void __launch_bounds__(kWave) k_colour_offsets(Dev d, int rounds_run)
     d.cnt[C_NCOLOURS] = serial > 0 ? kSerialColour + 1 : top;
     d.cnt[C_SERIAL] = serial;
     d.cnt[C_ROUND] += rounds_run;
+    if (d.nw == 1) d.wcolmax[0] = top - 1;  // the one world's highest normal colour
   }
 }
 
