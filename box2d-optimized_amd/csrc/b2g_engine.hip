@@ -192,8 +192,10 @@ struct b2g_batch {
   int radix_sort(int n, int bits);
   int build_tree(bool use_given_aabbs);
   int broadphase(bool use_given_aabbs);
+  void refresh_filters();
   bool tree_stale = true;     // bodies or shapes changed since the last tree build
   int refits_left = 0;        // steps that may refit the last tree before it is rebuilt
+  int static_shapes = 0;      // shapes of static bodies (their pair segments are sorted every broadphase)
   int tree_ns = -1;           // shape count of the last full build
   bool capturing = false;     // a step graph is being captured: full builds only
   void* qbuf = nullptr;       // scratch of the user queries
@@ -214,7 +216,7 @@ int b2g_batch::alloc() {
   A(d.btype, NB); A(d.bworld, NB);
   A(d.wpar, NW); A(d.wflag, NW); A(d.wshape0, NW); A(d.wshape1, NW); A(d.wbody, NW); A(d.worder, NW);
   A(d.wcol, (size_t)NW * (kColourSlots + 1));
-  A(d.wcolmax, NW); A(d.wfresh, NW); A(d.wrecolour, NW); A(d.wbounds, 4 * NW); A(d.wleaf, 2 * NW);
+  A(d.wcolmax, NW); A(d.wfresh, NW); A(d.wrecolour, NW); A(d.wbounds, 4 * NW); A(d.wleaf, 2 * NW); A(d.wroot, NW);
   A(d.shdr, NS); A(d.geo, 9 * (size_t)NS); A(d.smat, NS); A(d.sfilt, NS); A(d.aabb, NS);
   A(d.joints, NJ); A(d.jsol, NJ); A(d.jorder, NJ); A(d.jisland, NJ); A(d.wj_off, NW + 1); A(d.wj_list, NJ);
   A(d.bj_off, NB + 1); A(d.bj_other, 2 * NJ);
@@ -275,7 +277,7 @@ void b2g_batch::release() {
   qbuf = nullptr;
   void* ptrs[] = {d.xf, d.pos, d.vel, d.prop, d.prop2, d.force, d.btype, d.bworld, d.wpar, d.wflag, d.wshape0, d.wshape1, d.wbody, d.worder, d.wcol, d.wcolmax, d.wfresh,
                   d.wrecolour,
-                  d.wbounds, d.wleaf, d.shdr, d.geo, d.smat, d.sfilt, d.aabb, d.joints, d.jsol,
+                  d.wbounds, d.wleaf, d.wroot, d.shdr, d.geo, d.smat, d.sfilt, d.aabb, d.joints, d.jsol,
                   d.jorder, d.wj_off, d.wj_list, d.jisland, d.bj_off, d.bj_other, d.ckey, d.cpair, d.cman, d.cmat, d.ccol,
                   d.nkey, d.npair, d.nman, d.nmat, d.ncol, d.touch, d.tpos, d.clist, d.cbody, d.ccolour,
                   d.csorted, d.cisland, d.cpend, d.events, d.col_count, d.col_off, d.col_cursor, d.bmask, d.bslot,
@@ -383,6 +385,15 @@ int b2g_batch::build_tree(bool use_given_aabbs) {
   return B2G_OK;
 }
 
+// bodies, shapes or joints changed: packed pair-filter words, and the count
+// of static shapes (which sizes the pair-segment sort)
+void b2g_batch::refresh_filters() {
+  hipLaunchKernelGGL(k_shape_filters, dim3(grid_for(d.ns)), dim3(kBlock), 0, stream, d);
+  filters_dirty = false;
+  static_shapes = 0;
+  for (int i = 0; i < d.ns; ++i) static_shapes += h_btype[h_shape_body[i]] == B2G_STATIC ? 1 : 0;
+}
+
 int b2g_batch::broadphase(bool use_given_aabbs) {
   const int ns = d.ns;
   if (ns == 0) {
@@ -391,17 +402,17 @@ int b2g_batch::broadphase(bool use_given_aabbs) {
     return B2G_OK;
   }
   const int gs = grid_for(ns);
-  if (filters_dirty) {  // bodies, shapes or joints changed: packed pair-filter words
-    hipLaunchKernelGGL(k_shape_filters, dim3(gs), dim3(kBlock), 0, stream, d);
-    filters_dirty = false;
-  }
+  if (filters_dirty) refresh_filters();
   int rc = build_tree(use_given_aabbs);
   if (rc) return rc;
   const int gq = grid_for((long long)ns * kBlock / kQueryBlock);  // blocks of kQueryBlock threads
   launch(B2G_KERNEL_PAIRS_WRITE, ns, k_lbvh_pairs, dim3(gq), dim3(kQueryBlock), 0, d);
   scan(d.pcount, nullptr, ns, d.poff, d.cnt + C_PAIRS, ns);
   hipLaunchKernelGGL(k_pairs_compact, dim3(gq), dim3(kQueryBlock), 0, stream, d);
-  hipLaunchKernelGGL(k_sort_pair_segments, dim3(64), dim3(kBlock), 0, stream, d);
+  // one block per segment to sort: static shapes' segments, plus a few
+  // over-full dynamic shapes
+  hipLaunchKernelGGL(k_sort_pair_segments, dim3(std::min(std::max(static_shapes + 32, 64), 4096)), dim3(kBlock), 0,
+                     stream, d);
   hipLaunchKernelGGL(k_merge_contacts, dim3(grid_for(std::max(est_contacts, ns) * 5 / 4)), dim3(kBlock), 0, stream,
                      d);
   if (d.events_on)
@@ -955,10 +966,7 @@ int b2g_batch::graph_step(int vi, int pi, unsigned flags) {
   const unsigned key_flags = flags & (B2G_STEP_SKIP_BROADPHASE | B2G_STEP_EVENTS);
   const int block = use_per_world() ? world_block_now() * 2 + (world_lds_now() ? 1 : 0) : 0;
   last_path = block ? 2 : 1;
-  if (filters_dirty) {  // refreshed outside the captured sequence
-    hipLaunchKernelGGL(k_shape_filters, dim3(grid_for(d.ns)), dim3(kBlock), 0, stream, d);
-    filters_dirty = false;
-  }
+  if (filters_dirty) refresh_filters();  // outside the captured sequence
   GraphEntry* ge = nullptr;
   for (auto& g : graphs)
     if (g.parity == parity && g.vi == vi && g.pi == pi && g.K == graph_K && g.flags == key_flags &&

@@ -142,6 +142,7 @@ struct Dev {
   uint64_t* pslots;  // kPairSlots per shape: pairs found by the traversal, sorted
   int2* heavy;       // (first, count) of the pair segments of over-full shapes, to be sorted
   int* wleaf;        // 2 per world: first leaf, last leaf + 1
+  int* wroot;        // per world: the internal node whose leaves are exactly the world's (-1: one leaf)
   unsigned* wbounds; // 4 per world (ordered floats): min x, min y, max x, max y
   int* pcount;       // per shape: pairs whose lower shape it is (the pair list's segment length)
   int* pfound;       // per shape: pairs its traversal found (own + those of a static partner);
@@ -2086,7 +2087,10 @@ __global__ void __launch_bounds__(kBlock) k_world_leaves(Dev d, const uint64_t* 
   const int n = d.ns;
   B2G_FOR(k, 0, n) {
     const int w = (int)(keys[k] >> 32);
-    if (k == 0 || (int)(keys[k - 1] >> 32) != w) d.wleaf[2 * w] = k;
+    if (k == 0 || (int)(keys[k - 1] >> 32) != w) {
+      d.wleaf[2 * w] = k;
+      d.wroot[w] = -1;  // k_node_records finds the world's subtree
+    }
     if (k == n - 1 || (int)(keys[k + 1] >> 32) != w) d.wleaf[2 * w + 1] = k + 1;
   }
 }
@@ -2169,6 +2173,10 @@ __global__ void __launch_bounds__(kBlock) k_node_records(Dev d, const uint64_t* 
     o[0] = lb;
     o[1] = rb;
     o[2] = make_float4(__int_as_float(lcode), __int_as_float(rcode), __int_as_float(split), __int_as_float(sflags));
+    // keys sharing the world prefix form a subtree of the radix tree: its
+    // root is where that world's queries start
+    const int w = (int)(keys[r.x] >> 32);
+    if (d.wleaf[2 * w] == r.x && d.wleaf[2 * w + 1] == r.y + 1) d.wroot[w] = i;
   }
 }
 
@@ -2240,8 +2248,10 @@ __device__ __forceinline__ void lbvh_query(const Dev& d, int i, int* stk, Emit e
   const int w = d.shdr[i].w;
   const int wlo = d.wleaf[2 * w], whi = d.wleaf[2 * w + 1] - 1;
   const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
+  const int root = d.wroot[w];
+  if (root < 0) return;  // the world has no other shape
   int sp = 0;
-  stk[tid] = 0;
+  stk[tid] = root;  // the world's own subtree: no walk through nodes spanning worlds
   sp = 1;
   int visits = 0;
   // test both children of one node record; leaves are emitted, overlapping
