@@ -158,6 +158,7 @@ struct b2g_batch {
   bool worlds_contiguous = false;  // bodies and shapes laid out world after world
   int graph_tail_from = 0;         // first colour of the shared tail launch (from the last eager step)
   int max_world_bodies = 0;
+  int max_world_shapes = 0;
   int solver_mode = 0;       // B2G_SOLVER_AUTO / _COLOURS / _WORLDS
   int world_block = 0;       // threads per world workgroup (0: autotune)
   int tuned_block = 0;       // chosen by the autotuner (or world_block)
@@ -372,14 +373,17 @@ int b2g_batch::build_tree(bool use_given_aabbs) {
   }
   hipLaunchKernelGGL(k_leaf_aabbs, dim3(gs), dim3(kBlock), 0, stream, d, d.mval[0]);
   if (ns >= 2) {
+    // range unions are only looked up inside one world (a range spanning
+    // worlds gets an unbounded box first), so the table needs the levels of
+    // the largest world only
+    const int span = (!layout_dirty && max_world_shapes > 0) ? std::min(ns, std::max(max_world_shapes, 2)) : ns;
     int top = 0;
-    while ((2 << top) <= ns) ++top;  // floor(log2(ns))
-    for (int j = 1; j <= top; j += 3) {
-      Dev dj = d;
-      dj.sparse_levels = top + 1;
+    while ((2 << top) <= span) ++top;  // floor(log2(span))
+    Dev dj = d;
+    dj.sparse_levels = top + 1;  // levels built this time
+    for (int j = 1; j <= top; j += 3)
       hipLaunchKernelGGL(k_sparse_levels, dim3(gs), dim3(kBlock), 0, stream, dj, j);
-    }
-    hipLaunchKernelGGL(k_node_records, dim3(grid_for(ns - 1)), dim3(kBlock), 0, stream, d, d.mkey[0], d.mval[0]);
+    hipLaunchKernelGGL(k_node_records, dim3(grid_for(ns - 1)), dim3(kBlock), 0, stream, dj, d.mkey[0], d.mval[0]);
   }
   tree_stale = false;
   return B2G_OK;
@@ -714,12 +718,14 @@ int b2g_batch::update_world_layout() {
   std::vector<int2> wb(nw);
   std::vector<int> ws1(nw), order(nw);
   max_world_bodies = 0;
+  max_world_shapes = 0;
   for (int w = 0; w < nw; ++w) {
     if (bc[w] && b1[w] - b0[w] + 1 != bc[w]) ok = false;
     if (sc[w] && s1[w] - s0[w] + 1 != sc[w]) ok = false;
     wb[w] = bc[w] ? make_int2(b0[w], b1[w] + 1) : make_int2(0, 0);
     ws1[w] = sc[w] ? s1[w] + 1 : h_wshape0[w];
     max_world_bodies = std::max(max_world_bodies, bc[w]);
+    max_world_shapes = std::max(max_world_shapes, sc[w]);
     order[w] = w;
   }
   // largest worlds first: their workgroups start before the small ones fill in

@@ -2152,6 +2152,7 @@ __device__ __forceinline__ float4 range_box(const Dev& d, const uint64_t* keys, 
   if ((keys[lo] >> 32) != (keys[hi] >> 32)) return make_float4(-FLT_MAX, -FLT_MAX, FLT_MAX, FLT_MAX);
   const int len = hi - lo + 1;
   const int lev = 31 - __clz(len);
+  if (lev >= d.sparse_levels) return make_float4(-FLT_MAX, -FLT_MAX, FLT_MAX, FLT_MAX);  // not built: no bound
   const float4* t = d.sparse + (size_t)lev * n;
   return aabb_union(t[lo], t[hi - (1 << lev) + 1]);
 }
