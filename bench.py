@@ -171,6 +171,9 @@ def batched_section(pkg, worlds: int, warmup: int, steps: int, sample_every: int
         "ms_per_step": el / steps * 1e3,
         "worlds": worlds, "dynamic_bodies": nbodies, "steps": steps, "warmup": warmup,
         "contacts": counts.contacts, "touching": counts.touching, "colours": counts.colours,
+        "solver": {"path": "per-world workgroups" if batch.get_option(eng.OPT_LAST_SOLVER) == 2 else "colour sweeps",
+                   "tuned_workgroup": batch.get_option(eng.OPT_TUNED_WORLD_BLOCK),
+                   "tuned_lds_bodies": bool(batch.get_option(eng.OPT_TUNED_WORLD_LDS))},
         "build_s": build_s,
         "per_world_loop": {"value": loop_rate, "unit": "world-steps/s",
                            "sample": f"{len(sample)} worlds (every {sample_every}th), {loop_steps} steps each, "

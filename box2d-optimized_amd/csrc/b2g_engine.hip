@@ -159,6 +159,8 @@ struct b2g_batch {
   int graph_tail_from = 0;         // first colour of the shared tail launch (from the last eager step)
   int max_world_bodies = 0;
   int max_world_shapes = 0;
+  int big_worlds = 0;              // worlds of more than kBigWorldBodies bodies (first in worder)
+  int max_small_world_bodies = 0;  // largest of the others
   int solver_mode = 0;       // B2G_SOLVER_AUTO / _COLOURS / _WORLDS
   int world_block = 0;       // threads per world workgroup (0: autotune)
   int tuned_block = 0;       // chosen by the autotuner (or world_block)
@@ -698,6 +700,10 @@ int b2g_batch::check_status() {
 // World ranges for the per-world solver: valid when every world's bodies and
 // shapes are contiguous (the layout b2a_batch_from_scenes produces) and the
 // batch has no joints.
+// worlds above this many bodies get a solver launch of their own (see
+// launch_solve_worlds)
+constexpr int kBigWorldBodies = 1024;
+
 int b2g_batch::update_world_layout() {
   const int nw = std::max(d.nw, 1), nb = d.nb, ns = d.ns;
   std::vector<int> b0(nw, INT_MAX), b1(nw, -1), bc(nw, 0), s0(nw, INT_MAX), s1(nw, -1), sc(nw, 0);
@@ -730,6 +736,12 @@ int b2g_batch::update_world_layout() {
   }
   // largest worlds first: their workgroups start before the small ones fill in
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return bc[a] > bc[b]; });
+  big_worlds = 0;
+  max_small_world_bodies = 0;
+  for (int w = 0; w < nw; ++w) {
+    if (bc[w] > kBigWorldBodies) ++big_worlds;
+    else max_small_world_bodies = std::max(max_small_world_bodies, bc[w]);
+  }
   HIP_TRY(hipMemcpy(d.wbody, wb.data(), nw * sizeof(int2), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(d.wshape1, ws1.data(), nw * sizeof(int), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(d.worder, order.data(), nw * sizeof(int), hipMemcpyHostToDevice));
@@ -765,13 +777,34 @@ bool b2g_batch::world_lds_now() const {
   return std::min(tune_step, 2 * kWorldBlockCount - 1) >= kWorldBlockCount;
 }
 
+// Worlds of more than kBigWorldBodies bodies (the first ones in worder) get
+// full 1024-thread workgroups in a launch of their own, beside the launch of
+// the other worlds at the tuned workgroup size (on the side stream): one
+// large world on a small workgroup would otherwise set the whole batch's
+// solve time. Every world runs the same kernel either way.
+
 void b2g_batch::launch_solve_worlds(int vi, int pi) {
   const int wblock = world_block_now();
-  if (world_lds_now())
-    launch(B2G_KERNEL_SOLVER, est_cons, k_solve_worlds<true>, dim3(d.nw), dim3(wblock),
-           (uint32_t)(sizeof(float4) * (size_t)max_world_bodies), d, vi, pi);
-  else
-    launch(B2G_KERNEL_SOLVER, est_cons, k_solve_worlds<false>, dim3(d.nw), dim3(wblock), 0, d, vi, pi);
+  const bool lds = world_lds_now();
+  auto part = [&](hipStream_t st, int first, int count, int block, int bodies) {
+    if (lds)
+      hipLaunchKernelGGL(k_solve_worlds<true>, dim3(count), dim3(block), sizeof(float4) * (size_t)bodies, st, d, vi,
+                         pi, first);
+    else
+      hipLaunchKernelGGL(k_solve_worlds<false>, dim3(count), dim3(block), 0, st, d, vi, pi, first);
+  };
+  tbegin(B2G_KERNEL_SOLVER);
+  if (big_worlds > 0 && big_worlds < d.nw && wblock < 1024) {
+    (void)hipEventRecord(fork_ev, stream);
+    (void)hipStreamWaitEvent(side, fork_ev, 0);
+    part(side, big_worlds, d.nw - big_worlds, wblock, max_small_world_bodies);
+    part(stream, 0, big_worlds, 1024, max_world_bodies);
+    (void)hipEventRecord(join_ev, side);
+    (void)hipStreamWaitEvent(stream, join_ev, 0);
+  } else {
+    part(stream, 0, d.nw, wblock, max_world_bodies);
+  }
+  tend(B2G_KERNEL_SOLVER, est_cons);
 }
 
 int b2g_batch::step(int vi, int pi, unsigned flags) {

@@ -1786,10 +1786,10 @@ __global__ void __launch_bounds__(kBlock) k_world_colour_sort(Dev d, int rounds_
 // (dynamic shared memory of 16 bytes per body of the largest world); the
 // arithmetic is the same either way.
 template <bool kLds>
-__global__ void __launch_bounds__(1024) k_solve_worlds(Dev d, int vi, int pi) {
+__global__ void __launch_bounds__(1024) k_solve_worlds(Dev d, int vi, int pi, int first) {
   __shared__ int off[kColourSlots + 1];
   extern __shared__ float4 s_body[];
-  const int w = d.worder[blockIdx.x];
+  const int w = d.worder[first + blockIdx.x];  // worlds first .. first + gridDim.x - 1, largest first
   if (!world_active(d, w)) return;  // whole block
   for (int c = threadIdx.x; c <= kColourSlots; c += blockDim.x) off[c] = d.wcol[w * (kColourSlots + 1) + c];
   __syncthreads();
