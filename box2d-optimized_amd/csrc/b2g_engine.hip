@@ -229,7 +229,7 @@ int b2g_batch::alloc() {
   A(d.cisland, NC); A(d.cpend, NC); A(d.events, (size_t)kEventWords * NC);
   for (int k = 0; k < 10; ++k) A(d.vc[k], NC);
   for (int k = 0; k < 5; ++k) A(d.pc[k], NC);
-  A(d.col_count, kColourSlots + 1); A(d.col_off, kColourSlots + 1); A(d.col_cursor, kColourSlots + 1);
+  A(d.col_count, kColourSlots + 1); A(d.col_cursor, kColourSlots + 1);
   A(d.bmask, NB); A(d.bslot, NB);
   A(d.btab, (size_t)kBodyTab * NB); A(d.parent, NB); A(d.sepbad, 2 * (size_t)NB); A(d.jok, 2 * (size_t)NB); A(d.solved, NB);
   A(d.mkey[0], NS); A(d.mkey[1], NS); A(d.mval[0], NS); A(d.mval[1], NS);
@@ -247,7 +247,10 @@ int b2g_batch::alloc() {
   A(d.pcount, NS); A(d.pfound, NS); A(d.poff, NS);
   const size_t scan_n = std::max<size_t>(std::max<size_t>(NC, NS), kRadix * tiles);
   A(d.scan_tiles, scan_n / kScanTile + 2);
-  A(d.cnt, C_COUNT);
+  // the counters and, behind them, the colour offsets: the step's one
+  // read-back is a single copy
+  A(d.cnt, kCntColOff + kColourSlots + 1);
+  d.col_off = d.cnt + kCntColOff;
 #undef A
   d.ccap = NC;
   d.ecap = NC;
@@ -283,7 +286,7 @@ void b2g_batch::release() {
                   d.wbounds, d.wleaf, d.wroot, d.shdr, d.geo, d.smat, d.sfilt, d.aabb, d.joints, d.jsol,
                   d.jorder, d.wj_off, d.wj_list, d.jisland, d.bj_off, d.bj_other, d.ckey, d.cpair, d.cman, d.cmat, d.ccol,
                   d.nkey, d.npair, d.nman, d.nmat, d.ncol, d.touch, d.tpos, d.clist, d.cbody, d.ccolour,
-                  d.csorted, d.cisland, d.cpend, d.events, d.col_count, d.col_off, d.col_cursor, d.bmask, d.bslot,
+                  d.csorted, d.cisland, d.cpend, d.events, d.col_count, d.col_cursor, d.bmask, d.bslot,
                   d.btab, d.parent, d.sepbad, d.jok, d.solved, d.mkey[0], d.mkey[1], d.mval[0], d.mval[1],
                   d.sort_hist, d.node_child, d.node_parent, d.leaf_parent, d.node_range, d.node_rec,
                   d.sfast, d.sparse, d.pslots, d.heavy, d.pcount, d.pfound, d.poff, d.scan_tiles, d.cnt};
@@ -493,7 +496,8 @@ int b2g_batch::rebuild_joint_tables() {
 }
 
 int b2g_batch::read_counters() {
-  HIP_TRY(hipMemcpyAsync(pinned, d.cnt, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipMemcpyAsync(pinned, d.cnt, (kCntColOff + kColourSlots + 1) * sizeof(int), hipMemcpyDeviceToHost,
+                         stream));
   HIP_TRY(hipStreamSynchronize(stream));
   return B2G_OK;
 }
@@ -525,11 +529,20 @@ void b2g_batch::enqueue_collide_and_colour(int K, int rounds) {
   const int gc = grid_for(std::max(est_contacts, 256) * 5 / 4);
   const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
   const int gb = grid_for(nb);
-  launch(B2G_KERNEL_NARROWPHASE, est_contacts, k_narrowphase, dim3(gc), dim3(kBlock), 0, d);
-  // the bound only picks the scan's launch shape; any device count is handled
-  scan(d.touch, d.cnt + C_CONTACTS, 0, d.tpos, d.cnt + C_CONS, std::max(est_contacts, 1) * 5LL / 4);
-  hipLaunchKernelGGL(k_compact_touching, dim3(gc), dim3(kBlock), 0, stream, d);
-  hipLaunchKernelGGL(k_integrate_velocities, dim3(gb), dim3(kBlock), 0, stream, d);
+  // narrowphase, and the bodies' velocity integration in the same launch
+  launch(B2G_KERNEL_NARROWPHASE, est_contacts, k_narrowphase, dim3(std::max(gc, gb)), dim3(kBlock), 0, d);
+  // solved contacts: scan of the touching flags and the compacted list (the
+  // bound only picks the launch shape; any device count is handled)
+  const long long touch_bound = std::max(est_contacts, 1) * 5LL / 4;
+  const int touch_tiles = (int)((touch_bound + kScanTile - 1) / kScanTile);
+  if (touch_bound > 8192 && touch_tiles <= kScanFoldTiles) {
+    const int g = std::min(touch_tiles, 4096);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(g), dim3(kBlock), 0, stream, d.touch, d.cnt + C_CONTACTS, 0, d.scan_tiles);
+    hipLaunchKernelGGL(k_touch_scan_apply, dim3(g), dim3(kBlock), 0, stream, d, d.scan_tiles);
+  } else {
+    scan(d.touch, d.cnt + C_CONTACTS, 0, d.tpos, d.cnt + C_CONS, touch_bound);
+    hipLaunchKernelGGL(k_compact_touching, dim3(gc), dim3(kBlock), 0, stream, d);
+  }
   hipLaunchKernelGGL(k_colour_init, dim3(gk), dim3(kBlock), 0, stream, d, K);
   for (int r = 0; r < rounds; ++r) enqueue_colour_round(K);
 }
@@ -546,10 +559,9 @@ void b2g_batch::enqueue_colour_round(int K) {
 void b2g_batch::enqueue_colour_groups() {
   const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
   hipLaunchKernelGGL(k_colour_count, dim3(gk), dim3(kBlock), 0, stream, d);
-  hipLaunchKernelGGL(k_colour_offsets, dim3(1), dim3(kWave), 0, stream, d, rounds_pending);
+  hipLaunchKernelGGL(k_colour_scatter, dim3(gk), dim3(kBlock), 0, stream, d, rounds_pending);
   round_host += (unsigned)rounds_pending;
   rounds_pending = 0;
-  hipLaunchKernelGGL(k_colour_scatter, dim3(gk), dim3(kBlock), 0, stream, d);
   hipLaunchKernelGGL(k_sort_serial_group, dim3(1), dim3(1), 0, stream, d);
 }
 
@@ -907,7 +919,6 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
   last_path = 1;
   enqueue_colour_groups();
   enqueue_islands_and_init();
-  HIP_TRY(hipMemcpyAsync(pinned + 32, d.col_off, (kColourSlots + 1) * sizeof(int), hipMemcpyDeviceToHost, stream));
   if ((rc = read_counters())) return rc;
   est_cons = pinned[C_CONS];
   for (int extra = 0; pinned[C_REMAIN] > 0; extra += 2) {  // first steps / bursts of new contacts
@@ -919,12 +930,10 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
       hipLaunchKernelGGL(k_colour_clear_counts, dim3(1), dim3(kBlock), 0, stream, d);
       enqueue_colour_groups();
       hipLaunchKernelGGL(k_contact_init, dim3(grid_for(std::max(est_cons, 256) * 5 / 4)), dim3(kBlock), 0, stream, d);
-      HIP_TRY(hipMemcpyAsync(pinned + 32, d.col_off, (kColourSlots + 1) * sizeof(int), hipMemcpyDeviceToHost,
-                             stream));
       if ((rc = read_counters())) return rc;
     }
   }
-  const int* off = pinned + 32;
+  const int* off = pinned + kCntColOff;
   std::vector<int> colours, sizes, begins;
   int normal_colours = 0;
   for (int c = 0; c < kColourSlots; ++c) {

@@ -59,7 +59,8 @@ enum BodyType : int { kStaticBody = 0, kKinematicBody = 1, kDynamicBody = 2 };
 constexpr int kMaxColours = 64;           // bit-mask colours; colour 64 = serial group
 constexpr int kSerialColour = kMaxColours;
 constexpr int kColourSlots = kMaxColours + 1;
-constexpr int kBodyTab = 32;  // colours a body's warm-start gather covers
+constexpr int kBodyTab = 32;
+constexpr int kCntColOff = 32;  // the colour offsets sit this many ints into the counter block  // colours a body's warm-start gather covers
 
 // all device arrays, passed by value to every kernel
 struct Dev {
@@ -243,10 +244,51 @@ __global__ void k_events_reset(Dev d) {
 }
 
 // ===========================================================================
+// Velocity integration (b2_island.cpp:257-298, SolveOrphan :187-206)
+// ===========================================================================
+// the step's colour counters (block 0 of the launch that integrates velocities)
+__device__ __forceinline__ void reset_colour_counters(const Dev& d) {
+  for (int c = threadIdx.x; c < kColourSlots; c += blockDim.x) d.col_count[c] = 0;
+  if (threadIdx.x == 0) {
+    d.cnt[C_REMAIN] = 0;
+    d.cnt[C_PENDING] = 0;
+    d.cnt[C_NCOLOURS] = 0;
+    d.cnt[C_SERIAL] = 0;
+  }
+}
+__device__ __forceinline__ void integrate_velocity(const Dev& d, int b) {
+  {
+    d.parent[b] = b;
+    d.bmask[b] = 0ull;
+    const int w = d.bworld[b];
+    if (!world_active(d, w) || d.btype[b] != kDynamicBody) return;
+    const float4 wp = d.wpar[w];
+    const float h = wp.z;
+    const float4 v4 = d.vel[b], p = d.prop[b], p2 = d.prop2[b], f = d.force[b];
+    // v += h * invMass * (gravityScale * mass * gravity + force)
+    const float s1 = h * p.x;
+    const float s2 = p2.y * p2.x;
+    float vx = v4.x + s1 * (s2 * wp.x + f.x);
+    float vy = v4.y + s1 * (s2 * wp.y + f.y);
+    float w_ = v4.z + h * p.y * f.z;
+    const float ld = 1.0f + h * p2.z;
+    const float ad = 1.0f + h * p2.w;
+    vx = vx / ld;
+    vy = vy / ld;
+    w_ = w_ / ad;
+    d.vel[b] = make_float4(vx, vy, w_, 0.0f);
+  }
+}
+
+// ===========================================================================
 // Narrowphase: b2ContactManager::Collide + b2Contact::Update
 // (src/dynamics/b2_contact_manager.cpp:66-116, src/dynamics/b2_contact.cpp:126-210)
 // ===========================================================================
+// The same launch integrates the bodies' velocities (b2Island::Solve,
+// b2_island.cpp:257-298), which touches none of the contact data, and clears
+// the step's colour counters.
 __global__ void __launch_bounds__(kBlock) k_narrowphase(Dev d) {
+  if (blockIdx.x == 0) reset_colour_counters(d);
   const int n = d.cnt[C_CONTACTS];
   B2G_FOR(i, 0, n) {
     const int2 pr = d.cpair[i];
@@ -300,55 +342,55 @@ __global__ void __launch_bounds__(kBlock) k_narrowphase(Dev d) {
     }
     d.touch[i] = solve;
   }
+  B2G_FOR(b, 0, d.nb) integrate_velocity(d, b);
 }
 
 // compacted list of solved contacts (order preserved)
+__device__ __forceinline__ void compact_touching(const Dev& d, int i, int t) {
+  d.clist[t] = i;
+  const int2 pr = d.cpair[i];
+  const int ba = d.shdr[pr.x].y, bb = d.shdr[pr.y].y;
+  d.cbody[t] = make_int2(d.btype[ba] == kStaticBody ? ~ba : ba, d.btype[bb] == kStaticBody ? ~bb : bb);
+}
 __global__ void __launch_bounds__(kBlock) k_compact_touching(Dev d) {
   const int n = d.cnt[C_CONTACTS];
   B2G_FOR(i, 0, n) {
-    if (d.touch[i]) {
-      const int t = d.tpos[i];
-      d.clist[t] = i;
-      const int2 pr = d.cpair[i];
-      const int ba = d.shdr[pr.x].y, bb = d.shdr[pr.y].y;
-      d.cbody[t] = make_int2(d.btype[ba] == kStaticBody ? ~ba : ba, d.btype[bb] == kStaticBody ? ~bb : bb);
-    }
+    if (d.touch[i]) compact_touching(d, i, d.tpos[i]);
   }
 }
 
-// ===========================================================================
-// Velocity integration (b2_island.cpp:257-298, SolveOrphan :187-206)
-// ===========================================================================
-__global__ void __launch_bounds__(kBlock) k_integrate_velocities(Dev d) {
-  if (blockIdx.x == 0) {  // colour counters of this step
-    for (int c = threadIdx.x; c < kColourSlots; c += blockDim.x) d.col_count[c] = 0;
-    if (threadIdx.x == 0) {
-      d.cnt[C_REMAIN] = 0;
-      d.cnt[C_PENDING] = 0;
-      d.cnt[C_NCOLOURS] = 0;
-      d.cnt[C_SERIAL] = 0;
+// The scan's apply pass over the touching flags (tile offsets folded in, as
+// k_scan_apply_fold) with the compaction in the same launch: a solved
+// contact's place in the compacted list is its own scan value.
+__global__ void __launch_bounds__(kBlock) k_touch_scan_apply(Dev d, const int* __restrict__ tile_sums) {
+  __shared__ int lds[kBlock / kWave];
+  const int n = d.cnt[C_CONTACTS];
+  const int tiles = (n + kScanTile - 1) / kScanTile;
+  if (tiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) d.cnt[C_CONS] = 0;
+  for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
+    int part = 0;
+    for (int u = threadIdx.x; u < t; u += blockDim.x) part += tile_sums[u];
+    int offset;
+    (void)block_exclusive_scan(part, lds, &offset);
+    const int base = t * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      v[k] = base + k < n ? d.touch[base + k] : 0;
+      s += v[k];
     }
-  }
-  B2G_FOR(b, 0, d.nb) {
-    d.parent[b] = b;
-    d.bmask[b] = 0ull;
-    const int w = d.bworld[b];
-    if (!world_active(d, w) || d.btype[b] != kDynamicBody) continue;
-    const float4 wp = d.wpar[w];
-    const float h = wp.z;
-    const float4 v4 = d.vel[b], p = d.prop[b], p2 = d.prop2[b], f = d.force[b];
-    // v += h * invMass * (gravityScale * mass * gravity + force)
-    const float s1 = h * p.x;
-    const float s2 = p2.y * p2.x;
-    float vx = v4.x + s1 * (s2 * wp.x + f.x);
-    float vy = v4.y + s1 * (s2 * wp.y + f.y);
-    float w_ = v4.z + h * p.y * f.z;
-    const float ld = 1.0f + h * p2.z;
-    const float ad = 1.0f + h * p2.w;
-    vx = vx / ld;
-    vy = vy / ld;
-    w_ = w_ / ad;
-    d.vel[b] = make_float4(vx, vy, w_, 0.0f);
+    int total;
+    int ex = block_exclusive_scan(s, lds, &total) + offset;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      if (base + k < n) {
+        d.tpos[base + k] = ex;
+        if (v[k]) compact_touching(d, base + k, ex);
+      }
+      ex += v[k];
+    }
+    if (t == tiles - 1 && threadIdx.x == 0) d.cnt[C_CONS] = offset + total;
   }
 }
 
@@ -449,9 +491,11 @@ __global__ void __launch_bounds__(kBlock) k_colour_resolve(Dev d, int local_roun
 __global__ void __launch_bounds__(kBlock) k_colour_count(Dev d) {
   __shared__ int hist[kColourSlots];
   for (int c = threadIdx.x; c < kColourSlots; c += blockDim.x) hist[c] = 0;
+  if (blockIdx.x == 0)  // k_colour_scatter's placement cursors start at zero
+    for (int c = threadIdx.x; c < kColourSlots; c += blockDim.x) d.col_cursor[c] = 0;
   __syncthreads();
   const int n = d.cnt[C_CONS];
-  const bool per_world = d.nw > 1;  // one world: k_colour_offsets reads its top colour off the histogram
+  const bool per_world = d.nw > 1;  // one world: k_colour_scatter reads its top colour off the histogram
   B2G_FOR_ALL(k, valid, 0, n) {
     const int col = valid ? d.ccolour[k] : -1;  // < 0: still uncoloured, the host runs more rounds
     int w = 0;
@@ -515,32 +559,42 @@ __global__ void __launch_bounds__(kBlock) k_colour_leftover(Dev d) {
 // one wave: offsets of the colour groups (lane c scans colour c, lane 0 also
 // the serial slot); resets the scatter cursors and advances the claim-round
 // base past the rounds just run
-__global__ void __launch_bounds__(kWave) k_colour_offsets(Dev d, int rounds_run) {
-  static_assert(kColourSlots == kWave + 1, "one lane per normal colour plus the serial slot");
-  const int lane = (int)threadIdx.x;
-  const int cnt = d.col_count[lane];
-  const int inc = wave_inclusive_scan(cnt);
-  d.col_off[lane] = inc - cnt;
-  d.col_cursor[lane] = 0;
-  const int top = __popcll(__ballot(cnt > 0)) ? 64 - __clzll((long long)__ballot(cnt > 0)) : 0;
-  if (lane == kWave - 1) {
-    const int serial = d.col_count[kSerialColour];
-    d.col_off[kSerialColour] = inc;
-    d.col_off[kSerialColour + 1] = inc + serial;
-    d.col_cursor[kSerialColour] = 0;
-    d.cnt[C_NCOLOURS] = serial > 0 ? kSerialColour + 1 : top;
-    d.cnt[C_SERIAL] = serial;
-    d.cnt[C_ROUND] += rounds_run;
-    if (d.nw == 1) d.wcolmax[0] = top - 1;  // the one world's highest normal colour
-  }
-}
 
 // Each block reserves a contiguous range per colour with one atomic, then
 // places its constraints inside it (order inside a colour does not matter:
-// those constraints share no dynamic body).
-__global__ void __launch_bounds__(kBlock) k_colour_scatter(Dev d) {
+// those constraints share no dynamic body). Every block first scans the
+// colour counts into offsets itself (one wave, 65 entries); block 0 also
+// publishes them with the colour counters.
+__global__ void __launch_bounds__(kBlock) k_colour_scatter(Dev d, int rounds_run) {
+  static_assert(kColourSlots == kWave + 1, "one lane per normal colour plus the serial slot");
   __shared__ int hist[kColourSlots];
   __shared__ int base[kColourSlots];
+  __shared__ int off[kColourSlots + 1];
+  if (threadIdx.x < kWave) {
+    const int lane = (int)threadIdx.x;
+    const int cnt = d.col_count[lane];
+    const int inc = wave_inclusive_scan(cnt);
+    off[lane] = inc - cnt;
+    const uint64_t used = __ballot(cnt > 0);
+    const int top = used ? 64 - __clzll((long long)used) : 0;
+    const int serial = d.col_count[kSerialColour];
+    if (lane == kWave - 1) {
+      off[kSerialColour] = inc;
+      off[kSerialColour + 1] = inc + serial;
+    }
+    if (blockIdx.x == 0) {
+      d.col_off[lane] = inc - cnt;
+      if (lane == kWave - 1) {
+        d.col_off[kSerialColour] = inc;
+        d.col_off[kSerialColour + 1] = inc + serial;
+        d.cnt[C_NCOLOURS] = serial > 0 ? kSerialColour + 1 : top;
+        d.cnt[C_SERIAL] = serial;
+        d.cnt[C_ROUND] += rounds_run;
+        if (d.nw == 1) d.wcolmax[0] = top - 1;  // the one world's highest normal colour
+      }
+    }
+  }
+  __syncthreads();
   const int n = d.cnt[C_CONS];
   for (int start = xcd_block() * kBlock; start < n; start += gridDim.x * kBlock) {
     for (int c = threadIdx.x; c < kColourSlots; c += blockDim.x) hist[c] = 0;
@@ -550,7 +604,7 @@ __global__ void __launch_bounds__(kBlock) k_colour_scatter(Dev d) {
     const int local = col >= 0 ? atomicAdd(&hist[col], 1) : 0;
     __syncthreads();
     for (int c = threadIdx.x; c < kColourSlots; c += blockDim.x)
-      base[c] = hist[c] ? d.col_off[c] + atomicAdd(&d.col_cursor[c], hist[c]) : 0;
+      base[c] = hist[c] ? off[c] + atomicAdd(&d.col_cursor[c], hist[c]) : 0;
     __syncthreads();
     if (col >= 0) d.csorted[base[col] + local] = d.clist[k];
     __syncthreads();
