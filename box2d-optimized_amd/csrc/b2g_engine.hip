@@ -244,7 +244,7 @@ int b2g_batch::alloc() {
     A(d.sparse, (size_t)levels * NS);
   }
   A(d.pslots, (size_t)kPairSlots * NS); A(d.heavy, NS);
-  A(d.pcount, NS); A(d.pfound, NS); A(d.poff, NS);
+  A(d.pcount, NS); A(d.pfound, NS); A(d.sinv, NS); A(d.poff, NS);
   const size_t scan_n = std::max<size_t>(std::max<size_t>(NC, NS), kRadix * tiles);
   A(d.scan_tiles, scan_n / kScanTile + 2);
   // the counters and, behind them, the colour offsets: the step's one
@@ -289,7 +289,7 @@ void b2g_batch::release() {
                   d.csorted, d.cisland, d.cpend, d.events, d.col_count, d.col_cursor, d.bmask, d.bslot,
                   d.btab, d.parent, d.sepbad, d.jok, d.solved, d.mkey[0], d.mkey[1], d.mval[0], d.mval[1],
                   d.sort_hist, d.node_child, d.node_parent, d.leaf_parent, d.node_range, d.node_rec,
-                  d.sfast, d.sparse, d.pslots, d.heavy, d.pcount, d.pfound, d.poff, d.scan_tiles, d.cnt};
+                  d.sfast, d.sparse, d.pslots, d.heavy, d.pcount, d.pfound, d.sinv, d.poff, d.scan_tiles, d.cnt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int k = 0; k < 10; ++k)
@@ -376,7 +376,7 @@ int b2g_batch::build_tree(bool use_given_aabbs) {
     hipLaunchKernelGGL(k_world_leaves, dim3(gs), dim3(kBlock), 0, stream, d, d.mkey[0]);
     if (ns >= 2) hipLaunchKernelGGL(k_lbvh_build, dim3(grid_for(ns - 1)), dim3(kBlock), 0, stream, d, d.mkey[0]);
   }
-  hipLaunchKernelGGL(k_leaf_aabbs, dim3(gs), dim3(kBlock), 0, stream, d, d.mval[0]);
+  if (!refit) hipLaunchKernelGGL(k_leaf_aabbs, dim3(gs), dim3(kBlock), 0, stream, d, d.mval[0]);
   if (ns >= 2) {
     // range unions are only looked up inside one world (a range spanning
     // worlds gets an unbounded box first), so the table needs the levels of
@@ -386,7 +386,7 @@ int b2g_batch::build_tree(bool use_given_aabbs) {
     while ((2 << top) <= span) ++top;  // floor(log2(span))
     Dev dj = d;
     dj.sparse_levels = top + 1;  // levels built this time
-    for (int j = 1; j <= top; j += 3)
+    for (int j = 1; j <= top; j += kSparseLevelsPerLaunch)
       hipLaunchKernelGGL(k_sparse_levels, dim3(gs), dim3(kBlock), 0, stream, dj, j);
     hipLaunchKernelGGL(k_node_records, dim3(grid_for(ns - 1)), dim3(kBlock), 0, stream, dj, d.mkey[0], d.mval[0]);
   }
@@ -867,7 +867,6 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
     }
     est_cons = pinned[C_CONS];
     hipLaunchKernelGGL(k_world_colour_sort, dim3(d.nw), dim3(kBlock), 0, stream, d, rounds_pending);
-    hipLaunchKernelGGL(k_world_colour_stats, dim3(grid_for(d.nw)), dim3(kBlock), 0, stream, d);
     round_host += (unsigned)rounds_pending;
     rounds_pending = 0;
     enqueue_islands_and_init();
@@ -944,7 +943,6 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
     if (c < kMaxColours) normal_colours = c + 1;
   }
   last_ncolours = pinned[C_NCOLOURS];
-  hipLaunchKernelGGL(k_world_colour_stats, dim3(grid_for(d.nw)), dim3(kBlock), 0, stream, d);
   // colour limit for captured steps: what this step needed plus one
   graph_K = std::min(kMaxColours, std::max(normal_colours, 1) + 1);
   enqueue_solve(vi, pi, colours, sizes, begins, 0);
@@ -979,7 +977,6 @@ int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
                      d);
   if (use_per_world()) {
     hipLaunchKernelGGL(k_world_colour_sort, dim3(d.nw), dim3(kBlock), 0, stream, d, rounds_pending);
-    hipLaunchKernelGGL(k_world_colour_stats, dim3(grid_for(d.nw)), dim3(kBlock), 0, stream, d);
     round_host += (unsigned)rounds_pending;
     rounds_pending = 0;
     enqueue_islands_and_init();
@@ -993,7 +990,6 @@ int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
     return B2G_OK;
   }
   enqueue_colour_groups();
-  hipLaunchKernelGGL(k_world_colour_stats, dim3(grid_for(d.nw)), dim3(kBlock), 0, stream, d);
   enqueue_islands_and_init();
   std::vector<int> colours, sizes;
   for (int c = 0; c < K; ++c) {

@@ -146,6 +146,7 @@ struct Dev {
   int* wroot;        // per world: the internal node whose leaves are exactly the world's (-1: one leaf)
   unsigned* wbounds; // 4 per world (ordered floats): min x, min y, max x, max y
   int* pcount;       // per shape: pairs whose lower shape it is (the pair list's segment length)
+  int* sinv;         // per shape: its leaf's sorted position in the last full tree build
   int* pfound;       // per shape: pairs its traversal found (own + those of a static partner);
                      // for a static shape, the fill cursor of its segment
   int* poff;         // exclusive scan
@@ -529,17 +530,14 @@ __global__ void k_clear_pairs(Dev d) {
 // more colours than its last colouring from scratch, its next step colours
 // every constraint afresh. Each world decides from its own steps only.
 constexpr int kRecolourSlack = 3;
-__global__ void __launch_bounds__(kBlock) k_world_colour_stats(Dev d) {
-  B2G_FOR(w, 0, d.nw) {
-    if (!world_active(d, w)) continue;
-    const int used = d.wcolmax[w] + 1;
-    d.wcolmax[w] = -1;
-    if (d.wrecolour[w]) {
-      d.wfresh[w] = used > 0 ? used : 1;
-      d.wrecolour[w] = 0;
-    } else if (used >= d.wfresh[w] + kRecolourSlack) {
-      d.wrecolour[w] = 1;
-    }
+// `used`: the colours the world used this step (its highest normal colour + 1)
+__device__ __forceinline__ void world_colour_stats(const Dev& d, int w, int used) {
+  d.wcolmax[w] = -1;
+  if (d.wrecolour[w]) {
+    d.wfresh[w] = used > 0 ? used : 1;
+    d.wrecolour[w] = 0;
+  } else if (used >= d.wfresh[w] + kRecolourSlack) {
+    d.wrecolour[w] = 1;
   }
 }
 
@@ -1740,7 +1738,12 @@ __global__ void __launch_bounds__(kBlock) k_island_finalize(Dev d, int it) {
 
 // write back: SynchronizeTransform (b2_body.h:957-961) and ClearForces
 // (b2_world.cpp:1173-1180)
+// (also the per-world colouring upkeep of the colour-sweep path, which may run
+// any time between this step's colouring and the next)
 __global__ void __launch_bounds__(kBlock) k_finish_bodies(Dev d) {
+  B2G_FOR(w, 0, d.nw) {
+    if (world_active(d, w)) world_colour_stats(d, w, d.wcolmax[w] + 1);
+  }
   B2G_FOR(b, 0, d.nb) {
     const int w = d.bworld[b];
     if (!world_active(d, w)) continue;
@@ -1809,7 +1812,10 @@ __global__ void __launch_bounds__(kBlock) k_world_colour_sort(Dev d, int rounds_
     int topn = 0;  // normal colours only
     for (int c = 0; c < kMaxColours; ++c)
       if (hist[c]) topn = c + 1;
-    d.wcolmax[w] = topn - 1;
+    if (world_active(d, w))
+      world_colour_stats(d, w, topn);  // the colouring upkeep, for this world right away
+    else
+      d.wcolmax[w] = topn - 1;
   }
   __syncthreads();
   for (int c = threadIdx.x; c <= kColourSlots; c += blockDim.x) d.wcol[w * (kColourSlots + 1) + c] = t0 + off[c];
@@ -1985,12 +1991,15 @@ __device__ __forceinline__ float4 shape_aabb(int type, int count, const float4* 
   return make_float4(lo.x - r, lo.y - r, hi.x + r, hi.y + r);
 }
 
-// Shape AABBs; with `bounds`, also the world bounds of the AABB centres
-// (2 * centre, as b2BroadPhase::GetCenter2), which only the Morton keys of a
-// full tree build use. Bounds are reduced per wave, then per block (runs of
-// equal worlds merge), so one world's shapes do not all hit the same four
-// words.
+// Shape AABBs. For a full tree build (`bounds`), also the world bounds of the
+// AABB centres (2 * centre, as b2BroadPhase::GetCenter2), which only the
+// Morton keys use; they are reduced per wave, then per block (runs of equal
+// worlds merge), so one world's shapes do not all hit the same four words.
+// For a refit, each box also goes straight to its leaf of the sparse table
+// (the leaf order is the last build's) with the per-broadphase resets of
+// k_leaf_aabbs, which then has no launch of its own.
 __global__ void __launch_bounds__(kBlock) k_aabb(Dev d, int bounds) {
+  if (!bounds && blockIdx.x == 0 && threadIdx.x == 0) d.cnt[C_HEAVY] = 0;
   constexpr int kWaves = kBlock / kWave;
   __shared__ int s_w[kWaves];
   __shared__ float s_b[kWaves][4];
@@ -2001,6 +2010,10 @@ __global__ void __launch_bounds__(kBlock) k_aabb(Dev d, int bounds) {
       const int4 h = d.shdr[i];
       const float4 bb = shape_aabb(h.x, h.z, d.geo + 9 * (size_t)i, xform_of(d.xf[h.y]));
       d.aabb[i] = bb;
+      if (!bounds) {
+        d.sparse[d.sinv[i]] = bb;
+        d.pcount[i] = 0;
+      }
       cx = bb.x + bb.z;
       cy = bb.y + bb.w;
       w = h.w;
@@ -2152,7 +2165,7 @@ __global__ void __launch_bounds__(kBlock) k_world_leaves(Dev d, const uint64_t* 
 // Node bounds without a bottom-up walk: every internal node covers a
 // contiguous range of sorted leaves, so its AABB is a range union, answered in
 // O(1) from a sparse table (level j holds the union of 2^j leaves starting at
-// each position). Building the table is log2(n) element-wise passes (two per
+// each position). Building the table is log2(n) element-wise passes (four per
 // launch); no atomics, fences or inter-block ordering are involved.
 __device__ __forceinline__ float4 aabb_union(float4 a, float4 b) {
   return make_float4(minf(a.x, b.x), minf(a.y, b.y), maxf(a.z, b.z), maxf(a.w, b.w));
@@ -2167,31 +2180,25 @@ __global__ void __launch_bounds__(kBlock) k_leaf_aabbs(Dev d, const uint32_t* __
     const uint32_t i = vals[k];
     d.sparse[k] = d.aabb[i];
     d.pcount[i] = 0;
+    d.sinv[i] = k;
   }
 }
 
-// levels j, j + 1 and j + 2 from level j - 1 (2, 4 and 8 entries of it;
-// indices past the end clamp to the last leaf, which the unions absorb)
+// levels j .. j + 3 from level j - 1 (2, 4, 8 and 16 entries of it; indices
+// past the end clamp to the last leaf, which the unions absorb)
+constexpr int kSparseLevelsPerLaunch = 4;
 __global__ void __launch_bounds__(kBlock) k_sparse_levels(Dev d, int j) {
   const int n = d.ns;
   const int h = 1 << (j - 1);
   const float4* prev = d.sparse + (size_t)(j - 1) * n;
-  float4* l1 = d.sparse + (size_t)j * n;
-  float4* l2 = d.sparse + (size_t)(j + 1) * n;
-  float4* l3 = d.sparse + (size_t)(j + 2) * n;
-  const bool has2 = j + 1 < d.sparse_levels, has3 = j + 2 < d.sparse_levels;
   B2G_FOR(i, 0, n) {
-    const float4 u1 = aabb_union(prev[i], prev[min(i + h, n - 1)]);
-    l1[i] = u1;
-    if (has2) {
-      const float4 u2 = aabb_union(u1, aabb_union(prev[min(i + 2 * h, n - 1)], prev[min(i + 3 * h, n - 1)]));
-      l2[i] = u2;
-      if (has3) {
-        float4 u3 = u2;
-#pragma unroll
-        for (int k = 4; k < 8; ++k) u3 = aabb_union(u3, prev[min(i + k * h, n - 1)]);
-        l3[i] = u3;
-      }
+    // level j + lv is the union of 2^(lv + 1) entries of level j - 1, h apart
+    float4 u = prev[i];
+    int have = 1;
+    for (int lv = 0; lv < kSparseLevelsPerLaunch && j + lv < d.sparse_levels; ++lv) {
+      for (int k = have; k < 2 * have; ++k) u = aabb_union(u, prev[min(i + k * h, n - 1)]);
+      have *= 2;
+      d.sparse[(size_t)(j + lv) * n + i] = u;
     }
   }
 }
