@@ -181,7 +181,9 @@ struct b2g_batch {
   void enqueue_islands_and_init();
   void enqueue_islands_join();
   void enqueue_solve(int vi, int pi, const std::vector<int>& colours, const std::vector<int>& sizes,
-                     const std::vector<int>& begins, int graph_grid);
+                     const std::vector<int>& begins, int graph_grid, bool finish_in_broadphase);
+  bool finish_pending = false;  // the step's body finish is left to the broadphase's AABB launch
+  void flush_finish();
   int enqueue_graph_body(int vi, int pi, unsigned flags, int K);
   int check_status();
   int update_world_layout();
@@ -342,8 +344,16 @@ int b2g_batch::radix_sort(int n, int bits) {
 // (skipped by a refit), sparse table and node records.
 constexpr int kRefitSteps = 3;
 
+// the step's body finish, when no AABB launch is there to carry it
+void b2g_batch::flush_finish() {
+  if (!finish_pending) return;
+  hipLaunchKernelGGL(k_finish_bodies, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
+  finish_pending = false;
+}
+
 int b2g_batch::build_tree(bool use_given_aabbs) {
   const int ns = d.ns;
+  if (ns == 0 || use_given_aabbs) flush_finish();
   if (ns == 0) return B2G_OK;
   const int gs = grid_for(ns);
   // Refit: the last build's leaf order and topology with new boxes. Any tree
@@ -359,7 +369,9 @@ int b2g_batch::build_tree(bool use_given_aabbs) {
   }
   if (!refit) hipLaunchKernelGGL(k_world_bounds_reset, dim3(grid_for(d.nw)), dim3(kBlock), 0, stream, d);
   if (!use_given_aabbs) {
-    hipLaunchKernelGGL(k_aabb, dim3(gs), dim3(kBlock), 0, stream, d, refit ? 0 : 1);
+    hipLaunchKernelGGL(k_aabb, dim3(std::max(gs, finish_pending ? grid_for(d.nb) : 1)), dim3(kBlock), 0, stream, d,
+                       refit ? 0 : 1, finish_pending ? 1 : 0);
+    finish_pending = false;
   } else {
     // only the world bounds of the caller's AABBs
     hipLaunchKernelGGL(k_world_bounds_from_aabbs, dim3(gs), dim3(kBlock), 0, stream, d);
@@ -406,6 +418,7 @@ void b2g_batch::refresh_filters() {
 int b2g_batch::broadphase(bool use_given_aabbs) {
   const int ns = d.ns;
   if (ns == 0) {
+    flush_finish();
     hipLaunchKernelGGL(k_clear_pairs, dim3(1), dim3(1), 0, stream, d);
     hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, stream, d);
     return B2G_OK;
@@ -604,7 +617,7 @@ constexpr int kTailColourMax = 512;
 // colours: the slots to sweep; sizes[c] (eager) or -1 (graph: unknown);
 // begins[c]: first slot of each swept colour (eager; empty in graph mode)
 void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, const std::vector<int>& sizes,
-                              const std::vector<int>& begins, int graph_grid) {
+                              const std::vector<int>& begins, int graph_grid, bool finish_in_broadphase) {
   // graph mode (sizes unknown): the shared tail launch also runs the serial
   // group, after the last colour
   const bool tail = !sizes.empty() && sizes[0] < 0;
@@ -697,7 +710,10 @@ void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, c
     joint_sweep(k_joint_position, it);
     hipLaunchKernelGGL(k_island_finalize, dim3(gf), dim3(kBlock), 0, stream, d, it);
   }
-  hipLaunchKernelGGL(k_finish_bodies, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
+  if (finish_in_broadphase)
+    finish_pending = true;  // the broadphase's first launch (k_aabb) ends the step
+  else
+    hipLaunchKernelGGL(k_finish_bodies, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
 }
 
 int b2g_batch::check_status() {
@@ -820,6 +836,7 @@ void b2g_batch::launch_solve_worlds(int vi, int pi) {
 }
 
 int b2g_batch::step(int vi, int pi, unsigned flags) {
+  flush_finish();  // (a finish is only deferred to a broadphase of the same step)
   if (layout_dirty) {
     int rc = update_world_layout();
     if (rc) return rc;
@@ -945,7 +962,7 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
   last_ncolours = pinned[C_NCOLOURS];
   // colour limit for captured steps: what this step needed plus one
   graph_K = std::min(kMaxColours, std::max(normal_colours, 1) + 1);
-  enqueue_solve(vi, pi, colours, sizes, begins, 0);
+  enqueue_solve(vi, pi, colours, sizes, begins, 0, !(flags & B2G_STEP_SKIP_BROADPHASE));
   if (prof_on) HIP_TRY(hipEventRecord(ev[2], stream));
   if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
     if ((rc = broadphase(false))) return rc;
@@ -997,7 +1014,7 @@ int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
     sizes.push_back(-1);
   }
   const int per_colour = std::max(256, (int)(std::max(est_cons, 256) * 2LL / std::max(K, 1)));
-  enqueue_solve(vi, pi, colours, sizes, {}, grid_for(per_colour));
+  enqueue_solve(vi, pi, colours, sizes, {}, grid_for(per_colour), !(flags & B2G_STEP_SKIP_BROADPHASE));
   if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
     int rc = broadphase(false);
     if (rc) return rc;

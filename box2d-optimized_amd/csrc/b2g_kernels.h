@@ -1740,22 +1740,27 @@ __global__ void __launch_bounds__(kBlock) k_island_finalize(Dev d, int it) {
 // (b2_world.cpp:1173-1180)
 // (also the per-world colouring upkeep of the colour-sweep path, which may run
 // any time between this step's colouring and the next)
-__global__ void __launch_bounds__(kBlock) k_finish_bodies(Dev d) {
+// body b's transform from its final position, and whether the step moves it
+__device__ __forceinline__ float4 synced_xf(const Dev& d, int b) {
+  const float4 p = d.pos[b], pr = d.prop[b];
+  const rot2 q = rot_of(p.z);
+  const vec2 c = mk(p.x, p.y) - mul(q, mk(pr.z, pr.w));
+  return make_float4(c.x, c.y, q.s, q.c);
+}
+__device__ __forceinline__ bool synced_body(const Dev& d, int b) {
+  return world_active(d, d.bworld[b]) && d.btype[b] != kStaticBody;
+}
+__device__ __forceinline__ void finish_step(const Dev& d) {
   B2G_FOR(w, 0, d.nw) {
     if (world_active(d, w)) world_colour_stats(d, w, d.wcolmax[w] + 1);
   }
   B2G_FOR(b, 0, d.nb) {
-    const int w = d.bworld[b];
-    if (!world_active(d, w)) continue;
-    if (d.btype[b] != kStaticBody) {
-      const float4 p = d.pos[b], pr = d.prop[b];
-      const rot2 q = rot_of(p.z);
-      const vec2 c = mk(p.x, p.y) - mul(q, mk(pr.z, pr.w));
-      d.xf[b] = make_float4(c.x, c.y, q.s, q.c);
-    }
+    if (!world_active(d, d.bworld[b])) continue;
+    if (d.btype[b] != kStaticBody) d.xf[b] = synced_xf(d, b);
     d.force[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
 }
+__global__ void __launch_bounds__(kBlock) k_finish_bodies(Dev d) { finish_step(d); }
 
 
 // ===========================================================================
@@ -1998,8 +2003,12 @@ __device__ __forceinline__ float4 shape_aabb(int type, int count, const float4* 
 // For a refit, each box also goes straight to its leaf of the sparse table
 // (the leaf order is the last build's) with the per-broadphase resets of
 // k_leaf_aabbs, which then has no launch of its own.
-__global__ void __launch_bounds__(kBlock) k_aabb(Dev d, int bounds) {
+// With `finish`, the launch also ends the step (k_finish_bodies' work): the
+// shapes then take their bodies' transforms from the final positions, as the
+// body threads compute them, rather than read them while they are written.
+__global__ void __launch_bounds__(kBlock) k_aabb(Dev d, int bounds, int finish) {
   if (!bounds && blockIdx.x == 0 && threadIdx.x == 0) d.cnt[C_HEAVY] = 0;
+  if (finish) finish_step(d);
   constexpr int kWaves = kBlock / kWave;
   __shared__ int s_w[kWaves];
   __shared__ float s_b[kWaves][4];
@@ -2008,7 +2017,8 @@ __global__ void __launch_bounds__(kBlock) k_aabb(Dev d, int bounds) {
     float cx = 0.0f, cy = 0.0f;
     if (valid) {
       const int4 h = d.shdr[i];
-      const float4 bb = shape_aabb(h.x, h.z, d.geo + 9 * (size_t)i, xform_of(d.xf[h.y]));
+      const float4 xf = finish && synced_body(d, h.y) ? synced_xf(d, h.y) : d.xf[h.y];
+      const float4 bb = shape_aabb(h.x, h.z, d.geo + 9 * (size_t)i, xform_of(xf));
       d.aabb[i] = bb;
       if (!bounds) {
         d.sparse[d.sinv[i]] = bb;
