@@ -233,7 +233,7 @@ int b2g_batch::alloc() {
   for (int k = 0; k < 5; ++k) A(d.pc[k], NC);
   A(d.col_count, kColourSlots + 1); A(d.col_cursor, kColourSlots + 1);
   A(d.bmask, NB); A(d.bslot, NB);
-  A(d.btab, (size_t)kBodyTab * NB); A(d.parent, NB); A(d.sepbad, 2 * (size_t)NB); A(d.jok, 2 * (size_t)NB); A(d.solved, NB);
+  A(d.btab, (size_t)kBodyTab * NB); A(d.parent, NB); A(d.sepbad, 3 * (size_t)NB); A(d.jok, 3 * (size_t)NB); A(d.solved, NB);
   A(d.mkey[0], NS); A(d.mkey[1], NS); A(d.mval[0], NS); A(d.mval[1], NS);
   const size_t tiles = (size_t)(NS + kSortTile - 1) / kSortTile;
   A(d.sort_hist, kRadix * tiles);
@@ -704,11 +704,9 @@ void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, c
   (void)hipStreamWaitEvent(stream, join_ev, 0);
   hipLaunchKernelGGL(k_store_and_integrate, dim3(grid_for(std::max(est_cons + d.nj, d.nb))), dim3(kBlock), 0, stream,
                      d, 1);
-  const int gf = grid_for(std::max(est_cons, 1) + d.nj);
-  for (int it = 0; it < pi; ++it) {
+  for (int it = 0; it < pi; ++it) {  // each launch applies the island early exit itself
     contact_sweep(B2G_KERNEL_CONTACT_POSITION, k_contact_position, k_sweep_tail<kTailPosition>, it);
     joint_sweep(k_joint_position, it);
-    hipLaunchKernelGGL(k_island_finalize, dim3(gf), dim3(kBlock), 0, stream, d, it);
   }
   if (finish_in_broadphase)
     finish_pending = true;  // the broadphase's first launch (k_aabb) ends the step

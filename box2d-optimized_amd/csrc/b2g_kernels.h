@@ -124,9 +124,9 @@ struct Dev {
   unsigned long long* bslot;
   // islands
   int* parent;
-  unsigned* sepbad;  // 2 * nb: per position-iteration parity and island, 1 when a contact of the
+  unsigned* sepbad;  // 3 * nb: per island and position iteration (mod 3), 1 when a contact of the
                      // island separated by less than -3 linearSlop in that iteration
-  int* jok;          // 2 * nb
+  int* jok;          // 3 * nb: per island and position iteration (mod 3), 0 when a joint was out of slop
   int* solved;       // nb
   // broadphase
   uint64_t* mkey[2]; uint32_t* mval[2];
@@ -793,6 +793,32 @@ __device__ __forceinline__ void island_reset(const Dev& d, int isl) {
   d.jok[isl] = 1;
 }
 
+// Per-island early exit (b2_island.cpp:403-408): an island stops after the
+// first position iteration in which no contact separated by less than
+// -3 linearSlop and every joint was within its slop. Each constraint of
+// iteration `it` decides it from iteration it - 1's flags (and `solved`, set
+// by the first constraint to find it), so no launch between the iterations
+// is needed. The flags rotate over three slots: iteration it marks slot
+// it % 3, reads slot (it - 1) % 3 and clears slot (it + 1) % 3 for the next.
+__device__ __forceinline__ size_t island_slot(const Dev& d, int isl, int it) {
+  return (size_t)(it % 3) * d.nb + isl;
+}
+__device__ __forceinline__ bool island_done(const Dev& d, int isl, int it) {
+  if (d.solved[isl]) return true;
+  if (it == 0) return false;
+  const size_t prev = island_slot(d, isl, it - 1);
+  if (d.sepbad[prev] == 0u && d.jok[prev] != 0) {
+    d.solved[isl] = 1;
+    return true;
+  }
+  return false;
+}
+__device__ __forceinline__ void island_rearm(const Dev& d, int isl, int it) {
+  const size_t nxt = island_slot(d, isl, it + 1);
+  d.sepbad[nxt] = 0u;
+  d.jok[nxt] = 1;
+}
+
 // island of solved constraint slot k (k < n) or joint k - n, and that
 // island's position state reset
 __device__ __forceinline__ void island_assign(const Dev& d, int n, int k) {
@@ -1252,14 +1278,15 @@ __global__ void __launch_bounds__(kBlock) k_store_and_integrate(Dev d, int islan
 // b2ContactSolver::SolvePositionConstraints (:659-787) for slots [begin, end)
 // returns the minimum separation seen (0 when the island is already solved)
 template <class BP, class Src>
-__device__ __forceinline__ float contact_position_src(const Dev& d, int isl, BP pos, const Src& src) {
+__device__ __forceinline__ float contact_position_src(const Dev& d, int isl, int it, BP pos, const Src& src) {
   // the constraint's loads are issued before the island test, so they are in
   // flight together with it
   const float4 v2i = src.vc(2);
   const float4 q0 = src.pc(0), q1 = src.pc(1), q2 = src.pc(2), q3 = src.pc(3), q4 = src.pc(4);
   const int ia = as_int(v2i.x), ib = as_int(v2i.y);
   const float4 pa = *pos(ia), pb = *pos(ib);  // in flight with the island test too
-  if (d.solved[isl]) return 0.0f;
+  if (island_done(d, isl, it)) return 0.0f;
+  island_rearm(d, isl, it);
   const vec2 lp = mk(q0.x, q0.y), ln = mk(q0.z, q0.w);
   const vec2 lcA = mk(q2.x, q2.y), lcB = mk(q2.z, q2.w);
   const float mA = q3.x, mB = q3.y, iA = q3.z, iB = q3.w;
@@ -1317,11 +1344,11 @@ __device__ __forceinline__ float contact_position_src(const Dev& d, int isl, BP 
   return min_sep;
 }
 template <class BP>
-__device__ __forceinline__ float contact_position_t(const Dev& d, int s, int isl, BP pos) {
-  return contact_position_src(d, isl, pos, SlotFields{d, s});
+__device__ __forceinline__ float contact_position_t(const Dev& d, int s, int isl, int it, BP pos) {
+  return contact_position_src(d, isl, it, pos, SlotFields{d, s});
 }
-__device__ __forceinline__ float contact_position(const Dev& d, int s, int isl) {
-  return contact_position_t(d, s, isl, GlobalBodies{d.pos});
+__device__ __forceinline__ float contact_position(const Dev& d, int s, int isl, int it) {
+  return contact_position_t(d, s, isl, it, GlobalBodies{d.pos});
 }
 
 // The island early exit (b2_island.cpp:403-408) asks whether the island's
@@ -1337,7 +1364,7 @@ __device__ __forceinline__ void mark_island_sep(const Dev& d, int it, int isl, b
   const bool bad = valid && separation_bad(m);
   const uint64_t act = __ballot(bad);
   if (act == 0) return;
-  unsigned* base = d.sepbad + (size_t)(it & 1) * d.nb;
+  unsigned* base = d.sepbad + (size_t)(it % 3) * d.nb;
   const int first = __ffsll((long long)act) - 1;
   const int key0 = __shfl(isl, first, kWave);
   if (__ballot(bad && isl != key0) == 0) {
@@ -1350,8 +1377,8 @@ __device__ __forceinline__ void mark_island_sep(const Dev& d, int it, int isl, b
 __device__ __forceinline__ void contact_position_serial(const Dev& d, int begin, int end, int it) {
   for (int s = begin; s < end; ++s) {
     const int isl = d.cisland[s];
-    const float m = contact_position(d, s, isl);
-    if (separation_bad(m)) d.sepbad[(size_t)(it & 1) * d.nb + isl] = 1u;
+    const float m = contact_position(d, s, isl, it);
+    if (separation_bad(m)) d.sepbad[island_slot(d, isl, it)] = 1u;
   }
 }
 
@@ -1364,7 +1391,7 @@ __global__ void __launch_bounds__(kBlock) k_contact_position(Dev d, int colour, 
   }
   B2G_FOR_ALL(s, valid, begin, end) {
     const int isl = valid ? d.cisland[s] : 0;
-    const float m = valid ? contact_position(d, s, isl) : 0.0f;
+    const float m = valid ? contact_position(d, s, isl, it) : 0.0f;
     mark_island_sep(d, it, isl, valid != 0, m);
   }
 }
@@ -1485,7 +1512,7 @@ __global__ void __launch_bounds__(kTailThreads) k_sweep_tail(Dev d, TailSlots t,
           if (s_next >= 0) nxt = load_position_regs(d, s_next);
         }
         const bool valid = s_cur >= 0;
-        const float m = valid ? contact_position_src(d, cur.isl, GlobalBodies{d.pos}, cur) : 0.0f;
+        const float m = valid ? contact_position_src(d, cur.isl, it, GlobalBodies{d.pos}, cur) : 0.0f;
         mark_island_sep(d, it, valid ? cur.isl : 0, valid, m);
         __syncthreads();
       }
@@ -1503,7 +1530,7 @@ __global__ void __launch_bounds__(kTailThreads) k_sweep_tail(Dev d, TailSlots t,
           if (valid) contact_velocity(d, s);
         } else {
           const int isl = valid ? d.cisland[s] : 0;
-          const float m = valid ? contact_position(d, s, isl) : 0.0f;
+          const float m = valid ? contact_position(d, s, isl, it) : 0.0f;
           mark_island_sep(d, it, isl, valid, m);
         }
       }
@@ -1662,7 +1689,8 @@ __global__ void __launch_bounds__(kBlock) k_joint_velocity(Dev d, int begin, int
 __global__ void __launch_bounds__(kBlock) k_joint_position(Dev d, int begin, int end, int it) {
   B2G_FOR(q, begin, end) {
     const int isl = d.jisland[q];
-    if (isl < 0 || d.solved[isl]) continue;
+    if (isl < 0 || island_done(d, isl, it)) continue;
+    island_rearm(d, isl, it);
     const Joint j = d.joints[d.jorder[q]];
     const JointSolve s = d.jsol[q];
     const float4 pa = d.pos[j.a], pb = d.pos[j.b];
@@ -1705,14 +1733,10 @@ __global__ void __launch_bounds__(kBlock) k_joint_position(Dev d, int begin, int
     if (writes_body(d, j.a)) d.pos[j.a] = make_float4(cA.x, cA.y, aA, 0.0f);
     if (writes_body(d, j.b)) d.pos[j.b] = make_float4(cB.x, cB.y, aB, 0.0f);
     const bool ok = position_error <= kLinearSlop && angular_error <= kAngularSlop;
-    if (!ok) d.jok[(it & 1) * d.nb + isl] = 0;
+    if (!ok) d.jok[island_slot(d, isl, it)] = 0;
   }
 }
 
-// Per-island early exit after position iteration `it` (b2_island.cpp:403-408):
-// solved when min separation >= -3 slop and every joint is within slop. Also
-// re-arms the other parity buffer for iteration it + 1 (a different buffer,
-// so the two roles never race).
 // Island of every solved contact slot and joint, and the island's position
 // state reset (the islands come from a pass that runs beside the velocity
 // phase; the position phase is the first to need them).
@@ -1721,20 +1745,6 @@ __global__ void __launch_bounds__(kBlock) k_island_assign(Dev d) {
   B2G_FOR(k, 0, n + d.nj) island_assign(d, n, k);
 }
 
-__global__ void __launch_bounds__(kBlock) k_island_finalize(Dev d, int it) {
-  const int n = d.cnt[C_CONS];
-  B2G_FOR(k, 0, n + d.nj) {
-    const int isl = k < n ? d.cisland[k] : d.jisland[k - n];
-    if (isl < 0) continue;
-    const int nxt = ((it + 1) & 1) * d.nb + isl;
-    d.sepbad[nxt] = 0u;
-    d.jok[nxt] = 1;
-    if (d.solved[isl]) continue;
-    const int cur = (it & 1) * d.nb + isl;
-    const bool ok = d.sepbad[cur] == 0u && d.jok[cur] != 0;
-    if (ok) d.solved[isl] = 1;
-  }
-}
 
 // write back: SynchronizeTransform (b2_body.h:957-961) and ClearForces
 // (b2_world.cpp:1173-1180)
@@ -1910,8 +1920,9 @@ __global__ void __launch_bounds__(1024) k_solve_worlds(Dev d, int vi, int pi, in
           if (tid == 0)
             for (int s = lo; s < hi; ++s) {
               const int isl = d.cisland[s];
-              const float m = kLds ? contact_position_t(d, s, isl, lpos) : contact_position_t(d, s, isl, gpos);
-              if (separation_bad(m)) d.sepbad[(size_t)(it & 1) * d.nb + isl] = 1u;
+              const float m =
+                  kLds ? contact_position_t(d, s, isl, it, lpos) : contact_position_t(d, s, isl, it, gpos);
+              if (separation_bad(m)) d.sepbad[island_slot(d, isl, it)] = 1u;
             }
         } else {
           for (int base = lo; base < hi; base += nt) {  // every lane of a wave takes part in the min
@@ -1919,20 +1930,21 @@ __global__ void __launch_bounds__(1024) k_solve_worlds(Dev d, int vi, int pi, in
             const bool valid = s < hi;
             const int isl = valid ? d.cisland[s] : 0;
             float m = 0.0f;
-            if (valid) m = kLds ? contact_position_t(d, s, isl, lpos) : contact_position_t(d, s, isl, gpos);
+            if (valid) m = kLds ? contact_position_t(d, s, isl, it, lpos) : contact_position_t(d, s, isl, it, gpos);
             mark_island_sep(d, it, isl, valid, m);
           }
         }
         __syncthreads();
       }
-      // per-island early exit (b2_island.cpp:403-408), re-arming the other parity
+      // per-island early exit (b2_island.cpp:403-408) after this iteration,
+      // and the slot of iteration it + 1 cleared (see island_done)
       for (int k = t0 + tid; k < t1; k += nt) {
         const int isl = d.cisland[k];
-        const int nxt = ((it + 1) & 1) * d.nb + isl;
+        const size_t nxt = island_slot(d, isl, it + 1);
         d.sepbad[nxt] = 0u;
         d.jok[nxt] = 1;
         if (d.solved[isl]) continue;
-        const int cur = (it & 1) * d.nb + isl;
+        const size_t cur = island_slot(d, isl, it);
         if (d.sepbad[cur] == 0u && d.jok[cur] != 0) d.solved[isl] = 1;
       }
       __syncthreads();
