@@ -80,6 +80,29 @@ def test_gpu_mixed_warm_starting_matches_worlds_alone_bitexact(engine, solver):
     assert changed > 0  # switching warm starting off is visible
 
 
+def test_gpu_per_world_solver_with_large_worlds_bitexact(engine):
+    """Worlds above 1024 bodies get a per-world solver launch of their own,
+    beside the launch of the smaller worlds (on a second stream); every world
+    still equals itself stepped alone."""
+    scenes = [S.random_pile(1300, seed=31, half_w=22.0, height=60.0), S.pyramid(8),
+              S.random_pile(150, seed=32, half_w=8.0, height=20.0), S.random_pile(1100, seed=33, half_w=20.0, height=55.0),
+              S.batch_world(5, 90), S.batch_world(6, 300)]
+    assert sum(sc.body_count > 1024 for sc in scenes) == 2
+    steps = 25
+    batch = engine.Batch.from_scenes(scenes)
+    batch.set_option(engine.OPT_SOLVER, engine.SOLVER_WORLDS)
+    batch.set_option(engine.OPT_WORLD_BLOCK, 128)  # the small worlds' launch: 128-thread workgroups
+    got = batch.advance(steps).bodies()
+    assert batch.get_option(engine.OPT_LAST_SOLVER) == 2
+    for i, sc in enumerate(scenes):
+        alone = engine.Batch.from_scenes([sc]).advance(steps).bodies()
+        part = got[batch.world_bodies[i]:batch.world_bodies[i + 1]]
+        same = _state(part) == _state(alone)
+        print(f"world {i} ({sc.name}, {sc.body_count} bodies): {same.all(axis=1).sum()} of {len(alone)} bit-identical",
+              flush=True)
+        assert same.all(), sc.name
+
+
 def test_gpu_batch_world_matches_drop_in_world(engine):
     """A batch of one built by b2a_batch_from_scenes steps exactly like the
     drop-in b2World built from the same scene."""
