@@ -1144,7 +1144,7 @@ int b2g_batch_create(b2g_batch** out, const b2g_capacity* cap) {
   if (!out || !cap) return fail(B2G_ERR_INVALID, "null argument");
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(B2G_ERR_NO_DEVICE, "no HIP device visible");
-  if (cap->contacts >= (1 << 24)) return fail(B2G_ERR_INVALID, "contact capacity must stay below 2^24");
+  if (cap->contacts >= (1 << 28)) return fail(B2G_ERR_INVALID, "contact capacity must stay below 2^28");
   b2g_batch* b = new b2g_batch();
   b->cap = *cap;
   HIP_TRY(hipGetDevice(&b->dev_id));

@@ -1067,7 +1067,7 @@ int b2a_batch_from_scenes(b2g_batch** out, int32_t count, const b2g_scene* scene
     ns += scenes[i].fixture_count;
     nj += scenes[i].joint_count;
   }
-  if (nb >= (1ll << 31) || ns * contacts_per_shape >= (1ll << 24)) return B2G_ERR_CAPACITY;
+  if (nb >= (1ll << 31) || ns * contacts_per_shape >= (1ll << 28)) return B2G_ERR_CAPACITY;  // b2g_batch_create's limit
   std::vector<b2g_body> bodies(nb);
   std::vector<b2g_shape> shapes(ns);
   std::vector<b2g_revolute> joints(nj);
