@@ -162,6 +162,7 @@ struct b2g_batch {
   int big_worlds = 0;              // worlds of more than kBigWorldBodies bodies (first in worder)
   int max_small_world_bodies = 0;  // largest of the others
   int solver_mode = 0;       // B2G_SOLVER_AUTO / _COLOURS / _WORLDS
+  bool keep_idle = true;     // B2G_OPT_KEEP_IDLE
   int world_block = 0;       // threads per world workgroup (0: autotune)
   int tuned_block = 0;       // chosen by the autotuner (or world_block)
   bool tuned_lds = false;    // autotuner: stage world bodies in LDS
@@ -199,6 +200,9 @@ struct b2g_batch {
   int broadphase(bool use_given_aabbs);
   void refresh_filters();
   bool tree_stale = true;     // bodies or shapes changed since the last tree build
+  // the contact list is the last broadphase's pair list (segments in d.poff_prev), and no body,
+  // shape, joint or contact changed since: worlds that are not active may keep their pairs
+  bool pairs_current = false;
   int refits_left = 0;        // steps that may refit the last tree before it is rebuilt
   int static_shapes = 0;      // shapes of static bodies (their pair segments are sorted every broadphase)
   int tree_ns = -1;           // shape count of the last full build
@@ -234,7 +238,7 @@ int b2g_batch::alloc() {
   A(d.col_count, kColourSlots + 1); A(d.col_cursor, kColourSlots + 1);
   A(d.bmask, NB); A(d.bslot, NB);
   A(d.btab, (size_t)kBodyTab * NB); A(d.parent, NB); A(d.sepbad, 3 * (size_t)NB); A(d.jok, 3 * (size_t)NB); A(d.solved, NB);
-  A(d.mkey[0], NS); A(d.mkey[1], NS); A(d.mval[0], NS); A(d.mval[1], NS);
+  A(d.mkey[0], NS); A(d.mkey[1], NS); A(d.skey, NS); A(d.mval[0], NS); A(d.mval[1], NS);
   const size_t tiles = (size_t)(NS + kSortTile - 1) / kSortTile;
   A(d.sort_hist, kRadix * tiles);
   A(d.node_child, 2 * (size_t)NS); A(d.node_parent, NS); A(d.leaf_parent, NS); A(d.node_range, NS);
@@ -246,7 +250,7 @@ int b2g_batch::alloc() {
     A(d.sparse, (size_t)levels * NS);
   }
   A(d.pslots, (size_t)kPairSlots * NS); A(d.heavy, NS);
-  A(d.pcount, NS); A(d.pfound, NS); A(d.sinv, NS); A(d.poff, NS);
+  A(d.pcount, NS); A(d.pfound, NS); A(d.sinv, NS); A(d.poff, NS); A(d.poff_prev, NS);
   const size_t scan_n = std::max<size_t>(std::max<size_t>(NC, NS), kRadix * tiles);
   A(d.scan_tiles, scan_n / kScanTile + 2);
   // the counters and, behind them, the colour offsets: the step's one
@@ -289,9 +293,10 @@ void b2g_batch::release() {
                   d.jorder, d.wj_off, d.wj_list, d.jisland, d.bj_off, d.bj_other, d.ckey, d.cpair, d.cman, d.cmat, d.ccol,
                   d.nkey, d.npair, d.nman, d.nmat, d.ncol, d.touch, d.tpos, d.clist, d.cbody, d.ccolour,
                   d.csorted, d.cisland, d.cpend, d.events, d.col_count, d.col_cursor, d.bmask, d.bslot,
-                  d.btab, d.parent, d.sepbad, d.jok, d.solved, d.mkey[0], d.mkey[1], d.mval[0], d.mval[1],
+                  d.btab, d.parent, d.sepbad, d.jok, d.solved, d.mkey[0], d.mkey[1], d.skey, d.mval[0], d.mval[1],
                   d.sort_hist, d.node_child, d.node_parent, d.leaf_parent, d.node_range, d.node_rec,
-                  d.sfast, d.sparse, d.pslots, d.heavy, d.pcount, d.pfound, d.sinv, d.poff, d.scan_tiles, d.cnt};
+                  d.sfast, d.sparse, d.pslots, d.heavy, d.pcount, d.pfound, d.sinv, d.poff, d.poff_prev, d.scan_tiles,
+                  d.cnt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int k = 0; k < 10; ++k)
@@ -423,10 +428,18 @@ int b2g_batch::broadphase(bool use_given_aabbs) {
     hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, stream, d);
     return B2G_OK;
   }
-  const int gs = grid_for(ns);
+  // A world that is not active this step has not moved since the last
+  // broadphase (nothing was uploaded since, and that broadphase ran): its
+  // pairs are its segments of the current contact list, and on a refit its
+  // boxes, leaves and nodes are those of the last tree update.
+  const bool keep = keep_idle && pairs_current && !filters_dirty && !capturing && !use_given_aabbs;
   if (filters_dirty) refresh_filters();
+  d.idle = keep ? 1 : 0;
   int rc = build_tree(use_given_aabbs);
-  if (rc) return rc;
+  if (rc) {
+    d.idle = 0;
+    return rc;
+  }
   const int gq = grid_for((long long)ns * kBlock / kQueryBlock);  // blocks of kQueryBlock threads
   launch(B2G_KERNEL_PAIRS_WRITE, ns, k_lbvh_pairs, dim3(gq), dim3(kQueryBlock), 0, d);
   scan(d.pcount, nullptr, ns, d.poff, d.cnt + C_PAIRS, ns);
@@ -441,7 +454,10 @@ int b2g_batch::broadphase(bool use_given_aabbs) {
     hipLaunchKernelGGL(k_dead_contact_events, dim3(grid_for(std::max(est_contacts, 256))), dim3(kBlock), 0, stream,
                        d);
   hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, stream, d);
+  d.idle = 0;
   swap_contact_buffers();
+  // a captured broadphase has not run yet: the replay's caller sets this
+  if (!capturing) pairs_current = !use_given_aabbs;
   return B2G_OK;
 }
 
@@ -505,6 +521,7 @@ int b2g_batch::rebuild_joint_tables() {
   if (off[nb]) HIP_TRY(hipMemcpy(d.bj_other, other.data(), off[nb] * sizeof(int), hipMemcpyHostToDevice));
   joints_dirty = false;
   filters_dirty = true;
+  pairs_current = false;
   return B2G_OK;
 }
 
@@ -528,6 +545,7 @@ void b2g_batch::swap_contact_buffers() {
   std::swap(d.cman, d.nman);
   std::swap(d.cmat, d.nmat);
   std::swap(d.ccol, d.ncol);
+  std::swap(d.poff, d.poff_prev);  // the new list's segments
   parity ^= 1;
 }
 
@@ -850,7 +868,10 @@ int b2g_batch::step(int vi, int pi, unsigned flags) {
     round_host = 0;
   }
   d.events_on = (flags & B2G_STEP_EVENTS) ? 1 : 0;
-  if (flags & B2G_STEP_SKIP_BROADPHASE) tree_stale = true;  // bodies move without a rebuild
+  if (flags & B2G_STEP_SKIP_BROADPHASE) {  // bodies move without a rebuild or a pair query
+    tree_stale = true;
+    pairs_current = false;
+  }
   if (d.events_on) hipLaunchKernelGGL(k_events_reset, dim3(1), dim3(1), 0, stream, d);
   const bool prof_on = (flags & B2G_STEP_PROFILE) != 0;
   const bool graph_ok = (flags & B2G_STEP_GRAPH) && !prof_on && timed_kernel < 0 &&
@@ -1058,7 +1079,10 @@ int b2g_batch::graph_step(int vi, int pi, unsigned flags) {
   }
   HIP_TRY(hipGraphLaunch(ge->exec, stream));
   // mirror what the replay does to the host view
-  if (!(flags & B2G_STEP_SKIP_BROADPHASE)) swap_contact_buffers();
+  if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
+    swap_contact_buffers();
+    pairs_current = true;
+  }
   round_host += kGraphColourRounds;
   ++graph_steps;
   if (!(flags & B2G_STEP_ASYNC)) {
@@ -1171,8 +1195,9 @@ int b2g_set_world_params(b2g_batch* b, int32_t first, int32_t count, const b2g_w
   if (first + count > b->d.nw) {  // the world count is part of captured launches
     b->graphs_clear();
     b->graph_K = 0;
-  b->filters_dirty = true;
-  b->layout_dirty = true;
+    b->filters_dirty = true;
+    b->pairs_current = false;
+    b->layout_dirty = true;
   }
   std::vector<float4> wp(count);
   std::vector<int> wf(count);
@@ -1196,6 +1221,7 @@ int b2g_set_worlds(b2g_batch* b, int32_t count, const int32_t* body_begin, const
   b->graphs_clear();
   b->graph_K = 0;
   b->filters_dirty = true;
+  b->pairs_current = false;
   b->layout_dirty = true;
   b->d.nw = count;
   return b2g_set_world_params(b, 0, count, params);
@@ -1205,6 +1231,7 @@ int b2g_upload_bodies(b2g_batch* b, int32_t first, int32_t count, const b2g_body
   if (!b || first < 0 || count < 0 || first + count > b->cap.bodies) return fail(B2G_ERR_INVALID, "body range");
   if (count == 0) return B2G_OK;
   b->filters_dirty = true;  // body types feed the pair filter
+  b->pairs_current = false;
   b->tree_stale = true;
   if (first + count > b->d.nb) b->layout_dirty = true;
   std::vector<float4> xf(count), pos(count), vel(count), prop(count), prop2(count), force(count);
@@ -1239,6 +1266,7 @@ int b2g_upload_bodies(b2g_batch* b, int32_t first, int32_t count, const b2g_body
     b->graphs_clear();
     b->graph_K = 0;
   b->filters_dirty = true;
+  b->pairs_current = false;
   b->layout_dirty = true;
   }
   if (first + count > d.nb) {
@@ -1261,6 +1289,7 @@ int b2g_upload_shapes(b2g_batch* b, int32_t first, int32_t count, const b2g_shap
   b->graphs_clear();  // captured launch sizes no longer match
   b->graph_K = 0;
   b->filters_dirty = true;
+  b->pairs_current = false;
   b->layout_dirty = true;
   if (count == 0) return B2G_OK;
   std::vector<int4> hdr(count), filt(count);
@@ -1307,6 +1336,7 @@ int b2g_upload_joints(b2g_batch* b, int32_t first, int32_t count, const b2g_revo
   b->graphs_clear();  // captured launch sizes no longer match
   b->graph_K = 0;
   b->filters_dirty = true;
+  b->pairs_current = false;
   b->layout_dirty = true;
   if (count == 0) return B2G_OK;
   if ((int)b->h_joints.size() < first + count) b->h_joints.resize(first + count);
@@ -1340,6 +1370,7 @@ int b2g_upload_contacts(b2g_batch* b, int32_t count, const b2g_contact* in) {
   b->graphs_clear();  // captured launch sizes no longer match
   b->graph_K = 0;
   b->filters_dirty = true;
+  b->pairs_current = false;
   b->layout_dirty = true;
   std::vector<int> idx(count);
   std::vector<uint64_t> keys(count);
@@ -1432,6 +1463,10 @@ int b2g_set_option(b2g_batch* b, int32_t option, int32_t value) {
       b->tuned_block = 0;
       b->tune_step = 0;
       break;
+    case B2G_OPT_KEEP_IDLE:
+      if (value < 0 || value > 1) return fail(B2G_ERR_INVALID, "keep idle: 0 or 1");
+      b->keep_idle = value != 0;
+      break;
     default:
       return fail(B2G_ERR_INVALID, "unknown option");
   }
@@ -1453,6 +1488,7 @@ int b2g_get_option(b2g_batch* b, int32_t option, int32_t* value) {
     case B2G_OPT_PER_WORLD_ELIGIBLE: *value = b->per_world_ok ? 1 : 0; break;
     case B2G_OPT_WORLD_LDS: *value = b->world_lds; break;
     case B2G_OPT_TUNED_WORLD_LDS: *value = b->world_lds_now() ? 1 : 0; break;
+    case B2G_OPT_KEEP_IDLE: *value = b->keep_idle ? 1 : 0; break;
     default: return fail(B2G_ERR_INVALID, "unknown option");
   }
   return B2G_OK;

@@ -150,6 +150,12 @@ struct Dev {
   int* pfound;       // per shape: pairs its traversal found (own + those of a static partner);
                      // for a static shape, the fill cursor of its segment
   int* poff;         // exclusive scan
+  int* poff_prev;    // the last broadphase's poff: shape s's segment of the current contact list is
+                     // [poff_prev[s], poff_prev[s + 1]) (the contact list is that broadphase's pair list)
+  int idle;          // 1: the current contact list is the last broadphase's pair list and no body,
+                     // shape, joint or contact was uploaded since; a world that is not active then
+                     // keeps its pairs, its boxes and its part of the tree (leaf order, nodes, table)
+  uint64_t* skey;    // per shape: its Morton key at its world's last tree build
   int* scan_tiles;
   int* cnt;
 };
@@ -1969,6 +1975,21 @@ __global__ void __launch_bounds__(1024) k_solve_worlds(Dev d, int vi, int pi, in
 // filtered as b2ContactManager::QueryCallback filters it,
 // src/dynamics/b2_contact_manager.cpp:136-188).
 // ===========================================================================
+// Worlds that sit out a step. With d.idle set, such a world has not moved
+// since the last broadphase and nothing was uploaded since: its pairs are its
+// segments of the current contact list, and its boxes, its leaf order (the
+// keys of its last build, which the stable sort puts back in the same order),
+// its internal nodes and its sparse-table entries all stand.
+__device__ __forceinline__ bool idle_world(const Dev& d, int w) { return d.idle && !world_active(d, w); }
+// Shape s's segment [first, end) of the current contact list, which is the
+// last broadphase's pair list (valid while d.idle is set).
+__device__ __forceinline__ int2 prev_segment(const Dev& d, int s) {
+  const int n = d.cnt[C_CONTACTS];  // the current list (its count is replaced after the merge)
+  const int first = min(d.poff_prev[s], n);
+  const int end = s + 1 < d.ns ? min(d.poff_prev[s + 1], n) : n;
+  return make_int2(first, end);
+}
+
 __global__ void __launch_bounds__(kBlock) k_world_bounds_reset(Dev d) {
   B2G_FOR(w, 0, d.nw) {
     d.wbounds[4 * w + 0] = 0xffffffffu;
@@ -2027,28 +2048,36 @@ __global__ void __launch_bounds__(kBlock) k_aabb(Dev d, int bounds, int finish) 
   B2G_FOR_ALL(i, valid, 0, d.ns) {
     int w = 0;
     float cx = 0.0f, cy = 0.0f;
+    bool live = valid;
     if (valid) {
       const int4 h = d.shdr[i];
-      const float4 xf = finish && synced_body(d, h.y) ? synced_xf(d, h.y) : d.xf[h.y];
-      const float4 bb = shape_aabb(h.x, h.z, d.geo + 9 * (size_t)i, xform_of(xf));
-      d.aabb[i] = bb;
-      if (!bounds) {
-        d.sparse[d.sinv[i]] = bb;
-        d.pcount[i] = 0;
+      if (idle_world(d, h.w)) {
+        // box, leaf and tree stand; the pair count is the old segment's length
+        const int2 seg = prev_segment(d, i);
+        d.pcount[i] = seg.y - seg.x;
+        live = false;
+      } else {
+        const float4 xf = finish && synced_body(d, h.y) ? synced_xf(d, h.y) : d.xf[h.y];
+        const float4 bb = shape_aabb(h.x, h.z, d.geo + 9 * (size_t)i, xform_of(xf));
+        d.aabb[i] = bb;
+        if (!bounds) {
+          d.sparse[d.sinv[i]] = bb;
+          d.pcount[i] = 0;
+        }
+        cx = bb.x + bb.z;
+        cy = bb.y + bb.w;
+        w = h.w;
       }
-      cx = bb.x + bb.z;
-      cy = bb.y + bb.w;
-      w = h.w;
     }
     if (!bounds) continue;
     unsigned* wb = d.wbounds;
-    const uint64_t act = __ballot(valid);
+    const uint64_t act = __ballot(live);
     const int first = act ? __ffsll((long long)act) - 1 : 0;
     const int key0 = __shfl(w, first, kWave);
-    const bool uniform = __ballot(valid && w != key0) == 0;
-    const float x0 = wave_min_all(valid ? cx : FLT_MAX), y0 = wave_min_all(valid ? cy : FLT_MAX);
-    const float x1 = wave_max_all(valid ? cx : -FLT_MAX), y1 = wave_max_all(valid ? cy : -FLT_MAX);
-    if (!uniform && valid) {  // several worlds in this wave
+    const bool uniform = __ballot(live && w != key0) == 0;
+    const float x0 = wave_min_all(live ? cx : FLT_MAX), y0 = wave_min_all(live ? cy : FLT_MAX);
+    const float x1 = wave_max_all(live ? cx : -FLT_MAX), y1 = wave_max_all(live ? cy : -FLT_MAX);
+    if (!uniform && live) {  // several worlds in this wave
       atomicMin(wb + 4 * w + 0, float_to_ordered(cx));
       atomicMin(wb + 4 * w + 1, float_to_ordered(cy));
       atomicMax(wb + 4 * w + 2, float_to_ordered(cx));
@@ -2112,6 +2141,11 @@ __device__ __forceinline__ uint32_t spread15(uint32_t x) {
 __global__ void __launch_bounds__(kBlock) k_morton(Dev d) {
   B2G_FOR(i, 0, d.ns) {
     const int4 h = d.shdr[i];
+    if (idle_world(d, h.w)) {  // the key of its world's last build
+      d.mkey[0][i] = d.skey[i];
+      d.mval[0][i] = (uint32_t)i;
+      continue;
+    }
     const float4 bb = d.aabb[i];
     const unsigned* wb = d.wbounds + 4 * h.w;
     const float x0 = ordered_to_float(wb[0]), y0 = ordered_to_float(wb[1]);
@@ -2127,7 +2161,9 @@ __global__ void __launch_bounds__(kBlock) k_morton(Dev d) {
     // otherwise enlarge every node on its path, and every query would walk
     // that path
     if ((bb.z - bb.x) * 8.0f > x1 - x0 || (bb.w - bb.y) * 8.0f > y1 - y0) code |= 1u << 30;
-    d.mkey[0][i] = ((uint64_t)(uint32_t)h.w << 32) | code;
+    const uint64_t key = ((uint64_t)(uint32_t)h.w << 32) | code;
+    d.mkey[0][i] = key;
+    d.skey[i] = key;
     d.mval[0][i] = (uint32_t)i;
   }
 }
@@ -2143,6 +2179,16 @@ __device__ __forceinline__ int lbvh_delta(const uint64_t* __restrict__ k, int n,
 __global__ void __launch_bounds__(kBlock) k_lbvh_build(Dev d, const uint64_t* __restrict__ keys) {
   const int n = d.ns;
   B2G_FOR(i, 0, n - 1) {
+    // A node whose neighbours on both sides are leaves of its own world lies
+    // inside that world's subtree: world bits lead the key, so every
+    // cross-world prefix is shorter than any prefix inside a world, and the
+    // node depends only on its world's keys. An idle world's keys are those
+    // of its last build, so its nodes stand. (The world's two end positions
+    // hold its root and a node joining worlds; they are rebuilt.)
+    if (d.idle && i > 0) {
+      const uint64_t w = keys[i] >> 32;
+      if ((keys[i - 1] >> 32) == w && (keys[i + 1] >> 32) == w && !world_active(d, (int)w)) continue;
+    }
     const int dir = (lbvh_delta(keys, n, i, i + 1) - lbvh_delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
     const int dmin = lbvh_delta(keys, n, i, i - dir);
     int lmax = 2;
@@ -2176,6 +2222,7 @@ __global__ void __launch_bounds__(kBlock) k_world_leaves(Dev d, const uint64_t* 
   const int n = d.ns;
   B2G_FOR(k, 0, n) {
     const int w = (int)(keys[k] >> 32);
+    if (idle_world(d, w)) continue;  // its range and root stand
     if (k == 0 || (int)(keys[k - 1] >> 32) != w) {
       d.wleaf[2 * w] = k;
       d.wroot[w] = -1;  // k_node_records finds the world's subtree
@@ -2199,6 +2246,7 @@ __device__ __forceinline__ float4 aabb_union(float4 a, float4 b) {
 __global__ void __launch_bounds__(kBlock) k_leaf_aabbs(Dev d, const uint32_t* __restrict__ vals) {
   if (blockIdx.x == 0 && threadIdx.x == 0) d.cnt[C_HEAVY] = 0;
   B2G_FOR(k, 0, d.ns) {
+    if (d.idle && !world_active(d, (int)(d.mkey[0][k] >> 32))) continue;  // leaf stands (k_aabb set its count)
     const uint32_t i = vals[k];
     d.sparse[k] = d.aabb[i];
     d.pcount[i] = 0;
@@ -2214,6 +2262,9 @@ __global__ void __launch_bounds__(kBlock) k_sparse_levels(Dev d, int j) {
   const int h = 1 << (j - 1);
   const float4* prev = d.sparse + (size_t)(j - 1) * n;
   B2G_FOR(i, 0, n) {
+    // an idle world's entries stand (those that reach past the world's last
+    // leaf are never read)
+    if (d.idle && !world_active(d, (int)(d.mkey[0][i] >> 32))) continue;
     // level j + lv is the union of 2^(lv + 1) entries of level j - 1, h apart
     float4 u = prev[i];
     int have = 1;
@@ -2245,6 +2296,10 @@ __global__ void __launch_bounds__(kBlock) k_node_records(Dev d, const uint64_t* 
   const int n = d.ns;
   B2G_FOR(i, 0, n - 1) {
     const int2 r = d.node_range[i];
+    if (d.idle) {  // the nodes inside an idle world stand
+      const int w = (int)(keys[r.x] >> 32);
+      if ((int)(keys[r.y] >> 32) == w && !world_active(d, w)) continue;
+    }
     const int lc = d.node_child[2 * i], rc = d.node_child[2 * i + 1];
     const int split = lc < 0 ? ~lc : d.node_range[lc].y;
     const float4 lb = range_box(d, keys, n, r.x, split);
@@ -2405,6 +2460,8 @@ __global__ void __launch_bounds__(kQueryBlock) k_lbvh_pairs(Dev d) {
   __shared__ uint64_t slots[kPairSlots * kQueryBlock];  // thread-minor, like the stack
   B2G_FOR_ALL(t, valid, 0, d.ns) {
     if (valid) {
+      // an idle world's shapes keep their segments (k_aabb set their counts)
+      if (d.idle && !world_active(d, (int)(d.mkey[0][t] >> 32))) continue;
       const int i = (int)d.mval[0][t];
       const int tid = (int)threadIdx.x;
       int found = 0, own = 0;
@@ -2445,6 +2502,11 @@ __global__ void __launch_bounds__(kQueryBlock) k_pairs_compact(Dev d) {
       const bool stat = (d.sfast[i].y & 64) != 0;
       if (base + own > d.ccap) {
         atomicOr(&d.cnt[C_STATUS], S_CONTACT_OVERFLOW);
+        continue;
+      }
+      if (idle_world(d, d.shdr[i].w)) {  // the segment it had, already sorted (own is its length)
+        const int from = prev_segment(d, i).x;
+        for (int k = 0; k < own; ++k) d.nkey[base + k] = d.ckey[from + k];
         continue;
       }
       auto foreign = [&](uint64_t key, int s) {
@@ -2677,6 +2739,11 @@ __global__ void __launch_bounds__(kBlock) k_merge_contacts(Dev d) {
   B2G_FOR(p, 0, n) {
     const uint64_t key = d.nkey[p];
     int lo = 0, hi = old_count;
+    if (d.idle) {  // the old list is a pair list too: only the lower shape's segment can hold the key
+      const int2 seg = prev_segment(d, (int)(key >> 32));
+      lo = seg.x;
+      hi = seg.y;
+    }
     while (lo < hi) {
       const int mid = (lo + hi) >> 1;
       if (d.ckey[mid] < key) lo = mid + 1; else hi = mid;

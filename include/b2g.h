@@ -277,7 +277,14 @@ enum {
   B2G_OPT_LAST_SOLVER = 4,
   B2G_OPT_PER_WORLD_ELIGIBLE = 5,
   B2G_OPT_WORLD_LDS = 6,        /* -1 autotune (default), 0 off, 1 stage each world's bodies in LDS */
-  B2G_OPT_TUNED_WORLD_LDS = 7   /* read-only: whether the per-world solver stages bodies in LDS */
+  B2G_OPT_TUNED_WORLD_LDS = 7,  /* read-only: whether the per-world solver stages bodies in LDS */
+  /* 1 (default): a world that is not active in a step keeps its pairs and,
+   * on the steps that refit the tree, its boxes and tree nodes, instead of
+   * querying and refitting again (its bodies have not moved). The contact
+   * list is the same either way; 0 queries every world each step. Any
+   * upload, and a step with B2G_STEP_SKIP_BROADPHASE, makes the next
+   * broadphase query every world. */
+  B2G_OPT_KEEP_IDLE = 8
 };
 int b2g_set_option(b2g_batch* batch, int32_t option, int32_t value);
 int b2g_get_option(b2g_batch* batch, int32_t option, int32_t* value);

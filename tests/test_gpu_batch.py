@@ -103,6 +103,57 @@ def test_gpu_per_world_solver_with_large_worlds_bitexact(engine):
         assert same.all(), sc.name
 
 
+@pytest.mark.parametrize("solver", ["colours", "worlds"])
+def test_gpu_idle_worlds_keep_pairs_bitexact(engine, solver):
+    """A world that sits out a step keeps its pairs (its segments of the
+    contact list), its boxes and its part of the tree (leaf order, nodes,
+    range table) instead of being queried and rebuilt again
+    (B2G_OPT_KEEP_IDLE). On a schedule where worlds go idle for several steps
+    (across tree rebuilds) and come back, the contact list after every step,
+    the final bodies and AABB queries on the final tree equal, bit for bit,
+    those of the same schedule with every world queried and rebuilt each
+    step."""
+    scenes = [sc for sc in _mixed_scenes() if solver == "colours" or sc.joint_count == 0]
+    steps = 24
+    rng = np.random.default_rng(7)
+    sched = rng.random((steps, len(scenes))) < 0.45
+    sched[:3, 0] = True
+    sched[3:12, 0] = False  # world 0: idle for 9 steps, then back
+    dt = np.float32(1.0 / 60.0)
+    ratio = np.float32(np.float32(np.float32(1.0) / dt) * dt)
+    opt = engine.SOLVER_WORLDS if solver == "worlds" else engine.SOLVER_COLOURS
+    runs = []
+    for keep in (1, 0):
+        b = engine.Batch.from_scenes(scenes)
+        b.set_option(engine.OPT_SOLVER, opt)
+        b.set_option(engine.OPT_KEEP_IDLE, keep)
+        assert b.get_option(engine.OPT_KEEP_IDLE) == keep
+        b.advance(2)
+        lists = []
+        for t in range(steps):
+            b.set_world_params(0, [engine.WorldParams(g, dt, ratio, 1, int(sched[t, w]))
+                                   for w, g in enumerate(b.gravity)])
+            b.step(8, 3, 0)
+            lists.append(b.contacts())
+        # the kept tree answers queries like a rebuilt one: boxes over each world
+        qrng = np.random.default_rng(8)
+        qw = np.repeat(np.arange(len(scenes), dtype=np.int32), 16)
+        lo = qrng.uniform(-15.0, 15.0, (len(qw), 2)).astype(np.float32)
+        boxes = np.concatenate([lo, lo + qrng.uniform(0.5, 8.0, (len(qw), 2)).astype(np.float32)], axis=1)
+        found = [np.sort(r) for r in b.query_aabbs(qw, boxes)]
+        runs.append((lists, b.bodies(), found))
+    (kl, kb, kq), (fl, fb, fq) = runs
+    for t in range(steps):
+        assert len(kl[t]) == len(fl[t]) and len(kl[t]) > 0, t
+        assert kl[t].tobytes() == fl[t].tobytes(), f"contact list differs after step {t}"
+    assert (_state(kb) == _state(fb)).all()
+    assert sum(len(r) for r in kq) > 0
+    for a, c in zip(kq, fq):
+        assert np.array_equal(a, c)
+    print(f"{len(scenes)} worlds, {sched.sum()} world-steps of {sched.size} active, "
+          f"{len(kl[-1])} contacts at the end", flush=True)
+
+
 def test_gpu_batch_world_matches_drop_in_world(engine):
     """A batch of one built by b2a_batch_from_scenes steps exactly like the
     drop-in b2World built from the same scene."""
