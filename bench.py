@@ -14,9 +14,12 @@ Headline workload: the large random pile.
 Further sections of the same JSON line:
 - `gas_chains`: 100k circles + 500 revolute chains, one world.
 - `batched`: 4096 worlds, about 1.14M bodies, in one launch per stage,
-  against a plain per-world loop.
-- `serving`: Poisson requests against a resident pool; p50/p99 latency
-  beside throughput.
+  against a plain per-world loop; also the whole step's algorithmic HBM
+  bytes over its time (`roofline_step`) and the reference CPU Step on a
+  sample of the worlds, on one core and on all host cores.
+- `serving`: Poisson requests (20k/s, 1-8 substeps at dt 1/60 or 1/240)
+  against a resident pool of 10,000 worlds (16-4096 bodies, log-uniform)
+  under a 64 GB memory budget; p50/p99 latency beside throughput.
 
 roofline: HIP events around each run of consecutive launches of the
 dominant kernel (one sweep's per-colour launches), over a further measured
@@ -349,7 +352,7 @@ def main():
     ap.add_argument("--batch-worlds", type=int, default=4096, help="0 skips the batched section")
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--batch-sample", type=int, default=64, help="per-world loop / CPU sample stride")
-    ap.add_argument("--serve-worlds", type=int, default=1000, help="0 skips the serving section")
+    ap.add_argument("--serve-worlds", type=int, default=10000, help="0 skips the serving section")
     ap.add_argument("--gas-steps", type=int, default=100, help="0 skips the gas + chains section")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration) and exit")

@@ -262,7 +262,18 @@ int b2g_batch::alloc() {
   d.ecap = NC;
   HIP_TRY(hipHostMalloc((void**)&pinned, 256 * sizeof(int), hipHostMallocDefault));
   HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+  {
+    // The side stream runs beside the main one (island labels beside the
+    // contact set-up, the small worlds' solver beside the big worlds'). A
+    // process gets a few hardware queues that its streams share, and two
+    // streams on one queue run one after the other; a stream created with a
+    // CU mask gets a queue of its own, and this mask names every CU.
+    int dev = 0, cus = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    std::vector<uint32_t> all_cus((size_t)std::max(1, (cus + 31) / 32), 0xffffffffu);
+    HIP_TRY(hipExtStreamCreateWithCUMask(&side, (uint32_t)all_cus.size(), all_cus.data()));
+  }
   HIP_TRY(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
   for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
