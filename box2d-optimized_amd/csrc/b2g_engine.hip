@@ -77,6 +77,7 @@ struct b2g_batch {
   std::vector<int> h_shape_body;
   std::vector<b2g_shape> h_shape_rec;  // uploaded shape records (snapshots)
   std::vector<int> h_wshape0;   // first shape of each world
+  std::vector<int> h_wshape1;   // one past its last shape (world layout)
   std::vector<b2g_revolute> h_joints;
   std::vector<int> h_jcol_off;  // joint colour ranges (host colouring)
   std::vector<b2g_world_params> h_params;
@@ -195,7 +196,13 @@ struct b2g_batch {
   int eager_step(int vi, int pi, unsigned flags);
   int graph_step(int vi, int pi, unsigned flags);
   int scan(const int* in, const int* n_dev, int n_host, int* out, int* total, long long n_bound);
-  int radix_sort(int n, int bits);
+  int radix_sort(int n, int bits, uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1);
+  // worlds that are not active in the coming step (host view of the flags)
+  int idle_worlds() const {
+    int n = 0;
+    for (int w = 0; w < d.nw; ++w) n += h_params[w].active ? 0 : 1;
+    return n;
+  }
   int build_tree(bool use_given_aabbs);
   int broadphase(bool use_given_aabbs);
   void refresh_filters();
@@ -223,7 +230,7 @@ int b2g_batch::alloc() {
   if ((rc = dalloc(&(ptr), (size_t)(n))) != B2G_OK) return rc
   A(d.xf, NB); A(d.pos, NB); A(d.vel, NB); A(d.prop, NB); A(d.prop2, NB); A(d.force, NB);
   A(d.btype, NB); A(d.bworld, NB);
-  A(d.wpar, NW); A(d.wflag, NW); A(d.wshape0, NW); A(d.wshape1, NW); A(d.wbody, NW); A(d.worder, NW);
+  A(d.wpar, NW); A(d.wflag, NW); A(d.wsort, NW); A(d.wshape0, NW); A(d.wshape1, NW); A(d.wbody, NW); A(d.worder, NW);
   A(d.wcol, (size_t)NW * (kColourSlots + 1));
   A(d.wcolmax, NW); A(d.wfresh, NW); A(d.wrecolour, NW); A(d.wbounds, 4 * NW); A(d.wleaf, 2 * NW); A(d.wroot, NW);
   A(d.shdr, NS); A(d.geo, 9 * (size_t)NS); A(d.smat, NS); A(d.sfilt, NS); A(d.aabb, NS);
@@ -238,7 +245,7 @@ int b2g_batch::alloc() {
   A(d.col_count, kColourSlots + 1); A(d.col_cursor, kColourSlots + 1);
   A(d.bmask, NB); A(d.bslot, NB);
   A(d.btab, (size_t)kBodyTab * NB); A(d.parent, NB); A(d.sepbad, 3 * (size_t)NB); A(d.jok, 3 * (size_t)NB); A(d.solved, NB);
-  A(d.mkey[0], NS); A(d.mkey[1], NS); A(d.skey, NS); A(d.mval[0], NS); A(d.mval[1], NS);
+  A(d.mkey[0], NS); A(d.mkey[1], NS); A(d.skey, NS); A(d.xkey, NS); A(d.xval, NS); A(d.mval[0], NS); A(d.mval[1], NS);
   const size_t tiles = (size_t)(NS + kSortTile - 1) / kSortTile;
   A(d.sort_hist, kRadix * tiles);
   A(d.node_child, 2 * (size_t)NS); A(d.node_parent, NS); A(d.leaf_parent, NS); A(d.node_range, NS);
@@ -304,7 +311,7 @@ void b2g_batch::release() {
                   d.jorder, d.wj_off, d.wj_list, d.jisland, d.bj_off, d.bj_other, d.ckey, d.cpair, d.cman, d.cmat, d.ccol,
                   d.nkey, d.npair, d.nman, d.nmat, d.ncol, d.touch, d.tpos, d.clist, d.cbody, d.ccolour,
                   d.csorted, d.cisland, d.cpend, d.events, d.col_count, d.col_cursor, d.bmask, d.bslot,
-                  d.btab, d.parent, d.sepbad, d.jok, d.solved, d.mkey[0], d.mkey[1], d.skey, d.mval[0], d.mval[1],
+                  d.btab, d.parent, d.sepbad, d.jok, d.solved, d.mkey[0], d.mkey[1], d.skey, d.xkey, d.xval, d.wsort, d.mval[0], d.mval[1],
                   d.sort_hist, d.node_child, d.node_parent, d.leaf_parent, d.node_range, d.node_rec,
                   d.sfast, d.sparse, d.pslots, d.heavy, d.pcount, d.pfound, d.sinv, d.poff, d.poff_prev, d.scan_tiles,
                   d.cnt};
@@ -341,15 +348,19 @@ int b2g_batch::scan(const int* in, const int* n_dev, int n_host, int* out, int* 
   return B2G_OK;
 }
 
-int b2g_batch::radix_sort(int n, int bits) {
+// LSD radix sort of n (key, value) pairs starting in (k0, v0), ping-ponging
+// with (k1, v1)
+int b2g_batch::radix_sort(int n, int bits, uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1) {
+  uint64_t* keys[2] = {k0, k1};
+  uint32_t* vals[2] = {v0, v1};
   const int tiles = (n + kSortTile - 1) / kSortTile;
   const int g = std::min(tiles, 4096);
   int src = 0;
   for (int shift = 0; shift < bits; shift += kRadixBits) {
-    hipLaunchKernelGGL(k_radix_hist, dim3(g), dim3(kBlock), 0, stream, d.mkey[src], n, shift, d.sort_hist);
+    hipLaunchKernelGGL(k_radix_hist, dim3(g), dim3(kBlock), 0, stream, keys[src], n, shift, d.sort_hist);
     scan(d.sort_hist, nullptr, kRadix * tiles, d.sort_hist, nullptr, (long long)kRadix * tiles);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(g), dim3(kBlock), 0, stream, d.mkey[src], d.mval[src], n, shift,
-                       d.sort_hist, d.mkey[1 - src], d.mval[1 - src]);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(g), dim3(kBlock), 0, stream, keys[src], vals[src], n, shift,
+                       d.sort_hist, keys[1 - src], vals[1 - src]);
     src = 1 - src;
   }
   return src;  // buffer holding the result (0 for an even number of passes)
@@ -393,14 +404,35 @@ int b2g_batch::build_tree(bool use_given_aabbs) {
     hipLaunchKernelGGL(k_world_bounds_from_aabbs, dim3(gs), dim3(kBlock), 0, stream, d);
   }
   if (!refit) {
-    hipLaunchKernelGGL(k_morton, dim3(gs), dim3(kBlock), 0, stream, d);
+    // With idle worlds kept (their leaf order stands), only the active
+    // worlds' keys are sorted, then placed back at their worlds' leaf ranges.
+    long long m = ns;
+    if (d.idle && worlds_contiguous && idle_worlds() > 0) {
+      m = 0;
+      for (int w = 0; w < d.nw && w < (int)h_wshape1.size(); ++w)
+        if (h_params[w].active) m += h_wshape1[w] - h_wshape0[w];
+    }
+    const bool active_only = m < ns;
+    if (active_only) {
+      hipLaunchKernelGGL(k_active_world_sizes, dim3(grid_for(d.nw)), dim3(kBlock), 0, stream, d);
+      scan(d.wsort, nullptr, d.nw, d.wsort, nullptr, d.nw);
+    }
+    hipLaunchKernelGGL(k_morton, dim3(gs), dim3(kBlock), 0, stream, d, active_only ? 1 : 0);
     int bits = 32;
     int wbits = 0;
     while ((1 << wbits) < d.nw) ++wbits;
     bits += wbits;
     bits = ((bits + 2 * kRadixBits - 1) / (2 * kRadixBits)) * (2 * kRadixBits);  // even pass count
-    const int res = radix_sort(ns, bits);
-    if (res != 0) return fail(B2G_ERR_INVALID, "radix sort ended in the wrong buffer");
+    if (active_only) {
+      if (m > 0) {
+        const int res = radix_sort((int)m, bits, d.mkey[1], d.mval[1], d.xkey, d.xval);
+        if (res != 0) return fail(B2G_ERR_INVALID, "radix sort ended in the wrong buffer");
+        hipLaunchKernelGGL(k_place_active_leaves, dim3(grid_for(m)), dim3(kBlock), 0, stream, d, (int)m);
+      }
+    } else {
+      const int res = radix_sort(ns, bits, d.mkey[0], d.mval[0], d.mkey[1], d.mval[1]);
+      if (res != 0) return fail(B2G_ERR_INVALID, "radix sort ended in the wrong buffer");
+    }
     hipLaunchKernelGGL(k_world_leaves, dim3(gs), dim3(kBlock), 0, stream, d, d.mkey[0]);
     if (ns >= 2) hipLaunchKernelGGL(k_lbvh_build, dim3(grid_for(ns - 1)), dim3(kBlock), 0, stream, d, d.mkey[0]);
   }
@@ -443,7 +475,8 @@ int b2g_batch::broadphase(bool use_given_aabbs) {
   // broadphase (nothing was uploaded since, and that broadphase ran): its
   // pairs are its segments of the current contact list, and on a refit its
   // boxes, leaves and nodes are those of the last tree update.
-  const bool keep = keep_idle && pairs_current && !filters_dirty && !capturing && !use_given_aabbs;
+  const bool keep =
+      keep_idle && pairs_current && worlds_contiguous && !filters_dirty && !capturing && !use_given_aabbs;
   if (filters_dirty) refresh_filters();
   d.idle = keep ? 1 : 0;
   int rc = build_tree(use_given_aabbs);
@@ -455,6 +488,8 @@ int b2g_batch::broadphase(bool use_given_aabbs) {
   launch(B2G_KERNEL_PAIRS_WRITE, ns, k_lbvh_pairs, dim3(gq), dim3(kQueryBlock), 0, d);
   scan(d.pcount, nullptr, ns, d.poff, d.cnt + C_PAIRS, ns);
   hipLaunchKernelGGL(k_pairs_compact, dim3(gq), dim3(kQueryBlock), 0, stream, d);
+  if (d.idle && idle_worlds() > 0)
+    hipLaunchKernelGGL(k_keep_idle_pairs, dim3(std::min(d.nw, 4096)), dim3(kBlock), 0, stream, d);
   // one block per segment to sort: static shapes' segments, plus a few
   // over-full dynamic shapes
   hipLaunchKernelGGL(k_sort_pair_segments, dim3(std::min(std::max(static_shapes + 32, 64), 4096)), dim3(kBlock), 0,
@@ -783,6 +818,7 @@ int b2g_batch::update_world_layout() {
   for (int w = 0; w < nw; ++w) {
     if (bc[w] && b1[w] - b0[w] + 1 != bc[w]) ok = false;
     if (sc[w] && s1[w] - s0[w] + 1 != sc[w]) ok = false;
+    if (sc[w] && (w >= (int)h_wshape0.size() || h_wshape0[w] != s0[w])) ok = false;  // as uploaded
     wb[w] = bc[w] ? make_int2(b0[w], b1[w] + 1) : make_int2(0, 0);
     ws1[w] = sc[w] ? s1[w] + 1 : h_wshape0[w];
     max_world_bodies = std::max(max_world_bodies, bc[w]);
@@ -799,6 +835,7 @@ int b2g_batch::update_world_layout() {
   }
   HIP_TRY(hipMemcpy(d.wbody, wb.data(), nw * sizeof(int2), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(d.wshape1, ws1.data(), nw * sizeof(int), hipMemcpyHostToDevice));
+  h_wshape1 = ws1;
   HIP_TRY(hipMemcpy(d.worder, order.data(), nw * sizeof(int), hipMemcpyHostToDevice));
   worlds_contiguous = ok;
   per_world_ok = ok && d.nj == 0;

@@ -156,6 +156,10 @@ struct Dev {
                      // shape, joint or contact was uploaded since; a world that is not active then
                      // keeps its pairs, its boxes and its part of the tree (leaf order, nodes, table)
   uint64_t* skey;    // per shape: its Morton key at its world's last tree build
+  uint64_t* xkey;    // second buffer of the active shapes' key sort (the first is mkey[1])
+  uint32_t* xval;
+  int* wsort;        // per world: its shape count if active, scanned into the first position of its
+                     // keys among the active shapes' keys
   int* scan_tiles;
   int* cnt;
 };
@@ -2138,12 +2142,17 @@ __device__ __forceinline__ uint32_t spread15(uint32_t x) {
   return x;
 }
 
-__global__ void __launch_bounds__(kBlock) k_morton(Dev d) {
+// With `compact`, only the active worlds' shapes are sorted: each writes its
+// key to its world's block of the active keys (mkey[1]; the layout is world
+// after world), and k_place_active_leaves puts the sorted blocks back.
+__global__ void __launch_bounds__(kBlock) k_morton(Dev d, int compact) {
   B2G_FOR(i, 0, d.ns) {
     const int4 h = d.shdr[i];
     if (idle_world(d, h.w)) {  // the key of its world's last build
-      d.mkey[0][i] = d.skey[i];
-      d.mval[0][i] = (uint32_t)i;
+      if (!compact) {
+        d.mkey[0][i] = d.skey[i];
+        d.mval[0][i] = (uint32_t)i;
+      }
       continue;
     }
     const float4 bb = d.aabb[i];
@@ -2162,9 +2171,34 @@ __global__ void __launch_bounds__(kBlock) k_morton(Dev d) {
     // that path
     if ((bb.z - bb.x) * 8.0f > x1 - x0 || (bb.w - bb.y) * 8.0f > y1 - y0) code |= 1u << 30;
     const uint64_t key = ((uint64_t)(uint32_t)h.w << 32) | code;
-    d.mkey[0][i] = key;
     d.skey[i] = key;
-    d.mval[0][i] = (uint32_t)i;
+    if (compact) {
+      const int c = d.wsort[h.w] + (i - d.wshape0[h.w]);
+      d.mkey[1][c] = key;
+      d.mval[1][c] = (uint32_t)i;
+    } else {
+      d.mkey[0][i] = key;
+      d.mval[0][i] = (uint32_t)i;
+    }
+  }
+}
+
+// per world: its shape count if active, else 0 (scanned next)
+__global__ void __launch_bounds__(kBlock) k_active_world_sizes(Dev d) {
+  B2G_FOR(w, 0, d.nw) d.wsort[w] = world_active(d, w) ? d.wshape1[w] - d.wshape0[w] : 0;
+}
+
+// The sorted active keys back into the sorted order: world w's leaves are
+// the sorted positions [wleaf[2w], wleaf[2w + 1]), a range fixed while the
+// layout stands, and the sort kept each world's keys together and in order.
+__global__ void __launch_bounds__(kBlock) k_place_active_leaves(Dev d, int m) {
+  B2G_FOR(k, 0, m) {
+    const uint64_t key = d.mkey[1][k];
+    const int w = (int)(key >> 32);
+    const int pos = d.wleaf[2 * w] + (k - d.wsort[w]);
+    if (pos < d.wleaf[2 * w] || pos >= d.wleaf[2 * w + 1]) continue;  // (cannot happen)
+    d.mkey[0][pos] = key;
+    d.mval[0][pos] = d.mval[1][k];
   }
 }
 
@@ -2495,18 +2529,16 @@ __global__ void __launch_bounds__(kQueryBlock) k_lbvh_pairs(Dev d) {
 // Segments filled in traversal or cursor order are queued for sorting.
 __global__ void __launch_bounds__(kQueryBlock) k_pairs_compact(Dev d) {
   __shared__ int stk[kStack * kQueryBlock];
-  B2G_FOR_ALL(i, valid, 0, d.ns) {
+  B2G_FOR_ALL(t, valid, 0, d.ns) {
     if (valid) {
+      // an idle world's block of pairs is copied whole (k_keep_idle_pairs)
+      if (d.idle && !world_active(d, (int)(d.mkey[0][t] >> 32))) continue;
+      const int i = (int)d.mval[0][t];
       const int own = d.pcount[i];
       const int base = d.poff[i];
       const bool stat = (d.sfast[i].y & 64) != 0;
       if (base + own > d.ccap) {
         atomicOr(&d.cnt[C_STATUS], S_CONTACT_OVERFLOW);
-        continue;
-      }
-      if (idle_world(d, d.shdr[i].w)) {  // the segment it had, already sorted (own is its length)
-        const int from = prev_segment(d, i).x;
-        for (int k = 0; k < own; ++k) d.nkey[base + k] = d.ckey[from + k];
         continue;
       }
       auto foreign = [&](uint64_t key, int s) {
@@ -2542,6 +2574,27 @@ __global__ void __launch_bounds__(kQueryBlock) k_pairs_compact(Dev d) {
         if (own > 1) d.heavy[atomicAdd(&d.cnt[C_HEAVY], 1)] = make_int2(base, own);
       }
     }
+  }
+}
+
+// An idle world's pairs: its shapes are contiguous, so its segments form one
+// block in the current list and one block in the new list (their lengths were
+// kept), copied whole by one workgroup.
+__global__ void __launch_bounds__(kBlock) k_keep_idle_pairs(Dev d) {
+  const int n = d.cnt[C_CONTACTS];
+  for (int w = blockIdx.x; w < d.nw; w += gridDim.x) {
+    if (world_active(d, w)) continue;
+    const int f = d.wshape0[w], l = d.wshape1[w];
+    if (l <= f) continue;
+    const int src = min(d.poff_prev[f], n);
+    const int end = l < d.ns ? min(d.poff_prev[l], n) : n;
+    const int dst = d.poff[f];
+    const int len = end - src;
+    if (dst + len > d.ccap) {
+      if (threadIdx.x == 0) atomicOr(&d.cnt[C_STATUS], S_CONTACT_OVERFLOW);
+      continue;
+    }
+    for (int k = (int)threadIdx.x; k < len; k += (int)blockDim.x) d.nkey[dst + k] = d.ckey[src + k];
   }
 }
 
