@@ -485,9 +485,16 @@ int b2g_batch::broadphase(bool use_given_aabbs) {
     return rc;
   }
   const int gq = grid_for((long long)ns * kBlock / kQueryBlock);  // blocks of kQueryBlock threads
-  launch(B2G_KERNEL_PAIRS_WRITE, ns, k_lbvh_pairs, dim3(gq), dim3(kQueryBlock), 0, d);
+  const bool dense = ns >= kDenseQueryShapes;  // the same slot count for both kernels
+  if (dense)
+    launch(B2G_KERNEL_PAIRS_WRITE, ns, k_lbvh_pairs<kPairSlotsDense>, dim3(gq), dim3(kQueryBlock), 0, d);
+  else
+    launch(B2G_KERNEL_PAIRS_WRITE, ns, k_lbvh_pairs<kPairSlots>, dim3(gq), dim3(kQueryBlock), 0, d);
   scan(d.pcount, nullptr, ns, d.poff, d.cnt + C_PAIRS, ns);
-  hipLaunchKernelGGL(k_pairs_compact, dim3(gq), dim3(kQueryBlock), 0, stream, d);
+  if (dense)
+    hipLaunchKernelGGL(k_pairs_compact<kPairSlotsDense>, dim3(gq), dim3(kQueryBlock), 0, stream, d);
+  else
+    hipLaunchKernelGGL(k_pairs_compact<kPairSlots>, dim3(gq), dim3(kQueryBlock), 0, stream, d);
   if (d.idle && idle_worlds() > 0)
     hipLaunchKernelGGL(k_keep_idle_pairs, dim3(std::min(d.nw, 4096)), dim3(kBlock), 0, stream, d);
   // one block per segment to sort: static shapes' segments, plus a few

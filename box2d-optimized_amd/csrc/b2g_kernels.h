@@ -140,7 +140,7 @@ struct Dev {
                      // category | mask << 16, group
   float4* sparse;    // (levels) x ns AABB table over sorted leaves; level 0 = leaf AABBs
   int sparse_levels;
-  uint64_t* pslots;  // kPairSlots per shape: pairs found by the traversal, sorted
+  uint64_t* pslots;  // up to kPairSlots per shape: pairs found by the traversal, sorted
   int2* heavy;       // (first, count) of the pair segments of over-full shapes, to be sorted
   int* wleaf;        // 2 per world: first leaf, last leaf + 1
   int* wroot;        // per world: the internal node whose leaves are exactly the world's (-1: one leaf)
@@ -2395,8 +2395,13 @@ __device__ __forceinline__ bool pair_allowed(const Dev& d, int4 fa, int4 fb) {
 }
 
 constexpr int kStack = 64;
-constexpr int kPairSlots = 16;
 constexpr int kQueryBlock = 128;  // LDS traversal stack: kStack x kQueryBlock ints (32 KB)
+// Pair slots per query lane (LDS): 16 slots (48 KB per block with the stack)
+// re-traverse fewer shapes; 8 (40 KB) fit four blocks per CU instead of
+// three, which pays once the queries fill the GPU several times over.
+constexpr int kPairSlots = 16;
+constexpr int kPairSlotsDense = 8;
+constexpr int kDenseQueryShapes = 65536;  // from this many shapes on, 8 slots
 
 // Pair ownership. Every pair is found by exactly one traversal: a shape
 // queries for partners j > i and for static partners of any index, and a
@@ -2486,12 +2491,13 @@ __device__ __forceinline__ void lbvh_query(const Dev& d, int i, int* stk, Emit e
 
 // Queries run in Morton order (lane t takes the shape at sorted position t):
 // neighbouring lanes query neighbouring boxes and walk the same nodes. A lane
-// keeps up to kPairSlots keys, sorted (keys of static lower partners sort
-// first); the counts go on past kPairSlots, and k_pairs_compact traverses
-// such a shape again.
+// keeps up to kSlots keys, sorted (keys of static lower partners sort
+// first); the counts go on past kSlots, and k_pairs_compact traverses such a
+// shape again.
+template <int kSlots>
 __global__ void __launch_bounds__(kQueryBlock) k_lbvh_pairs(Dev d) {
   __shared__ int stk[kStack * kQueryBlock];
-  __shared__ uint64_t slots[kPairSlots * kQueryBlock];  // thread-minor, like the stack
+  __shared__ uint64_t slots[kSlots * kQueryBlock];  // thread-minor, like the stack
   B2G_FOR_ALL(t, valid, 0, d.ns) {
     if (valid) {
       // an idle world's shapes keep their segments (k_aabb set their counts)
@@ -2500,7 +2506,7 @@ __global__ void __launch_bounds__(kQueryBlock) k_lbvh_pairs(Dev d) {
       const int tid = (int)threadIdx.x;
       int found = 0, own = 0;
       lbvh_query(d, i, stk, [&](uint64_t key, int s) {
-        if (found < kPairSlots) {
+        if (found < kSlots) {
           int b = found - 1;  // sorted insert
           while (b >= 0 && slots[b * kQueryBlock + tid] > key) {
             slots[(b + 1) * kQueryBlock + tid] = slots[b * kQueryBlock + tid];
@@ -2514,8 +2520,8 @@ __global__ void __launch_bounds__(kQueryBlock) k_lbvh_pairs(Dev d) {
         else
           ++own;
       });
-      uint64_t* out = d.pslots + (size_t)i * kPairSlots;
-      const int kept = found < kPairSlots ? found : kPairSlots;
+      uint64_t* out = d.pslots + (size_t)i * kSlots;
+      const int kept = found < kSlots ? found : kSlots;
       for (int k = 0; k < kept; ++k) out[k] = slots[k * kQueryBlock + tid];
       d.pfound[i] = found;  // a static shape: 0, its segment's fill cursor
       if (!(d.sfast[i].y & 64)) d.pcount[i] = own;
@@ -2527,6 +2533,7 @@ __global__ void __launch_bounds__(kQueryBlock) k_lbvh_pairs(Dev d) {
 // (poff), keys of a static lower shape s to s's segment at a cursor. A shape
 // with more pairs than slots is traversed again straight into place.
 // Segments filled in traversal or cursor order are queued for sorting.
+template <int kSlots>
 __global__ void __launch_bounds__(kQueryBlock) k_pairs_compact(Dev d) {
   __shared__ int stk[kStack * kQueryBlock];
   B2G_FOR_ALL(t, valid, 0, d.ns) {
@@ -2550,8 +2557,8 @@ __global__ void __launch_bounds__(kQueryBlock) k_pairs_compact(Dev d) {
         continue;
       }
       const int found = d.pfound[i];
-      if (found <= kPairSlots) {
-        const uint64_t* src = d.pslots + (size_t)i * kPairSlots;
+      if (found <= kSlots) {
+        const uint64_t* src = d.pslots + (size_t)i * kSlots;
         int k = 0;
         for (int m = 0; m < found; ++m) {
           const uint64_t key = src[m];

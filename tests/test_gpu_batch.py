@@ -182,6 +182,29 @@ def test_gpu_batch_4096_worlds_steps(engine):
     assert c.touching > 0 and c.colours <= 65
 
 
+def test_gpu_dense_pair_queries_match_worlds_alone_bitexact(engine):
+    """From 65536 shapes on, the pair queries keep 8 pair slots per lane
+    instead of 16 (more shapes are traversed a second time, for occupancy).
+    A batch above that size steps its worlds exactly as each world stepped
+    alone, whose few shapes take the 16-slot queries."""
+    sizes = S.batched_sizes(320)
+    scenes = [S.batch_world(i, int(sizes[i])) for i in range(320)]
+    shapes = sum(sc.fixture_count for sc in scenes)
+    assert shapes >= 65536, shapes
+    steps = 20
+    got = engine.Batch.from_scenes(scenes).advance(steps)
+    print(f"320 worlds, {shapes} shapes, {got.counts().contacts} contacts", flush=True)
+    got_b = got.bodies()
+    sample = [0, 37, 101, 202, 319] + [int(np.argmax(sizes[:320]))]
+    for i in sample:
+        alone = engine.Batch.from_scenes([scenes[i]]).advance(steps).bodies()
+        part = got_b[got.world_bodies[i]:got.world_bodies[i + 1]]
+        same = _state(part) == _state(alone)
+        print(f"world {i} ({scenes[i].body_count} bodies): {same.all(axis=1).sum()} of {len(alone)} bit-identical",
+              flush=True)
+        assert same.all(), i
+
+
 def test_gpu_graph_steps_match_eager_steps(engine):
     """Replaying the captured step graph gives the eager step's results:
     bit-exact while no constraint falls back to the serial group (the colour
