@@ -1803,6 +1803,13 @@ __global__ void __launch_bounds__(kBlock) k_world_colour_sort(Dev d, int rounds_
   __shared__ int cur[kColourSlots];
   __shared__ int range[2];
   const int w = d.worder[blockIdx.x];
+  if (!world_active(d, w)) {  // no touching contact (the narrowphase left them all untouched): no colours
+    if (threadIdx.x == 0) {
+      if (blockIdx.x == 0) d.cnt[C_ROUND] += rounds_run;
+      d.wcolmax[w] = -1;
+    }
+    return;  // the whole block
+  }
   if (threadIdx.x == 0) {
     if (blockIdx.x == 0) d.cnt[C_ROUND] += rounds_run;
     const int nc = d.cnt[C_CONTACTS], total = d.cnt[C_CONS];

@@ -119,6 +119,8 @@ def test_gpu_idle_worlds_keep_pairs_bitexact(engine, solver):
     sched = rng.random((steps, len(scenes))) < 0.45
     sched[:3, 0] = True
     sched[3:12, 0] = False  # world 0: idle for 9 steps, then back
+    sched[13, :] = False    # a step with every world idle (nothing to sort or query)
+    sched[14, :] = True     # and one with every world active
     dt = np.float32(1.0 / 60.0)
     ratio = np.float32(np.float32(np.float32(1.0) / dt) * dt)
     opt = engine.SOLVER_WORLDS if solver == "worlds" else engine.SOLVER_COLOURS
