@@ -923,6 +923,7 @@ int b2g_batch::step(int vi, int pi, unsigned flags) {
     round_host = 0;
   }
   d.events_on = (flags & B2G_STEP_EVENTS) ? 1 : 0;
+  d.segs = 0;  // an eager step sets it once its contact list is known
   if (flags & B2G_STEP_SKIP_BROADPHASE) {  // bodies move without a rebuild or a pair query
     tree_stale = true;
     pairs_current = false;
@@ -943,6 +944,9 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
     int rc = broadphase(false);
     if (rc) return rc;
   }
+  // the contact list is the last broadphase's pair list: world contact
+  // ranges are segment offsets
+  d.segs = pairs_current && worlds_contiguous ? 1 : 0;
   enqueue_collide_and_colour(kMaxColours, kColourRoundsPerStep);
   if (prof_on) HIP_TRY(hipEventRecord(ev[1], stream));
   int rc;

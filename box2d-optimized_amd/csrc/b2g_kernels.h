@@ -155,6 +155,9 @@ struct Dev {
   int idle;          // 1: the current contact list is the last broadphase's pair list and no body,
                      // shape, joint or contact was uploaded since; a world that is not active then
                      // keeps its pairs, its boxes and its part of the tree (leaf order, nodes, table)
+  int segs;          // 1: the current contact list is the last broadphase's pair list and the
+                     // layout is world after world, so a world's contacts are [poff_prev[first shape],
+                     // poff_prev[one past its last shape])
   uint64_t* skey;    // per shape: its Morton key at its world's last tree build
   uint64_t* xkey;    // second buffer of the active shapes' key sort (the first is mkey[1])
   uint32_t* xval;
@@ -716,6 +719,22 @@ __device__ __forceinline__ int lower_bound_u64(const uint64_t* a, int n, uint64_
   return lo;
 }
 
+// World w's contacts in the current list, [first, end): two segment offsets
+// while d.segs is set, else two binary searches of the sorted keys (the
+// first contact whose lower shape is the world's first shape, and the first
+// past its last shape).
+__device__ __forceinline__ int2 world_contacts(const Dev& d, int w) {
+  const int nc = d.cnt[C_CONTACTS];
+  const int s0 = d.wshape0[w], s1 = d.wshape1[w];
+  if (d.segs) {
+    const int c0 = s0 < d.ns ? min(d.poff_prev[s0], nc) : nc;
+    const int c1 = s1 < d.ns ? min(d.poff_prev[s1], nc) : nc;
+    return make_int2(c0, c1);
+  }
+  return make_int2(lower_bound_u64(d.ckey, nc, (uint64_t)(uint32_t)s0 << 32),
+                   lower_bound_u64(d.ckey, nc, (uint64_t)(uint32_t)s1 << 32));
+}
+
 // Islands of one world per workgroup, with the union-find forest in LDS
 // (kCap >= the world's body count). Same forest rule as k_island_link (the
 // larger root hooks under the smaller), so every island is labelled by its
@@ -755,10 +774,9 @@ __global__ void __launch_bounds__(1024) k_world_islands(Dev d) {
   const int b0 = wb.x, nbw = wb.y - wb.x;
   if (threadIdx.x == 0) {
     const int nc = d.cnt[C_CONTACTS], total = d.cnt[C_CONS];
-    const int c0 = lower_bound_u64(d.ckey, nc, (uint64_t)(uint32_t)d.wshape0[w] << 32);
-    const int c1 = lower_bound_u64(d.ckey, nc, (uint64_t)(uint32_t)d.wshape1[w] << 32);
-    range[0] = c0 < nc ? d.tpos[c0] : total;
-    range[1] = c1 < nc ? d.tpos[c1] : total;
+    const int2 cr = world_contacts(d, w);
+    range[0] = cr.x < nc ? d.tpos[cr.x] : total;
+    range[1] = cr.y < nc ? d.tpos[cr.y] : total;
   }
   for (int i = threadIdx.x; i < nbw; i += blockDim.x) par[i] = i;
   __syncthreads();
@@ -1813,10 +1831,9 @@ __global__ void __launch_bounds__(kBlock) k_world_colour_sort(Dev d, int rounds_
   if (threadIdx.x == 0) {
     if (blockIdx.x == 0) d.cnt[C_ROUND] += rounds_run;
     const int nc = d.cnt[C_CONTACTS], total = d.cnt[C_CONS];
-    const int c0 = lower_bound_u64(d.ckey, nc, (uint64_t)(uint32_t)d.wshape0[w] << 32);
-    const int c1 = lower_bound_u64(d.ckey, nc, (uint64_t)(uint32_t)d.wshape1[w] << 32);
-    range[0] = c0 < nc ? d.tpos[c0] : total;
-    range[1] = c1 < nc ? d.tpos[c1] : total;
+    const int2 cr = world_contacts(d, w);
+    range[0] = cr.x < nc ? d.tpos[cr.x] : total;
+    range[1] = cr.y < nc ? d.tpos[cr.y] : total;
   }
   for (int c = threadIdx.x; c < kColourSlots; c += blockDim.x) {
     hist[c] = 0;
