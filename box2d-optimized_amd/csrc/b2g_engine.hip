@@ -485,16 +485,26 @@ int b2g_batch::broadphase(bool use_given_aabbs) {
     return rc;
   }
   const int gq = grid_for((long long)ns * kBlock / kQueryBlock);  // blocks of kQueryBlock threads
-  const bool dense = ns >= kDenseQueryShapes;  // the same slot count for both kernels
-  if (dense)
-    launch(B2G_KERNEL_PAIRS_WRITE, ns, k_lbvh_pairs<kPairSlotsDense>, dim3(gq), dim3(kQueryBlock), 0, d);
+  // The same slot count and stack for both kernels. Queries that fill the
+  // GPU several times over are occupancy-bound: they keep 8 pair slots and,
+  // while no world exceeds 65536 shapes, 16-bit stack entries (offsets inside
+  // the world's leaf range): 24 KB of LDS per block instead of 48 KB. Fewer
+  // queries keep 16 slots and 32-bit entries, which cost less per access.
+  const bool dense = ns >= kDenseQueryShapes;
+  const bool short_stack = dense && !layout_dirty && max_world_shapes > 0 && max_world_shapes <= 65536;
+  if (short_stack)
+    launch(B2G_KERNEL_PAIRS_WRITE, ns, k_lbvh_pairs<kPairSlotsDense, uint16_t>, dim3(gq), dim3(kQueryBlock), 0, d);
+  else if (dense)
+    launch(B2G_KERNEL_PAIRS_WRITE, ns, k_lbvh_pairs<kPairSlotsDense, int>, dim3(gq), dim3(kQueryBlock), 0, d);
   else
-    launch(B2G_KERNEL_PAIRS_WRITE, ns, k_lbvh_pairs<kPairSlots>, dim3(gq), dim3(kQueryBlock), 0, d);
+    launch(B2G_KERNEL_PAIRS_WRITE, ns, k_lbvh_pairs<kPairSlots, int>, dim3(gq), dim3(kQueryBlock), 0, d);
   scan(d.pcount, nullptr, ns, d.poff, d.cnt + C_PAIRS, ns);
-  if (dense)
-    hipLaunchKernelGGL(k_pairs_compact<kPairSlotsDense>, dim3(gq), dim3(kQueryBlock), 0, stream, d);
+  if (short_stack)
+    hipLaunchKernelGGL((k_pairs_compact<kPairSlotsDense, uint16_t>), dim3(gq), dim3(kQueryBlock), 0, stream, d);
+  else if (dense)
+    hipLaunchKernelGGL((k_pairs_compact<kPairSlotsDense, int>), dim3(gq), dim3(kQueryBlock), 0, stream, d);
   else
-    hipLaunchKernelGGL(k_pairs_compact<kPairSlots>, dim3(gq), dim3(kQueryBlock), 0, stream, d);
+    hipLaunchKernelGGL((k_pairs_compact<kPairSlots, int>), dim3(gq), dim3(kQueryBlock), 0, stream, d);
   if (d.idle && idle_worlds() > 0)
     hipLaunchKernelGGL(k_keep_idle_pairs, dim3(std::min(d.nw, 4096)), dim3(kBlock), 0, stream, d);
   // one block per segment to sort: static shapes' segments, plus a few

@@ -2440,8 +2440,13 @@ constexpr int kDenseQueryShapes = 65536;  // from this many shapes on, 8 slots
 // thread]), so lanes at different depths do not conflict. `emit(key, s)` is
 // called for every pair found; s >= 0 is the static lower shape of a key that
 // is not the querying shape's own.
-template <class Emit>
-__device__ __forceinline__ void lbvh_query(const Dev& d, int i, int* stk, Emit emit) {
+// Every internal node of a world's subtree covers leaves of that world only,
+// and a node's index is one end of its leaf range, so the nodes a query
+// walks lie in the world's leaf range: with a 16-bit stack (worlds of at most
+// 65536 shapes) entries are offsets from the world's first leaf, and the
+// stack takes half the LDS.
+template <class S, class Emit>
+__device__ __forceinline__ void lbvh_query(const Dev& d, int i, S* stk, Emit emit) {
   const int n = d.ns;
   if (n < 2) return;
   const int4 fi = d.sfast[i];
@@ -2452,8 +2457,16 @@ __device__ __forceinline__ void lbvh_query(const Dev& d, int i, int* stk, Emit e
   const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
   const int root = d.wroot[w];
   if (root < 0) return;  // the world has no other shape
+  const int base = sizeof(S) < sizeof(int) ? wlo : 0;
+  // a 16-bit entry holds any node of the world's range; a node outside it
+  // would mean an inconsistent tree, flagged rather than truncated
+  auto fits = [&](int node) { return sizeof(S) >= sizeof(int) || (unsigned)(node - base) <= 0xffffu; };
+  if (!fits(root)) {
+    atomicOr(&d.cnt[C_STATUS], S_TREE_CORRUPT);
+    return;
+  }
   int sp = 0;
-  stk[tid] = root;  // the world's own subtree: no walk through nodes spanning worlds
+  stk[tid] = (S)(root - base);  // the world's own subtree: no walk through nodes spanning worlds
   sp = 1;
   int visits = 0;
   // test both children of one node record; leaves are emitted, overlapping
@@ -2476,8 +2489,10 @@ __device__ __forceinline__ void lbvh_query(const Dev& d, int i, int* stk, Emit e
           emit(((uint64_t)(uint32_t)j << 32) | (uint32_t)i, j);
         else
           emit(((uint64_t)(uint32_t)i << 32) | (uint32_t)j, -1);
+      } else if (!fits(code)) {
+        atomicOr(&d.cnt[C_STATUS], S_TREE_CORRUPT);
       } else if (sp < kStack) {
-        stk[sp++ * nt + tid] = code;
+        stk[sp++ * nt + tid] = (S)(code - base);
       } else {
         atomicOr(&d.cnt[C_STATUS], S_STACK_OVERFLOW);
       }
@@ -2491,7 +2506,7 @@ __device__ __forceinline__ void lbvh_query(const Dev& d, int i, int* stk, Emit e
     int take = 0;
 #pragma unroll
     for (int u = 0; u < kTrip; ++u) {
-      nodes[u] = sp > 0 ? stk[--sp * nt + tid] : -1;
+      nodes[u] = sp > 0 ? base + (int)stk[--sp * nt + tid] : -1;
       take += nodes[u] >= 0 ? 1 : 0;
     }
     visits += take;
@@ -2518,9 +2533,9 @@ __device__ __forceinline__ void lbvh_query(const Dev& d, int i, int* stk, Emit e
 // keeps up to kSlots keys, sorted (keys of static lower partners sort
 // first); the counts go on past kSlots, and k_pairs_compact traverses such a
 // shape again.
-template <int kSlots>
+template <int kSlots, class S>
 __global__ void __launch_bounds__(kQueryBlock) k_lbvh_pairs(Dev d) {
-  __shared__ int stk[kStack * kQueryBlock];
+  __shared__ S stk[kStack * kQueryBlock];
   __shared__ uint64_t slots[kSlots * kQueryBlock];  // thread-minor, like the stack
   B2G_FOR_ALL(t, valid, 0, d.ns) {
     if (valid) {
@@ -2557,9 +2572,9 @@ __global__ void __launch_bounds__(kQueryBlock) k_lbvh_pairs(Dev d) {
 // (poff), keys of a static lower shape s to s's segment at a cursor. A shape
 // with more pairs than slots is traversed again straight into place.
 // Segments filled in traversal or cursor order are queued for sorting.
-template <int kSlots>
+template <int kSlots, class S>
 __global__ void __launch_bounds__(kQueryBlock) k_pairs_compact(Dev d) {
-  __shared__ int stk[kStack * kQueryBlock];
+  __shared__ S stk[kStack * kQueryBlock];
   B2G_FOR_ALL(t, valid, 0, d.ns) {
     if (valid) {
       // an idle world's block of pairs is copied whole (k_keep_idle_pairs)
