@@ -1848,23 +1848,23 @@ __global__ void __launch_bounds__(kBlock) k_world_colour_sort(Dev d, int rounds_
     d.ccol[d.clist[k]] = col;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int o = 0, top = 0;
-    for (int c = 0; c < kColourSlots; ++c) {
-      off[c] = o;
-      o += hist[c];
-      if (hist[c]) top = c + 1;
-    }
-    off[kColourSlots] = o;
-    if (top) atomicMax(&d.cnt[C_NCOLOURS], top);
-    if (hist[kSerialColour]) atomicAdd(&d.cnt[C_SERIAL], hist[kSerialColour]);
-    int topn = 0;  // normal colours only
-    for (int c = 0; c < kMaxColours; ++c)
-      if (hist[c]) topn = c + 1;
-    if (world_active(d, w))
+  static_assert(kMaxColours == kWave, "one lane per normal colour");
+  if (threadIdx.x < kWave) {  // the first wave: offsets of the 64 normal colours, then the serial group
+    const int c = (int)threadIdx.x;
+    const int h = hist[c];
+    const int inc = wave_inclusive_scan(h);
+    off[c] = inc - h;
+    const uint64_t used = __ballot(h != 0);
+    const int topn = used ? 64 - __clzll((long long)used) : 0;  // normal colours only
+    if (c == kWave - 1) {
+      const int ser = hist[kSerialColour];
+      off[kSerialColour] = inc;
+      off[kColourSlots] = inc + ser;
+      const int top = ser ? kColourSlots : topn;
+      if (top) atomicMax(&d.cnt[C_NCOLOURS], top);
+      if (ser) atomicAdd(&d.cnt[C_SERIAL], ser);
       world_colour_stats(d, w, topn);  // the colouring upkeep, for this world right away
-    else
-      d.wcolmax[w] = topn - 1;
+    }
   }
   __syncthreads();
   for (int c = threadIdx.x; c <= kColourSlots; c += blockDim.x) d.wcol[w * (kColourSlots + 1) + c] = t0 + off[c];
