@@ -272,10 +272,13 @@ __device__ __forceinline__ void reset_colour_counters(const Dev& d) {
 }
 __device__ __forceinline__ void integrate_velocity(const Dev& d, int b) {
   {
+    // a body of an idle world keeps its island label and colour mask (stale;
+    // every reader of them checks the world first)
+    const int w = d.bworld[b];
+    if (!world_active(d, w)) return;
     d.parent[b] = b;
     d.bmask[b] = 0ull;
-    const int w = d.bworld[b];
-    if (!world_active(d, w) || d.btype[b] != kDynamicBody) return;
+    if (d.btype[b] != kDynamicBody) return;
     const float4 wp = d.wpar[w];
     const float h = wp.z;
     const float4 v4 = d.vel[b], p = d.prop[b], p2 = d.prop2[b], f = d.force[b];
@@ -1432,6 +1435,7 @@ __global__ void __launch_bounds__(kBlock) k_contact_position(Dev d, int colour, 
 // serial group (not in the body masks) follows in its own launch.
 __global__ void __launch_bounds__(kBlock) k_warm_gather(Dev d) {
   B2G_FOR(b, 0, d.nb) {
+    if (!world_active(d, d.bworld[b])) continue;  // its colour mask is a past step's
     const unsigned long long mask = d.bmask[b];  // only dynamic bodies have bits
     if (mask == 0ull) continue;
     const int n = __popcll(mask);

@@ -45,14 +45,22 @@ def _mixed_trace(target_world, target_tick, nworlds, seed=3):
     return _requests(rows)
 
 
+def _pool(engine, scenes, solver):
+    b = engine.Batch.from_scenes(scenes)
+    if solver == "colours":  # colour-major sweeps (the warm-start gather over all bodies)
+        b.set_option(engine.OPT_SOLVER, engine.SOLVER_COLOURS)
+    return b.advance(2)
+
+
+@pytest.mark.parametrize("solver", ["auto", "colours"])
 @pytest.mark.parametrize("graph", [False, True])
-def test_gpu_serve_same_result_alone_and_in_mix(engine, graph):
+def test_gpu_serve_same_result_alone_and_in_mix(engine, graph, solver):
     scenes = _scenes()
     wbytes = np.array([engine.world_device_bytes(s) for s in scenes], np.int64)
     target = 3
-    alone = engine.Batch.from_scenes(scenes).advance(2)
-    mix = engine.Batch.from_scenes(scenes).advance(2)
-    own = engine.Batch.from_scenes([scenes[target]]).advance(2)
+    alone = _pool(engine, scenes, solver)
+    mix = _pool(engine, scenes, solver)
+    own = _pool(engine, [scenes[target]], solver)
     ra, sa, _ = engine.serve(alone, _requests([(0.0, target, 6, DT60)]), 1 << 40, wbytes, graph=graph,
                              realtime=False)
     budget = int(wbytes.max() * 4)  # about four worlds in flight: requests queue for memory
@@ -70,7 +78,7 @@ def test_gpu_serve_same_result_alone_and_in_mix(engine, graph):
     own.advance(6)
     assert np.array_equal(_world_state(alone, target), _world_state(own, 0))
     # worlds without requests did not move
-    untouched = engine.Batch.from_scenes(scenes).advance(2)
+    untouched = _pool(engine, scenes, solver)
     for w in range(len(scenes)):
         if w != target:
             assert np.array_equal(_world_state(alone, w), _world_state(untouched, w))
