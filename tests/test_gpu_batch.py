@@ -156,6 +156,46 @@ def test_gpu_idle_worlds_keep_pairs_bitexact(engine, solver):
           f"{len(kl[-1])} contacts at the end", flush=True)
 
 
+def test_gpu_upload_to_idle_world_requeries_its_pairs(engine):
+    """An upload makes the next broadphase query and rebuild every world: a
+    world teleported while idle gets the pairs of its new positions at once,
+    exactly as with idle keeping off."""
+    scenes = _mixed_scenes()
+    dt = np.float32(1.0 / 60.0)
+    ratio = np.float32(np.float32(np.float32(1.0) / dt) * dt)
+
+    def step(b, active):
+        b.set_world_params(0, [engine.WorldParams(g, dt, ratio, 1, int(active[w])) for w, g in enumerate(b.gravity)])
+        b.step(8, 3, 0)
+
+    runs = []
+    for keep in (1, 0):
+        b = engine.Batch.from_scenes(scenes)
+        b.set_option(engine.OPT_KEEP_IDLE, keep)
+        b.advance(3)
+        idle0 = [w != 1 for w in range(len(scenes))]
+        step(b, idle0)
+        step(b, idle0)
+        # world 1 (a pile), idle: every dynamic body shifted by 0.4 m
+        lo, hi = b.world_bodies[1], b.world_bodies[2]
+        rec = b.bodies(lo, hi - lo)
+        dyn = rec["type"] == 2
+        rec["xf"][dyn, 0] += np.float32(0.4)
+        rec["c"][dyn, 0] += np.float32(0.4)
+        b.upload_bodies(lo, rec)
+        lists = []
+        step(b, idle0)                      # world 1 still idle: the upload alone must refresh its pairs
+        lists.append(b.contacts())
+        for _ in range(3):
+            step(b, [True] * len(scenes))  # then everyone moves
+            lists.append(b.contacts())
+        runs.append((lists, b.bodies()))
+    (kl, kb), (fl, fb) = runs
+    for t in range(len(kl)):
+        assert len(kl[t]) > 0 and kl[t].tobytes() == fl[t].tobytes(), t
+    assert (_state(kb) == _state(fb)).all()
+
+
 def test_gpu_batch_world_matches_drop_in_world(engine):
     """A batch of one built by b2a_batch_from_scenes steps exactly like the
     drop-in b2World built from the same scene."""
