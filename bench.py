@@ -170,6 +170,16 @@ def batched_section(pkg, worlds: int, warmup: int, steps: int, sample_every: int
     batch.wait()
     el = time.perf_counter() - t0
     counts = batch.counts()
+    # the same steps as replays of the captured step graph (the other launch mode)
+    other = eng.STEP_ASYNC | (0 if mode & eng.STEP_GRAPH else eng.STEP_GRAPH)
+    for _ in range(2):  # capture (both contact-buffer parities) or first eager steps
+        batch.step(8, 3, other & ~eng.STEP_ASYNC)
+    batch.wait()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.step(8, 3, other)
+    batch.wait()
+    el_other = time.perf_counter() - t0
     nbodies = int(sum(s.dynamic_count for s in scenes))
     sample = list(range(0, worlds, sample_every))
     # plain per-world loop on the sample
@@ -200,6 +210,10 @@ def batched_section(pkg, worlds: int, warmup: int, steps: int, sample_every: int
                            "sample": f"{len(sample)} worlds (every {sample_every}th), {loop_steps} steps each, "
                                      f"one batch per world, host sync per step"},
         "speedup_vs_per_world_loop": (worlds * steps / el) / loop_rate,
+        "launch": "hipGraph replay of the whole step" if mode & eng.STEP_GRAPH else "eager",
+        "other_launch": {"launch": "eager" if mode & eng.STEP_GRAPH else "hipGraph replay of the whole step",
+                         "ms_per_step": el_other / steps * 1e3, "value": worlds * steps / el_other,
+                         "unit": "world-steps/s", "steps": steps},
     }
     # the whole step against HBM bandwidth: the genuinely HBM-sized case
     geo = float(np.mean([mean_geometry_bytes(sc) for sc in scenes]))
