@@ -72,6 +72,11 @@ struct b2g_batch {
   hipStream_t side = nullptr;  // the island pass, beside the velocity phase
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   int* pinned = nullptr;     // host mirror of counters + colour offsets
+  // pinned staging of b2g_set_world_params (copied in stream order, no host
+  // wait; params_ev guards the staging against refilling before its copy ran)
+  float4* stage_wpar = nullptr;
+  int* stage_wflag = nullptr;
+  hipEvent_t params_ev = nullptr;
   // host mirrors
   std::vector<int> h_btype, h_bworld;
   std::vector<int> h_shape_body;
@@ -268,6 +273,9 @@ int b2g_batch::alloc() {
   d.ccap = NC;
   d.ecap = NC;
   HIP_TRY(hipHostMalloc((void**)&pinned, 256 * sizeof(int), hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void**)&stage_wpar, (size_t)NW * sizeof(float4), hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void**)&stage_wflag, (size_t)NW * sizeof(int), hipHostMallocDefault));
+  HIP_TRY(hipEventCreateWithFlags(&params_ev, hipEventDisableTiming));
   HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   {
     // The side stream runs beside the main one (island labels beside the
@@ -322,6 +330,9 @@ void b2g_batch::release() {
   for (int k = 0; k < 5; ++k)
     if (d.pc[k]) (void)hipFree(d.pc[k]);
   if (pinned) (void)hipHostFree(pinned);
+  if (stage_wpar) (void)hipHostFree(stage_wpar);
+  if (stage_wflag) (void)hipHostFree(stage_wflag);
+  if (params_ev) (void)hipEventDestroy(params_ev);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   if (fork_ev) (void)hipEventDestroy(fork_ev);
@@ -1268,16 +1279,19 @@ int b2g_set_world_params(b2g_batch* b, int32_t first, int32_t count, const b2g_w
     b->pairs_current = false;
     b->layout_dirty = true;
   }
-  std::vector<float4> wp(count);
-  std::vector<int> wf(count);
+  // the last call's copies have left the staging buffers (normally long ago)
+  HIP_TRY(hipEventSynchronize(b->params_ev));
+  float4* wp = b->stage_wpar + first;
+  int* wf = b->stage_wflag + first;
   for (int i = 0; i < count; ++i) {
     b->h_params[first + i] = p[i];
     wp[i] = make_float4(p[i].gravity[0], p[i].gravity[1], p[i].dt, p[i].dt_ratio);
     wf[i] = (p[i].warm_starting ? 1 : 0) | (p[i].active ? 2 : 0);
   }
-  HIP_TRY(hipMemcpyAsync(b->d.wpar + first, wp.data(), count * sizeof(float4), hipMemcpyHostToDevice, b->stream));
-  HIP_TRY(hipMemcpyAsync(b->d.wflag + first, wf.data(), count * sizeof(int), hipMemcpyHostToDevice, b->stream));
-  HIP_TRY(hipStreamSynchronize(b->stream));
+  // in stream order: every later step sees the new parameters
+  HIP_TRY(hipMemcpyAsync(b->d.wpar + first, wp, count * sizeof(float4), hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipMemcpyAsync(b->d.wflag + first, wf, count * sizeof(int), hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipEventRecord(b->params_ev, b->stream));
   b->d.nw = std::max(b->d.nw, first + count);
   return B2G_OK;
 }

@@ -224,6 +224,9 @@ int b2g_batch_destroy(b2g_batch* batch);
 /* world ranges: bodies [body_begin, body_end) etc. are assigned by the caller */
 int b2g_set_worlds(b2g_batch* batch, int32_t count, const int32_t* body_begin,
                    const int32_t* shape_begin, const b2g_world_params* params);
+/* Parameters of worlds [first, first + count). The copy to the device is in
+ * stream order with the steps (the call returns without waiting for it);
+ * `params` may be reused as soon as the call returns. */
 int b2g_set_world_params(b2g_batch* batch, int32_t first, int32_t count,
                          const b2g_world_params* params);
 
