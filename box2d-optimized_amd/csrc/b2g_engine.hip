@@ -287,7 +287,12 @@ int b2g_batch::alloc() {
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     std::vector<uint32_t> all_cus((size_t)std::max(1, (cus + 31) / 32), 0xffffffffu);
-    HIP_TRY(hipExtStreamCreateWithCUMask(&side, (uint32_t)all_cus.size(), all_cus.data()));
+    if (hipExtStreamCreateWithCUMask(&side, (uint32_t)all_cus.size(), all_cus.data()) != hipSuccess) {
+      // no queue of its own to be had (many live batches): a shared one
+      (void)hipGetLastError();
+      side = nullptr;
+      HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    }
   }
   HIP_TRY(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));

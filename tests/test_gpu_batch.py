@@ -196,6 +196,19 @@ def test_gpu_upload_to_idle_world_requeries_its_pairs(engine):
     assert (_state(kb) == _state(fb)).all()
 
 
+def test_gpu_many_live_batches(engine):
+    """Each batch runs its side stream on a hardware queue of its own when
+    the runtime can give one, and shares one otherwise: 24 batches alive at
+    once all step, each exactly like a lone batch of the same world."""
+    sc = S.random_pile(120, seed=41, half_w=8.0, height=20.0)
+    ref = engine.Batch.from_scenes([sc, S.pyramid(5)]).advance(6).bodies()
+    live = [engine.Batch.from_scenes([sc, S.pyramid(5)]) for _ in range(24)]
+    for b in live:
+        b.advance(6)
+    for b in live:
+        assert (_state(b.bodies()) == _state(ref)).all()
+
+
 def test_gpu_batch_world_matches_drop_in_world(engine):
     """A batch of one built by b2a_batch_from_scenes steps exactly like the
     drop-in b2World built from the same scene."""
