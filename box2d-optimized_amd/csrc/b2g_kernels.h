@@ -1898,8 +1898,11 @@ __global__ void __launch_bounds__(kBlock) k_world_colour_sort(Dev d, int rounds_
 // the world's body velocities (then positions) live in LDS during the sweeps
 // (dynamic shared memory of 16 bytes per body of the largest world); the
 // arithmetic is the same either way.
+// (At least 5 waves per SIMD: 96 registers and a few spills measured
+// slightly faster than 4 waves without spills; the CU hides one world's
+// latency behind the others'.)
 template <bool kLds>
-__global__ void __launch_bounds__(1024) k_solve_worlds(Dev d, int vi, int pi, int first) {
+__global__ void __launch_bounds__(1024, 5) k_solve_worlds(Dev d, int vi, int pi, int first) {
   __shared__ int off[kColourSlots + 1];
   extern __shared__ float4 s_body[];
   const int w = d.worder[first + blockIdx.x];  // worlds first .. first + gridDim.x - 1, largest first
