@@ -881,7 +881,8 @@ __device__ __forceinline__ void island_assign(const Dev& d, int n, int k) {
 //   pc0 localPoint localNormal  pc1 localPoints[0..1]  pc2 localCenterA/B
 //   pc3 mA mB iA iB             pc4 radiusA radiusB type pointCount
 // ===========================================================================
-__global__ void __launch_bounds__(kBlock) k_contact_init(Dev d) {
+// (6 waves per SIMD: 80 registers, measured slightly faster than 5)
+__global__ void __launch_bounds__(kBlock, 6) k_contact_init(Dev d) {
   const int n = d.cnt[C_CONS];
   B2G_FOR(s, 0, n) {
     const int c = d.csorted[s];
