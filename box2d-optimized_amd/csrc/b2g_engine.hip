@@ -71,6 +71,7 @@ struct b2g_batch {
   hipEvent_t ev[7] = {};
   hipStream_t side = nullptr;  // the island pass, beside the velocity phase
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  hipEvent_t counts_ev = nullptr;  // the colour path's early counter read-back
   int* pinned = nullptr;     // host mirror of counters + colour offsets
   // pinned staging of b2g_set_world_params (copied in stream order, no host
   // wait; params_ev guards the staging against refilling before its copy ran)
@@ -296,6 +297,7 @@ int b2g_batch::alloc() {
   }
   HIP_TRY(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&counts_ev, hipEventDisableTiming));
   for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
   h_btype.assign(NB, 0);
   h_bworld.assign(NB, 0);
@@ -342,6 +344,7 @@ void b2g_batch::release() {
     if (e) (void)hipEventDestroy(e);
   if (fork_ev) (void)hipEventDestroy(fork_ev);
   if (join_ev) (void)hipEventDestroy(join_ev);
+  if (counts_ev) (void)hipEventDestroy(counts_ev);
   if (side) (void)hipStreamDestroy(side);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -1038,8 +1041,14 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
   }
   last_path = 1;
   enqueue_colour_groups();
+  // The counters and colour offsets are final here (the set-up kernels only
+  // read them): the copy goes now, and the host waits for it alone while the
+  // constraint set-up keeps the device busy.
+  HIP_TRY(hipMemcpyAsync(pinned, d.cnt, (kCntColOff + kColourSlots + 1) * sizeof(int), hipMemcpyDeviceToHost,
+                         stream));
+  HIP_TRY(hipEventRecord(counts_ev, stream));
   enqueue_islands_and_init();
-  if ((rc = read_counters())) return rc;
+  HIP_TRY(hipEventSynchronize(counts_ev));
   est_cons = pinned[C_CONS];
   for (int extra = 0; pinned[C_REMAIN] > 0; extra += 2) {  // first steps / bursts of new contacts
     if (extra > 4096) return fail(B2G_ERR_CAPACITY, "colouring did not converge");
