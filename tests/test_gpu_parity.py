@@ -202,6 +202,57 @@ def _order_spread(sc, g):
     return worst_c, worst_a
 
 
+@pytest.mark.parametrize("name", ["pile20k", "gas100k_chains500"])
+def test_gpu_full_size_one_step_vs_restatement_in_gpu_colour_order(engine, name):
+    """The benchmark workloads at full size: 20,000 shapes, and 100,000
+    circles with 500 chains of 30 revolute joints. The GPU steps the scene
+    from its start, then one more step from its own state; the restatement,
+    given that state and the GPU's colours, takes the same step.
+
+    Tolerance: the state-after-one-step bound of 1e-4 m and 1e-4 rad. The
+    arithmetic is the restatement's operation for operation except the
+    device's sin/cos/sqrt, whose last bits differ from the host library's;
+    at this size 8 Gauss-Seidel iterations carry that to a few 1e-4 m/s and
+    a few 1e-5 rad (measured: 3.4e-5 rad, 4e-4 m/s on the pile), and in the
+    1000 m gas box one float ULP of a position is 3e-5 m (measured: one
+    ULP)."""
+    S = load_pkg().scenes
+    sc, warm = (S.random_pile(20000, seed=0xC2), 400) if name == "pile20k" else (S.gas_and_chains(), 30)
+    w = engine.World(sc)
+    w.step(warm)
+    b = w.batch()
+    bodies0 = b.bodies()
+    contacts0 = b.contacts()
+    joints0 = b.joints(sc.joint_count) if sc.joint_count else None
+    b.step(8, 3, engine.STEP_SKIP_BROADPHASE)
+    gb, gc = b.bodies(), b.contacts()
+    colours = b.contact_colours(len(gc))
+    r = oracle.RestatedWorld(sc)
+    rb = r.bodies()
+    for f in ("c", "a", "v", "w", "xf"):
+        rb[f] = bodies0[f]
+    r.set_bodies(rb)
+    r.set_contacts(contacts0)
+    if sc.joint_count:
+        j = r.joints()
+        for f in ("impulse", "motor_impulse", "lower_impulse", "upper_impulse"):
+            j[f] = joints0[f]
+        r.set_joints(j)
+    r.set_inv_dt0(np.float32(1.0) / DT)
+    r.set_colours(oracle.contact_keys(gc), colours, _joint_colours(sc) if sc.joint_count else None)
+    r.step(1, flags=engine.STEP_SKIP_BROADPHASE)
+    ob = r.bodies()
+    dyn = np.asarray(sc.body_type) == 2
+    dc = np.abs(gb["c"] - ob["c"])[dyn].max()
+    da = np.abs(gb["a"] - ob["a"])[dyn].max()
+    dv = np.abs(gb["v"] - ob["v"])[dyn].max()
+    touching = int((gc["point_count"] > 0).sum())
+    print(f"{name}: {int(dyn.sum())} bodies, {touching} touching contacts, {colours.max() + 1} colours; "
+          f"max |dc| {dc:.3g} |da| {da:.3g} |dv| {dv:.3g}", flush=True)
+    assert touching > 1000
+    assert dc < 1e-4 and da < 1e-4 and dv < 1e-2
+
+
 @pytest.mark.parametrize("name", ["pyramid", "mixed", "chain", "pile"])
 def test_gpu_one_step_vs_reference(engine, name):
     sc, _ = scene_for(name)
