@@ -260,6 +260,27 @@ def test_gpu_dense_pair_queries_match_worlds_alone_bitexact(engine):
         assert same.all(), i
 
 
+def test_gpu_full_batch_worlds_match_worlds_alone_bitexact(engine):
+    """The batched benchmark at full size: 4096 worlds of lognormal sizes
+    (about 1.14M bodies) step together; sampled worlds, the largest among
+    them, equal themselves stepped alone bit for bit."""
+    sizes = S.batched_sizes(4096)
+    scenes = [S.batch_world(i, int(sizes[i])) for i in range(4096)]
+    steps = 20
+    batch = engine.Batch.from_scenes(scenes).advance(steps)
+    c = batch.counts()
+    print(f"4096 worlds: {c.bodies} bodies, {c.contacts} contacts, {c.touching} touching, "
+          f"solver path {batch.get_option(engine.OPT_LAST_SOLVER)}", flush=True)
+    got = batch.bodies()
+    for i in [0, 1234, 2048, 3333, 4095, int(np.argmax(sizes))]:
+        alone = engine.Batch.from_scenes([scenes[i]]).advance(steps).bodies()
+        part = got[batch.world_bodies[i]:batch.world_bodies[i + 1]]
+        same = _state(part) == _state(alone)
+        print(f"world {i} ({scenes[i].body_count} bodies): {same.all(axis=1).sum()} of {len(alone)} bit-identical",
+              flush=True)
+        assert same.all(), i
+
+
 def test_gpu_graph_steps_match_eager_steps(engine):
     """Replaying the captured step graph gives the eager step's results:
     bit-exact while no constraint falls back to the serial group (the colour
