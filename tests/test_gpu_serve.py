@@ -99,3 +99,35 @@ def test_gpu_serve_realtime_poisson(engine):
     assert st.completed == len(trace)
     assert (res["completed"] >= res["admitted"]).all() and (res["admitted"] >= res["arrival"]).all()
     assert np.isfinite(lat).all() and lat.min() > 0
+
+
+def test_gpu_serve_full_pool_same_result_as_alone(engine):
+    """The serving workload's full resident pool: 10,000 worlds (16-4096
+    bodies, log-uniform; about 6.3M bodies). Worlds that sit out ticks keep
+    their pairs and their part of the tree, queries take the dense path, and
+    tree rebuilds sort only the active worlds. A target world's request runs
+    among several hundred others; its state equals the world stepped alone."""
+    sizes = S.serving_sizes(10000, seed=0xC5)
+    print("building the 10,000-world pool", flush=True)
+    scenes = [S.batch_world(200 + i, int(s)) for i, s in enumerate(sizes)]
+    wbytes = np.array([engine.world_device_bytes(s) for s in scenes], np.int64)
+    pool = engine.Batch.from_scenes(scenes).advance(2)
+    targets = [int(np.argmax(sizes)), 17, 5003]
+    rng = np.random.default_rng(11)
+    rows = [(2.0, t, 6, DT60) for t in targets]
+    for t in range(0, 10):
+        for w in rng.choice(10000, size=40, replace=False):
+            if int(w) not in targets:
+                rows.append((float(t), int(w), int(rng.integers(1, 9)), DT60 if rng.random() < 0.5 else DT240))
+    rows.sort(key=lambda r: r[0])
+    trace = _requests(rows)
+    res, st, _ = engine.serve(pool, trace, 64 << 30, wbytes, realtime=False)
+    print(f"{st.completed} requests in {st.ticks} ticks, max {st.max_in_flight} in flight", flush=True)
+    assert st.completed == len(trace) and st.max_in_flight > 100
+    for t in targets:
+        own = engine.Batch.from_scenes([scenes[t]]).advance(2)
+        own.advance(6)
+        same = _world_state(pool, t) == _world_state(own, 0)
+        print(f"world {t} ({scenes[t].body_count} bodies): {same.all(axis=1).sum()} of {len(same)} bit-identical",
+              flush=True)
+        assert same.all(), t
