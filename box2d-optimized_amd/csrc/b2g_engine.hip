@@ -194,7 +194,12 @@ struct b2g_batch {
   void flush_finish();
   int enqueue_graph_body(int vi, int pi, unsigned flags, int K);
   int check_status();
+  int status_error() const;
+  int collide_only_step(unsigned flags);
+  hipEvent_t graph_ev = nullptr;  // the last graph replay (its counter copy) has finished
+  bool graph_ev_live = false;
   int update_world_layout();
+  int update_first_shapes();
   bool use_per_world() const;
   int world_block_now() const;
   bool world_lds_now() const;
@@ -298,6 +303,7 @@ int b2g_batch::alloc() {
   HIP_TRY(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&counts_ev, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&graph_ev, hipEventDisableTiming));
   for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
   h_btype.assign(NB, 0);
   h_bworld.assign(NB, 0);
@@ -345,6 +351,7 @@ void b2g_batch::release() {
   if (fork_ev) (void)hipEventDestroy(fork_ev);
   if (join_ev) (void)hipEventDestroy(join_ev);
   if (counts_ev) (void)hipEventDestroy(counts_ev);
+  if (graph_ev) (void)hipEventDestroy(graph_ev);
   if (side) (void)hipStreamDestroy(side);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -643,7 +650,7 @@ void b2g_batch::enqueue_collide_and_colour(int K, int rounds) {
   const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
   const int gb = grid_for(nb);
   // narrowphase, and the bodies' velocity integration in the same launch
-  launch(B2G_KERNEL_NARROWPHASE, est_contacts, k_narrowphase, dim3(std::max(gc, gb)), dim3(kBlock), 0, d);
+  launch(B2G_KERNEL_NARROWPHASE, est_contacts, k_narrowphase, dim3(std::max(gc, gb)), dim3(kBlock), 0, d, 1);
   // solved contacts: scan of the touching flags and the compacted list (the
   // bound only picks the launch shape; any device count is handled)
   const long long touch_bound = std::max(est_contacts, 1) * 5LL / 4;
@@ -814,13 +821,19 @@ void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, c
     hipLaunchKernelGGL(k_finish_bodies, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
 }
 
+// the sticky error bits of the last counter read-back (pinned), as a status
+int b2g_batch::status_error() const {
+  const int s = pinned[C_STATUS];
+  if (s & S_CONTACT_OVERFLOW)
+    return fail(B2G_ERR_CAPACITY, "contact capacity exceeded (" + std::to_string(pinned[C_PAIRS]) + " pairs)");
+  if (s & S_STACK_OVERFLOW) return fail(B2G_ERR_CAPACITY, "BVH traversal stack overflow");
+  if (s & S_TREE_CORRUPT) return fail(B2G_ERR_INVALID, "BVH inconsistent (sort/build)");
+  return B2G_OK;
+}
+
 int b2g_batch::check_status() {
   est_contacts = pinned[C_CONTACTS];
-  if (pinned[C_STATUS] & S_CONTACT_OVERFLOW)
-    return fail(B2G_ERR_CAPACITY, "contact capacity exceeded (" + std::to_string(pinned[C_PAIRS]) + " pairs)");
-  if (pinned[C_STATUS] & S_STACK_OVERFLOW) return fail(B2G_ERR_CAPACITY, "BVH traversal stack overflow");
-  if (pinned[C_STATUS] & S_TREE_CORRUPT) return fail(B2G_ERR_INVALID, "BVH inconsistent (sort/build)");
-  return B2G_OK;
+  return status_error();
 }
 
 // World ranges for the per-world solver: valid when every world's bodies and
@@ -830,17 +843,46 @@ int b2g_batch::check_status() {
 // launch_solve_worlds)
 constexpr int kBigWorldBodies = 1024;
 
+// world-local shape numbering: first shape of every world; local indices
+// must fit 24 bits (the colouring rank packs two of them)
+int b2g_batch::update_first_shapes() {
+  std::vector<int>& w0 = h_wshape0;
+  std::fill(w0.begin(), w0.end(), INT_MAX);
+  for (int i = 0; i < d.ns; ++i) {
+    const int w = h_bworld[h_shape_body[i]];
+    w0[w] = std::min(w0[w], i);
+  }
+  for (int i = 0; i < d.ns; ++i)
+    if (i - w0[h_bworld[h_shape_body[i]]] >= (1 << 24))
+      return fail(B2G_ERR_CAPACITY, "a world spans more than 2^24 shape indices");
+  for (int& v : w0)
+    if (v == INT_MAX) v = 0;
+  HIP_TRY(hipMemcpy(d.wshape0, w0.data(), w0.size() * sizeof(int), hipMemcpyHostToDevice));
+  return B2G_OK;
+}
+
 int b2g_batch::update_world_layout() {
   const int nw = std::max(d.nw, 1), nb = d.nb, ns = d.ns;
+  // every body must belong to a declared world: the per-world tables below
+  // (and the kernels' world records) have d.nw entries
+  for (int b = 0; b < nb; ++b) {
+    const int w = h_bworld[b];
+    if (w < 0 || w >= d.nw)
+      return fail(B2G_ERR_INVALID, "body " + std::to_string(b) + " belongs to world " + std::to_string(w) +
+                                       ", but " + std::to_string(d.nw) +
+                                       " worlds are declared (b2g_set_worlds / b2g_set_world_params)");
+  }
   std::vector<int> b0(nw, INT_MAX), b1(nw, -1), bc(nw, 0), s0(nw, INT_MAX), s1(nw, -1), sc(nw, 0);
   for (int b = 0; b < nb; ++b) {
     const int w = h_bworld[b];
-    if (w < 0 || w >= nw) continue;
     b0[w] = std::min(b0[w], b);
     b1[w] = std::max(b1[w], b);
     ++bc[w];
   }
   for (int i = 0; i < ns; ++i) {
+    if (h_shape_body[i] < 0 || h_shape_body[i] >= nb)
+      return fail(B2G_ERR_INVALID, "shape " + std::to_string(i) + " refers to body " +
+                                       std::to_string(h_shape_body[i]) + " of " + std::to_string(nb));
     const int w = h_bworld[h_shape_body[i]];
     s0[w] = std::min(s0[w], i);
     s1[w] = std::max(s1[w], i);
@@ -953,17 +995,41 @@ int b2g_batch::step(int vi, int pi, unsigned flags) {
   }
   d.events_on = (flags & B2G_STEP_EVENTS) ? 1 : 0;
   d.segs = 0;  // an eager step sets it once its contact list is known
-  if (flags & B2G_STEP_SKIP_BROADPHASE) {  // bodies move without a rebuild or a pair query
+  const bool collide_only = (flags & B2G_STEP_COLLIDE_ONLY) != 0;
+  if ((flags & B2G_STEP_SKIP_BROADPHASE) && !collide_only) {  // bodies move without a rebuild or a pair query
     tree_stale = true;
     pairs_current = false;
   }
   if (d.events_on) hipLaunchKernelGGL(k_events_reset, dim3(1), dim3(1), 0, stream, d);
+  if (collide_only) return collide_only_step(flags);
   const bool prof_on = (flags & B2G_STEP_PROFILE) != 0;
   const bool graph_ok = (flags & B2G_STEP_GRAPH) && !prof_on && timed_kernel < 0 &&
                         !(flags & B2G_STEP_FIND_NEW_CONTACTS) && graph_K > 0 &&
                         (!use_per_world() || world_block > 0 || tuned_block > 0);
   if (graph_ok) return graph_step(vi, pi, flags);
   return eager_step(vi, pi, flags);
+}
+
+// b2World::Step with dt <= 0 (src/dynamics/b2_world.cpp:1114-1138, 1164-1167):
+// the broadphase when new fixtures wait for it, then Collide (manifolds,
+// impulse carry-over, touching flags and their events) and ClearForces. Solve
+// and its broadphase do not run, so bodies, joint impulses and the contacts'
+// stored impulses stay as they are.
+int b2g_batch::collide_only_step(unsigned flags) {
+  if (flags & B2G_STEP_FIND_NEW_CONTACTS) {
+    int rc = broadphase(false);
+    if (rc) return rc;
+  }
+  flush_finish();
+  const int gc = grid_for(std::max(est_contacts, 256) * 5 / 4);
+  launch(B2G_KERNEL_NARROWPHASE, est_contacts, k_narrowphase, dim3(gc), dim3(kBlock), 0, d, 0);
+  hipLaunchKernelGGL(k_clear_forces, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
+  if (!(flags & B2G_STEP_ASYNC)) {
+    int rc = read_counters();
+    if (rc) return rc;
+    return check_status();
+  }
+  return B2G_OK;
 }
 
 int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
@@ -982,6 +1048,7 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
   if (use_per_world()) {
     last_path = 2;
     if ((rc = read_counters())) return rc;
+    if ((rc = status_error())) return rc;  // an earlier (asynchronous) step overflowed
     const int pw_colours = pinned[C_NCOLOURS];  // last step's (normal colours are what it reports)
     for (int extra = 0; pinned[C_REMAIN] > 0; extra += 2) {  // colour everything first
       if (extra > 4096) return fail(B2G_ERR_CAPACITY, "colouring did not converge");
@@ -1049,6 +1116,7 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
   HIP_TRY(hipEventRecord(counts_ev, stream));
   enqueue_islands_and_init();
   HIP_TRY(hipEventSynchronize(counts_ev));
+  if ((rc = status_error())) return rc;  // an earlier (asynchronous) step overflowed
   est_cons = pinned[C_CONS];
   for (int extra = 0; pinned[C_REMAIN] > 0; extra += 2) {  // first steps / bursts of new contacts
     if (extra > 4096) return fail(B2G_ERR_CAPACITY, "colouring did not converge");
@@ -1140,6 +1208,13 @@ int b2g_batch::graph_step(int vi, int pi, unsigned flags) {
   const unsigned key_flags = flags & (B2G_STEP_SKIP_BROADPHASE | B2G_STEP_EVENTS);
   const int block = use_per_world() ? world_block_now() * 2 + (world_lds_now() ? 1 : 0) : 0;
   last_path = block ? 2 : 1;
+  if (graph_ev_live && hipEventQuery(graph_ev) == hipSuccess) {
+    // the last replay and its counter copy have finished: refuse to build on
+    // a step that overflowed
+    graph_ev_live = false;
+    int rc = status_error();
+    if (rc) return rc;
+  }
   if (filters_dirty) refresh_filters();  // outside the captured sequence
   GraphEntry* ge = nullptr;
   for (auto& g : graphs)
@@ -1172,6 +1247,8 @@ int b2g_batch::graph_step(int vi, int pi, unsigned flags) {
     ge = &graphs.back();
   }
   HIP_TRY(hipGraphLaunch(ge->exec, stream));
+  HIP_TRY(hipEventRecord(graph_ev, stream));
+  graph_ev_live = true;
   // mirror what the replay does to the host view
   if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
     swap_contact_buffers();
@@ -1192,7 +1269,8 @@ int b2g_batch::graph_step(int vi, int pi, unsigned flags) {
 // ---------------------------------------------------------------------------
 // record conversion helpers
 // ---------------------------------------------------------------------------
-static void shape_to_device(const b2g_shape& s, int world, int4* hdr, float4* geo, float4* mat, int4* filt) {
+__host__ __device__ static inline void shape_to_device(const b2g_shape& s, int world, int4* hdr, float4* geo,
+                                                       float4* mat, int4* filt) {
   *hdr = make_int4(s.type, s.body, s.count, world);
   geo[0] = make_float4(s.radius, s.centroid[0], s.centroid[1], 0.0f);
   for (int k = 0; k < 4; ++k) {
@@ -1203,7 +1281,7 @@ static void shape_to_device(const b2g_shape& s, int world, int4* hdr, float4* ge
   *filt = make_int4(s.category, s.mask, s.group, s.is_sensor);
 }
 
-static void manifold_to_record(const float4* m, b2g_contact* o) {
+__host__ __device__ static inline void manifold_to_record(const float4* m, b2g_contact* o) {
   const int packed = __builtin_bit_cast(int, m[3].z);
   o->type = packed & 0xff;
   o->point_count = (packed >> 8) & 0xff;
@@ -1324,6 +1402,39 @@ int b2g_set_worlds(b2g_batch* b, int32_t count, const int32_t* body_begin, const
   return b2g_set_world_params(b, 0, count, params);
 }
 
+int b2g_set_counts(b2g_batch* b, int32_t bodies, int32_t shapes, int32_t joints) {
+  if (!b || bodies < 0 || shapes < 0 || joints < 0 || bodies > b->cap.bodies || shapes > b->cap.shapes ||
+      joints > b->cap.joints)
+    return fail(B2G_ERR_INVALID, "record counts outside the capacity");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->flush_finish();
+  Dev& d = b->d;
+  d.nb = bodies;
+  d.ns = shapes;
+  d.nj = joints;
+  for (int i = 0; i < shapes; ++i)  // records above the old count are uploaded next; keep them in range
+    if (b->h_shape_body[i] >= bodies) b->h_shape_body[i] = 0;
+  if ((int)b->h_shape_rec.size() > shapes) b->h_shape_rec.resize(shapes);
+  b->h_joints.resize(joints);
+  for (b2g_revolute& j : b->h_joints) {
+    if (j.body_a >= bodies) j.body_a = 0;
+    if (j.body_b >= bodies) j.body_b = 0;
+  }
+  // the contact list named the old indices: the caller uploads the survivors
+  const int zero = 0;
+  HIP_TRY(hipMemcpy(d.cnt + C_CONTACTS, &zero, sizeof(int), hipMemcpyHostToDevice));
+  b->est_contacts = 0;
+  b->graphs_clear();
+  b->graph_K = 0;
+  b->filters_dirty = true;
+  b->pairs_current = false;
+  b->layout_dirty = true;
+  b->joints_dirty = true;
+  b->tree_stale = true;
+  b->tree_ns = -1;
+  return bodies > 0 && shapes > 0 ? b->update_first_shapes() : B2G_OK;
+}
+
 int b2g_upload_bodies(b2g_batch* b, int32_t first, int32_t count, const b2g_body* in) {
   if (!b || first < 0 || count < 0 || first + count > b->cap.bodies) return fail(B2G_ERR_INVALID, "body range");
   if (count == 0) return B2G_OK;
@@ -1411,21 +1522,7 @@ int b2g_upload_shapes(b2g_batch* b, int32_t first, int32_t count, const b2g_shap
   HIP_TRY(hipMemcpyAsync(d.sfilt + first, filt.data(), count * sizeof(int4), hipMemcpyHostToDevice, st));
   HIP_TRY(hipStreamSynchronize(st));
   d.ns = std::max(d.ns, first + count);
-  // world-local shape numbering: first shape of every world; local indices
-  // must fit 24 bits (the colouring rank packs two of them)
-  std::vector<int>& w0 = b->h_wshape0;
-  std::fill(w0.begin(), w0.end(), INT_MAX);
-  for (int i = 0; i < d.ns; ++i) {
-    const int w = b->h_bworld[b->h_shape_body[i]];
-    w0[w] = std::min(w0[w], i);
-  }
-  for (int i = 0; i < d.ns; ++i)
-    if (i - w0[b->h_bworld[b->h_shape_body[i]]] >= (1 << 24))
-      return fail(B2G_ERR_CAPACITY, "a world spans more than 2^24 shape indices");
-  for (int& v : w0)
-    if (v == INT_MAX) v = 0;
-  HIP_TRY(hipMemcpy(d.wshape0, w0.data(), w0.size() * sizeof(int), hipMemcpyHostToDevice));
-  return B2G_OK;
+  return b->update_first_shapes();
 }
 
 int b2g_upload_joints(b2g_batch* b, int32_t first, int32_t count, const b2g_revolute* in) {
@@ -1514,10 +1611,7 @@ int b2g_batch_wait(b2g_batch* b) {
   if (!b) return fail(B2G_ERR_INVALID, "null batch");
   int rc = b->read_counters();
   if (rc) return rc;
-  b->est_contacts = b->pinned[C_CONTACTS];
-  if (b->pinned[C_STATUS] & S_CONTACT_OVERFLOW) return fail(B2G_ERR_CAPACITY, "contact capacity exceeded");
-  if (b->pinned[C_STATUS] & S_STACK_OVERFLOW) return fail(B2G_ERR_CAPACITY, "BVH traversal stack overflow");
-  return B2G_OK;
+  return b->check_status();
 }
 
 int b2g_get_counts(b2g_batch* b, b2g_counts* out) {
@@ -1674,6 +1768,20 @@ int b2g_download_contacts(b2g_batch* b, int32_t first, int32_t count, b2g_contac
   return B2G_OK;
 }
 
+int b2g_download_solver_points(b2g_batch* b, int32_t first, int32_t count, int32_t* out) {
+  if (!b || (count && !out)) return fail(B2G_ERR_INVALID, "null argument");
+  int rc = b->read_counters();
+  if (rc) return rc;
+  if (first < 0 || count < 0 || first + count > b->pinned[C_CONTACTS]) return fail(B2G_ERR_INVALID, "contact range");
+  if (count == 0) return B2G_OK;
+  // the fourth float4 of each manifold (ids, packed type / counts)
+  std::vector<float4> m3(count);
+  HIP_TRY(hipMemcpy2D(m3.data(), sizeof(float4), b->d.cman + 4 * (size_t)first + 3, 4 * sizeof(float4),
+                      sizeof(float4), count, hipMemcpyDeviceToHost));
+  for (int i = 0; i < count; ++i) out[i] = (__builtin_bit_cast(int, m3[i].z) >> 16) & 0xff;
+  return B2G_OK;
+}
+
 int b2g_download_contact_colours(b2g_batch* b, int32_t first, int32_t count, int32_t* out) {
   if (!b || (count && !out)) return fail(B2G_ERR_INVALID, "null argument");
   int rc = b->read_counters();
@@ -1777,6 +1885,8 @@ int b2g_download_events(b2g_batch* b, int32_t capacity, b2g_contact_event* out, 
     b2g_contact_event& o = ev[i];
     std::memset(&o, 0, sizeof(o));
     o.kind = __builtin_bit_cast(int, e[0].z);
+    // the step's stored point count travels in the packed manifold word
+    o.solver_points = o.kind == B2G_EVENT_DESTROYED ? (__builtin_bit_cast(int, e[4].z) >> 16) & 0xff : 0;
     o.contact.shape_a = __builtin_bit_cast(int, e[0].x);
     o.contact.shape_b = __builtin_bit_cast(int, e[0].y);
     manifold_to_record(e + 1, &o.contact);
@@ -2031,3 +2141,4 @@ int b2g_find_pairs(int32_t n, const float* aabbs, const int32_t* shape_body, con
 }
 
 }  // extern "C"
+

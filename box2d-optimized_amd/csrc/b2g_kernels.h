@@ -303,8 +303,9 @@ __device__ __forceinline__ void integrate_velocity(const Dev& d, int b) {
 // ===========================================================================
 // The same launch integrates the bodies' velocities (b2Island::Solve,
 // b2_island.cpp:257-298), which touches none of the contact data, and clears
-// the step's colour counters.
-__global__ void __launch_bounds__(kBlock) k_narrowphase(Dev d) {
+// the step's colour counters. A step with dt <= 0 runs Collide alone
+// (integrate = 0).
+__global__ void __launch_bounds__(kBlock) k_narrowphase(Dev d, int integrate) {
   if (blockIdx.x == 0) reset_colour_counters(d);
   const int n = d.cnt[C_CONTACTS];
   B2G_FOR(i, 0, n) {
@@ -359,7 +360,7 @@ __global__ void __launch_bounds__(kBlock) k_narrowphase(Dev d) {
     }
     d.touch[i] = solve;
   }
-  B2G_FOR(b, 0, d.nb) integrate_velocity(d, b);
+  if (integrate) B2G_FOR(b, 0, d.nb) integrate_velocity(d, b);
 }
 
 // compacted list of solved contacts (order preserved)
@@ -1241,13 +1242,19 @@ __global__ void __launch_bounds__(kBlock) k_contact_velocity(Dev d, int colour, 
   B2G_FOR(s, begin, end) contact_velocity(d, s);
 }
 
-// b2ContactSolver::StoreImpulses (:641-657): only the vc pointCount points
+// b2ContactSolver::StoreImpulses (:641-657): only the vc pointCount points.
+// That count (the b2ContactImpulse count b2Island::Report hands to PostSolve,
+// b2_island.cpp:621-647) goes to bits 16-23 of the manifold's packed word;
+// the narrowphase rewrites the word every step, so the bits name the points
+// the last step solved (0: not solved).
 __device__ __forceinline__ void store_impulses(const Dev& d, int s) {
   {
     const float4 v2i = d.vc[2][s];
     const int count = as_int(v2i.z) & 0xff, c = as_int(v2i.w);
     const float4 q0 = d.vc[4][s];
     float4* mp = d.cman + 4 * (size_t)c;
+    int* packed = reinterpret_cast<int*>(mp + 3) + 2;
+    *packed = (*packed & 0xffff) | (count << 16);
     float4 m0 = mp[0];
     m0.z = q0.x;
     m0.w = q0.y;
@@ -1804,6 +1811,13 @@ __device__ __forceinline__ void finish_step(const Dev& d) {
   }
 }
 __global__ void __launch_bounds__(kBlock) k_finish_bodies(Dev d) { finish_step(d); }
+
+// ClearForces alone (b2_world.cpp:1173-1180), for a step without a solve
+__global__ void __launch_bounds__(kBlock) k_clear_forces(Dev d) {
+  B2G_FOR(b, 0, d.nb) {
+    if (world_active(d, d.bworld[b])) d.force[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+}
 
 
 // ===========================================================================

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libbox2d_amd.so")
 # status codes (include/b2g.h)
 OK, ERR_HIP, ERR_OOM, ERR_CAPACITY, ERR_INVALID, ERR_UNSUPPORTED, ERR_NO_DEVICE = range(7)
 STEP_FIND_NEW_CONTACTS, STEP_PROFILE, STEP_ASYNC, STEP_SKIP_BROADPHASE, STEP_GRAPH, STEP_EVENTS = 1, 2, 4, 8, 16, 32
+STEP_COLLIDE_ONLY = 64
 EVENT_BEGIN, EVENT_END, EVENT_DESTROYED = 1, 2, 3
 
 BODY_DTYPE = np.dtype([
@@ -84,6 +85,8 @@ EXPORTS = [
     "b2g_time_kernel", "b2g_kernel_time", "b2a_scene_records", "b2a_batch_from_scenes", "b2s_run",
     "b2g_set_option", "b2g_get_option", "b2g_upload_contact_colours", "b2g_snapshot_save",
     "b2g_snapshot_load", "b2g_download_events", "b2g_query_aabbs", "b2g_raycast",
+    "b2g_set_counts", "b2g_download_solver_points", "b2a_world_destroy_body", "b2a_world_destroy_joint",
+    "b2a_world_destroy_fixture", "b2a_world_set_filter", "b2a_world_body_count",
 ]
 SOLVER_AUTO, SOLVER_COLOURS, SOLVER_WORLDS = 0, 1, 2
 OPT_SOLVER, OPT_WORLD_BLOCK, OPT_TUNED_WORLD_BLOCK, OPT_LAST_SOLVER, OPT_PER_WORLD_ELIGIBLE = 1, 2, 3, 4, 5
@@ -92,7 +95,7 @@ KERNEL_NARROWPHASE, KERNEL_WARM_START, KERNEL_CONTACT_VELOCITY, KERNEL_CONTACT_P
     KERNEL_PAIRS_WRITE, KERNEL_SOLVER = range(6)
 
 
-EVENT_DTYPE = np.dtype([("kind", "<i4"), ("reserved", "<i4"), ("contact", CONTACT_DTYPE)])
+EVENT_DTYPE = np.dtype([("kind", "<i4"), ("solver_points", "<i4"), ("contact", CONTACT_DTYPE)])
 RAY_HIT_DTYPE = np.dtype([("shape", "<i4"), ("fraction", "<f4"), ("normal", "<f4", 2)])
 REQUEST_DTYPE = np.dtype([("arrival", "<f8"), ("world", "<i4"), ("substeps", "<i4"), ("dt", "<f4"),
                           ("reserved", "<i4")])
@@ -131,7 +134,7 @@ def lib():
         L.b2g_set_world_params.argtypes = [_P, _I, _I, _P]
         for f in ("b2g_upload_bodies", "b2g_upload_shapes", "b2g_upload_joints",
                   "b2g_download_bodies", "b2g_download_contacts", "b2g_download_joints",
-                  "b2g_download_contact_colours"):
+                  "b2g_download_contact_colours", "b2g_download_solver_points"):
             getattr(L, f).argtypes = [_P, _I, _I, _P]
         L.b2g_upload_contacts.argtypes = [_P, _I, _P]
         L.b2g_batch_step.argtypes = [_P, _I, _I, ctypes.c_uint32]
@@ -162,6 +165,11 @@ def lib():
         L.b2g_raycast.argtypes = [_P, _I, _P, _P, _I, _P, _P]
         L.b2g_snapshot_load.argtypes = [_P, ctypes.c_char_p]
         L.b2g_get_option.argtypes = [_P, _I, _P]
+        L.b2g_set_counts.argtypes = [_P, _I, _I, _I]
+        for f in ("b2a_world_destroy_body", "b2a_world_destroy_joint", "b2a_world_destroy_fixture"):
+            getattr(L, f).argtypes = [_P, _I]
+        L.b2a_world_set_filter.argtypes = [_P, _I, _I, _I, _I]
+        L.b2a_world_body_count.argtypes = [_P, _P]
         _lib = L
     return _lib
 
@@ -388,6 +396,20 @@ class Batch:
             check(self.L.b2g_download_contacts(self.h, 0, n, _ptr(out)))
         return out
 
+    def set_counts(self, bodies, shapes, joints):
+        check(self.L.b2g_set_counts(self.h, bodies, shapes, joints))
+
+    def solver_points(self, n):
+        """Points the last step's solver stored per contact (PostSolve's count; 0 = not solved)."""
+        out = np.zeros(n, dtype=np.int32)
+        if n:
+            check(self.L.b2g_download_solver_points(self.h, 0, n, _ptr(out)))
+        return out
+
+    def upload_contact_colours(self, colours):
+        c = np.ascontiguousarray(colours, dtype=np.int32)
+        check(self.L.b2g_upload_contact_colours(self.h, len(c), _ptr(c) if len(c) else None))
+
     def contact_colours(self, n):
         out = np.zeros(n, dtype=np.int32)
         if n:
@@ -459,6 +481,28 @@ class World:
     def step(self, n=1, dt=1.0 / 60.0, vi=8, pi=3):
         for _ in range(n):
             check(self.L.b2a_world_step(self.h, dt, vi, pi))
+
+    def step_status(self, dt=1.0 / 60.0, vi=8, pi=3) -> int:
+        """One b2World::Step; returns its status instead of raising."""
+        return self.L.b2a_world_step(self.h, dt, vi, pi)
+
+    def destroy_body(self, body):
+        check(self.L.b2a_world_destroy_body(self.h, body))
+        self._recount()
+
+    def destroy_joint(self, joint):
+        check(self.L.b2a_world_destroy_joint(self.h, joint))
+
+    def destroy_fixture(self, fixture):
+        check(self.L.b2a_world_destroy_fixture(self.h, fixture))
+
+    def set_filter(self, fixture, category, mask, group):
+        check(self.L.b2a_world_set_filter(self.h, fixture, category, mask, group))
+
+    def _recount(self):
+        n = ctypes.c_int32(0)
+        check(self.L.b2a_world_body_count(self.h, ctypes.byref(n)))
+        self.nb = n.value
 
     def bodies(self):
         """10 floats per body: centre c (2), angle, v (2), w, transform (p 2, q.s, q.c)."""

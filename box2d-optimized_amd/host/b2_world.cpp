@@ -6,6 +6,7 @@
 // reference's single-precision expressions (files cited per function) so the
 // device receives the same shapes and mass properties the reference computes.
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <unordered_set>
 
@@ -284,6 +285,10 @@ b2Body::b2Body(const b2BodyDef* bd, b2World* world, int32 index) {  // b2_body.c
   m_angularDamping = bd->angularDamping;
   m_gravityScale = bd->gravityScale;
   m_fixedRotation = bd->fixedRotation;
+  m_bullet = bd->bullet;
+  m_sleepingAllowed = bd->allowSleep;
+  m_awake = bd->awake && bd->type != b2_staticBody;
+  m_enabled = bd->enabled;
   m_userData = bd->userData;
 }
 
@@ -359,6 +364,8 @@ b2Fixture* b2Body::CreateFixture(const b2FixtureDef* def) {  // b2_body.cpp:230-
   m_fixtureList = f;
   ++m_fixtureCount;
   m_world->m_fixtures.push_back(f);
+  m_world->m_refilter.push_back(0);
+  if (f->m_isSensor) ++m_world->m_sensorFixtures;
   if (f->m_density > 0.0f) ResetMassData();
   MarkDirty();
   m_world->m_newContacts = true;
@@ -370,6 +377,147 @@ b2Fixture* b2Body::CreateFixture(const b2Shape* shape, float density) {
   def.shape = shape;
   def.density = density;
   return CreateFixture(&def);
+}
+
+// b2_body.cpp:129-226
+void b2Body::SetType(b2BodyType type) {
+  b2Assert(!m_world->IsLocked());
+  if (m_world->IsLocked() || m_type == type) return;
+  b2World* w = m_world;
+  w->BeginEdit();
+  const bool was_static = m_type == b2_staticBody;
+  if (was_static || type == b2_staticBody) {  // statics sit at the tail of the body list
+    if (m_prev) m_prev->m_next = m_next; else w->m_bodyListHead = m_next;
+    if (m_next) m_next->m_prev = m_prev; else w->m_bodyListTail = m_prev;
+    m_prev = m_next = nullptr;
+    if (type == b2_staticBody) {
+      m_prev = w->m_bodyListTail;
+      if (w->m_bodyListTail) w->m_bodyListTail->m_next = this; else w->m_bodyListHead = this;
+      w->m_bodyListTail = this;
+    } else {
+      m_next = w->m_bodyListHead;
+      if (w->m_bodyListHead) w->m_bodyListHead->m_prev = this; else w->m_bodyListTail = this;
+      w->m_bodyListHead = this;
+    }
+  }
+  if (!m_awake && m_type != b2_staticBody) --w->m_sleepingBodies;
+  m_type = type;
+  ResetMassData();
+  if (m_type == b2_staticBody) {
+    m_linearVelocity.SetZero();
+    m_angularVelocity = 0.0f;
+    m_awake = false;
+  } else {
+    m_awake = true;  // SetAwake(true)
+  }
+  m_force.SetZero();
+  m_torque = 0.0f;
+  // the body's contacts are destroyed (EndContact for touching ones)
+  const b2Body* self = this;
+  w->RemoveContactsIf(
+      [](const b2World* wd, const b2g_contact& c, const void* arg) {
+        const b2Body* b = static_cast<const b2Body*>(arg);
+        return wd->m_fixtures[c.shape_a]->m_body == b || wd->m_fixtures[c.shape_b]->m_body == b;
+      },
+      self);
+  MarkDirty();
+}
+
+// b2_body.cpp:280-350
+void b2Body::DestroyFixture(b2Fixture* fixture) {
+  if (fixture == nullptr) return;
+  b2Assert(!m_world->IsLocked());
+  if (m_world->IsLocked()) return;
+  b2Assert(fixture->m_body == this);
+  b2Fixture** node = &m_fixtureList;
+  bool found = false;
+  while (*node) {
+    if (*node == fixture) {
+      *node = fixture->m_next;
+      found = true;
+      break;
+    }
+    node = &(*node)->m_next;
+  }
+  b2Assert(found);
+  if (!found) return;
+  b2World* w = m_world;
+  w->BeginEdit();
+  const int32 index = fixture->m_index;
+  w->RemoveContactsIf(
+      [](const b2World*, const b2g_contact& c, const void* arg) {
+        const int32 k = *static_cast<const int32*>(arg);
+        return c.shape_a == k || c.shape_b == k;
+      },
+      &index);
+  if (fixture->m_isSensor) --w->m_sensorFixtures;
+  w->RemoveFixtureIndex(index);
+  fixture->m_body = nullptr;
+  fixture->m_next = nullptr;
+  delete fixture;
+  --m_fixtureCount;
+  ResetMassData();
+  MarkDirty();
+}
+
+// b2_body.cpp:418-455
+void b2Body::SetMassData(const b2MassData* massData) {
+  b2Assert(!m_world->IsLocked());
+  if (m_world->IsLocked() || m_type != b2_dynamicBody) return;
+  Sync();
+  m_invMass = 0.0f;
+  m_I = 0.0f;
+  m_invI = 0.0f;
+  m_mass = massData->mass;
+  if (m_mass <= 0.0f) m_mass = 1.0f;
+  m_invMass = 1.0f / m_mass;
+  if (massData->I > 0.0f && !m_fixedRotation) {
+    m_I = massData->I - m_mass * b2Dot(massData->center, massData->center);
+    b2Assert(m_I > 0.0f);
+    m_invI = 1.0f / m_I;
+  }
+  b2Vec2 oldCenter = m_c;
+  m_localCenter = massData->center;
+  m_c = b2Mul(m_xf, m_localCenter);
+  m_linearVelocity += b2Cross(m_angularVelocity, m_c - oldCenter);
+  MarkDirty();
+}
+
+// include/box2d/b2_body.h:719-740
+void b2Body::SetAwake(bool flag) {
+  if (m_type == b2_staticBody) return;
+  Sync();
+  if (flag) {
+    if (!m_awake) --m_world->m_sleepingBodies;
+    m_awake = true;
+  } else {
+    if (m_awake) ++m_world->m_sleepingBodies;
+    m_awake = false;
+    m_linearVelocity.SetZero();
+    m_angularVelocity = 0.0f;
+    m_force.SetZero();
+    m_torque = 0.0f;
+  }
+  MarkDirty();
+}
+
+// b2_body.cpp:512-558: enabling lets the next Step find the body's contacts
+void b2Body::SetEnabled(bool flag) {
+  b2Assert(!m_world->IsLocked());
+  if (flag == m_enabled) return;
+  m_enabled = flag;
+  m_world->m_disabledBodies += flag ? -1 : 1;
+  if (flag) m_world->m_newContacts = true;
+}
+
+// b2_body.cpp:560-580
+void b2Body::SetFixedRotation(bool flag) {
+  if (flag == m_fixedRotation) return;
+  Sync();
+  m_fixedRotation = flag;
+  m_angularVelocity = 0.0f;
+  ResetMassData();
+  MarkDirty();
 }
 
 void b2Body::SetTransform(const b2Vec2& position, float angle) {  // b2_body.cpp:486-504
@@ -411,9 +559,12 @@ void b2Body::SetAngularVelocity(float w) {
   MarkDirty();
 }
 
+// The appliers wake a sleeping body when asked and do nothing to a body that
+// stays asleep (include/box2d/b2_body.h:803-905).
 void b2Body::ApplyForce(const b2Vec2& force, const b2Vec2& point, bool wake) {
-  B2_NOT_USED(wake);
   if (m_type != b2_dynamicBody) return;
+  if (wake && !m_awake) SetAwake(true);
+  if (!m_awake) return;
   Sync();
   m_force += force;
   m_torque += b2Cross(point - m_c, force);
@@ -421,24 +572,27 @@ void b2Body::ApplyForce(const b2Vec2& force, const b2Vec2& point, bool wake) {
 }
 
 void b2Body::ApplyForceToCenter(const b2Vec2& force, bool wake) {
-  B2_NOT_USED(wake);
   if (m_type != b2_dynamicBody) return;
+  if (wake && !m_awake) SetAwake(true);
+  if (!m_awake) return;
   Sync();
   m_force += force;
   MarkDirty();
 }
 
 void b2Body::ApplyTorque(float torque, bool wake) {
-  B2_NOT_USED(wake);
   if (m_type != b2_dynamicBody) return;
+  if (wake && !m_awake) SetAwake(true);
+  if (!m_awake) return;
   Sync();
   m_torque += torque;
   MarkDirty();
 }
 
 void b2Body::ApplyLinearImpulse(const b2Vec2& impulse, const b2Vec2& point, bool wake) {
-  B2_NOT_USED(wake);
   if (m_type != b2_dynamicBody) return;
+  if (wake && !m_awake) SetAwake(true);
+  if (!m_awake) return;
   Sync();
   m_linearVelocity += m_invMass * impulse;
   m_angularVelocity += m_invI * b2Cross(point - m_c, impulse);
@@ -446,16 +600,18 @@ void b2Body::ApplyLinearImpulse(const b2Vec2& impulse, const b2Vec2& point, bool
 }
 
 void b2Body::ApplyLinearImpulseToCenter(const b2Vec2& impulse, bool wake) {
-  B2_NOT_USED(wake);
   if (m_type != b2_dynamicBody) return;
+  if (wake && !m_awake) SetAwake(true);
+  if (!m_awake) return;
   Sync();
   m_linearVelocity += m_invMass * impulse;
   MarkDirty();
 }
 
 void b2Body::ApplyAngularImpulse(float impulse, bool wake) {
-  B2_NOT_USED(wake);
   if (m_type != b2_dynamicBody) return;
+  if (wake && !m_awake) SetAwake(true);
+  if (!m_awake) return;
   Sync();
   m_angularVelocity += m_invI * impulse;
   MarkDirty();
@@ -485,6 +641,78 @@ void b2Body::FromRecord(const b2g_body& r) {
   m_force.Set(r.force[0], r.force[1]);
   m_torque = r.torque;
 }
+
+void b2Fixture::MarkShapeDirty() {
+  if (m_body) m_body->m_world->m_shapesDirty = true;
+}
+
+// b2_fixture.cpp:142-183
+void b2Fixture::SetFilterData(const b2Filter& filter) {
+  m_filter = filter;
+  Refilter();
+}
+
+void b2Fixture::Refilter() {
+  if (m_body == nullptr) return;
+  b2World* w = m_body->m_world;
+  w->m_refilter[m_index] = 1;
+  w->m_anyRefilter = true;
+  w->m_shapesDirty = true;
+}
+
+void b2Fixture::SetSensor(bool sensor) {  // b2_fixture.cpp:185-194
+  if (sensor == m_isSensor) return;
+  if (m_body) {
+    m_body->SetAwake(true);
+    m_body->m_world->m_sensorFixtures += sensor ? 1 : -1;
+  }
+  m_isSensor = sensor;
+}
+
+void b2Fixture::SetDensity(float density) { m_density = density; }  // ResetMassData applies it
+void b2Fixture::SetFriction(float friction) { m_friction = friction; MarkShapeDirty(); }
+void b2Fixture::SetRestitution(float restitution) { m_restitution = restitution; MarkShapeDirty(); }
+void b2Fixture::SetRestitutionThreshold(float threshold) { m_restitutionThreshold = threshold; MarkShapeDirty(); }
+
+// src/dynamics/b2_world_callbacks.cpp:28-40
+bool b2ContactFilter::ShouldCollide(b2Fixture* fixtureA, b2Fixture* fixtureB) {
+  const b2Filter& filterA = fixtureA->GetFilterData();
+  const b2Filter& filterB = fixtureB->GetFilterData();
+  if (filterA.groupIndex == filterB.groupIndex && filterA.groupIndex != 0) return filterA.groupIndex > 0;
+  return (filterA.maskBits & filterB.categoryBits) != 0 && (filterA.categoryBits & filterB.maskBits) != 0;
+}
+
+void b2ContactListener::PreSolve(b2Contact* contact, const b2Manifold* oldManifold) {
+  B2_NOT_USED(contact);
+  B2_NOT_USED(oldManifold);
+}
+
+void b2ContactListener::PostSolve(b2Contact* contact, const b2ContactImpulse* impulse) {
+  B2_NOT_USED(contact);
+  B2_NOT_USED(impulse);
+}
+
+// Whether an object's class overrides a virtual function: the function a
+// virtual call on the object reaches, against the one it reaches on the base
+// class (GNU C++'s bound member function pointer, taken in this file only).
+#pragma GCC diagnostic push
+#pragma GCC diagnostic ignored "-Wpmf-conversions"
+static bool overrides_pre_solve(b2ContactListener* l) {
+  static b2ContactListener base;
+  typedef void (*Fn)(b2ContactListener*, b2Contact*, const b2Manifold*);
+  return (Fn)(l->*(&b2ContactListener::PreSolve)) != (Fn)(base.*(&b2ContactListener::PreSolve));
+}
+static bool overrides_post_solve(b2ContactListener* l) {
+  static b2ContactListener base;
+  typedef void (*Fn)(b2ContactListener*, b2Contact*, const b2ContactImpulse*);
+  return (Fn)(l->*(&b2ContactListener::PostSolve)) != (Fn)(base.*(&b2ContactListener::PostSolve));
+}
+static bool overrides_should_collide(b2ContactFilter* f) {
+  static b2ContactFilter base;
+  typedef bool (*Fn)(b2ContactFilter*, b2Fixture*, b2Fixture*);
+  return (Fn)(f->*(&b2ContactFilter::ShouldCollide)) != (Fn)(base.*(&b2ContactFilter::ShouldCollide));
+}
+#pragma GCC diagnostic pop
 
 // ---------------------------------------------------------------------------
 // joints
@@ -566,16 +794,187 @@ b2Body* b2World::CreateBody(const b2BodyDef* def) {  // b2_world.cpp:140-176
   m_bodies.push_back(b);
   m_dirty.push_back(1);
   m_anyDirty = true;
+  if (!b->m_awake && b->m_type != b2_staticBody) ++m_sleepingBodies;
+  if (!b->m_enabled) ++m_disabledBodies;
   // statics at the tail, others at the head of the body list
   if (b->m_type == b2_staticBody) {
+    b->m_prev = m_bodyListTail;
     if (m_bodyListTail) m_bodyListTail->m_next = b; else m_bodyListHead = b;
     m_bodyListTail = b;
   } else {
     b->m_next = m_bodyListHead;
+    if (m_bodyListHead) m_bodyListHead->m_prev = b; else m_bodyListTail = b;
     m_bodyListHead = b;
-    if (!m_bodyListTail) m_bodyListTail = b;
   }
   return b;
+}
+
+// b2_world.cpp:178-266
+void b2World::DestroyBody(b2Body* b) {
+  b2Assert(!m_bodies.empty());
+  b2Assert(!IsLocked());
+  if (IsLocked() || b == nullptr || b->m_world != this) return;
+  BeginEdit();
+  // attached joints first
+  for (size_t q = 0; q < m_joints.size();) {
+    b2Joint* j = m_joints[q];
+    if (j->m_bodyA == b || j->m_bodyB == b) {
+      if (m_destructionListener) m_destructionListener->SayGoodbye(j);
+      DestroyJoint(j);
+    } else {
+      ++q;
+    }
+  }
+  // then the contacts (EndContact for touching ones)
+  RemoveContactsIf(
+      [](const b2World* w, const b2g_contact& c, const void* arg) {
+        const b2Body* body = static_cast<const b2Body*>(arg);
+        return w->m_fixtures[c.shape_a]->m_body == body || w->m_fixtures[c.shape_b]->m_body == body;
+      },
+      b);
+  // then the fixtures
+  b2Fixture* f = b->m_fixtureList;
+  while (f) {
+    b2Fixture* f0 = f;
+    f = f->m_next;
+    if (m_destructionListener) m_destructionListener->SayGoodbye(f0);
+    if (f0->m_isSensor) --m_sensorFixtures;
+    RemoveFixtureIndex(f0->m_index);
+    delete f0;
+    b->m_fixtureList = f;
+    --b->m_fixtureCount;
+  }
+  b->m_fixtureList = nullptr;
+  b->m_fixtureCount = 0;
+  if (!b->m_awake && b->m_type != b2_staticBody) --m_sleepingBodies;
+  if (!b->m_enabled) --m_disabledBodies;
+  if (b->m_prev) b->m_prev->m_next = b->m_next; else m_bodyListHead = b->m_next;
+  if (b->m_next) b->m_next->m_prev = b->m_prev; else m_bodyListTail = b->m_prev;
+  const int32 index = b->m_index;
+  m_bodies.erase(m_bodies.begin() + index);
+  m_dirty.erase(m_dirty.begin() + index);
+  for (size_t i = index; i < m_bodies.size(); ++i) m_bodies[i]->m_index = (int32)i;
+  delete b;
+}
+
+// b2_world.cpp:326-418. A joint with collideConnected == false kept its
+// bodies from colliding; the next broadphase may now pair them.
+void b2World::DestroyJoint(b2Joint* j) {
+  b2Assert(!IsLocked());
+  if (IsLocked() || j == nullptr) return;
+  BeginEdit();
+  if (j->m_prev) j->m_prev->m_next = j->m_next; else m_jointList = j->m_next;
+  if (j->m_next) j->m_next->m_prev = j->m_prev;
+  j->m_bodyA->SetAwake(true);
+  j->m_bodyB->SetAwake(true);
+  const int32 index = j->m_index;
+  m_joints.erase(m_joints.begin() + index);
+  for (size_t i = index; i < m_joints.size(); ++i) m_joints[i]->m_index = (int32)i;
+  delete j;
+}
+
+void b2World::SetContactListener(b2ContactListener* listener) {
+  m_listener = listener;
+  m_listenerPreSolve = listener != nullptr && overrides_pre_solve(listener);
+  m_listenerPostSolve = listener != nullptr && overrides_post_solve(listener);
+}
+
+// Everything that changes the contact list or renumbers records outside a
+// step starts here: the device's bodies, joint impulses and contacts (with
+// their colours, so the solve order of the survivors is kept) come to the
+// host once; EnsureDevice sends them back renumbered.
+void b2World::BeginEdit() {
+  if (m_editing) return;
+  m_editing = true;
+  m_editContacts.clear();
+  m_editColours.clear();
+  if (!m_batch) return;
+  SyncBodies();
+  SyncJoints();
+  const int n = b2g_download_contacts(m_batch, 0, 0, nullptr);
+  if (n > 0) {
+    m_editContacts.resize(n);
+    m_editColours.resize(n);
+    if (b2g_download_contacts(m_batch, 0, n, m_editContacts.data()) != B2G_OK ||
+        b2g_download_contact_colours(m_batch, 0, n, m_editColours.data()) != B2G_OK) {
+      m_editContacts.clear();
+      m_editColours.clear();
+    }
+  }
+  m_contactsValid = false;
+}
+
+// b2ContactManager::Destroy (b2_contact_manager.cpp:48-62): EndContact for a
+// touching contact, then it leaves the list
+void b2World::RemoveContactsIf(bool (*pred)(const b2World*, const b2g_contact&, const void*), const void* arg) {
+  size_t k = 0;
+  for (size_t i = 0; i < m_editContacts.size(); ++i) {
+    const b2g_contact& c = m_editContacts[i];
+    if (pred(this, c, arg)) {
+      if (m_listener && c.touching) {
+        b2Contact hc;
+        ContactFromRecord(c, &hc);
+        m_listener->EndContact(&hc);
+      }
+      continue;
+    }
+    m_editContacts[k] = c;
+    if (k < m_editColours.size() && i < m_editColours.size()) m_editColours[k] = m_editColours[i];
+    ++k;
+  }
+  m_editContacts.resize(k);
+  if (m_editColours.size() > k) m_editColours.resize(k);
+  m_contactsValid = false;
+}
+
+// fixture `index` is gone: later fixtures move down one, and so do the
+// contacts' shape indices (no contact names the fixture any more)
+void b2World::RemoveFixtureIndex(int32 index) {
+  m_fixtures.erase(m_fixtures.begin() + index);
+  m_refilter.erase(m_refilter.begin() + index);
+  for (size_t i = index; i < m_fixtures.size(); ++i) m_fixtures[i]->m_index = (int32)i;
+  for (b2g_contact& c : m_editContacts) {
+    if (c.shape_a > index) --c.shape_a;
+    if (c.shape_b > index) --c.shape_b;
+  }
+  m_shapesDirty = true;
+}
+
+// b2ContactManager::Collide's re-filtering of flagged contacts
+// (b2_contact_manager.cpp:76-92): b2Body::ShouldCollide (b2_body.cpp:459-480)
+// and the filter, at the start of the step
+void b2World::ApplyRefilter() {
+  m_anyRefilter = false;
+  BeginEdit();
+  RemoveContactsIf(
+      [](const b2World* w, const b2g_contact& c, const void*) {
+        b2Fixture* fa = w->m_fixtures[c.shape_a];
+        b2Fixture* fb = w->m_fixtures[c.shape_b];
+        if (!w->m_refilter[c.shape_a] && !w->m_refilter[c.shape_b]) return false;
+        const b2Body* ba = fa->m_body;
+        const b2Body* bb = fb->m_body;
+        if (ba->m_type != b2_dynamicBody && bb->m_type != b2_dynamicBody) return true;
+        for (const b2Joint* j : w->m_joints) {
+          const bool links = (j->m_bodyA == ba && j->m_bodyB == bb) || (j->m_bodyA == bb && j->m_bodyB == ba);
+          if (links && !j->m_collideConnected) return true;
+        }
+        b2ContactFilter def;
+        return !def.ShouldCollide(fa, fb);
+      },
+      nullptr);
+  std::fill(m_refilter.begin(), m_refilter.end(), 0);
+}
+
+const char* b2World::UnsupportedReason() const {
+  if (m_continuousPhysics) return "continuous physics (TOI) is not run on the device: call SetContinuousPhysics(false)";
+  if (m_allowSleep) return "sleeping is not run on the device: call SetAllowSleeping(false)";
+  if (m_sleepingBodies > 0) return "a body is asleep (SetAwake(false) or created with awake = false)";
+  if (m_disabledBodies > 0) return "a body is disabled (SetEnabled(false) or created with enabled = false)";
+  if (m_sensorFixtures > 0) return "sensor fixtures are not run on the device";
+  if (m_contactFilter && overrides_should_collide(m_contactFilter))
+    return "a contact filter that overrides ShouldCollide cannot run on the device";
+  if (m_listenerPreSolve) return "a contact listener that overrides PreSolve cannot run on the device";
+  return nullptr;
 }
 
 b2Joint* b2World::CreateJoint(const b2JointDef* def) {  // b2_world.cpp:268-324
@@ -590,6 +989,7 @@ b2Joint* b2World::CreateJoint(const b2JointDef* def) {  // b2_world.cpp:268-324
   j->m_world = this;
   j->m_index = (int32)m_joints.size();
   j->m_next = m_jointList;
+  if (m_jointList) m_jointList->m_prev = j;
   m_jointList = j;
   m_joints.push_back(j);
   if (!def->collideConnected) m_newContacts = true;  // existing contacts are re-filtered
@@ -598,29 +998,43 @@ b2Joint* b2World::CreateJoint(const b2JointDef* def) {  // b2_world.cpp:268-324
 
 int b2World::EnsureDevice() {
   const int nb = (int)m_bodies.size(), ns = (int)m_fixtures.size(), nj = (int)m_joints.size();
-  if (m_batch && nb <= m_cap.bodies && ns <= m_cap.shapes && nj <= m_cap.joints) return B2G_OK;
+  const bool fits = m_batch && nb <= m_cap.bodies && ns <= m_cap.shapes && nj <= m_cap.joints;
+  if (fits && !m_editing) return B2G_OK;
+  // the contact list (with warm-start impulses and colours) survives a move
+  // to a larger batch and a renumbering
   std::vector<b2g_contact> keep;
-  if (m_batch) {
-    // grow: keep the contact list (with warm-start impulses) across the move
+  std::vector<int32_t> keep_colours;
+  if (m_editing) {
+    keep.swap(m_editContacts);
+    keep_colours.swap(m_editColours);
+  } else if (m_batch) {
     SyncBodies();
     SyncJoints();
     const int n = b2g_download_contacts(m_batch, 0, 0, nullptr);
     if (n > 0) {
       keep.resize(n);
-      b2g_download_contacts(m_batch, 0, n, keep.data());
+      keep_colours.resize(n);
+      int rc = b2g_download_contacts(m_batch, 0, n, keep.data());
+      if (!rc) rc = b2g_download_contact_colours(m_batch, 0, n, keep_colours.data());
+      if (rc) return rc;
     }
-    b2g_batch_destroy(m_batch);
-    m_batch = nullptr;
   }
-  m_cap.worlds = 1;
-  m_cap.bodies = std::max(64, nb * 2);
-  m_cap.shapes = std::max(64, ns * 2);
-  m_cap.joints = std::max(16, nj * 2);
-  m_cap.contacts = std::max(4096, std::min(ns * 16, (1 << 24) - 1));
-  int rc = b2g_batch_create(&m_batch, &m_cap);
-  if (rc) return rc;
-  b2g_world_params wp{{m_gravity.x, m_gravity.y}, 1.0f / 60.0f, 0.0f, m_warmStarting ? 1 : 0, 1};
-  if ((rc = b2g_set_worlds(m_batch, 1, nullptr, nullptr, &wp))) return rc;
+  m_editing = false;
+  int rc;
+  if (fits) {
+    if ((rc = b2g_set_counts(m_batch, nb, ns, nj))) return rc;
+  } else {
+    if (m_batch) b2g_batch_destroy(m_batch);
+    m_batch = nullptr;
+    m_cap.worlds = 1;
+    m_cap.bodies = std::max(64, nb * 2);
+    m_cap.shapes = std::max(64, ns * 2);
+    m_cap.joints = std::max(16, nj * 2);
+    m_cap.contacts = std::max(4096, std::min(ns * 16, (1 << 24) - 1));
+    if ((rc = b2g_batch_create(&m_batch, &m_cap))) return rc;
+    b2g_world_params wp{{m_gravity.x, m_gravity.y}, 1.0f / 60.0f, 0.0f, m_warmStarting ? 1 : 0, 1};
+    if ((rc = b2g_set_worlds(m_batch, 1, nullptr, nullptr, &wp))) return rc;
+  }
   m_uploadedBodies = m_uploadedShapes = m_uploadedJoints = 0;
   std::fill(m_dirty.begin(), m_dirty.end(), 1);
   m_anyDirty = true;
@@ -628,7 +1042,11 @@ int b2World::EnsureDevice() {
     // bodies/shapes must exist before contacts refer to them
     if ((rc = Upload())) return rc;
     if ((rc = b2g_upload_contacts(m_batch, (int32_t)keep.size(), keep.data()))) return rc;
+    if (keep_colours.size() == keep.size() &&
+        (rc = b2g_upload_contact_colours(m_batch, (int32_t)keep_colours.size(), keep_colours.data())))
+      return rc;
   }
+  m_contactsValid = false;
   return B2G_OK;
 }
 
@@ -662,6 +1080,10 @@ int b2World::Upload() {
     m_uploadedBodies = nb;
   }
   const int ns = (int)m_fixtures.size();
+  if (m_shapesDirty) {  // a fixture's filter or material changed, or fixtures moved
+    m_uploadedShapes = 0;
+    m_shapesDirty = false;
+  }
   if (ns > m_uploadedShapes) {
     std::vector<b2g_shape> rec(ns - m_uploadedShapes);
     for (int i = m_uploadedShapes; i < ns; ++i) {
@@ -855,6 +1277,12 @@ void b2World::SyncContacts() {
   if (m_contactsValid) return;
   m_contactsValid = true;
   m_contacts.clear();
+  if (m_editing) {  // the edited list has not gone back to the device yet
+    m_contacts.resize(m_editContacts.size());
+    for (size_t i = 0; i < m_editContacts.size(); ++i) ContactFromRecord(m_editContacts[i], &m_contacts[i]);
+    for (size_t i = 0; i + 1 < m_contacts.size(); ++i) m_contacts[i].m_next = &m_contacts[i + 1];
+    return;
+  }
   if (!m_batch) return;
   const int n = b2g_download_contacts(m_batch, 0, 0, nullptr);
   if (n <= 0) return;
@@ -871,6 +1299,7 @@ b2Contact* b2World::GetContactListStart() {
 }
 
 int32 b2World::GetContactCount() const {
+  if (m_editing) return (int32)m_editContacts.size();
   if (!m_batch) return 0;
   b2g_counts c;
   if (b2g_get_counts(m_batch, &c) != B2G_OK) return 0;
@@ -878,9 +1307,24 @@ int32 b2World::GetContactCount() const {
 }
 
 // b2World::Step (src/dynamics/b2_world.cpp:1108-1171): Collide, Solve with the
-// broadphase at its end, ClearForces; TOI and sleeping are not run on the device.
-void b2World::Step(float dt, int32 velocityIterations, int32 positionIterations) {
+// broadphase at its end, ClearForces; with dt <= 0, Collide and ClearForces
+// only (and dtRatio's inv_dt0 is kept). A world that needs a feature the
+// device does not run is refused before anything changes.
+void b2World::Step(float dt, int32 velocityIterations, int32 positionIterations, int32 particleIterations) {
+  B2_NOT_USED(particleIterations);  // no particle systems exist here
+  if (const char* why = UnsupportedReason()) {
+    m_status = B2G_ERR_UNSUPPORTED;
+    m_statusMessage = why;
+    if (m_lastRefusal != why) std::fprintf(stderr, "b2World::Step refused: %s\n", why);
+    m_lastRefusal = why;
+    b2Assert(false && "b2World::Step: a feature of this world does not run on the device");
+    return;
+  }
+  m_lastRefusal.clear();
+  const auto t0 = std::chrono::steady_clock::now();
   m_locked = true;
+  m_status = B2G_OK;
+  if (m_anyRefilter) ApplyRefilter();
   m_status = EnsureDevice();
   if (!m_status) m_status = Upload();
   if (!m_status) {
@@ -889,27 +1333,22 @@ void b2World::Step(float dt, int32 velocityIterations, int32 positionIterations)
     m_status = b2g_set_world_params(m_batch, 0, 1, &wp);
   }
   if (!m_status) {
-    uint32_t flags = B2G_STEP_PROFILE;
+    uint32_t flags = m_profiling ? B2G_STEP_PROFILE : 0u;
     if (m_newContacts) flags |= B2G_STEP_FIND_NEW_CONTACTS;
     if (m_listener) flags |= B2G_STEP_EVENTS;
-    int32 vi = velocityIterations, pi = positionIterations;
-    if (dt <= 0.0f) {  // Collide only: no solve, no broadphase (b2_world.cpp:1141-1163)
-      flags |= B2G_STEP_SKIP_BROADPHASE;
-      vi = 0;
-      pi = 0;
-    }
-    m_status = b2g_batch_step(m_batch, vi, pi, flags);
-    if (!m_status) {
+    if (dt <= 0.0f) flags |= B2G_STEP_COLLIDE_ONLY;
+    m_status = b2g_batch_step(m_batch, velocityIterations, positionIterations, flags);
+    if (!m_status && m_profiling) {
       b2g_profile p;
       b2g_get_profile(m_batch, &p);
-      m_profile.step = p.step;
       m_profile.collide = p.collide;
       m_profile.solve = p.solve;
       m_profile.broadphase = p.broadphase;
     }
   }
   if (m_status) {
-    std::fprintf(stderr, "b2World::Step (device): %s\n", b2g_last_error());
+    m_statusMessage = b2g_last_error();
+    std::fprintf(stderr, "b2World::Step (device): %s\n", m_statusMessage.c_str());
     b2Assert(false && "device step failed");
   }
   m_newContacts = false;
@@ -921,23 +1360,60 @@ void b2World::Step(float dt, int32 velocityIterations, int32 positionIterations)
     b->m_force.SetZero();
     b->m_torque = 0.0f;
   }
-  if (m_listener && !m_status) {
-    // Begin/End from the device event stream of this step: transitions seen
-    // by Collide, then touching contacts destroyed by the broadphase
-    int32_t n = 0;
-    if (b2g_download_events(m_batch, 0, nullptr, &n) == B2G_OK && n > 0) {
-      std::vector<b2g_contact_event> ev(n);
-      if (b2g_download_events(m_batch, n, ev.data(), &n) == B2G_OK) {
-        for (const b2g_contact_event& e : ev) {
-          b2Contact c;
-          ContactFromRecord(e.contact, &c);
-          if (e.kind == B2G_EVENT_BEGIN) m_listener->BeginContact(&c);
-          else m_listener->EndContact(&c);
-        }
+  if (m_listener && !m_status) DeliverEvents();
+  m_locked = false;
+  m_profile.step = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// The listener calls of the step just taken, in the reference's phase order:
+// Begin/EndContact from Collide (b2_contact.cpp:197-203), PostSolve for every
+// contact the solver stored impulses for (b2_island.cpp:621-647), then
+// EndContact for touching contacts the broadphase destroyed
+// (b2_contact_manager.cpp:48-51).
+void b2World::DeliverEvents() {
+  int32_t n = 0;
+  std::vector<b2g_contact_event> ev;
+  if (b2g_download_events(m_batch, 0, nullptr, &n) == B2G_OK && n > 0) {
+    ev.resize(n);
+    if (b2g_download_events(m_batch, n, ev.data(), &n) != B2G_OK) ev.clear();
+  }
+  for (const b2g_contact_event& e : ev) {
+    if (e.kind != B2G_EVENT_BEGIN && e.kind != B2G_EVENT_END) continue;
+    b2Contact c;
+    ContactFromRecord(e.contact, &c);
+    if (e.kind == B2G_EVENT_BEGIN) m_listener->BeginContact(&c);
+    else m_listener->EndContact(&c);
+  }
+  if (m_listenerPostSolve) {
+    auto post = [&](b2Contact* c, int32 points) {
+      b2ContactImpulse imp;
+      imp.count = points;
+      for (int k = 0; k < b2_maxManifoldPoints; ++k) {
+        imp.normalImpulses[k] = k < points ? c->m_manifold.points[k].normalImpulse : 0.0f;
+        imp.tangentImpulses[k] = k < points ? c->m_manifold.points[k].tangentImpulse : 0.0f;
       }
+      m_listener->PostSolve(c, &imp);
+    };
+    SyncContacts();
+    std::vector<int32_t> points(m_contacts.size(), 0);
+    if (!points.empty() &&
+        b2g_download_solver_points(m_batch, 0, (int32_t)points.size(), points.data()) != B2G_OK)
+      points.assign(points.size(), 0);
+    for (size_t i = 0; i < m_contacts.size(); ++i)
+      if (points[i] > 0) post(&m_contacts[i], points[i]);
+    for (const b2g_contact_event& e : ev) {
+      if (e.kind != B2G_EVENT_DESTROYED || e.solver_points <= 0) continue;
+      b2Contact c;
+      ContactFromRecord(e.contact, &c);
+      post(&c, e.solver_points);
     }
   }
-  m_locked = false;
+  for (const b2g_contact_event& e : ev) {
+    if (e.kind != B2G_EVENT_DESTROYED) continue;
+    b2Contact c;
+    ContactFromRecord(e.contact, &c);
+    m_listener->EndContact(&c);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -945,7 +1421,10 @@ void b2World::Step(float dt, int32 velocityIterations, int32 positionIterations)
 // ---------------------------------------------------------------------------
 struct b2a_world {
   b2World* world;
+  // the world's objects in creation order (as the device numbers them)
   std::vector<b2Body*> bodies;
+  std::vector<b2Fixture*> fixtures;
+  std::vector<b2Joint*> joints;
 };
 
 // Builds a b2World from a scene description through the public API
@@ -1022,7 +1501,7 @@ static b2a_world* world_from_scene(const b2g_scene* s, int32_t warm_starting) {
       fd.filter.groupIndex = (int16)s->fix_filter[4 * i + 2];
       fd.isSensor = s->fix_filter[4 * i + 3] != 0;
     }
-    w->bodies[s->fix_body[i]]->CreateFixture(&fd);
+    w->fixtures.push_back(w->bodies[s->fix_body[i]]->CreateFixture(&fd));
   }
   for (int i = 0; i < s->joint_count; ++i) {
     b2RevoluteJointDef jd;
@@ -1036,7 +1515,7 @@ static b2a_world* world_from_scene(const b2g_scene* s, int32_t warm_starting) {
     jd.upperAngle = p[3];
     jd.motorSpeed = p[4];
     jd.maxMotorTorque = p[5];
-    w->world->CreateJoint(&jd);
+    w->joints.push_back(w->world->CreateJoint(&jd));
   }
   return w;
 }
@@ -1153,6 +1632,54 @@ int b2a_world_get_contacts(b2a_world* w, int32_t capacity, b2g_contact* out, int
 }
 
 b2g_batch* b2a_world_batch(b2a_world* w) { return w ? w->world->GetDeviceBatch() : nullptr; }
+
+int b2a_world_destroy_body(b2a_world* w, int32_t body) {
+  if (!w || body < 0 || body >= (int32_t)w->bodies.size()) return B2G_ERR_INVALID;
+  b2Body* b = w->bodies[body];
+  // its fixtures and joints go with it
+  for (size_t i = 0; i < w->fixtures.size();) {
+    if (w->fixtures[i]->GetBody() == b) w->fixtures.erase(w->fixtures.begin() + i); else ++i;
+  }
+  for (size_t i = 0; i < w->joints.size();) {
+    if (w->joints[i]->GetBodyA() == b || w->joints[i]->GetBodyB() == b) w->joints.erase(w->joints.begin() + i);
+    else ++i;
+  }
+  w->bodies.erase(w->bodies.begin() + body);
+  w->world->DestroyBody(b);
+  return B2G_OK;
+}
+
+int b2a_world_destroy_joint(b2a_world* w, int32_t joint) {
+  if (!w || joint < 0 || joint >= (int32_t)w->joints.size()) return B2G_ERR_INVALID;
+  b2Joint* j = w->joints[joint];
+  w->joints.erase(w->joints.begin() + joint);
+  w->world->DestroyJoint(j);
+  return B2G_OK;
+}
+
+int b2a_world_destroy_fixture(b2a_world* w, int32_t fixture) {
+  if (!w || fixture < 0 || fixture >= (int32_t)w->fixtures.size()) return B2G_ERR_INVALID;
+  b2Fixture* f = w->fixtures[fixture];
+  w->fixtures.erase(w->fixtures.begin() + fixture);
+  f->GetBody()->DestroyFixture(f);
+  return B2G_OK;
+}
+
+int b2a_world_set_filter(b2a_world* w, int32_t fixture, int32_t category, int32_t mask, int32_t group) {
+  if (!w || fixture < 0 || fixture >= (int32_t)w->fixtures.size()) return B2G_ERR_INVALID;
+  b2Filter f;
+  f.categoryBits = (uint16)category;
+  f.maskBits = (uint16)mask;
+  f.groupIndex = (int16)group;
+  w->fixtures[fixture]->SetFilterData(f);
+  return B2G_OK;
+}
+
+int b2a_world_body_count(b2a_world* w, int32_t* out) {
+  if (!w || !out) return B2G_ERR_INVALID;
+  *out = (int32_t)w->bodies.size();
+  return B2G_OK;
+}
 
 int b2a_world_prepare(b2a_world* w) {
   if (!w) return B2G_ERR_INVALID;

@@ -9,10 +9,15 @@
 // runs on the GPU through the b2g C ABI (include/b2g.h); body state lives in
 // device memory between steps and host getters read a lazily refreshed copy.
 //
-// Device-path limits (each reported by b2Assert + a nullptr/no-op, as the
-// reference reports a locked world): sensors, PreSolve/PostSolve listeners,
-// custom contact filters, continuous physics (TOI) and sleeping are not
-// evaluated on the device; only revolute joints are created.
+// Device-path limits. Step() refuses to run, leaves the world unchanged,
+// sets GetLastStatus() to B2G_ERR_UNSUPPORTED, prints the reason to stderr and
+// fires b2Assert (compiled out under NDEBUG, as in the reference's Release
+// build) while any of these is in effect: continuous physics (TOI, on by
+// default in b2World), sleeping (on by default), a body put to sleep or
+// disabled, a sensor fixture, a contact filter that overrides ShouldCollide,
+// or a contact listener that overrides PreSolve. Only revolute joints are
+// created (CreateJoint of another type returns nullptr). Begin/EndContact and
+// PostSolve are delivered after each Step from the device's record of it.
 #ifndef B2AMD_BOX2D_H
 #define B2AMD_BOX2D_H
 
@@ -21,6 +26,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "b2g.h"
@@ -299,11 +305,24 @@ class b2Fixture {
   float GetRestitutionThreshold() const { return m_restitutionThreshold; }
   void GetMassData(b2MassData* massData) const { m_shape->ComputeMass(massData, m_density); }
   const b2AABB& GetAABB() const;
+  // b2_fixture.cpp:142-183: contacts of this fixture are re-filtered at the
+  // next Step's Collide (destroyed, with EndContact, if the pair no longer
+  // collides); new pairs appear at the next broadphase
+  void SetFilterData(const b2Filter& filter);
+  void Refilter();
+  // sensors do not run on the device: Step refuses while one exists
+  void SetSensor(bool sensor);
+  // the material of contacts created from now on (b2_fixture.h:189-216)
+  void SetDensity(float density);
+  void SetFriction(float friction);
+  void SetRestitution(float restitution);
+  void SetRestitutionThreshold(float threshold);
 
  private:
   friend class b2Body;
   friend class b2World;
   friend class b2Contact;
+  void MarkShapeDirty();
   b2Fixture() {}
   ~b2Fixture() { delete m_shape; }
   b2Body* m_body = nullptr;
@@ -371,9 +390,26 @@ class b2Body {
   float GetGravityScale() const { return m_gravityScale; }
   void SetGravityScale(float s) { m_gravityScale = s; MarkDirty(); }
   b2BodyType GetType() const { return m_type; }
-  bool IsBullet() const { return false; }
-  bool IsAwake() const { return m_type != b2_staticBody; }
-  bool IsEnabled() const { return true; }
+  // b2_body.cpp:129-226: destroys the body's contacts (EndContact for
+  // touching ones), resets the mass, and a static body stops
+  void SetType(b2BodyType type);
+  // b2_body.cpp:280-350: destroys the fixture's contacts (EndContact for
+  // touching ones) and resets the mass
+  void DestroyFixture(b2Fixture* fixture);
+  void SetMassData(const b2MassData* data);
+  void ResetMassData();
+  void SetBullet(bool flag) { m_bullet = flag; }
+  bool IsBullet() const { return m_bullet; }
+  // sleeping is not run on the device: a body put to sleep (SetAwake(false),
+  // or created with awake = false) makes Step refuse until it is woken
+  void SetSleepingAllowed(bool flag) { m_sleepingAllowed = flag; }
+  bool IsSleepingAllowed() const { return m_sleepingAllowed; }
+  void SetAwake(bool flag);
+  bool IsAwake() const { return m_awake; }
+  // disabled bodies are not run on the device: Step refuses while one exists
+  void SetEnabled(bool flag);
+  bool IsEnabled() const { return m_enabled; }
+  void SetFixedRotation(bool flag);
   bool IsFixedRotation() const { return m_fixedRotation; }
   b2Fixture* GetFixtureList() { return m_fixtureList; }
   b2Body* GetNext() { return m_next; }
@@ -387,7 +423,6 @@ class b2Body {
   friend class b2RevoluteJoint;
   b2Body(const b2BodyDef* bd, b2World* world, int32 index);
   ~b2Body();
-  void ResetMassData();
   void MarkDirty();
   void Sync() const;
   void ToRecord(b2g_body* r) const;
@@ -406,8 +441,10 @@ class b2Body {
   float m_mass, m_invMass, m_I, m_invI;
   float m_linearDamping, m_angularDamping, m_gravityScale;
   bool m_fixedRotation;
+  bool m_bullet = false, m_sleepingAllowed = true, m_awake = true, m_enabled = true;
   b2Fixture* m_fixtureList = nullptr;
   int32 m_fixtureCount = 0;
+  b2Body* m_prev = nullptr;
   b2Body* m_next = nullptr;
   b2BodyUserData m_userData;
 };
@@ -454,6 +491,7 @@ class b2Joint {
   b2JointType m_type = e_unknownJoint;
   b2Body* m_bodyA = nullptr;
   b2Body* m_bodyB = nullptr;
+  b2Joint* m_prev = nullptr;
   b2Joint* m_next = nullptr;
   bool m_collideConnected = false;
   int32 m_index = -1;
@@ -517,21 +555,39 @@ struct b2ContactImpulse {
   int32 count;
 };
 
-// Begin/End are delivered after each Step from the device contact list
-// (touching transitions); PreSolve/PostSolve are not evaluated on the device.
+// include/box2d/b2_world_callbacks.h:133-209. Begin/End and PostSolve are
+// delivered after each Step from the device's record of it, in the
+// reference's phase order (Collide's Begin/End, then PostSolve for every
+// solved contact, then EndContact for touching contacts the broadphase
+// destroyed); within a phase contacts come in shape-pair order. PreSolve
+// cannot run on the device: a listener that overrides it makes Step refuse.
 class b2ContactListener {
  public:
   virtual ~b2ContactListener() {}
   virtual void BeginContact(b2Contact* contact) { B2_NOT_USED(contact); }
   virtual void EndContact(b2Contact* contact) { B2_NOT_USED(contact); }
-  virtual void PreSolve(b2Contact* contact, const b2Manifold* oldManifold) {
-    B2_NOT_USED(contact);
-    B2_NOT_USED(oldManifold);
-  }
-  virtual void PostSolve(b2Contact* contact, const b2ContactImpulse* impulse) {
-    B2_NOT_USED(contact);
-    B2_NOT_USED(impulse);
-  }
+  // defined in the library (one definition, so the world can tell whether a
+  // listener overrides them)
+  virtual void PreSolve(b2Contact* contact, const b2Manifold* oldManifold);
+  virtual void PostSolve(b2Contact* contact, const b2ContactImpulse* impulse);
+};
+
+// include/box2d/b2_world_callbacks.h:43-72: called before a joint or fixture
+// is destroyed because its body is
+class b2DestructionListener {
+ public:
+  virtual ~b2DestructionListener() {}
+  virtual void SayGoodbye(b2Joint* joint) = 0;
+  virtual void SayGoodbye(b2Fixture* fixture) = 0;
+};
+
+// include/box2d/b2_world_callbacks.h:78-111. The default ShouldCollide is the
+// group/category/mask test (src/dynamics/b2_world_callbacks.cpp:28-40), which
+// the device evaluates itself; a filter that overrides it cannot run there.
+class b2ContactFilter {
+ public:
+  virtual ~b2ContactFilter() {}
+  virtual bool ShouldCollide(b2Fixture* fixtureA, b2Fixture* fixtureB);
 };
 
 // include/box2d/b2_world_callbacks.h:213-264
@@ -555,9 +611,23 @@ class b2World {
   b2World(const b2World&) = delete;
   b2World& operator=(const b2World&) = delete;
 
+  void SetDestructionListener(b2DestructionListener* listener) { m_destructionListener = listener; }
+  // a filter that overrides ShouldCollide makes Step refuse (see above)
+  void SetContactFilter(b2ContactFilter* filter) { m_contactFilter = filter; }
   b2Body* CreateBody(const b2BodyDef* def);
+  // b2_world.cpp:178-266: attached joints and fixtures go too (the
+  // destruction listener hears of each), touching contacts end (EndContact)
+  void DestroyBody(b2Body* body);
   b2Joint* CreateJoint(const b2JointDef* def);
-  void Step(float timeStep, int32 velocityIterations, int32 positionIterations);
+  // b2_world.cpp:326-418: the bodies may collide again from the next
+  // broadphase on
+  void DestroyJoint(b2Joint* joint);
+  // include/box2d/b2_world.h:120-142 (there are no particle systems here, so
+  // particleIterations has nothing to step)
+  void Step(float timeStep, int32 velocityIterations, int32 positionIterations, int32 particleIterations);
+  void Step(float timeStep, int32 velocityIterations, int32 positionIterations) {
+    Step(timeStep, velocityIterations, positionIterations, 1);
+  }
   // fixtures whose AABB overlaps `aabb`, on the device tree (as in the
   // reference's broadphase, the callback's return value does not stop it)
   void QueryAABB(b2QueryCallback* callback, const b2AABB& aabb);
@@ -565,7 +635,7 @@ class b2World {
   // return value clips (fraction), ignores (-1) or terminates (0)
   void RayCast(b2RayCastCallback* callback, const b2Vec2& point1, const b2Vec2& point2);
   void ClearForces() {}
-  void SetContactListener(b2ContactListener* listener) { m_listener = listener; }
+  void SetContactListener(b2ContactListener* listener);
   b2Body* GetBodyList() { return m_bodyListHead; }
   b2Joint* GetJointList() { return m_jointList; }
   b2Contact* GetContactListStart();
@@ -586,11 +656,15 @@ class b2World {
   bool IsLocked() const { return m_locked; }
   void SetAutoClearForces(bool flag) { B2_NOT_USED(flag); }
   bool GetAutoClearForces() const { return true; }
+  // step and collide/solve/broadphase times; the stage split is measured on
+  // the device only while profiling is on (events and a host wait per stage)
   const b2Profile& GetProfile() const { return m_profile; }
+  void SetProfiling(bool flag) { m_profiling = flag; }
   // device handle (batch of one world)
   b2g_batch* GetDeviceBatch() { return m_batch; }
-  // status of the last device call (0 = ok) and its message
+  // status of the last Step or device call (0 = ok) and its message
   int32 GetLastStatus() const { return m_status; }
+  const char* GetLastErrorMessage() const { return m_statusMessage.c_str(); }
   // allocate device memory and upload everything created so far
   int Prepare();
   // device records of everything created so far, as this world would upload
@@ -611,6 +685,15 @@ class b2World {
   void SyncContacts();
   void ContactFromRecord(const b2g_contact& r, b2Contact* c) const;
   void SyncJoints() const;
+  // destroying bodies, fixtures or joints, or re-filtering, edits the
+  // contact list and renumbers records: the device state comes to the host
+  // first and goes back, renumbered, before the next device call
+  void BeginEdit();
+  void RemoveContactsIf(bool (*pred)(const b2World*, const b2g_contact&, const void*), const void* arg);
+  void RemoveFixtureIndex(int32 index);
+  void ApplyRefilter();
+  const char* UnsupportedReason() const;
+  void DeliverEvents();
 
   b2Vec2 m_gravity;
   std::vector<b2Body*> m_bodies;        // creation order = device body index
@@ -620,6 +703,17 @@ class b2World {
   b2Body* m_bodyListTail = nullptr;
   b2Joint* m_jointList = nullptr;
   b2ContactListener* m_listener = nullptr;
+  b2DestructionListener* m_destructionListener = nullptr;
+  b2ContactFilter* m_contactFilter = nullptr;
+  // host copy of the device contact list (with colours) while records are
+  // being destroyed or re-filtered
+  bool m_editing = false;
+  std::vector<b2g_contact> m_editContacts;
+  std::vector<int32_t> m_editColours;
+  std::vector<char> m_refilter;  // per fixture: re-filter its contacts at the next Step
+  bool m_anyRefilter = false;
+  bool m_shapesDirty = false;
+  std::string m_statusMessage;
   std::vector<b2Contact> m_contacts;
   bool m_contactsValid = true;
   b2g_batch* m_batch = nullptr;
@@ -631,6 +725,11 @@ class b2World {
   mutable bool m_jointsValid = true;
   bool m_newContacts = false;
   bool m_warmStarting = true, m_allowSleep = true, m_continuousPhysics = true, m_locked = false;
+  bool m_profiling = false;
+  // what Step must refuse (kept up to date by the mutators)
+  int32 m_sleepingBodies = 0, m_disabledBodies = 0, m_sensorFixtures = 0;
+  bool m_listenerPreSolve = false, m_listenerPostSolve = false;
+  std::string m_lastRefusal;
   float m_inv_dt0 = 0.0f;
   b2Profile m_profile;
   int32 m_status = 0;

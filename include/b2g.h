@@ -209,6 +209,10 @@ enum {
   B2G_STEP_ASYNC = 1u << 2,              /* return without waiting for the device */
   B2G_STEP_SKIP_BROADPHASE = 1u << 3,    /* omit the end-of-step broadphase (testing only) */
   B2G_STEP_EVENTS = 1u << 5,             /* record contact events (b2g_download_events) */
+  B2G_STEP_COLLIDE_ONLY = 1u << 6,       /* b2World::Step with dt <= 0: the pending broadphase
+                                            (B2G_STEP_FIND_NEW_CONTACTS), Collide and ClearForces only;
+                                            no solve, so stored impulses and bodies are kept
+                                            (src/dynamics/b2_world.cpp:1114-1141, 1164-1167) */
   B2G_STEP_GRAPH = 1u << 4               /* replay a captured hipGraph of the whole step: no host round
                                             trip; colour slots and launch sizes fixed at capture from the
                                             last eager step (re-captured after uploads). The first step
@@ -229,6 +233,16 @@ int b2g_set_worlds(b2g_batch* batch, int32_t count, const int32_t* body_begin,
  * `params` may be reused as soon as the call returns. */
 int b2g_set_world_params(b2g_batch* batch, int32_t first, int32_t count,
                          const b2g_world_params* params);
+
+/* Live record counts after bodies, fixtures or joints were destroyed
+ * (b2World::DestroyBody src/dynamics/b2_world.cpp:178-266, b2World::DestroyJoint
+ * :326-418, b2Body::DestroyFixture src/dynamics/b2_body.cpp:280-350). Counts may
+ * be anything up to the capacity; the caller then uploads the records that are
+ * renumbered or new (the device keeps no tombstones: indices are positions in
+ * the caller's creation order). The contact list is emptied, so the caller
+ * re-uploads the surviving contacts (b2g_upload_contacts) with their colours;
+ * the next broadphase queries every world. */
+int b2g_set_counts(b2g_batch* batch, int32_t bodies, int32_t shapes, int32_t joints);
 
 int b2g_upload_bodies(b2g_batch* batch, int32_t first, int32_t count, const b2g_body* bodies);
 int b2g_upload_shapes(b2g_batch* batch, int32_t first, int32_t count, const b2g_shape* shapes);
@@ -300,6 +314,15 @@ int b2g_download_joints(b2g_batch* batch, int32_t first, int32_t count, b2g_revo
 /* constraint colour of each touching contact of the last step (-1 = not solved);
  * out has one entry per downloaded contact, same order */
 int b2g_download_contact_colours(b2g_batch* batch, int32_t first, int32_t count, int32_t* out);
+/* Points the last step's solver stored for each current contact: the
+ * b2ContactImpulse count b2Island::Report hands to PostSolve
+ * (src/dynamics/b2_island.cpp:621-647; 0 = not solved that step, else the
+ * velocity constraint's pointCount, which the block solver's condition check
+ * may have lowered to 1, src/dynamics/b2_contact_solver.cpp:244-255). The
+ * impulses themselves are the manifold's (b2g_download_contacts), as
+ * b2ContactSolver::StoreImpulses wrote them. Same order as the contacts. */
+int b2g_download_solver_points(b2g_batch* batch, int32_t first, int32_t count, int32_t* out);
+
 /* Sets the persisted colours of the current contacts (count = contact count,
  * in the key-sorted order b2g_download_contacts returns; -1 = uncoloured).
  * No reference counterpart: with b2g_upload_contacts it restores a solver
@@ -316,7 +339,8 @@ int b2g_upload_contact_colours(b2g_batch* batch, int32_t count, const int32_t* c
 enum { B2G_EVENT_BEGIN = 1, B2G_EVENT_END = 2, B2G_EVENT_DESTROYED = 3 };
 typedef struct b2g_contact_event {
   int32_t kind;
-  int32_t reserved;
+  int32_t solver_points;  /* B2G_EVENT_DESTROYED: points the step's solver stored for the contact
+                             before the broadphase destroyed it (PostSolve's count); else 0 */
   b2g_contact contact;
 } b2g_contact_event;
 int b2g_download_events(b2g_batch* batch, int32_t capacity, b2g_contact_event* out, int32_t* count);
@@ -382,6 +406,16 @@ int b2a_world_get_contacts(b2a_world* world, int32_t capacity, b2g_contact* out,
 b2g_batch* b2a_world_batch(b2a_world* world);
 /* allocate the device batch and upload bodies/shapes/joints without stepping */
 int b2a_world_prepare(b2a_world* world);
+/* b2World::DestroyBody (src/dynamics/b2_world.cpp:178-266), b2World::DestroyJoint
+ * (:326-418), b2Body::DestroyFixture (src/dynamics/b2_body.cpp:280-350) and
+ * b2Fixture::SetFilterData (src/dynamics/b2_fixture.cpp:142-146) on the
+ * world's objects, named by their index in the world's current creation order
+ * (later objects move down one index when one is destroyed). */
+int b2a_world_destroy_body(b2a_world* world, int32_t body);
+int b2a_world_destroy_joint(b2a_world* world, int32_t joint);
+int b2a_world_destroy_fixture(b2a_world* world, int32_t fixture);
+int b2a_world_set_filter(b2a_world* world, int32_t fixture, int32_t category, int32_t mask, int32_t group);
+int b2a_world_body_count(b2a_world* world, int32_t* out);
 
 /* Device records of a scene, built through the same b2World API (polygon hull
  * and welding, mass data) without touching the device: bodies tagged with

@@ -114,3 +114,26 @@ def test_python_records_match_the_c_compiler_layout(tmp_path):
     subprocess.check_call(["gcc", "-std=c99", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)])
     got = dict(line.split() for line in subprocess.check_output([str(exe)], text=True).splitlines())
     assert {k: int(v) for k, v in got.items()} == names
+
+
+def test_drop_in_refuses_unsupported_features_before_the_device():
+    """User code that turns on what the device does not run (continuous
+    physics and sleeping, b2World's defaults; a sleeping or disabled body; a
+    sensor; a PreSolve listener; a filter overriding ShouldCollide) gets
+    B2G_ERR_UNSUPPORTED with a message from Step, and the world is left as it
+    was; the check comes before any device call, so it holds on the CPU."""
+    import subprocess
+    import pytest
+    exe = os.path.join(REPO, "box2d-optimized_amd", "build", "unsupported_features")
+    if not os.path.exists(exe):
+        pytest.skip("example not built (run __graft_entry__.build())")
+    eng = load_pkg().engine
+    if eng.device_count() > 0:
+        pytest.skip("device present: covered by the GPU test")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    lines = r.stdout.strip().splitlines()
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert len(lines) == 8
+    assert all(": 5 " in ln for ln in lines[:7])
+    assert "SetContinuousPhysics(false)" in lines[0] and "SetAllowSleeping(false)" in lines[1]
+    assert lines[-1].split(":")[1].split()[0] == "6"  # B2G_ERR_NO_DEVICE: it got as far as the device

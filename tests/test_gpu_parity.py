@@ -359,15 +359,18 @@ def test_gpu_pyramid_1000_steps_stable(engine):
 def test_gpu_drop_in_cpp_example():
     """A C++ program written against the b2World API (examples/falling_box.cpp),
     linked against libbox2d_amd.so: the box rests at y = 1.01 +- 0.01 after 60
-    steps and BeginContact fires once (unit-test/hello_world.cpp:109-111)."""
+    steps and BeginContact fires once (unit-test/hello_world.cpp:109-111); the
+    last step's PostSolve normal impulses carry the box's weight (m g dt)."""
     import subprocess
     exe = os.path.join(REPO, "box2d-optimized_amd", "build", "falling_box")
     assert os.path.exists(exe), "example not built"
     print("running", exe, flush=True)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    x, y, a, begins, contacts = r.stdout.split()
+    print(r.stdout, flush=True)
+    x, y, a, begins, contacts, impulse = r.stdout.split()
     assert abs(float(y) - 1.01) < 0.01 and int(begins) == 1 and int(contacts) == 1
+    assert abs(float(impulse) - 4.0 * 10.0 / 60.0) < 0.01 * 4.0 * 10.0 / 60.0
 
 
 @pytest.mark.gpu
