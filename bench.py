@@ -13,7 +13,7 @@ Headline workload: the large random pile.
 
 Further sections of the same JSON line:
 - `gas_chains`: 100k circles + 500 revolute chains, one world.
-- `batched`: 4096 worlds, about 1.14M bodies, in one launch per stage,
+- `batched`: 4096 worlds, about 1.2M bodies, in one launch per stage,
   against a plain per-world loop; also the whole step's algorithmic HBM
   bytes over its time (`roofline_step`) and the reference CPU Step on a
   sample of the worlds, on one core and on all host cores.

@@ -18,6 +18,7 @@
 
 #include "b2g.h"
 #include "b2g_kernels.h"
+#include "b2g_partition.h"
 
 using namespace b2g;
 
@@ -176,13 +177,17 @@ struct b2g_batch {
   int world_lds = -1;        // -1 autotune, 0 off, 1 on (when the worlds fit)
   int tune_step = 0;         // autotuner progress: two timed steps per candidate
   float tune_ms[16] = {};
-  int last_path = 0;         // 1 colour-major sweeps, 2 per-world
+  int last_path = 0;         // 1 colour-major sweeps, 2 per-world, 3 cells
 
   int alloc();
   void release();
   void swap_contact_buffers();
   void graphs_clear();
   void enqueue_collide_and_colour(int K, int rounds);
+  void enqueue_collide();
+  bool use_cells() const;
+  void enqueue_cells_solve(int vi, int pi, bool finish_in_broadphase);
+  int cells_step(int vi, int pi, unsigned flags);
 
   void enqueue_colour_round(int K);
   void enqueue_colour_groups();
@@ -275,6 +280,11 @@ int b2g_batch::alloc() {
   // read-back is a single copy
   A(d.cnt, kCntColOff + kColourSlots + 1);
   d.col_off = d.cnt + kCntColOff;
+  // the cells solver
+  A(d.part_geo, 2); A(d.part_boff, 2 * kCells + 1); A(d.part_blist, 2 * (size_t)NB); A(d.part_brank, NB);
+  A(d.part_ccnt, kPartGroups); A(d.part_bucket, (size_t)2 * kCells * kCellCons); A(d.part_cbucket, NC);
+  A(d.part_coff, (size_t)kPartGroups * (kColourSlots + 1)); A(d.part_claim, NB); A(d.part_used, NB);
+  A(d.part_xcol, NC); A(d.sloc, NC);
 #undef A
   d.ccap = NC;
   d.ecap = NC;
@@ -645,9 +655,17 @@ void b2g_batch::graphs_clear() {
 }
 
 void b2g_batch::enqueue_collide_and_colour(int K, int rounds) {
+  enqueue_collide();
+  const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
+  hipLaunchKernelGGL(k_colour_init, dim3(gk), dim3(kBlock), 0, stream, d, K);
+  for (int r = 0; r < rounds; ++r) enqueue_colour_round(K);
+}
+
+// Collide (narrowphase), the bodies' velocity integration, and the compacted
+// list of the contacts the step solves
+void b2g_batch::enqueue_collide() {
   const int nb = d.nb;
   const int gc = grid_for(std::max(est_contacts, 256) * 5 / 4);
-  const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
   const int gb = grid_for(nb);
   // narrowphase, and the bodies' velocity integration in the same launch
   launch(B2G_KERNEL_NARROWPHASE, est_contacts, k_narrowphase, dim3(std::max(gc, gb)), dim3(kBlock), 0, d, 1);
@@ -663,8 +681,101 @@ void b2g_batch::enqueue_collide_and_colour(int K, int rounds) {
     scan(d.touch, d.cnt + C_CONTACTS, 0, d.tpos, d.cnt + C_CONS, touch_bound);
     hipLaunchKernelGGL(k_compact_touching, dim3(gc), dim3(kBlock), 0, stream, d);
   }
-  hipLaunchKernelGGL(k_colour_init, dim3(gk), dim3(kBlock), 0, stream, d, K);
-  for (int r = 0; r < rounds; ++r) enqueue_colour_round(K);
+}
+
+// ---------------------------------------------------------------------------
+// The cells solver (b2g_partition.h): a world's constraints split by the
+// cells of two staggered grids, three launches per iteration, no host read.
+// ---------------------------------------------------------------------------
+constexpr int kCellsMinBodies = 4096;  // B2G_SOLVER_AUTO: a single world at least this large
+
+bool b2g_batch::use_cells() const {
+  if (solver_mode == B2G_SOLVER_CELLS) return true;
+  if (solver_mode != B2G_SOLVER_AUTO) return false;
+  return d.nw == 1 && d.nb >= kCellsMinBodies;
+}
+
+void b2g_batch::enqueue_cells_solve(int vi, int pi, bool finish_in_broadphase) {
+  const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
+  hipLaunchKernelGGL(k_part_bodies, dim3(1), dim3(kPartGridThreads), 0, stream, d);
+  hipLaunchKernelGGL(k_part_classify, dim3(gk), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_part_colour_cells, dim3(2 * kCells), dim3(kCellThreads), 0, stream, d);
+  hipLaunchKernelGGL(k_part_colour_cross, dim3(1), dim3(kCrossThreads), 0, stream, d);
+  enqueue_islands_and_init();
+  const long long cons = std::max(est_cons, 1);
+  // one sweep: the A cells, the B cells, the cross group (timed as one run)
+  auto sweep = [&](int id, auto kernel, int it) {
+    tbegin(id);
+    hipLaunchKernelGGL(kernel, dim3(kCells), dim3(kCellThreads), 0, stream, d, 0, it);
+    hipLaunchKernelGGL(kernel, dim3(kCells), dim3(kCellThreads), 0, stream, d, 1, it);
+    hipLaunchKernelGGL(kernel, dim3(1), dim3(kCellThreads), 0, stream, d, 2, it);
+    tend(id, cons, 3);
+  };
+  auto joint_sweep = [&](auto kernel, auto... extra) {
+    const int njc = (int)h_jcol_off.size() - 1;
+    for (int c = 0; c < njc; ++c) {
+      const int b = h_jcol_off[c], e = h_jcol_off[c + 1];
+      if (e > b) hipLaunchKernelGGL(kernel, dim3(grid_for(e - b)), dim3(kBlock), 0, stream, d, b, e, extra...);
+    }
+  };
+  sweep(B2G_KERNEL_WARM_START, k_part_sweep<kPartWarm, kCellThreads>, 0);
+  joint_sweep(k_joint_init);
+  for (int it = 0; it < vi; ++it) {
+    joint_sweep(k_joint_velocity);
+    sweep(B2G_KERNEL_CONTACT_VELOCITY, k_part_sweep<kPartVelocity, kCellThreads>, 0);
+  }
+  (void)hipStreamWaitEvent(stream, join_ev, 0);
+  hipLaunchKernelGGL(k_store_and_integrate, dim3(grid_for(std::max(est_cons + d.nj, d.nb))), dim3(kBlock), 0, stream,
+                     d, 1);
+  for (int it = 0; it < pi; ++it) {
+    sweep(B2G_KERNEL_CONTACT_POSITION, k_part_sweep<kPartPosition, kCellThreads>, it);
+    joint_sweep(k_joint_position, it);
+  }
+  if (finish_in_broadphase)
+    finish_pending = true;
+  else
+    hipLaunchKernelGGL(k_finish_bodies, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
+}
+
+// an eager step by cells: the whole step is enqueued without a host read
+int b2g_batch::cells_step(int vi, int pi, unsigned flags) {
+  const bool prof_on = (flags & B2G_STEP_PROFILE) != 0;
+  last_path = 3;
+  graph_K = std::max(graph_K, 1);  // a captured step needs no colour count from an eager one
+  if (prof_on) HIP_TRY(hipEventRecord(ev[0], stream));
+  int rc;
+  if (flags & B2G_STEP_FIND_NEW_CONTACTS) {
+    if ((rc = broadphase(false))) return rc;
+  }
+  d.segs = pairs_current && worlds_contiguous ? 1 : 0;
+  d.part_on = 1;
+  enqueue_collide();
+  if (prof_on) HIP_TRY(hipEventRecord(ev[1], stream));
+  enqueue_cells_solve(vi, pi, !(flags & B2G_STEP_SKIP_BROADPHASE));
+  d.part_on = 0;
+  if (prof_on) HIP_TRY(hipEventRecord(ev[2], stream));
+  if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
+    if ((rc = broadphase(false))) return rc;
+  }
+  if (prof_on) HIP_TRY(hipEventRecord(ev[3], stream));
+  if (tev_used > 64) tcollect();
+  if (!(flags & B2G_STEP_ASYNC) || prof_on) {
+    if ((rc = read_counters())) return rc;
+    if ((rc = check_status())) return rc;
+    est_cons = pinned[C_CONS];
+  }
+  if (prof_on) {
+    float t01, t12, t23, t03;
+    HIP_TRY(hipEventElapsedTime(&t01, ev[0], ev[1]));
+    HIP_TRY(hipEventElapsedTime(&t12, ev[1], ev[2]));
+    HIP_TRY(hipEventElapsedTime(&t23, ev[2], ev[3]));
+    HIP_TRY(hipEventElapsedTime(&t03, ev[0], ev[3]));
+    prof.collide = t01;
+    prof.solve = t12 + t23;
+    prof.broadphase = t23;
+    prof.step = t03;
+  }
+  return B2G_OK;
 }
 
 void b2g_batch::enqueue_colour_round(int K) {
@@ -1033,6 +1144,7 @@ int b2g_batch::collide_only_step(unsigned flags) {
 }
 
 int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
+  if (use_cells()) return cells_step(vi, pi, flags);
   const bool prof_on = (flags & B2G_STEP_PROFILE) != 0;
   if (prof_on) HIP_TRY(hipEventRecord(ev[0], stream));
   if (flags & B2G_STEP_FIND_NEW_CONTACTS) {
@@ -1170,6 +1282,19 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
 
 // the whole step as one launch sequence with no host reads
 int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
+  if (use_cells()) {  // the eager sequence as it is: nothing in it waits for the host
+    d.segs = 0;  // a replay cannot know whether the list is the last broadphase's
+    d.part_on = 1;
+    enqueue_collide();
+    enqueue_cells_solve(vi, pi, !(flags & B2G_STEP_SKIP_BROADPHASE));
+    d.part_on = 0;
+    if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
+      int rc = broadphase(false);
+      if (rc) return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(pinned, d.cnt, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, stream));
+    return B2G_OK;
+  }
   enqueue_collide_and_colour(K, kGraphColourRounds);
   hipLaunchKernelGGL(k_colour_leftover, dim3(grid_for(std::max(est_cons, 256) * 5 / 4)), dim3(kBlock), 0, stream,
                      d);
@@ -1206,8 +1331,9 @@ int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
 
 int b2g_batch::graph_step(int vi, int pi, unsigned flags) {
   const unsigned key_flags = flags & (B2G_STEP_SKIP_BROADPHASE | B2G_STEP_EVENTS);
-  const int block = use_per_world() ? world_block_now() * 2 + (world_lds_now() ? 1 : 0) : 0;
-  last_path = block ? 2 : 1;
+  const bool cells = use_cells();
+  const int block = cells ? -1 : use_per_world() ? world_block_now() * 2 + (world_lds_now() ? 1 : 0) : 0;
+  last_path = cells ? 3 : block ? 2 : 1;
   if (graph_ev_live && hipEventQuery(graph_ev) == hipSuccess) {
     // the last replay and its counter copy have finished: refuse to build on
     // a step that overflowed
@@ -1254,7 +1380,7 @@ int b2g_batch::graph_step(int vi, int pi, unsigned flags) {
     swap_contact_buffers();
     pairs_current = true;
   }
-  round_host += kGraphColourRounds;
+  if (!cells) round_host += kGraphColourRounds;
   ++graph_steps;
   if (!(flags & B2G_STEP_ASYNC)) {
     HIP_TRY(hipStreamSynchronize(stream));
@@ -1637,7 +1763,7 @@ int b2g_set_option(b2g_batch* b, int32_t option, int32_t value) {
   if (!b) return fail(B2G_ERR_INVALID, "null batch");
   switch (option) {
     case B2G_OPT_SOLVER:
-      if (value < B2G_SOLVER_AUTO || value > B2G_SOLVER_WORLDS) return fail(B2G_ERR_INVALID, "solver mode");
+      if (value < B2G_SOLVER_AUTO || value > B2G_SOLVER_CELLS) return fail(B2G_ERR_INVALID, "solver mode");
       b->solver_mode = value;
       break;
     case B2G_OPT_WORLD_LDS:

@@ -48,6 +48,7 @@ enum Counter : int {
   C_PENDING = 9,    // entries of the pending-colour list of this step
   C_EVENTS = 11,    // contact events recorded this step
   C_HEAVY = 12,     // over-full shapes whose pair segments need sorting (this broadphase)
+  C_PART_CROSS = 13,  // constraints of the cells solver's cross group this step
   C_COUNT = 16
 };
 enum Status : int { S_CONTACT_OVERFLOW = 1, S_STACK_OVERFLOW = 2, S_TREE_CORRUPT = 4, S_EVENT_OVERFLOW = 8 };
@@ -165,6 +166,20 @@ struct Dev {
                      // keys among the active shapes' keys
   int* scan_tiles;
   int* cnt;
+  // the cells solver of large worlds (b2g_partition.h)
+  int part_on;        // this step solves by cells: contact_init leaves the colour tables alone
+  float4* part_geo;   // 2: (lo.x, lo.y, 1/L, L), (cells x, cells y, all-cross) as int bits
+  int* part_boff;     // 2 * kCells + 1: offsets of the cells' body lists (A cells, then B cells)
+  int* part_blist;    // 2 * nb: bodies of each cell
+  int2* part_brank;   // per body: its slot in its A cell and in its B cell (-1: not dynamic)
+  int* part_ccnt;     // kPartGroups: constraints per group (A cells, B cells, cross)
+  int* part_bucket;   // 2 * kCells * kCellCons: compacted constraints of each cell
+  int* part_cbucket;  // the cross group's compacted constraints
+  int* part_coff;     // kPartGroups * (kColourSlots + 1): slot offsets of each group's colours
+  unsigned long long* part_claim;  // per body: colouring claims of the cross group
+  unsigned long long* part_used;   // per body: colours the cross group gave its constraints
+  int* part_xcol;     // per cross-group constraint: its colour while colouring
+  int2* sloc;         // per slot: LDS slots of its two bodies in its cell (-1: global memory)
 };
 
 // grid-stride helpers with an XCD-aware block order: consecutive work tiles
@@ -1011,7 +1026,7 @@ __global__ void __launch_bounds__(kBlock, 6) k_contact_init(Dev d) {
     d.vc[2][s] = make_float4(as_float(ia), as_float(ib), as_float(vcount | (warm ? 256 : 0)), as_float(c));
     {  // this slot's place in each dynamic body's colour-ordered list (rank of
        // its colour among the body's colours)
-      const int col = d.ccol[c];
+      const int col = d.part_on ? -1 : d.ccol[c];
       if (col >= 0 && col < kBodyTab) {
         const unsigned long long below = (1ull << col) - 1ull;
         if (d.btype[ia] == kDynamicBody) d.btab[(size_t)ia * kBodyTab + __popcll(d.bmask[ia] & below)] = 2 * s;

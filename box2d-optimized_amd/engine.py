@@ -88,7 +88,7 @@ EXPORTS = [
     "b2g_set_counts", "b2g_download_solver_points", "b2a_world_destroy_body", "b2a_world_destroy_joint",
     "b2a_world_destroy_fixture", "b2a_world_set_filter", "b2a_world_body_count",
 ]
-SOLVER_AUTO, SOLVER_COLOURS, SOLVER_WORLDS = 0, 1, 2
+SOLVER_AUTO, SOLVER_COLOURS, SOLVER_WORLDS, SOLVER_CELLS = 0, 1, 2, 3
 OPT_SOLVER, OPT_WORLD_BLOCK, OPT_TUNED_WORLD_BLOCK, OPT_LAST_SOLVER, OPT_PER_WORLD_ELIGIBLE = 1, 2, 3, 4, 5
 OPT_WORLD_LDS, OPT_TUNED_WORLD_LDS, OPT_KEEP_IDLE = 6, 7, 8
 KERNEL_NARROWPHASE, KERNEL_WARM_START, KERNEL_CONTACT_VELOCITY, KERNEL_CONTACT_POSITION, \

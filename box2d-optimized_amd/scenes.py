@@ -362,7 +362,7 @@ def gas_and_chains(n_circles: int = 100000, n_chains: int = 500, links: int = 30
 
 
 def batched_sizes(worlds: int = 4096, seed: int = 0xC4, median: float = 200.0,
-                  sigma: float = 0.874, lo: int = 10, hi: int = 4000) -> np.ndarray:
+                  sigma: float = 0.95, lo: int = 10, hi: int = 4000) -> np.ndarray:
     rs = Stream(seed)
     z = rs.normal(worlds)
     return np.clip(np.round(median * np.exp(sigma * z)), lo, hi).astype(np.int64)

@@ -411,6 +411,33 @@ def ref_step_ordered(rw: RefWorld, dt=1.0 / 60.0, vi=8, pi=3):
     return keys[:nc.value], joints[:nj.value]
 
 
+def ref_last_events(rw: RefWorld):
+    """Begin (1) / End (2) contact events the reference's listener received
+    during the last ref_step_ordered, in call order: (kinds, fixture keys,
+    how many leading events Collide made; the rest are EndContact calls of
+    touching contacts the broadphase destroyed)."""
+    lib = Ref.lib()
+    lib.ref_world_last_events.argtypes = [_P, _I, _P, _P, _P]
+    n = lib.ref_world_last_events(rw.h, 0, None, None, None)
+    kinds = np.zeros(max(n, 1), np.int32)
+    keys = np.zeros(max(n, 1), np.uint64)
+    nc = ctypes.c_int(0)
+    lib.ref_world_last_events(rw.h, n, _ptr(kinds), _ptr(keys), ctypes.byref(nc))
+    return kinds[:n], keys[:n], nc.value
+
+
+def ref_last_postsolve(rw: RefWorld):
+    """PostSolve calls of the last ref_step_ordered: (contact keys, impulse
+    counts), in call order."""
+    lib = Ref.lib()
+    lib.ref_world_last_postsolve.argtypes = [_P, _I, _P, _P]
+    n = lib.ref_world_last_postsolve(rw.h, 0, None, None)
+    keys = np.zeros(max(n, 1), np.uint64)
+    counts = np.zeros(max(n, 1), np.int32)
+    lib.ref_world_last_postsolve(rw.h, n, _ptr(keys), _ptr(counts))
+    return keys[:n], counts[:n]
+
+
 def ref_state_to_records(rw: RefWorld, restated_bodies: np.ndarray) -> np.ndarray:
     """Body records with the reference's current sweep/velocity/transform and
     the restatement's (bit-identical) mass properties."""

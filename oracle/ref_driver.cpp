@@ -23,6 +23,14 @@ struct RefWorld {
   std::vector<b2Body*> bodies;
   std::vector<b2Fixture*> fixtures;
   std::vector<b2RevoluteJoint*> joints;
+  // Begin (1) / End (2) contact events of the last ref_world_step_ordered, in
+  // the order the reference called its listener, keyed by fixture indices
+  std::vector<int> ev_kind;
+  std::vector<uint64_t> ev_key;
+  int ev_collide = 0;  // leading events made by Collide; the rest by the broadphase's destroys
+  // PostSolve calls of that step: contact key and b2ContactImpulse::count
+  std::vector<uint64_t> ps_key;
+  std::vector<int> ps_count;
   explicit RefWorld(const b2Vec2& g) : world(g) {}
 };
 
@@ -132,13 +140,18 @@ b2Transform to_xf(const float* x) {
   return t;
 }
 
+// Without points the manifold's type, normal and point carry no meaning: the
+// reference leaves them unset in a new contact (b2Contact::b2Contact sets only
+// pointCount, src/dynamics/b2_contact.cpp:118) and the collide functions
+// return before writing them when they find nothing, so they are recorded as 0.
 void manifold_to_record(const b2Manifold& m, b2g_contact* out) {
-  out->type = (int)m.type;
+  const bool has = m.pointCount > 0;
+  out->type = has ? (int)m.type : 0;
   out->point_count = m.pointCount;
-  out->local_normal[0] = m.localNormal.x;
-  out->local_normal[1] = m.localNormal.y;
-  out->local_point[0] = m.localPoint.x;
-  out->local_point[1] = m.localPoint.y;
+  out->local_normal[0] = has ? m.localNormal.x : 0.0f;
+  out->local_normal[1] = has ? m.localNormal.y : 0.0f;
+  out->local_point[0] = has ? m.localPoint.x : 0.0f;
+  out->local_point[1] = has ? m.localPoint.y : 0.0f;
   for (int k = 0; k < 2; ++k) {
     if (k < m.pointCount) {
       out->point_local[2 * k] = m.points[k].localPoint.x;
@@ -374,9 +387,40 @@ void ref_world_profile(void* w, float* out) {
 // out_keys receives (min << 32 | max) fixture keys of the solved contacts in
 // solve order; out_joints the scene indices of the solved joints in order.
 namespace {
+uint64_t fixture_key(b2Contact* c) {
+  const uint64_t fa = c->GetFixtureA()->GetUserData().pointer;
+  const uint64_t fb = c->GetFixtureB()->GetUserData().pointer;
+  return ((fa < fb ? fa : fb) << 32) | (fa < fb ? fb : fa);
+}
+
+// b2ContactListener calls of one Step: BeginContact / EndContact as the
+// reference makes them (Collide's touching transitions, src/dynamics/
+// b2_contact.cpp:197-203; EndContact of a touching contact destroyed by the
+// broadphase, b2_contact_manager.cpp:48-51), PreSolve's touching set and
+// PostSolve's impulse counts (src/dynamics/b2_island.cpp:621-647). Solve
+// reports every island before it looks for new contacts
+// (src/dynamics/b2_world.cpp:640-666), so the events before the first
+// PostSolve are Collide's.
 struct TouchRecorder : public b2ContactListener {
   std::unordered_set<const b2Contact*> touching;
+  RefWorld* rw = nullptr;
+  bool solved = false;
+  void BeginContact(b2Contact* c) override {
+    rw->ev_kind.push_back(1);
+    rw->ev_key.push_back(fixture_key(c));
+    if (!solved) rw->ev_collide = (int)rw->ev_kind.size();
+  }
+  void EndContact(b2Contact* c) override {
+    rw->ev_kind.push_back(2);
+    rw->ev_key.push_back(fixture_key(c));
+    if (!solved) rw->ev_collide = (int)rw->ev_kind.size();
+  }
   void PreSolve(b2Contact* c, const b2Manifold*) override { touching.insert(c); }
+  void PostSolve(b2Contact* c, const b2ContactImpulse* impulse) override {
+    solved = true;
+    rw->ps_key.push_back(fixture_key(c));
+    rw->ps_count.push_back(impulse->count);
+  }
 };
 }  // namespace
 
@@ -412,6 +456,12 @@ extern "C" int ref_world_step_ordered(void* w, float dt, int vi, int pi, int cap
       bjoints[b].push_back({je->joint, je->other});
   }
   TouchRecorder rec;
+  rw->ev_kind.clear();
+  rw->ev_key.clear();
+  rw->ev_collide = 0;
+  rw->ps_key.clear();
+  rw->ps_count.clear();
+  rec.rw = rw;
   rw->world.SetContactListener(&rec);
   rw->world.Step(dt, vi, pi);
   rw->world.SetContactListener(nullptr);
@@ -465,6 +515,32 @@ extern "C" int ref_world_step_ordered(void* w, float dt, int vi, int pi, int cap
   *out_nc = nc;
   if (out_nj) *out_nj = nj;
   return nc <= cap ? 0 : 1;
+}
+
+// The contact events of the last ref_world_step_ordered: kinds (1 Begin,
+// 2 End) and fixture keys, up to cap, and how many of them Collide made
+// (the rest came from the broadphase's destroys); returns the count.
+extern "C" int ref_world_last_events(void* w, int cap, int* kinds, uint64_t* keys, int* n_collide) {
+  RefWorld* rw = (RefWorld*)w;
+  const int n = (int)rw->ev_kind.size();
+  for (int i = 0; i < n && i < cap; ++i) {
+    kinds[i] = rw->ev_kind[i];
+    keys[i] = rw->ev_key[i];
+  }
+  if (n_collide) *n_collide = rw->ev_collide;
+  return n;
+}
+
+// PostSolve calls of the last ref_world_step_ordered: contact keys and
+// impulse counts, up to cap; returns the count.
+extern "C" int ref_world_last_postsolve(void* w, int cap, uint64_t* keys, int* counts) {
+  RefWorld* rw = (RefWorld*)w;
+  const int n = (int)rw->ps_key.size();
+  for (int i = 0; i < n && i < cap; ++i) {
+    keys[i] = rw->ps_key[i];
+    counts[i] = rw->ps_count[i];
+  }
+  return n;
 }
 
 // 4 floats per fixture: the AABB the reference broadphase last used

@@ -13,7 +13,9 @@ Outputs (inputs and expected outputs only):
   onestep_<scene>.npz
                    reference state after N steps (bodies, contacts with
                    impulses, joint impulses) -> reference state after N+1,
-                   its contact list, and the order its island solver used
+                   its contact list, the order its island solver used, the
+                   Begin/End contact events its listener received and
+                   PostSolve's impulse count per solved contact
   trajectory_pyramid.npz
                    reference pyramid (20 rows) body states every 50 steps
                    over 1000 steps
@@ -92,7 +94,16 @@ SCENES = {
     "mixed": (lambda: S.small_mixed(), 200),
     "chain": (chain_scene, 60),
     "pile": (lambda: S.random_pile(1500, seed=9, half_w=25, height=60), 120),
+    # the same two scenes once they have settled
+    "pyramid_settled": (lambda: S.pyramid(20), 600),
+    "pile_settled": (lambda: S.random_pile(1500, seed=9, half_w=25, height=60), 400),
 }
+
+
+def chains_scene():
+    """Circles around 16 chains of 30 revolute-jointed links: neighbouring
+    links overlap at every joint (collideConnected false)."""
+    return S.gas_and_chains(n_circles=3000, n_chains=16, links=30, seed=0x5C3, half=80.0)
 
 
 def narrowphase(rng_seed=1234, per_kind=400):
@@ -157,7 +168,8 @@ def pairs():
     out = {}
     for name, (mk, steps) in {"pile2000": (lambda: S.random_pile(2000, seed=21, half_w=30, height=70), 60),
                               "filtered": (filtered_scene, 30),
-                              "mixed": (lambda: S.small_mixed(), 100)}.items():
+                              "mixed": (lambda: S.small_mixed(), 100),
+                              "chains": (chains_scene, 60)}.items():
         sc = mk()
         rw = oracle.RefWorld(sc)
         rw.step(steps)
@@ -170,6 +182,10 @@ def pairs():
         out[f"{name}_filter"] = arr["fix_filter"][:, :3].copy()
         out[f"{name}_body_type"] = arr["body_type"]
         out[f"{name}_keys"] = keys
+        if sc.joint_count:
+            out[f"{name}_joint_bodies"] = np.asarray(sc.joint_bodies, np.int32).reshape(-1, 2)
+            out[f"{name}_joint_flags"] = np.asarray(sc.joint_flags, np.int32)
+            out[f"{name}_state"] = rw.bodies()
         print("pairs:", name, len(keys), "pairs over", len(ab), "fixtures")
     np.savez_compressed(os.path.join(HERE, "pairs.npz"), **out)
 
@@ -183,13 +199,18 @@ def onestep():
         contacts0 = rw.contacts()
         joints0 = rw.joint_impulses()
         order_keys, order_joints = oracle.ref_step_ordered(rw)
+        ev_kind, ev_key, ev_collide = oracle.ref_last_events(rw)
+        ps_key, ps_count = oracle.ref_last_postsolve(rw)
         state1 = rw.bodies()
         contacts1 = rw.contacts()
         joints1 = rw.joint_impulses()
         np.savez_compressed(os.path.join(HERE, f"onestep_{name}.npz"), steps=np.int32(steps), state0=state0,
                             contacts0=contacts0, joints0=joints0, order_keys=order_keys,
-                            order_joints=order_joints, state1=state1, contacts1=contacts1, joints1=joints1)
-        print("onestep:", name, "bodies", len(state0), "contacts", len(contacts0), "solved", len(order_keys))
+                            order_joints=order_joints, state1=state1, contacts1=contacts1, joints1=joints1,
+                            events_kind=ev_kind, events_key=ev_key, events_collide=np.int32(ev_collide),
+                            postsolve_key=ps_key, postsolve_count=ps_count)
+        print("onestep:", name, "bodies", len(state0), "contacts", len(contacts0), "solved", len(order_keys),
+              "events", len(ev_kind))
 
 
 def trajectory():
@@ -220,8 +241,7 @@ def trajectory():
 if __name__ == "__main__":
     if not oracle.Ref.available():
         oracle.build(ref=True)
-    narrowphase()
-    shapes()
-    pairs()
-    onestep()
-    trajectory()
+    parts = {"narrowphase": narrowphase, "shapes": shapes, "pairs": pairs, "onestep": onestep,
+             "trajectory": trajectory}
+    for part in (sys.argv[1:] or list(parts)):  # e.g. `make_golden.py onestep pairs`
+        parts[part]()

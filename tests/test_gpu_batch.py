@@ -382,6 +382,55 @@ def test_gpu_snapshot_resume_bitexact(engine, tmp_path):
     assert np.array_equal(bits(ca["normal_impulse"]), bits(cb["normal_impulse"]))
 
 
+@pytest.mark.parametrize("name", ["pile", "chain"])
+def test_gpu_snapshot_load_then_step_vs_restatement(engine, tmp_path, name):
+    """A batch loaded from a snapshot holds the saved state bit for bit
+    (bodies, contacts with manifolds, impulses and colours, joint impulses),
+    and its next step is the step the restatement takes from that state in
+    the loaded batch's colour order (1e-5 m / 1e-5 rad, as the one-step
+    parity tests). The reference's own b2World::Dump
+    (src/dynamics/b2_world.cpp:1469-1531) drops the contact state, so the
+    restatement is the checker."""
+    import oracle
+    sc = scene_for(name)[0]
+    a = engine.Batch.from_scenes([sc]).advance(70)
+    path = str(tmp_path / "one.b2gsnap")
+    a.save(path)
+    b = engine.Batch.load(path, like=a)
+    bodies0, contacts0 = b.bodies(), b.contacts()
+    assert np.array_equal(_state(bodies0), _state(a.bodies()))
+    assert np.array_equal(contacts0.view(np.uint8), a.contacts().view(np.uint8))
+    assert np.array_equal(b.contact_colours(len(contacts0)), a.contact_colours(len(contacts0)))
+    joints0 = b.joints(sc.joint_count) if sc.joint_count else None
+    if sc.joint_count:
+        assert np.array_equal(joints0.view(np.uint8), a.joints(sc.joint_count).view(np.uint8))
+    b.step(8, 3, engine.STEP_SKIP_BROADPHASE)
+    gb, gc = b.bodies(), b.contacts()
+    r = oracle.RestatedWorld(sc)
+    rb = r.bodies()
+    for f in ("c", "a", "v", "w", "xf"):
+        rb[f] = bodies0[f]
+    r.set_bodies(rb)
+    r.set_contacts(contacts0)
+    if sc.joint_count:
+        j = r.joints()
+        for f in ("impulse", "motor_impulse", "lower_impulse", "upper_impulse"):
+            j[f] = joints0[f]
+        r.set_joints(j)
+    r.set_inv_dt0(np.float32(1.0) / np.float32(1.0 / 60.0))
+    from test_gpu_parity import _joint_colours
+    r.set_colours(oracle.contact_keys(gc), b.contact_colours(len(gc)), _joint_colours(sc) if sc.joint_count else None)
+    r.step(1, flags=engine.STEP_SKIP_BROADPHASE)
+    ob = r.bodies()
+    dyn = np.asarray(sc.body_type) == 2
+    dc = np.abs(gb["c"] - ob["c"])[dyn].max()
+    da = np.abs(gb["a"] - ob["a"])[dyn].max()
+    print(f"{name}: step after load vs restatement max |dc| {dc:.3g} |da| {da:.3g}; "
+          f"touching {(gc['point_count'] > 0).sum()}", flush=True)
+    assert dc < 1e-5 and da < 1e-5
+    assert np.array_equal(oracle.contact_keys(gc), oracle.contact_keys(r.contacts()))
+
+
 def _touching_keys(contacts):
     t = contacts[contacts["point_count"] > 0]
     a = np.minimum(t["shape_a"], t["shape_b"]).astype(np.uint64)

@@ -1,0 +1,139 @@
+"""GPU tests of the cells solver: one large world's constraints split by the
+cells of two staggered grids and a small cross group, each cell solved by one
+workgroup with its bodies in LDS (three launches per iteration).
+
+Its constraint order is another valid Gauss-Seidel order: every constraint
+once per iteration, constraints sharing a dynamic body one after the other.
+The device reports it per contact (phase * 65536 + colour), and the CPU
+restatement solving in that order takes the same step (1e-5 m / 1e-5 rad, the
+bound of the other one-step parity tests: only the device's sin/cos/sqrt
+differ in the last bits). Joints keep their own sweeps (before the contacts in
+the velocity iterations, after them in the position iterations,
+b2_island.cpp:329-338, 389-401).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from util import bits, load_pkg, scene_for
+
+pytestmark = pytest.mark.gpu
+S = load_pkg().scenes
+DT = np.float32(1.0 / 60.0)
+PHASE = 65536
+
+
+def _state(rec):
+    return np.concatenate([bits(rec[f]).reshape(len(rec), -1) for f in ("xf", "c", "a", "v", "w")], axis=1)
+
+
+def _scenes():
+    return {
+        "pile3000": (lambda: S.random_pile(3000, seed=5, half_w=30.0, height=80.0), 150),
+        "pyramid40": (lambda: S.pyramid(40), 60),
+        "gas_chains": (lambda: S.gas_and_chains(n_circles=4000, n_chains=16, links=30, seed=0x5C3, half=80.0), 30),
+        "chain": (lambda: scene_for("chain")[0], 45),
+    }
+
+
+def _cells_batch(engine, sc):
+    b = engine.Batch.from_scenes([sc])
+    b.set_option(engine.OPT_SOLVER, engine.SOLVER_CELLS)
+    return b
+
+
+def _joint_colours(sc):
+    from test_gpu_parity import _joint_colours as jc
+    return jc(sc)
+
+
+@pytest.mark.parametrize("name", list(_scenes()))
+def test_gpu_cells_one_step_vs_restatement_in_gpu_order(engine, name):
+    mk, warm = _scenes()[name]
+    sc = mk()
+    b = _cells_batch(engine, sc).advance(warm)
+    assert b.get_option(engine.OPT_LAST_SOLVER) == 3
+    bodies0, contacts0 = b.bodies(), b.contacts()
+    joints0 = b.joints(sc.joint_count) if sc.joint_count else None
+    b.step(8, 3, engine.STEP_SKIP_BROADPHASE)
+    gb, gc = b.bodies(), b.contacts()
+    colours = b.contact_colours(len(gc))
+    solved = colours >= 0
+    assert np.array_equal(solved, gc["point_count"] > 0)
+    phase = colours[solved] // PHASE
+    n_a, n_b, n_c = (int((phase == p).sum()) for p in range(3))
+    r = oracle.RestatedWorld(sc)
+    rb = r.bodies()
+    for f in ("c", "a", "v", "w", "xf"):
+        rb[f] = bodies0[f]
+    r.set_bodies(rb)
+    r.set_contacts(contacts0)
+    if sc.joint_count:
+        j = r.joints()
+        for f in ("impulse", "motor_impulse", "lower_impulse", "upper_impulse"):
+            j[f] = joints0[f]
+        r.set_joints(j)
+    r.set_inv_dt0(np.float32(1.0) / DT)
+    r.set_colours(oracle.contact_keys(gc), colours, _joint_colours(sc) if sc.joint_count else None)
+    r.step(1, flags=engine.STEP_SKIP_BROADPHASE)
+    ob = r.bodies()
+    dyn = np.asarray(sc.body_type) == 2
+    dc = np.abs(gb["c"] - ob["c"])[dyn].max()
+    da = np.abs(gb["a"] - ob["a"])[dyn].max()
+    dv = np.abs(gb["v"] - ob["v"])[dyn].max()
+    print(f"{name}: {int(dyn.sum())} bodies, {int(solved.sum())} solved contacts in phases A/B/C "
+          f"{n_a}/{n_b}/{n_c}, colours per phase {[int(colours[solved][phase == p].max() % PHASE) + 1 if (phase == p).any() else 0 for p in range(3)]}; "
+          f"max |dc| {dc:.3g} |da| {da:.3g} |dv| {dv:.3g}", flush=True)
+    assert n_a > 0
+    assert dc < 1e-5 and da < 1e-5 and dv < 1e-3
+    oc = r.contacts()
+    assert np.array_equal(oracle.contact_keys(gc), oracle.contact_keys(oc))
+    assert np.array_equal(gc["point_count"], oc["point_count"])
+
+
+def test_gpu_cells_steps_are_deterministic_and_graph_equals_eager(engine):
+    """Two runs give the same bits at every step (the colours come from fixed
+    priorities; which lane or LDS slot a body or constraint gets does not
+    change any arithmetic), and replays of the captured step equal eager
+    steps (the captured sequence is the eager one: no step of this solver
+    waits for the host)."""
+    sc = S.random_pile(2500, seed=17, half_w=25.0, height=70.0)
+    a = _cells_batch(engine, sc).advance(5)
+    b = _cells_batch(engine, sc).advance(5)
+    g = _cells_batch(engine, sc).advance(5)
+    for k in range(40):
+        a.step(8, 3, 0)
+        b.step(8, 3, 0)
+        g.step(8, 3, engine.STEP_GRAPH)
+        if k % 8 == 7:
+            sa = _state(a.bodies())
+            assert np.array_equal(sa, _state(b.bodies())), f"step {k}: runs differ"
+            assert np.array_equal(sa, _state(g.bodies())), f"step {k}: graph replay differs"
+    assert g.counts().graph_steps == 40
+    ca, cg = a.contacts(), g.contacts()
+    assert np.array_equal(ca.view(np.uint8), cg.view(np.uint8))
+
+
+def test_gpu_cells_pile_settles_like_the_colour_sweeps(engine):
+    """Over 300 steps of a 4000-body pile the cells order and the colour
+    order keep the same physics: kinetic energy and penetration statistics
+    agree (the two orders are different Gauss-Seidel orders, so the bodies
+    do not agree one by one), and the deepest penetration stays small."""
+    sc = S.random_pile(4000, seed=23, half_w=35.0, height=90.0)
+    cells = _cells_batch(engine, sc)
+    cols = engine.Batch.from_scenes([sc])
+    cols.set_option(engine.OPT_SOLVER, engine.SOLVER_COLOURS)
+    cells.advance(300)
+    cols.advance(300)
+    assert cols.get_option(engine.OPT_LAST_SOLVER) == 1 and cells.get_option(engine.OPT_LAST_SOLVER) == 3
+    m = np.asarray(sc.arrays()["body_type"]) == 2
+    stats = []
+    for bt in (cells, cols):
+        bd = bt.bodies()
+        v = bd["v"][m]
+        ke = float((v * v).sum())
+        stats.append((ke, float(np.median(bd["c"][m][:, 1])), bt.counts().touching))
+    print(f"cells KE {stats[0][0]:.4g} median y {stats[0][1]:.4g} touching {stats[0][2]}; "
+          f"colours KE {stats[1][0]:.4g} median y {stats[1][1]:.4g} touching {stats[1][2]}", flush=True)
+    assert abs(stats[0][1] - stats[1][1]) < 0.5
+    assert abs(stats[0][2] - stats[1][2]) < 0.05 * stats[1][2]

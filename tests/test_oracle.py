@@ -66,7 +66,7 @@ def test_shape_setup_and_mass_match_reference_golden_bitexact():
         assert np.array_equal(bits(body["c"]), bits(rb[:, 0:2])), name
 
 
-@pytest.mark.parametrize("name", ["pyramid", "mixed", "chain", "pile"])
+@pytest.mark.parametrize("name", ["pyramid", "mixed", "chain", "pile", "pyramid_settled", "pile_settled"])
 def test_one_step_in_reference_order_is_bitexact(name):
     """From the reference's state after N steps, one restated step that visits
     the constraints in the reference's island order reproduces the
@@ -116,6 +116,26 @@ def test_pair_set_matches_reference_golden(name):
     keys = oracle.orc_find_pairs(g[f"{name}_aabbs"], g[f"{name}_shape_body"], g[f"{name}_shape_type"],
                                  g[f"{name}_filter"], g[f"{name}_body_type"])
     assert np.array_equal(keys, g[f"{name}_keys"])
+
+
+def test_reference_golden_excludes_exactly_the_joint_connected_pairs():
+    """The chains golden pins the joint rule of the pair filter: the
+    reference's pair set equals every allowed overlapping pair (the
+    restatement's pair finder, itself checked against brute force below)
+    less the pairs of bodies a joint with collideConnected false connects
+    (b2Body::ShouldCollide, src/dynamics/b2_body.cpp:459-480) - and the scene
+    has hundreds of those, one at every chain joint."""
+    g = golden("pairs.npz")
+    allp = oracle.orc_find_pairs(g["chains_aabbs"], g["chains_shape_body"], g["chains_shape_type"],
+                                 g["chains_filter"], g["chains_body_type"])
+    body = g["chains_shape_body"]
+    joined = {(min(a, b), max(a, b)) for (a, b), f in zip(g["chains_joint_bodies"].tolist(),
+                                                         g["chains_joint_flags"].tolist()) if not (f & 1)}
+    ba = body[(allp >> np.uint64(32)).astype(np.int64)]
+    bb = body[(allp & np.uint64(0xffffffff)).astype(np.int64)]
+    jointed = np.array([(min(x, y), max(x, y)) in joined for x, y in zip(ba.tolist(), bb.tolist())], bool)
+    assert jointed.sum() >= 16 * 29
+    assert np.array_equal(allp[~jointed], g["chains_keys"])
 
 
 def test_pair_set_matches_brute_force():
