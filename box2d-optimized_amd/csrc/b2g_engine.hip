@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2209,6 +2210,120 @@ int b2g_collide_pairs(int32_t n, const b2g_shape* sa, const float* xfa, const b2
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// b2g_collide_pairs without allocations: the same narrowphase kernel, with
+// every staging buffer carved out of one file-scope device array. Each GPU
+// holds its own copy of the array until the process ends; calls that overlap
+// on one GPU share it and race.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kNoAllocMaxPairs = 4096;
+constexpr size_t kNoAllocScratchBytes = size_t(4) << 20;
+constexpr int kNoAllocMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// the array's device address per GPU: the array's name is a host-side
+// stand-in, so host code only ever passes these addresses on
+void* g_scratch_ptr[kNoAllocMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_for_current_device(unsigned char** out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return fail(B2G_ERR_INVALID, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(B2G_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+  return B2G_OK;
+}
+
+// caller records -> the narrowphase's header and geometry arrays
+__global__ void __launch_bounds__(kBlock) k_pair_shapes(int n, const b2g_shape* __restrict__ sa,
+                                                        const b2g_shape* __restrict__ sb, int4* __restrict__ ha,
+                                                        float4* __restrict__ ga, int4* __restrict__ hb,
+                                                        float4* __restrict__ gb) {
+  B2G_FOR(i, 0, n) {
+    float4 mat;
+    int4 filt;
+    shape_to_device(sa[i], 0, &ha[i], ga + 9 * (size_t)i, &mat, &filt);
+    shape_to_device(sb[i], 0, &hb[i], gb + 9 * (size_t)i, &mat, &filt);
+  }
+}
+
+// the narrowphase's packed manifolds -> caller contact records, as
+// b2g_collide_pairs fills them on the host
+__global__ void __launch_bounds__(kBlock) k_pair_records(int n, const b2g_shape* __restrict__ sa,
+                                                         const b2g_shape* __restrict__ sb,
+                                                         const float4* __restrict__ man, const int* __restrict__ ok,
+                                                         b2g_contact* __restrict__ out) {
+  B2G_FOR(i, 0, n) {
+    b2g_contact o{};
+    o.shape_a = sa[i].body;
+    o.shape_b = sb[i].body;
+    manifold_to_record(man + 4 * (size_t)i, &o);
+    if (!ok[i]) o.point_count = -1;
+    out[i] = o;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int b2g_collide_pairs_noalloc(int32_t n, const b2g_shape* sa, const float* xfa, const b2g_shape* sb,
+                              const float* xfb, b2g_contact* out) {
+  if (n < 0) return fail(B2G_ERR_INVALID, "negative count");
+  if (n > kNoAllocMaxPairs)
+    return fail(B2G_ERR_CAPACITY, "b2g_collide_pairs_noalloc takes at most " + std::to_string(kNoAllocMaxPairs) +
+                                      " pairs per call (got " + std::to_string(n) + "); use b2g_collide_pairs");
+  if (n == 0) return B2G_OK;
+  if (!sa || !xfa || !sb || !xfb || !out) return fail(B2G_ERR_INVALID, "null argument");
+  unsigned char* base = nullptr;
+  int rc = scratch_for_current_device(&base);
+  if (rc) return rc;
+  const size_t N = (size_t)n;
+  const size_t o_sa = 0, o_sb = align256(o_sa + N * sizeof(b2g_shape)), o_xa = align256(o_sb + N * sizeof(b2g_shape)),
+               o_xb = align256(o_xa + N * sizeof(float4)), o_ha = align256(o_xb + N * sizeof(float4)),
+               o_hb = align256(o_ha + N * sizeof(int4)), o_ga = align256(o_hb + N * sizeof(int4)),
+               o_gb = align256(o_ga + 9 * N * sizeof(float4)), o_man = align256(o_gb + 9 * N * sizeof(float4)),
+               o_ok = align256(o_man + 4 * N * sizeof(float4)), o_out = align256(o_ok + N * sizeof(int)),
+               total = o_out + N * sizeof(b2g_contact);
+  if (total > kNoAllocScratchBytes) return fail(B2G_ERR_CAPACITY, "scratch layout exceeds the scratch array");
+  b2g_shape* dsa = reinterpret_cast<b2g_shape*>(base + o_sa);
+  b2g_shape* dsb = reinterpret_cast<b2g_shape*>(base + o_sb);
+  float4* dxa = reinterpret_cast<float4*>(base + o_xa);
+  float4* dxb = reinterpret_cast<float4*>(base + o_xb);
+  int4* dha = reinterpret_cast<int4*>(base + o_ha);
+  int4* dhb = reinterpret_cast<int4*>(base + o_hb);
+  float4* dga = reinterpret_cast<float4*>(base + o_ga);
+  float4* dgb = reinterpret_cast<float4*>(base + o_gb);
+  float4* dman = reinterpret_cast<float4*>(base + o_man);
+  int* dok = reinterpret_cast<int*>(base + o_ok);
+  b2g_contact* dout = reinterpret_cast<b2g_contact*>(base + o_out);
+  HIP_TRY(hipMemcpy(dsa, sa, N * sizeof(b2g_shape), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dsb, sb, N * sizeof(b2g_shape), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dxa, xfa, N * sizeof(float4), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dxb, xfb, N * sizeof(float4), hipMemcpyHostToDevice));
+  const dim3 grid(grid_for(n)), block(kBlock);
+  hipLaunchKernelGGL(k_pair_shapes, grid, block, 0, 0, n, dsa, dsb, dha, dga, dhb, dgb);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_collide_pairs, grid, block, 0, 0, n, dha, dga, dxa, dhb, dgb, dxb, dman, dok);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_pair_records, grid, block, 0, 0, n, dsa, dsb, dman, dok, dout);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpy(out, dout, N * sizeof(b2g_contact), hipMemcpyDeviceToHost));
+  return B2G_OK;
 }
 
 int b2g_find_pairs(int32_t n, const float* aabbs, const int32_t* shape_body, const int32_t* shape_type,

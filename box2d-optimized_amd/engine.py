@@ -79,7 +79,8 @@ EXPORTS = [
     "b2g_set_worlds", "b2g_set_world_params", "b2g_upload_bodies", "b2g_upload_shapes",
     "b2g_upload_joints", "b2g_upload_contacts", "b2g_batch_step", "b2g_batch_wait",
     "b2g_get_counts", "b2g_get_profile", "b2g_download_bodies", "b2g_download_contacts",
-    "b2g_download_joints", "b2g_download_contact_colours", "b2g_collide_pairs", "b2g_find_pairs",
+    "b2g_download_joints", "b2g_download_contact_colours", "b2g_collide_pairs", "b2g_collide_pairs_noalloc",
+    "b2g_find_pairs",
     "b2a_world_from_scene", "b2a_world_destroy", "b2a_world_step", "b2a_world_get_bodies",
     "b2a_world_contact_count", "b2a_world_get_contacts", "b2a_world_batch", "b2a_world_prepare",
     "b2g_time_kernel", "b2g_kernel_time", "b2a_scene_records", "b2a_batch_from_scenes", "b2s_run",
@@ -142,6 +143,7 @@ def lib():
         L.b2g_get_counts.argtypes = [_P, _P]
         L.b2g_get_profile.argtypes = [_P, _P]
         L.b2g_collide_pairs.argtypes = [_I, _P, _P, _P, _P, _P]
+        L.b2g_collide_pairs_noalloc.argtypes = [_I, _P, _P, _P, _P, _P]
         L.b2g_find_pairs.argtypes = [_I, _P, _P, _P, _P, _I, _P, ctypes.c_int64, _P, _P]
         L.b2a_world_from_scene.argtypes = [_P, _P, _I]
         L.b2a_world_destroy.argtypes = [_P]
@@ -526,16 +528,23 @@ class World:
         return Batch(handle=self.L.b2a_world_batch(self.h))
 
 
-def collide_pairs(shape_a, xf_a, shape_b, xf_b):
+def collide_pairs(shape_a, xf_a, shape_b, xf_b, noalloc=False):
+    """Narrowphase manifolds of n shape pairs (b2g_collide_pairs). With
+    noalloc, b2g_collide_pairs_noalloc: no device allocation, at most
+    NOALLOC_MAX_PAIRS pairs per call."""
     sa = np.ascontiguousarray(shape_a, dtype=SHAPE_DTYPE)
     sb = np.ascontiguousarray(shape_b, dtype=SHAPE_DTYPE)
     xa = np.ascontiguousarray(xf_a, dtype=np.float32).reshape(-1, 4)
     xb = np.ascontiguousarray(xf_b, dtype=np.float32).reshape(-1, 4)
     n = len(sa)
     out = np.zeros(n, dtype=CONTACT_DTYPE)
-    if n:
-        check(lib().b2g_collide_pairs(n, _ptr(sa), _ptr(xa), _ptr(sb), _ptr(xb), _ptr(out)))
+    if n or noalloc:
+        fn = lib().b2g_collide_pairs_noalloc if noalloc else lib().b2g_collide_pairs
+        check(fn(n, _ptr(sa), _ptr(xa), _ptr(sb), _ptr(xb), _ptr(out)))
     return out
+
+
+NOALLOC_MAX_PAIRS = 4096
 
 
 def find_pairs(aabbs, shape_body, shape_type, filt, body_type, max_pairs=None):

@@ -387,6 +387,17 @@ int b2g_snapshot_load(b2g_batch** out, const char* path);
 int b2g_collide_pairs(int32_t n, const b2g_shape* shape_a, const float* xf_a,
                       const b2g_shape* shape_b, const float* xf_b, b2g_contact* out);
 
+/* b2g_collide_pairs for callers that cannot allocate device memory (code
+ * that runs under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). No counterpart in the reference. Same arguments and
+ * results as b2g_collide_pairs, bit for bit; the staging lives in a 4 MiB
+ * device array that each GPU holds from the first call until the process
+ * ends, so a call takes at most 4096 pairs (B2G_ERR_CAPACITY beyond). Runs on
+ * the null stream of the current device and returns when `out` is written;
+ * calls that overlap on one GPU share the array and race. */
+int b2g_collide_pairs_noalloc(int32_t n, const b2g_shape* shape_a, const float* xf_a,
+                              const b2g_shape* shape_b, const float* xf_b, b2g_contact* out);
+
 /* Broadphase: all pairs of the n AABBs (4 floats each: lower.x, lower.y,
  * upper.x, upper.y) that overlap (inclusive test, b2TestOverlap) and whose
  * shapes may collide (different bodies, at least one dynamic body, filter
