@@ -206,6 +206,10 @@ def test_gpu_destroy_bodies_joint_fixture_mid_run_vs_restatement(engine):
     w.destroy_fixture(fixtures_left.index(kill_fixture_orig))
     w.prepare()
     b = w.batch()
+    # the drop-in's Step sets dtRatio = inv_dt0 * dt itself; stepping the batch
+    # directly below, the same ratio is set here
+    ratio = np.float32(np.float32(1.0) / DT) * DT
+    b.set_world_params(0, [engine.WorldParams((sc.gravity[0], sc.gravity[1]), DT, ratio, 1, 1)])
     assert b.counts().bodies == edited.body_count and b.counts().shapes == edited.fixture_count
     assert b.counts().joints == edited.joint_count
     # survivors of the contact list, renumbered
