@@ -172,6 +172,13 @@ struct b2g_batch {
   int max_small_world_bodies = 0;  // largest of the others
   int solver_mode = 0;       // B2G_SOLVER_AUTO / _COLOURS / _WORLDS
   bool keep_idle = true;     // B2G_OPT_KEEP_IDLE
+  bool asleep_uploaded = false;  // a body was uploaded asleep: the awake flags apply from then on
+  bool sleep_now() const {       // this step applies the awake flags and sleep timers
+    if (asleep_uploaded) return true;
+    for (int w = 0; w < d.nw; ++w)
+      if (h_params[w].active && h_params[w].allow_sleep) return true;
+    return false;
+  }
   int world_block = 0;       // threads per world workgroup (0: autotune)
   int tuned_block = 0;       // chosen by the autotuner (or world_block)
   bool tuned_lds = false;    // autotuner: stage world bodies in LDS
@@ -189,6 +196,7 @@ struct b2g_batch {
   bool use_cells() const;
   void enqueue_cells_solve(int vi, int pi, bool finish_in_broadphase);
   int cells_step(int vi, int pi, unsigned flags);
+  int sleep_step(int vi, int pi, unsigned flags);
 
   void enqueue_colour_round(int K);
   void enqueue_colour_groups();
@@ -282,7 +290,9 @@ int b2g_batch::alloc() {
   A(d.cnt, kCntColOff + kColourSlots + 1);
   d.col_off = d.cnt + kCntColOff;
   // the cells solver
-  A(d.part_geo, 2); A(d.part_boff, 2 * kCells + 1); A(d.part_blist, 2 * (size_t)NB); A(d.part_brank, NB);
+  A(d.bsflag, NB); A(d.bstime, NB); A(d.bwake, NB); A(d.bactive, NB); A(d.bncontact, NB); A(d.bjointed, NB);
+  A(d.isl_awake, NB); A(d.isl_min, NB); A(d.isl_cons, NB);
+  A(d.part_geo, 2); A(d.part_boff, 2 * kCells + 1); A(d.part_bcur, 2 * kCells); A(d.part_acc, 5); A(d.part_sync, 4); A(d.part_blist, 2 * (size_t)NB); A(d.part_brank, NB);
   A(d.part_ccnt, kPartGroups); A(d.part_bucket, (size_t)2 * kCells * kCellCons); A(d.part_cbucket, NC);
   A(d.part_coff, (size_t)kPartGroups * (kColourSlots + 1)); A(d.part_claim, NB); A(d.part_used, NB);
   A(d.part_xcol, NC); A(d.sloc, NC);
@@ -321,6 +331,11 @@ int b2g_batch::alloc() {
   h_shape_body.assign(NS, 0);
   h_wshape0.assign(NW, 0);
   HIP_TRY(hipMemcpy(d.wshape0, h_wshape0.data(), NW * sizeof(int), hipMemcpyHostToDevice));
+  {  // the cells solver's bounds accumulator starts empty
+    const unsigned acc[5] = {0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+    HIP_TRY(hipMemcpy(d.part_acc, acc, sizeof(acc), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(d.part_sync, 0, 4 * sizeof(unsigned)));  // each launch leaves its counter at 0
+  }
   {  // colouring upkeep: nothing used yet, every world colours from scratch first
     std::vector<int> minus1(NW, -1), one(NW, 1);
     HIP_TRY(hipMemcpy(d.wcolmax, minus1.data(), NW * sizeof(int), hipMemcpyHostToDevice));
@@ -550,7 +565,7 @@ int b2g_batch::broadphase(bool use_given_aabbs) {
                      stream, d);
   hipLaunchKernelGGL(k_merge_contacts, dim3(grid_for(std::max(est_contacts, ns) * 5 / 4)), dim3(kBlock), 0, stream,
                      d);
-  if (d.events_on)
+  if (d.events_on || d.sleep_on)  // EndContact events; wake-ups of the destroyed touching contacts' bodies
     hipLaunchKernelGGL(k_dead_contact_events, dim3(grid_for(std::max(est_contacts, 256))), dim3(kBlock), 0, stream,
                        d);
   hipLaunchKernelGGL(k_finish_pairs, dim3(1), dim3(1), 0, stream, d);
@@ -602,6 +617,11 @@ int b2g_batch::rebuild_joint_tables() {
     if (h_joints[q].flags & 1) continue;
     other[pos[h_joints[q].body_a]++] = h_joints[q].body_b;
     other[pos[h_joints[q].body_b]++] = h_joints[q].body_a;
+  }
+  {  // bodies a joint attaches are no orphans (b2Body::GetJointList, b2_world.cpp:545)
+    std::vector<int> jointed(std::max(nb, 1), 0);
+    for (int q = 0; q < nj; ++q) jointed[h_joints[q].body_a] = jointed[h_joints[q].body_b] = 1;
+    if (nb) HIP_TRY(hipMemcpy(d.bjointed, jointed.data(), nb * sizeof(int), hipMemcpyHostToDevice));
   }
   if (nj) HIP_TRY(hipMemcpy(d.jorder, order.data(), nj * sizeof(int), hipMemcpyHostToDevice));
   {  // joints grouped by world (of body A), for the per-world island pass
@@ -698,19 +718,26 @@ bool b2g_batch::use_cells() const {
 
 void b2g_batch::enqueue_cells_solve(int vi, int pi, bool finish_in_broadphase) {
   const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
-  hipLaunchKernelGGL(k_part_bodies, dim3(1), dim3(kPartGridThreads), 0, stream, d);
-  hipLaunchKernelGGL(k_part_classify, dim3(gk), dim3(kBlock), 0, stream, d);
-  hipLaunchKernelGGL(k_part_colour_cells, dim3(2 * kCells), dim3(kCellThreads), 0, stream, d);
-  hipLaunchKernelGGL(k_part_colour_cross, dim3(1), dim3(kCrossThreads), 0, stream, d);
+  const int gb = grid_for(d.nb);
+  // the grid (bounds, then the cell size in the bounds launch's last
+  // workgroup), the cells' body counts (scanned in the count launch's last
+  // workgroup), the bodies and constraints into their cells, the colours
+  // (the cross group's in the colour launch's last workgroup)
+  hipLaunchKernelGGL(k_part_bounds, dim3(std::min(gb, kPartBoundsBlocks)), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_part_hist, dim3(std::min(gb, kPartBoundsBlocks)), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_part_classify, dim3(std::max(gk, gb)), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_part_colour_cells, dim3(2 * kCells + 1), dim3(kCellThreads), 0, stream, d);
   enqueue_islands_and_init();
-  const long long cons = std::max(est_cons, 1);
-  // one sweep: the A cells, the B cells, the cross group (timed as one run)
-  auto sweep = [&](int id, auto kernel, int it) {
+  // the cells' share of the constraints (the cross group's count is the last
+  // read-back's): the timed run is the two cell launches
+  const long long cells_cons = std::max((long long)est_cons - (long long)pinned[C_PART_CROSS], 1LL);
+  // one sweep: the A cells, the B cells, then the cross group
+  auto sweep = [&](int id, auto kernel, auto cross_kernel, int it) {
     tbegin(id);
     hipLaunchKernelGGL(kernel, dim3(kCells), dim3(kCellThreads), 0, stream, d, 0, it);
     hipLaunchKernelGGL(kernel, dim3(kCells), dim3(kCellThreads), 0, stream, d, 1, it);
-    hipLaunchKernelGGL(kernel, dim3(1), dim3(kCellThreads), 0, stream, d, 2, it);
-    tend(id, cons, 3);
+    tend(id, cells_cons, 2);
+    hipLaunchKernelGGL(cross_kernel, dim3(1), dim3(kCrossThreads), 0, stream, d, 2, it);
   };
   auto joint_sweep = [&](auto kernel, auto... extra) {
     const int njc = (int)h_jcol_off.size() - 1;
@@ -719,17 +746,19 @@ void b2g_batch::enqueue_cells_solve(int vi, int pi, bool finish_in_broadphase) {
       if (e > b) hipLaunchKernelGGL(kernel, dim3(grid_for(e - b)), dim3(kBlock), 0, stream, d, b, e, extra...);
     }
   };
-  sweep(B2G_KERNEL_WARM_START, k_part_sweep<kPartWarm, kCellThreads>, 0);
+  sweep(B2G_KERNEL_WARM_START, k_part_sweep<kPartWarm, kCellThreads>, k_part_sweep<kPartWarm, kCrossThreads>, 0);
   joint_sweep(k_joint_init);
   for (int it = 0; it < vi; ++it) {
     joint_sweep(k_joint_velocity);
-    sweep(B2G_KERNEL_CONTACT_VELOCITY, k_part_sweep<kPartVelocity, kCellThreads>, 0);
+    sweep(B2G_KERNEL_CONTACT_VELOCITY, k_part_sweep<kPartVelocity, kCellThreads>,
+          k_part_sweep<kPartVelocity, kCrossThreads>, 0);
   }
   (void)hipStreamWaitEvent(stream, join_ev, 0);
   hipLaunchKernelGGL(k_store_and_integrate, dim3(grid_for(std::max(est_cons + d.nj, d.nb))), dim3(kBlock), 0, stream,
                      d, 1);
   for (int it = 0; it < pi; ++it) {
-    sweep(B2G_KERNEL_CONTACT_POSITION, k_part_sweep<kPartPosition, kCellThreads>, it);
+    sweep(B2G_KERNEL_CONTACT_POSITION, k_part_sweep<kPartPosition, kCellThreads>,
+          k_part_sweep<kPartPosition, kCrossThreads>, it);
     joint_sweep(k_joint_position, it);
   }
   if (finish_in_broadphase)
@@ -806,21 +835,31 @@ constexpr int kIslandLdsSmall = 2048, kIslandLdsMid = 8192, kIslandLdsLarge = 32
 // position phase joins it (enqueue_islands_join).
 void b2g_batch::enqueue_islands_and_init() {
   const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
-  (void)hipEventRecord(fork_ev, stream);
-  (void)hipStreamWaitEvent(side, fork_ev, 0);
+  // With sleeping, the islands decide which bodies move at all, so they come
+  // first, on the main stream: seeds, wake-ups and the velocity integration of
+  // the bodies that move follow them.
+  hipStream_t is = d.sleep_on ? stream : side;
+  if (!d.sleep_on) {
+    (void)hipEventRecord(fork_ev, stream);
+    (void)hipStreamWaitEvent(side, fork_ev, 0);
+  }
   if (worlds_contiguous && max_world_bodies <= kIslandLdsLarge) {
     // one workgroup per world, the forest in LDS
     if (max_world_bodies <= kIslandLdsSmall)
-      hipLaunchKernelGGL(k_world_islands<kIslandLdsSmall>, dim3(d.nw), dim3(256), 0, side, d);
+      hipLaunchKernelGGL(k_world_islands<kIslandLdsSmall>, dim3(d.nw), dim3(256), 0, is, d);
     else if (max_world_bodies <= kIslandLdsMid)
-      hipLaunchKernelGGL(k_world_islands<kIslandLdsMid>, dim3(d.nw), dim3(512), 0, side, d);
+      hipLaunchKernelGGL(k_world_islands<kIslandLdsMid>, dim3(d.nw), dim3(512), 0, is, d);
     else
-      hipLaunchKernelGGL(k_world_islands<kIslandLdsLarge>, dim3(d.nw), dim3(1024), 0, side, d);
+      hipLaunchKernelGGL(k_world_islands<kIslandLdsLarge>, dim3(d.nw), dim3(1024), 0, is, d);
   } else {
-    hipLaunchKernelGGL(k_island_link, dim3(grid_for(std::max(est_cons, 1) + d.nj)), dim3(kBlock), 0, side, d);
-    hipLaunchKernelGGL(k_island_label, dim3(grid_for(d.nb)), dim3(kBlock), 0, side, d);
+    hipLaunchKernelGGL(k_island_link, dim3(grid_for(std::max(est_cons, 1) + d.nj)), dim3(kBlock), 0, is, d);
+    hipLaunchKernelGGL(k_island_label, dim3(grid_for(d.nb)), dim3(kBlock), 0, is, d);
   }
-  (void)hipEventRecord(join_ev, side);
+  if (d.sleep_on) {
+    hipLaunchKernelGGL(k_sleep_seed, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
+    hipLaunchKernelGGL(k_sleep_activate, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
+  }
+  (void)hipEventRecord(join_ev, is);
   hipLaunchKernelGGL(k_contact_init, dim3(gk), dim3(kBlock), 0, stream, d);
   if (d.nj) hipLaunchKernelGGL(k_joint_prepare, dim3(grid_for(d.nj)), dim3(kBlock), 0, stream, d);
 }
@@ -1106,6 +1145,7 @@ int b2g_batch::step(int vi, int pi, unsigned flags) {
     round_host = 0;
   }
   d.events_on = (flags & B2G_STEP_EVENTS) ? 1 : 0;
+  d.sleep_on = sleep_now() ? 1 : 0;
   d.segs = 0;  // an eager step sets it once its contact list is known
   const bool collide_only = (flags & B2G_STEP_COLLIDE_ONLY) != 0;
   if ((flags & B2G_STEP_SKIP_BROADPHASE) && !collide_only) {  // bodies move without a rebuild or a pair query
@@ -1115,7 +1155,7 @@ int b2g_batch::step(int vi, int pi, unsigned flags) {
   if (d.events_on) hipLaunchKernelGGL(k_events_reset, dim3(1), dim3(1), 0, stream, d);
   if (collide_only) return collide_only_step(flags);
   const bool prof_on = (flags & B2G_STEP_PROFILE) != 0;
-  const bool graph_ok = (flags & B2G_STEP_GRAPH) && !prof_on && timed_kernel < 0 &&
+  const bool graph_ok = (flags & B2G_STEP_GRAPH) && !prof_on && timed_kernel < 0 && !d.sleep_on &&
                         !(flags & B2G_STEP_FIND_NEW_CONTACTS) && graph_K > 0 &&
                         (!use_per_world() || world_block > 0 || tuned_block > 0);
   if (graph_ok) return graph_step(vi, pi, flags);
@@ -1134,7 +1174,9 @@ int b2g_batch::collide_only_step(unsigned flags) {
   }
   flush_finish();
   const int gc = grid_for(std::max(est_contacts, 256) * 5 / 4);
+  if (d.sleep_on) hipLaunchKernelGGL(k_sleep_reset, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
   launch(B2G_KERNEL_NARROWPHASE, est_contacts, k_narrowphase, dim3(gc), dim3(kBlock), 0, d, 0);
+  if (d.sleep_on) hipLaunchKernelGGL(k_sleep_wake, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
   hipLaunchKernelGGL(k_clear_forces, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
   if (!(flags & B2G_STEP_ASYNC)) {
     int rc = read_counters();
@@ -1144,7 +1186,75 @@ int b2g_batch::collide_only_step(unsigned flags) {
   return B2G_OK;
 }
 
+// A step that applies the awake flags (a world allows sleeping, or a body
+// sleeps): the colour sweeps, with the islands ahead of the velocity phase
+// and the sleep timers after the position phase, before the broadphase's
+// destroyed contacts wake their bodies (b2_world.cpp:526-666).
+int b2g_batch::sleep_step(int vi, int pi, unsigned flags) {
+  const bool prof_on = (flags & B2G_STEP_PROFILE) != 0;
+  last_path = 1;
+  if (prof_on) HIP_TRY(hipEventRecord(ev[0], stream));
+  int rc;
+  if (flags & B2G_STEP_FIND_NEW_CONTACTS) {
+    if ((rc = broadphase(false))) return rc;
+  }
+  d.segs = pairs_current && worlds_contiguous ? 1 : 0;
+  hipLaunchKernelGGL(k_sleep_reset, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
+  enqueue_collide_and_colour(kMaxColours, kColourRoundsPerStep);
+  if (prof_on) HIP_TRY(hipEventRecord(ev[1], stream));
+  enqueue_colour_groups();
+  enqueue_islands_and_init();
+  if ((rc = read_counters())) return rc;
+  if ((rc = status_error())) return rc;
+  est_cons = pinned[C_CONS];
+  for (int extra = 0; pinned[C_REMAIN] > 0; extra += 2) {
+    if (extra > 4096) return fail(B2G_ERR_CAPACITY, "colouring did not converge");
+    enqueue_colour_round(kMaxColours);
+    enqueue_colour_round(kMaxColours);
+    if ((rc = read_counters())) return rc;
+    if (pinned[C_REMAIN] == 0) {
+      hipLaunchKernelGGL(k_colour_clear_counts, dim3(1), dim3(kBlock), 0, stream, d);
+      enqueue_colour_groups();
+      hipLaunchKernelGGL(k_contact_init, dim3(grid_for(std::max(est_cons, 256) * 5 / 4)), dim3(kBlock), 0, stream, d);
+      if ((rc = read_counters())) return rc;
+    }
+  }
+  const int* off = pinned + kCntColOff;
+  std::vector<int> colours, sizes, begins;
+  for (int c = 0; c < kColourSlots; ++c) {
+    if (off[c + 1] <= off[c]) continue;
+    colours.push_back(c);
+    sizes.push_back(off[c + 1] - off[c]);
+    begins.push_back(off[c]);
+  }
+  last_ncolours = pinned[C_NCOLOURS];
+  enqueue_solve(vi, pi, colours, sizes, begins, 0, !(flags & B2G_STEP_SKIP_BROADPHASE));
+  hipLaunchKernelGGL(k_sleep_times, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_sleep_apply, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d, pi);
+  if (prof_on) HIP_TRY(hipEventRecord(ev[2], stream));
+  if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
+    if ((rc = broadphase(false))) return rc;
+  }
+  if (prof_on) HIP_TRY(hipEventRecord(ev[3], stream));
+  if (tev_used > 64) tcollect();
+  if ((rc = read_counters())) return rc;
+  if ((rc = check_status())) return rc;
+  if (prof_on) {
+    float t01, t12, t23, t03;
+    HIP_TRY(hipEventElapsedTime(&t01, ev[0], ev[1]));
+    HIP_TRY(hipEventElapsedTime(&t12, ev[1], ev[2]));
+    HIP_TRY(hipEventElapsedTime(&t23, ev[2], ev[3]));
+    HIP_TRY(hipEventElapsedTime(&t03, ev[0], ev[3]));
+    prof.collide = t01;
+    prof.solve = t12 + t23;
+    prof.broadphase = t23;
+    prof.step = t03;
+  }
+  return B2G_OK;
+}
+
 int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
+  if (d.sleep_on) return sleep_step(vi, pi, flags);
   if (use_cells()) return cells_step(vi, pi, flags);
   const bool prof_on = (flags & B2G_STEP_PROFILE) != 0;
   if (prof_on) HIP_TRY(hipEventRecord(ev[0], stream));
@@ -1505,7 +1615,7 @@ int b2g_set_world_params(b2g_batch* b, int32_t first, int32_t count, const b2g_w
   for (int i = 0; i < count; ++i) {
     b->h_params[first + i] = p[i];
     wp[i] = make_float4(p[i].gravity[0], p[i].gravity[1], p[i].dt, p[i].dt_ratio);
-    wf[i] = (p[i].warm_starting ? 1 : 0) | (p[i].active ? 2 : 0);
+    wf[i] = (p[i].warm_starting ? 1 : 0) | (p[i].active ? 2 : 0) | (p[i].allow_sleep ? 4 : 0);
   }
   // in stream order: every later step sees the new parameters
   HIP_TRY(hipMemcpyAsync(b->d.wpar + first, wp, count * sizeof(float4), hipMemcpyHostToDevice, b->stream));
@@ -1570,10 +1680,15 @@ int b2g_upload_bodies(b2g_batch* b, int32_t first, int32_t count, const b2g_body
   b->tree_stale = true;
   if (first + count > b->d.nb) b->layout_dirty = true;
   std::vector<float4> xf(count), pos(count), vel(count), prop(count), prop2(count), force(count);
-  std::vector<int> type(count), world(count);
+  std::vector<int> type(count), world(count), sflag(count);
+  std::vector<float> stime(count);
   for (int i = 0; i < count; ++i) {
     const b2g_body& s = in[i];
     if (s.world < 0 || s.world >= b->cap.worlds) return fail(B2G_ERR_INVALID, "body world index");
+    // a static body is never asleep (b2Body::SetAwake returns for it)
+    sflag[i] = s.flags & (s.type == B2G_STATIC ? B2G_BODY_NO_AUTOSLEEP : (B2G_BODY_ASLEEP | B2G_BODY_NO_AUTOSLEEP));
+    stime[i] = s.sleep_time;
+    if (sflag[i] & B2G_BODY_ASLEEP) b->asleep_uploaded = true;
     xf[i] = make_float4(s.xf[0], s.xf[1], s.xf[2], s.xf[3]);
     pos[i] = make_float4(s.c[0], s.c[1], s.a, 0.0f);
     vel[i] = make_float4(s.v[0], s.v[1], s.w, 0.0f);
@@ -1596,6 +1711,8 @@ int b2g_upload_bodies(b2g_batch* b, int32_t first, int32_t count, const b2g_body
   HIP_TRY(hipMemcpyAsync(d.force + first, force.data(), count * sizeof(float4), hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(d.btype + first, type.data(), count * sizeof(int), hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(d.bworld + first, world.data(), count * sizeof(int), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(d.bsflag + first, sflag.data(), count * sizeof(int), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(d.bstime + first, stime.data(), count * sizeof(float), hipMemcpyHostToDevice, st));
   HIP_TRY(hipStreamSynchronize(st));
   if (first + count > d.nb) {
     b->graphs_clear();
@@ -1849,6 +1966,10 @@ int b2g_download_bodies(b2g_batch* b, int32_t first, int32_t count, b2g_body* ou
   HIP_TRY(hipMemcpyAsync(prop.data(), d.prop + first, count * sizeof(float4), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(prop2.data(), d.prop2 + first, count * sizeof(float4), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(force.data(), d.force + first, count * sizeof(float4), hipMemcpyDeviceToHost, st));
+  std::vector<int> sflag(count);
+  std::vector<float> stime(count);
+  HIP_TRY(hipMemcpyAsync(sflag.data(), d.bsflag + first, count * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(stime.data(), d.bstime + first, count * sizeof(float), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   for (int i = 0; i < count; ++i) {
     b2g_body& o = out[i];
@@ -1863,6 +1984,8 @@ int b2g_download_bodies(b2g_batch* b, int32_t first, int32_t count, b2g_body* ou
     o.mass = prop2[i].x; o.gravity_scale = prop2[i].y;
     o.linear_damping = prop2[i].z; o.angular_damping = prop2[i].w;
     o.force[0] = force[i].x; o.force[1] = force[i].y; o.torque = force[i].z;
+    o.flags = sflag[i];
+    o.sleep_time = stime[i];
   }
   return B2G_OK;
 }

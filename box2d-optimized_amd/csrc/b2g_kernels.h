@@ -166,10 +166,24 @@ struct Dev {
                      // keys among the active shapes' keys
   int* scan_tiles;
   int* cnt;
+  // sleep (b2Body e_awakeFlag / m_sleepTime, b2_island.cpp:444-483)
+  int sleep_on;       // this step applies the awake flags (a world allows sleeping, or a body sleeps)
+  int* bsflag;        // per body: bit0 asleep, bit1 not allowed to sleep
+  float* bstime;      // per body: sleep time
+  int* bwake;         // per body: this step's Collide changed a touching state of it
+  int* bactive;       // per body: in an island with an awake body this step
+  int* bncontact;     // per body: contacts in the list during the solve (orphans have none)
+  int* bjointed;      // per body: a joint attaches it
+  int* isl_awake;     // per island root: the island holds an awake body
+  unsigned* isl_min;  // per island root: least sleep time of its bodies (float bits, >= 0)
+  int* isl_cons;      // per island root: the island has constraints this step
   // the cells solver of large worlds (b2g_partition.h)
   int part_on;        // this step solves by cells: contact_init leaves the colour tables alone
   float4* part_geo;   // 2: (lo.x, lo.y, 1/L, L), (cells x, cells y, all-cross) as int bits
   int* part_boff;     // 2 * kCells + 1: offsets of the cells' body lists (A cells, then B cells)
+  int* part_bcur;     // 2 * kCells: fill cursors of those lists
+  unsigned* part_acc; // 5: bounds of the dynamic bodies' centres (ordered floats) and their count
+  unsigned* part_sync;  // 4: finished-workgroup counters of the launches whose last workgroup goes on
   int* part_blist;    // 2 * nb: bodies of each cell
   int2* part_brank;   // per body: its slot in its A cell and in its B cell (-1: not dynamic)
   int* part_ccnt;     // kPartGroups: constraints per group (A cells, B cells, cross)
@@ -285,15 +299,28 @@ __device__ __forceinline__ void reset_colour_counters(const Dev& d) {
     d.cnt[C_SERIAL] = 0;
   }
 }
-__device__ __forceinline__ void integrate_velocity(const Dev& d, int b) {
+constexpr int kBodyAsleep = 1, kBodyNoAutoSleep = 2;
+// a body that counts as awake for Collide: not static, awake at the start of
+// the step (b2_contact_manager.cpp:100-106)
+__device__ __forceinline__ bool body_awake0(const Dev& d, int b) {
+  return d.btype[b] != kStaticBody && (d.bsflag[b] & kBodyAsleep) == 0;
+}
+__device__ __forceinline__ bool world_sleeps(const Dev& d, int w) { return (d.wflag[w] & 4) != 0; }
+
+// integrate = 1: the step's per-body resets and the velocity integration;
+// 0: the resets only (the integration follows the islands, which decide which
+// bodies move when sleeping applies)
+__device__ __forceinline__ void integrate_velocity(const Dev& d, int b, int integrate = 1) {
   {
     // a body of an idle world keeps its island label and colour mask (stale;
     // every reader of them checks the world first)
     const int w = d.bworld[b];
     if (!world_active(d, w)) return;
-    d.parent[b] = b;
-    d.bmask[b] = 0ull;
-    if (d.btype[b] != kDynamicBody) return;
+    if (integrate != 2) {
+      d.parent[b] = b;
+      d.bmask[b] = 0ull;
+    }
+    if (d.btype[b] != kDynamicBody || integrate == 0) return;
     const float4 wp = d.wpar[w];
     const float h = wp.z;
     const float4 v4 = d.vel[b], p = d.prop[b], p2 = d.prop2[b], f = d.force[b];
@@ -327,6 +354,20 @@ __global__ void __launch_bounds__(kBlock) k_narrowphase(Dev d, int integrate) {
     const int2 pr = d.cpair[i];
     const int4 ha = d.shdr[pr.x], hb = d.shdr[pr.y];
     int solve = 0;
+    if (world_active(d, ha.w) && d.sleep_on) {
+      // b2Body::GetContactCount for the orphan test, and a contact between
+      // two sleeping (or static) bodies keeps its manifold and touching state
+      atomicAdd(&d.bncontact[ha.y], 1);
+      atomicAdd(&d.bncontact[hb.y], 1);
+      if (!body_awake0(d, ha.y) && !body_awake0(d, hb.y)) {
+        float4* mp = d.cman + 4 * (size_t)i;
+        int* packed = reinterpret_cast<int*>(mp + 3) + 2;
+        const int w0 = *packed;
+        *packed = w0 & 0xffff;  // solved by no step since
+        d.touch[i] = ((w0 >> 8) & 0xff) > 0 ? 1 : 0;
+        continue;
+      }
+    }
     if (world_active(d, ha.w)) {
       Geo ga, gb;
       load_geo(ga, ha.x, ha.z, d.geo + 9 * (size_t)pr.x);
@@ -363,6 +404,12 @@ __global__ void __launch_bounds__(kBlock) k_narrowphase(Dev d, int integrate) {
       mp[3] = make_float4(uint_as_float(m.count > 0 ? m.id0 : 0u), uint_as_float(m.count > 1 ? m.id1 : 0u),
                           as_float((m.type & 0xff) | (m.count << 8)), 0.0f);
       solve = m.count > 0 ? 1 : 0;
+      // a touching transition wakes both bodies (b2_contact.cpp:183-188); the
+      // wake-up applies after Collide, when the islands are seeded
+      if (d.sleep_on && ((old_count > 0) != (m.count > 0))) {
+        d.bwake[ha.y] = 1;
+        d.bwake[hb.y] = 1;
+      }
       // BeginContact / EndContact on touching transitions (b2_contact.cpp:197-203)
       if (d.events_on && ((old_count > 0) != (m.count > 0))) {
         const float4 man[4] = {mp[0], mp[1], mp[2], mp[3]};
@@ -375,7 +422,7 @@ __global__ void __launch_bounds__(kBlock) k_narrowphase(Dev d, int integrate) {
     }
     d.touch[i] = solve;
   }
-  if (integrate) B2G_FOR(b, 0, d.nb) integrate_velocity(d, b);
+  if (integrate) B2G_FOR(b, 0, d.nb) integrate_velocity(d, b, d.sleep_on ? 0 : 1);
 }
 
 // compacted list of solved contacts (order preserved)
@@ -1003,9 +1050,14 @@ __global__ void __launch_bounds__(kBlock, 6) k_contact_init(Dev d) {
       const float vrel = dot(normal, vB + cross(wB, rB1) - (vA + cross(wA, rA1)));
       if (vrel < -mat.z) bias1 = -mat.y * vrel;
     }
-    int vcount = count;
+    int vcount = count, pcount = count;
+    if (d.sleep_on) {  // a sleeping island is not built (b2_world.cpp:535-539): no points to solve
+      const int isl = d.btype[ia] != kStaticBody ? d.parent[ia] : d.parent[ib];
+      if (d.isl_awake[isl]) d.isl_cons[isl] = 1;
+      else vcount = pcount = 0;
+    }
     mat22 K = {mk(0.0f, 0.0f), mk(0.0f, 0.0f)}, NM = K;
-    if (count == 2) {  // block solver preparation (:236-256)
+    if (count == 2 && vcount == 2) {  // block solver preparation (:236-256)
       const float rn1A = cross(rA0, normal), rn1B = cross(rB0, normal);
       const float rn2A = cross(rA1, normal), rn2B = cross(rB1, normal);
       const float k11 = mA + mB + iA * rn1A * rn1A + iB * rn1B * rn1B;
@@ -1044,7 +1096,7 @@ __global__ void __launch_bounds__(kBlock, 6) k_contact_init(Dev d) {
     d.pc[1][s] = make_float4(lp0.x, lp0.y, lp1.x, lp1.y);
     d.pc[2][s] = make_float4(lcA.x, lcA.y, lcB.x, lcB.y);
     d.pc[3][s] = make_float4(mA, mB, iA, iB);
-    d.pc[4][s] = make_float4(rA, rB, as_float(mtype), as_float(count));
+    d.pc[4][s] = make_float4(rA, rB, as_float(mtype), as_float(pcount));
   }
 }
 
@@ -1065,13 +1117,13 @@ struct LdsBodies {
   __device__ __forceinline__ bool writes(const Dev& d, int i) const { return writes_body(d, i); }
 };
 
-template <class BV>
-__device__ __forceinline__ void warm_start_t(const Dev& d, int s, BV vel) {
+template <class BV, class Src>
+__device__ __forceinline__ void warm_start_src(const Dev& d, BV vel, const Src& src) {
   {
-    const float4 v2i = d.vc[2][s];
+    const float4 v2i = src.vc(2);
     const int ia = as_int(v2i.x), ib = as_int(v2i.y), count = as_int(v2i.z) & 0xff;
     if ((as_int(v2i.z) & 256) == 0) return;  // no warm starting in this world
-    const float4 n4 = d.vc[0][s], m = d.vc[1][s];
+    const float4 n4 = src.vc(0), m = src.vc(1);
     const vec2 normal = mk(n4.x, n4.y), tangent = cross(normal, 1.0f);
     float4 va = *vel(ia), vb = *vel(ib);
     vec2 vA = mk(va.x, va.y), vB = mk(vb.x, vb.y);
@@ -1079,7 +1131,7 @@ __device__ __forceinline__ void warm_start_t(const Dev& d, int s, BV vel) {
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       if (j < count) {
-        const float4 r = d.vc[3 + 2 * j][s], im = d.vc[4 + 2 * j][s];
+        const float4 r = src.vc(3 + 2 * j), im = src.vc(4 + 2 * j);
         const vec2 P = im.x * normal + im.y * tangent;
         wA -= m.z * cross(mk(r.x, r.y), P);
         vA = vA - m.x * P;
@@ -1091,22 +1143,47 @@ __device__ __forceinline__ void warm_start_t(const Dev& d, int s, BV vel) {
     if (vel.writes(d, ib)) *vel(ib) = make_float4(vB.x, vB.y, wB, 0.0f);
   }
 }
-__device__ __forceinline__ void warm_start(const Dev& d, int s) { warm_start_t(d, s, GlobalBodies{d.vel}); }
 
 // b2ContactSolver::SolveVelocityConstraints (:301-639) for slots [begin, end)
 // Where a constraint solve reads its slot's fields: straight from the
 // constraint arrays (each load issued where the solve needs it), or from a
 // copy loaded ahead into registers (the tail launch loads the next colour's
 // constraints while the current colour is solved).
+// A source also says where the solved impulses go and whether the island's
+// position iterations are over (these two default to the constraint arrays
+// and the island state in global memory).
+__device__ __forceinline__ void put_impulses_global(const Dev& d, int s, float4 q0, float4 q1, int count) {
+  d.vc[4][s] = q0;
+  if (count > 1) d.vc[6][s] = q1;
+}
+__device__ __forceinline__ bool island_skip_global(const Dev& d, int isl, int it) {
+  if (island_done(d, isl, it)) return true;
+  island_rearm(d, isl, it);
+  return false;
+}
 struct SlotFields {
   const Dev& d;
   int s;
   __device__ __forceinline__ float4 vc(int k) const { return d.vc[k][s]; }
   __device__ __forceinline__ float4 pc(int k) const { return d.pc[k][s]; }
+  __device__ __forceinline__ void put_impulses(const Dev& dd, int ss, float4 q0, float4 q1, int n) const {
+    put_impulses_global(dd, ss, q0, q1, n);
+  }
+  __device__ __forceinline__ bool island_skip(const Dev& dd, int isl, int it) const {
+    return island_skip_global(dd, isl, it);
+  }
 };
+template <class BV>
+__device__ __forceinline__ void warm_start_t(const Dev& d, int s, BV vel) {
+  warm_start_src(d, vel, SlotFields{d, s});
+}
+__device__ __forceinline__ void warm_start(const Dev& d, int s) { warm_start_t(d, s, GlobalBodies{d.vel}); }
 struct VelocityRegs {
   float4 v[10];
   __device__ __forceinline__ float4 vc(int k) const { return v[k]; }
+  __device__ __forceinline__ void put_impulses(const Dev& d, int s, float4 q0, float4 q1, int n) const {
+    put_impulses_global(d, s, q0, q1, n);
+  }
 };
 __device__ __forceinline__ VelocityRegs load_velocity_regs(const Dev& d, int s) {
   VelocityRegs r;
@@ -1120,6 +1197,9 @@ struct PositionRegs {
   int isl;
   __device__ __forceinline__ float4 vc(int) const { return v2; }  // only field 2 is read
   __device__ __forceinline__ float4 pc(int k) const { return p[k]; }
+  __device__ __forceinline__ bool island_skip(const Dev& d, int is, int it) const {
+    return island_skip_global(d, is, it);
+  }
 };
 __device__ __forceinline__ PositionRegs load_position_regs(const Dev& d, int s) {
   PositionRegs r;
@@ -1134,6 +1214,7 @@ template <class BV, class Src>
 __device__ __forceinline__ void contact_velocity_src(const Dev& d, int s, BV vel, const Src& src) {
   const float4 v2i = src.vc(2);
   const int ia = as_int(v2i.x), ib = as_int(v2i.y), count = as_int(v2i.z) & 0xff;
+  if (count == 0) return;  // a constraint of a sleeping island
   const float4 n4 = src.vc(0), m = src.vc(1);
   const float mA = m.x, mB = m.y, iA = m.z, iB = m.w;
   const vec2 normal = mk(n4.x, n4.y);
@@ -1226,8 +1307,7 @@ __device__ __forceinline__ void contact_velocity_src(const Dev& d, int s, BV vel
       q1.x = x.y;
     }
   }
-  d.vc[4][s] = q0;
-  if (count > 1) d.vc[6][s] = q1;
+  src.put_impulses(d, s, q0, q1, count);
   if (vel.writes(d, ia)) *vel(ia) = make_float4(vA.x, vA.y, wA, 0.0f);
   if (vel.writes(d, ib)) *vel(ib) = make_float4(vB.x, vB.y, wB, 0.0f);
 }
@@ -1270,6 +1350,7 @@ __device__ __forceinline__ void store_impulses(const Dev& d, int s) {
     float4* mp = d.cman + 4 * (size_t)c;
     int* packed = reinterpret_cast<int*>(mp + 3) + 2;
     *packed = (*packed & 0xffff) | (count << 16);
+    if (count == 0) return;  // not solved: the manifold keeps its impulses
     float4 m0 = mp[0];
     m0.z = q0.x;
     m0.w = q0.y;
@@ -1292,6 +1373,7 @@ __device__ __forceinline__ void integrate_position(const Dev& d, int b) {
   {
     const int w = d.bworld[b];
     if (!world_active(d, w) || d.btype[b] == kStaticBody) return;
+    if (d.sleep_on && !d.bactive[b]) return;  // a sleeping island does not move
     const float h = d.wpar[w].z;
     const float4 v4 = d.vel[b];
     vec2 v = mk(v4.x, v4.y);
@@ -1339,8 +1421,8 @@ __device__ __forceinline__ float contact_position_src(const Dev& d, int isl, int
   const float4 q0 = src.pc(0), q1 = src.pc(1), q2 = src.pc(2), q3 = src.pc(3), q4 = src.pc(4);
   const int ia = as_int(v2i.x), ib = as_int(v2i.y);
   const float4 pa = *pos(ia), pb = *pos(ib);  // in flight with the island test too
-  if (island_done(d, isl, it)) return 0.0f;
-  island_rearm(d, isl, it);
+  if (as_int(q4.w) == 0) return 0.0f;  // a constraint of a sleeping island
+  if (src.island_skip(d, isl, it)) return 0.0f;
   const vec2 lp = mk(q0.x, q0.y), ln = mk(q0.z, q0.w);
   const vec2 lcA = mk(q2.x, q2.y), lcB = mk(q2.z, q2.w);
   const float mA = q3.x, mB = q3.y, iA = q3.z, iB = q3.w;
@@ -1611,7 +1693,12 @@ __global__ void __launch_bounds__(kBlock) k_joint_prepare(Dev d) {
   B2G_FOR(q, 0, d.nj) {
     const Joint& j = d.joints[d.jorder[q]];
     // solved or not; the island itself is filled in when the islands are assigned
-    const bool on = world_active(d, j.world) && (d.btype[j.a] != kStaticBody || d.btype[j.b] != kStaticBody);
+    bool on = world_active(d, j.world) && (d.btype[j.a] != kStaticBody || d.btype[j.b] != kStaticBody);
+    if (on && d.sleep_on) {  // a joint of a sleeping island is not solved
+      const int isl = island_of(d, j.a, j.b);
+      if (d.isl_awake[isl]) d.isl_cons[isl] = 1;
+      else on = false;
+    }
     d.jisland[q] = on ? 0 : -1;
   }
 }
@@ -1821,11 +1908,137 @@ __device__ __forceinline__ void finish_step(const Dev& d) {
   }
   B2G_FOR(b, 0, d.nb) {
     if (!world_active(d, d.bworld[b])) continue;
-    if (d.btype[b] != kStaticBody) d.xf[b] = synced_xf(d, b);
+    if (d.btype[b] != kStaticBody && (!d.sleep_on || d.bactive[b])) d.xf[b] = synced_xf(d, b);
     d.force[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
 }
 __global__ void __launch_bounds__(kBlock) k_finish_bodies(Dev d) { finish_step(d); }
+
+// ===========================================================================
+// Sleep (b2World::Solve's island seeds, b2_world.cpp:526-574; b2Island::Solve's
+// sleep timers, b2_island.cpp:444-483; SolveOrphan's, :233-250), applied when
+// a world allows sleeping or a body sleeps. The islands are the union-find
+// labels of the step (every island gets its smallest body as the label).
+// ===========================================================================
+constexpr float kTimeToSleep = 0.5f;                                // b2_common.h:176
+constexpr float kLinearSleepTolerance = 0.01f * 1.0f;               // b2_common.h:179
+constexpr float kAngularSleepTolerance = 2.0f / 180.0f * kPi;       // b2_common.h:182
+
+// before Collide: the step's per-body sleep records
+__global__ void __launch_bounds__(kBlock) k_sleep_reset(Dev d) {
+  B2G_FOR(b, 0, d.nb) {
+    d.bncontact[b] = 0;
+    d.bwake[b] = 0;
+    d.bactive[b] = 0;
+    d.isl_awake[b] = 0;
+    d.isl_min[b] = 0x7f7fffffu;  // FLT_MAX
+    d.isl_cons[b] = 0;
+  }
+}
+
+// after the islands: Collide's wake-ups (b2Body::SetAwake(true) resets the
+// timer), then every island with an awake body is seeded
+__global__ void __launch_bounds__(kBlock) k_sleep_seed(Dev d) {
+  B2G_FOR(b, 0, d.nb) {
+    if (!world_active(d, d.bworld[b]) || d.btype[b] == kStaticBody) continue;
+    if (d.bwake[b]) {
+      d.bsflag[b] &= ~kBodyAsleep;
+      d.bstime[b] = 0.0f;
+    }
+    if ((d.bsflag[b] & kBodyAsleep) == 0) d.isl_awake[d.parent[b]] = 1;
+  }
+}
+
+// a step without a solve (dt <= 0): Collide's wake-ups only
+__global__ void __launch_bounds__(kBlock) k_sleep_wake(Dev d) {
+  B2G_FOR(b, 0, d.nb) {
+    if (!world_active(d, d.bworld[b]) || d.btype[b] == kStaticBody || !d.bwake[b]) continue;
+    d.bsflag[b] &= ~kBodyAsleep;
+    d.bstime[b] = 0.0f;
+  }
+}
+
+// the bodies of seeded islands are solved this step: the search wakes them
+// (without a timer reset) and their velocities are integrated
+__global__ void __launch_bounds__(kBlock) k_sleep_activate(Dev d) {
+  B2G_FOR(b, 0, d.nb) {
+    if (!world_active(d, d.bworld[b]) || d.btype[b] == kStaticBody) continue;
+    if (!d.isl_awake[d.parent[b]]) continue;
+    d.bactive[b] = 1;
+    d.bsflag[b] &= ~kBodyAsleep;
+    integrate_velocity(d, b, 2);
+  }
+}
+
+// b2Body::SetAwake(false) (include/box2d/b2_body.h:719-740)
+__device__ __forceinline__ void put_to_sleep(const Dev& d, int b) {
+  d.bsflag[b] |= kBodyAsleep;
+  d.bstime[b] = 0.0f;
+  d.vel[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  d.force[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+__device__ __forceinline__ bool sleep_candidate(const Dev& d, int b) {
+  return world_active(d, d.bworld[b]) && world_sleeps(d, d.bworld[b]) && d.btype[b] != kStaticBody &&
+         d.bactive[b];
+}
+// an orphan has no contact and no joint (b2_world.cpp:545-549)
+__device__ __forceinline__ bool orphan_body(const Dev& d, int b) { return d.bncontact[b] == 0 && !d.bjointed[b]; }
+
+// after the position iterations: the sleep timers, and each island's least
+// timer. An orphan runs its own: when it runs out, the body's velocities and
+// forces are cleared, and the seed flags set right after SolveOrphan keep it
+// awake (b2_world.cpp:547-549).
+__global__ void __launch_bounds__(kBlock) k_sleep_times(Dev d) {
+  const float lin_tol_sq = kLinearSleepTolerance * kLinearSleepTolerance;
+  const float ang_tol_sq = kAngularSleepTolerance * kAngularSleepTolerance;
+  B2G_FOR(b, 0, d.nb) {
+    if (!sleep_candidate(d, b)) continue;
+    const float4 v = d.vel[b];
+    const bool slow = (d.bsflag[b] & kBodyNoAutoSleep) == 0 && !(v.z * v.z > ang_tol_sq) &&
+                      !(v.x * v.x + v.y * v.y > lin_tol_sq);
+    const float h = d.wpar[d.bworld[b]].z;
+    if (orphan_body(d, b)) {
+      if (!slow) {
+        d.bstime[b] = 0.0f;
+      } else {
+        const float t = d.bstime[b] + h;
+        d.bstime[b] = t;
+        if (t >= kTimeToSleep) {
+          put_to_sleep(d, b);
+          d.bsflag[b] &= ~kBodyAsleep;
+        }
+      }
+      continue;
+    }
+    const int r = d.parent[b];
+    if (!slow) {
+      d.bstime[b] = 0.0f;
+      atomicMin(&d.isl_min[r], 0u);
+    } else {
+      const float t = d.bstime[b] + h;
+      d.bstime[b] = t;
+      atomicMin(&d.isl_min[r], __float_as_uint(t));  // non-negative floats order as their bits
+    }
+  }
+}
+
+// an island whose bodies all slept long enough and whose position iterations
+// converged goes to sleep; an island without constraints converges at its
+// first position iteration
+__global__ void __launch_bounds__(kBlock) k_sleep_apply(Dev d, int pi) {
+  B2G_FOR(b, 0, d.nb) {
+    if (!sleep_candidate(d, b) || orphan_body(d, b)) continue;
+    const int r = d.parent[b];
+    bool solved;
+    if (!d.isl_cons[r]) {
+      solved = pi > 0;
+    } else {
+      const size_t last = island_slot(d, r, pi > 0 ? pi - 1 : 0);
+      solved = d.solved[r] != 0 || (pi > 0 && d.sepbad[last] == 0u && d.jok[last] != 0);
+    }
+    if (__uint_as_float(d.isl_min[r]) >= kTimeToSleep && solved) put_to_sleep(d, b);
+  }
+}
 
 // ClearForces alone (b2_world.cpp:1173-1180), for a step without a solve
 __global__ void __launch_bounds__(kBlock) k_clear_forces(Dev d) {
@@ -2910,6 +3123,8 @@ __global__ void __launch_bounds__(kBlock) k_merge_contacts(Dev d) {
 // EndContact for touching contacts the broadphase no longer reports
 // (b2ContactManager::Destroy, src/dynamics/b2_contact_manager.cpp:48-51):
 // old contacts whose key is missing from the new, sorted pair list.
+// The same contacts wake their bodies when sleeping applies (b2Contact::Destroy,
+// src/dynamics/b2_contact.cpp:79-91).
 __global__ void __launch_bounds__(kBlock) k_dead_contact_events(Dev d) {
   const int old_count = d.cnt[C_CONTACTS];
   const int total = d.cnt[C_PAIRS];
@@ -2925,8 +3140,19 @@ __global__ void __launch_bounds__(kBlock) k_dead_contact_events(Dev d) {
       if (d.nkey[mid] < key) lo = mid + 1; else hi = mid;
     }
     if (lo < n && d.nkey[lo] == key) continue;  // persists
-    const float4 man[4] = {mp[0], mp[1], mp[2], m3};
-    record_event(d, d.cpair[i], kEventDestroyed, man, d.cmat[i]);
+    const int2 pr = d.cpair[i];
+    if (d.events_on) {
+      const float4 man[4] = {mp[0], mp[1], mp[2], m3};
+      record_event(d, pr, kEventDestroyed, man, d.cmat[i]);
+    }
+    if (d.sleep_on) {
+      for (int side = 0; side < 2; ++side) {
+        const int b = d.shdr[side ? pr.y : pr.x].y;
+        if (d.btype[b] == kStaticBody) continue;
+        atomicAnd(&d.bsflag[b], ~kBodyAsleep);  // b2Body::SetAwake(true)
+        d.bstime[b] = 0.0f;
+      }
+    }
   }
 }
 

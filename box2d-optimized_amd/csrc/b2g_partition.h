@@ -26,23 +26,22 @@ namespace b2g {
 constexpr int kCells = 1024;              // cells per grid (A, and the shifted B)
 constexpr int kPartGroups = 2 * kCells + 1;  // A cells, B cells, the cross group C
 constexpr int kCrossGroup = 2 * kCells;
-constexpr int kCellBodies = 512;          // bodies of one cell (LDS); larger cells make the grid finer
+constexpr int kCellBodies = 512;          // bodies of one cell (LDS); a fuller cell's constraints go to C
 constexpr int kCellCons = 512;            // constraints of one cell; more go to C
 constexpr int kCellTarget = 48;           // bodies per cell the grid aims at
 constexpr int kPhaseStride = 1 << 16;     // reported colour: phase * kPhaseStride + colour
-constexpr int kPartGridThreads = 1024;
 constexpr int kCellThreads = 128;         // workgroup of a cell (colouring and sweeps)
-constexpr int kCrossThreads = 512;        // workgroup of the cross group
+constexpr int kCrossThreads = 512;        // workgroup of the cross group's sweeps
 
 // grid geometry: part_geo[0] = (lo.x, lo.y, 1/L, L), part_geo[1] = (Gx, Gy,
-// all-cross, -) with the ints as bits
+// -, -) with the ints as bits
 __device__ __forceinline__ int part_coord(float v, float lo, float inv_l, float shift, int n) {
   const float f = (v - lo) * inv_l + shift;
   return f >= 0.0f ? (int)fminf(f, (float)(n - 1)) : 0;  // NaN lands in 0
 }
 struct PartGrid {
   float lx, ly, inv_l;
-  int gx, gy, all_cross;
+  int gx, gy;
   __device__ __forceinline__ int cell_a(float4 p) const {
     return part_coord(p.x, lx, inv_l, 0.0f, gx) + gx * part_coord(p.y, ly, inv_l, 0.0f, gy);
   }
@@ -52,7 +51,7 @@ struct PartGrid {
 };
 __device__ __forceinline__ PartGrid part_grid(const Dev& d) {
   const float4 g0 = d.part_geo[0], g1 = d.part_geo[1];
-  return PartGrid{g0.x, g0.y, g0.z, as_int(g1.x), as_int(g1.y), as_int(g1.z)};
+  return PartGrid{g0.x, g0.y, g0.z, as_int(g1.x), as_int(g1.y)};
 }
 __device__ __forceinline__ bool part_dynamic(const Dev& d, int b) { return d.btype[b] == kDynamicBody; }
 
@@ -74,164 +73,212 @@ __device__ __forceinline__ int part_block_scan(int v, int* lds, int* total) {
   return before + inc - v;
 }
 
-// One workgroup: the grid for this step and every dynamic body's cells. The
-// bounds of the dynamic bodies' centres give a cell size for about
-// kCellTarget bodies per cell; it grows until no cell of either grid holds
-// more than kCellBodies bodies and both grids fit kCells cells. Then the
-// bodies are listed cell by cell (A cells, then B cells) with their slot in
-// each. Also clears the constraint counts of the step.
-__global__ void __launch_bounds__(kPartGridThreads) k_part_bodies(Dev d) {
-  __shared__ int hist[2 * kCells];
-  __shared__ float red[4][kPartGridThreads / kWave];
-  __shared__ int redn[kPartGridThreads / kWave];
-  __shared__ float s_geo[4];
-  __shared__ int s_int[4];
-  __shared__ int scan_lds[16];
-  const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
-  for (int g = tid; g < kPartGroups; g += nt) d.part_ccnt[g] = 0;
+// True in the workgroup of a launch that finishes last (every workgroup of
+// the launch must call it once, uniformly); the other workgroups' global
+// writes are visible to it. It puts the counter back to 0 for the next launch.
+// One lane fences: after the barrier the workgroup's stores have reached its
+// L2, and one agent-scope release writes that L2 back (a fence in every wave
+// would write it back once per wave, which costs more than a launch).
+__device__ __forceinline__ bool part_last_block(unsigned* counter) {
+  __shared__ int last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();  // this workgroup's writes, device-wide
+    const unsigned done = atomicAdd(counter, 1u);
+    last = done == gridDim.x - 1 ? 1 : 0;
+    if (last) {
+      atomicExch(counter, 0u);
+      __threadfence();  // and the others' writes before this workgroup reads them
+    }
+  }
+  __syncthreads();
+  return last != 0;
+}
+
+// The grid of a step, in three launches over the bodies and constraints:
+//   k_part_bounds    bounds of the dynamic bodies' centres (ordered-float
+//                    atomics, one set per workgroup), accumulated in
+//                    part_acc; its last workgroup then sets the cell size for
+//                    about kCellTarget bodies per cell, grown until both
+//                    grids fit kCells cells, clears the histograms and
+//                    constraint counts, and resets part_acc for the next step;
+//   k_part_hist      bodies per cell of either grid; its last workgroup
+//                    turns them into the offsets of the cells' body lists
+//                    (A cells, then B cells);
+//   k_part_classify  every dynamic body into its two cells' lists (its slot
+//                    in each), and every solved constraint into its group.
+// A cell of more than kCellBodies bodies (its LDS stage would not hold them)
+// hands its constraints to the cross group.
+__device__ __forceinline__ bool part_counts(const Dev& d, int b) {
+  return part_dynamic(d, b) && world_active(d, d.bworld[b]);
+}
+
+// Bounds: each lane over a grid stride, then the workgroup (shuffles, then
+// LDS across its waves), then one set of atomics per workgroup; the launch's
+// last workgroup goes on to the grid (part_grid_from_bounds). Launched with
+// at most kPartBoundsBlocks workgroups of kBlock threads.
+constexpr int kPartBoundsBlocks = 64;
+__device__ __forceinline__ void part_grid_from_bounds(const Dev& d);
+__global__ void __launch_bounds__(kBlock) k_part_bounds(Dev d) {
+  __shared__ float s_b[4][kBlock / kWave];
+  __shared__ int s_n[kBlock / kWave];
   float lx = INFINITY, ly = INFINITY, hx = -INFINITY, hy = -INFINITY;
-  int ndyn = 0;
-  for (int b = tid; b < d.nb; b += nt) {
-    if (!part_dynamic(d, b) || !world_active(d, d.bworld[b])) continue;
+  int n = 0;
+  B2G_FOR(b, 0, d.nb) {
+    if (!part_counts(d, b)) continue;
     const float4 p = d.pos[b];
-    if (!(fabsf(p.x) <= 1e30f) || !(fabsf(p.y) <= 1e30f)) continue;  // NaN or far out: an edge cell
-    lx = fminf(lx, p.x);
-    ly = fminf(ly, p.y);
-    hx = fmaxf(hx, p.x);
-    hy = fmaxf(hy, p.y);
-    ++ndyn;
+    if (fabsf(p.x) <= 1e30f && fabsf(p.y) <= 1e30f) {  // NaN or far out: an edge cell
+      lx = fminf(lx, p.x);
+      ly = fminf(ly, p.y);
+      hx = fmaxf(hx, p.x);
+      hy = fmaxf(hy, p.y);
+      ++n;
+    }
   }
   for (int off = kWave / 2; off > 0; off >>= 1) {
     lx = fminf(lx, __shfl_xor(lx, off, kWave));
     ly = fminf(ly, __shfl_xor(ly, off, kWave));
     hx = fmaxf(hx, __shfl_xor(hx, off, kWave));
     hy = fmaxf(hy, __shfl_xor(hy, off, kWave));
-    ndyn += __shfl_xor(ndyn, off, kWave);
+    n += __shfl_xor(n, off, kWave);
   }
   if (lane_id() == 0) {
-    red[0][wave_id()] = lx;
-    red[1][wave_id()] = ly;
-    red[2][wave_id()] = hx;
-    red[3][wave_id()] = hy;
-    redn[wave_id()] = ndyn;
+    s_b[0][wave_id()] = lx;
+    s_b[1][wave_id()] = ly;
+    s_b[2][wave_id()] = hx;
+    s_b[3][wave_id()] = hy;
+    s_n[wave_id()] = n;
   }
   __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)blockDim.x / kWave; ++w) {
+      lx = fminf(lx, s_b[0][w]);
+      ly = fminf(ly, s_b[1][w]);
+      hx = fmaxf(hx, s_b[2][w]);
+      hy = fmaxf(hy, s_b[3][w]);
+      n += s_n[w];
+    }
+    if (n > 0) {
+      atomicMin(&d.part_acc[0], float_to_ordered(lx));
+      atomicMin(&d.part_acc[1], float_to_ordered(ly));
+      atomicMax(&d.part_acc[2], float_to_ordered(hx));
+      atomicMax(&d.part_acc[3], float_to_ordered(hy));
+      atomicAdd(&d.part_acc[4], (unsigned)n);
+    }
+  }
+  if (part_last_block(d.part_sync + 0)) part_grid_from_bounds(d);
+}
+
+// the empty accumulator of k_part_bounds
+__device__ __forceinline__ void part_acc_reset(const Dev& d) {
+  d.part_acc[0] = 0xffffffffu;
+  d.part_acc[1] = 0xffffffffu;
+  d.part_acc[2] = 0u;
+  d.part_acc[3] = 0u;
+  d.part_acc[4] = 0u;
+}
+
+__device__ __forceinline__ void part_grid_from_bounds(const Dev& d) {
+  const int tid = (int)threadIdx.x;
   if (tid == 0) {
-    int n = 0;
-    for (int w = 0; w < nt / kWave; ++w) {
-      lx = fminf(lx, red[0][w]);
-      ly = fminf(ly, red[1][w]);
-      hx = fmaxf(hx, red[2][w]);
-      hy = fmaxf(hy, red[3][w]);
-      n += redn[w];
+    const unsigned n = d.part_acc[4];
+    float lx = 0.0f, ly = 0.0f, hx = 0.0f, hy = 0.0f;
+    if (n > 0) {
+      lx = ordered_to_float(d.part_acc[0]);
+      ly = ordered_to_float(d.part_acc[1]);
+      hx = ordered_to_float(d.part_acc[2]);
+      hy = ordered_to_float(d.part_acc[3]);
     }
-    if (n == 0) lx = ly = hx = hy = 0.0f;
     const float w = fminf(fmaxf(hx - lx, 1e-3f), 1e30f), h = fminf(fmaxf(hy - ly, 1e-3f), 1e30f);
-    float L = sqrtf(w * h * (float)kCellTarget / (float)(n > 0 ? n : 1));
-    L = fmaxf(L, 1e-3f);
-    s_geo[0] = lx;
-    s_geo[1] = ly;
-    s_geo[2] = w;
-    s_geo[3] = h;
-    s_int[0] = __float_as_int(L);
-  }
-  __syncthreads();
-  // grow the cell until both grids fit kCells cells and no cell is over full
-  int all_cross = 1;
-  for (int attempt = 0; attempt < 24; ++attempt) {
-    float L = __int_as_float(s_int[0]);
+    float L = fmaxf(sqrtf(w * h * (float)kCellTarget / (float)(n > 0 ? n : 1)), 1e-3f);
     // (in floats: the cell counts are converted once they are known to be small)
-    while ((s_geo[2] / L + 2.0f) * (s_geo[3] / L + 2.0f) > (float)kCells) L *= 1.25f;
-    const int gx = (int)(s_geo[2] / L) + 1, gy = (int)(s_geo[3] / L) + 1;
-    const PartGrid pg{s_geo[0], s_geo[1], 1.0f / L, gx, gy, 0};
-    for (int c = tid; c < 2 * kCells; c += nt) hist[c] = 0;
-    __syncthreads();
-    for (int b = tid; b < d.nb; b += nt) {
-      if (!part_dynamic(d, b) || !world_active(d, d.bworld[b])) continue;
-      const float4 p = d.pos[b];
-      atomicAdd(&hist[pg.cell_a(p)], 1);
-      atomicAdd(&hist[kCells + pg.cell_b(p)], 1);
-    }
-    __syncthreads();
-    int over = 0;
-    for (int c = tid; c < 2 * kCells; c += nt) over |= hist[c] > kCellBodies;
-    over = __syncthreads_or(over);
-    if (!over) {
-      all_cross = 0;
-      if (tid == 0) {
-        d.part_geo[0] = make_float4(pg.lx, pg.ly, pg.inv_l, L);
-        d.part_geo[1] = make_float4(as_float(gx), as_float(gy), as_float(0), 0.0f);
-      }
-      break;
-    }
-    if (tid == 0) s_int[0] = __float_as_int(L * 1.25f);
-    __syncthreads();
+    while ((w / L + 2.0f) * (h / L + 2.0f) > (float)kCells) L *= 1.25f;
+    const int gx = (int)(w / L) + 1, gy = (int)(h / L) + 1;
+    d.part_geo[0] = make_float4(lx, ly, 1.0f / L, L);
+    d.part_geo[1] = make_float4(as_float(gx), as_float(gy), as_float(0), 0.0f);
+    part_acc_reset(d);
   }
-  if (all_cross) {  // no grid keeps the cells small enough: every constraint goes to C
-    if (tid == 0) {
-      d.part_geo[0] = make_float4(0.0f, 0.0f, 1.0f, 1.0f);
-      d.part_geo[1] = make_float4(as_float(1), as_float(1), as_float(1), 0.0f);
-    }
-    for (int c = tid; c < 2 * kCells; c += nt) hist[c] = 0;
-    __syncthreads();
-  }
-  // offsets of the cells' body lists (A cells, then B cells)
-  constexpr int kPer = 2 * kCells / kPartGridThreads;
-  static_assert(kPer * kPartGridThreads == 2 * kCells, "two cells per thread");
+  for (int c = tid; c < 2 * kCells; c += blockDim.x) d.part_boff[c] = 0;  // the histogram, scanned in place
+  for (int g = tid; g < kPartGroups; g += blockDim.x) d.part_ccnt[g] = 0;
+}
+
+// exclusive scan of the 2 * kCells body counts in place; the fill cursors
+// start at the offsets (one workgroup of kBlock threads)
+__device__ __forceinline__ void part_scan_counts(const Dev& d) {
+  __shared__ int scan_lds[16];
+  constexpr int kPer = 2 * kCells / kBlock;
+  static_assert(kPer * kBlock == 2 * kCells, "whole cells per thread");
+  const int tid = (int)threadIdx.x;
   int v[kPer], s = 0;
   for (int k = 0; k < kPer; ++k) {
-    v[k] = hist[tid * kPer + k];
+    v[k] = d.part_boff[tid * kPer + k];
     s += v[k];
   }
   int total;
   int run = part_block_scan(s, scan_lds, &total);
   for (int k = 0; k < kPer; ++k) {
     d.part_boff[tid * kPer + k] = run;
-    hist[tid * kPer + k] = run;  // becomes the fill cursor
+    d.part_bcur[tid * kPer + k] = run;
     run += v[k];
   }
   if (tid == 0) d.part_boff[2 * kCells] = total;
-  __syncthreads();
-  if (all_cross) return;
-  const PartGrid pg = part_grid(d);  // written by thread 0 above, visible after the barrier
-  for (int b = tid; b < d.nb; b += nt) {
-    if (!part_dynamic(d, b) || !world_active(d, d.bworld[b])) {
-      d.part_brank[b] = make_int2(-1, -1);
-      continue;
-    }
-    const float4 p = d.pos[b];
-    const int ca = pg.cell_a(p), cb = kCells + pg.cell_b(p);
-    const int ra = atomicAdd(&hist[ca], 1), rb = atomicAdd(&hist[cb], 1);
-    d.part_blist[ra] = b;
-    d.part_blist[rb] = b;
-    d.part_brank[b] = make_int2(ra - d.part_boff[ca], rb - d.part_boff[cb]);
-  }
 }
 
-// Group of each solved constraint (compacted slot k), into its group's bucket.
+// bodies per cell of either grid; the launch's last workgroup scans them
+// (launched with at most kPartBoundsBlocks workgroups)
+__global__ void __launch_bounds__(kBlock) k_part_hist(Dev d) {
+  const PartGrid pg = part_grid(d);
+  B2G_FOR(b, 0, d.nb) {
+    if (!part_counts(d, b)) continue;
+    const float4 p = d.pos[b];
+    atomicAdd(&d.part_boff[pg.cell_a(p)], 1);
+    atomicAdd(&d.part_boff[kCells + pg.cell_b(p)], 1);
+  }
+  if (part_last_block(d.part_sync + 1)) part_scan_counts(d);
+}
+
+__device__ __forceinline__ bool part_cell_fits(const Dev& d, int cell) {
+  return d.part_boff[cell + 1] - d.part_boff[cell] <= kCellBodies;
+}
+
+// Bodies into their cells' lists; solved constraints (compacted slot k) into
+// their groups' buckets.
 __global__ void __launch_bounds__(kBlock) k_part_classify(Dev d) {
   const int n = d.cnt[C_CONS];
   const PartGrid pg = part_grid(d);
-  B2G_FOR(k, 0, n) {
+  const int m = max(n, d.nb);
+  B2G_FOR(k, 0, m) {
+    if (k < d.nb) {
+      const int b = k;
+      if (!part_counts(d, b)) {
+        d.part_brank[b] = make_int2(-1, -1);
+      } else {
+        const float4 p = d.pos[b];
+        const int ca = pg.cell_a(p), cb = kCells + pg.cell_b(p);
+        const int ra = atomicAdd(&d.part_bcur[ca], 1), rb = atomicAdd(&d.part_bcur[cb], 1);
+        d.part_blist[ra] = b;
+        d.part_blist[rb] = b;
+        d.part_brank[b] = make_int2(ra - d.part_boff[ca], rb - d.part_boff[cb]);
+      }
+    }
+    if (k >= n) continue;
     const int2 bp = d.cbody[k];
     const int ia = bp.x >= 0 ? bp.x : ~bp.x, ib = bp.y >= 0 ? bp.y : ~bp.y;
     const bool da = part_dynamic(d, ia), db = part_dynamic(d, ib);
     int g = kCrossGroup;
-    if (!pg.all_cross) {
-      if (da && db) {
-        const float4 pa = d.pos[ia], pb = d.pos[ib];
-        const int a0 = pg.cell_a(pa), a1 = pg.cell_a(pb);
-        if (a0 == a1) {
-          g = a0;
-        } else {
-          const int b0 = pg.cell_b(pa), b1 = pg.cell_b(pb);
-          if (b0 == b1) g = kCells + b0;
-        }
-      } else if (da || db) {
-        g = pg.cell_a(d.pos[da ? ia : ib]);
+    if (da && db) {
+      const float4 pa = d.pos[ia], pb = d.pos[ib];
+      const int a0 = pg.cell_a(pa), a1 = pg.cell_a(pb);
+      if (a0 == a1) {
+        g = a0;
+      } else {
+        const int b0 = pg.cell_b(pa), b1 = pg.cell_b(pb);
+        if (b0 == b1) g = kCells + b0;
       }
+    } else if (da || db) {
+      g = pg.cell_a(d.pos[da ? ia : ib]);
     }
-    if (g != kCrossGroup) {
+    if (g != kCrossGroup && part_cell_fits(d, g)) {
       const int r = atomicAdd(&d.part_ccnt[g], 1);
       if (r < kCellCons) {
         d.part_bucket[(size_t)g * kCellCons + r] = k;
@@ -398,8 +445,14 @@ __device__ __forceinline__ int part_group_start(const Dev& d, int g, int* scan_l
   return total;
 }
 
-// one workgroup per cell of either grid
+__device__ __forceinline__ void part_colour_cross(const Dev& d);
+
+// one workgroup per cell of either grid, then one for the cross group
 __global__ void __launch_bounds__(kCellThreads) k_part_colour_cells(Dev d) {
+  if ((int)blockIdx.x == 2 * kCells) {
+    part_colour_cross(d);
+    return;
+  }
   __shared__ int s_k[kCellCons];
   __shared__ int2 s_l[kCellCons];
   __shared__ int s_col[kCellCons];
@@ -425,7 +478,7 @@ __global__ void __launch_bounds__(kCellThreads) k_part_colour_cells(Dev d) {
 }
 
 // the cross group, one workgroup; its per-constraint colours in global memory
-__global__ void __launch_bounds__(kCrossThreads) k_part_colour_cross(Dev d) {
+__device__ __forceinline__ void part_colour_cross(const Dev& d) {
   __shared__ int hist[kColourSlots + 1];
   __shared__ int cursor[kColourSlots];
   __shared__ int scan_lds[16];
@@ -436,8 +489,10 @@ __global__ void __launch_bounds__(kCrossThreads) k_part_colour_cross(Dev d) {
                     hist, cursor);
 }
 
-// Body access of a cell's sweep: the constraint's dynamic bodies in the
-// cell's LDS stage, a static or kinematic one (read only) in global memory.
+// Body access of a cell's sweep: LDS slots [0, kCellBodies) hold the cell's
+// dynamic bodies (written back at the end), the slots after them a copy of
+// each staged constraint's static or kinematic body (read only); a body
+// without a slot (loc < 0) is read and written in global memory.
 struct CellBodies {
   float4* lds;
   float4* glob;
@@ -447,72 +502,187 @@ struct CellBodies {
     const int l = i == ia ? loc.x : loc.y;
     return l >= 0 ? lds + l : glob + i;
   }
-  __device__ __forceinline__ bool writes(const Dev& d, int i) const { return writes_body(d, i); }
+  __device__ __forceinline__ bool writes(const Dev& d, int i) const {
+    const int l = i == ia ? loc.x : loc.y;
+    return l >= 0 ? l < kCellBodies : writes_body(d, i);
+  }
 };
 
 enum PartOp : int { kPartWarm = 0, kPartVelocity = 1, kPartPosition = 2 };
 
-// One phase of a sweep: every cell of the phase's grid (a workgroup each; a
-// workgroup past the grid's cells has nothing to do), or the cross group.
-template <int kOp, int kThreads>
-__global__ void __launch_bounds__(kThreads) k_part_sweep(Dev d, int phase, int it) {
-  __shared__ int off[kColourSlots + 1];
-  __shared__ float4 s_body[kCellBodies];
-  const int g = phase == 2 ? kCrossGroup : phase * kCells + (int)blockIdx.x;
+// A group's constraint records staged in LDS for one sweep: the group's slots
+// are contiguous, so one coalesced pass loads them all at the start and every
+// colour round then reads LDS. Warm start and velocity: the ten velocity
+// words; position: word 2 of the velocity record, the five position words
+// and the island. Slots past kStageCons are read from global memory.
+constexpr int kStageCons = 256;
+template <int kOp>
+struct PartStage {
+  static constexpr int kWords = kOp == kPartPosition ? 6 : 10;
+};
+// A staged constraint's solved impulses stay in LDS until the sweep's end, and
+// its island's position test is made once at the start of the sweep (it only
+// reads the previous iteration's marks): the colour rounds then touch no
+// global memory, so their barriers wait for LDS alone.
+struct StagedVelocity {  // velocity word k of staged constraint i
+  float4* base;
+  int i;
+  __device__ __forceinline__ float4 vc(int k) const { return base[k * kStageCons + i]; }
+  __device__ __forceinline__ void put_impulses(const Dev&, int, float4 q0, float4 q1, int count) const {
+    base[4 * kStageCons + i] = q0;
+    if (count > 1) base[6 * kStageCons + i] = q1;
+  }
+};
+struct StagedPosition {  // velocity word 2, then position words 0..4
+  const float4* base;
+  const int* skip;
+  int i;
+  __device__ __forceinline__ float4 vc(int) const { return base[i]; }
+  __device__ __forceinline__ float4 pc(int k) const { return base[(1 + k) * kStageCons + i]; }
+  __device__ __forceinline__ bool island_skip(const Dev&, int, int) const { return skip[i] != 0; }
+};
+
+// LDS of one group's sweep
+template <int kOp>
+struct PartLds {
+  int off[kColourSlots + 1];
+  float4 body[kCellBodies + 2 * kStageCons];
+  float4 con[PartStage<kOp>::kWords * kStageCons];
+  int isl[kOp == kPartPosition ? kStageCons : 1];
+  int skip[kOp == kPartPosition ? kStageCons : 1];  // the island's iterations are over
+  float sep[kOp == kPartPosition ? kStageCons : 1];  // minimum separation of the solve
+  int2 loc[kStageCons];
+};
+
+// One group's share of a sweep: a cell (its bodies and constraints staged in
+// LDS) or the cross group (global memory).
+template <int kOp>
+__device__ __forceinline__ void part_sweep_group(const Dev& d, int g, int it, PartLds<kOp>& L) {
   const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
-  const int* coff = d.part_coff + (size_t)g * (kColourSlots + 1);
-  for (int c = tid; c <= kColourSlots; c += nt) off[c] = coff[c];
-  __syncthreads();
-  if (off[kColourSlots] <= off[0]) return;  // the whole workgroup
+  const bool staged = g != kCrossGroup;
   float4* body = kOp == kPartPosition ? d.pos : d.vel;
-  const bool staged = phase < 2;
+  const int* coff = d.part_coff + (size_t)g * (kColourSlots + 1);
   int b0 = 0, b1 = 0;
-  if (staged) {
+  if (staged) {  // (issued together with the colour offsets)
     b0 = d.part_boff[g];
     b1 = d.part_boff[g + 1];
-    for (int r = tid; r < b1 - b0; r += nt) s_body[r] = body[d.part_blist[b0 + r]];
-    __syncthreads();
   }
-  auto solve = [&](int s, bool valid) {
-    const int2 loc = valid ? (staged ? d.sloc[s] : make_int2(-1, -1)) : make_int2(-1, -1);
-    const int ia = valid ? as_int(d.vc[2][s].x) : 0;
-    const CellBodies bv{s_body, body, ia, loc};
-    if (kOp == kPartWarm) {
-      if (valid) warm_start_t(d, s, bv);
-    } else if (kOp == kPartVelocity) {
-      if (valid) contact_velocity_t(d, s, bv);
+  for (int c = tid; c <= kColourSlots; c += nt) L.off[c] = coff[c];
+  __syncthreads();
+  const int s0 = L.off[0];
+  if (L.off[kColourSlots] <= s0) return;  // the whole workgroup
+  const int staged_n = min(L.off[kColourSlots] - s0, kStageCons);
+  for (int r = tid; r < b1 - b0; r += nt) L.body[r] = body[d.part_blist[b0 + r]];
+  for (int i = tid; i < staged_n; i += nt) {
+    float4 v2;
+    if (kOp == kPartPosition) {
+      v2 = d.vc[2][s0 + i];
+      L.con[i] = v2;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) L.con[(1 + k) * kStageCons + i] = d.pc[k][s0 + i];
+      const int isl = d.cisland[s0 + i];
+      L.isl[i] = isl;
+      L.skip[i] = as_int(L.con[5 * kStageCons + i].w) != 0 && island_skip_global(d, isl, it) ? 1 : 0;
+      L.sep[i] = 0.0f;
     } else {
-      const int isl = valid ? d.cisland[s] : 0;
-      const float m = valid ? contact_position_t(d, s, isl, it, bv) : 0.0f;
-      mark_island_sep(d, it, isl, valid, m);
+#pragma unroll
+      for (int k = 0; k < 10; ++k) L.con[k * kStageCons + i] = d.vc[k][s0 + i];
+      v2 = L.con[2 * kStageCons + i];
     }
+    int2 loc = make_int2(-1, -1);
+    if (staged) {  // a cell's constraint: its static or kinematic body gets a read-only slot
+      loc = d.sloc[s0 + i];
+      if (loc.x < 0) {
+        loc.x = kCellBodies + 2 * i;
+        L.body[loc.x] = body[as_int(v2.x)];
+      }
+      if (loc.y < 0) {
+        loc.y = kCellBodies + 2 * i + 1;
+        L.body[loc.y] = body[as_int(v2.y)];
+      }
+    }
+    L.loc[i] = loc;
+  }
+  __syncthreads();
+  // slot s (valid) with its bodies in the cell's stage or in global memory
+  auto solve = [&](int s, bool valid) {
+    const int i = s - s0;
+    const bool in_lds = valid && i < staged_n;
+    const int2 loc = !valid || !staged ? make_int2(-1, -1) : in_lds ? L.loc[i] : d.sloc[s];
+    const float4 v2 = !valid ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : in_lds ? L.con[(kOp == kPartPosition ? 0 : 2) * kStageCons + i] : d.vc[2][s];
+    const CellBodies bv{L.body, body, as_int(v2.x), loc};
+    if (kOp == kPartWarm) {
+      if (in_lds)
+        warm_start_src(d, bv, StagedVelocity{L.con, i});
+      else if (valid)
+        warm_start_src(d, bv, SlotFields{d, s});
+    } else if (kOp == kPartVelocity) {
+      if (in_lds)
+        contact_velocity_src(d, s, bv, StagedVelocity{L.con, i});
+      else if (valid)
+        contact_velocity_src(d, s, bv, SlotFields{d, s});
+    } else {
+      const int isl = !valid ? 0 : in_lds ? L.isl[i] : d.cisland[s];
+      float m = 0.0f;
+      if (in_lds)
+        m = contact_position_src(d, isl, it, bv, StagedPosition{L.con, L.skip, i});
+      else if (valid)
+        m = contact_position_src(d, isl, it, bv, SlotFields{d, s});
+      return m;
+    }
+    return 0.0f;
   };
   for (int c = 0; c < kColourSlots; ++c) {
-    const int lo = off[c], hi = off[c + 1];
+    const int lo = L.off[c], hi = L.off[c + 1];
     if (hi <= lo) continue;  // uniform across the workgroup
     if (c == kSerialColour) {
       if (tid == 0) {
         for (int s = lo; s < hi; ++s) {
-          const int2 loc = staged ? d.sloc[s] : make_int2(-1, -1);
-          const CellBodies bv{s_body, body, as_int(d.vc[2][s].x), loc};
-          if (kOp == kPartWarm) {
-            warm_start_t(d, s, bv);
-          } else if (kOp == kPartVelocity) {
-            contact_velocity_t(d, s, bv);
-          } else {
-            const int isl = d.cisland[s];
-            const float m = contact_position_t(d, s, isl, it, bv);
-            if (separation_bad(m)) d.sepbad[island_slot(d, isl, it)] = 1u;
+          const float m = solve(s, true);
+          if (kOp == kPartPosition && separation_bad(m)) {
+            const int isl = s - s0 < staged_n ? L.isl[s - s0] : d.cisland[s];
+            d.sepbad[island_slot(d, isl, it)] = 1u;
           }
         }
       }
     } else {
-      for (int base = lo; base < hi; base += nt) solve(base + tid, base + tid < hi);
+      for (int base = lo; base < hi; base += nt) {  // every lane of a wave takes part in the island marks
+        const int s = base + tid;
+        const bool valid = s < hi;
+        const float m = solve(s, valid);
+        if (kOp == kPartPosition) {
+          const bool in_lds = valid && s - s0 < staged_n;
+          if (in_lds) L.sep[s - s0] = m;  // marked at the end of the sweep
+          mark_island_sep(d, it, in_lds || !valid ? 0 : d.cisland[s], valid && !in_lds, m);
+        }
+      }
     }
     __syncthreads();
   }
-  if (staged)
-    for (int r = tid; r < b1 - b0; r += nt) body[d.part_blist[b0 + r]] = s_body[r];
+  if (kOp == kPartVelocity) {  // the staged constraints' impulses
+    for (int i = tid; i < staged_n; i += nt) {
+      d.vc[4][s0 + i] = L.con[4 * kStageCons + i];
+      if ((as_int(L.con[2 * kStageCons + i].z) & 0xff) > 1) d.vc[6][s0 + i] = L.con[6 * kStageCons + i];
+    }
+  }
+  if (kOp == kPartPosition) {  // the staged constraints' island marks
+    for (int base = 0; base < staged_n; base += nt) {
+      const int i = base + tid;
+      const bool valid = i < staged_n;
+      mark_island_sep(d, it, valid ? L.isl[i] : 0, valid, valid ? L.sep[i] : 0.0f);
+    }
+  }
+  for (int r = tid; r < b1 - b0; r += nt) body[d.part_blist[b0 + r]] = L.body[r];
+}
+
+// One phase of a sweep: the cells of grid A (phase 0) or of grid B (phase
+// 1), a workgroup each (one past the grid's cells has nothing to do), or the
+// cross group (phase 2, one workgroup), whose constraints touch bodies of
+// both grids' cells.
+template <int kOp, int kThreads>
+__global__ void __launch_bounds__(kThreads) k_part_sweep(Dev d, int phase, int it) {
+  __shared__ PartLds<kOp> L;
+  part_sweep_group<kOp>(d, phase == 2 ? kCrossGroup : phase * kCells + (int)blockIdx.x, it, L);
 }
 
 }  // namespace b2g

@@ -25,7 +25,9 @@ BODY_DTYPE = np.dtype([
     ("local_center", "<f4", 2), ("v", "<f4", 2), ("w", "<f4"), ("mass", "<f4"),
     ("inv_mass", "<f4"), ("inv_i", "<f4"), ("linear_damping", "<f4"),
     ("angular_damping", "<f4"), ("gravity_scale", "<f4"), ("force", "<f4", 2), ("torque", "<f4"),
+    ("flags", "<i4"), ("sleep_time", "<f4"),
 ])
+BODY_ASLEEP, BODY_NO_AUTOSLEEP = 1, 2
 SHAPE_DTYPE = np.dtype([
     ("type", "<i4"), ("body", "<i4"), ("count", "<i4"), ("radius", "<f4"),
     ("centroid", "<f4", 2), ("vertices", "<f4", 16), ("normals", "<f4", 16),
@@ -55,7 +57,7 @@ class Capacity(ctypes.Structure):
 
 class WorldParams(ctypes.Structure):
     _fields_ = [("gravity", ctypes.c_float * 2), ("dt", ctypes.c_float), ("dt_ratio", ctypes.c_float),
-                ("warm_starting", ctypes.c_int32), ("active", ctypes.c_int32)]
+                ("warm_starting", ctypes.c_int32), ("active", ctypes.c_int32), ("allow_sleep", ctypes.c_int32)]
 
 
 class Counts(ctypes.Structure):
@@ -87,7 +89,7 @@ EXPORTS = [
     "b2g_set_option", "b2g_get_option", "b2g_upload_contact_colours", "b2g_snapshot_save",
     "b2g_snapshot_load", "b2g_download_events", "b2g_query_aabbs", "b2g_raycast",
     "b2g_set_counts", "b2g_download_solver_points", "b2a_world_destroy_body", "b2a_world_destroy_joint",
-    "b2a_world_destroy_fixture", "b2a_world_set_filter", "b2a_world_body_count",
+    "b2a_world_destroy_fixture", "b2a_world_set_filter", "b2a_world_body_count", "b2a_world_set_allow_sleep",
 ]
 SOLVER_AUTO, SOLVER_COLOURS, SOLVER_WORLDS, SOLVER_CELLS = 0, 1, 2, 3
 OPT_SOLVER, OPT_WORLD_BLOCK, OPT_TUNED_WORLD_BLOCK, OPT_LAST_SOLVER, OPT_PER_WORLD_ELIGIBLE = 1, 2, 3, 4, 5
@@ -172,6 +174,7 @@ def lib():
             getattr(L, f).argtypes = [_P, _I]
         L.b2a_world_set_filter.argtypes = [_P, _I, _I, _I, _I]
         L.b2a_world_body_count.argtypes = [_P, _P]
+        L.b2a_world_set_allow_sleep.argtypes = [_P, _I]
         _lib = L
     return _lib
 
@@ -250,18 +253,21 @@ class Batch:
         except Exception:
             pass
 
-    def advance(self, steps=1, dt=1.0 / 60.0, vi=8, pi=3, flags=0, warm=None):
+    def advance(self, steps=1, dt=1.0 / 60.0, vi=8, pi=3, flags=0, warm=None, sleep=None):
         """Step every world `steps` times the way b2World::Step sequences
         them (b2_world.cpp:1108-1171): the first step of a fresh batch finds
         contacts first and has dtRatio 0; later steps use inv_dt0 * dt.
-        `warm`: per-world b2World::SetWarmStarting flags (default all on)."""
+        `warm`: per-world b2World::SetWarmStarting flags (default all on);
+        `sleep`: per-world b2World::SetAllowSleeping flags (default all off)."""
         dt = np.float32(dt)
         nw = len(self.gravity)
         warm = [1] * nw if warm is None else [1 if w else 0 for w in warm]
+        sleep = [0] * nw if sleep is None else [1 if z else 0 for z in sleep]
         for _ in range(steps):
             inv_dt0 = getattr(self, "inv_dt0", np.float32(0.0))
             ratio = np.float32(inv_dt0 * dt)
-            self.set_world_params(0, [WorldParams(g, dt, ratio, warm[k], 1) for k, g in enumerate(self.gravity)])
+            self.set_world_params(0, [WorldParams(g, dt, ratio, warm[k], 1, sleep[k])
+                                      for k, g in enumerate(self.gravity)])
             first = not getattr(self, "stepped", False)
             self.step(vi, pi, flags | (STEP_FIND_NEW_CONTACTS if first else 0))
             self.stepped = True
@@ -500,6 +506,10 @@ class World:
 
     def set_filter(self, fixture, category, mask, group):
         check(self.L.b2a_world_set_filter(self.h, fixture, category, mask, group))
+
+    def set_allow_sleep(self, flag):
+        """b2World::SetAllowSleeping."""
+        check(self.L.b2a_world_set_allow_sleep(self.h, int(flag)))
 
     def _recount(self):
         n = ctypes.c_int32(0)

@@ -1,9 +1,9 @@
 // unsupported_features.cpp -- user code that turns on, one at a time, what the
-// device path does not run (two of them are b2World's defaults) and checks
+// device path does not run (continuous physics is b2World's default) and checks
 // that b2World::Step refuses each one loudly: status B2G_ERR_UNSUPPORTED, a
 // message, and a world left as it was. Features the device does serve (a
-// PostSolve listener, a contact filter that keeps the default ShouldCollide)
-// are not refused. Once everything is off the world steps.
+// PostSolve listener, a contact filter that keeps the default ShouldCollide,
+// sleeping and bodies put to sleep) are not refused. Once everything is off the world steps.
 //
 // Prints one line per case ("<case>: <status> <message>") and exits 0 when
 // every case behaved so, the last step included (it ran on a device, or
@@ -64,14 +64,14 @@ int main() {
   shape.SetAsBox(0.5f, 0.5f);
   b2Fixture* fixture = box->CreateFixture(&shape, 1.0f);
 
-  expect_refused(world, box, "default world (sleeping and continuous physics on)");
+  expect_refused(world, box, "default world (continuous physics on)");
   world.SetContinuousPhysics(false);
-  expect_refused(world, box, "sleeping on");
-  world.SetAllowSleeping(false);
-
+  expect_stepped(world, "sleeping on");
   box->SetAwake(false);
-  expect_refused(world, box, "a body put to sleep");
+  expect_stepped(world, "a body put to sleep");
   box->SetAwake(true);
+  world.SetAllowSleeping(false);
+  expect_stepped(world, "sleeping off");
   box->SetEnabled(false);
   expect_refused(world, box, "a disabled body");
   box->SetEnabled(true);

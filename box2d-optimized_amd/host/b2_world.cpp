@@ -400,7 +400,6 @@ void b2Body::SetType(b2BodyType type) {
       w->m_bodyListHead = this;
     }
   }
-  if (!m_awake && m_type != b2_staticBody) --w->m_sleepingBodies;
   m_type = type;
   ResetMassData();
   if (m_type == b2_staticBody) {
@@ -409,6 +408,7 @@ void b2Body::SetType(b2BodyType type) {
     m_awake = false;
   } else {
     m_awake = true;  // SetAwake(true)
+    m_sleepTime = 0.0f;
   }
   m_force.SetZero();
   m_torque = 0.0f;
@@ -483,15 +483,14 @@ void b2Body::SetMassData(const b2MassData* massData) {
   MarkDirty();
 }
 
-// include/box2d/b2_body.h:719-740
+// include/box2d/b2_body.h:719-740 (either way the sleep timer restarts)
 void b2Body::SetAwake(bool flag) {
   if (m_type == b2_staticBody) return;
   Sync();
+  m_sleepTime = 0.0f;
   if (flag) {
-    if (!m_awake) --m_world->m_sleepingBodies;
     m_awake = true;
   } else {
-    if (m_awake) ++m_world->m_sleepingBodies;
     m_awake = false;
     m_linearVelocity.SetZero();
     m_angularVelocity = 0.0f;
@@ -499,6 +498,19 @@ void b2Body::SetAwake(bool flag) {
     m_torque = 0.0f;
   }
   MarkDirty();
+}
+
+// include/box2d/b2_body.h:747-758
+void b2Body::SetSleepingAllowed(bool flag) {
+  Sync();
+  m_sleepingAllowed = flag;
+  if (!flag) SetAwake(true);
+  MarkDirty();
+}
+
+bool b2Body::IsAwake() const {
+  Sync();  // the device puts bodies to sleep and wakes them
+  return m_awake;
 }
 
 // b2_body.cpp:512-558: enabling lets the next Step find the body's contacts
@@ -628,6 +640,8 @@ void b2Body::ToRecord(b2g_body* r) const {
   r->mass = m_mass; r->inv_mass = m_invMass; r->inv_i = m_invI;
   r->linear_damping = m_linearDamping; r->angular_damping = m_angularDamping; r->gravity_scale = m_gravityScale;
   r->force[0] = m_force.x; r->force[1] = m_force.y; r->torque = m_torque;
+  r->flags = (m_awake || m_type == b2_staticBody ? 0 : B2G_BODY_ASLEEP) | (m_sleepingAllowed ? 0 : B2G_BODY_NO_AUTOSLEEP);
+  r->sleep_time = m_sleepTime;
 }
 
 void b2Body::FromRecord(const b2g_body& r) {
@@ -640,6 +654,8 @@ void b2Body::FromRecord(const b2g_body& r) {
   m_angularVelocity = r.w;
   m_force.Set(r.force[0], r.force[1]);
   m_torque = r.torque;
+  m_awake = m_type == b2_staticBody || (r.flags & B2G_BODY_ASLEEP) == 0;
+  m_sleepTime = r.sleep_time;
 }
 
 void b2Fixture::MarkShapeDirty() {
@@ -794,7 +810,6 @@ b2Body* b2World::CreateBody(const b2BodyDef* def) {  // b2_world.cpp:140-176
   m_bodies.push_back(b);
   m_dirty.push_back(1);
   m_anyDirty = true;
-  if (!b->m_awake && b->m_type != b2_staticBody) ++m_sleepingBodies;
   if (!b->m_enabled) ++m_disabledBodies;
   // statics at the tail, others at the head of the body list
   if (b->m_type == b2_staticBody) {
@@ -846,7 +861,6 @@ void b2World::DestroyBody(b2Body* b) {
   }
   b->m_fixtureList = nullptr;
   b->m_fixtureCount = 0;
-  if (!b->m_awake && b->m_type != b2_staticBody) --m_sleepingBodies;
   if (!b->m_enabled) --m_disabledBodies;
   if (b->m_prev) b->m_prev->m_next = b->m_next; else m_bodyListHead = b->m_next;
   if (b->m_next) b->m_next->m_prev = b->m_prev; else m_bodyListTail = b->m_prev;
@@ -871,6 +885,13 @@ void b2World::DestroyJoint(b2Joint* j) {
   m_joints.erase(m_joints.begin() + index);
   for (size_t i = index; i < m_joints.size(); ++i) m_joints[i]->m_index = (int32)i;
   delete j;
+}
+
+void b2World::SetAllowSleeping(bool flag) {
+  if (flag == m_allowSleep) return;
+  m_allowSleep = flag;
+  if (!m_allowSleep)
+    for (b2Body* b : m_bodies) b->SetAwake(true);
 }
 
 void b2World::SetContactListener(b2ContactListener* listener) {
@@ -967,8 +988,6 @@ void b2World::ApplyRefilter() {
 
 const char* b2World::UnsupportedReason() const {
   if (m_continuousPhysics) return "continuous physics (TOI) is not run on the device: call SetContinuousPhysics(false)";
-  if (m_allowSleep) return "sleeping is not run on the device: call SetAllowSleeping(false)";
-  if (m_sleepingBodies > 0) return "a body is asleep (SetAwake(false) or created with awake = false)";
   if (m_disabledBodies > 0) return "a body is disabled (SetEnabled(false) or created with enabled = false)";
   if (m_sensorFixtures > 0) return "sensor fixtures are not run on the device";
   if (m_contactFilter && overrides_should_collide(m_contactFilter))
@@ -1032,7 +1051,7 @@ int b2World::EnsureDevice() {
     m_cap.joints = std::max(16, nj * 2);
     m_cap.contacts = std::max(4096, std::min(ns * 16, (1 << 24) - 1));
     if ((rc = b2g_batch_create(&m_batch, &m_cap))) return rc;
-    b2g_world_params wp{{m_gravity.x, m_gravity.y}, 1.0f / 60.0f, 0.0f, m_warmStarting ? 1 : 0, 1};
+    b2g_world_params wp{{m_gravity.x, m_gravity.y}, 1.0f / 60.0f, 0.0f, m_warmStarting ? 1 : 0, 1, m_allowSleep ? 1 : 0};
     if ((rc = b2g_set_worlds(m_batch, 1, nullptr, nullptr, &wp))) return rc;
   }
   m_uploadedBodies = m_uploadedShapes = m_uploadedJoints = 0;
@@ -1054,7 +1073,7 @@ int b2World::Prepare() {
   int rc = EnsureDevice();
   if (!rc) rc = Upload();
   if (!rc) {
-    b2g_world_params wp{{m_gravity.x, m_gravity.y}, 1.0f / 60.0f, 0.0f, m_warmStarting ? 1 : 0, 1};
+    b2g_world_params wp{{m_gravity.x, m_gravity.y}, 1.0f / 60.0f, 0.0f, m_warmStarting ? 1 : 0, 1, m_allowSleep ? 1 : 0};
     rc = b2g_set_world_params(m_batch, 0, 1, &wp);
   }
   return rc;
@@ -1329,7 +1348,7 @@ void b2World::Step(float dt, int32 velocityIterations, int32 positionIterations,
   if (!m_status) m_status = Upload();
   if (!m_status) {
     const float dtRatio = m_inv_dt0 * dt;
-    b2g_world_params wp{{m_gravity.x, m_gravity.y}, dt, dtRatio, m_warmStarting ? 1 : 0, 1};
+    b2g_world_params wp{{m_gravity.x, m_gravity.y}, dt, dtRatio, m_warmStarting ? 1 : 0, 1, m_allowSleep ? 1 : 0};
     m_status = b2g_set_world_params(m_batch, 0, 1, &wp);
   }
   if (!m_status) {
@@ -1453,6 +1472,8 @@ static b2a_world* world_from_scene(const b2g_scene* s, int32_t warm_starting) {
     if (s->body_flags) {
       bd.fixedRotation = (s->body_flags[i] & 1) != 0;
       bd.bullet = (s->body_flags[i] & 2) != 0;
+      bd.allowSleep = (s->body_flags[i] & 4) == 0;
+      bd.awake = (s->body_flags[i] & 8) == 0;
     }
     w->bodies[i] = w->world->CreateBody(&bd);
   }
@@ -1672,6 +1693,12 @@ int b2a_world_set_filter(b2a_world* w, int32_t fixture, int32_t category, int32_
   f.maskBits = (uint16)mask;
   f.groupIndex = (int16)group;
   w->fixtures[fixture]->SetFilterData(f);
+  return B2G_OK;
+}
+
+int b2a_world_set_allow_sleep(b2a_world* w, int32_t flag) {
+  if (!w) return B2G_ERR_INVALID;
+  w->world->SetAllowSleeping(flag != 0);
   return B2G_OK;
 }
 

@@ -13,9 +13,9 @@
 // sets GetLastStatus() to B2G_ERR_UNSUPPORTED, prints the reason to stderr and
 // fires b2Assert (compiled out under NDEBUG, as in the reference's Release
 // build) while any of these is in effect: continuous physics (TOI, on by
-// default in b2World), sleeping (on by default), a body put to sleep or
-// disabled, a sensor fixture, a contact filter that overrides ShouldCollide,
-// or a contact listener that overrides PreSolve. Only revolute joints are
+// default in b2World), a disabled body, a sensor fixture, a contact filter
+// that overrides ShouldCollide, or a contact listener that overrides
+// PreSolve. Sleeping (on by default) runs on the device. Only revolute joints are
 // created (CreateJoint of another type returns nullptr). Begin/EndContact and
 // PostSolve are delivered after each Step from the device's record of it.
 #ifndef B2AMD_BOX2D_H
@@ -400,12 +400,12 @@ class b2Body {
   void ResetMassData();
   void SetBullet(bool flag) { m_bullet = flag; }
   bool IsBullet() const { return m_bullet; }
-  // sleeping is not run on the device: a body put to sleep (SetAwake(false),
-  // or created with awake = false) makes Step refuse until it is woken
-  void SetSleepingAllowed(bool flag) { m_sleepingAllowed = flag; }
+  // sleeping runs on the device (islands at rest fall asleep when the world
+  // allows it); these read and set the device's flags
+  void SetSleepingAllowed(bool flag);
   bool IsSleepingAllowed() const { return m_sleepingAllowed; }
   void SetAwake(bool flag);
-  bool IsAwake() const { return m_awake; }
+  bool IsAwake() const;
   // disabled bodies are not run on the device: Step refuses while one exists
   void SetEnabled(bool flag);
   bool IsEnabled() const { return m_enabled; }
@@ -442,6 +442,7 @@ class b2Body {
   float m_linearDamping, m_angularDamping, m_gravityScale;
   bool m_fixedRotation;
   bool m_bullet = false, m_sleepingAllowed = true, m_awake = true, m_enabled = true;
+  float m_sleepTime = 0.0f;
   b2Fixture* m_fixtureList = nullptr;
   int32 m_fixtureCount = 0;
   b2Body* m_prev = nullptr;
@@ -640,7 +641,8 @@ class b2World {
   b2Joint* GetJointList() { return m_jointList; }
   b2Contact* GetContactListStart();
   b2Contact* GetContactListEnd() { return nullptr; }
-  void SetAllowSleeping(bool flag) { m_allowSleep = flag; }
+  // b2_world.cpp:474-489: turning sleeping off wakes every body
+  void SetAllowSleeping(bool flag);
   bool GetAllowSleeping() const { return m_allowSleep; }
   void SetWarmStarting(bool flag) { m_warmStarting = flag; }
   bool GetWarmStarting() const { return m_warmStarting; }
@@ -727,7 +729,7 @@ class b2World {
   bool m_warmStarting = true, m_allowSleep = true, m_continuousPhysics = true, m_locked = false;
   bool m_profiling = false;
   // what Step must refuse (kept up to date by the mutators)
-  int32 m_sleepingBodies = 0, m_disabledBodies = 0, m_sensorFixtures = 0;
+  int32 m_disabledBodies = 0, m_sensorFixtures = 0;
   bool m_listenerPreSolve = false, m_listenerPostSolve = false;
   std::string m_lastRefusal;
   float m_inv_dt0 = 0.0f;

@@ -77,13 +77,20 @@ class Scene:
     joint_params: list = field(default_factory=list)
     joint_flags: list = field(default_factory=list)
     name: str = ""
+    # per body: bit0 fixedRotation, bit1 bullet, bit2 !allowSleep, bit3 !awake
+    # (b2BodyDef); empty = all 0
+    body_flags: list = field(default_factory=list)
 
     # ---- building -------------------------------------------------------
-    def add_body(self, btype, x, y, angle=0.0, vx=0.0, vy=0.0, w=0.0) -> int:
+    def add_body(self, btype, x, y, angle=0.0, vx=0.0, vy=0.0, w=0.0, flags=0) -> int:
+        if flags and not self.body_flags:
+            self.body_flags = [0] * len(self.body_type)
         self.body_type.append(btype)
         self.body_pos.append((x, y))
         self.body_angle.append(angle)
         self.body_vel.append((vx, vy, w))
+        if self.body_flags:
+            self.body_flags.append(flags)
         return len(self.body_type) - 1
 
     def _add_fix(self, body, kind, geom, density, friction, restitution=0.0,
@@ -168,6 +175,7 @@ class Scene:
             joint_bodies=np.asarray(self.joint_bodies, dtype=np.int32).reshape(j, 2),
             joint_params=np.asarray(self.joint_params, dtype=np.float32).reshape(j, 6),
             joint_flags=np.asarray(self.joint_flags, dtype=np.int32).reshape(j),
+            body_flags=np.asarray(self.body_flags if self.body_flags else [0] * n, dtype=np.int32).reshape(n),
         )
 
     def to_c(self):
@@ -212,7 +220,7 @@ class CScene:
         s.body_vel = a["body_vel"].ctypes.data
         s.body_damping = None
         s.body_gravity_scale = None
-        s.body_flags = None
+        s.body_flags = a["body_flags"].ctypes.data if scene.body_flags else None
         s.fixture_count = scene.fixture_count
         s.fix_body = a["fix_body"].ctypes.data
         s.fix_kind = a["fix_kind"].ctypes.data

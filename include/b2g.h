@@ -91,7 +91,8 @@ typedef struct b2g_scene {
   const float* body_vel;           /* [3n] vx, vy, w; NULL = zero */
   const float* body_damping;       /* [2n] linear, angular; NULL = zero */
   const float* body_gravity_scale; /* [n]; NULL = 1 */
-  const int32_t* body_flags;       /* [n] bit0 fixedRotation, bit1 bullet; NULL = 0 */
+  const int32_t* body_flags;       /* [n] bit0 fixedRotation, bit1 bullet, bit2 !allowSleep,
+                                      bit3 !awake (b2BodyDef); NULL = 0 */
   int32_t fixture_count;
   const int32_t* fix_body;         /* [m] */
   const int32_t* fix_kind;         /* [m] */
@@ -122,7 +123,11 @@ typedef struct b2g_body {
   float linear_damping, angular_damping, gravity_scale;
   float force[2];
   float torque;
+  int32_t flags;          /* B2G_BODY_ASLEEP: !b2Body::IsAwake(); B2G_BODY_NO_AUTOSLEEP: sleeping not
+                             allowed (b2BodyDef::allowSleep false); 0 = awake, may sleep (the defaults) */
+  float sleep_time;       /* m_sleepTime (include/box2d/b2_body.h:541-543) */
 } b2g_body;
+enum { B2G_BODY_ASLEEP = 1, B2G_BODY_NO_AUTOSLEEP = 2 };
 
 /* One fixture with its processed shape (after b2PolygonShape::Set etc.). */
 typedef struct b2g_shape {
@@ -201,6 +206,8 @@ typedef struct b2g_world_params {
   float dt_ratio;             /* m_inv_dt0 * dt (first step: 0) */
   int32_t warm_starting;
   int32_t active;             /* 0 = skip this world in the launch */
+  int32_t allow_sleep;        /* b2World::SetAllowSleeping (include/box2d/b2_world.h:195-197): bodies of
+                                 islands at rest for b2_timeToSleep fall asleep (b2_island.cpp:444-483) */
 } b2g_world_params;
 
 enum {
@@ -432,6 +439,8 @@ int b2a_world_destroy_joint(b2a_world* world, int32_t joint);
 int b2a_world_destroy_fixture(b2a_world* world, int32_t fixture);
 int b2a_world_set_filter(b2a_world* world, int32_t fixture, int32_t category, int32_t mask, int32_t group);
 int b2a_world_body_count(b2a_world* world, int32_t* out);
+/* b2World::SetAllowSleeping (src/dynamics/b2_world.cpp:474-489) */
+int b2a_world_set_allow_sleep(b2a_world* world, int32_t flag);
 
 /* Device records of a scene, built through the same b2World API (polygon hull
  * and welding, mass data) without touching the device: bodies tagged with

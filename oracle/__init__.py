@@ -123,13 +123,14 @@ class Ref:
 
 
 class RefWorld:
-    """A reference b2World built from a scene (continuous physics and sleep off)."""
+    """A reference b2World built from a scene (continuous physics off; sleep
+    off unless allow_sleep)."""
 
-    def __init__(self, scene, warm_starting=True):
+    def __init__(self, scene, warm_starting=True, allow_sleep=False):
         self.lib = Ref.lib()
         self.scene = scene
         self.cscene = scene.to_c()
-        self.h = self.lib.ref_world_from_scene(self.cscene.ptr(), int(warm_starting), 0, 0)
+        self.h = self.lib.ref_world_from_scene(self.cscene.ptr(), int(warm_starting), int(allow_sleep), 0)
         self.nb = scene.body_count
         self.nj = scene.joint_count
 
@@ -149,6 +150,13 @@ class RefWorld:
     def bodies(self) -> np.ndarray:
         out = np.zeros((self.nb, 10), dtype=np.float32)
         self.lib.ref_world_get_bodies(self.h, _ptr(out))
+        return out
+
+    def awake(self) -> np.ndarray:
+        """b2Body::IsAwake per body."""
+        out = np.zeros(self.nb, dtype=np.int32)
+        self.lib.ref_world_get_awake.argtypes = [_P, _P]
+        self.lib.ref_world_get_awake(self.h, _ptr(out))
         return out
 
     def mass(self) -> np.ndarray:
@@ -229,6 +237,7 @@ BODY_DTYPE = np.dtype([
     ("local_center", "<f4", 2), ("v", "<f4", 2), ("w", "<f4"), ("mass", "<f4"),
     ("inv_mass", "<f4"), ("inv_i", "<f4"), ("linear_damping", "<f4"),
     ("angular_damping", "<f4"), ("gravity_scale", "<f4"), ("force", "<f4", 2), ("torque", "<f4"),
+    ("flags", "<i4"), ("sleep_time", "<f4"),
 ])
 JOINT_DTYPE = np.dtype([
     ("body_a", "<i4"), ("body_b", "<i4"), ("local_anchor_a", "<f4", 2),
@@ -339,6 +348,18 @@ class RestatedWorld:
 
     def set_inv_dt0(self, v):
         self.lib.orc_world_set_inv_dt0(self.h, float(v))
+
+    def set_allow_sleep(self, flag):
+        self.lib.orc_world_set_allow_sleep.argtypes = [_P, _I]
+        self.lib.orc_world_set_allow_sleep(self.h, int(flag))
+
+
+    def set_collide_order(self, keys):
+        """The order the next step's Collide walks the contacts (the
+        reference's list order), wake-ups counting at once."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        self.lib.orc_world_set_collide_order.argtypes = [_P, _I, _P]
+        self.lib.orc_world_set_collide_order(self.h, len(keys), _ptr(keys))
 
     def set_colours(self, keys, colours, joint_colours=None):
         keys = np.ascontiguousarray(keys, dtype=np.uint64)

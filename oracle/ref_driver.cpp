@@ -195,6 +195,8 @@ void* ref_world_from_scene(const b2g_scene* s, int warm_starting, int allow_slee
     if (s->body_flags) {
       bd.fixedRotation = (s->body_flags[i] & 1) != 0;
       bd.bullet = (s->body_flags[i] & 2) != 0;
+      bd.allowSleep = (s->body_flags[i] & 4) == 0;
+      bd.awake = (s->body_flags[i] & 8) == 0;
     }
     bd.userData.pointer = (uintptr_t)i;
     rw->bodies[i] = rw->world.CreateBody(&bd);
@@ -263,6 +265,12 @@ void ref_world_get_bodies(void* w, float* out) {
     o[3] = v.x; o[4] = v.y; o[5] = b->GetAngularVelocity();
     o[6] = xf.p.x; o[7] = xf.p.y; o[8] = xf.q.s; o[9] = xf.q.c;
   }
+}
+
+// b2Body::IsAwake per body (the sleep timer itself is private to b2Body)
+void ref_world_get_awake(void* w, int* awake) {
+  RefWorld* rw = (RefWorld*)w;
+  for (size_t i = 0; i < rw->bodies.size(); ++i) awake[i] = rw->bodies[i]->IsAwake() ? 1 : 0;
 }
 
 // 4 floats per body: mass, rotational inertia about the centre, localCenter
@@ -381,9 +389,14 @@ void ref_world_profile(void* w, float* out) {
 // One reference Step that also reports the order in which the reference's
 // island solver visits constraints: b2World::Solve's DFS (src/dynamics/
 // b2_world.cpp:526-659) replayed over the body list, the per-body contact
-// arrays and joint lists as they stand before the step, with the touching set
-// recorded during the step's Collide through b2ContactListener::PreSolve
-// (called for every touching non-sensor contact, src/dynamics/b2_contact.cpp:206-208).
+// arrays and joint lists as they stand before the step. The touching set at
+// the solve is what Collide left: the contacts it updated and found touching
+// (b2ContactListener::PreSolve, called for every touching non-sensor contact
+// it updates, src/dynamics/b2_contact.cpp:206-208) and the contacts it did
+// not update (both bodies asleep or static, b2_contact_manager.cpp:100-111)
+// that were touching before the step. Islands are seeded from the bodies
+// awake at the solve: awake before the step or woken by one of Collide's
+// touching transitions.
 // out_keys receives (min << 32 | max) fixture keys of the solved contacts in
 // solve order; out_joints the scene indices of the solved joints in order.
 namespace {
@@ -403,17 +416,31 @@ uint64_t fixture_key(b2Contact* c) {
 // PostSolve are Collide's.
 struct TouchRecorder : public b2ContactListener {
   std::unordered_set<const b2Contact*> touching;
+  // Collide's touching transitions: the contacts, and the bodies they woke
+  // (b2_contact.cpp:183-188)
+  std::unordered_set<const b2Contact*> changed;
+  std::unordered_set<const b2Body*> woken;
   RefWorld* rw = nullptr;
   bool solved = false;
+  void transition(b2Contact* c) {
+    changed.insert(c);
+    woken.insert(c->GetFixtureA()->GetBody());
+    woken.insert(c->GetFixtureB()->GetBody());
+  }
   void BeginContact(b2Contact* c) override {
     rw->ev_kind.push_back(1);
     rw->ev_key.push_back(fixture_key(c));
     if (!solved) rw->ev_collide = (int)rw->ev_kind.size();
+    transition(c);
   }
   void EndContact(b2Contact* c) override {
     rw->ev_kind.push_back(2);
     rw->ev_key.push_back(fixture_key(c));
     if (!solved) rw->ev_collide = (int)rw->ev_kind.size();
+    // Collide clears the touching flag before it calls EndContact; the
+    // broadphase's destroy calls it on a contact still flagged touching
+    // (b2_contact_manager.cpp:48-51)
+    if (!c->IsTouching()) transition(c);
   }
   void PreSolve(b2Contact* c, const b2Manifold*) override { touching.insert(c); }
   void PostSolve(b2Contact* c, const b2ContactImpulse* impulse) override {
@@ -438,8 +465,13 @@ extern "C" int ref_world_step_ordered(void* w, float dt, int vi, int pi, int cap
   std::unordered_map<b2Body*, std::vector<const b2Contact*>> bcontacts;
   std::unordered_map<b2Body*, std::vector<std::pair<const b2Joint*, b2Body*>>> bjoints;
   std::unordered_map<const b2Joint*, int> jindex;
+  std::unordered_set<const b2Contact*> touch0;
+  std::unordered_set<const b2Body*> awake0;
   for (size_t i = 0; i < rw->joints.size(); ++i) jindex[rw->joints[i]] = (int)i;
+  for (b2Contact* c = rw->world.GetContactListStart(); c != rw->world.GetContactListEnd(); c = c->GetNext())
+    if (c->IsTouching() && !c->GetFixtureA()->IsSensor() && !c->GetFixtureB()->IsSensor()) touch0.insert(c);
   for (b2Body* b = rw->world.GetBodyList(); b; b = b->GetNext()) {
+    if (b->IsAwake()) awake0.insert(b);
     order.push_back(b);
     std::vector<const b2Contact*>& v = bcontacts[b];
     for (int i = 0; i < b->GetContactCount(); ++i) {
@@ -471,8 +503,12 @@ extern "C" int ref_world_step_ordered(void* w, float dt, int vi, int pi, int cap
   std::unordered_set<const b2Joint*> jflag;
   int nc = 0, nj = 0;
   std::vector<b2Body*> stack;
+  auto touching_at_solve = [&](const b2Contact* c) {
+    return rec.touching.count(c) != 0 || (touch0.count(c) != 0 && rec.changed.count(c) == 0);
+  };
   for (b2Body* seed : order) {
     if (bflag.count(seed)) continue;
+    if (!awake0.count(seed) && !rec.woken.count(seed)) continue;
     if (seed->GetType() == b2_staticBody) break;
     if (bcontacts[seed].empty() && bjoints[seed].empty()) {
       bflag.insert(seed);
@@ -489,7 +525,7 @@ extern "C" int ref_world_step_ordered(void* w, float dt, int vi, int pi, int cap
       if (b->GetType() == b2_staticBody) continue;
       for (const b2Contact* c : bcontacts[b]) {
         if (cflag.count(c)) continue;
-        if (!rec.touching.count(c)) continue;
+        if (!touching_at_solve(c)) continue;
         const CSnap& cs = csnap[c];
         if (nc < cap) out_keys[nc] = cs.key;
         ++nc;

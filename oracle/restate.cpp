@@ -27,6 +27,9 @@ const float kAngularSlop = 2.0f / 180.0f * kPi;         // b2_common.h:132
 const float kPolygonRadius = 2.0f * kLinearSlop;        // b2_common.h:137
 const float kMaxLinearCorrection = 0.2f;                // b2_common.h:150
 const float kMaxAngularCorrection = 8.0f / 180.0f * kPi;
+const float kTimeToSleep = 0.5f;                          // b2_common.h:176
+const float kLinearSleepTolerance = 0.01f * 1.0f;         // b2_common.h:179 (lengthUnitsPerMeter 1)
+const float kAngularSleepTolerance = 2.0f / 180.0f * 3.14159265359f;  // b2_common.h:182 (b2_pi)
 const float kMaxTranslation = 2.0f;
 const float kMaxTranslationSq = kMaxTranslation * kMaxTranslation;
 const float kMaxRotation = 0.5f * kPi;
@@ -816,6 +819,11 @@ struct Body {
   float linDamp, angDamp, gravScale;
   V2 force;
   float torque;
+  // sleep (b2_body.h:541-543; flags e_awakeFlag / e_autoSleepFlag): zero means
+  // awake and allowed to sleep, the b2BodyDef defaults
+  bool asleep;
+  bool no_autosleep;
+  float sleep_time;
 };
 
 struct Contact {
@@ -869,6 +877,7 @@ struct JointSolver {  // b2RevoluteJoint solver temporaries
 struct orc_world {
   V2 gravity;
   bool warm;
+  bool allow_sleep = false;  // b2World::m_allowSleep
   float inv_dt0 = 0.0f;
   bool new_contacts = true;
   std::vector<Body> bodies;
@@ -876,6 +885,9 @@ struct orc_world {
   std::vector<Joint> joints;
   std::vector<Contact> contacts;  // sorted by key
   std::unordered_map<uint64_t, int> colour_of;
+  // order in which the next Collide walks the contacts (keys); empty = the
+  // parallel walk described in collide()
+  std::vector<uint64_t> collide_order;
   std::vector<int32_t> joint_colour;
 };
 
@@ -905,6 +917,19 @@ bool bodies_collide(const orc_world* w, int ba, int bb) {
 void sync_transform(Body& b) {  // b2_body.h:957-961
   b.xf.q = rot(b.a);
   b.xf.p = b.c - mul(b.xf.q, b.lc);
+}
+
+// b2Body::SetAwake (include/box2d/b2_body.h:719-740)
+void set_awake(Body& b, bool flag) {
+  if (b.type == B2G_STATIC) return;
+  b.sleep_time = 0.0f;
+  b.asleep = !flag;
+  if (!flag) {
+    b.v = v2(0.0f, 0.0f);
+    b.w = 0.0f;
+    b.force = v2(0.0f, 0.0f);
+    b.torque = 0.0f;
+  }
 }
 
 // Pair set of b2BroadPhase::UpdateAndQuery + QueryCallback: sort-and-sweep on x
@@ -978,10 +1003,18 @@ void broadphase(orc_world* w) {
   std::vector<Contact> next;
   next.reserve(keys.size());
   size_t oi = 0;
+  // a contact that leaves the list with points wakes its bodies
+  // (b2Contact::Destroy, src/dynamics/b2_contact.cpp:79-91)
+  auto destroyed = [&](const Contact& c) {
+    if (c.m.count > 0 && !w->shapes[c.a].is_sensor && !w->shapes[c.b].is_sensor) {
+      set_awake(w->bodies[w->shapes[c.a].body], true);
+      set_awake(w->bodies[w->shapes[c.b].body], true);
+    }
+  };
   for (uint64_t k : keys) {
-    while (oi < w->contacts.size() && w->contacts[oi].key < k) ++oi;
+    while (oi < w->contacts.size() && w->contacts[oi].key < k) destroyed(w->contacts[oi++]);
     if (oi < w->contacts.size() && w->contacts[oi].key == k) {
-      next.push_back(w->contacts[oi]);
+      next.push_back(w->contacts[oi++]);
       continue;
     }
     const int i = (int)(k >> 32), j = (int)(k & 0xffffffffu);
@@ -1000,16 +1033,48 @@ void broadphase(orc_world* w) {
     c.tangentSpeed = 0.0f;
     next.push_back(c);
   }
+  for (; oi < w->contacts.size(); ++oi) destroyed(w->contacts[oi]);
   w->contacts.swap(next);
 }
 
 // b2ContactManager::Collide + b2Contact::Update (b2_contact_manager.cpp:66-116,
 // b2_contact.cpp:126-210): new manifold, impulse carry-over by feature key.
 void collide(orc_world* w) {
-  for (Contact& c : w->contacts) {
+  // A contact is updated when one of its bodies is awake and not static
+  // (b2_contact_manager.cpp:100-106). The reference walks its list in order,
+  // so a body woken by an earlier contact's touching change already counts
+  // for later contacts of the same walk. With a walk order given
+  // (collide_order, the reference's list order) that is what happens here;
+  // without one the flags are those at the start of Collide and a wake-up
+  // counts from the next step on (as on the device, where the contacts are
+  // updated in parallel).
+  const bool walk = !w->collide_order.empty();
+  std::vector<char> active0(w->bodies.size());
+  for (size_t i = 0; i < w->bodies.size(); ++i)
+    active0[i] = !w->bodies[i].asleep && w->bodies[i].type != B2G_STATIC;
+  std::vector<int> visit;
+  visit.reserve(w->contacts.size());
+  if (walk) {
+    std::unordered_map<uint64_t, int> at;
+    for (int i = 0; i < (int)w->contacts.size(); ++i) at[w->contacts[i].key] = i;
+    std::vector<char> seen(w->contacts.size(), 0);
+    for (uint64_t k : w->collide_order) {
+      auto it = at.find(k);
+      if (it != at.end() && !seen[it->second]) { visit.push_back(it->second); seen[it->second] = 1; }
+    }
+    for (int i = 0; i < (int)w->contacts.size(); ++i)
+      if (!seen[i]) visit.push_back(i);
+    w->collide_order.clear();
+  } else {
+    for (int i = 0; i < (int)w->contacts.size(); ++i) visit.push_back(i);
+  }
+  for (int ci : visit) {
+    Contact& c = w->contacts[ci];
     const b2g_shape &A = w->shapes[c.a], &B = w->shapes[c.b];
     const Body &ba = w->bodies[A.body], &bb = w->bodies[B.body];
-    if (ba.type == B2G_STATIC && bb.type == B2G_STATIC) continue;  // inactive (:100-106)
+    const bool activeA = walk ? (!ba.asleep && ba.type != B2G_STATIC) : active0[A.body];
+    const bool activeB = walk ? (!bb.asleep && bb.type != B2G_STATIC) : active0[B.body];
+    if (!activeA && !activeB) continue;
     Manifold old = c.m;
     evaluate(&c.m, A, ba.xf, B, bb.xf);
     for (int i = 0; i < c.m.count; ++i) {
@@ -1024,7 +1089,12 @@ void collide(orc_world* w) {
         }
       }
     }
+    const bool was = c.touching;
     c.touching = c.m.count > 0;
+    if (c.touching != was) {  // b2_contact.cpp:183-188
+      set_awake(w->bodies[A.body], true);
+      set_awake(w->bodies[B.body], true);
+    }
   }
 }
 
@@ -1049,24 +1119,8 @@ void solve(orc_world* w, float dt, int vi, int pi) {
   const int nb = (int)w->bodies.size();
   std::vector<V2> pc(nb), vv(nb);
   std::vector<float> pa(nb), vw(nb);
-  // Integrate velocities and damping (b2_island.cpp:257-298; SolveOrphan :187-206)
-  for (int i = 0; i < nb; ++i) {
-    Body& b = w->bodies[i];
-    V2 v = b.v;
-    float wv = b.w;
-    if (b.type == B2G_DYNAMIC) {
-      v = v + h * b.invMass * ((b.gravScale * b.mass) * w->gravity + b.force);
-      wv += h * b.invI * b.torque;
-      v = V2{v.x / (1.0f + h * b.linDamp), v.y / (1.0f + h * b.linDamp)};
-      wv = wv / (1.0f + h * b.angDamp);
-    }
-    pc[i] = b.c;
-    pa[i] = b.a;
-    vv[i] = v;
-    vw[i] = wv;
-  }
-  // Constraint lists (touching, enabled, non-sensor contacts; joints with a
-  // non-static body), ordered by colour then index.
+  // Constraint candidates (touching, enabled, non-sensor contacts; joints
+  // with a non-static body)
   std::vector<int> cl;
   for (int i = 0; i < (int)w->contacts.size(); ++i) {
     const Contact& c = w->contacts[i];
@@ -1074,19 +1128,12 @@ void solve(orc_world* w, float dt, int vi, int pi) {
     if (w->shapes[c.a].is_sensor || w->shapes[c.b].is_sensor) continue;
     cl.push_back(i);
   }
-  auto colour = [&](int ci) -> int {
-    auto it = w->colour_of.find(w->contacts[ci].key);
-    return it == w->colour_of.end() ? 0 : it->second;
-  };
-  std::stable_sort(cl.begin(), cl.end(), [&](int x, int y) { return colour(x) < colour(y); });
   std::vector<int> jl;
   for (int i = 0; i < (int)w->joints.size(); ++i) {
     const Joint& j = w->joints[i];
     if (w->bodies[j.r.body_a].type == B2G_STATIC && w->bodies[j.r.body_b].type == B2G_STATIC) continue;
     jl.push_back(i);
   }
-  if (!w->joint_colour.empty())
-    std::stable_sort(jl.begin(), jl.end(), [&](int x, int y) { return w->joint_colour[x] < w->joint_colour[y]; });
   // Islands: connected components through non-static bodies (b2_world.cpp:526-647)
   UF uf(nb);
   auto link = [&](int a, int b) {
@@ -1097,6 +1144,54 @@ void solve(orc_world* w, float dt, int vi, int pi) {
   auto island_of = [&](int ba, int bb) {
     return w->bodies[ba].type != B2G_STATIC ? uf.find(ba) : uf.find(bb);
   };
+  // Only islands with an awake body are built and solved (the seeds,
+  // b2_world.cpp:526-552); the depth-first search wakes every body it
+  // reaches without resetting its sleep timer (:573-574).
+  std::vector<char> isl_awake(nb, 0), active(nb, 0);
+  for (int i = 0; i < nb; ++i)
+    if (w->bodies[i].type != B2G_STATIC && !w->bodies[i].asleep) isl_awake[uf.find(i)] = 1;
+  for (int i = 0; i < nb; ++i) {
+    Body& b = w->bodies[i];
+    if (b.type == B2G_STATIC || !isl_awake[uf.find(i)]) continue;
+    active[i] = 1;
+    b.asleep = false;
+  }
+  {
+    std::vector<int> keep;
+    for (int ci : cl) {
+      const Contact& c = w->contacts[ci];
+      if (isl_awake[island_of(w->shapes[c.a].body, w->shapes[c.b].body)]) keep.push_back(ci);
+    }
+    cl.swap(keep);
+    keep.clear();
+    for (int ji : jl)
+      if (isl_awake[island_of(w->joints[ji].r.body_a, w->joints[ji].r.body_b)]) keep.push_back(ji);
+    jl.swap(keep);
+  }
+  // ordered by colour then index
+  auto colour = [&](int ci) -> int {
+    auto it = w->colour_of.find(w->contacts[ci].key);
+    return it == w->colour_of.end() ? 0 : it->second;
+  };
+  std::stable_sort(cl.begin(), cl.end(), [&](int x, int y) { return colour(x) < colour(y); });
+  if (!w->joint_colour.empty())
+    std::stable_sort(jl.begin(), jl.end(), [&](int x, int y) { return w->joint_colour[x] < w->joint_colour[y]; });
+  // Integrate velocities and damping (b2_island.cpp:257-298; SolveOrphan :187-206)
+  for (int i = 0; i < nb; ++i) {
+    Body& b = w->bodies[i];
+    V2 v = b.v;
+    float wv = b.w;
+    if (b.type == B2G_DYNAMIC && active[i]) {
+      v = v + h * b.invMass * ((b.gravScale * b.mass) * w->gravity + b.force);
+      wv += h * b.invI * b.torque;
+      v = V2{v.x / (1.0f + h * b.linDamp), v.y / (1.0f + h * b.linDamp)};
+      wv = wv / (1.0f + h * b.angDamp);
+    }
+    pc[i] = b.c;
+    pa[i] = b.a;
+    vv[i] = v;
+    vw[i] = wv;
+  }
 
   // b2ContactSolver::Initialize + InitializeVelocityConstraints (b2_contact_solver.cpp:65-258)
   const int nc = (int)cl.size();
@@ -1387,7 +1482,7 @@ void solve(orc_world* w, float dt, int vi, int pi) {
   }
   // Integrate positions with the translation/rotation clamp (b2_island.cpp:341-385)
   for (int i = 0; i < nb; ++i) {
-    if (w->bodies[i].type == B2G_STATIC) continue;
+    if (!active[i]) continue;
     V2 v = vv[i];
     float wv = vw[i];
     V2 translation = h * v;
@@ -1524,12 +1619,62 @@ void solve(orc_world* w, float dt, int vi, int pi) {
   // Write back (b2_island.cpp:430-438)
   for (int i = 0; i < nb; ++i) {
     Body& b = w->bodies[i];
-    if (b.type == B2G_STATIC) continue;
+    if (!active[i]) continue;
     b.c = pc[i];
     b.a = pa[i];
     b.v = vv[i];
     b.w = vw[i];
     sync_transform(b);
+  }
+  if (!w->allow_sleep) return;
+  // Sleep (b2_island.cpp:444-483; orphans :233-250). An orphan is a body
+  // without any contact or joint (b2_world.cpp:545-549); it runs its own
+  // timer, and its seed flags set the awake flag again right after
+  // SolveOrphan, so an orphan whose timer runs out has its velocities and
+  // forces cleared and stays awake.
+  const float linTolSqr = kLinearSleepTolerance * kLinearSleepTolerance;
+  const float angTolSqr = kAngularSleepTolerance * kAngularSleepTolerance;
+  std::vector<int> ncontact(nb, 0);
+  std::vector<char> hasjoint(nb, 0);
+  for (const Contact& c : w->contacts) {
+    ++ncontact[w->shapes[c.a].body];
+    ++ncontact[w->shapes[c.b].body];
+  }
+  for (const Joint& j : w->joints) hasjoint[j.r.body_a] = hasjoint[j.r.body_b] = 1;
+  std::vector<float> min_sleep(nb, FLT_MAX);
+  for (int i = 0; i < nb; ++i) {
+    if (!active[i]) continue;
+    Body& b = w->bodies[i];
+    const bool slow = !b.no_autosleep && !(b.w * b.w > angTolSqr) && !(dot(b.v, b.v) > linTolSqr);
+    if (ncontact[i] == 0 && !hasjoint[i]) {
+      if (!slow) {
+        b.sleep_time = 0.0f;
+      } else {
+        b.sleep_time += h;
+        if (b.sleep_time >= kTimeToSleep) set_awake(b, false);
+      }
+      b.asleep = false;
+      continue;
+    }
+    float& m = min_sleep[uf.find(i)];
+    if (!slow) {
+      b.sleep_time = 0.0f;
+      m = 0.0f;
+    } else {
+      b.sleep_time += h;
+      m = fmin_(m, b.sleep_time);
+    }
+  }
+  // positionSolved: the island's early exit happened; an island without
+  // constraints exits at its first position iteration
+  std::vector<char> has_cons(nb, 0);
+  for (int k = 0; k < nc; ++k) has_cons[c_island[k]] = 1;
+  for (int k = 0; k < nj; ++k) has_cons[j_island[k]] = 1;
+  for (int i = 0; i < nb; ++i) {
+    if (!active[i] || (ncontact[i] == 0 && !hasjoint[i])) continue;
+    const int r = uf.find(i);
+    const bool position_solved = has_cons[r] ? solved[r] : pi > 0;
+    if (min_sleep[r] >= kTimeToSleep && position_solved) set_awake(w->bodies[i], false);
   }
 }
 
@@ -1557,6 +1702,10 @@ orc_world* orc_world_from_scene(const b2g_scene* s, int warm_starting) {
     if (s->body_vel) {
       b.v = v2(s->body_vel[3 * i], s->body_vel[3 * i + 1]);
       b.w = s->body_vel[3 * i + 2];
+    }
+    if (s->body_flags) {  // b2BodyDef::allowSleep, ::awake (b2_body.cpp:52-61)
+      b.no_autosleep = (s->body_flags[i] & 4) != 0;
+      b.asleep = (s->body_flags[i] & 8) != 0 && b.type != B2G_STATIC;
     }
     if (s->body_damping) {
       b.linDamp = s->body_damping[2 * i];
@@ -1710,6 +1859,8 @@ void orc_world_get_bodies(orc_world* w, b2g_body* out) {
     o.mass = b.mass; o.inv_mass = b.invMass; o.inv_i = b.invI;
     o.linear_damping = b.linDamp; o.angular_damping = b.angDamp; o.gravity_scale = b.gravScale;
     o.force[0] = b.force.x; o.force[1] = b.force.y; o.torque = b.torque;
+    o.flags = (b.asleep ? B2G_BODY_ASLEEP : 0) | (b.no_autosleep ? B2G_BODY_NO_AUTOSLEEP : 0);
+    o.sleep_time = b.sleep_time;
   }
 }
 
@@ -1725,6 +1876,9 @@ void orc_world_set_bodies(orc_world* w, const b2g_body* in) {
     b.mass = o.mass; b.invMass = o.inv_mass; b.invI = o.inv_i;
     b.linDamp = o.linear_damping; b.angDamp = o.angular_damping; b.gravScale = o.gravity_scale;
     b.force = v2(o.force[0], o.force[1]); b.torque = o.torque;
+    b.asleep = (o.flags & B2G_BODY_ASLEEP) != 0 && b.type != B2G_STATIC;
+    b.no_autosleep = (o.flags & B2G_BODY_NO_AUTOSLEEP) != 0;
+    b.sleep_time = o.sleep_time;
   }
 }
 
@@ -1770,6 +1924,10 @@ void orc_world_set_contacts(orc_world* w, int n, const b2g_contact* in) {
 
 void orc_world_set_inv_dt0(orc_world* w, float inv_dt0) { w->inv_dt0 = inv_dt0; }
 
+void orc_world_set_allow_sleep(orc_world* w, int flag) { w->allow_sleep = flag != 0; }
+
+
+
 void orc_world_set_colours(orc_world* w, int n, const uint64_t* keys, const int32_t* colours,
                            const int32_t* joint_colours) {
   w->colour_of.clear();
@@ -1778,7 +1936,11 @@ void orc_world_set_colours(orc_world* w, int n, const uint64_t* keys, const int3
   if (joint_colours) w->joint_colour.assign(joint_colours, joint_colours + w->joints.size());
 }
 
-// b2World::Step (b2_world.cpp:1108-1171) with continuous physics and sleep off.
+void orc_world_set_collide_order(orc_world* w, int n, const uint64_t* keys) {
+  w->collide_order.assign(keys, keys + n);
+}
+
+// b2World::Step (b2_world.cpp:1108-1171) with continuous physics off.
 void orc_world_step(orc_world* w, float dt, int vi, int pi, unsigned flags) {
   if (w->new_contacts || (flags & B2G_STEP_FIND_NEW_CONTACTS)) {
     broadphase(w);

@@ -63,11 +63,19 @@ void orc_world_get_contacts(orc_world* w, b2g_contact* out);
 void orc_world_set_contacts(orc_world* w, int n, const b2g_contact* in);
 /* dt ratio carried between steps (m_inv_dt0 * dt) */
 void orc_world_set_inv_dt0(orc_world* w, float inv_dt0);
+/* b2World::SetAllowSleeping (the bodies' sleep state travels in b2g_body) */
+void orc_world_set_allow_sleep(orc_world* w, int flag);
 
 /* Solver order for the next steps: colour of each touching contact, looked up
  * by key (min << 32 | max), and of each joint. NULL = all colour 0. */
 void orc_world_set_colours(orc_world* w, int n, const uint64_t* contact_keys,
                            const int32_t* contact_colours, const int32_t* joint_colours);
+
+/* The order in which the next step's Collide walks the contacts (keys), as
+ * the reference walks its contact list, a wake-up counting at once for the
+ * contacts after it; used by that one step only. Without it the contacts
+ * are updated from the awake flags at the start of Collide. */
+void orc_world_set_collide_order(orc_world* w, int n, const uint64_t* contact_keys);
 
 /* flags: B2G_STEP_FIND_NEW_CONTACTS forces a broadphase before Collide;
  * B2G_STEP_SKIP_BROADPHASE omits the end-of-step broadphase. */

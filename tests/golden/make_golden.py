@@ -106,6 +106,80 @@ def chains_scene():
     return S.gas_and_chains(n_circles=3000, n_chains=16, links=30, seed=0x5C3, half=80.0)
 
 
+def sleep_scene(ball_height=150.0):
+    """Sleeping on: an 8-row pyramid that settles and sleeps, a ball dropped
+    onto it later (wakes the pyramid's island), a resting box that may not
+    sleep (b2BodyDef::allowSleep false), a box created asleep in the air with
+    a box falling onto it (woken by the new touch), and a 4-link chain
+    hanging at rest from a static anchor (a joint island that sleeps)."""
+    s = S.pyramid(8)
+    s.name = "sleep"
+    b = s.add_body(S.DYNAMIC, 20.0, 0.5, flags=4)
+    s.add_box(b, 0.5, 0.5)
+    b = s.add_body(S.DYNAMIC, 25.0, 5.0, flags=8)
+    s.add_box(b, 0.5, 0.5)
+    b = s.add_body(S.DYNAMIC, 25.2, 8.0)
+    s.add_box(b, 0.4, 0.4)
+    prev = s.add_body(S.STATIC, -25.0, 15.0)
+    for k in range(4):
+        b = s.add_body(S.DYNAMIC, -25.0, 15.0 - 0.5 - k)
+        s.add_box(b, 0.1, 0.5)
+        s.add_revolute(prev, b, (-25.0, 15.0 - k))
+        prev = b
+    b = s.add_body(S.DYNAMIC, -3.0625, 9.5 + ball_height)
+    s.add_circle(b, 0.5)
+    return s
+
+
+SLEEP_STEPS = 420
+SLEEP_STATE_EVERY = 5
+
+
+def sleep():
+    """The reference with sleeping on, step by step: the order of its contact
+    list (Collide's walk), the order its island
+    solver visited contacts and joints, the fixture A of every contact its
+    broadphase created in the step (the order of a new pair follows its tree
+    build, b2_broad_phase.h UpdateAndQuery), every body's awake flag after
+    each step, and the body states every SLEEP_STATE_EVERY steps."""
+    sc = sleep_scene()
+    rw = oracle.RefWorld(sc, allow_sleep=True)
+    keys, koff, joints, joff, awake, states = [], [0], [], [0], [], []
+    new_keys, new_a, noff = [], [], [0]
+    walk, woff = [], [0]
+    oriented = {}
+    for k in range(SLEEP_STEPS):
+        # Collide's walk order: the contact list, as ranks among the sorted keys
+        lk = oracle.contact_keys(rw.contacts())
+        walk.append(np.argsort(np.argsort(lk, kind="stable"), kind="stable").astype(np.int16))
+        woff.append(woff[-1] + len(lk))
+        ok, oj = oracle.ref_step_ordered(rw)
+        keys.append(ok)
+        koff.append(koff[-1] + len(ok))
+        joints.append(oj)
+        joff.append(joff[-1] + len(oj))
+        rc = rw.contacts()
+        now = dict(zip(oracle.contact_keys(rc).tolist(), rc["shape_a"].tolist()))
+        fresh = [(key, a) for key, a in now.items() if oriented.get(key) != a]
+        oriented = now
+        new_keys.extend(key for key, _ in fresh)
+        new_a.extend(a for _, a in fresh)
+        noff.append(noff[-1] + len(fresh))
+        awake.append(rw.awake())
+        if (k + 1) % SLEEP_STATE_EVERY == 0:
+            states.append(rw.bodies())
+    aw = np.array(awake, np.uint8)
+    np.savez_compressed(os.path.join(HERE, "sleep_trajectory.npz"), steps=np.int32(SLEEP_STEPS),
+                        state_every=np.int32(SLEEP_STATE_EVERY), order_keys=np.concatenate(keys),
+                        order_offsets=np.array(koff, np.int64), order_joints=np.concatenate(joints).astype(np.int32),
+                        joint_offsets=np.array(joff, np.int64), new_keys=np.array(new_keys, np.uint64),
+                        new_shape_a=np.array(new_a, np.int32), new_offsets=np.array(noff, np.int64),
+                        walk_ranks=np.concatenate(walk), walk_offsets=np.array(woff, np.int64),
+                        awake=aw, states=np.array(states, np.float32))
+    print("sleep:", sc.body_count, "bodies,", SLEEP_STEPS, "steps; awake counts at steps 1/100/330/last:",
+          int(aw[0].sum()), int(aw[99].sum()), int(aw[329].sum()), int(aw[-1].sum()))
+
+
 def narrowphase(rng_seed=1234, per_kind=400):
     src = S.small_mixed(seed=3, n=90)
     pile = S.random_pile(300, seed=77, half_w=20, height=40)
@@ -242,6 +316,6 @@ if __name__ == "__main__":
     if not oracle.Ref.available():
         oracle.build(ref=True)
     parts = {"narrowphase": narrowphase, "shapes": shapes, "pairs": pairs, "onestep": onestep,
-             "trajectory": trajectory}
+             "trajectory": trajectory, "sleep": sleep}
     for part in (sys.argv[1:] or list(parts)):  # e.g. `make_golden.py onestep pairs`
         parts[part]()

@@ -26,12 +26,12 @@ def test_library_exports_every_declared_entry_point():
 def test_record_sizes_match_header():
     eng = load_pkg().engine
     # 4-byte fields only: sizes are the field counts times four
-    assert eng.BODY_DTYPE.itemsize == 4 * 23
+    assert eng.BODY_DTYPE.itemsize == 4 * 25
     assert eng.SHAPE_DTYPE.itemsize == 4 * (4 + 2 + 16 + 16 + 3 + 4)
     assert eng.JOINT_DTYPE.itemsize == 4 * (2 + 4 + 5 + 1 + 5)
     assert eng.CONTACT_DTYPE.itemsize == 4 * (5 + 2 + 2 + 4 + 2 + 2 + 2 + 4)
     assert ctypes.sizeof(eng.Capacity) == 20
-    assert ctypes.sizeof(eng.WorldParams) == 24
+    assert ctypes.sizeof(eng.WorldParams) == 28
     assert ctypes.sizeof(eng.Counts) == 4 * 12
     # serving records: natural alignment, as the C compiler lays them out
     assert eng.REQUEST_DTYPE.itemsize == 8 + 4 * 4
@@ -118,10 +118,12 @@ def test_python_records_match_the_c_compiler_layout(tmp_path):
 
 def test_drop_in_refuses_unsupported_features_before_the_device():
     """User code that turns on what the device does not run (continuous
-    physics and sleeping, b2World's defaults; a sleeping or disabled body; a
-    sensor; a PreSolve listener; a filter overriding ShouldCollide) gets
-    B2G_ERR_UNSUPPORTED with a message from Step, and the world is left as it
-    was; the check comes before any device call, so it holds on the CPU."""
+    physics, b2World's default; a disabled body; a sensor; a PreSolve
+    listener; a filter overriding ShouldCollide) gets B2G_ERR_UNSUPPORTED
+    with a message from Step, and the world is left as it was; the check
+    comes before any device call, so it holds on the CPU. Sleeping (the other
+    default) and a body put to sleep are served: those steps get as far as
+    the device."""
     import subprocess
     import pytest
     exe = os.path.join(REPO, "box2d-optimized_amd", "build", "unsupported_features")
@@ -133,7 +135,8 @@ def test_drop_in_refuses_unsupported_features_before_the_device():
     r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     lines = r.stdout.strip().splitlines()
     assert r.returncode == 0, r.stdout + r.stderr
-    assert len(lines) == 8
-    assert all(": 5 " in ln for ln in lines[:7])
-    assert "SetContinuousPhysics(false)" in lines[0] and "SetAllowSleeping(false)" in lines[1]
-    assert lines[-1].split(":")[1].split()[0] == "6"  # B2G_ERR_NO_DEVICE: it got as far as the device
+    assert len(lines) == 9
+    status = [int(ln.split(":")[1].split()[0]) for ln in lines]
+    # 5 = B2G_ERR_UNSUPPORTED, 6 = B2G_ERR_NO_DEVICE (it got as far as the device)
+    assert status == [5, 6, 6, 6, 5, 5, 5, 5, 6]
+    assert "SetContinuousPhysics(false)" in lines[0]

@@ -109,12 +109,14 @@ def _compare(sc, gb, ob, label):
 
 def test_gpu_unsupported_features_are_refused_and_the_rest_steps(engine):
     """The refusal example on a device: every unsupported feature is refused
-    with the world untouched, and the final step runs (status 0)."""
+    with the world untouched (status 5), and the served cases (sleeping on,
+    a body put to sleep, sleeping off, a PostSolve listener) step (status 0)."""
     exe = os.path.join(REPO, "box2d-optimized_amd", "build", "unsupported_features")
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(r.stdout, flush=True)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.strip().splitlines()[-1].split(":")[1].split()[0] == "0"
+    status = [int(ln.split(":")[1].split()[0]) for ln in r.stdout.strip().splitlines()]
+    assert status == [5, 0, 0, 0, 5, 5, 5, 5, 0]
 
 
 def test_gpu_step_zero_is_collide_only(engine):

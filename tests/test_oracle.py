@@ -110,6 +110,65 @@ def test_one_step_in_reference_order_is_bitexact(name):
         assert np.array_equal(bits(ji["impulse"]), bits(g["joints1"][:, :2]))
 
 
+def _sleep_scene():
+    import sys
+    from util import GOLDEN
+    sys.path.insert(0, GOLDEN)
+    import make_golden
+    return make_golden.sleep_scene()
+
+
+def test_sleeping_steps_in_reference_order_are_bitexact():
+    """Sleeping on (b2World::SetAllowSleeping(true)), 420 steps in lock-step
+    with the reference's island order, its contact-list order (Collide's
+    walk: a body woken by a contact counts at once for the contacts after it)
+    and its fixture order for each new contact (the reference's free
+    choices; the restatement takes them from the fixture): every body's
+    awake flag after every step and the
+    body states every 5 steps equal the reference's bit for bit.
+    The run covers islands falling asleep together (b2_island.cpp:420-454),
+    a body that may not sleep, a body created asleep and woken by a new touch
+    (b2_contact.cpp:224-228), a joint island, and a sleeping island woken
+    when a falling ball reaches it (the depth-first search of
+    b2_world.cpp:560-575)."""
+    g = golden("sleep_trajectory.npz")
+    sc = _sleep_scene()
+    w = oracle.RestatedWorld(sc)
+    w.set_allow_sleep(True)
+    ko, jo, no, every = g["order_offsets"], g["joint_offsets"], g["new_offsets"], int(g["state_every"])
+    wo = g["walk_offsets"]
+    aw = g["awake"].astype(bool)
+    # the fixture holds the events named above: everything asleep but the
+    # sleepless box and the falling ball, then the pyramid awake again
+    assert aw[0].sum() == sc.body_count - 3 and aw[99].sum() == 4 and aw[-1].sum() > 30
+    for k in range(int(g["steps"])):
+        keys = g["order_keys"][ko[k]:ko[k + 1]]
+        jr = np.zeros(sc.joint_count, np.int32)
+        jr[g["order_joints"][jo[k]:jo[k + 1]]] = np.arange(jo[k + 1] - jo[k], dtype=np.int32)
+        w.set_colours(keys, np.arange(len(keys), dtype=np.int32), jr)
+        listed = oracle.contact_keys(w.contacts())
+        assert len(listed) == wo[k + 1] - wo[k]
+        w.set_collide_order(listed[g["walk_ranks"][wo[k]:wo[k + 1]]])
+        w.step(1)
+        c = w.contacts()
+        row = dict(zip(oracle.contact_keys(c).tolist(), range(len(c))))
+        for key, a in zip(g["new_keys"][no[k]:no[k + 1]].tolist(), g["new_shape_a"][no[k]:no[k + 1]]):
+            i = row[key]
+            if c["shape_a"][i] != a:
+                assert c["point_count"][i] == 0  # only a new, empty contact is turned round
+                c["shape_a"][i], c["shape_b"][i] = c["shape_b"][i], c["shape_a"][i]
+        assert len(c) == len(row) and len(set(row)) == len(c)
+        w.set_contacts(c)
+        b = w.bodies()
+        awake = (b["flags"] & 1) == 0
+        awake[np.asarray(sc.body_type) == S.STATIC] = False  # b2Body::IsAwake of a static body is false
+        assert np.array_equal(awake, aw[k]), f"step {k + 1}: awake flags differ at {np.nonzero(awake != aw[k])[0]}"
+        if (k + 1) % every == 0:
+            st = g["states"][(k + 1) // every - 1]
+            got = np.concatenate([b["c"], b["a"][:, None], b["v"], b["w"][:, None], b["xf"]], axis=1)
+            assert np.array_equal(bits(got), bits(st)), f"step {k + 1}: body states differ"
+
+
 @pytest.mark.parametrize("name", ["pile2000", "filtered", "mixed"])
 def test_pair_set_matches_reference_golden(name):
     g = golden("pairs.npz")
