@@ -363,6 +363,8 @@ def main():
     ap.add_argument("--launch", choices=["eager", "graph"], default="eager",
                     help="timed steps: kernels launched from the host with one colour read-back per step "
                          "(eager, default) or a replayed hipGraph of the whole step (graph)")
+    ap.add_argument("--solver", choices=["auto", "colours", "cells"], default="auto",
+                    help="contact solver of the headline world (auto: the colour sweeps for one world)")
     ap.add_argument("--batch-worlds", type=int, default=4096, help="0 skips the batched section")
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--batch-sample", type=int, default=64, help="per-world loop / CPU sample stride")
@@ -400,6 +402,8 @@ def main():
     dt = np.float32(1.0 / 60.0)
     ratio = np.float32(np.float32(1.0) / dt) * dt
     batch.set_world_params(0, [eng.WorldParams((scene.gravity[0], scene.gravity[1]), dt, ratio, 1, 1)])
+    batch.set_option(eng.OPT_SOLVER, {"auto": eng.SOLVER_AUTO, "colours": eng.SOLVER_COLOURS,
+                                      "cells": eng.SOLVER_CELLS}[args.solver])
     for _ in range(2):  # last warm-up steps: capture the step graph of both contact-buffer parities
         batch.step(8, 3, mode & ~eng.STEP_ASYNC)
     batch.wait()
@@ -409,6 +413,8 @@ def main():
     batch.wait()
     elapsed = time.perf_counter() - t0
     counts = batch.counts()
+    solver_path = {1: "colour sweeps", 2: "per-world workgroups", 3: "cells"}.get(
+        batch.get_option(eng.OPT_LAST_SOLVER), "unknown")
 
     kid = {"narrowphase": eng.KERNEL_NARROWPHASE, "contact_velocity": eng.KERNEL_CONTACT_VELOCITY,
            "contact_position": eng.KERNEL_CONTACT_POSITION, "pairs_write": eng.KERNEL_PAIRS_WRITE}[args.kernel]
@@ -468,7 +474,8 @@ def main():
                    "contacts": counts.contacts, "touching": counts.touching, "colours": counts.colours,
                    "steps_timed": f"{args.warmup + 1}-{args.warmup + args.steps}",
                    "launch": "eager" if args.eager else "hipGraph replay of the whole step",
-                   "graph_steps": counts.graph_steps, "serial_constraints": counts.serial},
+                   "graph_steps": counts.graph_steps, "serial_constraints": counts.serial,
+                   "solver": solver_path},
         "roofline": {"bound": "hbm", "kernel": args.kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (ratio * bytes_per_launch) if ratio else None,

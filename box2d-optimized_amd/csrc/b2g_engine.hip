@@ -52,7 +52,13 @@ int dalloc(T** p, size_t n) {
 constexpr int kColourRoundsPerStep = 4;
 // claim rounds of a captured step: a round with nothing pending returns at
 // once, so spare rounds cost only their launch
-constexpr int kGraphColourRounds = 16;  // enough for a world colouring from scratch
+// claim rounds of a captured step: the colouring is incremental (a contact
+// keeps its colour while it persists), so a replay colours only the new
+// contacts in these multi-workgroup rounds; a burst of new contacts (or a
+// world coloured afresh) is finished by up to kGraphFinishRounds rounds in
+// one workgroup, and anything still uncoloured joins the serial group
+constexpr int kGraphColourRounds = 4;
+constexpr int kGraphFinishRounds = 64;
 
 inline int grid_for(long long n, int max_blocks = 4096) {
   long long g = (n + kBlock - 1) / kBlock;
@@ -166,6 +172,7 @@ struct b2g_batch {
   bool per_world_ok = false; // joint-free, bodies and shapes laid out world after world
   bool worlds_contiguous = false;  // bodies and shapes laid out world after world
   int graph_tail_from = 0;         // first colour of the shared tail launch (from the last eager step)
+  int graph_big_items = 0;         // largest colour with a launch of its own (from the last eager step)
   int max_world_bodies = 0;
   int max_world_shapes = 0;
   int big_worlds = 0;              // worlds of more than kBigWorldBodies bodies (first in worder)
@@ -292,7 +299,7 @@ int b2g_batch::alloc() {
   // the cells solver
   A(d.bsflag, NB); A(d.bstime, NB); A(d.bwake, NB); A(d.bactive, NB); A(d.bncontact, NB); A(d.bjointed, NB);
   A(d.isl_awake, NB); A(d.isl_min, NB); A(d.isl_cons, NB);
-  A(d.part_geo, 2); A(d.part_boff, 2 * kCells + 1); A(d.part_bcur, 2 * kCells); A(d.part_acc, 5); A(d.part_sync, 4); A(d.part_blist, 2 * (size_t)NB); A(d.part_brank, NB);
+  A(d.part_geo, 1); A(d.part_boff, 2 * kCells + 1); A(d.part_bcur, 2 * kCells); A(d.part_acc, 8); A(d.part_sync, 4); A(d.part_blist, 2 * (size_t)NB); A(d.part_brank, NB);
   A(d.part_ccnt, kPartGroups); A(d.part_bucket, (size_t)2 * kCells * kCellCons); A(d.part_cbucket, NC);
   A(d.part_coff, (size_t)kPartGroups * (kColourSlots + 1)); A(d.part_claim, NB); A(d.part_used, NB);
   A(d.part_xcol, NC); A(d.sloc, NC);
@@ -332,7 +339,7 @@ int b2g_batch::alloc() {
   h_wshape0.assign(NW, 0);
   HIP_TRY(hipMemcpy(d.wshape0, h_wshape0.data(), NW * sizeof(int), hipMemcpyHostToDevice));
   {  // the cells solver's bounds accumulator starts empty
-    const unsigned acc[5] = {0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+    const unsigned acc[8] = {0xffffffffu, 0xffffffffu, 0u, 0u, 0u, 0u, 0u, 0u};
     HIP_TRY(hipMemcpy(d.part_acc, acc, sizeof(acc), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(d.part_sync, 0, 4 * sizeof(unsigned)));  // each launch leaves its counter at 0
   }
@@ -708,13 +715,11 @@ void b2g_batch::enqueue_collide() {
 // The cells solver (b2g_partition.h): a world's constraints split by the
 // cells of two staggered grids, three launches per iteration, no host read.
 // ---------------------------------------------------------------------------
-constexpr int kCellsMinBodies = 4096;  // B2G_SOLVER_AUTO: a single world at least this large
-
-bool b2g_batch::use_cells() const {
-  if (solver_mode == B2G_SOLVER_CELLS) return true;
-  if (solver_mode != B2G_SOLVER_AUTO) return false;
-  return d.nw == 1 && d.nb >= kCellsMinBodies;
-}
+// Only on request: on the 20,000-body pile the colour sweeps take 0.73 ms
+// per step (steps 6-25) and 0.89 ms (steps 21-500) against the cells' 0.93
+// and 1.06 ms, whose per-colour rounds inside a workgroup cost about as much
+// as a launch per colour, so B2G_SOLVER_AUTO does not choose them.
+bool b2g_batch::use_cells() const { return solver_mode == B2G_SOLVER_CELLS; }
 
 void b2g_batch::enqueue_cells_solve(int vi, int pi, bool finish_in_broadphase) {
   const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
@@ -771,7 +776,6 @@ void b2g_batch::enqueue_cells_solve(int vi, int pi, bool finish_in_broadphase) {
 int b2g_batch::cells_step(int vi, int pi, unsigned flags) {
   const bool prof_on = (flags & B2G_STEP_PROFILE) != 0;
   last_path = 3;
-  graph_K = std::max(graph_K, 1);  // a captured step needs no colour count from an eager one
   if (prof_on) HIP_TRY(hipEventRecord(ev[0], stream));
   int rc;
   if (flags & B2G_STEP_FIND_NEW_CONTACTS) {
@@ -890,8 +894,13 @@ void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, c
     while (tq > 0 && sizes[tq - 1] <= kTailColourMax && nnorm - tq < kTailMax) --tq;
     if (nnorm - tq < 2) tq = nnorm;  // one small colour: its own launch is as cheap
     graph_tail_from = tq < nnorm ? colours[tq] : (nnorm ? colours[nnorm - 1] + 1 : 0);
+    graph_big_items = 0;
+    for (int q = 0; q < tq; ++q) graph_big_items = std::max(graph_big_items, sizes[q]);
   } else if (nnorm > 0) {
-    tq = std::max(std::max(0, nnorm - kTailMax), std::min(nnorm - 1, graph_tail_from));
+    // captured steps: the colours the last eager step launched one by one,
+    // then one tail launch over every further colour slot (it reads their
+    // offsets on the device and skips the empty ones) and the serial group
+    tq = std::min(nnorm - 1, graph_tail_from);
   }
   TailSlots ts{};
   int tail_items = 0;
@@ -940,7 +949,14 @@ void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, c
   };
   // warm start: a gather per body when every colour has a body-table entry
   const int top_colour = nnorm ? colours[nnorm - 1] : -1;
-  if (top_colour < kBodyTab) {
+  if (tail) {  // captured steps: the gather, then the colours past the body table and the serial group
+    launch(B2G_KERNEL_WARM_START, -1, k_warm_gather, dim3(grid_for(d.nb)), dim3(kBlock), 0, d);
+    TailSlots wt{};
+    wt.c0 = kBodyTab;
+    wt.n = kMaxColours - kBodyTab;
+    wt.known = 0;
+    launch(kUntimed, -1, k_sweep_tail<kTailWarm>, dim3(1), dim3(kTailThreads), 0, d, wt, 0, 1);
+  } else if (top_colour < kBodyTab) {
     int items = 0;
     for (int q = 0; q < nnorm; ++q) items += std::max(sizes[q], 0);
     launch(B2G_KERNEL_WARM_START, tail ? -1 : items, k_warm_gather, dim3(grid_for(d.nb)), dim3(kBlock), 0, d);
@@ -1156,7 +1172,8 @@ int b2g_batch::step(int vi, int pi, unsigned flags) {
   if (collide_only) return collide_only_step(flags);
   const bool prof_on = (flags & B2G_STEP_PROFILE) != 0;
   const bool graph_ok = (flags & B2G_STEP_GRAPH) && !prof_on && timed_kernel < 0 && !d.sleep_on &&
-                        !(flags & B2G_STEP_FIND_NEW_CONTACTS) && graph_K > 0 &&
+                        !(flags & B2G_STEP_FIND_NEW_CONTACTS) &&
+                        (graph_K > 0 || use_cells()) &&  // a cells step needs no colour count from an eager one
                         (!use_per_world() || world_block > 0 || tuned_block > 0);
   if (graph_ok) return graph_step(vi, pi, flags);
   return eager_step(vi, pi, flags);
@@ -1406,7 +1423,13 @@ int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
     HIP_TRY(hipMemcpyAsync(pinned, d.cnt, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, stream));
     return B2G_OK;
   }
-  enqueue_collide_and_colour(K, kGraphColourRounds);
+  // per-world solver: colours below K; colour sweeps: any colour (those the
+  // last eager step did not launch alone go to the tail launch)
+  const int colour_limit = use_per_world() ? K : kMaxColours;
+  enqueue_collide_and_colour(colour_limit, kGraphColourRounds);
+  hipLaunchKernelGGL(k_colour_finish, dim3(1), dim3(kFinishThreads), 0, stream, d, rounds_pending,
+                     kGraphFinishRounds, colour_limit);
+  rounds_pending += kGraphFinishRounds;
   hipLaunchKernelGGL(k_colour_leftover, dim3(grid_for(std::max(est_cons, 256) * 5 / 4)), dim3(kBlock), 0, stream,
                      d);
   if (use_per_world()) {
@@ -1426,11 +1449,13 @@ int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
   enqueue_colour_groups();
   enqueue_islands_and_init();
   std::vector<int> colours, sizes;
-  for (int c = 0; c < K; ++c) {
+  for (int c = 0; c < kMaxColours; ++c) {
     colours.push_back(c);
     sizes.push_back(-1);
   }
-  const int per_colour = std::max(256, (int)(std::max(est_cons, 256) * 2LL / std::max(K, 1)));
+  // per-colour launches sized for the last eager step's largest colour and a
+  // half (a larger colour is still covered: the launches stride over it)
+  const int per_colour = std::max(256, graph_big_items + graph_big_items / 2);
   enqueue_solve(vi, pi, colours, sizes, {}, grid_for(per_colour), !(flags & B2G_STEP_SKIP_BROADPHASE));
   if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
     int rc = broadphase(false);
@@ -1491,7 +1516,7 @@ int b2g_batch::graph_step(int vi, int pi, unsigned flags) {
     swap_contact_buffers();
     pairs_current = true;
   }
-  if (!cells) round_host += kGraphColourRounds;
+  if (!cells) round_host += kGraphColourRounds + kGraphFinishRounds;
   ++graph_steps;
   if (!(flags & B2G_STEP_ASYNC)) {
     HIP_TRY(hipStreamSynchronize(stream));
@@ -1882,6 +1907,7 @@ int b2g_set_option(b2g_batch* b, int32_t option, int32_t value) {
   switch (option) {
     case B2G_OPT_SOLVER:
       if (value < B2G_SOLVER_AUTO || value > B2G_SOLVER_CELLS) return fail(B2G_ERR_INVALID, "solver mode");
+      if (b->solver_mode != value) b->graphs_clear();  // the captured steps are the other solver's
       b->solver_mode = value;
       break;
     case B2G_OPT_WORLD_LDS:

@@ -179,10 +179,11 @@ struct Dev {
   int* isl_cons;      // per island root: the island has constraints this step
   // the cells solver of large worlds (b2g_partition.h)
   int part_on;        // this step solves by cells: contact_init leaves the colour tables alone
-  float4* part_geo;   // 2: (lo.x, lo.y, 1/L, L), (cells x, cells y, all-cross) as int bits
+  float4* part_geo;   // 1: (lo.x, lo.y, 1/L, L) of the cells
   int* part_boff;     // 2 * kCells + 1: offsets of the cells' body lists (A cells, then B cells)
   int* part_bcur;     // 2 * kCells: fill cursors of those lists
-  unsigned* part_acc; // 5: bounds of the dynamic bodies' centres (ordered floats) and their count
+  unsigned* part_acc; // 8: bounds of the dynamic bodies' centres (ordered floats), their count, -,
+                      // and (words 6-7, 64 bits) the fixed-point sum of their footprints
   unsigned* part_sync;  // 4: finished-workgroup counters of the launches whose last workgroup goes on
   int* part_blist;    // 2 * nb: bodies of each cell
   int2* part_brank;   // per body: its slot in its A cell and in its B cell (-1: not dynamic)
@@ -617,6 +618,62 @@ __device__ __forceinline__ void world_colour_stats(const Dev& d, int w, int used
     d.wrecolour[w] = 0;
   } else if (used >= d.wfresh[w] + kRecolourSlack) {
     d.wrecolour[w] = 1;
+  }
+}
+
+// The claim rounds of a captured step that the multi-workgroup rounds left
+// unfinished, in one workgroup (barriers between the claim and resolve
+// passes instead of launches), up to `rounds` more rounds: the same claims,
+// priorities and colour choice as k_colour_claim / k_colour_resolve. The
+// claim words are read with atomic loads: other waves wrote them with
+// atomics. Launched with one workgroup of kFinishThreads threads.
+constexpr int kFinishThreads = 1024;
+__global__ void __launch_bounds__(kFinishThreads) k_colour_finish(Dev d, int local_round0, int rounds, int K) {
+  if (d.cnt[C_REMAIN] == 0) return;
+  const int n = d.cnt[C_PENDING];
+  const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
+  for (int r = 0; r < rounds; ++r) {
+    const int round = d.cnt[C_ROUND] + local_round0 + r;
+    for (int q = tid; q < n; q += nt) {
+      const int k = d.cpend[q];
+      if (d.ccolour[k] >= 0) continue;
+      const int2 pr = d.cpair[d.clist[k]];
+      const int ba = d.shdr[pr.x].y, bb = d.shdr[pr.y].y;
+      const unsigned long long v = colour_priority(d, round, pr);
+      if (is_dyn(d, ba)) atomicMax(&d.bslot[ba], v);
+      if (is_dyn(d, bb)) atomicMax(&d.bslot[bb], v);
+    }
+    __syncthreads();
+    int left = 0;
+    for (int q = tid; q < n; q += nt) {
+      const int k = d.cpend[q];
+      if (d.ccolour[k] >= 0) continue;
+      const int2 pr = d.cpair[d.clist[k]];
+      const int ba = d.shdr[pr.x].y, bb = d.shdr[pr.y].y;
+      const unsigned long long v = colour_priority(d, round, pr);
+      const bool da = is_dyn(d, ba), db = is_dyn(d, bb);
+      const bool win =
+          (!da || __hip_atomic_load(&d.bslot[ba], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == v) &&
+          (!db || __hip_atomic_load(&d.bslot[bb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == v);
+      if (!win) {
+        left = 1;
+        continue;
+      }
+      const unsigned long long used =
+          (da ? __hip_atomic_load(&d.bmask[ba], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull) |
+          (db ? __hip_atomic_load(&d.bmask[bb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull);
+      int col = kSerialColour;
+      if (~used != 0ull) col = __ffsll((long long)~used) - 1;
+      if (col >= K) col = kSerialColour;
+      d.ccolour[k] = col;
+      if (col < kMaxColours) {
+        const unsigned long long bit = 1ull << col;
+        if (da) atomicOr(&d.bmask[ba], bit);
+        if (db) atomicOr(&d.bmask[bb], bit);
+      }
+      atomicAdd(&d.cnt[C_REMAIN], -1);
+    }
+    if (!__syncthreads_or(left)) break;
   }
 }
 
@@ -1604,7 +1661,7 @@ __global__ void __launch_bounds__(kBlock) k_warm_gather(Dev d) {
 constexpr int kTailThreads = 512;
 constexpr int kTailMax = 16;  // colours per tail launch
 struct TailSlots {
-  int c0, n, known;
+  int c0, n, known;       // colours c0 .. c0 + n - 1 (n <= kTailMax when known)
   int off[kTailMax + 1];  // known: slot offsets of colours c0 .. c0 + n
 };
 enum TailOp : int { kTailWarm = 0, kTailVelocity = 1, kTailPosition = 2 };
@@ -1655,9 +1712,15 @@ __global__ void __launch_bounds__(kTailThreads) k_sweep_tail(Dev d, TailSlots t,
       }
     }
   } else {
+    __shared__ int s_off[kColourSlots + 1];  // offsets read on the device: all of them at once
+    if (!t.known) {
+      for (int q = tid; q <= t.n; q += nt) s_off[q] = d.col_off[t.c0 + q];
+      __syncthreads();
+    }
     for (int q = 0; q < t.n; ++q) {
-      const int begin = t.known ? t.off[q] : d.col_off[t.c0 + q];
-      const int end = t.known ? t.off[q + 1] : d.col_off[t.c0 + q + 1];
+      const int begin = t.known ? t.off[q] : s_off[q];
+      const int end = t.known ? t.off[q + 1] : s_off[q + 1];
+      if (end <= begin) continue;  // an empty colour (uniform across the workgroup)
       for (int base = begin; base < end; base += nt) {
         const int s = base + tid;
         const bool valid = s < end;

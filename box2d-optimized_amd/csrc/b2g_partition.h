@@ -33,25 +33,36 @@ constexpr int kPhaseStride = 1 << 16;     // reported colour: phase * kPhaseStri
 constexpr int kCellThreads = 128;         // workgroup of a cell (colouring and sweeps)
 constexpr int kCrossThreads = 512;        // workgroup of the cross group's sweeps
 
-// grid geometry: part_geo[0] = (lo.x, lo.y, 1/L, L), part_geo[1] = (Gx, Gy,
-// -, -) with the ints as bits
-__device__ __forceinline__ int part_coord(float v, float lo, float inv_l, float shift, int n) {
+// Grid geometry: part_geo[0] = (lo.x, lo.y, 1/L, L). Cell (i, j) of grid A
+// holds the centres with floor((p - lo) / L) = (i, j); grid B is shifted by
+// half a cell along both axes. The cells of each grid are hashed onto its
+// kCells buckets: cells far apart may share a bucket (one workgroup then
+// solves both, whose constraints are still internal to it), so the size of
+// the region the bodies cover does not bound the cell size.
+constexpr float kPartCoordMax = 1048576.0f;
+__device__ __forceinline__ int part_coord(float v, float lo, float inv_l, float shift) {
   const float f = (v - lo) * inv_l + shift;
-  return f >= 0.0f ? (int)fminf(f, (float)(n - 1)) : 0;  // NaN lands in 0
+  return f >= 0.0f ? (int)fminf(f, kPartCoordMax) : 0;  // NaN lands in 0
+}
+__device__ __forceinline__ int part_bucket(int i, int j, unsigned seed) {
+  unsigned h = (unsigned)i * 0x9E3779B1u ^ ((unsigned)j + seed) * 0x85EBCA77u;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  return (int)(h % (unsigned)kCells);
 }
 struct PartGrid {
   float lx, ly, inv_l;
-  int gx, gy;
   __device__ __forceinline__ int cell_a(float4 p) const {
-    return part_coord(p.x, lx, inv_l, 0.0f, gx) + gx * part_coord(p.y, ly, inv_l, 0.0f, gy);
+    return part_bucket(part_coord(p.x, lx, inv_l, 0.0f), part_coord(p.y, ly, inv_l, 0.0f), 0u);
   }
   __device__ __forceinline__ int cell_b(float4 p) const {
-    return part_coord(p.x, lx, inv_l, 0.5f, gx + 1) + (gx + 1) * part_coord(p.y, ly, inv_l, 0.5f, gy + 1);
+    return part_bucket(part_coord(p.x, lx, inv_l, 0.5f), part_coord(p.y, ly, inv_l, 0.5f), 0x5BD1E995u);
   }
 };
 __device__ __forceinline__ PartGrid part_grid(const Dev& d) {
-  const float4 g0 = d.part_geo[0], g1 = d.part_geo[1];
-  return PartGrid{g0.x, g0.y, g0.z, as_int(g1.x), as_int(g1.y)};
+  const float4 g0 = d.part_geo[0];
+  return PartGrid{g0.x, g0.y, g0.z};
 }
 __device__ __forceinline__ bool part_dynamic(const Dev& d, int b) { return d.btype[b] == kDynamicBody; }
 
@@ -99,9 +110,9 @@ __device__ __forceinline__ bool part_last_block(unsigned* counter) {
 //   k_part_bounds    bounds of the dynamic bodies' centres (ordered-float
 //                    atomics, one set per workgroup), accumulated in
 //                    part_acc; its last workgroup then sets the cell size for
-//                    about kCellTarget bodies per cell, grown until both
-//                    grids fit kCells cells, clears the histograms and
-//                    constraint counts, and resets part_acc for the next step;
+//                    about kCellTarget bodies per cell, clears the
+//                    histograms and constraint counts, and resets part_acc
+//                    for the next step;
 //   k_part_hist      bodies per cell of either grid; its last workgroup
 //                    turns them into the offsets of the cells' body lists
 //                    (A cells, then B cells);
@@ -116,14 +127,25 @@ __device__ __forceinline__ bool part_counts(const Dev& d, int b) {
 // Bounds: each lane over a grid stride, then the workgroup (shuffles, then
 // LDS across its waves), then one set of atomics per workgroup; the launch's
 // last workgroup goes on to the grid (part_grid_from_bounds). Launched with
-// at most kPartBoundsBlocks workgroups of kBlock threads.
+// at most kPartBoundsBlocks workgroups of kBlock threads. Beside the bounds
+// it sums the bodies' footprints, 8 I / m (the square around a disc of the
+// same radius of gyration), in fixed point so that the sum, and with it the
+// grid, does not depend on the order of the additions.
 constexpr int kPartBoundsBlocks = 64;
+constexpr float kPartAreaScale = 256.0f;  // fixed point of the footprint sum (1/256 m^2)
+__device__ __forceinline__ unsigned long long* part_area_acc(const Dev& d) {
+  return reinterpret_cast<unsigned long long*>(d.part_acc + 6);
+}
 __device__ __forceinline__ void part_grid_from_bounds(const Dev& d);
 __global__ void __launch_bounds__(kBlock) k_part_bounds(Dev d) {
   __shared__ float s_b[4][kBlock / kWave];
   __shared__ int s_n[kBlock / kWave];
+  __shared__ unsigned long long s_area;
+  if (threadIdx.x == 0) s_area = 0ull;
+  __syncthreads();
   float lx = INFINITY, ly = INFINITY, hx = -INFINITY, hy = -INFINITY;
   int n = 0;
+  unsigned long long area = 0ull;
   B2G_FOR(b, 0, d.nb) {
     if (!part_counts(d, b)) continue;
     const float4 p = d.pos[b];
@@ -133,8 +155,11 @@ __global__ void __launch_bounds__(kBlock) k_part_bounds(Dev d) {
       hx = fmaxf(hx, p.x);
       hy = fmaxf(hy, p.y);
       ++n;
+      const float4 pr = d.prop[b];  // inverse mass, inverse rotational inertia
+      if (pr.y > 0.0f) area += (unsigned long long)(fminf(8.0f * pr.x / pr.y, 1e6f) * kPartAreaScale);
     }
   }
+  if (area) atomicAdd(&s_area, area);
   for (int off = kWave / 2; off > 0; off >>= 1) {
     lx = fminf(lx, __shfl_xor(lx, off, kWave));
     ly = fminf(ly, __shfl_xor(ly, off, kWave));
@@ -164,6 +189,7 @@ __global__ void __launch_bounds__(kBlock) k_part_bounds(Dev d) {
       atomicMax(&d.part_acc[2], float_to_ordered(hx));
       atomicMax(&d.part_acc[3], float_to_ordered(hy));
       atomicAdd(&d.part_acc[4], (unsigned)n);
+      if (s_area) atomicAdd(part_area_acc(d), s_area);
     }
   }
   if (part_last_block(d.part_sync + 0)) part_grid_from_bounds(d);
@@ -176,6 +202,7 @@ __device__ __forceinline__ void part_acc_reset(const Dev& d) {
   d.part_acc[2] = 0u;
   d.part_acc[3] = 0u;
   d.part_acc[4] = 0u;
+  *part_area_acc(d) = 0ull;
 }
 
 __device__ __forceinline__ void part_grid_from_bounds(const Dev& d) {
@@ -190,12 +217,13 @@ __device__ __forceinline__ void part_grid_from_bounds(const Dev& d) {
       hy = ordered_to_float(d.part_acc[3]);
     }
     const float w = fminf(fmaxf(hx - lx, 1e-3f), 1e30f), h = fminf(fmaxf(hy - ly, 1e-3f), 1e30f);
-    float L = fmaxf(sqrtf(w * h * (float)kCellTarget / (float)(n > 0 ? n : 1)), 1e-3f);
-    // (in floats: the cell counts are converted once they are known to be small)
-    while ((w / L + 2.0f) * (h / L + 2.0f) > (float)kCells) L *= 1.25f;
-    const int gx = (int)(w / L) + 1, gy = (int)(h / L) + 1;
+    const float per = 1.0f / (float)(n > 0 ? n : 1);
+    // the cell for about kCellTarget bodies: of the mean footprint where the
+    // bodies are packed, of the bounds where they are spread out
+    const float l_bounds = sqrtf(w * h * (float)kCellTarget * per);
+    const float l_packed = sqrtf((float)*part_area_acc(d) / kPartAreaScale * (float)kCellTarget * per);
+    const float L = fmaxf(l_packed > 0.0f ? fminf(l_bounds, l_packed) : l_bounds, 1e-3f);
     d.part_geo[0] = make_float4(lx, ly, 1.0f / L, L);
-    d.part_geo[1] = make_float4(as_float(gx), as_float(gy), as_float(0), 0.0f);
     part_acc_reset(d);
   }
   for (int c = tid; c < 2 * kCells; c += blockDim.x) d.part_boff[c] = 0;  // the histogram, scanned in place

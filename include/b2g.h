@@ -281,17 +281,17 @@ enum {
 };
 int b2g_time_kernel(b2g_batch* batch, int32_t kernel);
 
-/* Solver layout and tuning. B2G_OPT_SOLVER: B2G_SOLVER_AUTO (default: cells
- * for a batch of one world of at least 4096 bodies; one workgroup per world
- * when the batch has several joint-free worlds of at most 8192 bodies laid out
- * world after world; else colour-major sweeps), B2G_SOLVER_COLOURS (one launch
+/* Solver layout and tuning. B2G_OPT_SOLVER: B2G_SOLVER_AUTO (default: one
+ * workgroup per world when the batch has several joint-free worlds of at most
+ * 8192 bodies laid out world after world; else colour-major sweeps),
+ * B2G_SOLVER_COLOURS (one launch
  * per colour over all worlds), B2G_SOLVER_WORLDS (one workgroup solves a
  * whole world, barriers between colours; needs that layout; the same results
  * as the colour sweeps) or B2G_SOLVER_CELLS (constraints split by the cells
  * of two staggered grids and a small cross group, each cell solved by one
  * workgroup with its bodies in LDS: three launches per iteration and no host
  * read per step; another valid constraint order, reported by
- * b2g_download_contact_colours as phase * 65536 + colour).
+ * b2g_download_contact_colours as phase * 65536 + colour; on request only).
  * B2G_OPT_WORLD_BLOCK: threads per world workgroup, 0 = autotune: the first
  * per-world steps time 64..1024 threads, each with the world's bodies in
  * global memory and staged in LDS (when the largest world fits 63 KB), one
