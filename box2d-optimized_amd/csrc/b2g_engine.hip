@@ -50,6 +50,10 @@ int dalloc(T** p, size_t n) {
 }
 
 constexpr int kColourRoundsPerStep = 4;
+// one world: two device-wide rounds, then the one-workgroup finishing pass
+// (k_colour_finish), whose rounds cost a barrier each instead of two launches
+constexpr int kColourRoundsOneWorld = 2;
+constexpr int kFinishMaxLeft = 4096;  // more left (a world colouring afresh): device-wide rounds instead
 // claim rounds of a captured step: a round with nothing pending returns at
 // once, so spare rounds cost only their launch
 // claim rounds of a captured step: the colouring is incremental (a contact
@@ -199,6 +203,7 @@ struct b2g_batch {
   void swap_contact_buffers();
   void graphs_clear();
   void enqueue_collide_and_colour(int K, int rounds);
+  void enqueue_collide_and_colour_eager();
   void enqueue_collide();
   bool use_cells() const;
   void enqueue_cells_solve(int vi, int pi, bool finish_in_broadphase);
@@ -687,6 +692,21 @@ void b2g_batch::enqueue_collide_and_colour(int K, int rounds) {
   const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
   hipLaunchKernelGGL(k_colour_init, dim3(gk), dim3(kBlock), 0, stream, d, K);
   for (int r = 0; r < rounds; ++r) enqueue_colour_round(K);
+}
+
+// An eager step's colouring: device-wide rounds; a single world finishes in
+// one workgroup (batches keep their rounds device-wide: worlds that colour
+// afresh would otherwise all land on one CU). The host still runs more
+// rounds if the read-back reports leftovers.
+void b2g_batch::enqueue_collide_and_colour_eager() {
+  if (d.nw == 1) {
+    enqueue_collide_and_colour(kMaxColours, kColourRoundsOneWorld);
+    hipLaunchKernelGGL(k_colour_finish, dim3(1), dim3(kFinishThreads), 0, stream, d, rounds_pending,
+                       kGraphFinishRounds, kMaxColours, kFinishMaxLeft);
+    rounds_pending += kGraphFinishRounds;
+  } else {
+    enqueue_collide_and_colour(kMaxColours, kColourRoundsPerStep);
+  }
 }
 
 // Collide (narrowphase), the bodies' velocity integration, and the compacted
@@ -1217,7 +1237,7 @@ int b2g_batch::sleep_step(int vi, int pi, unsigned flags) {
   }
   d.segs = pairs_current && worlds_contiguous ? 1 : 0;
   hipLaunchKernelGGL(k_sleep_reset, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
-  enqueue_collide_and_colour(kMaxColours, kColourRoundsPerStep);
+  enqueue_collide_and_colour_eager();
   if (prof_on) HIP_TRY(hipEventRecord(ev[1], stream));
   enqueue_colour_groups();
   enqueue_islands_and_init();
@@ -1282,7 +1302,7 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
   // the contact list is the last broadphase's pair list: world contact
   // ranges are segment offsets
   d.segs = pairs_current && worlds_contiguous ? 1 : 0;
-  enqueue_collide_and_colour(kMaxColours, kColourRoundsPerStep);
+  enqueue_collide_and_colour_eager();
   if (prof_on) HIP_TRY(hipEventRecord(ev[1], stream));
   int rc;
   if (use_per_world()) {
@@ -1428,7 +1448,7 @@ int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
   const int colour_limit = use_per_world() ? K : kMaxColours;
   enqueue_collide_and_colour(colour_limit, kGraphColourRounds);
   hipLaunchKernelGGL(k_colour_finish, dim3(1), dim3(kFinishThreads), 0, stream, d, rounds_pending,
-                     kGraphFinishRounds, colour_limit);
+                     kGraphFinishRounds, colour_limit, INT_MAX);  // no host loop follows a replay
   rounds_pending += kGraphFinishRounds;
   hipLaunchKernelGGL(k_colour_leftover, dim3(grid_for(std::max(est_cons, 256) * 5 / 4)), dim3(kBlock), 0, stream,
                      d);

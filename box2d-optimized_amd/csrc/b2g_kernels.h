@@ -626,10 +626,14 @@ __device__ __forceinline__ void world_colour_stats(const Dev& d, int w, int used
 // passes instead of launches), up to `rounds` more rounds: the same claims,
 // priorities and colour choice as k_colour_claim / k_colour_resolve. The
 // claim words are read with atomic loads: other waves wrote them with
-// atomics. Launched with one workgroup of kFinishThreads threads.
+// atomics. Launched with one workgroup of kFinishThreads threads. With more
+// than `max_left` constraints left it does nothing (a world colouring afresh
+// is faster in device-wide rounds, which the eager host loop then runs).
 constexpr int kFinishThreads = 1024;
-__global__ void __launch_bounds__(kFinishThreads) k_colour_finish(Dev d, int local_round0, int rounds, int K) {
-  if (d.cnt[C_REMAIN] == 0) return;
+__global__ void __launch_bounds__(kFinishThreads) k_colour_finish(Dev d, int local_round0, int rounds, int K,
+                                                                  int max_left) {
+  const int left0 = d.cnt[C_REMAIN];
+  if (left0 == 0 || left0 > max_left) return;
   const int n = d.cnt[C_PENDING];
   const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
   for (int r = 0; r < rounds; ++r) {
@@ -955,10 +959,14 @@ __device__ __forceinline__ size_t island_slot(const Dev& d, int isl, int it) {
   return (size_t)(it % 3) * d.nb + isl;
 }
 __device__ __forceinline__ bool island_done(const Dev& d, int isl, int it) {
-  if (d.solved[isl]) return true;
+  // the three words are loaded together (one dependent round trip, not two)
+  const size_t prev = island_slot(d, isl, it > 0 ? it - 1 : 0);
+  const int solved = d.solved[isl];
+  const unsigned bad = d.sepbad[prev];
+  const int jok = d.jok[prev];
+  if (solved) return true;
   if (it == 0) return false;
-  const size_t prev = island_slot(d, isl, it - 1);
-  if (d.sepbad[prev] == 0u && d.jok[prev] != 0) {
+  if (bad == 0u && jok != 0) {
     d.solved[isl] = 1;
     return true;
   }
