@@ -19,6 +19,7 @@
 
 #include "b2g.h"
 #include "b2g_kernels.h"
+#include "b2g_grid.h"
 #include "b2g_partition.h"
 
 using namespace b2g;
@@ -183,6 +184,11 @@ struct b2g_batch {
   int max_small_world_bodies = 0;  // largest of the others
   int solver_mode = 0;       // B2G_SOLVER_AUTO / _COLOURS / _WORLDS
   bool keep_idle = true;     // B2G_OPT_KEEP_IDLE
+  int pair_search = 0;       // B2G_OPT_PAIR_SEARCH
+  int last_pair_search = 0;  // 1 LBVH, 2 grid
+  bool use_grid() const {
+    return pair_search == 0 && d.grid_T > 0 && d.nw == 1 && static_shapes <= kGridStaticMax && d.ns >= 2;
+  }
   bool asleep_uploaded = false;  // a body was uploaded asleep: the awake flags apply from then on
   bool sleep_now() const {       // this step applies the awake flags and sleep timers
     if (asleep_uploaded) return true;
@@ -242,6 +248,8 @@ struct b2g_batch {
   }
   int build_tree(bool use_given_aabbs);
   int broadphase(bool use_given_aabbs);
+  int grid_pairs();
+  int finish_pair_list(bool use_given_aabbs);
   void refresh_filters();
   bool tree_stale = true;     // bodies or shapes changed since the last tree build
   // the contact list is the last broadphase's pair list (segments in d.poff_prev), and no body,
@@ -304,7 +312,13 @@ int b2g_batch::alloc() {
   // the cells solver
   A(d.bsflag, NB); A(d.bstime, NB); A(d.bwake, NB); A(d.bactive, NB); A(d.bncontact, NB); A(d.bjointed, NB);
   A(d.isl_awake, NB); A(d.isl_min, NB); A(d.isl_cons, NB);
-  A(d.part_geo, 1); A(d.part_boff, 2 * kCells + 1); A(d.part_bcur, 2 * kCells); A(d.part_acc, 8); A(d.part_sync, 4); A(d.part_blist, 2 * (size_t)NB); A(d.part_brank, NB);
+  A(d.part_geo, 1); A(d.part_boff, 2 * kCells + 1); A(d.part_bcur, 2 * kCells); A(d.part_acc, 8); A(d.part_sync, 4);
+  if (cap.worlds == 1) {  // the grid pair search of one world: about two buckets per shape
+    int t = 1024;
+    while (t < 2 * NS) t <<= 1;
+    d.grid_T = t;
+    A(d.grid_acc, 2); A(d.gstatic, kGridStaticMax); A(d.gcount, t + 1); A(d.goff, t + 1); A(d.gids, NS); A(d.gbox, NS); A(d.gfast, NS);
+  } A(d.part_blist, 2 * (size_t)NB); A(d.part_brank, NB);
   A(d.part_ccnt, kPartGroups); A(d.part_bucket, (size_t)2 * kCells * kCellCons); A(d.part_cbucket, NC);
   A(d.part_coff, (size_t)kPartGroups * (kColourSlots + 1)); A(d.part_claim, NB); A(d.part_used, NB);
   A(d.part_xcol, NC); A(d.sloc, NC);
@@ -525,6 +539,31 @@ void b2g_batch::refresh_filters() {
   filters_dirty = false;
   static_shapes = 0;
   for (int i = 0; i < d.ns; ++i) static_shapes += h_btype[h_shape_body[i]] == B2G_STATIC ? 1 : 0;
+  if (d.grid_T > 0 && d.ns > 0) {  // the grid's cell size and static list follow the shapes and body types
+    hipLaunchKernelGGL(k_grid_reset, dim3(1), dim3(1), 0, stream, d);
+    hipLaunchKernelGGL(k_grid_prepare, dim3(std::min(grid_for(d.ns), 64)), dim3(kBlock), 0, stream, d);
+  }
+}
+
+// The broadphase's pair search on the grid (b2g_grid.h): AABBs, buckets,
+// pairs into the slots, and the compacted pair list.
+int b2g_batch::grid_pairs() {
+  const int ns = d.ns;
+  const int gs = grid_for(ns);
+  hipLaunchKernelGGL(k_grid_aabb, dim3(std::max(gs, finish_pending ? grid_for(d.nb) : 1)), dim3(kBlock), 0, stream, d,
+                     finish_pending ? 1 : 0);
+  finish_pending = false;
+  hipLaunchKernelGGL(k_grid_count, dim3(gs), dim3(kBlock), 0, stream, d);
+  int rc = scan(d.gcount, nullptr, d.grid_T, d.goff, d.goff + d.grid_T, d.grid_T);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_grid_scatter, dim3(gs), dim3(kBlock), 0, stream, d);
+  const int gq = grid_for((long long)ns * kBlock / kQueryBlock);  // blocks of kQueryBlock threads
+  launch(B2G_KERNEL_PAIRS_WRITE, ns, k_grid_pairs<kPairSlots>, dim3(gq), dim3(kQueryBlock), 0, d);
+  if ((rc = scan(d.pcount, nullptr, ns, d.poff, d.cnt + C_PAIRS, ns))) return rc;
+  hipLaunchKernelGGL(k_grid_pairs_compact<kPairSlots>, dim3(gq), dim3(kQueryBlock), 0, stream, d);
+  tree_stale = true;  // the LBVH is not kept: a user query builds it first
+  last_pair_search = 2;
+  return B2G_OK;
 }
 
 int b2g_batch::broadphase(bool use_given_aabbs) {
@@ -542,6 +581,13 @@ int b2g_batch::broadphase(bool use_given_aabbs) {
   const bool keep =
       keep_idle && pairs_current && worlds_contiguous && !filters_dirty && !capturing && !use_given_aabbs;
   if (filters_dirty) refresh_filters();
+  if (!use_given_aabbs && use_grid()) {
+    d.idle = 0;
+    int rc = grid_pairs();
+    if (rc) return rc;
+    return finish_pair_list(use_given_aabbs);
+  }
+  last_pair_search = 1;
   d.idle = keep ? 1 : 0;
   int rc = build_tree(use_given_aabbs);
   if (rc) {
@@ -571,11 +617,17 @@ int b2g_batch::broadphase(bool use_given_aabbs) {
     hipLaunchKernelGGL((k_pairs_compact<kPairSlots, int>), dim3(gq), dim3(kQueryBlock), 0, stream, d);
   if (d.idle && idle_worlds() > 0)
     hipLaunchKernelGGL(k_keep_idle_pairs, dim3(std::min(d.nw, 4096)), dim3(kBlock), 0, stream, d);
+  return finish_pair_list(use_given_aabbs);
+}
+
+// The pair list's segments sorted, merged with the contact list, and the
+// buffers swapped.
+int b2g_batch::finish_pair_list(bool use_given_aabbs) {
   // one block per segment to sort: static shapes' segments, plus a few
   // over-full dynamic shapes
   hipLaunchKernelGGL(k_sort_pair_segments, dim3(std::min(std::max(static_shapes + 32, 64), 4096)), dim3(kBlock), 0,
                      stream, d);
-  hipLaunchKernelGGL(k_merge_contacts, dim3(grid_for(std::max(est_contacts, ns) * 5 / 4)), dim3(kBlock), 0, stream,
+  hipLaunchKernelGGL(k_merge_contacts, dim3(grid_for(std::max(est_contacts, d.ns) * 5 / 4)), dim3(kBlock), 0, stream,
                      d);
   if (d.events_on || d.sleep_on)  // EndContact events; wake-ups of the destroyed touching contacts' bodies
     hipLaunchKernelGGL(k_dead_contact_events, dim3(grid_for(std::max(est_contacts, 256))), dim3(kBlock), 0, stream,
@@ -1948,6 +2000,14 @@ int b2g_set_option(b2g_batch* b, int32_t option, int32_t value) {
       if (value < 0 || value > 1) return fail(B2G_ERR_INVALID, "keep idle: 0 or 1");
       b->keep_idle = value != 0;
       break;
+    case B2G_OPT_PAIR_SEARCH:
+      if (value < 0 || value > 1) return fail(B2G_ERR_INVALID, "pair search: 0 (grid for one world) or 1 (LBVH)");
+      if (value != b->pair_search) {
+        b->graphs_clear();  // the captured steps hold the other search
+        b->tree_stale = true;
+      }
+      b->pair_search = value;
+      break;
     default:
       return fail(B2G_ERR_INVALID, "unknown option");
   }
@@ -1970,6 +2030,8 @@ int b2g_get_option(b2g_batch* b, int32_t option, int32_t* value) {
     case B2G_OPT_WORLD_LDS: *value = b->world_lds; break;
     case B2G_OPT_TUNED_WORLD_LDS: *value = b->world_lds_now() ? 1 : 0; break;
     case B2G_OPT_KEEP_IDLE: *value = b->keep_idle ? 1 : 0; break;
+    case B2G_OPT_PAIR_SEARCH: *value = b->pair_search; break;
+    case B2G_OPT_LAST_PAIR_SEARCH: *value = b->last_pair_search; break;
     default: return fail(B2G_ERR_INVALID, "unknown option");
   }
   return B2G_OK;

@@ -94,6 +94,7 @@ EXPORTS = [
 SOLVER_AUTO, SOLVER_COLOURS, SOLVER_WORLDS, SOLVER_CELLS = 0, 1, 2, 3
 OPT_SOLVER, OPT_WORLD_BLOCK, OPT_TUNED_WORLD_BLOCK, OPT_LAST_SOLVER, OPT_PER_WORLD_ELIGIBLE = 1, 2, 3, 4, 5
 OPT_WORLD_LDS, OPT_TUNED_WORLD_LDS, OPT_KEEP_IDLE = 6, 7, 8
+OPT_PAIR_SEARCH, OPT_LAST_PAIR_SEARCH = 9, 10
 KERNEL_NARROWPHASE, KERNEL_WARM_START, KERNEL_CONTACT_VELOCITY, KERNEL_CONTACT_POSITION, \
     KERNEL_PAIRS_WRITE, KERNEL_SOLVER = range(6)
 

@@ -313,7 +313,13 @@ enum {
    * list is the same either way; 0 queries every world each step. Any
    * upload, and a step with B2G_STEP_SKIP_BROADPHASE, makes the next
    * broadphase query every world. */
-  B2G_OPT_KEEP_IDLE = 8
+  B2G_OPT_KEEP_IDLE = 8,
+  /* Pair search of the broadphase: 0 (default) a hashed uniform grid for a
+   * batch of one world with at most 64 static shapes, else the LBVH; 1 always
+   * the LBVH. The pair list is the same either way. B2G_OPT_LAST_PAIR_SEARCH
+   * (read-only): 1 LBVH, 2 grid in the last broadphase. */
+  B2G_OPT_PAIR_SEARCH = 9,
+  B2G_OPT_LAST_PAIR_SEARCH = 10
 };
 int b2g_set_option(b2g_batch* batch, int32_t option, int32_t value);
 int b2g_get_option(b2g_batch* batch, int32_t option, int32_t* value);

@@ -17,12 +17,13 @@ Tolerances:
     different Gauss-Seidel order moves one step.
 """
 import os
+import sys
 
 import numpy as np
 import pytest
 
 import oracle
-from util import REPO, bits, golden, load_pkg, manifold_fields_equal, scene_for
+from util import GOLDEN, REPO, bits, golden, load_pkg, manifold_fields_equal, scene_for
 
 pytestmark = pytest.mark.gpu
 S = load_pkg().scenes
@@ -90,6 +91,60 @@ def test_gpu_pair_set_static_and_crowded_shapes_vs_restatement(engine):
     print("pairs", len(ref), "largest segments", np.sort(per_shape)[-4:], flush=True)
     assert per_shape[0] > 100 and per_shape[3000] > 16  # static segments filled by partners
     assert total == len(ref) and np.array_equal(keys, ref)
+
+
+def _grid_scenes():
+    sys.path.insert(0, GOLDEN)
+    import make_golden
+    big = S.random_pile(1500, seed=41, half_w=25.0, height=60.0)
+    b = big.add_body(S.DYNAMIC, 0.0, 70.0)
+    big.add_box(b, 6.0, 0.5)  # one long dynamic box: the cell size follows it
+    small_static = S.random_pile(1200, seed=43, half_w=20.0, height=50.0)
+    for k in range(30):  # small static boxes among the dynamic ones
+        g = small_static.add_body(S.STATIC, -18.0 + 1.2 * k, 8.0 + (k % 5))
+        small_static.add_box(g, 0.3, 0.3)
+    many_static = S.random_pile(800, seed=47, half_w=20.0, height=40.0)
+    for k in range(70):  # more static shapes than the grid tests per query: the LBVH runs
+        g = many_static.add_body(S.STATIC, -19.0 + 0.55 * k, 5.0)
+        many_static.add_circle(g, 0.2)
+    return {
+        "pile4000": (S.random_pile(4000, seed=3, half_w=35.0, height=90.0), 2),
+        "mixed": (S.small_mixed(), 2),
+        "filtered": (make_golden.filtered_scene(), 2),
+        "chains": (make_golden.chains_scene(), 2),
+        "long_box": (big, 2),
+        "small_static": (small_static, 2),
+        "many_static": (many_static, 1),
+    }
+
+
+@pytest.mark.parametrize("name", list(_grid_scenes()))
+def test_gpu_grid_pair_search_equals_lbvh_bitexact(engine, name):
+    """The grid pair search of a one-world batch gives the LBVH's pair list:
+    over 120 steps of each scene the two runs keep bit-identical contact lists
+    (keys, manifolds, impulses) and bodies. The scenes cover filters, joints
+    that switch collision off, a long dynamic box (which sets the cell size),
+    small static shapes, and more static shapes than the grid takes (there
+    both runs use the LBVH)."""
+    sc, expect = _grid_scenes()[name]
+    runs = []
+    for search in (1, 0):
+        b = engine.Batch.from_scenes([sc])
+        b.set_option(engine.OPT_PAIR_SEARCH, search)
+        runs.append(b)
+    pairs = 0
+    for k in range(12):
+        for b in runs:
+            b.advance(10)
+        ca, cb = runs[0].contacts(), runs[1].contacts()
+        assert len(ca) == len(cb), f"step {10 * (k + 1)}: {len(ca)} vs {len(cb)} contacts"
+        assert np.array_equal(ca.view(np.uint8), cb.view(np.uint8)), f"step {10 * (k + 1)}: contact lists differ"
+        ba, bb = runs[0].bodies(), runs[1].bodies()
+        assert np.array_equal(ba.view(np.uint8), bb.view(np.uint8)), f"step {10 * (k + 1)}: bodies differ"
+        pairs = max(pairs, len(ca))
+    assert runs[0].get_option(engine.OPT_LAST_PAIR_SEARCH) == 1
+    assert runs[1].get_option(engine.OPT_LAST_PAIR_SEARCH) == expect
+    print(f"{name}: up to {pairs} contacts, search {expect}", flush=True)
 
 
 def _gpu_world_at(engine, sc, g):
