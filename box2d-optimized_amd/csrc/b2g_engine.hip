@@ -558,7 +558,11 @@ int b2g_batch::grid_pairs() {
   if (rc) return rc;
   hipLaunchKernelGGL(k_grid_scatter, dim3(gs), dim3(kBlock), 0, stream, d);
   const int gq = grid_for((long long)ns * kBlock / kQueryBlock);  // blocks of kQueryBlock threads
-  launch(B2G_KERNEL_PAIRS_WRITE, ns, k_grid_pairs<kPairSlots>, dim3(gq), dim3(kQueryBlock), 0, d);
+  // one-wave workgroups: a world of 20,000 shapes then spreads its queries
+  // over every CU (157 workgroups of 128 lanes left a third of them idle)
+  constexpr int kGridQueryThreads = 64;
+  launch(B2G_KERNEL_PAIRS_WRITE, ns, k_grid_pairs<kPairSlots, kGridQueryThreads>,
+         dim3(grid_for((long long)ns * kBlock / kGridQueryThreads)), dim3(kGridQueryThreads), 0, d);
   if ((rc = scan(d.pcount, nullptr, ns, d.poff, d.cnt + C_PAIRS, ns))) return rc;
   hipLaunchKernelGGL(k_grid_pairs_compact<kPairSlots>, dim3(gq), dim3(kQueryBlock), 0, stream, d);
   tree_stale = true;  // the LBVH is not kept: a user query builds it first

@@ -175,15 +175,16 @@ __device__ __forceinline__ void grid_query(const Dev& d, const GridGeom& gg, int
   };
   const long long cx = grid_coord(q.x, gg.inv), cy = grid_coord(q.y, gg.inv);
   // the 3 x 3 block's buckets, each once; their ranges loaded together
-  int lo[9], hi[9];
+  int hs[9], lo[9], hi[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) hs[c] = grid_bucket(cx + (c % 3) - 1, cy + (c / 3) - 1, gg.mask);
 #pragma unroll
   for (int c = 0; c < 9; ++c) {
-    const int h = grid_bucket(cx + (c % 3) - 1, cy + (c / 3) - 1, gg.mask);
     bool again = false;
 #pragma unroll
-    for (int e = 0; e < c; ++e) again |= grid_bucket(cx + (e % 3) - 1, cy + (e / 3) - 1, gg.mask) == h;
-    lo[c] = again ? 0 : d.goff[h];
-    hi[c] = again ? 0 : d.goff[h + 1];
+    for (int e = 0; e < c; ++e) again |= hs[e] == hs[c];
+    lo[c] = again ? 0 : d.goff[hs[c]];
+    hi[c] = again ? 0 : d.goff[hs[c] + 1];
   }
   // the candidates of the nine ranges as one sequence f = 0 .. total - 1,
   // kChunk boxes loaded before any is tested (position by compares, so the
@@ -239,9 +240,9 @@ __device__ __forceinline__ void grid_query(const Dev& d, const GridGeom& gg, int
 // As k_lbvh_pairs: up to kSlots sorted keys per shape, the counts past them;
 // lanes take the shapes in bucket order (neighbouring lanes, neighbouring
 // cells).
-template <int kSlots>
-__global__ void __launch_bounds__(kQueryBlock) k_grid_pairs(Dev d) {
-  __shared__ uint64_t slots[kSlots * kQueryBlock];  // thread-minor
+template <int kSlots, int kThreads>
+__global__ void __launch_bounds__(kThreads) k_grid_pairs(Dev d) {
+  __shared__ uint64_t slots[kSlots * kThreads];  // thread-minor
   const GridGeom gg = grid_geom(d);
   const int nq = d.goff[d.grid_T];  // binned shapes
   B2G_FOR(t, 0, nq) {
@@ -251,11 +252,11 @@ __global__ void __launch_bounds__(kQueryBlock) k_grid_pairs(Dev d) {
     grid_query(d, gg, i, [&](uint64_t key, int s) {
       if (found < kSlots) {
         int b = found - 1;  // sorted insert
-        while (b >= 0 && slots[b * kQueryBlock + tid] > key) {
-          slots[(b + 1) * kQueryBlock + tid] = slots[b * kQueryBlock + tid];
+        while (b >= 0 && slots[b * kThreads + tid] > key) {
+          slots[(b + 1) * kThreads + tid] = slots[b * kThreads + tid];
           --b;
         }
-        slots[(b + 1) * kQueryBlock + tid] = key;
+        slots[(b + 1) * kThreads + tid] = key;
       }
       ++found;
       if (s >= 0)
@@ -265,7 +266,7 @@ __global__ void __launch_bounds__(kQueryBlock) k_grid_pairs(Dev d) {
     });
     uint64_t* out = d.pslots + (size_t)i * kSlots;
     const int kept = found < kSlots ? found : kSlots;
-    for (int k = 0; k < kept; ++k) out[k] = slots[k * kQueryBlock + tid];
+    for (int k = 0; k < kept; ++k) out[k] = slots[k * kThreads + tid];
     d.pfound[i] = found;
     d.pcount[i] = own;
   }
