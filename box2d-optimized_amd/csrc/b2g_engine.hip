@@ -186,9 +186,8 @@ struct b2g_batch {
   bool keep_idle = true;     // B2G_OPT_KEEP_IDLE
   int pair_search = 0;       // B2G_OPT_PAIR_SEARCH
   int last_pair_search = 0;  // 1 LBVH, 2 grid
-  bool use_grid() const {
-    return pair_search == 0 && d.grid_T > 0 && d.nw == 1 && static_shapes <= kGridStaticMax && d.ns >= 2;
-  }
+  int world_static_max = 0;  // most static shapes in one world
+  bool use_grid() const { return pair_search == 0 && world_static_max <= kGridStaticMax && d.ns >= 2; }
   bool asleep_uploaded = false;  // a body was uploaded asleep: the awake flags apply from then on
   bool sleep_now() const {       // this step applies the awake flags and sleep timers
     if (asleep_uploaded) return true;
@@ -313,11 +312,12 @@ int b2g_batch::alloc() {
   A(d.bsflag, NB); A(d.bstime, NB); A(d.bwake, NB); A(d.bactive, NB); A(d.bncontact, NB); A(d.bjointed, NB);
   A(d.isl_awake, NB); A(d.isl_min, NB); A(d.isl_cons, NB);
   A(d.part_geo, 1); A(d.part_boff, 2 * kCells + 1); A(d.part_bcur, 2 * kCells); A(d.part_acc, 8); A(d.part_sync, 4);
-  if (cap.worlds == 1) {  // the grid pair search of one world: about two buckets per shape
+  {  // the grid pair search: about two buckets per shape
     int t = 1024;
     while (t < 2 * NS) t <<= 1;
     d.grid_T = t;
-    A(d.grid_acc, 2); A(d.gstatic, kGridStaticMax); A(d.gcount, t + 1); A(d.goff, t + 1); A(d.gids, NS); A(d.gbox, NS); A(d.gfast, NS);
+    A(d.grid_acc, 1); A(d.gstatic, NS); A(d.gsoff, NW + 1); A(d.gcount, t + 1); A(d.goff, t + 1);
+    A(d.gids, NS); A(d.gbox, NS); A(d.gfast, NS); A(d.gworld, NS);
   } A(d.part_blist, 2 * (size_t)NB); A(d.part_brank, NB);
   A(d.part_ccnt, kPartGroups); A(d.part_bucket, (size_t)2 * kCells * kCellCons); A(d.part_cbucket, NC);
   A(d.part_coff, (size_t)kPartGroups * (kColourSlots + 1)); A(d.part_claim, NB); A(d.part_used, NB);
@@ -539,9 +539,26 @@ void b2g_batch::refresh_filters() {
   filters_dirty = false;
   static_shapes = 0;
   for (int i = 0; i < d.ns; ++i) static_shapes += h_btype[h_shape_body[i]] == B2G_STATIC ? 1 : 0;
-  if (d.grid_T > 0 && d.ns > 0) {  // the grid's cell size and static list follow the shapes and body types
+  if (d.ns > 0) {  // the grid's cell size and static lists follow the shapes and body types
     hipLaunchKernelGGL(k_grid_reset, dim3(1), dim3(1), 0, stream, d);
     hipLaunchKernelGGL(k_grid_prepare, dim3(std::min(grid_for(d.ns), 64)), dim3(kBlock), 0, stream, d);
+    const int nw = std::max(d.nw, 1);
+    auto world_of = [&](int b) { return std::min(std::max(h_bworld[b], 0), nw - 1); };
+    std::vector<int> off(nw + 1, 0);
+    for (int i = 0; i < d.ns; ++i)
+      if (h_btype[h_shape_body[i]] == B2G_STATIC) off[world_of(h_shape_body[i]) + 1]++;
+    world_static_max = 0;
+    for (int w = 0; w < nw; ++w) {
+      world_static_max = std::max(world_static_max, off[w + 1]);
+      off[w + 1] += off[w];
+    }
+    std::vector<int> list(std::max(off[nw], 1), 0), cur(off.begin(), off.end() - 1);
+    for (int i = 0; i < d.ns; ++i)  // world after world, shape order inside each
+      if (h_btype[h_shape_body[i]] == B2G_STATIC) list[cur[world_of(h_shape_body[i])]++] = i;
+    (void)hipMemcpyAsync(d.gsoff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, stream);
+    if (off[nw] > 0)
+      (void)hipMemcpyAsync(d.gstatic, list.data(), off[nw] * sizeof(int), hipMemcpyHostToDevice, stream);
+    (void)hipStreamSynchronize(stream);  // the host vectors end here
   }
 }
 
@@ -550,21 +567,32 @@ void b2g_batch::refresh_filters() {
 int b2g_batch::grid_pairs() {
   const int ns = d.ns;
   const int gs = grid_for(ns);
-  hipLaunchKernelGGL(k_grid_aabb, dim3(std::max(gs, finish_pending ? grid_for(d.nb) : 1)), dim3(kBlock), 0, stream, d,
+  // buckets for about two per binned shape: with idle worlds kept, only the
+  // active worlds' shapes are binned (a serving tick has a few percent)
+  long long binned = ns;
+  if (d.idle && worlds_contiguous) {
+    binned = 0;
+    for (int w = 0; w < d.nw && w < (int)h_wshape1.size(); ++w)
+      if (h_params[w].active) binned += h_wshape1[w] - h_wshape0[w];
+  }
+  Dev dg = d;
+  dg.grid_T = 1024;
+  while (dg.grid_T < 2 * binned && dg.grid_T < d.grid_T) dg.grid_T <<= 1;
+  hipLaunchKernelGGL(k_grid_aabb, dim3(std::max(gs, finish_pending ? grid_for(d.nb) : 1)), dim3(kBlock), 0, stream, dg,
                      finish_pending ? 1 : 0);
   finish_pending = false;
-  hipLaunchKernelGGL(k_grid_count, dim3(gs), dim3(kBlock), 0, stream, d);
-  int rc = scan(d.gcount, nullptr, d.grid_T, d.goff, d.goff + d.grid_T, d.grid_T);
+  hipLaunchKernelGGL(k_grid_count, dim3(gs), dim3(kBlock), 0, stream, dg);
+  int rc = scan(dg.gcount, nullptr, dg.grid_T, dg.goff, dg.goff + dg.grid_T, dg.grid_T);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_grid_scatter, dim3(gs), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_grid_scatter, dim3(gs), dim3(kBlock), 0, stream, dg);
   const int gq = grid_for((long long)ns * kBlock / kQueryBlock);  // blocks of kQueryBlock threads
   // one-wave workgroups: a world of 20,000 shapes then spreads its queries
   // over every CU (157 workgroups of 128 lanes left a third of them idle)
   constexpr int kGridQueryThreads = 64;
   launch(B2G_KERNEL_PAIRS_WRITE, ns, k_grid_pairs<kPairSlots, kGridQueryThreads>,
-         dim3(grid_for((long long)ns * kBlock / kGridQueryThreads)), dim3(kGridQueryThreads), 0, d);
+         dim3(grid_for(binned * kBlock / kGridQueryThreads)), dim3(kGridQueryThreads), 0, dg);
   if ((rc = scan(d.pcount, nullptr, ns, d.poff, d.cnt + C_PAIRS, ns))) return rc;
-  hipLaunchKernelGGL(k_grid_pairs_compact<kPairSlots>, dim3(gq), dim3(kQueryBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_grid_pairs_compact<kPairSlots>, dim3(gq), dim3(kQueryBlock), 0, stream, dg);
   tree_stale = true;  // the LBVH is not kept: a user query builds it first
   last_pair_search = 2;
   return B2G_OK;
@@ -586,9 +614,14 @@ int b2g_batch::broadphase(bool use_given_aabbs) {
       keep_idle && pairs_current && worlds_contiguous && !filters_dirty && !capturing && !use_given_aabbs;
   if (filters_dirty) refresh_filters();
   if (!use_given_aabbs && use_grid()) {
-    d.idle = 0;
+    d.idle = keep ? 1 : 0;
     int rc = grid_pairs();
-    if (rc) return rc;
+    if (rc) {
+      d.idle = 0;
+      return rc;
+    }
+    if (d.idle && idle_worlds() > 0)
+      hipLaunchKernelGGL(k_keep_idle_pairs, dim3(std::min(d.nw, 4096)), dim3(kBlock), 0, stream, d);
     return finish_pair_list(use_given_aabbs);
   }
   last_pair_search = 1;
@@ -2005,7 +2038,7 @@ int b2g_set_option(b2g_batch* b, int32_t option, int32_t value) {
       b->keep_idle = value != 0;
       break;
     case B2G_OPT_PAIR_SEARCH:
-      if (value < 0 || value > 1) return fail(B2G_ERR_INVALID, "pair search: 0 (grid for one world) or 1 (LBVH)");
+      if (value < 0 || value > 1) return fail(B2G_ERR_INVALID, "pair search: 0 (grid where it applies) or 1 (LBVH)");
       if (value != b->pair_search) {
         b->graphs_clear();  // the captured steps hold the other search
         b->tree_stale = true;

@@ -1,5 +1,6 @@
-// b2g_grid.h -- the broadphase's pair search for a batch of one world, on a
-// hashed uniform grid instead of the LBVH.
+// b2g_grid.h -- the broadphase's pair search on a hashed uniform grid
+// instead of the LBVH, for any batch whose worlds have at most
+// kGridStaticMax static shapes each.
 //
 // The pair set is the same: every pair of shapes whose AABBs overlap
 // (b2TestOverlap, inclusive) and that pair_allowed admits, each found once
@@ -14,12 +15,15 @@
 // have lower corners less than G apart on each axis, so their cells (taken
 // at the lower corner, in double precision) are equal or adjacent: a shape
 // finds every non-static partner in the 3 x 3 cells around its own. Cells
-// are hashed onto a power-of-two table; cells that share a bucket only cost
-// extra overlap tests, and a bucket reached twice from one 3 x 3 block is
-// scanned once. Static shapes (grounds, walls: any size) are not binned: at
-// most kGridStaticMax of them are tested by every query.
+// (of each world: the world is part of the hash) are hashed onto a
+// power-of-two table; cells that share a bucket only cost extra overlap
+// tests, and a bucket reached twice from one 3 x 3 block is scanned once.
+// Static shapes (grounds, walls: any size) are not binned: a query tests its
+// world's, at most kGridStaticMax of them. A world that is not active in the
+// step keeps its pairs (its shapes are neither binned nor queried; the pair
+// list's blocks of such worlds are copied whole, as with the LBVH).
 //
-// Steps of a broadphase (one world, eager or captured):
+// Steps of a broadphase (eager or captured):
 //   k_grid_aabb     shape AABBs (and the step's finish, as k_aabb), zeroed
 //                   pair counts and bucket counts;
 //   k_grid_count    shapes per bucket;  scan -> bucket offsets;
@@ -28,7 +32,7 @@
 //   scan of the pair counts; k_grid_pairs_compact (over-full shapes query
 //   again straight into place). The segment sort and the merge follow as for
 //   the LBVH. When shapes or body types change, k_grid_prepare recomputes G
-//   and the static list.
+//   and the host lists each world's static shapes.
 #pragma once
 
 #include "b2g_kernels.h"
@@ -60,25 +64,16 @@ __device__ __forceinline__ float shape_reach(int type, int count, const float4* 
 }
 
 __global__ void k_grid_reset(Dev d) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    d.grid_acc[0] = 0u;  // ordered-float maximum reach
-    d.grid_acc[1] = 0u;  // static shapes listed
-  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.grid_acc[0] = 0u;  // ordered-float maximum reach
 }
 
-// the maximum reach of the non-static shapes (one atomic per block) and the
-// list of static shapes
+// the maximum reach of the non-static shapes (one atomic per block)
 __global__ void __launch_bounds__(kBlock) k_grid_prepare(Dev d) {
   __shared__ float s_m[kBlock / kWave];
   float m = 0.0f;
   B2G_FOR(i, 0, d.ns) {
     const int4 h = d.shdr[i];
-    if (d.btype[h.y] == kStaticBody) {
-      const unsigned k = atomicAdd(&d.grid_acc[1], 1u);
-      if (k < (unsigned)kGridStaticMax) d.gstatic[k] = i;
-    } else {
-      m = fmaxf(m, shape_reach(h.x, h.z, d.geo + 9 * (size_t)i));
-    }
+    if (d.btype[h.y] != kStaticBody) m = fmaxf(m, shape_reach(h.x, h.z, d.geo + 9 * (size_t)i));
   }
   m = wave_max_all(m);
   if (lane_id() == 0) s_m[wave_id()] = m;
@@ -107,16 +102,22 @@ __device__ __forceinline__ long long grid_coord(float v, double inv) {
 // the tile) and tiles are hashed: shapes sorted by bucket are sorted by place
 // within a tile, so the lanes of a wave query neighbouring cells and share
 // their cache lines.
-__device__ __forceinline__ int grid_bucket(long long cx, long long cy, unsigned mask) {
+__device__ __forceinline__ int grid_bucket(long long cx, long long cy, int w, unsigned mask) {
   const unsigned long long tx = (unsigned long long)(cx >> 3), ty = (unsigned long long)(cy >> 3);
-  const unsigned long long h = tx * 0x9E3779B97F4A7C15ull ^ (ty + 0x632BE59BD9B4E019ull) * 0xC2B2AE3D27D4EB4Full;
+  const unsigned long long h = tx * 0x9E3779B97F4A7C15ull ^ (ty + 0x632BE59BD9B4E019ull) * 0xC2B2AE3D27D4EB4Full ^
+                               ((unsigned long long)(unsigned)w + 0x165667B19E3779F9ull) * 0xD6E8FEB86659FD93ull;
   const unsigned x = (unsigned)cx & 7u, y = (unsigned)cy & 7u;
   const unsigned m = (x & 1u) | ((y & 1u) << 1) | ((x & 2u) << 1) | ((y & 2u) << 2) | ((x & 4u) << 2) | ((y & 4u) << 3);
   return (int)((((h ^ (h >> 29)) << 6) | m) & mask);
 }
-__device__ __forceinline__ bool grid_binned(const Dev& d, int i) { return (d.sfast[i].y & 64) == 0; }
+// binned: a non-static shape of a world that is active in the step
+__device__ __forceinline__ bool grid_binned(const Dev& d, int i) {
+  return (d.sfast[i].y & 64) == 0 && !idle_world(d, d.shdr[i].w);
+}
 
-// AABBs, the step's finish (as k_aabb), and cleared pair and bucket counts
+// AABBs, the step's finish (as k_aabb), and cleared pair and bucket counts;
+// an idle world's shape keeps its box and takes its old segment's length as
+// its pair count
 __global__ void __launch_bounds__(kBlock) k_grid_aabb(Dev d, int finish) {
   if (blockIdx.x == 0 && threadIdx.x == 0) d.cnt[C_HEAVY] = 0;
   if (finish) finish_step(d);
@@ -124,9 +125,15 @@ __global__ void __launch_bounds__(kBlock) k_grid_aabb(Dev d, int finish) {
     d.gcount[c] = 0;
   B2G_FOR(i, 0, d.ns) {
     const int4 h = d.shdr[i];
+    if (idle_world(d, h.w)) {
+      const int2 seg = prev_segment(d, i);
+      d.pcount[i] = seg.y - seg.x;
+      continue;
+    }
     const float4 xf = finish && synced_body(d, h.y) ? synced_xf(d, h.y) : d.xf[h.y];
     d.aabb[i] = shape_aabb(h.x, h.z, d.geo + 9 * (size_t)i, xform_of(xf));
     d.pcount[i] = 0;
+    d.pfound[i] = 0;  // a static shape's fill cursor; a queried shape's count is rewritten
   }
 }
 
@@ -135,7 +142,7 @@ __global__ void __launch_bounds__(kBlock) k_grid_count(Dev d) {
   B2G_FOR(i, 0, d.ns) {
     if (!grid_binned(d, i)) continue;
     const float4 b = d.aabb[i];
-    atomicAdd(&d.gcount[grid_bucket(grid_coord(b.x, gg.inv), grid_coord(b.y, gg.inv), gg.mask)], 1);
+    atomicAdd(&d.gcount[grid_bucket(grid_coord(b.x, gg.inv), grid_coord(b.y, gg.inv), d.shdr[i].w, gg.mask)], 1);
   }
 }
 
@@ -147,11 +154,12 @@ __global__ void __launch_bounds__(kBlock) k_grid_scatter(Dev d) {
   B2G_FOR(i, 0, d.ns) {
     if (!grid_binned(d, i)) continue;
     const float4 b = d.aabb[i];
-    const int h = grid_bucket(grid_coord(b.x, gg.inv), grid_coord(b.y, gg.inv), gg.mask);
+    const int h = grid_bucket(grid_coord(b.x, gg.inv), grid_coord(b.y, gg.inv), d.shdr[i].w, gg.mask);
     const int at = d.goff[h] + atomicSub(&d.gcount[h], 1) - 1;
     d.gids[at] = i;
     d.gbox[at] = b;
     d.gfast[at] = d.sfast[i];
+    d.gworld[at] = d.shdr[i].w;
   }
 }
 
@@ -173,11 +181,12 @@ __device__ __forceinline__ void grid_query(const Dev& d, const GridGeom& gg, int
     else
       emit(((uint64_t)(uint32_t)i << 32) | (uint32_t)j, -1);
   };
+  const int w = d.shdr[i].w;
   const long long cx = grid_coord(q.x, gg.inv), cy = grid_coord(q.y, gg.inv);
   // the 3 x 3 block's buckets, each once; their ranges loaded together
   int hs[9], lo[9], hi[9];
 #pragma unroll
-  for (int c = 0; c < 9; ++c) hs[c] = grid_bucket(cx + (c % 3) - 1, cy + (c / 3) - 1, gg.mask);
+  for (int c = 0; c < 9; ++c) hs[c] = grid_bucket(cx + (c % 3) - 1, cy + (c / 3) - 1, w, gg.mask);
 #pragma unroll
   for (int c = 0; c < 9; ++c) {
     bool again = false;
@@ -206,25 +215,28 @@ __device__ __forceinline__ void grid_query(const Dev& d, const GridGeom& gg, int
         if (f >= pre[c] && f < pre[c + 1]) p = lo[c] + (f - pre[c]);
       pos[u] = p;
     }
-    int ids[kChunk];
+    int ids[kChunk], ws[kChunk];
     int4 fs[kChunk];
 #pragma unroll
-    for (int u = 0; u < kChunk; ++u) {  // the candidates' boxes, indices and filter words, all in flight
-      const int at = pos[u] >= 0 ? pos[u] : lo[0];
+    for (int u = 0; u < kChunk; ++u) {  // the candidates' boxes, indices, filter words and worlds, all in flight
+      const int at = pos[u] >= 0 ? pos[u] : 0;  // (entry 0 exists: the querying shape has one)
       bx[u] = d.gbox[at];
       ids[u] = d.gids[at];
       fs[u] = d.gfast[at];
+      ws[u] = d.gworld[at];
     }
+    // (a bucket can hold cells of other worlds: their shapes are never partners)
 #pragma unroll
     for (int u = 0; u < kChunk; ++u)
-      if (pos[u] >= 0 && overlap(q, bx[u])) test(ids[u], fs[u]);
+      if (pos[u] >= 0 && ws[u] == w && overlap(q, bx[u])) test(ids[u], fs[u]);
   }
-  const int nst = min((int)d.grid_acc[1], kGridStaticMax);
-  for (int k0 = 0; k0 < nst; k0 += 4) {
+  // the world's static shapes
+  const int s0 = d.gsoff[w], s1 = d.gsoff[w + 1];
+  for (int k0 = s0; k0 < s1; k0 += 4) {
     int js[4];
     float4 sb[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) js[u] = k0 + u < nst ? d.gstatic[k0 + u] : -1;
+    for (int u = 0; u < 4; ++u) js[u] = k0 + u < s1 ? d.gstatic[k0 + u] : -1;
     int4 sf[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -270,10 +282,6 @@ __global__ void __launch_bounds__(kThreads) k_grid_pairs(Dev d) {
     d.pfound[i] = found;
     d.pcount[i] = own;
   }
-  // a static shape: no query; its fill cursor starts at 0
-  const int nst = min((int)d.grid_acc[1], kGridStaticMax);
-  if (blockIdx.x == 0)
-    for (int k = (int)threadIdx.x; k < nst; k += (int)blockDim.x) d.pfound[d.gstatic[k]] = 0;
 }
 
 // As k_pairs_compact, with the grid query for over-full shapes.
@@ -281,6 +289,8 @@ template <int kSlots>
 __global__ void __launch_bounds__(kQueryBlock) k_grid_pairs_compact(Dev d) {
   const GridGeom gg = grid_geom(d);
   B2G_FOR(i, 0, d.ns) {
+    // an idle world's block of pairs is copied whole (k_keep_idle_pairs)
+    if (idle_world(d, d.shdr[i].w)) continue;
     const int own = d.pcount[i];
     const int base = d.poff[i];
     const bool stat = (d.sfast[i].y & 64) != 0;

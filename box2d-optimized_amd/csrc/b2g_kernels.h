@@ -182,15 +182,17 @@ struct Dev {
   float4* part_geo;   // 1: (lo.x, lo.y, 1/L, L) of the cells
   int* part_boff;     // 2 * kCells + 1: offsets of the cells' body lists (A cells, then B cells)
   int* part_bcur;     // 2 * kCells: fill cursors of those lists
-  // the grid pair search of one world (b2g_grid.h)
-  int grid_T;         // buckets (a power of two; 0: not allocated)
-  unsigned* grid_acc; // 2: ordered-float maximum reach of the non-static shapes, static shapes listed
-  int* gstatic;       // kGridStaticMax: the static shapes
+  // the grid pair search (b2g_grid.h)
+  int grid_T;         // buckets (a power of two)
+  unsigned* grid_acc; // 1: ordered-float maximum reach of the non-static shapes
+  int* gstatic;       // the static shapes, world after world
+  int* gsoff;         // nw + 1: each world's range of gstatic
   int* gcount;        // grid_T + 1: shapes per bucket (back to 0 after the scatter)
   int* goff;          // grid_T + 1: bucket offsets, then the binned shape count
   int* gids;          // per binned shape, by bucket
   float4* gbox;       // their AABBs, in the same order
   int4* gfast;        // their pair-filter words, in the same order
+  int* gworld;        // their worlds, in the same order
   unsigned* part_acc; // 8: bounds of the dynamic bodies' centres (ordered floats), their count, -,
                       // and (words 6-7, 64 bits) the fixed-point sum of their footprints
   unsigned* part_sync;  // 4: finished-workgroup counters of the launches whose last workgroup goes on
