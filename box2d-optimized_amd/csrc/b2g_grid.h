@@ -17,7 +17,9 @@
 // finds every non-static partner in the 3 x 3 cells around its own. Cells
 // (of each world: the world is part of the hash) are hashed onto a
 // power-of-two table; cells that share a bucket only cost extra overlap
-// tests, and a bucket reached twice from one 3 x 3 block is scanned once.
+// tests (entries of another world in a shared bucket are skipped by their
+// world index), and a bucket reached twice from one 3 x 3 block is scanned
+// once.
 // Static shapes (grounds, walls: any size) are not binned: a query tests its
 // world's, at most kGridStaticMax of them. A world that is not active in the
 // step keeps its pairs (its shapes are neither binned nor queried; the pair

@@ -314,8 +314,8 @@ enum {
    * upload, and a step with B2G_STEP_SKIP_BROADPHASE, makes the next
    * broadphase query every world. */
   B2G_OPT_KEEP_IDLE = 8,
-  /* Pair search of the broadphase: 0 (default) a hashed uniform grid for a
-   * batch of one world with at most 64 static shapes, else the LBVH; 1 always
+  /* Pair search of the broadphase: 0 (default) a hashed uniform grid when no
+   * world of the batch has more than 64 static shapes, else the LBVH; 1 always
    * the LBVH. The pair list is the same either way. B2G_OPT_LAST_PAIR_SEARCH
    * (read-only): 1 LBVH, 2 grid in the last broadphase. */
   B2G_OPT_PAIR_SEARCH = 9,

@@ -147,6 +147,42 @@ def test_gpu_grid_pair_search_equals_lbvh_bitexact(engine, name):
     print(f"{name}: up to {pairs} contacts, search {expect}", flush=True)
 
 
+def test_gpu_grid_pair_search_in_a_batch_equals_lbvh_bitexact(engine):
+    """A batch of eight worlds laid over the same place (their shapes share
+    grid buckets, so the world check is what keeps them apart), stepped on a
+    schedule where worlds sit out steps and keep their pairs: the grid and the
+    LBVH give bit-identical contact lists after every step and the same
+    bodies."""
+    from test_gpu_batch import _mixed_scenes
+    scenes = _mixed_scenes()
+    steps = 30
+    sched = np.random.default_rng(11).random((steps, len(scenes))) < 0.5
+    sched[:4, :] = True
+    sched[10, :] = False  # every world idle
+    dt = np.float32(1.0 / 60.0)
+    ratio = np.float32(np.float32(np.float32(1.0) / dt) * dt)
+    runs = []
+    for search in (1, 0):
+        b = engine.Batch.from_scenes(scenes)
+        b.set_option(engine.OPT_PAIR_SEARCH, search)
+        b.advance(2)
+        lists = []
+        for t in range(steps):
+            b.set_world_params(0, [engine.WorldParams(g, dt, ratio, 1, int(sched[t, w]))
+                                   for w, g in enumerate(b.gravity)])
+            b.step(8, 3, 0)
+            lists.append(b.contacts())
+        assert b.get_option(engine.OPT_LAST_PAIR_SEARCH) == 1 + (1 - search)
+        runs.append((lists, b.bodies()))
+    (tl, tb), (gl, gb) = runs
+    for t in range(steps):
+        assert len(tl[t]) == len(gl[t]) and len(tl[t]) > 0, t
+        assert tl[t].tobytes() == gl[t].tobytes(), f"contact list differs after step {t}"
+    assert tb.tobytes() == gb.tobytes()
+    print(f"{len(scenes)} worlds, {int(sched.sum())} of {sched.size} world-steps active, "
+          f"{len(gl[-1])} contacts at the end", flush=True)
+
+
 def _gpu_world_at(engine, sc, g):
     """GPU world built by the drop-in API, then set to the reference state."""
     w = engine.World(sc)
