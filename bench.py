@@ -160,26 +160,30 @@ def batched_section(pkg, worlds: int, warmup: int, steps: int, sample_every: int
     scenes = [S.batch_world(i, int(sizes[i])) for i in range(worlds)]
     batch = eng.Batch.from_scenes(scenes)
     build_s = time.perf_counter() - t0
-    batch.advance(max(warmup - 2, 1))
-    for _ in range(2):  # capture the step graph of both contact-buffer parities
-        batch.step(8, 3, mode & ~eng.STEP_ASYNC)
-    batch.wait()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        batch.step(8, 3, mode)
-    batch.wait()
-    el = time.perf_counter() - t0
+
+    def timed(b, m):
+        b.advance(max(warmup - 2, 1))
+        for _ in range(2):  # capture the step graph of both contact-buffer parities
+            b.step(8, 3, m & ~eng.STEP_ASYNC)
+        b.wait()
+        t = time.perf_counter()
+        for _ in range(steps):
+            b.step(8, 3, m)
+        b.wait()
+        return time.perf_counter() - t
+
+    el = timed(batch, mode)
     counts = batch.counts()
-    # the same steps as replays of the captured step graph (the other launch mode)
+    solver = {"path": "per-world workgroups" if batch.get_option(eng.OPT_LAST_SOLVER) == 2 else "colour sweeps",
+              "tuned_workgroup": batch.get_option(eng.OPT_TUNED_WORLD_BLOCK),
+              "tuned_lds_bodies": bool(batch.get_option(eng.OPT_TUNED_WORLD_LDS))}
+    batch.close()
+    # the same steps (a second batch of the same worlds, advanced alike) in
+    # the other launch mode: replays of the captured step graph, or eager
     other = eng.STEP_ASYNC | (0 if mode & eng.STEP_GRAPH else eng.STEP_GRAPH)
-    for _ in range(2):  # capture (both contact-buffer parities) or first eager steps
-        batch.step(8, 3, other & ~eng.STEP_ASYNC)
-    batch.wait()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        batch.step(8, 3, other)
-    batch.wait()
-    el_other = time.perf_counter() - t0
+    batch = eng.Batch.from_scenes(scenes)
+    el_other = timed(batch, other)
+    batch.close()
     nbodies = int(sum(s.dynamic_count for s in scenes))
     sample = list(range(0, worlds, sample_every))
     # plain per-world loop on the sample
@@ -202,9 +206,7 @@ def batched_section(pkg, worlds: int, warmup: int, steps: int, sample_every: int
         "ms_per_step": el / steps * 1e3,
         "worlds": worlds, "dynamic_bodies": nbodies, "steps": steps, "warmup": warmup,
         "contacts": counts.contacts, "touching": counts.touching, "colours": counts.colours,
-        "solver": {"path": "per-world workgroups" if batch.get_option(eng.OPT_LAST_SOLVER) == 2 else "colour sweeps",
-                   "tuned_workgroup": batch.get_option(eng.OPT_TUNED_WORLD_BLOCK),
-                   "tuned_lds_bodies": bool(batch.get_option(eng.OPT_TUNED_WORLD_LDS))},
+        "solver": solver,
         "build_s": build_s,
         "per_world_loop": {"value": loop_rate, "unit": "world-steps/s",
                            "sample": f"{len(sample)} worlds (every {sample_every}th), {loop_steps} steps each, "
@@ -213,7 +215,8 @@ def batched_section(pkg, worlds: int, warmup: int, steps: int, sample_every: int
         "launch": "hipGraph replay of the whole step" if mode & eng.STEP_GRAPH else "eager",
         "other_launch": {"launch": "eager" if mode & eng.STEP_GRAPH else "hipGraph replay of the whole step",
                          "ms_per_step": el_other / steps * 1e3, "value": worlds * steps / el_other,
-                         "unit": "world-steps/s", "steps": steps},
+                         "unit": "world-steps/s", "steps": steps,
+                         "note": "a second batch of the same worlds, the same step range"},
     }
     # the whole step against HBM bandwidth: the genuinely HBM-sized case
     geo = float(np.mean([mean_geometry_bytes(sc) for sc in scenes]))
@@ -258,7 +261,6 @@ def batched_section(pkg, worlds: int, warmup: int, steps: int, sample_every: int
                                                    f"{cores} threads, {t_mt:.2f} s"}
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
         out["speedup_vs_cpu_all_cores"] = out["value"] / out["cpu_baseline_all_cores"]["value"]
-    batch.close()
     return out
 
 

@@ -187,6 +187,7 @@ struct b2g_batch {
   int pair_search = 0;       // B2G_OPT_PAIR_SEARCH
   int last_pair_search = 0;  // 1 LBVH, 2 grid
   int world_static_max = 0;  // most static shapes in one world
+  bool grid_local_ok = false;  // every world fits a bucket table of its own (k_grid_bin_worlds)
   bool use_grid() const { return pair_search == 0 && world_static_max <= kGridStaticMax && d.ns >= 2; }
   bool asleep_uploaded = false;  // a body was uploaded asleep: the awake flags apply from then on
   bool sleep_now() const {       // this step applies the awake flags and sleep timers
@@ -207,7 +208,7 @@ struct b2g_batch {
   void release();
   void swap_contact_buffers();
   void graphs_clear();
-  void enqueue_collide_and_colour(int K, int rounds);
+  void enqueue_collide_and_colour(int K, int rounds, bool collect_last = false);
   void enqueue_collide_and_colour_eager();
   void enqueue_collide();
   bool use_cells() const;
@@ -215,7 +216,7 @@ struct b2g_batch {
   int cells_step(int vi, int pi, unsigned flags);
   int sleep_step(int vi, int pi, unsigned flags);
 
-  void enqueue_colour_round(int K);
+  void enqueue_colour_round(int K, bool collect = false);
   void enqueue_colour_groups();
   void enqueue_islands_and_init();
   void enqueue_islands_join();
@@ -283,7 +284,7 @@ int b2g_batch::alloc() {
   A(d.ckey, NC); A(d.cpair, NC); A(d.cman, 4 * (size_t)NC); A(d.cmat, NC); A(d.ccol, NC);
   A(d.nkey, NC); A(d.npair, NC); A(d.nman, 4 * (size_t)NC); A(d.nmat, NC); A(d.ncol, NC);
   A(d.touch, NC); A(d.tpos, NC); A(d.clist, NC); A(d.cbody, NC); A(d.ccolour, NC); A(d.csorted, NC);
-  A(d.cisland, NC); A(d.cpend, NC); A(d.events, (size_t)kEventWords * NC);
+  A(d.cisland, NC); A(d.cpend, NC); A(d.cleft, NC); A(d.events, (size_t)kEventWords * NC);
   for (int k = 0; k < 10; ++k) A(d.vc[k], NC);
   for (int k = 0; k < 5; ++k) A(d.pc[k], NC);
   A(d.col_count, kColourSlots + 1); A(d.col_cursor, kColourSlots + 1);
@@ -318,6 +319,9 @@ int b2g_batch::alloc() {
     d.grid_T = t;
     A(d.grid_acc, 1); A(d.gstatic, NS); A(d.gsoff, NW + 1); A(d.gcount, t + 1); A(d.goff, t + 1);
     A(d.gids, NS); A(d.gbox, NS); A(d.gfast, NS); A(d.gworld, NS);
+    // the worlds' own tables: T + 1 words each, T a power of two of at least
+    // 64 and below 4 x the world's shapes (or 64)
+    A(d.gltab, NW + 1); A(d.gloff, 65 * (size_t)NW + 4 * (size_t)NS);
   } A(d.part_blist, 2 * (size_t)NB); A(d.part_brank, NB);
   A(d.part_ccnt, kPartGroups); A(d.part_bucket, (size_t)2 * kCells * kCellCons); A(d.part_cbucket, NC);
   A(d.part_coff, (size_t)kPartGroups * (kColourSlots + 1)); A(d.part_claim, NB); A(d.part_used, NB);
@@ -383,11 +387,18 @@ void b2g_batch::release() {
                   d.wbounds, d.wleaf, d.wroot, d.shdr, d.geo, d.smat, d.sfilt, d.aabb, d.joints, d.jsol,
                   d.jorder, d.wj_off, d.wj_list, d.jisland, d.bj_off, d.bj_other, d.ckey, d.cpair, d.cman, d.cmat, d.ccol,
                   d.nkey, d.npair, d.nman, d.nmat, d.ncol, d.touch, d.tpos, d.clist, d.cbody, d.ccolour,
-                  d.csorted, d.cisland, d.cpend, d.events, d.col_count, d.col_cursor, d.bmask, d.bslot,
+                  d.csorted, d.cisland, d.cpend, d.cleft, d.events, d.col_count, d.col_cursor, d.bmask, d.bslot,
                   d.btab, d.parent, d.sepbad, d.jok, d.solved, d.mkey[0], d.mkey[1], d.skey, d.xkey, d.xval, d.wsort, d.mval[0], d.mval[1],
                   d.sort_hist, d.node_child, d.node_parent, d.leaf_parent, d.node_range, d.node_rec,
-                  d.sfast, d.sparse, d.pslots, d.heavy, d.pcount, d.pfound, d.sinv, d.poff, d.poff_prev, d.scan_tiles,
-                  d.cnt};
+                  d.gltab, d.gloff, d.sfast, d.sparse, d.pslots, d.heavy, d.pcount, d.pfound, d.sinv, d.poff, d.poff_prev, d.scan_tiles,
+                  d.cnt,
+                  // sleep
+                  d.bsflag, d.bstime, d.bwake, d.bactive, d.bncontact, d.bjointed, d.isl_awake, d.isl_min, d.isl_cons,
+                  // the cells solver
+                  d.part_geo, d.part_boff, d.part_bcur, d.part_acc, d.part_sync, d.part_blist, d.part_brank, d.part_ccnt,
+                  d.part_bucket, d.part_cbucket, d.part_coff, d.part_claim, d.part_used, d.part_xcol, d.sloc,
+                  // the grid pair search
+                  d.grid_acc, d.gstatic, d.gsoff, d.gcount, d.goff, d.gids, d.gbox, d.gfast, d.gworld};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int k = 0; k < 10; ++k)
@@ -558,6 +569,23 @@ void b2g_batch::refresh_filters() {
     (void)hipMemcpyAsync(d.gsoff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, stream);
     if (off[nw] > 0)
       (void)hipMemcpyAsync(d.gstatic, list.data(), off[nw] * sizeof(int), hipMemcpyHostToDevice, stream);
+    // a bucket table per world (about two buckets per binned shape) when the
+    // worlds are laid out one after the other and none bins more than
+    // kGridLocalShapes shapes
+    grid_local_ok = false;
+    std::vector<int> tab(nw + 1, 0);
+    if (d.nw > 1 && worlds_contiguous && (int)h_wshape1.size() >= d.nw && (int)h_wshape0.size() >= d.nw) {
+      grid_local_ok = true;
+      for (int w = 0; w < d.nw && grid_local_ok; ++w) {
+        const int n = (h_wshape1[w] - h_wshape0[w]) - (off[w + 1] - off[w]);
+        int t = 64;
+        while (t < 2 * n) t <<= 1;
+        tab[w + 1] = tab[w] + t + 1;
+        grid_local_ok = n <= kGridLocalShapes;
+      }
+      if (grid_local_ok)
+        (void)hipMemcpyAsync(d.gltab, tab.data(), (d.nw + 1) * sizeof(int), hipMemcpyHostToDevice, stream);
+    }
     (void)hipStreamSynchronize(stream);  // the host vectors end here
   }
 }
@@ -567,6 +595,27 @@ void b2g_batch::refresh_filters() {
 int b2g_batch::grid_pairs() {
   const int ns = d.ns;
   const int gs = grid_for(ns);
+  const int gq = grid_for((long long)ns * kBlock / kQueryBlock);  // blocks of kQueryBlock threads
+  // one-wave workgroups: a world of 20,000 shapes then spreads its queries
+  // over every CU (157 workgroups of 128 lanes left a third of them idle)
+  constexpr int kGridQueryThreads = 64;
+  if (grid_local_ok && worlds_contiguous && !(d.idle && idle_worlds() > 0)) {
+    // every world active and small: each bins into a table of its own
+    Dev dg = d;
+    dg.grid_T = 0;  // (the shared table's counts are not used)
+    hipLaunchKernelGGL(k_grid_aabb, dim3(std::max(gs, finish_pending ? grid_for(d.nb) : 1)), dim3(kBlock), 0, stream,
+                       dg, finish_pending ? 1 : 0);
+    finish_pending = false;
+    hipLaunchKernelGGL(k_grid_bin_worlds, dim3(d.nw), dim3(kBlock), 0, stream, d);
+    launch(B2G_KERNEL_PAIRS_WRITE, ns, k_grid_pairs<kPairSlots, kGridQueryThreads, true>,
+           dim3(grid_for((long long)ns * kBlock / kGridQueryThreads)), dim3(kGridQueryThreads), 0, d);
+    int rc = scan(d.pcount, nullptr, ns, d.poff, d.cnt + C_PAIRS, ns);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_grid_pairs_compact<kPairSlots, true>), dim3(gq), dim3(kQueryBlock), 0, stream, d);
+    tree_stale = true;
+    last_pair_search = 2;
+    return B2G_OK;
+  }
   // buckets for about two per binned shape: with idle worlds kept, only the
   // active worlds' shapes are binned (a serving tick has a few percent)
   long long binned = ns;
@@ -585,14 +634,10 @@ int b2g_batch::grid_pairs() {
   int rc = scan(dg.gcount, nullptr, dg.grid_T, dg.goff, dg.goff + dg.grid_T, dg.grid_T);
   if (rc) return rc;
   hipLaunchKernelGGL(k_grid_scatter, dim3(gs), dim3(kBlock), 0, stream, dg);
-  const int gq = grid_for((long long)ns * kBlock / kQueryBlock);  // blocks of kQueryBlock threads
-  // one-wave workgroups: a world of 20,000 shapes then spreads its queries
-  // over every CU (157 workgroups of 128 lanes left a third of them idle)
-  constexpr int kGridQueryThreads = 64;
-  launch(B2G_KERNEL_PAIRS_WRITE, ns, k_grid_pairs<kPairSlots, kGridQueryThreads>,
+  launch(B2G_KERNEL_PAIRS_WRITE, ns, k_grid_pairs<kPairSlots, kGridQueryThreads, false>,
          dim3(grid_for(binned * kBlock / kGridQueryThreads)), dim3(kGridQueryThreads), 0, dg);
   if ((rc = scan(d.pcount, nullptr, ns, d.poff, d.cnt + C_PAIRS, ns))) return rc;
-  hipLaunchKernelGGL(k_grid_pairs_compact<kPairSlots>, dim3(gq), dim3(kQueryBlock), 0, stream, dg);
+  hipLaunchKernelGGL((k_grid_pairs_compact<kPairSlots, false>), dim3(gq), dim3(kQueryBlock), 0, stream, dg);
   tree_stale = true;  // the LBVH is not kept: a user query builds it first
   last_pair_search = 2;
   return B2G_OK;
@@ -776,11 +821,12 @@ void b2g_batch::graphs_clear() {
   graphs.clear();
 }
 
-void b2g_batch::enqueue_collide_and_colour(int K, int rounds) {
+// `collect_last`: the last round lists its losers for k_colour_finish
+void b2g_batch::enqueue_collide_and_colour(int K, int rounds, bool collect_last) {
   enqueue_collide();
   const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
   hipLaunchKernelGGL(k_colour_init, dim3(gk), dim3(kBlock), 0, stream, d, K);
-  for (int r = 0; r < rounds; ++r) enqueue_colour_round(K);
+  for (int r = 0; r < rounds; ++r) enqueue_colour_round(K, collect_last && r == rounds - 1);
 }
 
 // An eager step's colouring: device-wide rounds; a single world finishes in
@@ -789,7 +835,7 @@ void b2g_batch::enqueue_collide_and_colour(int K, int rounds) {
 // rounds if the read-back reports leftovers.
 void b2g_batch::enqueue_collide_and_colour_eager() {
   if (d.nw == 1) {
-    enqueue_collide_and_colour(kMaxColours, kColourRoundsOneWorld);
+    enqueue_collide_and_colour(kMaxColours, kColourRoundsOneWorld, true);
     hipLaunchKernelGGL(k_colour_finish, dim3(1), dim3(kFinishThreads), 0, stream, d, rounds_pending,
                        kGraphFinishRounds, kMaxColours, kFinishMaxLeft);
     rounds_pending += kGraphFinishRounds;
@@ -921,12 +967,12 @@ int b2g_batch::cells_step(int vi, int pi, unsigned flags) {
   return B2G_OK;
 }
 
-void b2g_batch::enqueue_colour_round(int K) {
+void b2g_batch::enqueue_colour_round(int K, bool collect) {
   // the rounds walk the pending list: after the first steps it holds only the
   // newly touching constraints, so a modest grid covers it
   const int gk = std::min(grid_for(std::max(est_cons, 256) * 5 / 4), 512);
   hipLaunchKernelGGL(k_colour_claim, dim3(gk), dim3(kBlock), 0, stream, d, rounds_pending);
-  hipLaunchKernelGGL(k_colour_resolve, dim3(gk), dim3(kBlock), 0, stream, d, rounds_pending, K);
+  hipLaunchKernelGGL(k_colour_resolve, dim3(gk), dim3(kBlock), 0, stream, d, rounds_pending, K, collect ? 1 : 0);
   ++rounds_pending;
 }
 
@@ -1241,10 +1287,13 @@ void b2g_batch::launch_solve_worlds(int vi, int pi) {
   };
   tbegin(B2G_KERNEL_SOLVER);
   if (big_worlds > 0 && big_worlds < d.nw && wblock < 1024) {
+    // the large worlds (the longest workgroups) are submitted first: where
+    // both launches share a queue (a graph replay may), their workgroups are
+    // then dispatched before the thousands of small ones
     (void)hipEventRecord(fork_ev, stream);
     (void)hipStreamWaitEvent(side, fork_ev, 0);
-    part(side, big_worlds, d.nw - big_worlds, wblock, max_small_world_bodies);
     part(stream, 0, big_worlds, 1024, max_world_bodies);
+    part(side, big_worlds, d.nw - big_worlds, wblock, max_small_world_bodies);
     (void)hipEventRecord(join_ev, side);
     (void)hipStreamWaitEvent(stream, join_ev, 0);
   } else {
@@ -1535,7 +1584,7 @@ int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
   // per-world solver: colours below K; colour sweeps: any colour (those the
   // last eager step did not launch alone go to the tail launch)
   const int colour_limit = use_per_world() ? K : kMaxColours;
-  enqueue_collide_and_colour(colour_limit, kGraphColourRounds);
+  enqueue_collide_and_colour(colour_limit, kGraphColourRounds, true);
   hipLaunchKernelGGL(k_colour_finish, dim3(1), dim3(kFinishThreads), 0, stream, d, rounds_pending,
                      kGraphFinishRounds, colour_limit, INT_MAX);  // no host loop follows a replay
   rounds_pending += kGraphFinishRounds;

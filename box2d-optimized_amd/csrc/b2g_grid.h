@@ -165,9 +165,68 @@ __global__ void __launch_bounds__(kBlock) k_grid_scatter(Dev d) {
   }
 }
 
+// Batches of worlds of at most kGridLocalShapes binned shapes, every world
+// active: each world bins its shapes into a table of its own (one workgroup
+// per world, counts, offsets and placement in LDS), so no global atomic is
+// needed and no bucket holds another world's shapes. World w's entries take
+// the slots of its own shapes, from wshape0[w] on (static shapes are not
+// binned, so they end before wshape1[w]).
+constexpr int kGridLocalShapes = 4096;
+constexpr int kGridLocalTable = 2 * kGridLocalShapes;  // buckets of the largest table
+__global__ void __launch_bounds__(kBlock) k_grid_bin_worlds(Dev d) {
+  __shared__ int cnt[kGridLocalTable];
+  __shared__ int lds[kBlock / kWave];
+  const GridGeom gg = grid_geom(d);
+  const int tid = (int)threadIdx.x;
+  for (int w = (int)blockIdx.x; w < d.nw; w += (int)gridDim.x) {
+    const int s0 = d.wshape0[w], s1 = d.wshape1[w];
+    const int tab = d.gltab[w];
+    const int T = d.gltab[w + 1] - tab - 1;  // a power of two, 64 .. kGridLocalTable
+    const unsigned mask = (unsigned)T - 1u;
+    for (int h = tid; h < T; h += kBlock) cnt[h] = 0;
+    __syncthreads();
+    for (int i = s0 + tid; i < s1; i += kBlock) {
+      if (d.sfast[i].y & 64) continue;
+      const float4 b = d.aabb[i];
+      atomicAdd(&cnt[grid_bucket(grid_coord(b.x, gg.inv), grid_coord(b.y, gg.inv), w, mask)], 1);
+    }
+    __syncthreads();
+    // exclusive scan of the counts: thread t takes buckets t * per .. + per
+    const int per = T / kBlock > 0 ? T / kBlock : 1;
+    const int h0 = tid * per;
+    int sum = 0;
+    for (int k = 0; k < per; ++k)
+      if (h0 + k < T) sum += cnt[h0 + k];
+    int total;
+    int ex = block_exclusive_scan(sum, lds, &total);
+    for (int k = 0; k < per; ++k) {
+      if (h0 + k < T) {
+        const int c = cnt[h0 + k];
+        cnt[h0 + k] = ex;  // the bucket's fill cursor
+        d.gloff[tab + h0 + k] = s0 + ex;
+        ex += c;
+      }
+    }
+    if (tid == 0) d.gloff[tab + T] = s0 + total;
+    __syncthreads();
+    for (int i = s0 + tid; i < s1; i += kBlock) {
+      const int4 f = d.sfast[i];
+      if (f.y & 64) continue;
+      const float4 b = d.aabb[i];
+      const int at = s0 + atomicAdd(&cnt[grid_bucket(grid_coord(b.x, gg.inv), grid_coord(b.y, gg.inv), w, mask)], 1);
+      d.gids[at] = i;
+      d.gbox[at] = b;
+      d.gfast[at] = f;
+    }
+    __syncthreads();  // cnt is the next world's
+  }
+}
+
 // Every pair of shape i under the ownership rule: emit(key, s) as for
 // lbvh_query (s >= 0: the static lower shape of a key that is not i's own).
-template <class Emit>
+// kLocal: the buckets are those of i's world's own table (k_grid_bin_worlds),
+// which hold no other world's shapes.
+template <bool kLocal, class Emit>
 __device__ __forceinline__ void grid_query(const Dev& d, const GridGeom& gg, int i, Emit emit) {
   const int4 fi = d.sfast[i];
   if (fi.y & 64) return;  // static: found by its partners
@@ -185,17 +244,24 @@ __device__ __forceinline__ void grid_query(const Dev& d, const GridGeom& gg, int
   };
   const int w = d.shdr[i].w;
   const long long cx = grid_coord(q.x, gg.inv), cy = grid_coord(q.y, gg.inv);
+  const int* off = d.goff;
+  unsigned mask = gg.mask;
+  if (kLocal) {
+    const int tab = d.gltab[w];
+    off = d.gloff + tab;
+    mask = (unsigned)(d.gltab[w + 1] - tab - 2);
+  }
   // the 3 x 3 block's buckets, each once; their ranges loaded together
   int hs[9], lo[9], hi[9];
 #pragma unroll
-  for (int c = 0; c < 9; ++c) hs[c] = grid_bucket(cx + (c % 3) - 1, cy + (c / 3) - 1, w, gg.mask);
+  for (int c = 0; c < 9; ++c) hs[c] = grid_bucket(cx + (c % 3) - 1, cy + (c / 3) - 1, w, mask);
 #pragma unroll
   for (int c = 0; c < 9; ++c) {
     bool again = false;
 #pragma unroll
     for (int e = 0; e < c; ++e) again |= hs[e] == hs[c];
-    lo[c] = again ? 0 : d.goff[hs[c]];
-    hi[c] = again ? 0 : d.goff[hs[c] + 1];
+    lo[c] = again ? 0 : off[hs[c]];
+    hi[c] = again ? 0 : off[hs[c] + 1];
   }
   // the candidates of the nine ranges as one sequence f = 0 .. total - 1,
   // kChunk boxes loaded before any is tested (position by compares, so the
@@ -221,13 +287,14 @@ __device__ __forceinline__ void grid_query(const Dev& d, const GridGeom& gg, int
     int4 fs[kChunk];
 #pragma unroll
     for (int u = 0; u < kChunk; ++u) {  // the candidates' boxes, indices, filter words and worlds, all in flight
-      const int at = pos[u] >= 0 ? pos[u] : 0;  // (entry 0 exists: the querying shape has one)
+      const int at = pos[u] >= 0 ? pos[u] : 0;  // (in bounds; read but not used)
       bx[u] = d.gbox[at];
       ids[u] = d.gids[at];
       fs[u] = d.gfast[at];
-      ws[u] = d.gworld[at];
+      ws[u] = kLocal ? w : d.gworld[at];
     }
-    // (a bucket can hold cells of other worlds: their shapes are never partners)
+    // (a shared table's bucket can hold cells of other worlds: their shapes
+    // are never partners)
 #pragma unroll
     for (int u = 0; u < kChunk; ++u)
       if (pos[u] >= 0 && ws[u] == w && overlap(q, bx[u])) test(ids[u], fs[u]);
@@ -253,17 +320,20 @@ __device__ __forceinline__ void grid_query(const Dev& d, const GridGeom& gg, int
 
 // As k_lbvh_pairs: up to kSlots sorted keys per shape, the counts past them;
 // lanes take the shapes in bucket order (neighbouring lanes, neighbouring
-// cells).
-template <int kSlots, int kThreads>
+// cells). Shared table: entries 0 .. binned count. Tables of their own: world
+// w's entries start at its first shape's index and end at its table's last
+// word; the entry slots past that (its static shapes') are skipped.
+template <int kSlots, int kThreads, bool kLocal>
 __global__ void __launch_bounds__(kThreads) k_grid_pairs(Dev d) {
   __shared__ uint64_t slots[kSlots * kThreads];  // thread-minor
   const GridGeom gg = grid_geom(d);
-  const int nq = d.goff[d.grid_T];  // binned shapes
+  const int nq = kLocal ? d.ns : d.goff[d.grid_T];  // binned shapes
   B2G_FOR(t, 0, nq) {
+    if (kLocal && t >= d.gloff[d.gltab[d.shdr[t].w + 1] - 1]) continue;
     const int i = d.gids[t];
     const int tid = (int)threadIdx.x;
     int found = 0, own = 0;
-    grid_query(d, gg, i, [&](uint64_t key, int s) {
+    grid_query<kLocal>(d, gg, i, [&](uint64_t key, int s) {
       if (found < kSlots) {
         int b = found - 1;  // sorted insert
         while (b >= 0 && slots[b * kThreads + tid] > key) {
@@ -287,7 +357,7 @@ __global__ void __launch_bounds__(kThreads) k_grid_pairs(Dev d) {
 }
 
 // As k_pairs_compact, with the grid query for over-full shapes.
-template <int kSlots>
+template <int kSlots, bool kLocal>
 __global__ void __launch_bounds__(kQueryBlock) k_grid_pairs_compact(Dev d) {
   const GridGeom gg = grid_geom(d);
   B2G_FOR(i, 0, d.ns) {
@@ -323,7 +393,7 @@ __global__ void __launch_bounds__(kQueryBlock) k_grid_pairs_compact(Dev d) {
     } else {
       // over-full shape: in query order, sorted afterwards by k_sort_pair_segments
       int k = 0;
-      grid_query(d, gg, i, [&](uint64_t key, int s) {
+      grid_query<kLocal>(d, gg, i, [&](uint64_t key, int s) {
         if (s >= 0)
           foreign(key, s);
         else if (k < own)

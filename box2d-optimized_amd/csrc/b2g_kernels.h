@@ -46,6 +46,7 @@ enum Counter : int {
   C_ROUND = 7,      // colouring round base (claim priorities carry base + local round)
   C_SERIAL = 8,     // constraints placed in the serial group by the last step
   C_PENDING = 9,    // entries of the pending-colour list of this step
+  C_LEFT = 10,      // entries of the list of constraints a collecting round left uncoloured
   C_EVENTS = 11,    // contact events recorded this step
   C_HEAVY = 12,     // over-full shapes whose pair segments need sorting (this broadphase)
   C_PART_CROSS = 13,  // constraints of the cells solver's cross group this step
@@ -117,6 +118,7 @@ struct Dev {
   float4* pc[5];
   int* cisland;
   int* cpend;     // constraints (compacted slots) without a colour at the start of this step
+  int* cleft;     // those a collecting claim round left uncoloured (k_colour_finish's list)
   float4* events; // kEventWords per contact event (recorded when events_on)
   int events_on;
   int ecap;
@@ -193,6 +195,9 @@ struct Dev {
   float4* gbox;       // their AABBs, in the same order
   int4* gfast;        // their pair-filter words, in the same order
   int* gworld;        // their worlds, in the same order
+  int* gltab;         // nw + 1: each world's own bucket table in gloff (batches of small worlds)
+  int* gloff;         // those tables: world w's T + 1 entries (T = gltab[w + 1] - gltab[w] - 1, a
+                      // power of two) are bucket offsets into gids, then the end of its entries
   unsigned* part_acc; // 8: bounds of the dynamic bodies' centres (ordered floats), their count, -,
                       // and (words 6-7, 64 bits) the fixed-point sum of their footprints
   unsigned* part_sync;  // 4: finished-workgroup counters of the launches whose last workgroup goes on
@@ -307,6 +312,7 @@ __device__ __forceinline__ void reset_colour_counters(const Dev& d) {
   if (threadIdx.x == 0) {
     d.cnt[C_REMAIN] = 0;
     d.cnt[C_PENDING] = 0;
+    d.cnt[C_LEFT] = 0;
     d.cnt[C_NCOLOURS] = 0;
     d.cnt[C_SERIAL] = 0;
   }
@@ -552,7 +558,10 @@ __global__ void __launch_bounds__(kBlock) k_colour_claim(Dev d, int local_round)
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_colour_resolve(Dev d, int local_round, int K) {
+// `collect`: the round's losers also go to the list k_colour_finish walks
+// (the last device-wide round before it), so its rounds do not walk the
+// whole pending list again
+__global__ void __launch_bounds__(kBlock) k_colour_resolve(Dev d, int local_round, int K, int collect) {
   if (d.cnt[C_REMAIN] == 0) return;
   const int round = d.cnt[C_ROUND] + local_round;
   const int n = d.cnt[C_PENDING];
@@ -565,7 +574,10 @@ __global__ void __launch_bounds__(kBlock) k_colour_resolve(Dev d, int local_roun
     const unsigned long long v = colour_priority(d, round, pr);
     const bool da = is_dyn(d, ba), db = is_dyn(d, bb);
     const bool win = (!da || d.bslot[ba] == v) && (!db || d.bslot[bb] == v);
-    if (!win) continue;
+    if (!win) {
+      if (collect) d.cleft[wave_append(&d.cnt[C_LEFT])] = k;
+      continue;
+    }
     const unsigned long long used = (da ? d.bmask[ba] : 0ull) | (db ? d.bmask[bb] : 0ull);
     int col = kSerialColour;
     if (~used != 0ull) col = __ffsll((long long)~used) - 1;
@@ -635,8 +647,10 @@ __device__ __forceinline__ void world_colour_stats(const Dev& d, int w, int used
 // The claim rounds of a captured step that the multi-workgroup rounds left
 // unfinished, in one workgroup (barriers between the claim and resolve
 // passes instead of launches), up to `rounds` more rounds: the same claims,
-// priorities and colour choice as k_colour_claim / k_colour_resolve. The
-// claim words are read with atomic loads: other waves wrote them with
+// priorities and colour choice as k_colour_claim / k_colour_resolve, over the
+// list the last (collecting) device-wide round left. Which lane takes which
+// constraint changes nothing: the winners of a round share no dynamic body.
+// The claim words are read with atomic loads: other waves wrote them with
 // atomics. Launched with one workgroup of kFinishThreads threads. With more
 // than `max_left` constraints left it does nothing (a world colouring afresh
 // is faster in device-wide rounds, which the eager host loop then runs).
@@ -645,12 +659,12 @@ __global__ void __launch_bounds__(kFinishThreads) k_colour_finish(Dev d, int loc
                                                                   int max_left) {
   const int left0 = d.cnt[C_REMAIN];
   if (left0 == 0 || left0 > max_left) return;
-  const int n = d.cnt[C_PENDING];
+  const int n = d.cnt[C_LEFT];
   const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
   for (int r = 0; r < rounds; ++r) {
     const int round = d.cnt[C_ROUND] + local_round0 + r;
     for (int q = tid; q < n; q += nt) {
-      const int k = d.cpend[q];
+      const int k = d.cleft[q];
       if (d.ccolour[k] >= 0) continue;
       const int2 pr = d.cpair[d.clist[k]];
       const int ba = d.shdr[pr.x].y, bb = d.shdr[pr.y].y;
@@ -661,7 +675,7 @@ __global__ void __launch_bounds__(kFinishThreads) k_colour_finish(Dev d, int loc
     __syncthreads();
     int left = 0;
     for (int q = tid; q < n; q += nt) {
-      const int k = d.cpend[q];
+      const int k = d.cleft[q];
       if (d.ccolour[k] >= 0) continue;
       const int2 pr = d.cpair[d.clist[k]];
       const int ba = d.shdr[pr.x].y, bb = d.shdr[pr.y].y;
