@@ -152,9 +152,12 @@ def batched_section(pkg, worlds: int, warmup: int, steps: int, sample_every: int
     stage. Compared with a plain loop of single-world GPU steps (one world per
     batch, stepped one after another with a host sync per step, as a caller
     looping over b2World::Step would) on a sample of the same worlds, and with
-    the reference CPU Step on the same sample."""
+    the reference CPU Step on the same sample. The warm-up is at least 12
+    steps: the per-world solver's workgroup autotuner takes the first 10
+    (eager steps, whatever the launch mode)."""
     import numpy as np
     S, eng = pkg.scenes, pkg.engine
+    warmup = max(warmup, 12)
     sizes = S.batched_sizes(worlds)
     t0 = time.perf_counter()
     scenes = [S.batch_world(i, int(sizes[i])) for i in range(worlds)]

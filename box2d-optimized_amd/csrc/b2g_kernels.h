@@ -361,84 +361,116 @@ __device__ __forceinline__ void integrate_velocity(const Dev& d, int b, int inte
 // Narrowphase: b2ContactManager::Collide + b2Contact::Update
 // (src/dynamics/b2_contact_manager.cpp:66-116, src/dynamics/b2_contact.cpp:126-210)
 // ===========================================================================
-// The same launch integrates the bodies' velocities (b2Island::Solve,
-// b2_island.cpp:257-298), which touches none of the contact data, and clears
-// the step's colour counters. A step with dt <= 0 runs Collide alone
-// (integrate = 0).
+// Contact i's Collide step (b2Contact::Update)
+__device__ __forceinline__ void narrow_contact(const Dev& d, int i) {
+  const int2 pr = d.cpair[i];
+  const int4 ha = d.shdr[pr.x], hb = d.shdr[pr.y];
+  int solve = 0;
+  if (world_active(d, ha.w) && d.sleep_on) {
+    // b2Body::GetContactCount for the orphan test, and a contact between
+    // two sleeping (or static) bodies keeps its manifold and touching state
+    atomicAdd(&d.bncontact[ha.y], 1);
+    atomicAdd(&d.bncontact[hb.y], 1);
+    if (!body_awake0(d, ha.y) && !body_awake0(d, hb.y)) {
+      float4* mp = d.cman + 4 * (size_t)i;
+      int* packed = reinterpret_cast<int*>(mp + 3) + 2;
+      const int w0 = *packed;
+      *packed = w0 & 0xffff;  // solved by no step since
+      d.touch[i] = ((w0 >> 8) & 0xff) > 0 ? 1 : 0;
+      return;
+    }
+  }
+  if (world_active(d, ha.w)) {
+    Geo ga, gb;
+    load_geo(ga, ha.x, ha.z, d.geo + 9 * (size_t)pr.x);
+    load_geo(gb, hb.x, hb.z, d.geo + 9 * (size_t)pr.y);
+    const xform xa = xform_of(d.xf[ha.y]);
+    const xform xb = xform_of(d.xf[hb.y]);
+    float4* mp = d.cman + 4 * (size_t)i;
+    const float4 o0 = mp[0], o1 = mp[1], o3 = mp[3];
+    const int old_count = (as_int(o3.z) >> 8) & 0xff;
+    Man m;
+    m.count = 0;
+    m.type = (as_int(o3.z) & 0xff);
+    const float4 o2 = mp[2];
+    m.ln = mk(o2.x, o2.y);
+    m.lp = mk(o2.z, o2.w);
+    m.lp0 = mk(0.0f, 0.0f);
+    m.lp1 = mk(0.0f, 0.0f);
+    m.id0 = m.id1 = 0;
+    evaluate(m, ga, xa, gb, xb);
+    // match old feature keys to carry the impulses (b2_contact.cpp:158-181)
+    float ni0 = 0.0f, ti0 = 0.0f, ni1 = 0.0f, ti1 = 0.0f;
+    const uint32_t oid0 = as_uint(o3.x), oid1 = as_uint(o3.y);
+    if (m.count > 0) {
+      if (old_count > 0 && oid0 == m.id0) { ni0 = o0.z; ti0 = o0.w; }
+      else if (old_count > 1 && oid1 == m.id0) { ni0 = o1.z; ti0 = o1.w; }
+    }
+    if (m.count > 1) {
+      if (old_count > 0 && oid0 == m.id1) { ni1 = o0.z; ti1 = o0.w; }
+      else if (old_count > 1 && oid1 == m.id1) { ni1 = o1.z; ti1 = o1.w; }
+    }
+    mp[0] = make_float4(m.lp0.x, m.lp0.y, ni0, ti0);
+    mp[1] = make_float4(m.count > 1 ? m.lp1.x : 0.0f, m.count > 1 ? m.lp1.y : 0.0f, ni1, ti1);
+    mp[2] = make_float4(m.ln.x, m.ln.y, m.lp.x, m.lp.y);
+    mp[3] = make_float4(uint_as_float(m.count > 0 ? m.id0 : 0u), uint_as_float(m.count > 1 ? m.id1 : 0u),
+                        as_float((m.type & 0xff) | (m.count << 8)), 0.0f);
+    solve = m.count > 0 ? 1 : 0;
+    // a touching transition wakes both bodies (b2_contact.cpp:183-188); the
+    // wake-up applies after Collide, when the islands are seeded
+    if (d.sleep_on && ((old_count > 0) != (m.count > 0))) {
+      d.bwake[ha.y] = 1;
+      d.bwake[hb.y] = 1;
+    }
+    // BeginContact / EndContact on touching transitions (b2_contact.cpp:197-203)
+    if (d.events_on && ((old_count > 0) != (m.count > 0))) {
+      const float4 man[4] = {mp[0], mp[1], mp[2], mp[3]};
+      record_event(d, pr, m.count > 0 ? kEventBegin : kEventEnd, man, d.cmat[i]);
+    }
+    // a contact that is not solved this step gives up its colour: colours
+    // kept across steps are then always a subset of the previous step's
+    // valid colouring
+    if (!solve) d.ccol[i] = -1;
+  }
+  d.touch[i] = solve;
+}
+
+// A block's lanes take its tile of kBlock contacts sorted by the pair's shape
+// types (a counting sort in LDS), so a wave mostly runs one collide function
+// instead of every one its contacts need, each masked; which lane takes a
+// contact changes nothing in its result. The same launch integrates the
+// bodies' velocities (b2Island::Solve, b2_island.cpp:257-298), which touches
+// none of the contact data, and clears the step's colour counters. A step
+// with dt <= 0 runs Collide alone (integrate = 0).
 __global__ void __launch_bounds__(kBlock) k_narrowphase(Dev d, int integrate) {
+  __shared__ int s_cnt[17];  // 9 ordered type pairs; [16]: the tile's contacts
+  __shared__ int s_perm[kBlock];
   if (blockIdx.x == 0) reset_colour_counters(d);
   const int n = d.cnt[C_CONTACTS];
-  B2G_FOR(i, 0, n) {
-    const int2 pr = d.cpair[i];
-    const int4 ha = d.shdr[pr.x], hb = d.shdr[pr.y];
-    int solve = 0;
-    if (world_active(d, ha.w) && d.sleep_on) {
-      // b2Body::GetContactCount for the orphan test, and a contact between
-      // two sleeping (or static) bodies keeps its manifold and touching state
-      atomicAdd(&d.bncontact[ha.y], 1);
-      atomicAdd(&d.bncontact[hb.y], 1);
-      if (!body_awake0(d, ha.y) && !body_awake0(d, hb.y)) {
-        float4* mp = d.cman + 4 * (size_t)i;
-        int* packed = reinterpret_cast<int*>(mp + 3) + 2;
-        const int w0 = *packed;
-        *packed = w0 & 0xffff;  // solved by no step since
-        d.touch[i] = ((w0 >> 8) & 0xff) > 0 ? 1 : 0;
-        continue;
-      }
+  B2G_FOR_ALL(i, valid, 0, n) {
+    if (threadIdx.x < 17) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int key = 0, rank = 0;
+    if (valid) {
+      const int2 pr = d.cpair[i];
+      key = 3 * d.shdr[pr.x].x + d.shdr[pr.y].x;  // kCircle 0, kEdge 1, kPolygon 2
+      rank = atomicAdd(&s_cnt[key], 1);
     }
-    if (world_active(d, ha.w)) {
-      Geo ga, gb;
-      load_geo(ga, ha.x, ha.z, d.geo + 9 * (size_t)pr.x);
-      load_geo(gb, hb.x, hb.z, d.geo + 9 * (size_t)pr.y);
-      const xform xa = xform_of(d.xf[ha.y]);
-      const xform xb = xform_of(d.xf[hb.y]);
-      float4* mp = d.cman + 4 * (size_t)i;
-      const float4 o0 = mp[0], o1 = mp[1], o3 = mp[3];
-      const int old_count = (as_int(o3.z) >> 8) & 0xff;
-      Man m;
-      m.count = 0;
-      m.type = (as_int(o3.z) & 0xff);
-      const float4 o2 = mp[2];
-      m.ln = mk(o2.x, o2.y);
-      m.lp = mk(o2.z, o2.w);
-      m.lp0 = mk(0.0f, 0.0f);
-      m.lp1 = mk(0.0f, 0.0f);
-      m.id0 = m.id1 = 0;
-      evaluate(m, ga, xa, gb, xb);
-      // match old feature keys to carry the impulses (b2_contact.cpp:158-181)
-      float ni0 = 0.0f, ti0 = 0.0f, ni1 = 0.0f, ti1 = 0.0f;
-      const uint32_t oid0 = as_uint(o3.x), oid1 = as_uint(o3.y);
-      if (m.count > 0) {
-        if (old_count > 0 && oid0 == m.id0) { ni0 = o0.z; ti0 = o0.w; }
-        else if (old_count > 1 && oid1 == m.id0) { ni0 = o1.z; ti0 = o1.w; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int run = 0;
+      for (int k = 0; k < 9; ++k) {
+        const int c = s_cnt[k];
+        s_cnt[k] = run;
+        run += c;
       }
-      if (m.count > 1) {
-        if (old_count > 0 && oid0 == m.id1) { ni1 = o0.z; ti1 = o0.w; }
-        else if (old_count > 1 && oid1 == m.id1) { ni1 = o1.z; ti1 = o1.w; }
-      }
-      mp[0] = make_float4(m.lp0.x, m.lp0.y, ni0, ti0);
-      mp[1] = make_float4(m.count > 1 ? m.lp1.x : 0.0f, m.count > 1 ? m.lp1.y : 0.0f, ni1, ti1);
-      mp[2] = make_float4(m.ln.x, m.ln.y, m.lp.x, m.lp.y);
-      mp[3] = make_float4(uint_as_float(m.count > 0 ? m.id0 : 0u), uint_as_float(m.count > 1 ? m.id1 : 0u),
-                          as_float((m.type & 0xff) | (m.count << 8)), 0.0f);
-      solve = m.count > 0 ? 1 : 0;
-      // a touching transition wakes both bodies (b2_contact.cpp:183-188); the
-      // wake-up applies after Collide, when the islands are seeded
-      if (d.sleep_on && ((old_count > 0) != (m.count > 0))) {
-        d.bwake[ha.y] = 1;
-        d.bwake[hb.y] = 1;
-      }
-      // BeginContact / EndContact on touching transitions (b2_contact.cpp:197-203)
-      if (d.events_on && ((old_count > 0) != (m.count > 0))) {
-        const float4 man[4] = {mp[0], mp[1], mp[2], mp[3]};
-        record_event(d, pr, m.count > 0 ? kEventBegin : kEventEnd, man, d.cmat[i]);
-      }
-      // a contact that is not solved this step gives up its colour: colours
-      // kept across steps are then always a subset of the previous step's
-      // valid colouring
-      if (!solve) d.ccol[i] = -1;
+      s_cnt[16] = run;
     }
-    d.touch[i] = solve;
+    __syncthreads();
+    if (valid) s_perm[s_cnt[key] + rank] = i;
+    __syncthreads();
+    if ((int)threadIdx.x < s_cnt[16]) narrow_contact(d, s_perm[threadIdx.x]);
+    __syncthreads();  // s_cnt and s_perm are the next tile's
   }
   if (integrate) B2G_FOR(b, 0, d.nb) integrate_velocity(d, b, d.sleep_on ? 0 : 1);
 }
