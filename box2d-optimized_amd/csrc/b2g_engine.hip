@@ -351,8 +351,11 @@ int b2g_batch::alloc() {
       HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
     }
   }
-  HIP_TRY(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
-  HIP_TRY(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
+  // fork and join order the side stream's launches against the main
+  // stream's on this device only: no system-scope fence (by default an event
+  // record writes back and invalidates the caches for the host's sake)
+  HIP_TRY(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming | hipEventDisableSystemFence));
+  HIP_TRY(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming | hipEventDisableSystemFence));
   HIP_TRY(hipEventCreateWithFlags(&counts_ev, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&graph_ev, hipEventDisableTiming));
   for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
