@@ -91,6 +91,11 @@ struct b2g_batch {
   float4* stage_wpar = nullptr;
   int* stage_wflag = nullptr;
   hipEvent_t params_ev = nullptr;
+  // the active worlds of a mostly idle step, for the per-world solver's
+  // launches (order_ev guards the staging as params_ev does)
+  std::vector<int> h_worder;
+  int* stage_order = nullptr;
+  hipEvent_t order_ev = nullptr;
   // host mirrors
   std::vector<int> h_btype, h_bworld;
   std::vector<int> h_shape_body;
@@ -124,9 +129,11 @@ struct b2g_batch {
   void tbegin(int id) {
     if (id != timed_kernel) return;
     if (tev_used + 2 > tev.size()) {
+      // timing only: no system-scope fence (a cache writeback and
+      // invalidate at each record would be timed with the launches)
       hipEvent_t e0, e1;
-      (void)hipEventCreate(&e0);
-      (void)hipEventCreate(&e1);
+      (void)hipEventCreateWithFlags(&e0, hipEventDisableSystemFence);
+      (void)hipEventCreateWithFlags(&e1, hipEventDisableSystemFence);
       tev.push_back(e0);
       tev.push_back(e1);
     }
@@ -275,7 +282,7 @@ int b2g_batch::alloc() {
   if ((rc = dalloc(&(ptr), (size_t)(n))) != B2G_OK) return rc
   A(d.xf, NB); A(d.pos, NB); A(d.vel, NB); A(d.prop, NB); A(d.prop2, NB); A(d.force, NB);
   A(d.btype, NB); A(d.bworld, NB);
-  A(d.wpar, NW); A(d.wflag, NW); A(d.wsort, NW); A(d.wshape0, NW); A(d.wshape1, NW); A(d.wbody, NW); A(d.worder, NW);
+  A(d.wpar, NW); A(d.wflag, NW); A(d.wsort, NW); A(d.wshape0, NW); A(d.wshape1, NW); A(d.wbody, NW); A(d.worder, NW); A(d.wact, NW);
   A(d.wcol, (size_t)NW * (kColourSlots + 1));
   A(d.wcolmax, NW); A(d.wfresh, NW); A(d.wrecolour, NW); A(d.wbounds, 4 * NW); A(d.wleaf, 2 * NW); A(d.wroot, NW);
   A(d.shdr, NS); A(d.geo, 9 * (size_t)NS); A(d.smat, NS); A(d.sfilt, NS); A(d.aabb, NS);
@@ -313,9 +320,9 @@ int b2g_batch::alloc() {
   A(d.bsflag, NB); A(d.bstime, NB); A(d.bwake, NB); A(d.bactive, NB); A(d.bncontact, NB); A(d.bjointed, NB);
   A(d.isl_awake, NB); A(d.isl_min, NB); A(d.isl_cons, NB);
   A(d.part_geo, 1); A(d.part_boff, 2 * kCells + 1); A(d.part_bcur, 2 * kCells); A(d.part_acc, 8); A(d.part_sync, 4);
-  {  // the grid pair search: about two buckets per shape
+  {  // the grid pair search: about four buckets per shape at most
     int t = 1024;
-    while (t < 2 * NS) t <<= 1;
+    while (t < 4 * NS) t <<= 1;
     d.grid_T = t;
     A(d.grid_acc, 1); A(d.gstatic, NS); A(d.gsoff, NW + 1); A(d.gcount, t + 1); A(d.goff, t + 1);
     A(d.gids, NS); A(d.gbox, NS); A(d.gfast, NS); A(d.gworld, NS);
@@ -332,7 +339,9 @@ int b2g_batch::alloc() {
   HIP_TRY(hipHostMalloc((void**)&pinned, 256 * sizeof(int), hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void**)&stage_wpar, (size_t)NW * sizeof(float4), hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void**)&stage_wflag, (size_t)NW * sizeof(int), hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void**)&stage_order, (size_t)NW * sizeof(int), hipHostMallocDefault));
   HIP_TRY(hipEventCreateWithFlags(&params_ev, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&order_ev, hipEventDisableTiming));
   HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   {
     // The side stream runs beside the main one (island labels beside the
@@ -358,7 +367,7 @@ int b2g_batch::alloc() {
   HIP_TRY(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming | hipEventDisableSystemFence));
   HIP_TRY(hipEventCreateWithFlags(&counts_ev, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&graph_ev, hipEventDisableTiming));
-  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+  for (auto& e : ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));  // timing only
   h_btype.assign(NB, 0);
   h_bworld.assign(NB, 0);
   h_shape_body.assign(NS, 0);
@@ -385,7 +394,7 @@ void b2g_batch::release() {
   graphs_clear();
   if (qbuf) (void)hipFree(qbuf);
   qbuf = nullptr;
-  void* ptrs[] = {d.xf, d.pos, d.vel, d.prop, d.prop2, d.force, d.btype, d.bworld, d.wpar, d.wflag, d.wshape0, d.wshape1, d.wbody, d.worder, d.wcol, d.wcolmax, d.wfresh,
+  void* ptrs[] = {d.xf, d.pos, d.vel, d.prop, d.prop2, d.force, d.btype, d.bworld, d.wpar, d.wflag, d.wshape0, d.wshape1, d.wbody, d.worder, d.wact, d.wcol, d.wcolmax, d.wfresh,
                   d.wrecolour,
                   d.wbounds, d.wleaf, d.wroot, d.shdr, d.geo, d.smat, d.sfilt, d.aabb, d.joints, d.jsol,
                   d.jorder, d.wj_off, d.wj_list, d.jisland, d.bj_off, d.bj_other, d.ckey, d.cpair, d.cman, d.cmat, d.ccol,
@@ -411,7 +420,9 @@ void b2g_batch::release() {
   if (pinned) (void)hipHostFree(pinned);
   if (stage_wpar) (void)hipHostFree(stage_wpar);
   if (stage_wflag) (void)hipHostFree(stage_wflag);
+  if (stage_order) (void)hipHostFree(stage_order);
   if (params_ev) (void)hipEventDestroy(params_ev);
+  if (order_ev) (void)hipEventDestroy(order_ev);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   if (fork_ev) (void)hipEventDestroy(fork_ev);
@@ -572,9 +583,10 @@ void b2g_batch::refresh_filters() {
     (void)hipMemcpyAsync(d.gsoff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, stream);
     if (off[nw] > 0)
       (void)hipMemcpyAsync(d.gstatic, list.data(), off[nw] * sizeof(int), hipMemcpyHostToDevice, stream);
-    // a bucket table per world (about two buckets per binned shape) when the
-    // worlds are laid out one after the other and none bins more than
-    // kGridLocalShapes shapes
+    // a bucket table per world (about two buckets per binned shape: four
+    // measured slower on the 4096-world batch, whose worlds' tables are
+    // cleared and scanned in LDS) when the worlds are laid out one after the
+    // other and none bins more than kGridLocalShapes shapes
     grid_local_ok = false;
     std::vector<int> tab(nw + 1, 0);
     if (d.nw > 1 && worlds_contiguous && (int)h_wshape1.size() >= d.nw && (int)h_wshape0.size() >= d.nw) {
@@ -610,16 +622,17 @@ int b2g_batch::grid_pairs() {
                        dg, finish_pending ? 1 : 0);
     finish_pending = false;
     hipLaunchKernelGGL(k_grid_bin_worlds, dim3(d.nw), dim3(kBlock), 0, stream, d);
-    launch(B2G_KERNEL_PAIRS_WRITE, ns, k_grid_pairs<kPairSlots, kGridQueryThreads, true>,
+    launch(B2G_KERNEL_PAIRS_WRITE, ns, k_grid_pairs<kPairSlots, kGridQueryThreads, kGridLocal>,
            dim3(grid_for((long long)ns * kBlock / kGridQueryThreads)), dim3(kGridQueryThreads), 0, d);
     int rc = scan(d.pcount, nullptr, ns, d.poff, d.cnt + C_PAIRS, ns);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_grid_pairs_compact<kPairSlots, true>), dim3(gq), dim3(kQueryBlock), 0, stream, d);
+    hipLaunchKernelGGL((k_grid_pairs_compact<kPairSlots, kGridLocal>), dim3(gq), dim3(kQueryBlock), 0, stream, d);
     tree_stale = true;
     last_pair_search = 2;
     return B2G_OK;
   }
-  // buckets for about two per binned shape: with idle worlds kept, only the
+  // about four buckets per binned shape (two measured 40% slower queries on
+  // the 20k pile: more tiles share a bucket): with idle worlds kept, only the
   // active worlds' shapes are binned (a serving tick has a few percent)
   long long binned = ns;
   if (d.idle && worlds_contiguous) {
@@ -629,7 +642,7 @@ int b2g_batch::grid_pairs() {
   }
   Dev dg = d;
   dg.grid_T = 1024;
-  while (dg.grid_T < 2 * binned && dg.grid_T < d.grid_T) dg.grid_T <<= 1;
+  while (dg.grid_T < 4 * binned && dg.grid_T < d.grid_T) dg.grid_T <<= 1;
   hipLaunchKernelGGL(k_grid_aabb, dim3(std::max(gs, finish_pending ? grid_for(d.nb) : 1)), dim3(kBlock), 0, stream, dg,
                      finish_pending ? 1 : 0);
   finish_pending = false;
@@ -637,10 +650,17 @@ int b2g_batch::grid_pairs() {
   int rc = scan(dg.gcount, nullptr, dg.grid_T, dg.goff, dg.goff + dg.grid_T, dg.grid_T);
   if (rc) return rc;
   hipLaunchKernelGGL(k_grid_scatter, dim3(gs), dim3(kBlock), 0, stream, dg);
-  launch(B2G_KERNEL_PAIRS_WRITE, ns, k_grid_pairs<kPairSlots, kGridQueryThreads, false>,
-         dim3(grid_for(binned * kBlock / kGridQueryThreads)), dim3(kGridQueryThreads), 0, dg);
+  if (d.nw > 1)
+    launch(B2G_KERNEL_PAIRS_WRITE, ns, k_grid_pairs<kPairSlots, kGridQueryThreads, kGridShared>,
+           dim3(grid_for(binned * kBlock / kGridQueryThreads)), dim3(kGridQueryThreads), 0, dg);
+  else
+    launch(B2G_KERNEL_PAIRS_WRITE, ns, k_grid_pairs<kPairSlots, kGridQueryThreads, kGridOneWorld>,
+           dim3(grid_for(binned * kBlock / kGridQueryThreads)), dim3(kGridQueryThreads), 0, dg);
   if ((rc = scan(d.pcount, nullptr, ns, d.poff, d.cnt + C_PAIRS, ns))) return rc;
-  hipLaunchKernelGGL((k_grid_pairs_compact<kPairSlots, false>), dim3(gq), dim3(kQueryBlock), 0, stream, dg);
+  if (d.nw > 1)
+    hipLaunchKernelGGL((k_grid_pairs_compact<kPairSlots, kGridShared>), dim3(gq), dim3(kQueryBlock), 0, stream, dg);
+  else
+    hipLaunchKernelGGL((k_grid_pairs_compact<kPairSlots, kGridOneWorld>), dim3(gq), dim3(kQueryBlock), 0, stream, dg);
   tree_stale = true;  // the LBVH is not kept: a user query builds it first
   last_pair_search = 2;
   return B2G_OK;
@@ -1240,6 +1260,7 @@ int b2g_batch::update_world_layout() {
   HIP_TRY(hipMemcpy(d.wshape1, ws1.data(), nw * sizeof(int), hipMemcpyHostToDevice));
   h_wshape1 = ws1;
   HIP_TRY(hipMemcpy(d.worder, order.data(), nw * sizeof(int), hipMemcpyHostToDevice));
+  h_worder = order;
   worlds_contiguous = ok;
   per_world_ok = ok && d.nj == 0;
   layout_dirty = false;
@@ -1254,7 +1275,9 @@ bool b2g_batch::use_per_world() const {
 }
 
 // autotuner candidates: workgroup size x bodies staged in LDS or not (one
-// timed step each); LDS staging needs the largest world to fit 63 KB
+// timed step each); a launch stages its worlds' bodies in LDS when its
+// largest world fits 63 KB (with the large worlds launched apart, the others
+// always fit)
 static const int kWorldBlocks[] = {64, 128, 256, 512, 1024};
 constexpr int kWorldBlockCount = 5;
 constexpr int kLdsBodyLimit = (63 * 1024) / 16;
@@ -1266,7 +1289,7 @@ int b2g_batch::world_block_now() const {
 }
 
 bool b2g_batch::world_lds_now() const {
-  if (max_world_bodies > kLdsBodyLimit || world_lds == 0) return false;
+  if ((big_worlds == 0 && max_world_bodies > kLdsBodyLimit) || world_lds == 0) return false;
   if (world_lds > 0) return true;
   if (world_block > 0 || tuned_block > 0) return tuned_lds;
   return std::min(tune_step, 2 * kWorldBlockCount - 1) >= kWorldBlockCount;
@@ -1281,26 +1304,44 @@ bool b2g_batch::world_lds_now() const {
 void b2g_batch::launch_solve_worlds(int vi, int pi) {
   const int wblock = world_block_now();
   const bool lds = world_lds_now();
+  // a mostly idle batch (serving) launches its active worlds only; a
+  // captured step keeps every world's workgroup (its activity is not known)
+  const int* order = d.worder;
+  int n_all = d.nw, n_big = big_worlds;
+  if (!capturing && (int)h_worder.size() >= d.nw && 2 * idle_worlds() > d.nw) {
+    (void)hipEventSynchronize(order_ev);  // the last step's copy has left the staging
+    n_all = n_big = 0;
+    for (int q = 0; q < d.nw; ++q) {
+      const int w = h_worder[q];
+      if (!h_params[w].active) continue;
+      stage_order[n_all++] = w;
+      if (q < big_worlds) ++n_big;
+    }
+    if (n_all > 0) (void)hipMemcpyAsync(d.wact, stage_order, n_all * sizeof(int), hipMemcpyHostToDevice, stream);
+    (void)hipEventRecord(order_ev, stream);
+    order = d.wact;
+  }
   auto part = [&](hipStream_t st, int first, int count, int block, int bodies) {
-    if (lds)
+    if (count <= 0) return;
+    if (lds && bodies <= kLdsBodyLimit)
       hipLaunchKernelGGL(k_solve_worlds<true>, dim3(count), dim3(block), sizeof(float4) * (size_t)bodies, st, d, vi,
-                         pi, first);
+                         pi, first, order);
     else
-      hipLaunchKernelGGL(k_solve_worlds<false>, dim3(count), dim3(block), 0, st, d, vi, pi, first);
+      hipLaunchKernelGGL(k_solve_worlds<false>, dim3(count), dim3(block), 0, st, d, vi, pi, first, order);
   };
   tbegin(B2G_KERNEL_SOLVER);
-  if (big_worlds > 0 && big_worlds < d.nw && wblock < 1024) {
+  if (n_big > 0 && n_big < n_all && wblock < 1024) {
     // the large worlds (the longest workgroups) are submitted first: where
     // both launches share a queue (a graph replay may), their workgroups are
     // then dispatched before the thousands of small ones
     (void)hipEventRecord(fork_ev, stream);
     (void)hipStreamWaitEvent(side, fork_ev, 0);
-    part(stream, 0, big_worlds, 1024, max_world_bodies);
-    part(side, big_worlds, d.nw - big_worlds, wblock, max_small_world_bodies);
+    part(stream, 0, n_big, 1024, max_world_bodies);
+    part(side, n_big, n_all - n_big, wblock, max_small_world_bodies);
     (void)hipEventRecord(join_ev, side);
     (void)hipStreamWaitEvent(stream, join_ev, 0);
   } else {
-    part(stream, 0, d.nw, wblock, max_world_bodies);
+    part(stream, 0, n_all, wblock, max_world_bodies);
   }
   tend(B2G_KERNEL_SOLVER, est_cons);
 }
@@ -1471,7 +1512,7 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
       HIP_TRY(hipEventSynchronize(ev[6]));
       float ms = 0.0f;
       HIP_TRY(hipEventElapsedTime(&ms, ev[5], ev[6]));
-      const bool lds_ok = max_world_bodies <= kLdsBodyLimit && world_lds != 0;
+      const bool lds_ok = (big_worlds > 0 || max_world_bodies <= kLdsBodyLimit) && world_lds != 0;
       const int ncand = lds_ok ? 2 * kWorldBlockCount : kWorldBlockCount;
       tune_ms[tune_step] = ms;
       if (++tune_step >= ncand) {

@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(kBlock) k_grid_scatter(Dev d) {
     d.gids[at] = i;
     d.gbox[at] = b;
     d.gfast[at] = d.sfast[i];
-    d.gworld[at] = d.shdr[i].w;
+    if (d.nw > 1) d.gworld[at] = d.shdr[i].w;
   }
 }
 
@@ -224,10 +224,14 @@ __global__ void __launch_bounds__(kBlock) k_grid_bin_worlds(Dev d) {
 
 // Every pair of shape i under the ownership rule: emit(key, s) as for
 // lbvh_query (s >= 0: the static lower shape of a key that is not i's own).
-// kLocal: the buckets are those of i's world's own table (k_grid_bin_worlds),
-// which hold no other world's shapes.
-template <bool kLocal, class Emit>
+// Where the buckets are: kGridOneWorld, the shared table of a one-world
+// batch; kGridShared, the shared table of several worlds (an entry of another
+// world that shares a bucket is skipped); kGridLocal, the table of i's own
+// world (k_grid_bin_worlds), which holds no other world's shapes.
+enum GridMode : int { kGridOneWorld = 0, kGridShared = 1, kGridLocal = 2 };
+template <int kMode, class Emit>
 __device__ __forceinline__ void grid_query(const Dev& d, const GridGeom& gg, int i, Emit emit) {
+  constexpr bool kLocal = kMode == kGridLocal;
   const int4 fi = d.sfast[i];
   if (fi.y & 64) return;  // static: found by its partners
   const float4 q = d.aabb[i];
@@ -291,7 +295,7 @@ __device__ __forceinline__ void grid_query(const Dev& d, const GridGeom& gg, int
       bx[u] = d.gbox[at];
       ids[u] = d.gids[at];
       fs[u] = d.gfast[at];
-      ws[u] = kLocal ? w : d.gworld[at];
+      ws[u] = kMode == kGridShared ? d.gworld[at] : w;
     }
     // (a shared table's bucket can hold cells of other worlds: their shapes
     // are never partners)
@@ -323,8 +327,9 @@ __device__ __forceinline__ void grid_query(const Dev& d, const GridGeom& gg, int
 // cells). Shared table: entries 0 .. binned count. Tables of their own: world
 // w's entries start at its first shape's index and end at its table's last
 // word; the entry slots past that (its static shapes') are skipped.
-template <int kSlots, int kThreads, bool kLocal>
+template <int kSlots, int kThreads, int kMode>
 __global__ void __launch_bounds__(kThreads) k_grid_pairs(Dev d) {
+  constexpr bool kLocal = kMode == kGridLocal;
   __shared__ uint64_t slots[kSlots * kThreads];  // thread-minor
   const GridGeom gg = grid_geom(d);
   const int nq = kLocal ? d.ns : d.goff[d.grid_T];  // binned shapes
@@ -333,7 +338,7 @@ __global__ void __launch_bounds__(kThreads) k_grid_pairs(Dev d) {
     const int i = d.gids[t];
     const int tid = (int)threadIdx.x;
     int found = 0, own = 0;
-    grid_query<kLocal>(d, gg, i, [&](uint64_t key, int s) {
+    grid_query<kMode>(d, gg, i, [&](uint64_t key, int s) {
       if (found < kSlots) {
         int b = found - 1;  // sorted insert
         while (b >= 0 && slots[b * kThreads + tid] > key) {
@@ -357,7 +362,7 @@ __global__ void __launch_bounds__(kThreads) k_grid_pairs(Dev d) {
 }
 
 // As k_pairs_compact, with the grid query for over-full shapes.
-template <int kSlots, bool kLocal>
+template <int kSlots, int kMode>
 __global__ void __launch_bounds__(kQueryBlock) k_grid_pairs_compact(Dev d) {
   const GridGeom gg = grid_geom(d);
   B2G_FOR(i, 0, d.ns) {
@@ -393,7 +398,7 @@ __global__ void __launch_bounds__(kQueryBlock) k_grid_pairs_compact(Dev d) {
     } else {
       // over-full shape: in query order, sorted afterwards by k_sort_pair_segments
       int k = 0;
-      grid_query<kLocal>(d, gg, i, [&](uint64_t key, int s) {
+      grid_query<kMode>(d, gg, i, [&](uint64_t key, int s) {
         if (s >= 0)
           foreign(key, s);
         else if (k < own)

@@ -83,6 +83,7 @@ struct Dev {
   int* wshape1;   // one past the last shape index of each world
   int2* wbody;    // body range of each world (worlds laid out world after world)
   int* worder;    // worlds by decreasing size: the per-world solver's block order
+  int* wact;      // the worlds active in the step, in the same order (mostly idle batches)
   int* wcol;      // per world: kColourSlots + 1 slot offsets of its colour groups
   int* wcolmax;   // per world: highest normal colour used this step (-1: none)
   int* wfresh;    // per world: colours of its last colouring from scratch (0: none yet)
@@ -2273,10 +2274,10 @@ __global__ void __launch_bounds__(kBlock) k_world_colour_sort(Dev d, int rounds_
 // slightly faster than 4 waves without spills; the CU hides one world's
 // latency behind the others'.)
 template <bool kLds>
-__global__ void __launch_bounds__(1024, 5) k_solve_worlds(Dev d, int vi, int pi, int first) {
+__global__ void __launch_bounds__(1024, 5) k_solve_worlds(Dev d, int vi, int pi, int first, const int* order) {
   __shared__ int off[kColourSlots + 1];
   extern __shared__ float4 s_body[];
-  const int w = d.worder[first + blockIdx.x];  // worlds first .. first + gridDim.x - 1, largest first
+  const int w = order[first + blockIdx.x];  // worlds first .. first + gridDim.x - 1, largest first
   if (!world_active(d, w)) return;  // whole block
   for (int c = threadIdx.x; c <= kColourSlots; c += blockDim.x) off[c] = d.wcol[w * (kColourSlots + 1) + c];
   __syncthreads();
