@@ -55,13 +55,13 @@ constexpr int kColourRoundsPerStep = 4;
 // (k_colour_finish), whose rounds cost a barrier each instead of two launches
 constexpr int kColourRoundsOneWorld = 2;
 constexpr int kFinishMaxLeft = 4096;  // more left (a world colouring afresh): device-wide rounds instead
-// claim rounds of a captured step: a round with nothing pending returns at
-// once, so spare rounds cost only their launch
-// claim rounds of a captured step: the colouring is incremental (a contact
-// keeps its colour while it persists), so a replay colours only the new
-// contacts in these multi-workgroup rounds; a burst of new contacts (or a
-// world coloured afresh) is finished by up to kGraphFinishRounds rounds in
-// one workgroup, and anything still uncoloured joins the serial group
+// claim rounds of a captured step of several worlds: the colouring is
+// incremental (a contact keeps its colour while it persists), so a replay
+// colours only the new contacts in these multi-workgroup rounds (a round with
+// nothing pending returns at once, so spare rounds cost only their launch); a
+// burst of new contacts (or a world coloured afresh) is finished by up to
+// kGraphFinishRounds rounds in one workgroup, and anything still uncoloured
+// joins the serial group
 constexpr int kGraphColourRounds = 4;
 constexpr int kGraphFinishRounds = 64;
 
@@ -217,6 +217,10 @@ struct b2g_batch {
   void graphs_clear();
   void enqueue_collide_and_colour(int K, int rounds, bool collect_last = false);
   void enqueue_collide_and_colour_eager();
+  // device-wide claim rounds of a captured step before the finishing
+  // workgroup: as an eager step's (a single world's few new contacts finish
+  // in the workgroup; a batch's worlds colouring afresh need the whole GPU)
+  int graph_colour_rounds() const { return d.nw == 1 ? kColourRoundsOneWorld : kGraphColourRounds; }
   void enqueue_collide();
   bool use_cells() const;
   void enqueue_cells_solve(int vi, int pi, bool finish_in_broadphase);
@@ -1628,7 +1632,7 @@ int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
   // per-world solver: colours below K; colour sweeps: any colour (those the
   // last eager step did not launch alone go to the tail launch)
   const int colour_limit = use_per_world() ? K : kMaxColours;
-  enqueue_collide_and_colour(colour_limit, kGraphColourRounds, true);
+  enqueue_collide_and_colour(colour_limit, graph_colour_rounds(), true);
   hipLaunchKernelGGL(k_colour_finish, dim3(1), dim3(kFinishThreads), 0, stream, d, rounds_pending,
                      kGraphFinishRounds, colour_limit, INT_MAX);  // no host loop follows a replay
   rounds_pending += kGraphFinishRounds;
@@ -1718,7 +1722,7 @@ int b2g_batch::graph_step(int vi, int pi, unsigned flags) {
     swap_contact_buffers();
     pairs_current = true;
   }
-  if (!cells) round_host += kGraphColourRounds + kGraphFinishRounds;
+  if (!cells) round_host += graph_colour_rounds() + kGraphFinishRounds;
   ++graph_steps;
   if (!(flags & B2G_STEP_ASYNC)) {
     HIP_TRY(hipStreamSynchronize(stream));

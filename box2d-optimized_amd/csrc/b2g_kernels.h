@@ -1735,40 +1735,65 @@ enum TailOp : int { kTailWarm = 0, kTailVelocity = 1, kTailPosition = 2 };
 template <int kOp>
 __global__ void __launch_bounds__(kTailThreads) k_sweep_tail(Dev d, TailSlots t, int it, int serial) {
   const int nt = (int)blockDim.x, tid = (int)threadIdx.x;
-  bool one_pass = t.known != 0 && kOp != kTailWarm;
-  for (int q = 0; q < t.n && one_pass; ++q) one_pass = t.off[q + 1] - t.off[q] <= nt;
-  if (one_pass) {
+  // the colours' slot offsets (given, or read on the device), and the
+  // non-empty colours in order
+  __shared__ int s_off[kColourSlots + 1];
+  __shared__ int s_q[kColourSlots];
+  __shared__ int s_nq, s_one;
+  if (t.known) {
+    if (tid == 0)
+      for (int q = 0; q <= t.n; ++q) s_off[q] = t.off[q];  // (uniform index: the argument stays in registers)
+  } else {
+    for (int q = tid; q <= t.n; q += nt) s_off[q] = d.col_off[t.c0 + q];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int k = 0;
+    bool one = kOp != kTailWarm;
+    for (int q = 0; q < t.n; ++q) {
+      const int len = s_off[q + 1] - s_off[q];
+      if (len <= 0) continue;
+      s_q[k++] = q;
+      one = one && len <= nt;
+    }
+    s_nq = k;
+    s_one = one ? 1 : 0;
+  }
+  __syncthreads();
+  const int nq = s_nq;
+  if (s_one) {
     // every colour fits one pass: lane tid takes slot off[q] + tid of colour
     // q, and loads the next colour's constraint while this one is solved
     // (the constraint fields do not depend on the bodies; only the bodies
     // wait for the barrier)
-    auto slot_of = [&](int q) {
-      const int s = t.off[q] + tid;
-      return s < t.off[q + 1] ? s : -1;
+    auto slot_of = [&](int k) {
+      const int q = s_q[k];
+      const int s = s_off[q] + tid;
+      return s < s_off[q + 1] ? s : -1;
     };
     if (kOp == kTailVelocity) {
-      int s_next = slot_of(0);
+      int s_next = nq > 0 ? slot_of(0) : -1;
       VelocityRegs nxt{};
       if (s_next >= 0) nxt = load_velocity_regs(d, s_next);
-      for (int q = 0; q < t.n; ++q) {
+      for (int k = 0; k < nq; ++k) {
         const int s_cur = s_next;
         const VelocityRegs cur = nxt;
-        if (q + 1 < t.n) {
-          s_next = slot_of(q + 1);
+        if (k + 1 < nq) {
+          s_next = slot_of(k + 1);
           if (s_next >= 0) nxt = load_velocity_regs(d, s_next);
         }
         if (s_cur >= 0) contact_velocity_src(d, s_cur, GlobalBodies{d.vel}, cur);
         __syncthreads();  // the next colour sees this colour's body updates
       }
     } else {
-      int s_next = slot_of(0);
+      int s_next = nq > 0 ? slot_of(0) : -1;
       PositionRegs nxt{};
       if (s_next >= 0) nxt = load_position_regs(d, s_next);
-      for (int q = 0; q < t.n; ++q) {
+      for (int k = 0; k < nq; ++k) {
         const int s_cur = s_next;
         const PositionRegs cur = nxt;
-        if (q + 1 < t.n) {
-          s_next = slot_of(q + 1);
+        if (k + 1 < nq) {
+          s_next = slot_of(k + 1);
           if (s_next >= 0) nxt = load_position_regs(d, s_next);
         }
         const bool valid = s_cur >= 0;
@@ -1778,15 +1803,9 @@ __global__ void __launch_bounds__(kTailThreads) k_sweep_tail(Dev d, TailSlots t,
       }
     }
   } else {
-    __shared__ int s_off[kColourSlots + 1];  // offsets read on the device: all of them at once
-    if (!t.known) {
-      for (int q = tid; q <= t.n; q += nt) s_off[q] = d.col_off[t.c0 + q];
-      __syncthreads();
-    }
-    for (int q = 0; q < t.n; ++q) {
-      const int begin = t.known ? t.off[q] : s_off[q];
-      const int end = t.known ? t.off[q + 1] : s_off[q + 1];
-      if (end <= begin) continue;  // an empty colour (uniform across the workgroup)
+    for (int k = 0; k < nq; ++k) {
+      const int q = s_q[k];
+      const int begin = s_off[q], end = s_off[q + 1];
       for (int base = begin; base < end; base += nt) {
         const int s = base + tid;
         const bool valid = s < end;
