@@ -444,18 +444,23 @@ __device__ __forceinline__ void narrow_contact(const Dev& d, int i) {
 // none of the contact data, and clears the step's colour counters. A step
 // with dt <= 0 runs Collide alone (integrate = 0).
 __global__ void __launch_bounds__(kBlock) k_narrowphase(Dev d, int integrate) {
-  __shared__ int s_cnt[17];  // 9 ordered type pairs; [16]: the tile's contacts
+  __shared__ int s_cnt[17];  // 9 ordered type pairs; [16]: the tile's contacts of active worlds
   __shared__ int s_perm[kBlock];
   if (blockIdx.x == 0) reset_colour_counters(d);
   const int n = d.cnt[C_CONTACTS];
   B2G_FOR_ALL(i, valid, 0, n) {
     if (threadIdx.x < 17) s_cnt[threadIdx.x] = 0;
     __syncthreads();
-    int key = 0, rank = 0;
+    int key = -1, rank = 0;
     if (valid) {
       const int2 pr = d.cpair[i];
-      key = 3 * d.shdr[pr.x].x + d.shdr[pr.y].x;  // kCircle 0, kEdge 1, kPolygon 2
-      rank = atomicAdd(&s_cnt[key], 1);
+      const int4 ha = d.shdr[pr.x];
+      if (world_active(d, ha.w)) {
+        key = 3 * ha.x + d.shdr[pr.y].x;  // kCircle 0, kEdge 1, kPolygon 2
+        rank = atomicAdd(&s_cnt[key], 1);
+      } else {
+        d.touch[i] = 0;  // a world that sits out the step: not solved, manifold kept
+      }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -468,7 +473,7 @@ __global__ void __launch_bounds__(kBlock) k_narrowphase(Dev d, int integrate) {
       s_cnt[16] = run;
     }
     __syncthreads();
-    if (valid) s_perm[s_cnt[key] + rank] = i;
+    if (key >= 0) s_perm[s_cnt[key] + rank] = i;
     __syncthreads();
     if ((int)threadIdx.x < s_cnt[16]) narrow_contact(d, s_perm[threadIdx.x]);
     __syncthreads();  // s_cnt and s_perm are the next tile's
