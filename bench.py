@@ -327,7 +327,10 @@ def serving_section(pkg, worlds: int, rate: float, duration: float, budget_gb: f
         "requests": int(len(trace)), "completed": int(st.completed), "offered_rate_per_s": rate,
         "duration_s": duration, "wall_s": st.wall_s, "ticks": int(st.ticks),
         "mean_in_flight": st.mean_in_flight, "max_in_flight": int(st.max_in_flight),
-        "device_busy_frac": st.busy_s / st.wall_s,
+        # device time of the ticks (HIP events around each step) over the wall
+        # time, and the host's time inside the step calls
+        "device_busy_frac": st.device_s / st.wall_s if st.device_s > 0 else None,
+        "host_step_frac": st.busy_s / st.wall_s,
         "pool": {"worlds": worlds, "bodies": int(sum(s.body_count for s in scenes)),
                  "sizes": "log-uniform 16-4096", "memory_budget_gb": budget_gb, "build_s": build_s},
         "launch": "hipGraph replay" if graph else "eager",

@@ -116,7 +116,7 @@ class ServeConfig(ctypes.Structure):
 class ServeStats(ctypes.Structure):
     _fields_ = [("wall_s", ctypes.c_double), ("busy_s", ctypes.c_double), ("mean_in_flight", ctypes.c_double),
                 ("ticks", ctypes.c_int64), ("world_steps", ctypes.c_int64), ("body_steps", ctypes.c_int64),
-                ("max_in_flight", ctypes.c_int32), ("completed", ctypes.c_int32)]
+                ("max_in_flight", ctypes.c_int32), ("completed", ctypes.c_int32), ("device_s", ctypes.c_double)]
 
 
 class B2GError(RuntimeError):

@@ -116,11 +116,16 @@ int b2s_run(b2g_batch* pool, int32_t world_count, const int32_t* world_bodies, c
     }
     int rc = b2g_set_world_params(pool, 0, world_count, params.data());
     if (rc) return rc;
-    // 4. one step of every in-flight world
+    // 4. one step of every in-flight world (an eager step also times itself
+    // on the device)
+    const bool timed = (step_flags & B2G_STEP_GRAPH) == 0;
     const Clock::time_point s0 = Clock::now();
-    rc = b2g_batch_step(pool, cfg->velocity_iterations, cfg->position_iterations, step_flags);
+    rc = b2g_batch_step(pool, cfg->velocity_iterations, cfg->position_iterations,
+                        step_flags | (timed ? B2G_STEP_PROFILE : 0u));
     if (rc) return rc;
     stats->busy_s += seconds_since(s0);
+    b2g_profile prof{};
+    if (timed && b2g_get_profile(pool, &prof) == B2G_OK) stats->device_s += 1e-3 * (double)prof.step;
     ++tick;
     ++stats->ticks;
     in_flight_sum += (double)flights.size();

@@ -485,10 +485,14 @@ typedef struct b2s_config {
   int32_t realtime;           /* 1: arrivals on the wall clock; 0: arrivals on the tick counter */
 } b2s_config;
 
+/* wall_s: the loop's wall time; busy_s: host time inside b2g_batch_step;
+ * device_s: device time of the ticks (HIP events around each step; eager
+ * ticks only, 0 with B2G_STEP_GRAPH). */
 typedef struct b2s_stats {
   double wall_s, busy_s, mean_in_flight;
   int64_t ticks, world_steps, body_steps;
   int32_t max_in_flight, completed;
+  double device_s;
 } b2s_stats;
 
 /* Runs requests (sorted by arrival) against `pool`, a batch whose worlds

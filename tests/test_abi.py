@@ -37,7 +37,7 @@ def test_record_sizes_match_header():
     assert eng.REQUEST_DTYPE.itemsize == 8 + 4 * 4
     assert eng.RESULT_DTYPE.itemsize == 3 * 8 + 2 * 4
     assert ctypes.sizeof(eng.ServeConfig) == 8 + 4 * 4
-    assert ctypes.sizeof(eng.ServeStats) == 3 * 8 + 3 * 8 + 2 * 4
+    assert ctypes.sizeof(eng.ServeStats) == 3 * 8 + 3 * 8 + 2 * 4 + 8
 
 
 def test_errors_without_device_are_reported_not_faked():

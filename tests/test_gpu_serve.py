@@ -99,6 +99,10 @@ def test_gpu_serve_realtime_poisson(engine):
     assert st.completed == len(trace)
     assert (res["completed"] >= res["admitted"]).all() and (res["admitted"] >= res["arrival"]).all()
     assert np.isfinite(lat).all() and lat.min() > 0
+    # the ticks' device time (events around each eager step) lies within
+    # the host's time inside the step calls, which lies within the wall time
+    print(f"device {st.device_s:.4f} s, host in steps {st.busy_s:.4f} s, wall {st.wall_s:.4f} s", flush=True)
+    assert 0.0 < st.device_s <= st.busy_s <= st.wall_s
 
 
 def test_gpu_serve_full_pool_same_result_as_alone(engine):
