@@ -26,7 +26,7 @@ namespace b2g {
 constexpr int kCells = 1024;              // cells per grid (A, and the shifted B)
 constexpr int kPartGroups = 2 * kCells + 1;  // A cells, B cells, the cross group C
 constexpr int kCrossGroup = 2 * kCells;
-constexpr int kCellBodies = 512;          // bodies of one cell (LDS); a fuller cell's constraints go to C
+constexpr int kCellBodies = 256;          // bodies of one cell (LDS); a fuller cell's constraints go to C
 constexpr int kCellCons = 512;            // constraints of one cell; more go to C
 constexpr int kCellTarget = 48;           // bodies per cell the grid aims at
 constexpr int kPhaseStride = 1 << 16;     // reported colour: phase * kPhaseStride + colour
@@ -543,7 +543,7 @@ enum PartOp : int { kPartWarm = 0, kPartVelocity = 1, kPartPosition = 2 };
 // colour round then reads LDS. Warm start and velocity: the ten velocity
 // words; position: word 2 of the velocity record, the five position words
 // and the island. Slots past kStageCons are read from global memory.
-constexpr int kStageCons = 256;
+constexpr int kStageCons = 96;
 template <int kOp>
 struct PartStage {
   static constexpr int kWords = kOp == kPartPosition ? 6 : 10;
