@@ -183,6 +183,53 @@ def test_gpu_grid_pair_search_in_a_batch_equals_lbvh_bitexact(engine):
           f"{len(gl[-1])} contacts at the end", flush=True)
 
 
+def _edge_worlds(big):
+    """Worlds at the edges of the per-world bucket tables: static shapes only,
+    one dynamic body, a small pile, and (big) a world of more binned shapes
+    than a table of its own takes, which puts the whole batch on the shared
+    table."""
+    stat = S.Scene(name="statics")
+    g = stat.add_body(S.STATIC, 0.0, 0.0)
+    stat.add_box(g, 10.0, 0.5)
+    for k in range(5):
+        h = stat.add_body(S.STATIC, -4.0 + 2.0 * k, 2.0)
+        stat.add_circle(h, 0.4)
+    one = S.Scene(name="one")
+    g = one.add_body(S.STATIC, 0.0, 0.0)
+    one.add_box(g, 10.0, 0.5)
+    b = one.add_body(S.DYNAMIC, 0.0, 3.0)
+    one.add_box(b, 0.5, 0.5)
+    worlds = [stat, one, S.random_pile(300, seed=61, half_w=10.0, height=25.0), S.pyramid(10)]
+    if big:
+        worlds.append(S.random_pile(4200, seed=62, half_w=40.0, height=100.0))
+    return worlds
+
+
+@pytest.mark.parametrize("big", [False, True])
+def test_gpu_grid_pair_search_edge_worlds_equals_lbvh_bitexact(engine, big):
+    """Edge worlds (static shapes only, a single body) in a batch on tables of
+    their own, and the same worlds beside one of more than 4096 binned shapes
+    (the shared table): over 60 steps the grid keeps the LBVH's contact lists
+    and bodies bit for bit."""
+    scenes = _edge_worlds(big)
+    runs = []
+    for search in (1, 0):
+        b = engine.Batch.from_scenes(scenes)
+        b.set_option(engine.OPT_PAIR_SEARCH, search)
+        runs.append(b)
+    for k in range(6):
+        for b in runs:
+            b.advance(10)
+        ca, cb = runs[0].contacts(), runs[1].contacts()
+        assert len(ca) == len(cb) and len(ca) > 0, f"step {10 * (k + 1)}"
+        assert ca.tobytes() == cb.tobytes(), f"step {10 * (k + 1)}: contact lists differ"
+        assert runs[0].bodies().tobytes() == runs[1].bodies().tobytes(), f"step {10 * (k + 1)}: bodies differ"
+    assert runs[0].get_option(engine.OPT_LAST_PAIR_SEARCH) == 1
+    assert runs[1].get_option(engine.OPT_LAST_PAIR_SEARCH) == 2
+    print(f"{len(scenes)} worlds, {sum(s.body_count for s in scenes)} bodies, {len(runs[1].contacts())} contacts",
+          flush=True)
+
+
 def _gpu_world_at(engine, sc, g):
     """GPU world built by the drop-in API, then set to the reference state."""
     w = engine.World(sc)
