@@ -1279,12 +1279,13 @@ bool b2g_batch::use_per_world() const {
 }
 
 // autotuner candidates: workgroup size x bodies staged in LDS or not (one
-// timed step each); a launch stages its worlds' bodies in LDS when its
-// largest world fits 63 KB (with the large worlds launched apart, the others
-// always fit)
+// timed step each); a launch's stage holds up to 156 KB of bodies (a gfx950
+// workgroup may take 160 KB of LDS, the whole CU's: every world the
+// per-world solver takes, 8192 bodies at most, fits)
 static const int kWorldBlocks[] = {64, 128, 256, 512, 1024};
 constexpr int kWorldBlockCount = 5;
-constexpr int kLdsBodyLimit = (63 * 1024) / 16;
+constexpr int kLdsBodyLimit = (156 * 1024) / 16;
+constexpr int kSmallWorldLds = 512;
 
 int b2g_batch::world_block_now() const {
   if (world_block > 0) return world_block;
@@ -1293,7 +1294,7 @@ int b2g_batch::world_block_now() const {
 }
 
 bool b2g_batch::world_lds_now() const {
-  if ((big_worlds == 0 && max_world_bodies > kLdsBodyLimit) || world_lds == 0) return false;
+  if (world_lds == 0) return false;  // (a world past a launch's stage reads its bodies from global memory)
   if (world_lds > 0) return true;
   if (world_block > 0 || tuned_block > 0) return tuned_lds;
   return std::min(tune_step, 2 * kWorldBlockCount - 1) >= kWorldBlockCount;
@@ -1325,13 +1326,16 @@ void b2g_batch::launch_solve_worlds(int vi, int pi) {
     (void)hipEventRecord(order_ev, stream);
     order = d.wact;
   }
+  // `bodies`: the LDS stage of each workgroup of the launch (its worlds of
+  // more bodies read theirs from global memory)
   auto part = [&](hipStream_t st, int first, int count, int block, int bodies) {
     if (count <= 0) return;
-    if (lds && bodies <= kLdsBodyLimit)
+    bodies = std::min(bodies, kLdsBodyLimit);
+    if (lds && bodies > 0)
       hipLaunchKernelGGL(k_solve_worlds<true>, dim3(count), dim3(block), sizeof(float4) * (size_t)bodies, st, d, vi,
-                         pi, first, order);
+                         pi, first, order, bodies);
     else
-      hipLaunchKernelGGL(k_solve_worlds<false>, dim3(count), dim3(block), 0, st, d, vi, pi, first, order);
+      hipLaunchKernelGGL(k_solve_worlds<false>, dim3(count), dim3(block), 0, st, d, vi, pi, first, order, 0);
   };
   tbegin(B2G_KERNEL_SOLVER);
   if (n_big > 0 && n_big < n_all && wblock < 1024) {
@@ -1341,7 +1345,10 @@ void b2g_batch::launch_solve_worlds(int vi, int pi) {
     (void)hipEventRecord(fork_ev, stream);
     (void)hipStreamWaitEvent(side, fork_ev, 0);
     part(stream, 0, n_big, 1024, max_world_bodies);
-    part(side, n_big, n_all - n_big, wblock, max_small_world_bodies);
+    // the small worlds' launch stages up to kSmallWorldLds bodies: a stage
+    // sized for its largest world (1024 bodies, 16 KB) held 10 workgroups
+    // per CU, and its thousands of worlds ran in two rounds
+    part(side, n_big, n_all - n_big, wblock, std::min(max_small_world_bodies, kSmallWorldLds));
     (void)hipEventRecord(join_ev, side);
     (void)hipStreamWaitEvent(stream, join_ev, 0);
   } else {
@@ -1516,7 +1523,7 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
       HIP_TRY(hipEventSynchronize(ev[6]));
       float ms = 0.0f;
       HIP_TRY(hipEventElapsedTime(&ms, ev[5], ev[6]));
-      const bool lds_ok = (big_worlds > 0 || max_world_bodies <= kLdsBodyLimit) && world_lds != 0;
+      const bool lds_ok = world_lds != 0;
       const int ncand = lds_ok ? 2 * kWorldBlockCount : kWorldBlockCount;
       tune_ms[tune_step] = ms;
       if (++tune_step >= ncand) {

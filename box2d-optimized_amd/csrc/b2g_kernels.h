@@ -2291,14 +2291,16 @@ __global__ void __launch_bounds__(kBlock) k_world_colour_sort(Dev d, int rounds_
 // The solve of world worder[blockIdx.x]: warm start, velocity iterations,
 // store impulses + integrate positions, position iterations with the
 // per-island early exit, write-back. Worlds without joints only. With kLds
-// the world's body velocities (then positions) live in LDS during the sweeps
-// (dynamic shared memory of 16 bytes per body of the largest world); the
+// the body velocities (then positions) of a world of at most `lds_bodies`
+// bodies live in LDS during the sweeps (dynamic shared memory of 16 bytes per
+// body); a larger world of the launch reads them from global memory. The
 // arithmetic is the same either way.
 // (At least 5 waves per SIMD: 96 registers and a few spills measured
 // slightly faster than 4 waves without spills; the CU hides one world's
 // latency behind the others'.)
 template <bool kLds>
-__global__ void __launch_bounds__(1024, 5) k_solve_worlds(Dev d, int vi, int pi, int first, const int* order) {
+__global__ void __launch_bounds__(1024, 5) k_solve_worlds(Dev d, int vi, int pi, int first, const int* order,
+                                                          int lds_bodies) {
   __shared__ int off[kColourSlots + 1];
   extern __shared__ float4 s_body[];
   const int w = order[first + blockIdx.x];  // worlds first .. first + gridDim.x - 1, largest first
@@ -2308,6 +2310,7 @@ __global__ void __launch_bounds__(1024, 5) k_solve_worlds(Dev d, int vi, int pi,
   const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
   const int t0 = off[0], t1 = off[kColourSlots];
   const int2 wb = d.wbody[w];
+  const bool lds = kLds && wb.y - wb.x <= lds_bodies;  // uniform across the block
   // one colour at a time; the serial group on one lane, in slot order
   auto sweep = [&](auto fn) {
     for (int c = 0; c < kColourSlots; ++c) {
@@ -2331,7 +2334,7 @@ __global__ void __launch_bounds__(1024, 5) k_solve_worlds(Dev d, int vi, int pi,
     __syncthreads();
   };
   if (t1 > t0) {
-    if (kLds) {
+    if (lds) {
       stage_in(d.vel);
       const LdsBodies vel{s_body, wb.x};
       sweep([&](int s) { warm_start_t(d, s, vel); });
@@ -2346,7 +2349,7 @@ __global__ void __launch_bounds__(1024, 5) k_solve_worlds(Dev d, int vi, int pi,
   for (int b = wb.x + tid; b < wb.y; b += nt) integrate_position(d, b);
   __syncthreads();
   if (t1 > t0) {
-    if (kLds) stage_in(d.pos);
+    if (lds) stage_in(d.pos);
     const LdsBodies lpos{s_body, wb.x};
     const GlobalBodies gpos{d.pos};
     for (int it = 0; it < pi; ++it) {
@@ -2358,7 +2361,7 @@ __global__ void __launch_bounds__(1024, 5) k_solve_worlds(Dev d, int vi, int pi,
             for (int s = lo; s < hi; ++s) {
               const int isl = d.cisland[s];
               const float m =
-                  kLds ? contact_position_t(d, s, isl, it, lpos) : contact_position_t(d, s, isl, it, gpos);
+                  lds ? contact_position_t(d, s, isl, it, lpos) : contact_position_t(d, s, isl, it, gpos);
               if (separation_bad(m)) d.sepbad[island_slot(d, isl, it)] = 1u;
             }
         } else {
@@ -2367,7 +2370,7 @@ __global__ void __launch_bounds__(1024, 5) k_solve_worlds(Dev d, int vi, int pi,
             const bool valid = s < hi;
             const int isl = valid ? d.cisland[s] : 0;
             float m = 0.0f;
-            if (valid) m = kLds ? contact_position_t(d, s, isl, it, lpos) : contact_position_t(d, s, isl, it, gpos);
+            if (valid) m = lds ? contact_position_t(d, s, isl, it, lpos) : contact_position_t(d, s, isl, it, gpos);
             mark_island_sep(d, it, isl, valid, m);
           }
         }
@@ -2386,7 +2389,7 @@ __global__ void __launch_bounds__(1024, 5) k_solve_worlds(Dev d, int vi, int pi,
       }
       __syncthreads();
     }
-    if (kLds) stage_out(d.pos);
+    if (lds) stage_out(d.pos);
   }
   // write back: SynchronizeTransform and ClearForces
   for (int b = wb.x + tid; b < wb.y; b += nt) {
