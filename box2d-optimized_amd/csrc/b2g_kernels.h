@@ -1729,7 +1729,7 @@ __global__ void __launch_bounds__(kBlock) k_warm_gather(Dev d) {
 // share one launch; the order of the updates is the same as with a launch per
 // colour, and so are the results. The colours'
 // slot ranges come from the host when it knows them, else from the offsets.
-constexpr int kTailThreads = 512;
+constexpr int kTailThreads = 1024;
 constexpr int kTailMax = 16;  // colours per tail launch
 struct TailSlots {
   int c0, n, known;       // colours c0 .. c0 + n - 1 (n <= kTailMax when known)
