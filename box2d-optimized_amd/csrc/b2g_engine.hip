@@ -1060,6 +1060,12 @@ void b2g_batch::enqueue_islands_join() {
 // there costs less than its own launch (0.71 against 0.72 ms per step); 2048
 // (two passes) measured slower (0.77)
 constexpr int kTailColourMax = 1024;
+// Per-colour sweep launches run 64-lane workgroups: a colour's few thousand
+// constraints then spread over as many CUs as possible, one wave each (the
+// launch is latency-bound). On the 20k pile: 0.685 ms per step against 0.701
+// with 128 lanes and 0.713 with 256 (steps 6-25), 0.858 against 0.880 and
+// 0.898 (steps 21-500).
+constexpr int kSweepThreads = 64;
 
 // colours: the slots to sweep; sizes[c] (eager) or -1 (graph: unknown);
 // begins[c]: first slot of each swept colour (eager; empty in graph mode)
@@ -1108,8 +1114,8 @@ void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, c
       const int c = colours[q];
       const int n = sizes[q];
       const int2 slots = begins.empty() ? make_int2(-1, -1) : make_int2(begins[q], begins[q] + n);
-      hipLaunchKernelGGL(kernel, dim3(n >= 0 ? grid_for(n) : graph_grid), dim3(kBlock), 0, stream, d, c, slots,
-                         extra...);
+      hipLaunchKernelGGL(kernel, dim3(n >= 0 ? grid_for((long long)n * kBlock / kSweepThreads) : graph_grid),
+                         dim3(kSweepThreads), 0, stream, d, c, slots, extra...);
       run_items += n;
     }
     if (tq > 0) tend(id, run_items, tq);
@@ -1129,7 +1135,9 @@ void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, c
     const int njc = (int)h_jcol_off.size() - 1;
     for (int c = 0; c < njc; ++c) {
       const int b = h_jcol_off[c], e = h_jcol_off[c + 1];
-      if (e > b) hipLaunchKernelGGL(kernel, dim3(grid_for(e - b)), dim3(kBlock), 0, stream, d, b, e, extra...);
+      if (e > b)
+        hipLaunchKernelGGL(kernel, dim3(grid_for((long long)(e - b) * kBlock / kSweepThreads)), dim3(kSweepThreads), 0,
+                           stream, d, b, e, extra...);
     }
   };
   // warm start: a gather per body when every colour has a body-table entry
@@ -1672,7 +1680,8 @@ int b2g_batch::enqueue_graph_body(int vi, int pi, unsigned flags, int K) {
   // per-colour launches sized for the last eager step's largest colour and a
   // half (a larger colour is still covered: the launches stride over it)
   const int per_colour = std::max(256, graph_big_items + graph_big_items / 2);
-  enqueue_solve(vi, pi, colours, sizes, {}, grid_for(per_colour), !(flags & B2G_STEP_SKIP_BROADPHASE));
+  enqueue_solve(vi, pi, colours, sizes, {}, grid_for((long long)per_colour * kBlock / kSweepThreads),
+                !(flags & B2G_STEP_SKIP_BROADPHASE));
   if (!(flags & B2G_STEP_SKIP_BROADPHASE)) {
     int rc = broadphase(false);
     if (rc) return rc;
