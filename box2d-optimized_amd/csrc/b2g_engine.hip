@@ -74,6 +74,19 @@ inline int grid_for(long long n, int max_blocks = 4096) {
   return (int)g;
 }
 
+// Launch shape of a grid-stride kernel (no LDS, no block-wide step) over n
+// items. Up to kSmallLaunchItems items the launch is latency-bound: 64-lane
+// workgroups spread it over as many CUs as possible, one wave each. Larger
+// launches take kBlock lanes (64-lane workgroups, capped at 4096, would loop).
+constexpr long long kSmallLaunchItems = 65536;
+struct LaunchShape {
+  dim3 grid, block;
+};
+inline LaunchShape shape_for(long long n, int max_blocks = 4096) {
+  if (n <= kSmallLaunchItems) return {dim3(grid_for(n * kBlock / kWave, max_blocks)), dim3(kWave)};
+  return {dim3(grid_for(n, max_blocks)), dim3(kBlock)};
+}
+
 }  // namespace
 
 struct b2g_batch {
@@ -851,8 +864,8 @@ void b2g_batch::graphs_clear() {
 // `collect_last`: the last round lists its losers for k_colour_finish
 void b2g_batch::enqueue_collide_and_colour(int K, int rounds, bool collect_last) {
   enqueue_collide();
-  const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
-  hipLaunchKernelGGL(k_colour_init, dim3(gk), dim3(kBlock), 0, stream, d, K);
+  const LaunchShape ck = shape_for(std::max(est_cons, 256) * 5 / 4);
+  hipLaunchKernelGGL(k_colour_init, ck.grid, ck.block, 0, stream, d, K);
   for (int r = 0; r < rounds; ++r) enqueue_colour_round(K, collect_last && r == rounds - 1);
 }
 
@@ -941,8 +954,10 @@ void b2g_batch::enqueue_cells_solve(int vi, int pi, bool finish_in_broadphase) {
           k_part_sweep<kPartVelocity, kCrossThreads>, 0);
   }
   (void)hipStreamWaitEvent(stream, join_ev, 0);
-  hipLaunchKernelGGL(k_store_and_integrate, dim3(grid_for(std::max(est_cons + d.nj, d.nb))), dim3(kBlock), 0, stream,
-                     d, 1);
+  {
+    const LaunchShape si = shape_for(std::max(est_cons + d.nj, d.nb));
+    hipLaunchKernelGGL(k_store_and_integrate, si.grid, si.block, 0, stream, d, 1);
+  }
   for (int it = 0; it < pi; ++it) {
     sweep(B2G_KERNEL_CONTACT_POSITION, k_part_sweep<kPartPosition, kCellThreads>,
           k_part_sweep<kPartPosition, kCrossThreads>, it);
@@ -997,9 +1012,9 @@ int b2g_batch::cells_step(int vi, int pi, unsigned flags) {
 void b2g_batch::enqueue_colour_round(int K, bool collect) {
   // the rounds walk the pending list: after the first steps it holds only the
   // newly touching constraints, so a modest grid covers it
-  const int gk = std::min(grid_for(std::max(est_cons, 256) * 5 / 4), 512);
-  hipLaunchKernelGGL(k_colour_claim, dim3(gk), dim3(kBlock), 0, stream, d, rounds_pending);
-  hipLaunchKernelGGL(k_colour_resolve, dim3(gk), dim3(kBlock), 0, stream, d, rounds_pending, K, collect ? 1 : 0);
+  const LaunchShape ck = shape_for(std::max(est_cons, 256) * 5 / 4, 2048);
+  hipLaunchKernelGGL(k_colour_claim, ck.grid, ck.block, 0, stream, d, rounds_pending);
+  hipLaunchKernelGGL(k_colour_resolve, ck.grid, ck.block, 0, stream, d, rounds_pending, K, collect ? 1 : 0);
   ++rounds_pending;
 }
 
@@ -1020,7 +1035,6 @@ constexpr int kIslandLdsSmall = 2048, kIslandLdsMid = 8192, kIslandLdsLarge = 32
 // contacts, which exist by now, and nothing of the velocity phase); the
 // position phase joins it (enqueue_islands_join).
 void b2g_batch::enqueue_islands_and_init() {
-  const int gk = grid_for(std::max(est_cons, 256) * 5 / 4);
   // With sleeping, the islands decide which bodies move at all, so they come
   // first, on the main stream: seeds, wake-ups and the velocity integration of
   // the bodies that move follow them.
@@ -1046,7 +1060,8 @@ void b2g_batch::enqueue_islands_and_init() {
     hipLaunchKernelGGL(k_sleep_activate, dim3(grid_for(d.nb)), dim3(kBlock), 0, stream, d);
   }
   (void)hipEventRecord(join_ev, is);
-  hipLaunchKernelGGL(k_contact_init, dim3(gk), dim3(kBlock), 0, stream, d);
+  const LaunchShape ci = shape_for(std::max(est_cons, 256) * 5 / 4);
+  hipLaunchKernelGGL(k_contact_init, ci.grid, ci.block, 0, stream, d);
   if (d.nj) hipLaunchKernelGGL(k_joint_prepare, dim3(grid_for(d.nj)), dim3(kBlock), 0, stream, d);
 }
 
@@ -1143,7 +1158,8 @@ void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, c
   // warm start: a gather per body when every colour has a body-table entry
   const int top_colour = nnorm ? colours[nnorm - 1] : -1;
   if (tail) {  // captured steps: the gather, then the colours past the body table and the serial group
-    launch(B2G_KERNEL_WARM_START, -1, k_warm_gather, dim3(grid_for(d.nb)), dim3(kBlock), 0, d);
+    const LaunchShape wg = shape_for(d.nb);
+    launch(B2G_KERNEL_WARM_START, -1, k_warm_gather, wg.grid, wg.block, 0, d);
     TailSlots wt{};
     wt.c0 = kBodyTab;
     wt.n = kMaxColours - kBodyTab;
@@ -1152,7 +1168,8 @@ void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, c
   } else if (top_colour < kBodyTab) {
     int items = 0;
     for (int q = 0; q < nnorm; ++q) items += std::max(sizes[q], 0);
-    launch(B2G_KERNEL_WARM_START, tail ? -1 : items, k_warm_gather, dim3(grid_for(d.nb)), dim3(kBlock), 0, d);
+    const LaunchShape wg = shape_for(d.nb);
+    launch(B2G_KERNEL_WARM_START, tail ? -1 : items, k_warm_gather, wg.grid, wg.block, 0, d);
     if (has_serial || tail) {  // captured steps: a serial group may appear
       const int2 slots = has_serial && !begins.empty() ? make_int2(begins[nq - 1], begins[nq - 1] + sizes[nq - 1])
                                                        : make_int2(-1, -1);
@@ -1169,8 +1186,10 @@ void b2g_batch::enqueue_solve(int vi, int pi, const std::vector<int>& colours, c
   // the island pass (side stream) ended long before the velocity phase did;
   // the same launch stores impulses, integrates positions and assigns islands
   (void)hipStreamWaitEvent(stream, join_ev, 0);
-  hipLaunchKernelGGL(k_store_and_integrate, dim3(grid_for(std::max(est_cons + d.nj, d.nb))), dim3(kBlock), 0, stream,
-                     d, 1);
+  {
+    const LaunchShape si = shape_for(std::max(est_cons + d.nj, d.nb));
+    hipLaunchKernelGGL(k_store_and_integrate, si.grid, si.block, 0, stream, d, 1);
+  }
   for (int it = 0; it < pi; ++it) {  // each launch applies the island early exit itself
     contact_sweep(B2G_KERNEL_CONTACT_POSITION, k_contact_position, k_sweep_tail<kTailPosition>, it);
     joint_sweep(k_joint_position, it);
@@ -1456,7 +1475,8 @@ int b2g_batch::sleep_step(int vi, int pi, unsigned flags) {
     if (pinned[C_REMAIN] == 0) {
       hipLaunchKernelGGL(k_colour_clear_counts, dim3(1), dim3(kBlock), 0, stream, d);
       enqueue_colour_groups();
-      hipLaunchKernelGGL(k_contact_init, dim3(grid_for(std::max(est_cons, 256) * 5 / 4)), dim3(kBlock), 0, stream, d);
+      const LaunchShape ci = shape_for(std::max(est_cons, 256) * 5 / 4);
+      hipLaunchKernelGGL(k_contact_init, ci.grid, ci.block, 0, stream, d);
       if ((rc = read_counters())) return rc;
     }
   }
@@ -1590,7 +1610,8 @@ int b2g_batch::eager_step(int vi, int pi, unsigned flags) {
     if (pinned[C_REMAIN] == 0) {
       hipLaunchKernelGGL(k_colour_clear_counts, dim3(1), dim3(kBlock), 0, stream, d);
       enqueue_colour_groups();
-      hipLaunchKernelGGL(k_contact_init, dim3(grid_for(std::max(est_cons, 256) * 5 / 4)), dim3(kBlock), 0, stream, d);
+      const LaunchShape ci = shape_for(std::max(est_cons, 256) * 5 / 4);
+      hipLaunchKernelGGL(k_contact_init, ci.grid, ci.block, 0, stream, d);
       if ((rc = read_counters())) return rc;
     }
   }
