@@ -635,8 +635,8 @@ int b2g_batch::grid_pairs() {
     // every world active and small: each bins into a table of its own
     Dev dg = d;
     dg.grid_T = 0;  // (the shared table's counts are not used)
-    hipLaunchKernelGGL(k_grid_aabb, dim3(std::max(gs, finish_pending ? grid_for(d.nb) : 1)), dim3(kBlock), 0, stream,
-                       dg, finish_pending ? 1 : 0);
+    const LaunchShape ga = shape_for(std::max<long long>(ns, finish_pending ? d.nb : 1));
+    hipLaunchKernelGGL(k_grid_aabb, ga.grid, ga.block, 0, stream, dg, finish_pending ? 1 : 0);
     finish_pending = false;
     hipLaunchKernelGGL(k_grid_bin_worlds, dim3(d.nw), dim3(kBlock), 0, stream, d);
     launch(B2G_KERNEL_PAIRS_WRITE, ns, k_grid_pairs<kPairSlots, kGridQueryThreads, kGridLocal>,
@@ -660,13 +660,15 @@ int b2g_batch::grid_pairs() {
   Dev dg = d;
   dg.grid_T = 1024;
   while (dg.grid_T < 4 * binned && dg.grid_T < d.grid_T) dg.grid_T <<= 1;
-  hipLaunchKernelGGL(k_grid_aabb, dim3(std::max(gs, finish_pending ? grid_for(d.nb) : 1)), dim3(kBlock), 0, stream, dg,
+  const LaunchShape ga = shape_for(std::max<long long>(ns, finish_pending ? d.nb : 1));
+  hipLaunchKernelGGL(k_grid_aabb, ga.grid, ga.block, 0, stream, dg,
                      finish_pending ? 1 : 0);
   finish_pending = false;
-  hipLaunchKernelGGL(k_grid_count, dim3(gs), dim3(kBlock), 0, stream, dg);
+  const LaunchShape gc = shape_for(ns);
+  hipLaunchKernelGGL(k_grid_count, gc.grid, gc.block, 0, stream, dg);
   int rc = scan(dg.gcount, nullptr, dg.grid_T, dg.goff, dg.goff + dg.grid_T, dg.grid_T);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_grid_scatter, dim3(gs), dim3(kBlock), 0, stream, dg);
+  hipLaunchKernelGGL(k_grid_scatter, gc.grid, gc.block, 0, stream, dg);
   if (d.nw > 1)
     launch(B2G_KERNEL_PAIRS_WRITE, ns, k_grid_pairs<kPairSlots, kGridQueryThreads, kGridShared>,
            dim3(grid_for(binned * kBlock / kGridQueryThreads)), dim3(kGridQueryThreads), 0, dg);
@@ -749,7 +751,8 @@ int b2g_batch::finish_pair_list(bool use_given_aabbs) {
   // over-full dynamic shapes
   hipLaunchKernelGGL(k_sort_pair_segments, dim3(std::min(std::max(static_shapes + 32, 64), 4096)), dim3(kBlock), 0,
                      stream, d);
-  hipLaunchKernelGGL(k_merge_contacts, dim3(grid_for(std::max(est_contacts, d.ns) * 5 / 4)), dim3(kBlock), 0, stream,
+  const LaunchShape mc = shape_for(std::max(est_contacts, d.ns) * 5 / 4);
+  hipLaunchKernelGGL(k_merge_contacts, mc.grid, mc.block, 0, stream,
                      d);
   if (d.events_on || d.sleep_on)  // EndContact events; wake-ups of the destroyed touching contacts' bodies
     hipLaunchKernelGGL(k_dead_contact_events, dim3(grid_for(std::max(est_contacts, 256))), dim3(kBlock), 0, stream,
