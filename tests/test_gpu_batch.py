@@ -304,6 +304,24 @@ def test_gpu_graph_steps_match_eager_steps(engine):
                                                                             bits(kb["normal_impulse"]))
 
 
+def test_gpu_graph_steps_match_eager_steps_one_world(engine):
+    """The same for a batch of one world (a captured step of one world runs
+    the eager step's two device-wide claim rounds, then the finishing
+    workgroup): replays equal eager steps bit for bit."""
+    sc = S.random_pile(1500, seed=37, half_w=20.0, height=45.0)
+    eager = engine.Batch.from_scenes([sc]).advance(25)
+    graph = engine.Batch.from_scenes([sc]).advance(25)
+    for k in range(40):
+        eager.step(8, 3, 0)
+        graph.step(8, 3, engine.STEP_GRAPH)
+        assert graph.counts().serial == 0, f"step {k}"
+    assert graph.counts().graph_steps == 40
+    a, b = eager.bodies(), graph.bodies()
+    for f in ("xf", "c", "a", "v", "w"):
+        assert np.array_equal(bits(a[f]), bits(b[f])), f
+    assert eager.contacts().tobytes() == graph.contacts().tobytes()
+
+
 def test_gpu_graph_steps_async_batch(engine):
     """Async graph replays of a batch (no host sync between steps), then one
     sync: same state as the same number of synchronous graph steps."""
