@@ -1041,8 +1041,11 @@ void b2g_batch::enqueue_islands_and_init() {
   // With sleeping, the islands decide which bodies move at all, so they come
   // first, on the main stream: seeds, wake-ups and the velocity integration of
   // the bodies that move follow them.
-  hipStream_t is = d.sleep_on ? stream : side;
-  if (!d.sleep_on) {
+  // A captured step keeps them on the main stream too: a graph's branch on
+  // another stream costs a dependency across hardware queues at each end.
+  const bool own_stream = d.sleep_on || capturing;
+  hipStream_t is = own_stream ? stream : side;
+  if (!own_stream) {
     (void)hipEventRecord(fork_ev, stream);
     (void)hipStreamWaitEvent(side, fork_ev, 0);
   }
